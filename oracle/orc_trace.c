@@ -1,0 +1,608 @@
+/*
+ * Oracle (test infrastructure only): the ray semantics of reference src/trace_dev.cu (__raygen__bits, __raygen__view)
+ * and src/params.h restated on the CPU, over a host LBVH (Karras 2012) whose build is the one the GPU build must
+ * reproduce bit for bit.
+ *
+ * What is restated, and how it differs from the reference where the reference cannot be reproduced off OptiX:
+ *  - offset_ray (trace_dev.cu:16-25), tmin 1e-5, the hemisphere cull (:101), the own-plane rule (:53-88) with its
+ *    constants (2^-22 x the triangle's extent, the 1/32 m gate, at most 3 own planes passed), views' frame, cells,
+ *    jitter hash, virtual ground and closest-hit loop (:177-232) with labels and panes.
+ *  - The triangle test is the watertight test of Woop, Benthin & Wald (JCGT 2(1), 2013), not OptiX's hardware test.
+ *  - The first hit of a visibility ray: OptiX reports any hit; here a ray is open when every hit on [1e-5, inf) is an
+ *    own-plane hit (gate, plane, facing) and they lie at 3 distinct distances at most. That is the reference's walk for
+ *    every order OptiX may report hits in, except exact ties of an own-plane hit with a blocker.
+ *  - Closest hits break exact ties of t by the lower triangle index.
+ *  - rsqrtf / sincospif (CUDA fast math) are 1 / sqrtf and a polynomial sincospi (Taylor to degree 9/10 in float fma),
+ *    the same operations the HIP kernels run; logs of view distances are double log.
+ * All float arithmetic here is compiled without contraction (-ffp-contract=off); fmaf is used only where named.
+ */
+#include "orc.h"
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- small helpers */
+static float f_as(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t u_as(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* offset_ray (trace_dev.cu:16-25): 256 ulps along n per axis, or 2^-16 n within 1/32 m of the origin */
+void orc_offset_ray(const float p[3], const float n[3], float out[3]) {
+  for (int a = 0; a < 3; ++a) {
+    const int o = (int)(256.f * n[a]);
+    const float stepped = f_as((uint32_t)((int32_t)u_as(p[a]) + (p[a] < 0.f ? -o : o)));
+    out[a] = fabsf(p[a]) < (1.f / 32.f) ? p[a] + (1.f / 65536.f) * n[a] : stepped;
+  }
+}
+
+/* pcg3d (Jarzynski & Olano 2020), params.h:64-70 */
+void orc_pcg3d(uint32_t x, uint32_t y, uint32_t z, uint32_t out[3]) {
+  x = x * 1664525u + 1013904223u; y = y * 1664525u + 1013904223u; z = z * 1664525u + 1013904223u;
+  x += y * z; y += z * x; z += x * y;
+  x ^= x >> 16; y ^= y >> 16; z ^= z >> 16;
+  x += y * z; y += z * x; z += x * y;
+  out[0] = x; out[1] = y; out[2] = z;
+}
+static float u01(uint32_t v) { return (float)(v >> 8) * (1.f / 16777216.f); }
+
+/* sin(pi x), cos(pi x) in float: reduce to |r| <= 1/4 by quarter turns, Taylor polynomials (degree 9 / 10) by fmaf */
+static void sincospi_f(float x, float* s, float* c) {
+  const float q = rintf(x * 2.f);          /* nearest quarter turn (x * 2 is exact) */
+  const float r = x - q * 0.5f;            /* exact: q / 2 is a nearby multiple of 1/2 */
+  const int k = (int)((long long)q & 3);
+  const float r2 = r * r;
+  float ps = 8.21458866e-2f;               /* pi^9 / 9! */
+  ps = fmaf(ps, r2, -5.99264528e-1f);      /* -pi^7 / 7! */
+  ps = fmaf(ps, r2, 2.55016404e+0f);       /* pi^5 / 5! */
+  ps = fmaf(ps, r2, -5.16771278e+0f);      /* -pi^3 / 3! */
+  ps = fmaf(ps, r2, 3.14159274e+0f);       /* pi */
+  const float sr = ps * r;
+  float pc = -2.58068914e-2f;              /* -pi^10 / 10! */
+  pc = fmaf(pc, r2, 2.35330630e-1f);       /* pi^8 / 8! */
+  pc = fmaf(pc, r2, -1.33526277e+0f);      /* -pi^6 / 6! */
+  pc = fmaf(pc, r2, 4.05871213e+0f);       /* pi^4 / 4! */
+  pc = fmaf(pc, r2, -4.93480220e+0f);      /* -pi^2 / 2! */
+  const float cr = fmaf(pc, r2, 1.f);
+  switch (k) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+  }
+}
+
+/* view_cell (params.h:77-85): the cell's point in the field frame (x right, y forward, z up) */
+void orc_view_cell(unsigned d, unsigned col, float rsLo, float rsHi, float az0, float dAz, int jitter, unsigned element,
+                   unsigned side, float out[3]) {
+  float a = 0.5f, s = 0.5f * (rsLo + rsHi);
+  if (jitter) {
+    uint32_t h[3];
+    orc_pcg3d(element, d, side, h);
+    a = u01(h[0]);
+    s = rsLo + u01(h[1]) * (rsHi - rsLo);
+  }
+  float sa, ca;
+  sincospi_f((az0 + ((float)col + a) * dAz) * (1.f / 180.f), &sa, &ca);
+  const float cc = sqrtf(fmaxf(0.f, 1.f - s * s));
+  out[0] = cc * sa; out[1] = cc * ca; out[2] = s;
+}
+
+/* ---------------------------------------------------------------- the watertight triangle test */
+typedef struct {
+  float o[3], d[3];
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+  float inv[3], oNear[3], oFar[3];   /* box test: per axis the padded plane offsets, near / far by the sign of d */
+} ray_t;
+
+static void ray_setup(ray_t* r, const float o[3], const float d[3], float pad) {
+  memcpy(r->o, o, 12); memcpy(r->d, d, 12);
+  const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+  r->kz = ax >= ay ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+  r->kx = r->kz == 2 ? 0 : r->kz + 1;
+  r->ky = r->kx == 2 ? 0 : r->kx + 1;
+  if (d[r->kz] < 0.f) { const int t = r->kx; r->kx = r->ky; r->ky = t; }
+  r->Sx = d[r->kx] / d[r->kz];
+  r->Sy = d[r->ky] / d[r->kz];
+  r->Sz = 1.f / d[r->kz];
+  for (int a = 0; a < 3; ++a) {
+    const float di = d[a] != 0.f ? d[a] : (signbit(d[a]) ? -1e-30f : 1e-30f);
+    r->inv[a] = 1.f / di;
+    /* t of the padded near plane = (lo - pad - o) inv for inv > 0: store o + pad / o - pad per side */
+    if (r->inv[a] >= 0.f) { r->oNear[a] = o[a] + pad; r->oFar[a] = o[a] - pad; }
+    else { r->oNear[a] = o[a] - pad; r->oFar[a] = o[a] + pad; }
+  }
+}
+
+/* Hit with t in [tmin, tmax]: 1 and t, front (the ray meets the side the winding's normal faces: det > 0). */
+static int tri_hit(const ray_t* r, const float* v0, const float* v1, const float* v2, float tmin, float tmax, int cull, float* tOut,
+                   int* front) {
+  const float A[3] = {v0[0] - r->o[0], v0[1] - r->o[1], v0[2] - r->o[2]};
+  const float B[3] = {v1[0] - r->o[0], v1[1] - r->o[1], v1[2] - r->o[2]};
+  const float C[3] = {v2[0] - r->o[0], v2[1] - r->o[1], v2[2] - r->o[2]};
+  const float Ax = A[r->kx] - r->Sx * A[r->kz], Ay = A[r->ky] - r->Sy * A[r->kz];
+  const float Bx = B[r->kx] - r->Sx * B[r->kz], By = B[r->ky] - r->Sy * B[r->kz];
+  const float Cx = C[r->kx] - r->Sx * C[r->kz], Cy = C[r->ky] - r->Sy * C[r->kz];
+  float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {   /* on an edge in float: decide in double (exact products) */
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return 0;
+  const float det = U + V + W;
+  if (det == 0.f) return 0;
+  if (cull && det < 0.f) return 0;
+  const float Az = r->Sz * A[r->kz], Bz = r->Sz * B[r->kz], Cz = r->Sz * C[r->kz];
+  const float T = U * Az + V * Bz + W * Cz;
+  const float t = T / det;
+  if (!(t >= tmin && t <= tmax)) return 0;
+  *tOut = t;
+  *front = det > 0.f;
+  return 1;
+}
+
+/* own_hit (trace_dev.cu:60-72): under the 1/32 m gate, non-degenerate, its plane within 2^-22 x its extent of the
+ * origin and facing away from the sensor's normal n */
+static int own_hit(const float d[3], const float n[3], float t, int front, const float* v0, const float* v1, const float* v2) {
+  if (!(t * dot3(n, d) < 1.f / 32.f)) return 0;
+  const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+  const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+  const float ng[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  const float l2 = dot3(ng, ng);
+  if (!(l2 > 0.f)) return 0;
+  const float s = front ? 1.f / sqrtf(l2) : -(1.f / sqrtf(l2));
+  const float hn[3] = {s * ng[0], s * ng[1], s * ng[2]};
+  const float ext = fmaxf(fmaxf(fmaxf(fabsf(e1[0]), fabsf(e1[1])), fmaxf(fabsf(e1[2]), fabsf(e2[0]))), fmaxf(fabsf(e2[1]), fabsf(e2[2])));
+  return t * fabsf(dot3(d, hn)) < ext * 2.3841858e-7f && dot3(n, hn) <= 0.f;
+}
+
+/* ---------------------------------------------------------------- host LBVH (Karras 2012) */
+struct orc_scene {
+  long nt;
+  float pad;            /* box padding: 2^-18 x (largest |coordinate| + 1) */
+  float* tri;           /* [nt][12]: v0 xyz, v1 xyz, v2 xyz in sorted (leaf) order, 3 floats unused (prim as bits) */
+  uint32_t* prim;       /* original index per leaf */
+  float* node;          /* [nt - 1 or 1][16]: left box lo xyz hi xyz, right box lo xyz hi xyz, left, right (as bits), 2 unused */
+  long nNode;
+};
+
+static uint64_t spread21(uint32_t v) {
+  uint64_t x = v & 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+static int clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+static int clz32(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+typedef struct { uint64_t key; uint32_t idx; } kv_t;
+static int kv_cmp(const void* a, const void* b) {
+  const kv_t *x = a, *y = b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : x->idx > y->idx;
+}
+static int delta(const kv_t* k, long n, long i, long j) {
+  if (j < 0 || j >= n) return -1;
+  if (k[i].key == k[j].key) return 64 + clz32((uint32_t)i ^ (uint32_t)j);
+  return clz64(k[i].key ^ k[j].key);
+}
+
+/* Morton code of each triangle's centroid ((v0 + v1) + v2) / 3 in the centroid box, 21 bits per axis, x highest. */
+void orc_lbvh_keys(const float* V, const uint32_t* I, long nt, uint64_t* keys) {
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  float* cen = malloc(sizeof(float) * 3 * (size_t)(nt ? nt : 1));
+  for (long t = 0; t < nt; ++t)
+    for (int a = 0; a < 3; ++a) {
+      const float c = ((V[3 * I[3 * t] + a] + V[3 * I[3 * t + 1] + a]) + V[3 * I[3 * t + 2] + a]) * (1.f / 3.f);
+      cen[3 * t + a] = c;
+      lo[a] = fminf(lo[a], c);
+      hi[a] = fmaxf(hi[a], c);
+    }
+  float sc[3];
+  for (int a = 0; a < 3; ++a) { const float e = hi[a] - lo[a]; sc[a] = e > 0.f ? 1.f / e : 0.f; }
+  for (long t = 0; t < nt; ++t) {
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+      const float u = (cen[3 * t + a] - lo[a]) * sc[a];
+      q[a] = (uint32_t)fminf(u * 2097152.f, 2097151.f);
+    }
+    keys[t] = spread21(q[0]) << 2 | spread21(q[1]) << 1 | spread21(q[2]);
+  }
+  free(cen);
+}
+
+/* Karras 2012: node i's children from the sorted keys. child >= 0x80000000: leaf (child & 0x7fffffff). */
+void orc_lbvh_topology(const uint64_t* sortedKeys, long n, uint32_t* left, uint32_t* right) {
+  kv_t* k = malloc(sizeof(kv_t) * (size_t)n);
+  for (long i = 0; i < n; ++i) { k[i].key = sortedKeys[i]; k[i].idx = (uint32_t)i; }
+  for (long i = 0; i + 1 < n; ++i) {
+    const int d = delta(k, n, i, i + 1) - delta(k, n, i, i - 1) > 0 ? 1 : -1;
+    const int dmin = delta(k, n, i, i - d);
+    long lmax = 2;
+    while (delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    long l = 0;
+    for (long t = lmax / 2; t >= 1; t /= 2)
+      if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const long j = i + l * d;
+    const int dnode = delta(k, n, i, j);
+    long s = 0;
+    for (long div = 2;; div *= 2) {
+      const long t = (l + div - 1) / div;
+      if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+      if (t <= 1) break;
+    }
+    const long g = i + s * d + (d < 0 ? -1 : 0);
+    const long mn = i < j ? i : j, mx = i < j ? j : i;
+    left[i] = mn == g ? 0x80000000u | (uint32_t)g : (uint32_t)g;
+    right[i] = mx == g + 1 ? 0x80000000u | (uint32_t)(g + 1) : (uint32_t)(g + 1);
+  }
+  free(k);
+}
+
+static void box_of_child(const orc_scene* s, uint32_t c, float b[6], const float* boxes) {
+  if (c & 0x80000000u) {
+    const float* t = s->tri + 12 * (c & 0x7fffffffu);
+    for (int a = 0; a < 3; ++a) {
+      b[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+      b[3 + a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+    }
+  } else memcpy(b, boxes + 6 * c, 24);
+}
+
+orc_scene* orc_scene_new(const float* V, long nv, const uint32_t* I, long nt) {
+  orc_scene* s = calloc(1, sizeof *s);
+  s->nt = nt;
+  float mx = 0.f;
+  for (long i = 0; i < 3 * nv; ++i) mx = fmaxf(mx, fabsf(V[i]));
+  s->pad = (mx + 1.f) * 3.81469727e-6f;   /* 2^-18 */
+  if (nt == 0) return s;
+  uint64_t* keys = malloc(sizeof(uint64_t) * (size_t)nt);
+  orc_lbvh_keys(V, I, nt, keys);
+  kv_t* kv = malloc(sizeof(kv_t) * (size_t)nt);
+  for (long t = 0; t < nt; ++t) { kv[t].key = keys[t]; kv[t].idx = (uint32_t)t; }
+  qsort(kv, (size_t)nt, sizeof(kv_t), kv_cmp);   /* (key, index): the order a stable radix sort gives */
+  s->tri = calloc((size_t)nt * 12, sizeof(float));
+  s->prim = malloc(sizeof(uint32_t) * (size_t)nt);
+  for (long k = 0; k < nt; ++k) {
+    const uint32_t t = kv[k].idx;
+    s->prim[k] = t;
+    for (int c = 0; c < 3; ++c) memcpy(s->tri + 12 * k + 3 * c, V + 3 * I[3 * t + c], 12);
+    const float pb = f_as(t);
+    s->tri[12 * k + 9] = pb;
+  }
+  for (long k = 0; k < nt; ++k) keys[k] = kv[k].key;
+  const long nn = nt > 1 ? nt - 1 : 1;
+  uint32_t* L = malloc(sizeof(uint32_t) * (size_t)nn);
+  uint32_t* R = malloc(sizeof(uint32_t) * (size_t)nn);
+  if (nt == 1) { L[0] = R[0] = 0x80000000u; }
+  else orc_lbvh_topology(keys, nt, L, R);
+  /* boxes bottom-up: post-order over the tree from the root */
+  float* boxes = malloc(sizeof(float) * 6 * (size_t)nn);
+  s->node = calloc((size_t)nn * 16, sizeof(float));
+  s->nNode = nn;
+  uint32_t* stack = malloc(sizeof(uint32_t) * 2 * (size_t)(nn + 2));
+  long sp = 0;
+  stack[sp++] = 0;   /* node, visited flag in bit 31 */
+  while (sp) {
+    const uint32_t e = stack[--sp], i = e & 0x7fffffffu;
+    if (!(e & 0x80000000u)) {
+      stack[sp++] = i | 0x80000000u;
+      if (!(R[i] & 0x80000000u)) stack[sp++] = R[i];
+      if (!(L[i] & 0x80000000u)) stack[sp++] = L[i];
+      continue;
+    }
+    float bl[6], br[6];
+    box_of_child(s, L[i], bl, boxes);
+    box_of_child(s, R[i], br, boxes);
+    float* nd = s->node + 16 * i;
+    memcpy(nd, bl, 24);
+    memcpy(nd + 6, br, 24);
+    nd[12] = f_as(L[i]);
+    nd[13] = f_as(R[i]);
+    for (int a = 0; a < 3; ++a) { boxes[6 * i + a] = fminf(bl[a], br[a]); boxes[6 * i + 3 + a] = fmaxf(bl[3 + a], br[3 + a]); }
+  }
+  free(stack); free(boxes); free(L); free(R); free(kv); free(keys);
+  return s;
+}
+
+void orc_scene_free(orc_scene* s) {
+  if (!s) return;
+  free(s->tri); free(s->prim); free(s->node); free(s);
+}
+
+/* The scene's arrays as the GPU build must produce them (tests compare them bit for bit). */
+long orc_scene_nodes(const orc_scene* s, const float** node, const float** tri, const uint32_t** prim) {
+  *node = s->node; *tri = s->tri; *prim = s->prim;
+  return s->nNode;
+}
+float orc_scene_pad(const orc_scene* s) { return s->pad; }
+
+/* slab test against the padded box; 1 with the entry distance */
+static int box_hit(const ray_t* r, const float* b, float tmin, float tmax, float* tEnter) {
+  float t0 = tmin, t1 = tmax;
+  for (int a = 0; a < 3; ++a) {
+    const float lo = r->inv[a] >= 0.f ? b[a] : b[3 + a], hi = r->inv[a] >= 0.f ? b[3 + a] : b[a];
+    const float tn = (lo - r->oNear[a]) * r->inv[a], tf = (hi - r->oFar[a]) * r->inv[a];
+    t0 = fmaxf(t0, tn);
+    t1 = fminf(t1, tf);
+  }
+  *tEnter = t0;
+  return t0 <= t1;
+}
+
+/* Every hit on [tmin, tmax]: f(leaf) returns 1 to stop. Depth-first, near child first. */
+typedef int (*leaf_fn)(void* ctx, long leaf);
+static void traverse(const orc_scene* s, const ray_t* r, float tmin, const float* tmax, leaf_fn f, void* ctx) {
+  if (s->nt == 0) return;
+  uint32_t stack[128];
+  int sp = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    const float* nd = s->node + 16 * cur;
+    float tl, tr;
+    const int hl = box_hit(r, nd, tmin, *tmax, &tl), hr = box_hit(r, nd + 6, tmin, *tmax, &tr);
+    const uint32_t cl = u_as(nd[12]), cr = u_as(nd[13]);
+    uint32_t next[2];
+    int nn = 0;
+    /* leaves first (their hits shrink tmax), then the nearer inner child */
+    if (hl && (cl & 0x80000000u)) { if (f(ctx, cl & 0x7fffffffu)) return; }
+    if (hr && (cr & 0x80000000u)) { if (f(ctx, cr & 0x7fffffffu)) return; }
+    const int il = hl && !(cl & 0x80000000u), ir = hr && !(cr & 0x80000000u);
+    if (il && ir) {
+      if (tl <= tr) { next[0] = cl; next[1] = cr; } else { next[0] = cr; next[1] = cl; }
+      nn = 2;
+    } else if (il) { next[0] = cl; nn = 1; }
+    else if (ir) { next[0] = cr; nn = 1; }
+    if (nn == 2) stack[sp++] = next[1];
+    if (nn >= 1) { cur = next[0]; continue; }
+    if (!sp) return;
+    cur = stack[--sp];
+  }
+}
+
+/* ---------------------------------------------------------------- visibility (bits) */
+typedef struct {
+  const orc_scene* s;
+  const ray_t* r;
+  const float* n;
+  int cull, blocked, nt;
+  float ts[3];
+} vis_ctx;
+static int vis_leaf(void* c, long leaf) {
+  vis_ctx* v = c;
+  const float* T = v->s->tri + 12 * leaf;
+  float t;
+  int front;
+  if (!tri_hit(v->r, T, T + 3, T + 6, 1e-5f, FLT_MAX, v->cull, &t, &front)) return 0;
+  if (!own_hit(v->r->d, v->n, t, front, T, T + 3, T + 6)) { v->blocked = 1; return 1; }
+  for (int k = 0; k < v->nt; ++k) if (v->ts[k] == t) return 0;
+  if (v->nt == 3) { v->blocked = 1; return 1; }   /* a fourth own plane: the walk gives up (trace_dev.cu:79-87) */
+  v->ts[v->nt++] = t;
+  return 0;
+}
+
+static int visible_from(const orc_scene* s, const float o[3], const float n[3], const float d[3], int cull) {
+  if (n[0] * d[0] + n[1] * d[1] + n[2] * d[2] <= 0.f) return 0;   /* hemisphere cull */
+  ray_t r;
+  ray_setup(&r, o, d, s->pad);
+  vis_ctx v = {s, &r, n, cull, 0, 0, {0, 0, 0}};
+  const float tmax = FLT_MAX;
+  traverse(s, &r, 1e-5f, &tmax, vis_leaf, &v);
+  return !v.blocked;
+}
+
+int orc_visible(const orc_scene* s, const float p[3], const float n[3], const float d[3], int cull) {
+  float o[3];
+  orc_offset_ray(p, n, o);
+  return visible_from(s, o, n, d, cull);
+}
+
+typedef struct {
+  const orc_scene* s;
+  const float *P, *N, *D;
+  long nc;
+  int nDir, cull;
+  uint32_t* bits;
+  long next;
+  pthread_mutex_t mu;
+  /* views */
+  const orc_view* v;
+  unsigned i0;
+  uint8_t* cls;
+  float* dist;
+} job_t;
+
+static long take(job_t* j, long chunk) {
+  pthread_mutex_lock(&j->mu);
+  const long i = j->next;
+  j->next += chunk;
+  pthread_mutex_unlock(&j->mu);
+  return i;
+}
+
+static void* bits_worker(void* arg) {
+  job_t* j = arg;
+  const int nWords = (j->nDir + 31) / 32;
+  for (;;) {
+    const long i0 = take(j, 16);
+    if (i0 >= j->nc) break;
+    const long i1 = i0 + 16 < j->nc ? i0 + 16 : j->nc;
+    for (long i = i0; i < i1; ++i) {
+      const float* n = j->N + 3 * i;
+      float o[3];
+      orc_offset_ray(j->P + 3 * i, n, o);
+      for (int w = 0; w < nWords; ++w) {
+        uint32_t word = 0;
+        for (int k = 0; k < 32 && w * 32 + k < j->nDir; ++k)
+          if (visible_from(j->s, o, n, j->D + 3 * (w * 32 + k), j->cull)) word |= 1u << k;
+        j->bits[(size_t)w * j->nc + i] = word;
+      }
+    }
+  }
+  return NULL;
+}
+
+static void run_workers(job_t* j, void* (*fn)(void*), int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  pthread_t th[256];
+  pthread_mutex_init(&j->mu, NULL);
+  for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, fn, j);
+  fn(j);
+  for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+  pthread_mutex_destroy(&j->mu);
+}
+
+void orc_trace_bits(const orc_scene* s, const float* P, const float* N, long nc, const float* D, int nDir, int cull, uint32_t* bits,
+                    int nthreads) {
+  job_t j;
+  memset(&j, 0, sizeof j);
+  j.s = s; j.P = P; j.N = N; j.nc = nc; j.D = D; j.nDir = nDir; j.cull = cull; j.bits = bits;
+  run_workers(&j, bits_worker, nthreads);
+}
+
+/* ---------------------------------------------------------------- views (closest hits) */
+typedef struct {
+  const orc_scene* s;
+  const ray_t* r;
+  float tmin, tmax;
+  int cull;
+  long best;        /* leaf of the closest hit, -1 none */
+  uint32_t bestPrim;
+  int front;
+} ch_ctx;
+static int ch_leaf(void* c, long leaf) {
+  ch_ctx* h = c;
+  const float* T = h->s->tri + 12 * leaf;
+  float t;
+  int front;
+  if (!tri_hit(h->r, T, T + 3, T + 6, h->tmin, h->tmax, h->cull, &t, &front)) return 0;
+  const uint32_t prim = h->s->prim[leaf];
+  if (h->best >= 0 && t == h->tmax && prim > h->bestPrim) return 0;   /* ties: the lower triangle index */
+  h->tmax = t;
+  h->best = leaf;
+  h->bestPrim = prim;
+  h->front = front;
+  return 0;
+}
+/* closest hit on [tmin, tmax]: 1 with t, leaf, front */
+static int closest(const orc_scene* s, const ray_t* r, float tmin, float tmax, int cull, float* t, long* leaf, int* front) {
+  ch_ctx h = {s, r, tmin, tmax, cull, -1, 0, 0};
+  traverse(s, r, tmin, &h.tmax, ch_leaf, &h);
+  if (h.best < 0) return 0;
+  *t = h.tmax; *leaf = h.best; *front = h.front;
+  return 1;
+}
+
+static void view_one(const job_t* j, long i) {
+  const orc_view* v = j->v;
+  const float* n = j->N + 3 * i;
+  const unsigned gi = j->i0 + (unsigned)i, element = gi / v->nS, side = v->nS == 2 ? gi % 2 : (unsigned)v->back;
+  float o[3];
+  orc_offset_ray(j->P + 3 * i, n, o);
+  const float inf = f_as(0x7f800000u);
+  const int none = !(dot3(n, n) > 0.f);
+  float f[3], r[3], u[3];
+  if (v->world) { f[0] = -v->sinN; f[1] = v->cosN; f[2] = 0.f; r[0] = v->cosN; r[1] = v->sinN; r[2] = 0.f; u[0] = 0.f; u[1] = 0.f; u[2] = 1.f; }
+  else {
+    const float l = sqrtf(n[0] * n[0] + n[1] * n[1]);
+    memcpy(f, n, 12);
+    if (l > 1e-3f) { r[0] = n[1] / l; r[1] = -n[0] / l; r[2] = 0.f; }
+    else {
+      r[0] = v->cosN; r[1] = v->sinN; r[2] = 0.f;
+      if (l > 0.f) {
+        const float k = dot3(r, n);
+        for (int a = 0; a < 3; ++a) r[a] = r[a] - k * n[a];
+        const float s = 1.f / sqrtf(dot3(r, r));
+        for (int a = 0; a < 3; ++a) r[a] = s * r[a];
+      }
+    }
+    u[0] = r[1] * f[2] - r[2] * f[1]; u[1] = r[2] * f[0] - r[0] * f[2]; u[2] = r[0] * f[1] - r[1] * f[0];
+  }
+  const unsigned K = v->nAz * v->nEl;
+  for (unsigned d = 0; d < K; ++d) {
+    float l[3];
+    const unsigned row = d / v->nAz;
+    orc_view_cell(d, d % v->nAz, v->rowSin[2 * row], v->rowSin[2 * row + 1], v->az0, v->dAz, v->jitter, element, side, l);
+    float dir[3];
+    for (int a = 0; a < 3; ++a) dir[a] = (l[0] * r[a] + l[1] * f[a]) + l[2] * u[a];
+    uint8_t cl = 2;
+    float t = inf;
+    if (none || (v->cull && dot3(n, dir) <= 0.f)) cl = 254;
+    else if (v->vground && dir[2] < 0.f && !(o[2] > v->groundZ)) { cl = 1; t = 0.1f; }
+    else {
+      const int plane = v->vground && dir[2] < 0.f;
+      const float tg = plane ? (o[2] - v->groundZ) / -dir[2] : inf;
+      const float tmax = plane ? (o[2] - v->groundZ + 0.01f) / -dir[2] : FLT_MAX;
+      ray_t ry;
+      ray_setup(&ry, o, dir, j->s->pad);
+      float tmin = 1e-5f;
+      unsigned own = 0, panes = 0;
+      for (;;) {
+        float th;
+        long leaf;
+        int front;
+        if (!closest(j->s, &ry, tmin, tmax, j->cull, &th, &leaf, &front)) { cl = dir[2] >= 0.f ? 0 : 1; t = tg; break; }
+        t = th;
+        const float* T = j->s->tri + 12 * leaf;
+        if (own_hit(dir, n, t, front, T, T + 3, T + 6)) { if (++own == 4) break; }
+        else {
+          const uint8_t lb = v->label ? v->label[j->s->prim[leaf]] : 2;
+          if (lb != 255) { cl = lb; break; }
+          if (++panes > 16) break;
+        }
+        tmin = f_as(u_as(t) + 1);
+      }
+      if (panes) cl |= 0x80;
+    }
+    j->cls[(size_t)d * j->nc + i] = cl;
+    j->dist[(size_t)d * j->nc + i] = t;
+  }
+}
+
+static void* view_worker(void* arg) {
+  job_t* j = arg;
+  for (;;) {
+    const long i0 = take(j, 4);
+    if (i0 >= j->nc) break;
+    for (long i = i0; i < i0 + 4 && i < j->nc; ++i) view_one(j, i);
+  }
+  return NULL;
+}
+
+void orc_trace_view(const orc_scene* s, const float* P, const float* N, long nc, unsigned i0, const orc_view* v, int cull, uint8_t* cls,
+                    float* dist, int nthreads) {
+  job_t j;
+  memset(&j, 0, sizeof j);
+  j.s = s; j.P = P; j.N = N; j.nc = nc; j.v = v; j.i0 = i0; j.cls = cls; j.dist = dist; j.cull = cull;
+  run_workers(&j, view_worker, nthreads);
+}
+
+/* Brute force over every triangle (no BVH): the hit sets the traversal must reproduce. */
+int orc_visible_brute(const float* V, const uint32_t* I, long nt, const float p[3], const float n[3], const float d[3], int cull) {
+  float o[3];
+  orc_offset_ray(p, n, o);
+  if (n[0] * d[0] + n[1] * d[1] + n[2] * d[2] <= 0.f) return 0;
+  ray_t r;
+  ray_setup(&r, o, d, 0.f);
+  float ts[3];
+  int nts = 0;
+  for (long k = 0; k < nt; ++k) {
+    const float *a = V + 3 * I[3 * k], *b = V + 3 * I[3 * k + 1], *c = V + 3 * I[3 * k + 2];
+    float t;
+    int front;
+    if (!tri_hit(&r, a, b, c, 1e-5f, FLT_MAX, cull, &t, &front)) continue;
+    if (!own_hit(d, n, t, front, a, b, c)) return 0;
+    int seen = 0;
+    for (int q = 0; q < nts; ++q) seen |= ts[q] == t;
+    if (seen) continue;
+    if (nts == 3) return 0;
+    ts[nts++] = t;
+  }
+  return 1;
+}

@@ -1,0 +1,5 @@
+/* Oracle build helper: glibc's <time.h> declares `extern int daylight`, which collides with the reference's namespace
+ * `daylight`. Including <time.h> once under another name first keeps both. */
+#define daylight glibc_daylight_variable
+#include <time.h>
+#undef daylight
