@@ -1,0 +1,289 @@
+#!/usr/bin/env python3
+"""Throughput of the daylight trace-and-accumulate path on one MI355X: sensor-hours/s of an annual run with its series.
+
+Workload (seeded urban block): 400 extruded boxes (footprint sides U(10, 40) m, heights lognormal with median 15 m and
+sigma 0.5) on a 20 x 20 grid of 50 m plots are the context; the roofs and facades of the 40 buildings nearest the centre,
+cut into quads of at most 1 m, are the sensors (they sit on the context boxes' faces and do not block themselves,
+--no-self-occlude). Weather: the library's own clear sky (Ineichen, turbidity 3) at lat 52.52 lon 13.41 tz 1, with an
+hourly cloud cover c ~ Beta(2, 3) taking (1 - c) of the beam and moving half of what it took into the diffuse part,
+written to an EPW. Run: --sky-density 2 (577 patches), 8760 hourly steps, radiation plus its --timeseries; the series
+stays on the GPU (dlx_run_series_device), file output is excluded from the timing.
+
+A step is one such run through the C ABI with the context scene already resident on the GPU: the sky and sun of every
+step (FP64), the visibility traversal, the per-sensor reductions and the series contraction. The timed region holds
+exactly --steps runs, bracketed by torch.cuda.synchronize(). The line also carries the traversal's roofline (HIP-event
+times of its launches, counted node and triangle fetches), the three contraction methods side by side (FP32, BF16x3,
+BF16x6) and the oracle's CPU time on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "kit-sim_daylight_amd"))
+
+LAT, LON, TZ = 52.52, 13.41, 1.0
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES = 64, 48
+
+
+# ---------------------------------------------------------------- synthetic inputs
+
+def city(seed=1, n_side=20, pitch=50.0, n_sensor_buildings=40):
+    """Boxes (x0, y0, x1, y1, h) of the block and the indices of the central sensor buildings."""
+    rng = np.random.default_rng(seed)
+    n = n_side * n_side
+    a = rng.uniform(10, 40, n)
+    b = rng.uniform(10, 40, n)
+    h = rng.lognormal(math.log(15.0), 0.5, n)
+    gx, gy = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")
+    cx = (gx.ravel() + 0.5 - n_side / 2) * pitch
+    cy = (gy.ravel() + 0.5 - n_side / 2) * pitch
+    # a 5 m street at least: the box moves within its plot
+    cx = cx + rng.uniform(-1, 1, n) * (pitch - 5 - a) / 2
+    cy = cy + rng.uniform(-1, 1, n) * (pitch - 5 - b) / 2
+    boxes = np.stack([cx - a / 2, cy - b / 2, cx + a / 2, cy + b / 2, h], 1)
+    central = np.argsort(np.hypot(cx, cy), kind="stable")[:n_sensor_buildings]
+    return boxes, np.sort(central)
+
+
+def box_mesh(boxes):
+    """The boxes as quads (bottom, top, 4 sides), outward normals."""
+    pts, idx = [], []
+    for k, (x0, y0, x1, y1, h) in enumerate(boxes):
+        v = np.array([[x0, y0, 0], [x1, y0, 0], [x1, y1, 0], [x0, y1, 0], [x0, y0, h], [x1, y0, h], [x1, y1, h], [x0, y1, h]])
+        pts.append(v)
+        q = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]]) + 8 * k
+        idx.append(q)
+    pts, idx = np.concatenate(pts), np.concatenate(idx)
+    return pts, np.full(len(idx), 4, np.int32), idx.ravel().astype(np.int32)
+
+
+def grid_quads(origin, u, v, nu, nv):
+    """nu x nv quads over the parallelogram origin + s u + t v (s, t in [0, 1]); normal u x v."""
+    s = np.linspace(0, 1, nu + 1)
+    t = np.linspace(0, 1, nv + 1)
+    S, T = np.meshgrid(s, t, indexing="ij")
+    P = origin + S[..., None] * u + T[..., None] * v
+    pts = P.reshape(-1, 3)
+    i = np.arange(nu)[:, None]
+    j = np.arange(nv)[None, :]
+    a = i * (nv + 1) + j
+    quads = np.stack([a, a + nv + 1, a + nv + 2, a + 1], -1).reshape(-1, 4)
+    return pts, quads
+
+
+def sensor_mesh(boxes):
+    """Roofs and facades of the boxes in quads of at most 1 m, facing out."""
+    pts, idx, base = [], [], 0
+    for x0, y0, x1, y1, h in boxes:
+        a, b = x1 - x0, y1 - y0
+        na, nb, nh = int(math.ceil(a)), int(math.ceil(b)), int(math.ceil(h))
+        faces = [((x0, y0, h), (a, 0, 0), (0, b, 0), na, nb),           # roof, +z
+                 ((x0, y0, 0), (a, 0, 0), (0, 0, h), na, nh),           # y0 side, -y
+                 ((x1, y0, 0), (0, b, 0), (0, 0, h), nb, nh),           # x1 side, +x
+                 ((x1, y1, 0), (-a, 0, 0), (0, 0, h), na, nh),          # y1 side, +y
+                 ((x0, y1, 0), (0, -b, 0), (0, 0, h), nb, nh)]          # x0 side, -x
+        for o, u, v, nu, nv in faces:
+            p, q = grid_quads(np.array(o, float), np.array(u, float), np.array(v, float), nu, nv)
+            pts.append(p)
+            idx.append(q + base)
+            base += len(p)
+    pts, idx = np.concatenate(pts), np.concatenate(idx)
+    return pts, np.full(len(idx), 4, np.int32), idx.ravel().astype(np.int32)
+
+
+def write_epw(path, dni, dhi):
+    rows = ["LOCATION,Synthetic,-,DEU,synthetic,000000,%.2f,%.2f,%.1f,0.0" % (LAT, LON, TZ),
+            "DESIGN CONDITIONS,0", "TYPICAL/EXTREME PERIODS,0", "GROUND TEMPERATURES,0",
+            "HOLIDAYS/DAYLIGHT SAVINGS,No,0,0,0", "COMMENTS 1,clear sky x Beta(2 3) cloud cover",
+            "COMMENTS 2,", "DATA PERIODS,1,1,Data,Sunday, 1/ 1,12/31"]
+    days = [31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31]
+    r = 0
+    for m, nd in enumerate(days, 1):
+        for d in range(1, nd + 1):
+            for hr in range(1, 25):
+                ghi = dni[r] + dhi[r]
+                f = [1999, m, d, hr, 0, "?", 10.0, 5.0, 70, 101325, 0, 0, 300, "%.1f" % ghi, "%.1f" % dni[r], "%.1f" % dhi[r]]
+                f += [0] * 19
+                rows.append(",".join(str(x) for x in f))
+                r += 1
+    with open(path, "w") as fh:
+        fh.write("\n".join(rows) + "\n")
+
+
+def weather(native, tmp, seed=2):
+    """Clear-sky DNI / DHI from the library (--clear-sky, --dump-sun), cloud-modulated; -> (dni, dhi) [8760] float32."""
+    sun = os.path.join(tmp, "sun.txt")
+    with native.SimulationMesh("probe", [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], [4], [0, 1, 2, 3]) as m:
+        native.run("daylight", ["--clear-sky", "--lat", LAT, "--lon", LON, "--tz", TZ, "--quantity", "sunhours",
+                                "--dump-sun", sun], m)
+    d = np.loadtxt(sun, comments="#")
+    alt, dni_cs, dhi_cs = d[:, 4], d[:, 7], d[:, 8]
+    assert len(d) == 8760, len(d)
+    c = np.random.default_rng(seed).beta(2, 3, 8760)
+    dni = dni_cs * (1 - c)
+    dhi = dhi_cs + 0.5 * c * dni_cs * np.maximum(0, np.sin(np.radians(alt)))
+    return dni.astype(np.float32), dhi.astype(np.float32)
+
+
+# ---------------------------------------------------------------- the CPU baseline (the oracle, on a bounded sample)
+
+def cpu_baseline(ctx, sens, dni, dhi, target_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.build()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    wx = dict(lat=LAT, lon=LON, tz=TZ, leap=False, dni=dni, dhi=dhi)
+    ctx_m = orc.MeshArrays("city", ctx[0], ctx[1], ctx[2], sensor=False)
+
+    def sample(nf):
+        pts, counts, idx = sens
+        sub = orc.MeshArrays("sensors", pts, counts[:nf], idx[: 4 * nf], sensor=True)
+        t0 = time.perf_counter()
+        r = orc.daylight([sub, ctx_m], weather=wx, quantities=("radiation",), mf=2, selfOcclude=False, series=True,
+                         nthreads=threads)
+        return time.perf_counter() - t0, r["durationHours"]
+
+    # two probes give the fixed cost (the sky of the year) and the cost per sensor; the sample fills target_s
+    n0, n1 = 64, 512
+    t0, _ = sample(n0)
+    t1, _ = sample(n1)
+    per = max((t1 - t0) / (n1 - n0), 1e-6)
+    nf = int(min(len(sens[1]), max(n1, (target_s - max(t0 - n0 * per, 0.0)) / per)))
+    t, hours = sample(nf)
+    return {"value": nf * hours / t, "unit": "sensor-hours/s", "cores": threads, "kind": "port",
+            "sample": "%d of the sensor faces (the first in mesh order), full year (%.0f h), 577 patches, same context and "
+                      "weather; oracle trace + FP64 sums + FP64 series, %d threads, %.1f s" % (nf, hours, threads, t)}
+
+
+# ---------------------------------------------------------------- main
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the path shards by sensor rows, "
+              "see the sharding notes)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import native
+    if not torch.cuda.is_available():
+        sys.exit("bench.py: no GPU visible")
+
+    boxes, central = city()
+    ctx = box_mesh(boxes)
+    sens = sensor_mesh(boxes[central])
+    n_faces = len(sens[1])
+    tmp = tempfile.mkdtemp(prefix="dlbench")
+    dni, dhi = weather(native, tmp)
+    epw = os.path.join(tmp, "block.epw")
+    write_epw(epw, dni, dhi)
+    args = ["--epw", epw, "--sky-density", "2", "--quantity", "radiation", "--timeseries", os.path.join(tmp, "series.npy"),
+            "--no-self-occlude"]
+    print("bench: %d context triangles, %d sensor faces; warmup %d, steps %d" % (2 * len(ctx[1]), n_faces, a.warmup, a.steps),
+          file=sys.stderr, flush=True)
+
+    def step(ms_series=None):
+        r = native.run_series_device(args, sensors=S, context=Cx)
+        return r
+
+    S = native.SimulationMesh("sensors", *sens)
+    Cx = native.SimulationMesh("city", *ctx)
+    os.environ.pop("DAYLIGHT_SERIES", None)            # the default contraction (BF16x6 on MFMA)
+    r0 = step()
+    assert r0.values.shape == (n_faces, 1), r0.values.shape
+    hours = float(r0.manifest["duration_h"])
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    c0 = native.gpu_counters()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ts = time.perf_counter()
+        r = step()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = native.gpu_counters()
+    assert r.stats["scene_reused"], "the context scene was rebuilt inside the timed region"
+    d = {k: c1[k] - c0[k] for k in c1}
+    sensor_hours = n_faces * hours
+    value = sensor_hours * a.steps / wall
+    rays = d["rays"] / a.steps
+    print("bench: %.1f ms/step, %.3g sensor-hours/s, trace %.1f ms/step over %d launches, series %.1f ms/step"
+          % (1e3 * wall / a.steps, value, d["trace_ms"] / a.steps, d["trace_launches"] / a.steps, d["series_ms"] / a.steps),
+          file=sys.stderr, flush=True)
+
+    # the traversal's algorithmic fetches: one counted run (untimed)
+    native.count_fetches(True)
+    k0 = native.gpu_counters()
+    step()
+    k1 = native.gpu_counters()
+    native.count_fetches(False)
+    launches = d["trace_launches"] / a.steps
+    per_launch_ms = d["trace_ms"] / d["trace_launches"]
+    fetch_bytes = (k1["node_fetches"] - k0["node_fetches"]) * NODE_BYTES + (k1["tri_fetches"] - k0["tri_fetches"]) * TRI_BYTES
+    n_words = math.ceil((k1["counted_rays"] - k0["counted_rays"]) / n_faces / 32)
+    io_bytes = n_faces * n_words * (4 + 24)             # a visibility word written, the sensor's point + normal read per word
+    alg_bytes_launch = (fetch_bytes + io_bytes) / launches
+    achieved = alg_bytes_launch / (per_launch_ms * 1e-3) / 1e9
+
+    # the contraction methods side by side (two runs each)
+    methods = {}
+    for name in ("fp32", "bf16x3", "bf16x6"):
+        os.environ["DAYLIGHT_SERIES"] = name
+        step()
+        m0 = native.gpu_counters()
+        tm = time.perf_counter()
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - tm) / 2
+        m1 = native.gpu_counters()
+        methods[name] = {"ms_per_step": round(1e3 * tm, 2), "series_ms_per_step": round((m1["series_ms"] - m0["series_ms"]) / 2, 3),
+                         "sensor_hours_per_s": round(sensor_hours / tm, 1)}
+    os.environ.pop("DAYLIGHT_SERIES", None)
+    print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
+
+    cpu = None if a.no_cpu else cpu_baseline(ctx, sens, dni, dhi, a.cpu_seconds)
+    lat_ms = np.array(lat) * 1e3
+    line = {
+        "metric": "sensor-hours/s at <=2e-6 rel vs FP64 (annual radiation + hourly series)",
+        "value": round(value, 1), "unit": "sensor-hours/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 trace, f64 sky and sums, bf16x6 mfma series", "data": "synthetic",
+        "config": {"workload": "seeded urban block: 400 boxes (4800 context triangles), roofs + facades of the 40 central "
+                               "buildings in <=1 m quads, 577 patches, 8760 h, Beta(2,3) cloudy clear-sky weather, "
+                               "radiation + device series",
+                   "sensors": n_faces, "context_triangles": 2 * len(ctx[1]), "patches": 577, "hours": hours,
+                   "series_columns": None, "contraction": "bf16x6"},
+        "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3), "rays_per_step": rays},
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
+        "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
+        "contraction_methods": methods,
+        "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "launch_ms": round(per_launch_ms, 4), "algorithmic_bytes_per_launch": round(alg_bytes_launch)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,94 @@
+/* SPDX-License-Identifier: Apache-2.0 */
+/*
+ * daylight_c.h - the C ABI of the MI355X build: the daylight and views tools in-process, with the same symbols, types,
+ * constants and semantics as reference src/daylight_c.h (what bindings/python, bindings/csharp and bindings/ts bind), so
+ * those bindings load libdaylight_amd.so in place of daylight_c.dll unchanged.
+ *
+ * A run takes the CLI's own flags (cli.h) and meshes as handles. Status codes are the CLI's exit codes (DL_OK, DL_USAGE,
+ * DL_RUNTIME); after a failure dl_last_error() holds the message (per thread). No exception crosses the ABI. Any thread
+ * may call; runs on the GPU take turns.
+ */
+#ifndef DAYLIGHT_C_H
+#define DAYLIGHT_C_H
+
+#include <stdint.h>
+
+#define DL_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DL_ABI_VERSION 1
+#define DL_OK 0
+#define DL_USAGE 1
+#define DL_RUNTIME 2
+#define DL_DAYLIGHT 0
+#define DL_VIEWS 1
+#define DL_STATS 10
+
+typedef struct dl_mesh dl_mesh;
+typedef struct dl_control dl_control;
+typedef struct dl_result dl_result;
+
+/* daylight_c.h:54 */
+DL_API int32_t dl_version(void);
+/* daylight_c.h:57: the last failed call's message on this thread ("" if none) */
+DL_API const char* dl_last_error(void);
+/* daylight_c.h:61: the tool's usage text; NULL for an unknown tool */
+DL_API const char* dl_usage(int32_t tool);
+
+/* daylight_c.h:72-73: a mesh from arrays, copied (the .dmesh arrays); NULL on error */
+DL_API dl_mesh* dl_mesh_create(const char* name, const double* xyz, int64_t n_points, const int32_t* counts, int64_t n_faces,
+                               const int32_t* indices, int64_t n_indices, const uint16_t* slots);
+/* daylight_c.h:77: the meshes of an .obj or .dmesh file */
+DL_API dl_mesh* dl_mesh_load(const char* path);
+/* daylight_c.h:80 */
+DL_API int32_t dl_mesh_count(const dl_mesh* mesh);
+/* daylight_c.h:83 */
+DL_API const char* dl_mesh_name(const dl_mesh* mesh, int32_t k);
+/* daylight_c.h:87 */
+DL_API void dl_mesh_free(dl_mesh* mesh);
+
+/* daylight_c.h:92-104: cancel flag and progress of one run */
+DL_API dl_control* dl_control_create(void);
+DL_API void dl_control_cancel(dl_control* control);
+DL_API double dl_control_progress(const dl_control* control);
+DL_API void dl_control_free(dl_control* control);
+
+/* daylight_c.h:119-120: one run of a tool with its CLI flags; sensors then context handles; *out a new result on DL_OK */
+DL_API int32_t dl_run(int32_t tool, int32_t argc, const char* const* argv, dl_mesh* const* sensors, int32_t n_sensors,
+                      dl_mesh* const* context, int32_t n_context, dl_control* control, dl_result** out);
+
+/* daylight_c.h:125-146: the result */
+DL_API int64_t dl_result_rows(const dl_result* result);
+DL_API int64_t dl_result_cols(const dl_result* result);
+DL_API const char* dl_result_manifest(const dl_result* result);
+DL_API int32_t dl_result_values(const dl_result* result, double* out, int64_t n);
+DL_API int32_t dl_result_values_f32(const dl_result* result, float* out, int64_t n);
+DL_API int32_t dl_result_stats(const dl_result* result, double* out);
+DL_API void dl_result_free(dl_result* result);
+
+/* ---- Extensions: no counterpart in the reference's daylight_c.h. For callers that consume the annual series on the
+ * GPU (a serving loop, a benchmark) and for monitoring; the calls above are untouched by them. */
+
+/* A chunk of the series: rows [row0, row0 + rows) x cols float32 values, row-major, in device memory, valid until the
+ * callback returns (NULL when the run has no series columns). Return DL_OK to go on; anything else stops the run. */
+typedef int32_t (*dlx_series_sink)(void* user, int64_t row0, int64_t rows, int64_t cols, const float* device);
+/* dl_run(DL_DAYLIGHT, ...) with the --timeseries output kept on the device and handed to sink chunk by chunk instead of
+ * being written to the --timeseries file (which is not opened). sink NULL: the series is computed and dropped. */
+DL_API int32_t dlx_run_series_device(int32_t argc, const char* const* argv, dl_mesh* const* sensors, int32_t n_sensors, dl_mesh* const* context,
+                                     int32_t n_context, dl_control* control, dlx_series_sink sink, void* user, dl_result** out);
+/* The process GPU's counters since it was made: [0] trace ms (HIP events on the library's stream), [1] trace launches,
+ * [2] ray slots, [3] traversal node visits, [4] triangle tests, [5] ray slots counted (the counts come from the counting
+ * variant, dlx_count_fetches), [6] reduce ms, [7] series ms, [8] sky ms, [9] build ms, [10] views ms, [11] last trace
+ * launch ms, [12] last series ms, [13] resident scene bytes, [14] tree height. Writes min(n, DLX_COUNTERS) values. */
+#define DLX_COUNTERS 15
+DL_API int32_t dlx_gpu_counters(double* out, int32_t n);
+/* on != 0: the traversal runs its counting variant (slower) and fills counters [3]-[5]. */
+DL_API void dlx_count_fetches(int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

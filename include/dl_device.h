@@ -1,0 +1,148 @@
+/* SPDX-License-Identifier: Apache-2.0 */
+/*
+ * dl_device.h - the GPU seam of the MI355X build: plain C entry points over HIP, with host pointers and sizes only.
+ *
+ * Each function replaces one of the reference's internal `gpu_*` functions (reference src/internal.h:125-220, the OptiX
+ * + CUDA layer of src/gpu.cu) or a host loop the reference runs beside them, and says which. The daylight / views
+ * orchestration (the reference's run.cpp / views.cpp) calls these; so do the tests, through ctypes.
+ *
+ * Conventions: status 0 ok, nonzero failed with the message in err[0..errLen) (NUL-terminated); one dlg_gpu per process
+ * (the reference's Gpu), made on first use; calls on one dlg_gpu run one at a time. Device buffers belong to the
+ * dlg_gpu and grow as needed. Layouts: visibility bits word-major bits[w nc + i]; views classes / distances
+ * direction-major [d nc + i]; downloads are transposed to row-major as the reference's are.
+ */
+#ifndef DL_DEVICE_H
+#define DL_DEVICE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dlg_gpu dlg_gpu;
+typedef struct dlg_scene dlg_scene;
+
+/* gpu_init / gpu_destroy (internal.h:127-128): the HIP context, the stream and events of device 0. */
+int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen);
+void dlg_destroy(dlg_gpu* g);
+/* the device in use: name, compute units, HBM bytes (for logs and the bench) */
+int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmBytes);
+
+/* gpu_build (internal.h:133): the occluder triangles (vertices v[3 nv], indices tri[3 nt]) as an LBVH (Karras 2012:
+ * 63-bit Morton codes of the centroids, radix sort, hierarchy, bottom-up boxes) installed in g. nt == 0: an empty
+ * scene (every ray misses). */
+int32_t dlg_build(dlg_gpu* g, const float* v, int64_t nv, const uint32_t* tri, int64_t nt, char* err, int32_t errLen);
+/* gpu_build_bytes (internal.h:136): device bytes a build of nt triangles over nv vertices peaks at. */
+int64_t dlg_build_bytes(int64_t nv, int64_t nt);
+/* gpu_trim (internal.h:138): frees the per-run buffers (with sensors: the sensors too). */
+void dlg_trim(dlg_gpu* g, int32_t sensors);
+/* gpu_detach / gpu_attach / gpu_free (internal.h:141-144): a built scene moved out of g and kept resident. */
+dlg_scene* dlg_detach(dlg_gpu* g);
+void dlg_attach(dlg_gpu* g, dlg_scene* s);
+void dlg_free_scene(dlg_scene* s);
+int64_t dlg_scene_bytes(const dlg_scene* s);
+/* gpu_free_bytes (internal.h:145) */
+int64_t dlg_free_bytes(void);
+/* The installed scene as built, for the bit-exact check against a host build: nodes [n][16] floats (left box lo xyz hi
+ * xyz, right box, left child, right child as bits, 2 unused; child >= 2^31 = leaf), leaves [nt][12] floats (v0, v1, v2,
+ * original index as bits, 2 unused), tree height. Sizes in *nNodes / *nTris when the pointers are null. */
+int32_t dlg_scene_download(dlg_gpu* g, float* nodes, int64_t* nNodes, float* leaves, int64_t* nTris, int32_t* height);
+
+/* gpu_set_scene (internal.h:146): resident sensors (points P[3 n], normals N[3 n], recentred) and directions D[3 nDir]. */
+int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens, const float* D, int64_t nDir, char* err, int32_t errLen);
+/* gpu_ray_flags (internal.h:218): the reference's OptiX flag values (TERMINATE_ON_FIRST_HIT 4 | DISABLE_ANYHIT 1 |
+ * CULL_BACK_FACING_TRIANGLES 16), kept so callers and tests read the same numbers. */
+uint32_t dlg_ray_flags(int32_t cullBackfaces, int32_t translucent);
+/* gpu_trace (internal.h:151): visibility bits of sensors [i0, i0 + nc) against every direction, on the device. */
+int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, char* err, int32_t errLen);
+/* gpu_reduce (internal.h:154), every quantity of a run in one pass over the bits: out[q nc + i] = sum over the visible
+ * directions d (ascending) of double(float(useCos[q] ? max(0, N_i . D_d) : 1)) * double(W[q nDir + d]). */
+int32_t dlg_reduce(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, double* out, char* err, int32_t errLen);
+/* gpu_download_bits (internal.h:182): [nWords x nc] as on the device. */
+int32_t dlg_download_bits(dlg_gpu* g, int64_t nc, uint32_t* out, char* err, int32_t errLen);
+
+/* The sky of a run on the GPU, FP64 (replaces the host loops of reference run.cpp:311-446 and sky.cpp's
+ * gendaymtx_step / solar.cpp's sun_noaa per step): the sun of every step, the Perez all-weather sky per step and patch,
+ * the cumulative per-patch weights and, for a series, the patch radiance matrix S [P x Tup] kept on the device. */
+typedef struct {
+  int32_t T;
+  const int32_t* month; const int32_t* day; const int32_t* jd;   /* [T] */
+  const double* hour; const double* dt;                          /* [T] */
+  const float* dni; const float* dhi;                            /* [T] */
+  double lat, lon, tz, north, grefl;
+  int32_t leap, mf;
+  int32_t wantRad, wantVis;     /* radiometric / visible cumulative weights */
+  int32_t series, seriesVis;    /* keep S (radiometric, or visible when seriesVis) and g for the non-night steps */
+  const double* tcos;           /* [1025] the table cosine (Radiance tcos.c): cos(k pi / 2048) */
+} dlg_sky_in;
+typedef struct {
+  double* alt; double* azi; int32_t* up; float* vec;             /* [T], vec [3 T] turned by north */
+  double* skyRad; double* skyVis;                                /* [P] */
+  double* gndRad; double* gndVis;                                /* [1] */
+  double* sunRad; double* sunVis;                                /* [T]: the step's sun weight (used where up) */
+  int32_t* Tup;                                                  /* non-night steps (series columns) */
+} dlg_sky_out;
+int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, char* err, int32_t errLen);
+/* The last dlg_sky's series inputs: S [P x Tup] and g [Tup] (floats), as the contraction reads them. */
+int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, char* err, int32_t errLen);
+
+/* gpu_series (internal.h:155-171): E[i T + t] for the chunk's sensors, S from the last dlg_sky (P x T, T = Tup).
+ * method 0: FP32 on the vector units, each sum in direction order by fma (the reference kernel's arithmetic);
+ * method 1: BF16x3 on MFMA (operands split in bf16 hi + lo, three products, FP32 accumulation in blocks of 32
+ * directions combined in FP64); method 2: BF16x6 (hi + mid + lo, six products), FP32-level accuracy.
+ * E: host array [nc x T], or null to keep it on the device (dlg_series_device gives the pointer). */
+typedef struct {
+  uint32_t nSky;                /* sky directions 0 .. nSky - 1 (P K) */
+  const int32_t* row;           /* [nSky] patch per direction */
+  const float* wsky;            /* [nSky] omega / K */
+  int32_t P, T;
+  const float* G;               /* [nc] ground view factor per sensor (the ground term per step g comes with S) */
+  uint32_t sun0, nSun;          /* sun directions */
+  const int32_t* sunStep;       /* [nSun] column of each sun direction */
+  const float* sunW;            /* [nSun] */
+} dlg_series_in;
+int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, float* E, char* err, int32_t errLen);
+/* the device copy of the last series (float [nc x T]) */
+const float* dlg_series_device(dlg_gpu* g);
+
+/* Views (ViewLaunch, internal.h:186-195): the grid of K = nAz x nEl cells per sensor, rows top-down, columns clockwise
+ * from az0 (degrees); world: the 360 frame (true north, horizon); cull: directions behind the sensor are own surface;
+ * jitter: a seeded point per cell (pcg3d of element, cell, side); vground: the virtual ground plane z = groundZ. */
+typedef struct {
+  uint32_t nAz, nEl;
+  float az0, dAz;
+  const float* rowSin;          /* [2 nEl] host: sin of each row's bottom and top elevation */
+  int32_t world, cull, jitter, back, vground;
+  uint32_t nS;                  /* sides per element (the seeds' element = sensor / nS) */
+  float groundZ;                /* recentred */
+  double north;                 /* degrees */
+} dlg_view_in;
+/* gpu_set_labels (internal.h:197): the class of a hit per triangle (input order; 2 obstruction, 3 + g target group g). */
+int32_t dlg_set_labels(dlg_gpu* g, const uint8_t* label, int64_t n, char* err, int32_t errLen);
+/* gpu_view (internal.h:201): classes and distances of sensors [i0, i0 + nc) on the device, direction-major [K x nc]. */
+int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const dlg_view_in* v, char* err, int32_t errLen);
+/* gpu_reduce_view (internal.h:208-209): out[i NA + a], NA = 3 nClass + 1 + nReach + glazing + (isoDim > 0): per class
+ * weight, weight x ln d, min d; own surface; reach sums; glazing; isovist area (isoDim 2) or volume (3). */
+int32_t dlg_reduce_view(dlg_gpu* g, int64_t nc, const double* wRow, uint32_t nClass, double farDist, const float* reach, uint32_t nReach,
+                        int32_t glazing, uint32_t isoDim, double* out, char* err, int32_t errLen);
+/* gpu_download_view (internal.h:210): classes and distances, row-major [nc x K] */
+int32_t dlg_download_view(dlg_gpu* g, int64_t nc, uint8_t* cls, float* dist, char* err, int32_t errLen);
+
+/* gpu_stats (internal.h:211-216), plus per-kernel device times of the last calls (HIP events on the library's stream). */
+typedef struct {
+  double buildMs, traceMs, reduceMs, seriesMs, skyMs, viewMs;
+  int64_t rays, traced, launches, sceneBytes, nodes, height;
+  double lastTraceMs, lastReduceMs, lastSeriesMs, lastSkyMs, lastViewMs, lastViewReduceMs;
+  /* the counting variant's totals (dlg_count_fetches): node visits and triangle tests of the visibility traversal, and
+   * the ray slots they were counted over */
+  int64_t nodeFetches, triFetches, countedRays;
+} dlg_stats_t;
+int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out);
+/* No counterpart in the reference (its OptiX traversal is opaque): on != 0 makes dlg_trace run the traversal's counting
+ * variant, which adds every node visit and triangle test to dlg_stats_t (the algorithmic bytes of the traversal). */
+void dlg_count_fetches(dlg_gpu* g, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

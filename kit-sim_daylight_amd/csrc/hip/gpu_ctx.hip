@@ -1,0 +1,132 @@
+// SPDX-License-Identifier: Apache-2.0
+// The device context of dl_device.h: one stream (default priority) and two timing events on device 0, resident sensors
+// and directions, scene residency. Replaces the OptiX context / pipeline / SBT host code of reference src/gpu.cu:277-497.
+#include <cstring>
+#include <string>
+
+#include "gpu_internal.h"
+
+extern "C" {
+
+int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { dlg_fail(err, errLen, "no HIP device"); return 1; }
+  DLG_CHECK(hipSetDevice(0));
+  dlg_gpu* g = new dlg_gpu;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) g->cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&g->ev[0]) != hipSuccess ||
+      hipEventCreate(&g->ev[1]) != hipSuccess) {
+    dlg_fail(err, errLen, "hip stream / events");
+    dlg_destroy(g);
+    return 1;
+  }
+  *out = g;
+  return 0;
+}
+
+void dlg_destroy(dlg_gpu* g) {
+  if (!g) return;
+  for (DevBuf* b : {&g->scene.nodes, &g->scene.leaves, &g->P, &g->N, &g->D, &g->bits, &g->scratch, &g->out, &g->skyS, &g->skyG,
+                    &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels})
+    b->release();
+  for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmBytes) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 1;
+  if (name && nameLen > 0) {
+    std::snprintf(name, size_t(nameLen), "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (hbmBytes) *hbmBytes = int64_t(prop.totalGlobalMem);
+  return 0;
+}
+
+void dlg_trim(dlg_gpu* g, int32_t sensors) {
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->view}) b->release();
+  if (sensors) {
+    g->P.release(); g->N.release(); g->labels.release();
+    g->nSens = 0;
+  }
+}
+
+dlg_scene* dlg_detach(dlg_gpu* g) {
+  dlg_scene* s = new dlg_scene(g->scene);
+  g->scene = dlg_scene{};
+  return s;
+}
+
+void dlg_attach(dlg_gpu* g, dlg_scene* s) {
+  g->scene.nodes.release();
+  g->scene.leaves.release();
+  g->scene = *s;
+  delete s;
+}
+
+void dlg_free_scene(dlg_scene* s) {
+  if (!s) return;
+  s->nodes.release();
+  s->leaves.release();
+  delete s;
+}
+
+int64_t dlg_scene_bytes(const dlg_scene* s) { return s ? s->bytes() : 0; }
+
+int64_t dlg_free_bytes(void) {
+  size_t f = 0, t = 0;
+  return hipMemGetInfo(&f, &t) == hipSuccess ? int64_t(f) : 0;
+}
+
+int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens, const float* D, int64_t nDir, char* err, int32_t errLen) {
+  g->nSens = nSens;
+  g->nDir = nDir;
+  g->nWords = (nDir + 31) / 32;
+  DLG_CHECK(g->P.grow(size_t(nSens) * 12));
+  DLG_CHECK(g->N.grow(size_t(nSens) * 12));
+  DLG_CHECK(g->D.grow(size_t(nDir) * 12));
+  if (nSens) {
+    DLG_CHECK(hipMemcpyAsync(g->P.p, P, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
+    DLG_CHECK(hipMemcpyAsync(g->N.p, N, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
+  }
+  if (nDir) DLG_CHECK(hipMemcpyAsync(g->D.p, D, size_t(nDir) * 12, hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+uint32_t dlg_ray_flags(int32_t cullBackfaces, int32_t translucent) { return 4u | (translucent ? 0u : 1u) | (cullBackfaces ? 16u : 0u); }
+
+int32_t dlg_download_bits(dlg_gpu* g, int64_t nc, uint32_t* out, char* err, int32_t errLen) {
+  if (nc != g->nc) { dlg_fail(err, errLen, "dlg_download_bits: no trace of this chunk"); return 1; }
+  const size_t n = size_t(g->nWords) * size_t(nc);
+  if (n) DLG_CHECK(hipMemcpyAsync(out, g->bits.p, n * 4, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int32_t dlg_scene_download(dlg_gpu* g, float* nodes, int64_t* nNodes, float* leaves, int64_t* nTris, int32_t* height) {
+  if (nNodes) *nNodes = g->scene.nNodes;
+  if (nTris) *nTris = g->scene.nTris;
+  if (height) *height = g->scene.height;
+  if (nodes && g->scene.nNodes && hipMemcpy(nodes, g->scene.nodes.p, size_t(g->scene.nNodes) * 64, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  if (leaves && g->scene.nTris && hipMemcpy(leaves, g->scene.leaves.p, size_t(g->scene.nTris) * 48, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  return 0;
+}
+
+void dlg_count_fetches(dlg_gpu* g, int32_t on) { g->countFetches = on ? 1 : 0; }
+
+int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out) {
+  *out = g->stats;
+  out->sceneBytes = g->scene.bytes();
+  out->nodes = g->scene.nNodes;
+  out->height = g->scene.height;
+  return 0;
+}
+
+const float* dlg_series_device(dlg_gpu* g) { return g->seriesE.as<float>(); }
+
+}  // extern "C"

@@ -1,0 +1,257 @@
+// SPDX-License-Identifier: Apache-2.0
+// The sky of a daylight run on the GPU in FP64: replaces the host loops of reference src/run.cpp:311-446 (sun_noaa per
+// step, gendaymtx_step per step, the cumulative per-patch weights) with kernels over steps and (step, patch).
+//   k_steps   a thread per step: sun position (NOAA, Ladybug conventions), Perez parameters, ground row, the sun's
+//             direct illuminance as AddDirect spreads it (for illuminance), the sun's weights
+//   k_lum     a thread per (step, patch): Perez relative luminance (float, as gendaymtx's array)
+//   k_norm    a thread per step: the horizontal-illuminance normalisation in patch order (table sine), scale factors
+//   k_apply   a thread per (step, patch): the patch radiance bright() per mode (double) and the series matrix S
+//   k_accum   a thread per patch: sum over steps in step order (the reference's double sums, same order)
+// Float / double rounding points follow gendaymtx (reference src/sky.cpp:297-474), so S and the weights agree with the
+// host model after float rounding; libm differences (device vs host double transcendental functions) stay below the
+// float rounding of the outputs except in rare last-bit cases.
+#include <vector>
+
+#include "dl_math.h"
+#include "gpu_internal.h"
+
+namespace {
+
+using namespace dlm;
+
+struct StepOut {
+  PerezStep q;
+  double altRad, aziRad;
+  double sfRad, sfVis;
+  int uniform;                 // zero sky: every patch 1
+  double gRadB, gVisB;         // ground bright (radiometric W/m2/sr, visible x179)
+  double sunVis, sunRad;       // the sun's weights of this step
+};
+
+__global__ void k_steps(int T, const int* __restrict__ month, const int* __restrict__ day, const int* __restrict__ jd,
+                        const double* __restrict__ hour, const double* __restrict__ dt, const float* __restrict__ dni,
+                        const float* __restrict__ dhi, double lat, double lon, double tz, double north, int leap, float grefl, int wantVis,
+                        int P, const float* __restrict__ palt, const float* __restrict__ pazi, const float* __restrict__ omega,
+                        StepOut* __restrict__ so, double* __restrict__ alt, double* __restrict__ azi, int* __restrict__ up,
+                        float* __restrict__ vec) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const SunPos s = sun_noaa(lat, lon, tz, month[t], day[t], hour[t], leap != 0);
+  alt[t] = s.alt;
+  azi[t] = s.azi;
+  up[t] = s.up ? 1 : 0;
+  const float3 v = rotate_z(s.vec, north);
+  vec[3 * t] = v.x; vec[3 * t + 1] = v.y; vec[3 * t + 2] = v.z;
+  StepOut o{};
+  o.altRad = s.alt * (kPi / 180.0);
+  o.aziRad = s.azi * (kPi / 180.0);
+  o.q = perez_step(jd[t], o.altRad, dni[t], dhi[t]);
+  o.gRadB = o.q.night ? 0.0 : ground_bright(o.q.gRad, grefl);
+  o.gVisB = o.q.night ? 0.0 : 179.0 * ground_bright(o.q.gVis, grefl);
+  o.sunRad = dni[t] * dt[t] / 1000.0;
+  if (wantVis && !o.q.night && o.q.dirVis > 1e-4) {   // AddDirect (-d, -O0): the 4 patch centres nearest the sun
+    double best[4] = {-1., -1., -1., -1.};
+    int bestP[4] = {0, 0, 0, 0};
+    const double ca = cos(o.altRad), sv0 = ca * sin(o.aziRad), sv1 = ca * cos(o.aziRad), sv2 = sin(o.altRad);
+    for (int p = 0; p < P; ++p) {
+      const double pa = cos((double)palt[p]);
+      const double dp = (pa * sin((double)pazi[p])) * sv0 + (pa * cos((double)pazi[p])) * sv1 + sin((double)palt[p]) * sv2;
+      for (int i = 0; i < 4; ++i)
+        if (dp > best[i]) {
+          for (int j = 3; j > i; --j) { best[j] = best[j - 1]; bestP[j] = bestP[j - 1]; }
+          best[i] = dp;
+          bestP[i] = p;
+          break;
+        }
+    }
+    double w[4], wsum = 0;
+    for (int i = 3; i >= 0; --i) wsum += w[i] = 1. / (1.002 - best[i]);
+    // the patches' green channel after AddDirect, summed x omega in patch order (zeros elsewhere)
+    int ord[4] = {0, 1, 2, 3};
+    for (int a = 0; a < 4; ++a)
+      for (int b = a + 1; b < 4; ++b)
+        if (bestP[ord[b]] < bestP[ord[a]]) { const int x = ord[a]; ord[a] = ord[b]; ord[b] = x; }
+    double dirIll = 0;
+    for (int k = 0; k < 4; ++k) {
+      const int i = ord[k];
+      float add = (float)(w[i] * o.q.dirVis / (179. * wsum));
+      add /= omega[bestP[i]];
+      const float val = 0.f + add * 1.f;
+      dirIll += (double)(val * omega[bestP[i]]);   // float x float, as run.cpp:462 multiplies the float row by omega
+    }
+    o.sunVis = 179.0 * dirIll * dt[t];
+  }
+  so[t] = o;
+}
+
+__global__ void k_lum(int T, int P, const StepOut* __restrict__ so, const float* __restrict__ palt, const float* __restrict__ pazi,
+                      float* __restrict__ L) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+  if (p >= P) return;
+  const StepOut& o = so[t];
+  L[size_t(t) * P + p] = o.q.night ? 0.f : perez_patch(o.q, palt[p], pazi[p], o.aziRad);
+}
+
+__global__ void k_norm(int T, int P, StepOut* __restrict__ so, const float* __restrict__ L, const double* __restrict__ tsinP,
+                       const float* __restrict__ omega) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  StepOut& o = so[t];
+  if (o.q.night) return;
+  double norm = 0.0;
+  const float* Lt = L + size_t(t) * P;
+  for (int p = 0; p < P; ++p) norm += Lt[p] * tsinP[p] * omega[p];
+  o.uniform = norm <= 1e-6;
+  if (o.uniform) norm = kPi;
+  o.sfRad = (o.q.difRad / norm) * (1. / 179.);
+  o.sfVis = (o.q.difVis / norm) * (1. / 179.);
+}
+
+__global__ void k_apply(int T, int P, const StepOut* __restrict__ so, const float* __restrict__ L, int wantRad, int wantVis,
+                        const int* __restrict__ colOf, int Tup, int seriesMode, double* __restrict__ LpRad, double* __restrict__ LpVis,
+                        float* __restrict__ S) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+  if (p >= P) return;
+  const StepOut& o = so[t];
+  const size_t at = size_t(t) * P + p;
+  double r = 0.0, v = 0.0;
+  if (!o.q.night) {
+    const float l = o.uniform ? 1.f : L[at];
+    if (wantRad || seriesMode == 1) r = sky_bright(l, o.sfRad);
+    if (wantVis || seriesMode == 2) v = 179.0 * sky_bright(l, o.sfVis);
+  }
+  if (wantRad) LpRad[at] = r;
+  if (wantVis) LpVis[at] = v;
+  const int col = colOf[t];
+  if (seriesMode && col >= 0) S[size_t(p) * Tup + col] = (float)(seriesMode == 1 ? r : v);
+}
+
+__global__ void k_accum(int T, int P, const StepOut* __restrict__ so, const double* __restrict__ dt, const double* __restrict__ LpRad,
+                        const double* __restrict__ LpVis, int wantRad, int wantVis, double* __restrict__ skyRad, double* __restrict__ skyVis) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double r = 0.0, v = 0.0;
+  for (int t = 0; t < T; ++t) {
+    if (so[t].q.night) continue;
+    const size_t at = size_t(t) * P + p;
+    if (wantRad) r += LpRad[at] * dt[t] / 1000.0;
+    if (wantVis) v += LpVis[at] * dt[t];
+  }
+  skyRad[p] = r;
+  skyVis[p] = v;
+}
+
+__global__ void k_ground(int T, const StepOut* __restrict__ so, const double* __restrict__ dt, const int* __restrict__ colOf, int seriesMode,
+                         double* __restrict__ gnd, float* __restrict__ gS, double* __restrict__ sunRad, double* __restrict__ sunVis) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double r = 0.0, v = 0.0;
+  for (int t = 0; t < T; ++t) {
+    const StepOut& o = so[t];
+    sunRad[t] = o.sunRad;
+    sunVis[t] = o.sunVis;
+    if (o.q.night) continue;
+    r += o.gRadB * dt[t] / 1000.0;
+    v += o.gVisB * dt[t];
+    if (seriesMode && colOf[t] >= 0) gS[colOf[t]] = (float)(seriesMode == 1 ? o.gRadB : o.gVisB);
+  }
+  gnd[0] = r;
+  gnd[1] = v;
+}
+
+inline unsigned blocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, char* err, int32_t errLen) {
+  const int T = in->T, mf = in->mf;
+  static const int kRowAz[7] = {30, 30, 24, 24, 18, 12, 6};
+  // the patch table (reference sky.cpp:240-270), as the oracle and gendaymtx compute it: float angles and solid angles
+  std::vector<float> palt, pazi, om;
+  const double alpha = (kPi / 2.) / (7 * mf + .5);
+  for (int r = 0; r < 7 * mf; ++r) {
+    const int n = kRowAz[r / mf] * mf;
+    const float a = (float)(alpha * (r + .5));
+    const float w = (float)(2. * kPi * (std::sin(alpha * (r + 1)) - std::sin(alpha * r)) / (double)n);
+    for (int j = 0; j < n; ++j) { palt.push_back(a); pazi.push_back((float)(2. * kPi * j / (double)n)); om.push_back(w); }
+  }
+  palt.push_back((float)(kPi / 2.)); pazi.push_back(0.f); om.push_back((float)(2. * kPi * (1. - std::cos(alpha * .5))));
+  const int P = int(palt.size());
+  std::vector<double> tsinP(static_cast<size_t>(P));
+  for (int p = 0; p < P; ++p) {   // Radiance tcos.c table sine of each patch altitude (1024 entries, nearest, no interpolation)
+    double x = (double)palt[p] - kPi / 2.;
+    if (x < 0.) x = -x;
+    int k = (int)((1024 * 2. / kPi) * x + 0.5);
+    while (k >= 4096) k -= 4096;
+    const int q = k >> 10;
+    tsinP[p] = q == 0 ? in->tcos[k] : q == 1 ? -in->tcos[2048 - k] : q == 2 ? -in->tcos[k - 2048] : in->tcos[4096 - k];
+  }
+  std::vector<int> colOf(static_cast<size_t>(T));
+  int Tup = 0;
+  for (int t = 0; t < T; ++t) colOf[t] = in->dni[t] + in->dhi[t] > 1e-4 ? Tup++ : -1;
+  *out->Tup = Tup;
+  const int seriesMode = in->series ? (in->seriesVis ? 2 : 1) : 0;
+  // device work area: inputs, step records, L, Lp (per mode), outputs
+  size_t off = 0;
+  const auto take = [&](size_t bytes) { const size_t at = off; off += (bytes + 255) & ~size_t(255); return at; };
+  const size_t oMonth = take(4 * T), oDay = take(4 * T), oJd = take(4 * T), oHour = take(8 * T), oDt = take(8 * T), oDni = take(4 * T),
+               oDhi = take(4 * T), oPalt = take(4 * P), oPazi = take(4 * P), oOm = take(4 * P), oTs = take(8 * P), oCol = take(4 * T),
+               oSo = take(sizeof(StepOut) * T), oL = take(size_t(4) * T * P), oLr = take(in->wantRad ? size_t(8) * T * P : 8),
+               oLv = take(in->wantVis ? size_t(8) * T * P : 8), oAlt = take(8 * T), oAzi = take(8 * T), oUp = take(4 * T),
+               oVec = take(12 * T), oSkyR = take(8 * P), oSkyV = take(8 * P), oGnd = take(16), oSunR = take(8 * T), oSunV = take(8 * T);
+  DLG_CHECK(g->skyWork.grow(off));
+  char* b = g->skyWork.as<char>();
+  const auto up = [&](size_t at, const void* src, size_t bytes) { return hipMemcpyAsync(b + at, src, bytes, hipMemcpyHostToDevice, g->stream); };
+  DLG_CHECK(up(oMonth, in->month, 4 * T)); DLG_CHECK(up(oDay, in->day, 4 * T)); DLG_CHECK(up(oJd, in->jd, 4 * T));
+  DLG_CHECK(up(oHour, in->hour, 8 * T)); DLG_CHECK(up(oDt, in->dt, 8 * T)); DLG_CHECK(up(oDni, in->dni, 4 * T));
+  DLG_CHECK(up(oDhi, in->dhi, 4 * T)); DLG_CHECK(up(oPalt, palt.data(), 4 * P)); DLG_CHECK(up(oPazi, pazi.data(), 4 * P));
+  DLG_CHECK(up(oOm, om.data(), 4 * P)); DLG_CHECK(up(oTs, tsinP.data(), 8 * P)); DLG_CHECK(up(oCol, colOf.data(), 4 * T));
+  if (seriesMode) {
+    DLG_CHECK(g->skyS.grow(size_t(P) * size_t(Tup > 0 ? Tup : 1) * 4));
+    DLG_CHECK(g->skyG.grow(size_t(Tup > 0 ? Tup : 1) * 4));
+    g->skyP = P;
+    g->skyT = Tup;
+  }
+#define PTR(T_, o) reinterpret_cast<T_*>(b + (o))
+  dlg_tbegin(g);
+  k_steps<<<blocks(T, 64), 64, 0, g->stream>>>(T, PTR(int, oMonth), PTR(int, oDay), PTR(int, oJd), PTR(double, oHour), PTR(double, oDt),
+                                               PTR(float, oDni), PTR(float, oDhi), in->lat, in->lon, in->tz, in->north, in->leap,
+                                               (float)in->grefl, in->wantVis, P, PTR(float, oPalt), PTR(float, oPazi), PTR(float, oOm),
+                                               PTR(StepOut, oSo), PTR(double, oAlt), PTR(double, oAzi), PTR(int, oUp), PTR(float, oVec));
+  if (T > 0) {
+    k_lum<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oPalt), PTR(float, oPazi), PTR(float, oL));
+    k_norm<<<blocks(T, 64), 64, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oTs), PTR(float, oOm));
+    k_apply<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), in->wantRad, in->wantVis,
+                                                                      PTR(int, oCol), Tup, seriesMode, PTR(double, oLr), PTR(double, oLv),
+                                                                      seriesMode ? g->skyS.as<float>() : nullptr);
+    k_accum<<<blocks(P, 64), 64, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(double, oDt), PTR(double, oLr), PTR(double, oLv), in->wantRad,
+                                                 in->wantVis, PTR(double, oSkyR), PTR(double, oSkyV));
+    k_ground<<<1, 64, 0, g->stream>>>(T, PTR(StepOut, oSo), PTR(double, oDt), PTR(int, oCol), seriesMode, PTR(double, oGnd),
+                                      seriesMode ? g->skyG.as<float>() : nullptr, PTR(double, oSunR), PTR(double, oSunV));
+  }
+  DLG_CHECK(hipGetLastError());
+  const float ms = dlg_tend(g);
+  g->stats.skyMs += ms;
+  g->stats.lastSkyMs = ms;
+  const auto down = [&](void* dst, size_t at, size_t bytes) { return hipMemcpyAsync(dst, b + at, bytes, hipMemcpyDeviceToHost, g->stream); };
+  DLG_CHECK(down(out->alt, oAlt, 8 * T)); DLG_CHECK(down(out->azi, oAzi, 8 * T)); DLG_CHECK(down(out->up, oUp, 4 * T));
+  DLG_CHECK(down(out->vec, oVec, 12 * T));
+  if (out->skyRad) DLG_CHECK(down(out->skyRad, oSkyR, 8 * P));
+  if (out->skyVis) DLG_CHECK(down(out->skyVis, oSkyV, 8 * P));
+  double gnd[2] = {0, 0};
+  DLG_CHECK(down(gnd, oGnd, 16));
+  if (out->sunRad) DLG_CHECK(down(out->sunRad, oSunR, 8 * T));
+  if (out->sunVis) DLG_CHECK(down(out->sunVis, oSunV, 8 * T));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  if (out->gndRad) *out->gndRad = gnd[0];
+  if (out->gndVis) *out->gndVis = gnd[1];
+#undef PTR
+  return 0;
+}
+
+extern "C" int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, char* err, int32_t errLen) {
+  if (!g->skyT) { dlg_fail(err, errLen, "dlg_sky_series: the last dlg_sky kept no series"); return 1; }
+  DLG_CHECK(hipMemcpyAsync(S, g->skyS.p, size_t(g->skyP) * size_t(g->skyT) * 4, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipMemcpyAsync(gS, g->skyG.p, size_t(g->skyT) * 4, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}

@@ -1,0 +1,352 @@
+// SPDX-License-Identifier: Apache-2.0
+// Views on CDNA4: replaces __raygen__view (reference src/trace_dev.cu:177-232), gpu_view (gpu.cu:572-610), k_view /
+// k_iso / gpu_reduce_view (gpu.cu:171-274, 671-697).
+//   k_view_rays  a wave per 64 consecutive sensors and one 32-cell word (grid y): frame, jittered cell, virtual ground,
+//                closest hit through the LBVH (exact ties by the lower triangle index), the sensor's own planes passed
+//                (at most 3; the 4th an obstruction there), the hit triangle's label. Direction-major cls / dist.
+//   k_view_sums  a wave per sensor, lane l sums directions l, l + 64, ... (class weights, weight x ln d, minima, own
+//                surface, reach, glazing), then a fixed shuffle tree (s = 32 .. 1): the same bits for any chunking.
+//   k_view_iso   a wave per sensor: the isovist's area (one row) or volume (tetrahedra between adjacent rows and
+//                columns), each piece in float, summed in double in the same fixed order.
+#include <cmath>
+#include <vector>
+
+#include "dl_math.h"
+#include "gpu_internal.h"
+
+namespace {
+
+using namespace dlm;
+constexpr int kWave = 64;
+constexpr unsigned kClasses = 3 + 16;
+
+struct BoxHit { bool hit; float t; };
+__device__ __forceinline__ BoxHit box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
+  const float nx = r.inv.x >= 0.f ? lx : hx, fx = r.inv.x >= 0.f ? hx : lx;
+  const float ny = r.inv.y >= 0.f ? ly : hy, fy = r.inv.y >= 0.f ? hy : ly;
+  const float nz = r.inv.z >= 0.f ? lz : hz, fz = r.inv.z >= 0.f ? hz : lz;
+  const float t0 = fmaxf(fmaxf(fmaxf(tmin, (nx - r.oNear.x) * r.inv.x), (ny - r.oNear.y) * r.inv.y), (nz - r.oNear.z) * r.inv.z);
+  const float t1 = fminf(fminf(fminf(tmax, (fx - r.oFar.x) * r.inv.x), (fy - r.oFar.y) * r.inv.y), (fz - r.oFar.z) * r.inv.z);
+  return {t0 <= t1, t0};
+}
+
+struct Closest { float t; uint32_t leaf, prim; bool front, hit; };
+
+__device__ __forceinline__ void leaf_test(const Ray& r, const float4* __restrict__ leaves, uint32_t leaf, float tmin, bool cull, Closest& c) {
+  const float4 a = leaves[3 * leaf], b = leaves[3 * leaf + 1], e = leaves[3 * leaf + 2];
+  float t;
+  bool front;
+  if (!tri_hit(r, make_float3(a.x, a.y, a.z), make_float3(a.w, b.x, b.y), make_float3(b.z, b.w, e.x), tmin, c.t, cull, t, front)) return;
+  const uint32_t prim = __float_as_uint(e.y);
+  if (c.hit && t == c.t && prim > c.prim) return;
+  c.t = t; c.leaf = leaf; c.prim = prim; c.front = front; c.hit = true;
+}
+
+template <int STACK>
+__device__ Closest closest_hit(const float4* __restrict__ nodes, const float4* __restrict__ leaves, const Ray& r, float tmin, float tmax, bool cull,
+                               uint32_t* __restrict__ stk, int lane) {
+  Closest c{tmax, 0u, 0u, false, false};
+  uint32_t cur = 0;
+  int sp = 0;
+  for (;;) {
+    const float4 A = nodes[4 * cur], B = nodes[4 * cur + 1], C = nodes[4 * cur + 2], E = nodes[4 * cur + 3];
+    const BoxHit hl = box_hit(r, A.x, A.y, A.z, A.w, B.x, B.y, tmin, c.t);
+    const BoxHit hr = box_hit(r, B.z, B.w, C.x, C.y, C.z, C.w, tmin, c.t);
+    const uint32_t cl = __float_as_uint(E.x), cr = __float_as_uint(E.y);
+    if (hl.hit && (cl & 0x80000000u)) leaf_test(r, leaves, cl & 0x7fffffffu, tmin, cull, c);
+    if (hr.hit && (cr & 0x80000000u)) leaf_test(r, leaves, cr & 0x7fffffffu, tmin, cull, c);
+    const bool il = hl.hit && !(cl & 0x80000000u), ir = hr.hit && !(cr & 0x80000000u);
+    if (il && ir) {
+      const bool leftFirst = hl.t <= hr.t;
+      stk[sp * kWave + lane] = leftFirst ? cr : cl;
+      ++sp;
+      cur = leftFirst ? cl : cr;
+    } else if (il) cur = cl;
+    else if (ir) cur = cr;
+    else {
+      if (sp == 0) return c;
+      --sp;
+      cur = stk[sp * kWave + lane];
+    }
+  }
+}
+
+struct ViewArgs {
+  const float4* nodes; const float4* leaves; int64_t nTris; float pad;
+  const float3* P; const float3* N;
+  const uint8_t* label;
+  const float* rowSin;       // [2 nEl]
+  uint32_t nAz, nEl, nc, i0, nS, back, world, cull, jitter, vground, cullBack;
+  float az0, dAz, groundZ, cosN, sinN;
+  uint8_t* cls; float* dist;
+};
+
+template <int STACK>
+__global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
+  __shared__ uint32_t stk[STACK * kWave];
+  const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
+  if (i >= a.nc) return;
+  const uint32_t gi = a.i0 + i, element = gi / a.nS, side = a.nS == 2 ? gi % 2 : a.back;
+  const float3 n = a.N[i];
+  const float3 o = offset_ray(a.P[i], n);
+  const float inf = __int_as_float(0x7f800000);
+  const bool none = !(dot3(n, n) > 0.f);
+  float3 f, r, u;
+  if (a.world) { f = make_float3(-a.sinN, a.cosN, 0.f); r = make_float3(a.cosN, a.sinN, 0.f); u = make_float3(0.f, 0.f, 1.f); }
+  else {
+    const float l = sqrtf(n.x * n.x + n.y * n.y);
+    f = n;
+    if (l > 1e-3f) r = make_float3(n.y / l, -n.x / l, 0.f);
+    else {
+      r = make_float3(a.cosN, a.sinN, 0.f);
+      if (l > 0.f) {
+        const float k = dot3(r, n);
+        r = make_float3(r.x - k * n.x, r.y - k * n.y, r.z - k * n.z);
+        const float s = 1.f / sqrtf(dot3(r, r));
+        r = make_float3(s * r.x, s * r.y, s * r.z);
+      }
+    }
+    u = make_float3(r.y * f.z - r.z * f.y, r.z * f.x - r.x * f.z, r.x * f.y - r.y * f.x);
+  }
+  const uint32_t K = a.nAz * a.nEl;
+  for (uint32_t k = 0; k < 32; ++k) {
+    const uint32_t d = w * 32 + k;
+    if (d >= K) break;
+    const uint32_t row = d / a.nAz;
+    const float3 lc = view_cell(d, d % a.nAz, a.rowSin[2 * row], a.rowSin[2 * row + 1], a.az0, a.dAz, a.jitter != 0, element, side);
+    const float3 dir = make_float3((lc.x * r.x + lc.y * f.x) + lc.z * u.x, (lc.x * r.y + lc.y * f.y) + lc.z * u.y, (lc.x * r.z + lc.y * f.z) + lc.z * u.z);
+    uint8_t cl = 2;
+    float t = inf;
+    if (none || (a.cull && dot3(n, dir) <= 0.f)) cl = 254;
+    else if (a.vground && dir.z < 0.f && !(o.z > a.groundZ)) { cl = 1; t = 0.1f; }
+    else {
+      const bool plane = a.vground && dir.z < 0.f;
+      const float tg = plane ? (o.z - a.groundZ) / -dir.z : inf;
+      const float tmax = plane ? (o.z - a.groundZ + 0.01f) / -dir.z : 3.40282347e38f;
+      Ray ray;
+      ray_setup(ray, o, dir, a.pad);
+      float tmin = 1e-5f;
+      uint32_t own = 0, panes = 0;
+      for (;;) {
+        Closest c{};
+        if (a.nTris > 0) c = closest_hit<STACK>(a.nodes, a.leaves, ray, tmin, tmax, a.cullBack != 0, stk, int(lane));
+        if (!c.hit) { cl = dir.z >= 0.f ? 0 : 1; t = tg; break; }
+        t = c.t;
+        const float4 A = a.leaves[3 * c.leaf], B = a.leaves[3 * c.leaf + 1], E = a.leaves[3 * c.leaf + 2];
+        if (own_hit(dir, n, t, c.front, make_float3(A.x, A.y, A.z), make_float3(A.w, B.x, B.y), make_float3(B.z, B.w, E.x))) {
+          if (++own == 4) break;
+        } else {
+          const uint8_t lb = a.label ? a.label[c.prim] : 2;
+          if (lb != 255) { cl = lb; break; }
+          if (++panes > 16) break;
+        }
+        tmin = __uint_as_float(__float_as_uint(t) + 1);
+      }
+      if (panes) cl |= 0x80;
+    }
+    const size_t at = size_t(d) * a.nc + i;
+    a.cls[at] = cl;
+    a.dist[at] = t;
+  }
+}
+
+__device__ __forceinline__ double tree_sum(double v) {
+  for (int s = kWave / 2; s; s >>= 1) v += __shfl_down(v, s, kWave);
+  return v;
+}
+__device__ __forceinline__ float tree_min(float v) {
+  for (int s = kWave / 2; s; s >>= 1) v = fminf(v, __shfl_down(v, s, kWave));
+  return v;
+}
+
+struct SumArgs {
+  const uint8_t* cls; const float* dist; const double* wRow;
+  uint32_t nc, K, nAz, nClass, nReach, glazing, NA;
+  float farF; double lnFar;
+  float reach[8];
+  double* out;
+};
+
+__global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= a.nc) return;   // whole waves
+  double w[kClasses], wl[kClasses], rc[8], own = 0, gz = 0;
+  float mn[kClasses];
+  for (uint32_t c = 0; c < kClasses; ++c) { w[c] = 0; wl[c] = 0; mn[c] = __int_as_float(0x7f800000); }
+  for (uint32_t r = 0; r < 8; ++r) rc[r] = 0;
+  for (uint32_t d = lane; d < a.K; d += kWave) {
+    uint32_t c = a.cls[size_t(d) * a.nc + i];
+    const float t = a.dist[size_t(d) * a.nc + i];
+    const double wd = a.wRow[d / a.nAz];
+    if (c == 254) { own += wd; continue; }
+    if (c & 0x80) { gz += wd; c &= 0x7f; }
+    const float dd = fminf(fmaxf(t, 0.1f), a.farF);
+    const double ld = dd == a.farF ? a.lnFar : dd == 0.1f ? -2.302585092994045684 : log((double)dd);
+    // registers indexed by class: a small switch keeps them out of scratch
+#pragma unroll
+    for (uint32_t q = 0; q < kClasses; ++q)
+      if (q == c) { w[q] += wd; wl[q] += wd * ld; mn[q] = fminf(mn[q], dd); }
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r) if (r < a.nReach && t >= a.reach[r]) rc[r] += wd;
+  }
+  double* o = a.out + size_t(i) * a.NA;
+#pragma unroll
+  for (uint32_t c = 0; c < kClasses; ++c) {
+    if (c >= a.nClass) break;
+    const double s0 = tree_sum(w[c]), s1 = tree_sum(wl[c]);
+    const float m = tree_min(mn[c]);
+    if (!lane) { o[3 * c] = s0; o[3 * c + 1] = s1; o[3 * c + 2] = m; }
+  }
+  const double ow = tree_sum(own);
+  if (!lane) o[3 * a.nClass] = ow;
+#pragma unroll
+  for (uint32_t r = 0; r < 8; ++r) {
+    if (r >= a.nReach) break;
+    const double s = tree_sum(rc[r]);
+    if (!lane) o[3 * a.nClass + 1 + r] = s;
+  }
+  if (a.glazing) {
+    const double s = tree_sum(gz);
+    if (!lane) o[3 * a.nClass + 1 + a.nReach] = s;
+  }
+}
+
+struct IsoArgs {
+  const uint8_t* cls; const float* dist; const float* rowSin;
+  uint32_t nc, nAz, nEl, i0, nS, back, world, jitter, dim, NA;
+  float az0, dAz, farF;
+  double* out;
+};
+
+__device__ __forceinline__ float det3(float3 a, float3 b, float3 c) {
+  const float3 u = make_float3(b.x - a.x, b.y - a.y, b.z - a.z), v = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
+  return a.x * (u.y * v.z - u.z * v.y) + a.y * (u.z * v.x - u.x * v.z) + a.z * (u.x * v.y - u.y * v.x);
+}
+
+__global__ __launch_bounds__(256) void k_view_iso(IsoArgs a) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= a.nc) return;
+  const uint32_t gi = a.i0 + i, element = gi / a.nS, side = a.nS == 2 ? gi % 2 : a.back;
+  const auto len = [&](uint32_t k) { const size_t at = size_t(k) * a.nc + i; return a.cls[at] == 254 ? 0.f : fminf(a.dist[at], a.farF); };
+  double sum = 0;
+  for (uint32_t base = 0; base < a.nAz; base += kWave) {
+    const uint32_t col = base + lane < a.nAz ? base + lane : a.nAz - 1, c1 = col + 1 == a.nAz ? 0 : col + 1;
+    const bool on = base + lane < a.nAz && (a.world || col + 1 < a.nAz);
+    float3 pa = view_cell(col, col, a.rowSin[0], a.rowSin[1], a.az0, a.dAz, a.jitter != 0, element, side);
+    float3 pb = view_cell(c1, c1, a.rowSin[0], a.rowSin[1], a.az0, a.dAz, a.jitter != 0, element, side);
+    float l0x = len(col), l0y = len(c1);
+    if (a.dim == 2) {
+      if (on) sum += (double)(0.5f * l0x * l0y * (pa.y * pb.x - pa.x * pb.y));
+      continue;
+    }
+    for (uint32_t r = 1; r < a.nEl; ++r) {
+      const uint32_t d = r * a.nAz + col;
+      const float l1x = len(r * a.nAz + col), l1y = len(r * a.nAz + c1);
+      const float3 pc = view_cell(d, col, a.rowSin[2 * r], a.rowSin[2 * r + 1], a.az0, a.dAz, a.jitter != 0, element, side);
+      const float3 pe = view_cell(d - col + c1, c1, a.rowSin[2 * r], a.rowSin[2 * r + 1], a.az0, a.dAz, a.jitter != 0, element, side);
+      if (on) sum += (double)(l0x * l0y * l1x * det3(pa, pb, pc) + l0y * l1y * l1x * det3(pb, pe, pc));
+      pa = pc; pb = pe; l0x = l1x; l0y = l1y;
+    }
+  }
+  const double s = tree_sum(sum);
+  if (!lane) a.out[size_t(i) * a.NA + a.NA - 1] = a.dim == 2 ? s : s / 6;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" int32_t dlg_set_labels(dlg_gpu* g, const uint8_t* label, int64_t n, char* err, int32_t errLen) {
+  g->labels.release();
+  if (n <= 0) return 0;
+  DLG_CHECK(g->labels.grow(size_t(n)));
+  DLG_CHECK(hipMemcpyAsync(g->labels.p, label, size_t(n), hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const dlg_view_in* v, char* err, int32_t errLen) {
+  if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_view: sensor range exceeds the scene"); return 1; }
+  const size_t K = size_t(v->nAz) * v->nEl, words = (K + 31) / 32;
+  const size_t oDist = align256(size_t(nc) * K), oSin = oDist + align256(size_t(nc) * K * 4);
+  DLG_CHECK(g->view.grow(oSin + align256(size_t(v->nEl) * 8)));
+  char* b = g->view.as<char>();
+  DLG_CHECK(hipMemcpyAsync(b + oSin, v->rowSin, size_t(v->nEl) * 8, hipMemcpyHostToDevice, g->stream));
+  g->viewN = nc; g->viewK = int64_t(K); g->viewI0 = i0; g->viewIn = *v; g->viewIn.rowSin = nullptr;
+  g->i0 = i0;
+  if (nc == 0 || K == 0) { DLG_CHECK(hipStreamSynchronize(g->stream)); return 0; }
+  const dlg_scene& s = g->scene;
+  ViewArgs a;
+  a.nodes = s.nodes.as<float4>(); a.leaves = s.leaves.as<float4>(); a.nTris = s.nTris; a.pad = s.pad;
+  a.P = g->P.as<float3>() + i0; a.N = g->N.as<float3>() + i0;
+  a.label = g->labels.as<uint8_t>();
+  a.rowSin = reinterpret_cast<const float*>(b + oSin);
+  a.nAz = v->nAz; a.nEl = v->nEl; a.nc = uint32_t(nc); a.i0 = uint32_t(i0); a.nS = v->nS; a.back = v->back; a.world = v->world; a.cull = v->cull;
+  a.jitter = v->jitter; a.vground = v->vground; a.cullBack = (rayFlags & 16u) ? 1u : 0u;
+  a.az0 = v->az0; a.dAz = v->dAz; a.groundZ = v->groundZ;
+  const double rad = v->north * 3.14159265358979323846 / 180;
+  a.cosN = float(std::cos(rad)); a.sinN = float(std::sin(rad));
+  a.cls = reinterpret_cast<uint8_t*>(b); a.dist = reinterpret_cast<float*>(b + oDist);
+  const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(words));
+  dlg_tbegin(g);
+  if (s.height <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
+  else if (s.height <= 160) k_view_rays<160><<<grid, kWave, 0, g->stream>>>(a);
+  else { dlg_fail(err, errLen, "dlg_view: the scene's tree is deeper than 160 levels"); return 1; }
+  DLG_CHECK(hipGetLastError());
+  const float ms = dlg_tend(g);
+  g->stats.viewMs += ms;
+  g->stats.traceMs += ms;
+  g->stats.lastViewMs = ms;
+  g->stats.rays += int64_t(nc) * int64_t(K);
+  g->stats.launches += 1;
+  return 0;
+}
+
+extern "C" int32_t dlg_reduce_view(dlg_gpu* g, int64_t nc, const double* wRow, uint32_t nClass, double farDist, const float* reach, uint32_t nReach,
+                                   int32_t glazing, uint32_t isoDim, double* out, char* err, int32_t errLen) {
+  if (nc == 0) return 0;
+  if (nc != g->viewN) { dlg_fail(err, errLen, "dlg_reduce_view: no dlg_view for this chunk"); return 1; }
+  if (nClass > kClasses || nReach > 8) { dlg_fail(err, errLen, "dlg_reduce_view: at most 16 groups and 8 reach distances"); return 1; }
+  const dlg_view_in& v = g->viewIn;
+  const uint32_t NA = 3 * nClass + 1 + nReach + (glazing ? 1 : 0) + (isoDim ? 1 : 0);
+  DLG_CHECK(g->scratch.grow(size_t(v.nEl) * 8));
+  DLG_CHECK(g->out.grow(size_t(nc) * NA * 8));
+  DLG_CHECK(hipMemcpyAsync(g->scratch.p, wRow, size_t(v.nEl) * 8, hipMemcpyHostToDevice, g->stream));
+  const size_t K = size_t(g->viewK), oDist = align256(size_t(nc) * K), oSin = oDist + align256(size_t(nc) * K * 4);
+  char* b = g->view.as<char>();
+  SumArgs s;
+  s.cls = reinterpret_cast<const uint8_t*>(b); s.dist = reinterpret_cast<const float*>(b + oDist); s.wRow = g->scratch.as<double>();
+  s.nc = uint32_t(nc); s.K = uint32_t(K); s.nAz = v.nAz; s.nClass = nClass; s.nReach = nReach; s.glazing = glazing ? 1u : 0u; s.NA = NA;
+  s.farF = float(farDist); s.lnFar = std::log(farDist);
+  for (uint32_t r = 0; r < 8; ++r) s.reach[r] = r < nReach ? reach[r] : 0.f;
+  s.out = g->out.as<double>();
+  dlg_tbegin(g);
+  k_view_sums<<<unsigned((nc + 3) / 4), 256, 0, g->stream>>>(s);
+  if (isoDim) {
+    IsoArgs a;
+    a.cls = s.cls; a.dist = s.dist; a.rowSin = reinterpret_cast<const float*>(b + oSin);
+    a.nc = uint32_t(nc); a.nAz = v.nAz; a.nEl = v.nEl; a.i0 = uint32_t(g->viewI0); a.nS = v.nS; a.back = v.back; a.world = v.world;
+    a.jitter = v.jitter; a.dim = isoDim; a.NA = NA; a.az0 = v.az0; a.dAz = v.dAz; a.farF = float(farDist); a.out = s.out;
+    k_view_iso<<<unsigned((nc + 3) / 4), 256, 0, g->stream>>>(a);
+  }
+  DLG_CHECK(hipGetLastError());
+  g->stats.lastViewReduceMs = dlg_tend(g);
+  DLG_CHECK(hipMemcpyAsync(out, g->out.p, size_t(nc) * NA * 8, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+extern "C" int32_t dlg_download_view(dlg_gpu* g, int64_t nc, uint8_t* cls, float* dist, char* err, int32_t errLen) {
+  if (nc != g->viewN) { dlg_fail(err, errLen, "dlg_download_view: no dlg_view for this chunk"); return 1; }
+  const size_t K = size_t(g->viewK), n = size_t(nc) * K;
+  if (!n) return 0;
+  std::vector<uint8_t> c(n);
+  std::vector<float> t(n);
+  const char* b = g->view.as<char>();
+  DLG_CHECK(hipMemcpyAsync(c.data(), b, n, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipMemcpyAsync(t.data(), b + align256(n), n * 4, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  for (size_t d = 0; d < K; ++d)
+    for (size_t i = 0; i < size_t(nc); ++i) { cls[i * K + d] = c[d * size_t(nc) + i]; dist[i * K + d] = t[d * size_t(nc) + i]; }
+  return 0;
+}

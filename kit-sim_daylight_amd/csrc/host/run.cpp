@@ -1,0 +1,529 @@
+// SPDX-License-Identifier: Apache-2.0
+// daylight::run (reference src/run.cpp:216-758): validation, clocks, steps, the sky of every step on the GPU (dlg_sky),
+// the direction table and per-quantity weights, the resident scene (LBVH in HBM, keyed), the sensors, and the chunk
+// loop: trace -> every quantity reduced in one pass -> (series) the contraction on MFMA, streamed to .npy.
+// Host work here is set-up of size O(steps + patches + directions) and the file I/O; everything per ray, per sensor
+// or per (step, patch) runs on the GPU.
+// Not in this build (the reference's next rows): transmitting slots (--transmission on with tau > 0),
+// reflected light (--bounces), the Ladybug emulation mode. A run that asks for them fails with a message saying so.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+
+#include "core.h"
+#include "dl_math.h"
+
+namespace daylight {
+namespace {
+
+using File = std::unique_ptr<std::FILE, decltype(&std::fclose)>;
+using Clk = std::chrono::steady_clock;
+double ms(Clk::time_point t0) { return std::chrono::duration<double, std::milli>(Clk::now() - t0).count(); }
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kRf = 0.265074126, kGf = 0.670114631, kBf = 0.064811243;
+constexpr double kNight = 1e-4;
+constexpr size_t kVramMargin = size_t(128) << 20;
+
+// The process's GPU and its resident scenes: `cur` installed, `parked` kept in HBM under their keys, most recent first.
+struct Resident {
+  dlg_gpu* gpu = nullptr;
+  Scene cur;
+  std::vector<Scene> parked;
+};
+Resident& resident() { static Resident r; return r; }
+std::string gib(size_t b) { char s[32]; std::snprintf(s, sizeof s, "%.2f GB", b / 1073741824.0); return s; }
+
+// The contraction's arithmetic for --timeseries: DAYLIGHT_SERIES = fp32 | bf16x3 | bf16x6 (default bf16x6, FP32-level).
+int series_method() {
+  const char* e = std::getenv("DAYLIGHT_SERIES");
+  if (!e) return 2;
+  if (!std::strcmp(e, "fp32")) return 0;
+  if (!std::strcmp(e, "bf16x3")) return 1;
+  return 2;
+}
+
+}  // namespace
+
+bool npy_write_header(std::FILE* f, size_t rows, size_t cols, const char* descr) {
+  char dict[256];
+  const int n = std::snprintf(dict, sizeof dict, "{'descr': '%s', 'fortran_order': False, 'shape': (%zu, %zu), }", descr, rows, cols);
+  if (n <= 0) return false;
+  const size_t total = 10 + size_t(n) + 1, pad = (64 - total % 64) % 64;
+  const uint16_t hlen = uint16_t(size_t(n) + pad + 1);
+  if (std::fwrite("\x93NUMPY\x01\x00", 1, 8, f) != 8 || std::fwrite(&hlen, 2, 1, f) != 1) return false;
+  if (std::fwrite(dict, 1, size_t(n), f) != size_t(n)) return false;
+  for (size_t i = 0; i < pad; ++i) std::fputc(' ', f);
+  std::fputc('\n', f);
+  return true;
+}
+
+dlg_gpu* process_gpu(std::string& err) {
+  Resident& R = resident();
+  if (!R.gpu) {
+    char e[512] = {};
+    if (dlg_init(&R.gpu, e, sizeof e)) { err = e; R.gpu = nullptr; }
+  }
+  return R.gpu;
+}
+
+bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, const std::string& key,
+                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage,
+                   std::string& err) {
+  Resident& R = resident();
+  const bool installed = !key.empty() && R.gpu && key == R.cur.key;
+  const auto parkedAt = [&] { return std::find_if(R.parked.begin(), R.parked.end(), [&](const Scene& p) { return p.key == key; }); };
+  const bool reuse = installed || (!key.empty() && R.gpu && parkedAt() != R.parked.end());
+  out.sceneReused = reuse;
+  s = Scene{};
+  if (reuse) { s = installed ? R.cur : *parkedAt(); s.parked = nullptr; }
+  if (!gpu) return true;   // a run without rays: nothing to install
+  if (cancel && cancel->load()) { err = "cancelled"; return false; }
+  dlg_gpu* g = process_gpu(err);
+  if (!g) return false;
+  *gpu = g;
+  if (installed) return true;
+  if (!R.cur.key.empty() && residentScenes > 1) {   // park the installed keyed scene
+    R.parked.insert(R.parked.begin(), R.cur);
+    R.parked.front().parked = dlg_detach(g);
+  } else dlg_free_scene(dlg_detach(g));
+  R.cur = Scene{};
+  if (auto it = parkedAt(); reuse && it != R.parked.end()) {
+    Scene p = *it;
+    R.parked.erase(it);
+    dlg_attach(g, p.parked);
+    p.parked = nullptr;
+    R.cur = p;
+    return true;
+  }
+  stage = "triangulating the scene";
+  const auto tPrep = Clk::now();
+  bbox_centre(meshes, s.c);
+  std::vector<size_t> vBase(meshes.size() + 1, 0), tBase(meshes.size() + 1, 0);
+  for (size_t k = 0; k < meshes.size(); ++k) {
+    vBase[k + 1] = vBase[k] + (blocks(k) ? meshes[k].mesh->numPoints() : 0);
+    tBase[k + 1] = tBase[k] + (blocks(k) ? tri_cap(*meshes[k].mesh) : 0);
+  }
+  const size_t nPts = vBase.back(), nTri = tBase.back();
+  stage = "making room on the GPU";
+  dlg_trim(g, 1);
+  const size_t need = size_t(dlg_build_bytes(int64_t(nPts), int64_t(nTri))) + kVramMargin;
+  while (!R.parked.empty() && size_t(dlg_free_bytes()) < need) { dlg_free_scene(R.parked.back().parked); R.parked.pop_back(); }
+  if (const size_t free = size_t(dlg_free_bytes()); free < need) {
+    err = "out of memory: building the scene (" + std::to_string(nTri) + " triangles) needs " + gib(need) + " of GPU memory, " + gib(free) +
+          " free (close other programs that use the GPU, or run fewer meshes)";
+    return false;
+  }
+  stage = "triangulating the scene";
+  Tris tris;
+  tris.v.resize(nPts);
+  tris.i.resize(nTri);
+  std::vector<size_t> made(meshes.size(), 0);
+  std::vector<int> skip(meshes.size(), 0);
+  par_for(meshes.size(), [&](size_t k) {
+    if (!blocks(k)) return;
+    made[k] = triangulate_into(*meshes[k].mesh, s.c, tris.v.data() + vBase[k], tris.i.data() + tBase[k], unsigned(vBase[k]), skip[k]);
+  });
+  size_t at = 0;   // close the gaps of faces that drew fewer than n - 2 triangles
+  for (size_t k = 0; k < meshes.size(); ++k) {
+    if (at != tBase[k] && made[k]) std::memmove(tris.i.data() + at, tris.i.data() + tBase[k], made[k] * sizeof(uint3));
+    if (blocks(k)) s.order.push_back({meshes[k].name, at, made[k], meshes[k].mesh});
+    at += made[k];
+    s.skipped += skip[k];
+  }
+  tris.i.resize(at);
+  out.stats.prepMs += ms(tPrep);
+  stage = "building the scene";
+  const auto tBuild = Clk::now();
+  char e[512] = {};
+  if (dlg_build(g, reinterpret_cast<const float*>(tris.v.data()), int64_t(tris.v.size()), reinterpret_cast<const uint32_t*>(tris.i.data()),
+                int64_t(tris.i.size()), e, sizeof e)) {
+    err = e;
+    return false;
+  }
+  out.stats.buildMs += ms(tBuild);
+  s.nTris = tris.i.size();
+  s.key = key;
+  R.cur = s;
+  return true;
+}
+
+bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
+                   std::string& err) {
+  Resident& R = resident();
+  dlg_trim(g, 1);
+  const auto need = [&] {
+    const size_t nc = std::min(nSens, std::max<size_t>(1, chunkBytes / perSensor));
+    return nSens * 2 * 12 + nc * chunkSensor + fixedBytes + kVramMargin;
+  };
+  while (!R.parked.empty() && (R.parked.size() + 1 > size_t(std::max(1, residentScenes)) || size_t(dlg_free_bytes()) < need())) {
+    dlg_free_scene(R.parked.back().parked);
+    R.parked.pop_back();
+  }
+  while (size_t(dlg_free_bytes()) < need() && chunkBytes > (size_t(32) << 20)) chunkBytes /= 2;
+  if (const size_t free = size_t(dlg_free_bytes()); free < need()) {
+    err = "out of memory: the run's " + std::to_string(nSens) + " sensors need " + gib(need()) + " of GPU memory beside the scene, " + gib(free) +
+          " free (close other programs that use the GPU, or use fewer receivers)";
+    return false;
+  }
+  return true;
+}
+
+std::string open_key(const std::vector<char>& open, bool selfOcclude) {
+  std::string s;
+  for (size_t k = 0; selfOcclude && k < open.size(); ++k) if (open[k]) s += (s.empty() ? "\nopen " : ",") + std::to_string(k);
+  return s;
+}
+
+void scene_forget(const std::string& key) {
+  Resident& R = resident();
+  if (R.cur.key == key) R.cur.key.clear();
+  for (auto it = R.parked.begin(); it != R.parked.end();)
+    if (it->key == key) { dlg_free_scene(it->parked); it = R.parked.erase(it); } else ++it;
+}
+
+bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
+                std::string& warn, std::string& err) {
+  const size_t nS = opt.sides == Sides::Both ? 2 : 1;
+  s.p.resize(nSens);
+  s.n.resize(nSens);
+  std::vector<size_t> sAt(meshes.size() + 1, 0), elems(meshes.size(), 0);
+  for (size_t k = 0; k < meshes.size(); ++k)
+    sAt[k + 1] = sAt[k] + (meshes[k].sensor ? (opt.onVertices ? meshes[k].mesh->numPoints() : meshes[k].mesh->numFaces()) * nS : 0);
+  std::vector<std::string> warns(meshes.size());
+  par_for(meshes.size(), [&](size_t k) {
+    if (meshes[k].sensor) elems[k] = make_sensors_into(*meshes[k].mesh, opt, c, s.p.data() + sAt[k], s.n.data() + sAt[k], warns[k]);
+  });
+  out.ranges.clear();
+  for (size_t k = 0; k < meshes.size(); ++k) {
+    if (!meshes[k].sensor) continue;
+    if (!warns[k].empty()) warn += (warn.empty() ? "" : "; ") + warns[k];
+    out.ranges.emplace_back(meshes[k].name, elems[k]);
+  }
+  if (sAt.back() != nSens) { err = "internal: " + std::to_string(sAt.back()) + " sensors made for " + std::to_string(nSens); return false; }
+  return true;
+}
+
+static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const Weather* w, const Options& opt, Result& out, std::string& err,
+                         const char*& stage, const SeriesSink* sink) {
+  const auto cancelled = [&] { if (opt.cancel && opt.cancel->load()) { err = "cancelled"; return true; } return false; };
+  out.stats = RunStats{};
+  // ---- validation (run.cpp:219-242)
+  if (!optics_ok(opt, err)) return false;
+  bool needWeather = false, wantVis = false, wantRad = false;
+  for (const Quantity q : opt.quantities)
+    if (q != Quantity::SunHours) { needWeather = true; if (q == Quantity::Illuminance) wantVis = true; else wantRad = true; }
+  if (needWeather && !w) { err = "weather (--epw) required for radiation/irradiance/illuminance"; return false; }
+  if (opt.mode == Mode::Ladybug) { err = "--mode ladybug is not part of this build (physical mode only)"; return false; }
+  if (!opt.dumpSky.empty()) { err = "--dump-sky is ladybug-mode only (with weather)"; return false; }
+  if (opt.mf != 1 && opt.mf != 2 && opt.mf != 4) { err = "--sky-density must be 1, 2 or 4"; return false; }
+  const int K = std::max(1, opt.patchSamples);
+  {
+    const int m = int(std::lround(std::sqrt(double(K))));
+    if (m * m != K) { err = "--patch-samples must be a square number (1, 4, 9, 16, ...)"; return false; }
+  }
+  const bool series = !opt.seriesPath.empty();
+  bool wantDc = series && needWeather;
+  for (const Quantity q : opt.quantities) wantDc = wantDc || (q != Quantity::Direct && q != Quantity::SunHours);
+  if (opt.bounces > 0 && wantDc) { err = "--bounces (reflected light) is not part of this build"; return false; }
+  double lat = w ? w->lat : opt.lat, lon = w ? w->lon : opt.lon, tz = w ? w->tz : opt.tz;
+  if (!std::isnan(opt.lat)) lat = opt.lat;
+  if (!std::isnan(opt.lon)) lon = opt.lon;
+  if (w) tz = w->tz + (lon - w->lon) / 15;
+  const double shift = w && !std::isnan(opt.tz) ? opt.tz - tz : 0;
+  if (std::isnan(lat) || std::isnan(lon) || std::isnan(tz)) { err = "location unknown: give --epw or --lat --lon --tz"; return false; }
+  const bool leap = w ? w->leap : false;
+  if (opt.verbose && w && w->shiftH != 0) std::fprintf(stderr, "info: PVGIS weather: irradiance moved %.2f h later to meet its sun\n", w->shiftH);
+
+  // ---- steps
+  std::vector<Step> steps;
+  if (!make_steps(opt, w, steps, err, shift)) return false;
+  if (steps.empty()) { err = "no timesteps selected"; return false; }
+  const size_t T = steps.size();
+  double durationHours = 0;
+  for (const Step& s : steps) durationHours += s.dt;
+  out.durationHours = durationHours;
+
+  // ---- geometry roles
+  std::vector<SlotOptics> slots;
+  std::vector<char> open;
+  if (!resolve_optics(meshes, opt.optics, slots, err) || !resolve_no_occlude(meshes, opt.noOcclude, open, err)) return false;
+  const auto blocks = [&](size_t k) { return !meshes[k].sensor || (opt.selfOcclude && !open[k]); };
+  if (opt.transmission) {   // a slot with faces in the scene that lets light through
+    for (size_t k = 0, b = 0; k < meshes.size(); b += slot_count(*meshes[k++].mesh)) {
+      const SimulationMesh& m = *meshes[k].mesh;
+      if (!blocks(k)) continue;
+      for (size_t f = 0; f < m.numFaces(); ++f)
+        if (slots[b + (m.slot.empty() ? 0 : m.slot[f])].tau > 0) { err = "--transmission on with transmitting slots is not part of this build"; return false; }
+    }
+  }
+  const std::string key = opt.sceneKey.empty() ? "" : opt.sceneKey + open_key(open, opt.selfOcclude);
+  const int nS = opt.sides == Sides::Both ? 2 : 1;
+  size_t nElem = 0;
+  for (const SimulationMeshView& v : meshes) if (v.sensor) nElem += opt.onVertices ? v.mesh->numPoints() : v.mesh->numFaces();
+  const size_t nSens = nElem * size_t(nS);
+  if (nSens == 0) { err = "no sensor elements (no sensor meshes, or all faces degenerate)"; return false; }
+
+  // ---- the sky of every step, on the GPU (sun, Perez, cumulative weights, series matrix)
+  if (cancelled()) return false;
+  const auto tSky = Clk::now();
+  stage = "computing the sky";
+  dlg_gpu* g = process_gpu(err);
+  if (!g) return false;
+  const Patches pa = make_patches(opt.mf);
+  const int P = pa.P;
+  const size_t Pz = size_t(P);
+  std::vector<int> month(T), day(T), jd(T);
+  std::vector<double> hour(T), dtv(T), alt(T), azi(T), sunRadT(T), sunVisT(T), skyRad(Pz, 0.0), skyVis(Pz, 0.0);
+  std::vector<float> dni(T), dhi(T), vec(3 * T);
+  std::vector<int> up(T);
+  for (size_t t = 0; t < T; ++t) {
+    month[t] = steps[t].month; day[t] = steps[t].day; jd[t] = steps[t].jd; hour[t] = steps[t].hour; dtv[t] = steps[t].dt;
+    dni[t] = steps[t].dni; dhi[t] = steps[t].dhi;
+  }
+  const bool seriesVis = series && opt.quantities[0] == Quantity::Illuminance;
+  const double grefl = opt.virtualGround ? opt.ground : 0;
+  double gndRad = 0, gndVis = 0;
+  int Tup = 0;
+  {
+    dlg_sky_in in{int32_t(T), month.data(), day.data(), jd.data(), hour.data(), dtv.data(), dni.data(), dhi.data(), lat, lon, tz, opt.north, grefl,
+                  leap ? 1 : 0, opt.mf, needWeather && wantRad ? 1 : 0, needWeather && wantVis ? 1 : 0, series && needWeather ? 1 : 0,
+                  seriesVis ? 1 : 0, tcos_table()};
+    dlg_sky_out so{alt.data(), azi.data(), up.data(), vec.data(), skyRad.data(), skyVis.data(), &gndRad, &gndVis, sunRadT.data(), sunVisT.data(), &Tup};
+    char e[512] = {};
+    if (dlg_sky(g, &in, &so, e, sizeof e)) { err = e; return false; }
+  }
+  // direction table [sky P K | ground = -sky | sun up-steps] (run.cpp:316-340)
+  std::vector<float3> D;
+  std::vector<int> rowOf;
+  size_t nSky = 0;
+  if (needWeather) {
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < K; ++k) { D.push_back(dlm::rotate_z(patch_sample(pa, p, k, K), opt.north)); rowOf.push_back(p); }
+    nSky = D.size();
+    for (size_t d = 0; d < nSky; ++d) D.push_back(make_float3(-D[d].x, -D[d].y, -D[d].z));
+  }
+  const size_t sun0 = D.size();
+  std::vector<int> sunStep;
+  for (size_t t = 0; t < T; ++t)
+    if (up[t]) { D.push_back(make_float3(vec[3 * t], vec[3 * t + 1], vec[3 * t + 2])); sunStep.push_back(int(t)); }
+  const size_t nSun = sunStep.size(), nDir = D.size();
+  out.nSky = nSky;
+  out.nSun = nSun;
+  std::vector<int> colOf(T, -1);
+  for (size_t t = 0, c = 0; t < T; ++t) if (steps[t].dni + steps[t].dhi > kNight) colOf[t] = int(c++);
+  if (!series) Tup = 0;
+  // ---- per-quantity weights (run.cpp:475-505); the ground visibility weights for the series
+  const size_t nQ = opt.quantities.size();
+  std::vector<float> Wq(nQ * nDir + (series ? nDir : 0), 0.f);
+  std::vector<double> sunRad(nSun, 0.0), sunVis(nSun, 0.0);
+  double droppedDni = 0;
+  for (size_t j = 0; j < nSun; ++j) { sunRad[j] = needWeather ? sunRadT[size_t(sunStep[j])] : 0.0; sunVis[j] = needWeather ? sunVisT[size_t(sunStep[j])] : 0.0; }
+  for (size_t t = 0; t < T && needWeather; ++t) if (!up[t] && steps[t].dni > 0) droppedDni += double(steps[t].dni) * steps[t].dt;
+  for (size_t qi = 0; qi < nQ; ++qi) {
+    const Quantity q = opt.quantities[qi];
+    float* W = Wq.data() + qi * nDir;
+    if (q == Quantity::SunHours) {
+      for (size_t j = 0; j < nSun; ++j) W[sun0 + j] = float(steps[size_t(sunStep[j])].dt);
+      continue;
+    }
+    const bool vis = q == Quantity::Illuminance, wantSky = q != Quantity::Direct, wantSun = q != Quantity::Diffuse;
+    for (size_t d = 0; d < nSky; ++d) {
+      const size_t p = size_t(rowOf[d]);
+      W[d] = float(wantSky ? (vis ? skyVis[p] : skyRad[p]) * pa.omega[p] / K : 0.0);
+    }
+    if (q != Quantity::Direct)
+      for (size_t d = nSky; d < 2 * nSky; ++d) W[d] = float((vis ? gndVis : gndRad) * pa.omega[size_t(rowOf[d - nSky])] / K);
+    if (wantSun) for (size_t j = 0; j < nSun; ++j) W[sun0 + j] = float(vis ? sunVis[j] : sunRad[j]);
+  }
+  if (series) {
+    float* Wg = Wq.data() + nQ * nDir;
+    for (size_t d = nSky; d < 2 * nSky; ++d) Wg[d] = pa.omega[size_t(rowOf[d - nSky])] / float(K);
+  }
+  out.stats.skyMs = ms(tSky);
+  if (opt.verbose && droppedDni > 0)
+    std::fprintf(stderr, "warn: %.1f Wh/m2 of DNI at steps with the sun below the horizon dropped (gendaymtx would bin it)\n", droppedDni);
+
+  // ---- the scene, the sensors, the result
+  const size_t nWords = (nDir + 31) / 32;
+  const size_t perSensor = std::max<size_t>(1, std::max(4 * nWords, series ? 4 * size_t(Tup) : size_t(0)));
+  size_t chunkBytes = opt.chunkBytes;
+  File sf(nullptr, std::fclose), bf(nullptr, std::fclose);
+  std::vector<double> seriesHoys;
+  if (series && !sink) {
+    sf.reset(std::fopen(opt.seriesPath.c_str(), "wb"));
+    if (!sf || !npy_write_header(sf.get(), nSens, size_t(Tup), "<f4")) { err = "cannot write " + opt.seriesPath; return false; }
+    for (size_t t = 0; t < T; ++t) if (colOf[t] >= 0) seriesHoys.push_back(steps[t].hoy);
+  }
+  if (!opt.dumpBitsPath.empty()) {
+    bf.reset(std::fopen(opt.dumpBitsPath.c_str(), "wb"));
+    if (!bf || !npy_write_header(bf.get(), nSens, nWords, "<u4")) { err = "cannot write " + opt.dumpBitsPath; return false; }
+  }
+  dlg_gpu* gp = nullptr;
+  Scene sc;
+  if (!scene_acquire(meshes, blocks, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
+  dlg_stats_t st0{};
+  if (gp) {
+    dlg_stats(gp, &st0);
+    const size_t buffers = 1 + (series ? 3 : 0);   // the bits; the series' output and its split operands
+    if (!run_admission(gp, nSens, perSensor, buffers * perSensor + 8 * (nQ + 1), 0, opt.residentScenes, chunkBytes, err)) return false;
+  }
+  stage = "making the sensors";
+  const auto tSens = Clk::now();
+  Sensors sens;
+  std::string warn;
+  if (!sensors_of(meshes, opt, sc.c, nSens, sens, out, warn, err)) return false;
+  out.stats.prepMs += ms(tSens);
+  if (opt.verbose) {
+    std::fprintf(stderr, "info: %zu sensors, %zu triangles%s, %zu steps (%.1f h)\n", nSens, sc.nTris, out.sceneReused ? " (scene reused)" : "", T, durationHours);
+    if (sc.skipped) std::fprintf(stderr, "warn: %d degenerate faces skipped in occluders\n", sc.skipped);
+    if (!warn.empty()) std::fprintf(stderr, "warn: %s\n", warn.c_str());
+  }
+  if (gp) {
+    char e[512] = {};
+    if (dlg_set_scene(gp, reinterpret_cast<const float*>(sens.p.data()), reinterpret_cast<const float*>(sens.n.data()), int64_t(nSens),
+                      reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e)) { err = e; return false; }
+  }
+  sens = Sensors{};
+
+  stage = "allocating the result";
+  const size_t C = nQ * size_t(nS);
+  out.columns.clear();
+  for (size_t qi = 0; qi < nQ; ++qi)
+    for (int s = 0; s < nS; ++s)
+      out.columns.push_back(std::string(kQuantityNames[size_t(opt.quantities[qi])]) +
+                            ((opt.sides == Sides::Back || (opt.sides == Sides::Both && s == 1)) ? "_back" : "_front"));
+  out.values.clear();
+  out.f32.clear();
+  if (opt.f32Columns) out.f32.assign(C * nElem, 0.f);
+  else out.values.assign(nElem * C, 0.0);
+  const auto store = [&](size_t si, size_t qi, double v) {
+    const Quantity q = opt.quantities[qi];
+    if (q == Quantity::Irradiance) v = durationHours > 0 ? v * 1000.0 / durationHours : 0.0;
+    else if (q == Quantity::Illuminance) v = durationHours > 0 ? v / durationHours : 0.0;
+    const size_t e = si / size_t(nS), col = qi * size_t(nS) + si % size_t(nS);
+    if (opt.f32Columns) out.f32[col * nElem + e] = float(v);
+    else out.values[e * C + col] = v;
+  };
+
+  if (gp) {
+    stage = "tracing";
+    const size_t ncMax = std::max<size_t>(1, chunkBytes / perSensor);
+    const uint32_t flags = dlg_ray_flags(opt.cullBackfaces ? 1 : 0, 0);
+    std::vector<int32_t> useCos(nQ + 1, 1);
+    for (size_t qi = 0; qi < nQ; ++qi) useCos[qi] = opt.quantities[qi] != Quantity::SunHours;
+    const int nR = int(nQ) + (series ? 1 : 0);
+    std::vector<double> tmp;
+    std::vector<float> G, E;
+    std::vector<uint32_t> bitsBuf, bitRows;
+    std::vector<float> wsky(nSky), sunW(nSun, 0.f);
+    std::vector<int32_t> sunCol(nSun, 0);
+    for (size_t d = 0; d < nSky; ++d) wsky[d] = pa.omega[size_t(rowOf[d])] / float(K);
+    for (size_t j = 0; j < nSun; ++j) {
+      const int c = colOf[size_t(sunStep[j])];
+      sunCol[j] = c < 0 ? 0 : c;
+      const double dtj = steps[size_t(sunStep[j])].dt;
+      sunW[j] = (c < 0 || dtj <= 0) ? 0.f : float((seriesVis ? sunVis[j] : sunRad[j] * 1000.0) / dtj);
+    }
+    const int method = series_method();
+    for (size_t i0 = 0; i0 < nSens; i0 += ncMax) {
+      if (cancelled()) return false;
+      const size_t nc = std::min(ncMax, nSens - i0);
+      char e[512] = {};
+      if (dlg_trace(gp, int64_t(i0), int64_t(nc), flags, e, sizeof e)) { err = e; return false; }
+      const auto tRed = Clk::now();
+      tmp.assign(size_t(nR) * nc, 0.0);
+      if (dlg_reduce(gp, int64_t(nc), nR, Wq.data(), useCos.data(), tmp.data(), e, sizeof e)) { err = e; return false; }
+      for (size_t qi = 0; qi < nQ; ++qi)
+        for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, tmp[qi * nc + i]);
+      out.stats.reduceMs += ms(tRed);
+      if (opt.progress) opt.progress->store(double(i0 + nc) / double(nSens));
+      if (bf) {
+        bitsBuf.resize(nWords * nc);
+        if (dlg_download_bits(gp, int64_t(nc), bitsBuf.data(), e, sizeof e)) { err = e; return false; }
+        bitRows.resize(nc * nWords);
+        for (size_t wi = 0; wi < nWords; ++wi)
+          for (size_t i = 0; i < nc; ++i) bitRows[i * nWords + wi] = bitsBuf[wi * nc + i];
+        if (std::fwrite(bitRows.data(), 4, bitRows.size(), bf.get()) != bitRows.size()) { err = "write failed: " + opt.dumpBitsPath; return false; }
+      }
+      if (series) {
+        G.resize(nc);
+        for (size_t i = 0; i < nc; ++i) G[i] = float(tmp[nQ * nc + i]);
+        const bool compute = Tup > 0 && needWeather;
+        dlg_series_in in{uint32_t(nSky), rowOf.data(), wsky.data(), P, Tup, G.data(), uint32_t(sun0), uint32_t(nSun), sunCol.data(), sunW.data()};
+        if (sink) {   // the series stays on the device: the sink reads each chunk there
+          if (compute && dlg_series(gp, int64_t(nc), &in, method, nullptr, e, sizeof e)) { err = e; return false; }
+          if (*sink && !(*sink)(i0, nc, size_t(Tup), compute ? dlg_series_device(gp) : nullptr, err)) return false;
+          continue;
+        }
+        E.assign(nc * size_t(Tup), 0.f);
+        if (compute && dlg_series(gp, int64_t(nc), &in, method, E.data(), e, sizeof e)) { err = e; return false; }
+        if (std::fwrite(E.data(), sizeof(float), E.size(), sf.get()) != E.size()) { err = "write failed: " + opt.seriesPath; return false; }
+      }
+    }
+    dlg_stats_t st1{};
+    dlg_stats(gp, &st1);
+    out.stats.traceMs = st1.traceMs - st0.traceMs;
+    out.stats.rays = size_t(st1.rays - st0.rays);
+    out.stats.launches = size_t(st1.launches - st0.launches);
+    out.stats.gasBytes = size_t(st1.sceneBytes);
+    if (opt.verbose)
+      std::fprintf(stderr, "info: scene %.1f MB (LBVH height %lld) build %.0f ms, %zu launches, %.3g rays in %.0f ms (%.2f GRays/s)\n",
+                   st1.sceneBytes / 1048576.0, (long long)st1.height, st1.buildMs, out.stats.launches, double(out.stats.rays), out.stats.traceMs,
+                   out.stats.traceMs > 0 ? double(out.stats.rays) / out.stats.traceMs / 1e6 : 0.0);
+  }
+  if (sf) {
+    File hf(std::fopen((opt.seriesPath + ".hoys.txt").c_str(), "w"), std::fclose);
+    if (hf) for (const double h : seriesHoys) std::fprintf(hf.get(), "%.4f\n", h);
+  }
+  if (!opt.dumpSun.empty()) {   // the NOAA sun per step (columns of the reference's dump; Radiance's sun.c columns repeat it)
+    File f(std::fopen(opt.dumpSun.c_str(), "w"), std::fclose);
+    if (!f) { err = "cannot write " + opt.dumpSun; return false; }
+    std::fprintf(f.get(), "# hoy month day hour alt_deg azi_deg up dni dhi   (north %.3f)\n", opt.north);
+    for (size_t t = 0; t < T; ++t)
+      std::fprintf(f.get(), "%.4f %d %d %.4f %.6f %.6f %d %.4f %.4f\n", steps[t].hoy, steps[t].month, steps[t].day, steps[t].hour, alt[t], azi[t],
+                   up[t], steps[t].dni, steps[t].dhi);
+  }
+  (void)kPi; (void)kRf; (void)kGf; (void)kBf;
+  return true;
+}
+
+bool run_series_device(const std::vector<SimulationMeshView>& meshes, const Weather* w, const Options& opt, const SeriesSink& sink, Result& out,
+                       std::string& err) {
+  const char* stage = "preparing the run";
+  try {
+    return run_daylight(meshes, w, opt, out, err, stage, &sink);
+  } catch (const std::bad_alloc&) {
+    out = Result{};
+    err = std::string("out of memory: host memory ran out while ") + stage;
+  } catch (const std::exception& e) {
+    out = Result{};
+    err = std::string("internal error while ") + stage + ": " + e.what();
+  }
+  return false;
+}
+
+bool run(const std::vector<SimulationMeshView>& meshes, const Weather* w, const Options& opt, Result& out, std::string& err) {
+  const char* stage = "preparing the run";
+  try {
+    return run_daylight(meshes, w, opt, out, err, stage, nullptr);
+  } catch (const std::bad_alloc&) {
+    out = Result{};
+    err = std::string("out of memory: host memory ran out while ") + stage;
+  } catch (const std::exception& e) {
+    out = Result{};
+    err = std::string("internal error while ") + stage + ": " + e.what();
+  }
+  return false;
+}
+
+bool run(const std::vector<SimulationMesh>& meshes, const Weather* w, const Options& opt, Result& out, std::string& err) {
+  return run(views_of(meshes), w, opt, out, err);
+}
+
+}  // namespace daylight

@@ -1,0 +1,277 @@
+// SPDX-License-Identifier: Apache-2.0
+// Host inputs of the sky and the sun: the Reinhart patch table and its samples (reference sky.cpp:240-294), Radiance's
+// tcos table, evaluation steps, timestamps, the EPW reader with its PVGIS realignment, Spencer / Ineichen clear sky
+// (reference solar.cpp). The per-step sun and Perez sky of a run are computed on the GPU (sky.hip); clear_sky_weather,
+// which makes an input file's worth of weather, calls the same __host__ __device__ sun as the kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
+#include "core.h"
+#include "dl_math.h"
+
+namespace daylight {
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kRowAz[7] = {30, 30, 24, 24, 18, 12, 6};
+double rad(double d) { return d * kPi / 180; }
+double pmod(double a, double m) { const double r = std::fmod(a, m); return r < 0 ? r + m : r; }
+int month_days(int m, bool leap) { return dlm::month_days(m, leap); }
+
+float3 altaz(double alt, double azi) { return dlm::altaz_vec(alt, azi); }
+
+// weather between the centres of hour-ending rows of a cyclic year: row r at r + 0.5
+float lerp_rows(const std::vector<float>& v, double t) {
+  const long n = long(v.size());
+  const double x = t - 0.5, f = x - std::floor(x);
+  const long a = ((long(std::floor(x)) % n) + n) % n, b = (a + 1) % n;
+  return float(v[size_t(a)] * (1 - f) + v[size_t(b)] * f);
+}
+
+// PVGIS's satellite irradiance runs ahead of its hour labels; measure the lead from each day's first and last lit rows
+// against solar noon and move DNI / DHI later by it when it is clearly off (reference solar.cpp:36-63).
+void pvgis_align(Weather& w, const std::vector<float>& ghi, double declared) {
+  if (std::fabs(declared + 0.5) < 0.05) return;
+  const double noon = 12 - (w.lon / 15 - w.tz);
+  double sum = 0;
+  int n = 0;
+  for (size_t day = 0; (day + 1) * 24 <= ghi.size(); ++day) {
+    int first = -1, last = -1;
+    for (int h = 0; h < 24; ++h)
+      if (ghi[day * 24 + size_t(h)] > 0) { if (first < 0) first = h; last = h; }
+    if (first <= 0 || last >= 23) continue;
+    sum += pmod((first + last) / 2.0 + 0.5 - noon + 12, 24) - 12;
+    ++n;
+  }
+  if (n < 30) return;
+  const double lead = sum / n;
+  if (std::fabs(lead) <= 0.3) return;
+  const std::vector<float> dni = w.dni, dhi = w.dhi;
+  for (size_t r = 0; r < dni.size(); ++r) {
+    const double t = double(r) + 0.5 + lead;
+    w.dni[r] = lerp_rows(dni, t);
+    w.dhi[r] = lerp_rows(dhi, t);
+  }
+  w.shiftH = -lead;
+}
+}  // namespace
+
+Patches make_patches(int mf) {
+  Patches pa;
+  const double alpha = (kPi / 2.) / (7 * mf + .5);
+  pa.alpha = float(alpha);
+  for (int r = 0; r < 7 * mf; ++r) {
+    const int n = kRowAz[r / mf] * mf;
+    const float a = float(alpha * (r + .5));
+    const float w = float(2. * kPi * (std::sin(alpha * (r + 1)) - std::sin(alpha * r)) / double(n));
+    for (int j = 0; j < n; ++j) {
+      pa.alt.push_back(a);
+      pa.azi.push_back(float(2. * kPi * j / double(n)));
+      pa.omega.push_back(w);
+      pa.row.push_back(r);
+      pa.ninrow.push_back(n);
+    }
+  }
+  pa.alt.push_back(float(kPi / 2.));
+  pa.azi.push_back(0.f);
+  pa.omega.push_back(float(2. * kPi * (1. - std::cos(alpha * .5))));
+  pa.row.push_back(7 * mf);
+  pa.ninrow.push_back(1);
+  pa.P = int(pa.alt.size());
+  for (int p = 0; p < pa.P; ++p) {
+    pa.centre.push_back(altaz(pa.alt[p], pa.azi[p]));
+    pa.lb.push_back(p == pa.P - 1 ? make_float3(0.f, 0.f, 1.f)
+                                  : altaz(std::atan(std::cos(kPi / pa.ninrow[p]) * std::tan(double(pa.alt[p]))), pa.azi[p]));
+  }
+  return pa;
+}
+
+float3 patch_sample(const Patches& pa, int p, int k, int K) {
+  if (K <= 1) return pa.centre[size_t(p)];
+  const int m = int(std::lround(std::sqrt(double(K))));
+  const double u = ((k % m) + 0.5) / m, v = ((k / m) + 0.5) / m, a = pa.alpha;
+  if (p == pa.P - 1) {   // zenith cap: cos theta uniform in [cos(alpha / 2), 1]
+    const double ct = 1 - v * (1 - std::cos(a / 2)), st = std::sqrt(1 - ct * ct), az = 2 * kPi * u;
+    return make_float3(float(st * std::sin(az)), float(st * std::cos(az)), float(ct));
+  }
+  const int r = pa.row[size_t(p)];
+  const double s0 = std::sin(a * r), s1 = std::sin(a * (r + 1));
+  return altaz(std::asin(s0 + v * (s1 - s0)), pa.azi[size_t(p)] + (u - 0.5) * 2 * kPi / pa.ninrow[size_t(p)]);
+}
+
+const double* tcos_table() {
+  static const std::vector<double> t = [] {
+    std::vector<double> v(1025);
+    for (int k = 0; k <= 1024; ++k) v[size_t(k)] = std::cos((kPi / 2. / 1024) * k);
+    return v;
+  }();
+  return t.data();
+}
+
+void hoy_to_date(double hoy, bool leap, int& month, int& day, double& hour) {
+  int d = int(std::floor(hoy / 24));
+  hour = hoy - d * 24.0;
+  month = 1;
+  while (month < 12 && d >= month_days(month, leap)) d -= month_days(month++, leap);
+  day = d + 1;
+}
+
+bool parse_hoy(const std::string& s, bool leap, double& hoy, std::string& err) {
+  int y, m, d, h, mi;
+  if (std::sscanf(s.c_str(), "%d-%d-%dT%d:%d", &y, &m, &d, &h, &mi) != 5 && std::sscanf(s.c_str(), "%d-%dT%d:%d", &m, &d, &h, &mi) != 4) {
+    err = "bad timestamp '" + s + "', expected [YYYY-]MM-DDTHH:MM";
+    return false;
+  }
+  if (m < 1 || m > 12 || d < 1 || d > month_days(m, leap) || h < 0 || h > 23 || mi < 0 || mi > 59) {
+    err = "timestamp out of range '" + s + "'";
+    return false;
+  }
+  hoy = 24.0 * (d - 1) + h + mi / 60.0;
+  for (int k = 1; k < m; ++k) hoy += 24 * month_days(k, leap);
+  return true;
+}
+
+bool make_steps(const Options& opt, const Weather* w, std::vector<Step>& out, std::string& err, double shift) {
+  out.clear();
+  const bool leap = w && w->leap;
+  const double nh = leap ? 8784 : 8760;
+  const auto wrap = [&](double t) { return t - nh * std::floor(t / nh); };
+  if (std::isnan(opt.t0)) {   // annual: a step per weather row, at its midpoint
+    if (!w) { err = "no weather and no time range: nothing to evaluate"; return false; }
+    out.reserve(w->dni.size());
+    for (size_t r = 0; r < w->dni.size(); ++r) {
+      Step s;
+      s.hoy = wrap(double(r) + 0.5 + shift);
+      s.jd = int(r / 24) + 1;
+      s.dt = 1;
+      s.dni = w->dni[r];
+      s.dhi = w->dhi[r];
+      hoy_to_date(double(r) + 0.5, leap, s.month, s.day, s.hour);
+      out.push_back(s);
+    }
+    return true;
+  }
+  if (opt.stepMin <= 0) { err = "step must be a positive number of minutes"; return false; }
+  const bool instant = std::isnan(opt.t1), window = opt.dayFrom > 0 || opt.dayTo < 1440;
+  const double step = opt.stepMin / 60.0;
+  double t1 = instant ? opt.t0 + step : opt.t1;
+  if (t1 > nh) t1 = nh;
+  if (opt.t0 < 0 || opt.t0 >= nh || t1 <= opt.t0) { err = "time range must satisfy 0 <= from < to <= end of year"; return false; }
+  for (int k = 0;; ++k) {
+    const double hoy = opt.t0 + k * step;
+    if (hoy >= t1 - 1e-9) break;
+    double hod = (hoy - 24 * std::floor(hoy / 24)) * 60;
+    if (hod >= 1440 - 1e-6) hod -= 1440;
+    if (hod < opt.dayFrom - 1e-6 || hod >= opt.dayTo - 1e-6) continue;
+    double dt = step;
+    if (!instant) dt = std::min(dt, t1 - hoy);
+    if (!instant && window) dt = std::min(dt, (opt.dayTo - hod) / 60);
+    Step s;
+    s.hoy = instant ? hoy : hoy + dt / 2;
+    const double at = wrap(s.hoy - shift);
+    s.jd = int(at / 24) + 1;
+    s.dt = dt;
+    hoy_to_date(at, leap, s.month, s.day, s.hour);
+    if (w && !w->dni.empty()) { s.dni = lerp_rows(w->dni, at); s.dhi = lerp_rows(w->dhi, at); }
+    out.push_back(s);
+  }
+  if (out.empty()) { err = "no step falls inside the --hours window"; return false; }
+  return true;
+}
+
+double extraterrestrial_irradiance(int doy) {
+  const double B = 2 * kPi * (doy - 1) / 365;
+  return 1366.1 * (1.00011 + 0.034221 * std::cos(B) + 0.00128 * std::sin(B) + 0.000719 * std::cos(2 * B) + 0.000077 * std::sin(2 * B));
+}
+
+void ineichen(double z, double TL, double altitude, double dniExtra, double& ghi, double& dni, double& dhi) {
+  z = std::min(std::max(z, 0.0), 90.0);
+  const double cz = std::cos(rad(z));
+  const double am = 1 / (cz + 0.50572 * std::pow(6.07995 + (90 - z), -1.6364));
+  const double pres = 100 * std::pow((44331.514 - altitude) / 11880.516, 1 / 0.1902632);
+  const double ama = am * pres / 101325;
+  const double fh1 = std::exp(-altitude / 8000), fh2 = std::exp(-altitude / 1250);
+  const double cg1 = 5.09e-5 * altitude + 0.868, cg2 = 3.92e-5 * altitude + 0.0387;
+  ghi = std::max(0.0, cg1 * dniExtra * cz * std::exp(-cg2 * ama * (fh1 + fh2 * (TL - 1))));
+  const double b = 0.664 + 0.163 / fh1;
+  const double bnci = dniExtra * std::max(0.0, b * std::exp(-0.09 * ama * (TL - 1)));
+  const double bnci2 = ghi * std::min(std::max((1 - (0.1 - 0.2 * std::exp(-TL)) / (0.1 + 0.882 / fh1)) / cz, 0.0), 1e20);
+  dni = std::min(bnci, bnci2);
+  dhi = ghi - dni * cz;
+}
+
+bool clear_sky_weather(double lat, double lon, double tz, double turbidity, double altitude, Weather& out, std::string& err) {
+  if (turbidity < 1) { err = "turbidity must be >= 1"; return false; }
+  if (std::fabs(lat) > 90) { err = "latitude out of range"; return false; }
+  out.lat = lat; out.lon = lon; out.tz = tz; out.leap = false; out.missing = 0; out.shiftH = 0;
+  out.dni.assign(8760, 0.f);
+  out.dhi.assign(8760, 0.f);
+  for (int r = 0; r < 8760; ++r) {
+    int month, day;
+    double hour;
+    hoy_to_date(r + 0.5, false, month, day, hour);
+    const dlm::SunPos s = dlm::sun_noaa(lat, lon, tz, month, day, hour, false);
+    if (!s.up) continue;
+    double ghi, dni, dhi;
+    ineichen(90 - s.alt, turbidity, altitude, extraterrestrial_irradiance(r / 24 + 1), ghi, dni, dhi);
+    out.dni[size_t(r)] = float(dni);
+    out.dhi[size_t(r)] = float(dhi);
+  }
+  return true;
+}
+
+bool read_epw(const std::string& path, Weather& w, std::string& err) {
+  std::ifstream f(path);
+  if (!f) { err = "cannot open EPW " + path; return false; }
+  return read_epw(f, path, w, err);
+}
+
+bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string& err) {
+  std::string line;
+  std::vector<std::string> cols;
+  const auto split = [&](const std::string& l) {
+    cols.clear();
+    std::stringstream ss(l);
+    for (std::string c; std::getline(ss, c, ',');) cols.push_back(c);
+  };
+  bool hdrLeap = false, pvgis = false;
+  double declared = NAN;
+  for (int i = 0; i < 8; ++i) {
+    if (!std::getline(f, line)) { err = "EPW header truncated: " + label; return false; }
+    split(line);
+    if (i == 0) {
+      if (cols.size() < 10 || cols[0] != "LOCATION") { err = "EPW: bad LOCATION line: " + label; return false; }
+      w.lat = std::atof(cols[6].c_str()); w.lon = std::atof(cols[7].c_str()); w.tz = std::atof(cols[8].c_str());
+    }
+    if (i == 4 && cols.size() > 1 && cols[1] == "Yes") hdrLeap = true;
+    if (const size_t p = line.find("Irradiance Time Offset"); p != std::string::npos) {
+      pvgis = true;
+      if (const size_t c = line.find(':', p); c != std::string::npos) declared = std::atof(line.c_str() + c + 1);
+    }
+  }
+  w.dni.clear(); w.dhi.clear(); w.missing = 0; w.shiftH = 0;
+  std::vector<float> ghi;
+  while (std::getline(f, line)) {
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    split(line);
+    if (cols.size() < 16) { err = "EPW: short data row " + std::to_string(w.dni.size() + 1) + ": " + label; return false; }
+    double dni = std::atof(cols[14].c_str()), dhi = std::atof(cols[15].c_str());
+    if (dni >= 9999 || dhi >= 9999) { ++w.missing; if (dni >= 9999) dni = 0; if (dhi >= 9999) dhi = 0; }
+    w.dni.push_back(float(dni));
+    w.dhi.push_back(float(dhi));
+    if (pvgis) { const double g = std::atof(cols[13].c_str()); ghi.push_back(g < 9999 ? float(g) : 0.f); }
+  }
+  const size_t n = w.dni.size();
+  w.leap = hdrLeap || n == 8784;
+  if (n != (w.leap ? 8784u : 8760u)) {
+    err = "EPW: " + std::to_string(n) + " data rows, expected " + (w.leap ? "8784" : "8760") + ": " + label;
+    return false;
+  }
+  if (pvgis) pvgis_align(w, ghi, declared);
+  return true;
+}
+
+}  // namespace daylight

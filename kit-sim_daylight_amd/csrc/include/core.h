@@ -1,0 +1,91 @@
+// SPDX-License-Identifier: Apache-2.0
+// Internal contracts of the host layer (the reference's src/internal.h, minus its OptiX seam, which dl_device.h
+// replaces). The host prepares inputs (steps, patches, directions, triangles, sensors) and orchestrates the device;
+// every per-ray, per-sensor or per-(step, patch) loop of the hot path runs on the GPU.
+#pragma once
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>   // float3 / uint3
+
+#include "daylight.h"
+#include "dl_device.h"
+#include "views.h"
+
+namespace daylight {
+
+// ---------------------------------------------------------------- sky (reference sky.cpp:240-294)
+struct Patches {
+  int P = 0;
+  float alpha = 0;
+  std::vector<float> alt, azi, omega;
+  std::vector<int> row, ninrow;
+  std::vector<float3> centre, lb;
+};
+Patches make_patches(int mf);
+float3 patch_sample(const Patches& pa, int p, int k, int K);
+const double* tcos_table();     // [1025] cos(k pi / 2048), Radiance tcos.c's table
+
+// ---------------------------------------------------------------- solar (reference solar.cpp)
+bool make_steps(const Options& opt, const Weather* w, std::vector<Step>& out, std::string& err, double shift = 0);
+bool parse_hoy(const std::string& s, bool leap, double& hoy, std::string& err);
+void hoy_to_date(double hoy, bool leap, int& month, int& day, double& hour);
+double extraterrestrial_irradiance(int doy);
+void ineichen(double apparentZenithDeg, double turbidity, double altitudeM, double dniExtra, double& ghi, double& dni, double& dhi);
+
+// ---------------------------------------------------------------- meshes (reference mesh.cpp)
+struct Tris { std::vector<float3> v; std::vector<uint3> i; };
+struct Sensors { std::vector<float3> p, n; };
+inline std::vector<SimulationMeshView> views_of(const std::vector<SimulationMesh>& meshes) {
+  std::vector<SimulationMeshView> v;
+  v.reserve(meshes.size());
+  for (const SimulationMesh& m : meshes) v.push_back({&m, m.name, m.sensor});
+  return v;
+}
+void par_for(size_t n, const std::function<void(size_t)>& f);
+void bbox_centre(const std::vector<SimulationMeshView>& meshes, double c[3]);
+size_t tri_cap(const SimulationMesh& m);
+size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* v, uint3* tri, unsigned base, int& skipped);
+size_t slot_count(const SimulationMesh& m);
+bool mesh_ok(const SimulationMesh& m, const std::string& label, std::string& err);
+bool optics_ok(const Options& opt, std::string& err);
+bool optics_rows_ok(const std::vector<Optics>& rows, std::string& err);
+// per global slot: does the slot transmit (tau > 0; glass with tau > 0) once resolved by precedence NAME/SLOT > NAME > *
+struct SlotOptics { float rho = 0.2f, tau = 0.f; bool glass = false; };
+bool resolve_optics(const std::vector<SimulationMeshView>& meshes, const std::vector<Optics>& rows, std::vector<SlotOptics>& out, std::string& err);
+bool resolve_no_occlude(const std::vector<SimulationMeshView>& meshes, const std::vector<std::string>& names, std::vector<char>& open, std::string& err);
+bool resolve_targets(const std::vector<SimulationMeshView>& meshes, const std::vector<views::Target>& targets, bool selfOcclude,
+                     const std::vector<char>& open, std::vector<std::string>& groups, std::vector<uint8_t>& cls, std::string& err);
+size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* p, float3* n, std::string& warn);
+
+// ---------------------------------------------------------------- scene residency and runs (reference run.cpp:38-213)
+bool npy_write_header(std::FILE* f, size_t rows, size_t cols, const char* descr);
+struct SceneMesh { std::string name; size_t tri0 = 0, nTri = 0; const SimulationMesh* mesh = nullptr; };
+struct Scene {
+  std::string key;
+  double c[3] = {0, 0, 0};
+  size_t nTris = 0;
+  int skipped = 0;
+  std::vector<SceneMesh> order;
+  dlg_scene* parked = nullptr;
+};
+// The process's GPU (made on first use, never destroyed), or null with err.
+dlg_gpu* process_gpu(std::string& err);
+bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, const std::string& key,
+                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err);
+bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
+                   std::string& err);
+std::string open_key(const std::vector<char>& open, bool selfOcclude);
+void scene_forget(const std::string& key);
+bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
+                std::string& warn, std::string& err);
+double ms_since(const void* clockStart);
+// daylight::run with the --timeseries output kept on the device: per chunk of sensors, sink(i0, nc, T, E) with E the
+// device array float [nc x T] (null when the run has no series columns); an empty sink drops the series.
+using SeriesSink = std::function<bool(size_t i0, size_t nc, size_t T, const float* deviceE, std::string& err)>;
+bool run_series_device(const std::vector<SimulationMeshView>& meshes, const Weather* w, const Options& opt, const SeriesSink& sink, Result& out,
+                       std::string& err);
+
+}  // namespace daylight

@@ -1,0 +1,85 @@
+// SPDX-License-Identifier: Apache-2.0
+// The device state behind dl_device.h, shared by the HIP translation units of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "dl_device.h"
+
+#define DLG_CHECK(call)                                                                                             \
+  do {                                                                                                              \
+    hipError_t e_ = (call);                                                                                         \
+    if (e_ != hipSuccess) { dlg_fail(err, errLen, std::string(#call) + ": " + hipGetErrorString(e_)); return 1; } \
+  } while (0)
+
+inline void dlg_fail(char* err, int32_t errLen, const std::string& msg) {
+  if (err && errLen > 0) {
+    const size_t n = msg.size() < size_t(errLen - 1) ? msg.size() : size_t(errLen - 1);
+    for (size_t k = 0; k < n; ++k) err[k] = msg[k];
+    err[n] = 0;
+  }
+}
+
+// A device buffer that grows (never shrinks until trimmed).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t grow(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// A built occluder scene: LBVH nodes (16 floats each), leaf triangles (12 floats each), its padding and height.
+struct dlg_scene {
+  DevBuf nodes, leaves;
+  int64_t nNodes = 0, nTris = 0;
+  int32_t height = 0;
+  float pad = 0.f;
+  int64_t bytes() const { return int64_t(nodes.cap + leaves.cap); }
+};
+
+struct dlg_gpu {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {};
+  dlg_scene scene;
+  // resident sensors and directions
+  DevBuf P, N, D;
+  int64_t nSens = 0, nDir = 0, nWords = 0;
+  int64_t i0 = 0, nc = 0;          // the last traced chunk
+  DevBuf bits;                     // [nWords x nc]
+  DevBuf scratch, out;             // per-call inputs / outputs of the reduces
+  // sky (dlg_sky): S [P x Tup] and g [Tup] of the last run, the tcos table
+  DevBuf skyS, skyG, skyWork;
+  int32_t skyP = 0, skyT = 0;
+  // series
+  DevBuf seriesE, seriesWork;
+  int64_t seriesN = 0;
+  // views: classes | distances | row sines of the last dlg_view
+  DevBuf view, labels;
+  int64_t viewN = 0, viewK = 0, viewI0 = 0;
+  dlg_view_in viewIn{};
+  dlg_stats_t stats{};
+  int32_t cus = 0;
+  // the traversal's counting variant (dlg_count_fetches) and its two counters
+  int32_t countFetches = 0;
+  DevBuf counts;
+};
+
+// event timing on the library's stream: ms of the work enqueued between begin and end
+inline void dlg_tbegin(dlg_gpu* g) { (void)hipEventRecord(g->ev[0], g->stream); }
+inline float dlg_tend(dlg_gpu* g) {
+  float ms = 0.f;
+  (void)hipEventRecord(g->ev[1], g->stream);
+  (void)hipEventSynchronize(g->ev[1]);
+  (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
+  return ms;
+}

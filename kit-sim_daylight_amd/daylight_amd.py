@@ -1,0 +1,247 @@
+"""ctypes binding of the MI355X library (lib/libdaylight_amd.so): the public C ABI of include/daylight_c.h (the binding the
+reference's bindings/python/daylight/_native.py makes over daylight_c.dll) and the GPU seam of include/dl_device.h.
+
+The library is the product: nothing here computes results; it moves arrays in and out. Loading fails loudly when the
+built library is missing (run kit-sim_daylight_amd/build.py).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdaylight_amd.so")
+
+F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+U8P = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+DL_OK, DL_USAGE, DL_RUNTIME = 0, 1, 2
+DL_DAYLIGHT, DL_VIEWS = 0, 1
+DL_STATS = 10
+
+
+class SkyIn(C.Structure):
+    _fields_ = [("T", C.c_int32), ("month", C.c_void_p), ("day", C.c_void_p), ("jd", C.c_void_p), ("hour", C.c_void_p),
+                ("dt", C.c_void_p), ("dni", C.c_void_p), ("dhi", C.c_void_p), ("lat", C.c_double), ("lon", C.c_double),
+                ("tz", C.c_double), ("north", C.c_double), ("grefl", C.c_double), ("leap", C.c_int32), ("mf", C.c_int32),
+                ("wantRad", C.c_int32), ("wantVis", C.c_int32), ("series", C.c_int32), ("seriesVis", C.c_int32), ("tcos", C.c_void_p)]
+
+
+class SkyOut(C.Structure):
+    _fields_ = [("alt", C.c_void_p), ("azi", C.c_void_p), ("up", C.c_void_p), ("vec", C.c_void_p), ("skyRad", C.c_void_p),
+                ("skyVis", C.c_void_p), ("gndRad", C.c_void_p), ("gndVis", C.c_void_p), ("sunRad", C.c_void_p),
+                ("sunVis", C.c_void_p), ("Tup", C.c_void_p)]
+
+
+class SeriesIn(C.Structure):
+    _fields_ = [("nSky", C.c_uint32), ("row", C.c_void_p), ("wsky", C.c_void_p), ("P", C.c_int32), ("T", C.c_int32),
+                ("G", C.c_void_p), ("sun0", C.c_uint32), ("nSun", C.c_uint32), ("sunStep", C.c_void_p),
+                ("sunW", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("buildMs", "traceMs", "reduceMs", "seriesMs", "skyMs", "viewMs")] + \
+               [(n, C.c_int64) for n in ("rays", "traced", "launches", "sceneBytes", "nodes", "height")] + \
+               [(n, C.c_double) for n in ("lastTraceMs", "lastReduceMs", "lastSeriesMs", "lastSkyMs", "lastViewMs", "lastViewReduceMs")] + \
+               [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays")]
+
+
+class ViewIn(C.Structure):
+    _fields_ = [("nAz", C.c_uint32), ("nEl", C.c_uint32), ("az0", C.c_float), ("dAz", C.c_float), ("rowSin", C.c_void_p),
+                ("world", C.c_int32), ("cull", C.c_int32), ("jitter", C.c_int32), ("back", C.c_int32), ("vground", C.c_int32),
+                ("nS", C.c_uint32), ("groundZ", C.c_float), ("north", C.c_double)]
+
+
+SERIES_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p)
+DLX_COUNTERS = 15
+
+
+# every function include/*.h declares: (name, restype, argtypes)
+C_ABI = [
+    ("dl_version", C.c_int32, []),
+    ("dl_last_error", C.c_char_p, []),
+    ("dl_usage", C.c_char_p, [C.c_int32]),
+    ("dl_mesh_create", C.c_void_p, [C.c_char_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("dl_mesh_load", C.c_void_p, [C.c_char_p]),
+    ("dl_mesh_count", C.c_int32, [C.c_void_p]),
+    ("dl_mesh_name", C.c_char_p, [C.c_void_p, C.c_int32]),
+    ("dl_mesh_free", None, [C.c_void_p]),
+    ("dl_control_create", C.c_void_p, []),
+    ("dl_control_cancel", None, [C.c_void_p]),
+    ("dl_control_progress", C.c_double, [C.c_void_p]),
+    ("dl_control_free", None, [C.c_void_p]),
+    ("dl_run", C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("dl_result_rows", C.c_int64, [C.c_void_p]),
+    ("dl_result_cols", C.c_int64, [C.c_void_p]),
+    ("dl_result_manifest", C.c_char_p, [C.c_void_p]),
+    ("dl_result_values", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("dl_result_values_f32", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("dl_result_stats", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("dl_result_free", None, [C.c_void_p]),
+    ("dlx_run_series_device", C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
+    ("dlx_gpu_counters", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("dlx_count_fetches", None, [C.c_int32]),
+]
+SEAM = [
+    ("dlg_init", C.c_int32, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int32]),
+    ("dlg_destroy", None, [C.c_void_p]),
+    ("dlg_device_info", C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    ("dlg_build", C.c_int32, [C.c_void_p, F32P, C.c_int64, U32P, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_build_bytes", C.c_int64, [C.c_int64, C.c_int64]),
+    ("dlg_trim", None, [C.c_void_p, C.c_int32]),
+    ("dlg_detach", C.c_void_p, [C.c_void_p]),
+    ("dlg_attach", None, [C.c_void_p, C.c_void_p]),
+    ("dlg_free_scene", None, [C.c_void_p]),
+    ("dlg_scene_bytes", C.c_int64, [C.c_void_p]),
+    ("dlg_free_bytes", C.c_int64, []),
+    ("dlg_scene_download", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    ("dlg_set_scene", C.c_int32, [C.c_void_p, F32P, F32P, C.c_int64, F32P, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_ray_flags", C.c_uint32, [C.c_int32, C.c_int32]),
+    ("dlg_trace", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint32, C.c_char_p, C.c_int32]),
+    ("dlg_reduce", C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, F32P, I32P, F64P, C.c_char_p, C.c_int32]),
+    ("dlg_download_bits", C.c_int32, [C.c_void_p, C.c_int64, U32P, C.c_char_p, C.c_int32]),
+    ("dlg_sky", C.c_int32, [C.c_void_p, C.POINTER(SkyIn), C.POINTER(SkyOut), C.c_char_p, C.c_int32]),
+    ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_char_p, C.c_int32]),
+    ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("dlg_series_device", C.c_void_p, [C.c_void_p]),
+    ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
+    ("dlg_count_fetches", None, [C.c_void_p, C.c_int32]),
+    ("dlg_set_labels", C.c_int32, [C.c_void_p, U8P, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_view", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint32, C.POINTER(ViewIn), C.c_char_p, C.c_int32]),
+    ("dlg_reduce_view", C.c_int32, [C.c_void_p, C.c_int64, F64P, C.c_uint32, C.c_double, C.c_void_p, C.c_uint32, C.c_int32, C.c_uint32,
+                                    F64P, C.c_char_p, C.c_int32]),
+    ("dlg_download_view", C.c_int32, [C.c_void_p, C.c_int64, U8P, F32P, C.c_char_p, C.c_int32]),
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libdaylight_amd.so is not built: run python kit-sim_daylight_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in C_ABI + SEAM:
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported(names):
+    """The names of `names` the library exports (no call is made)."""
+    L = C.CDLL(LIB_PATH)
+    return [n for n in names if hasattr(L, n)]
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+def _check(rc, err):
+    if rc:
+        raise GpuError(err.value.decode(errors="replace"))
+
+
+class Gpu:
+    """The seam (include/dl_device.h) over one dlg_gpu: arrays in, arrays out."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = C.c_void_p()
+        self.err = C.create_string_buffer(1024)
+        _check(self.L.dlg_init(C.byref(self.h), self.err, 1024), self.err)
+        self.nDir = 0
+
+    def close(self):
+        if self.h:
+            self.L.dlg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def build(self, V, I):
+        V = np.ascontiguousarray(V, np.float32).reshape(-1)
+        I = np.ascontiguousarray(I, np.uint32).reshape(-1)
+        if V.size == 0:
+            V = np.zeros(3, np.float32)
+        if I.size == 0:
+            _check(self.L.dlg_build(self.h, V, 0, np.zeros(3, np.uint32), 0, self.err, 1024), self.err)
+            return
+        _check(self.L.dlg_build(self.h, V, V.size // 3, I, I.size // 3, self.err, 1024), self.err)
+
+    def scene(self):
+        nn, nt, h = C.c_int64(), C.c_int64(), C.c_int32()
+        self.L.dlg_scene_download(self.h, None, C.byref(nn), None, C.byref(nt), C.byref(h))
+        nodes = np.zeros(max(1, nn.value) * 16, np.float32)
+        leaves = np.zeros(max(1, nt.value) * 12, np.float32)
+        if self.L.dlg_scene_download(self.h, nodes.ctypes.data, C.byref(nn), leaves.ctypes.data, C.byref(nt), C.byref(h)):
+            raise GpuError("scene download failed")
+        return nodes[: nn.value * 16].reshape(-1, 16), leaves[: nt.value * 12].reshape(-1, 12), h.value
+
+    def set_scene(self, P, N, D):
+        P = np.ascontiguousarray(P, np.float32).reshape(-1)
+        N = np.ascontiguousarray(N, np.float32).reshape(-1)
+        D = np.ascontiguousarray(D, np.float32).reshape(-1)
+        self.nDir = D.size // 3
+        self.nSens = P.size // 3
+        _check(self.L.dlg_set_scene(self.h, P, N, P.size // 3, D if D.size else np.zeros(3, np.float32), D.size // 3, self.err, 1024), self.err)
+
+    def trace(self, i0, nc, cull=False):
+        _check(self.L.dlg_trace(self.h, i0, nc, self.L.dlg_ray_flags(int(cull), 0), self.err, 1024), self.err)
+
+    def bits(self, nc):
+        out = np.zeros(max(1, ((self.nDir + 31) // 32) * nc), np.uint32)
+        _check(self.L.dlg_download_bits(self.h, nc, out, self.err, 1024), self.err)
+        return out[: ((self.nDir + 31) // 32) * nc]
+
+    def reduce(self, nc, W, useCos):
+        W = np.ascontiguousarray(W, np.float32).reshape(len(useCos), -1)
+        out = np.zeros(len(useCos) * nc, np.float64)
+        _check(self.L.dlg_reduce(self.h, nc, len(useCos), W.reshape(-1), np.array(useCos, np.int32), out, self.err, 1024), self.err)
+        return out.reshape(len(useCos), nc)
+
+    def sky(self, steps, lat, lon, tz, leap, north, grefl, mf, tcos, wantRad=True, wantVis=False, series=False, seriesVis=False):
+        T = len(steps["month"])
+        arr = {k: np.ascontiguousarray(v) for k, v in steps.items()}
+        keep = [arr, tcos]
+        sin = SkyIn(T, arr["month"].ctypes.data, arr["day"].ctypes.data, arr["jd"].ctypes.data, arr["hour"].ctypes.data,
+                    arr["dt"].ctypes.data, arr["dni"].ctypes.data, arr["dhi"].ctypes.data, lat, lon, tz, north, grefl, int(leap), mf,
+                    int(wantRad), int(wantVis), int(series), int(seriesVis), tcos.ctypes.data)
+        P = {1: 145, 2: 577, 4: 2305}[mf]
+        out = dict(alt=np.zeros(T), azi=np.zeros(T), up=np.zeros(T, np.int32), vec=np.zeros(3 * T, np.float32), skyRad=np.zeros(P),
+                   skyVis=np.zeros(P), gndRad=np.zeros(1), gndVis=np.zeros(1), sunRad=np.zeros(T), sunVis=np.zeros(T),
+                   Tup=np.zeros(1, np.int32))
+        so = SkyOut(*[out[k].ctypes.data for k in ("alt", "azi", "up", "vec", "skyRad", "skyVis", "gndRad", "gndVis", "sunRad", "sunVis", "Tup")])
+        _check(self.L.dlg_sky(self.h, C.byref(sin), C.byref(so), self.err, 1024), self.err)
+        del keep
+        out["vec"] = out["vec"].reshape(T, 3)
+        out["Tup"] = int(out["Tup"][0])
+        if series:
+            S, g = np.zeros(max(1, P * out["Tup"]), np.float32), np.zeros(max(1, out["Tup"]), np.float32)
+            _check(self.L.dlg_sky_series(self.h, S, g, self.err, 1024), self.err)
+            out["S"], out["g"] = S[: P * out["Tup"]].reshape(P, out["Tup"]), g[: out["Tup"]]
+        return out
+
+    def series(self, nc, nSky, row, wsky, P, T, G, sun0, sunStep, sunW, method, download=True):
+        row = np.ascontiguousarray(row, np.int32)
+        wsky = np.ascontiguousarray(wsky, np.float32)
+        G = np.ascontiguousarray(G, np.float32)
+        sunStep = np.ascontiguousarray(sunStep, np.int32)
+        sunW = np.ascontiguousarray(sunW, np.float32)
+        si = SeriesIn(nSky, row.ctypes.data, wsky.ctypes.data, P, T, G.ctypes.data, sun0, len(sunW), sunStep.ctypes.data,
+                      sunW.ctypes.data)
+        E = np.zeros(nc * T, np.float32) if download else None
+        _check(self.L.dlg_series(self.h, nc, C.byref(si), method, E.ctypes.data if download else None, self.err, 1024), self.err)
+        return E.reshape(nc, T) if download else None
+
+    def stats(self):
+        s = Stats()
+        self.L.dlg_stats(self.h, C.byref(s))
+        return {n: getattr(s, n) for n, _ in Stats._fields_}

@@ -1,0 +1,197 @@
+"""The daylight and views tools in-process through libdaylight_amd.so: the interface of the reference's Python binding
+(bindings/python/daylight/native.py: SimulationMesh, Control, Result, run, usage; same names, argument meaning, errors
+and stats keys) over this build's C ABI (include/daylight_c.h)::
+
+    from native import SimulationMesh, run
+    with SimulationMesh.load('tests/golden/box.obj') as box:
+        r = run('daylight', ['--epw', 'site.epw', '--quantity', 'radiation'], box)
+    print(r.manifest['columns'], r.values.shape)
+
+run_series_device() is this build's addition: a daylight run whose --timeseries output stays on the GPU.
+"""
+import ctypes as C
+import json
+import os
+import weakref
+from typing import NamedTuple
+
+import numpy as np
+
+import daylight_amd as _d
+
+__all__ = ["Control", "DaylightError", "SimulationMesh", "Result", "run", "run_series_device", "usage", "gpu_counters",
+           "count_fetches"]
+
+# dl_result_stats order (reference bindings/python/daylight/native.py:24-25)
+_STATS = (("scene_reused", bool), ("prep_ms", float), ("sky_ms", float), ("build_ms", float), ("trace_ms", float),
+          ("reduce_ms", float), ("bounce_ms", float), ("rays", int), ("launches", int), ("gas_bytes", int))
+# dlx_gpu_counters order (include/daylight_c.h)
+_COUNTERS = ("trace_ms", "trace_launches", "rays", "node_fetches", "tri_fetches", "counted_rays", "reduce_ms", "series_ms",
+             "sky_ms", "build_ms", "views_ms", "last_trace_ms", "last_series_ms", "scene_bytes", "tree_height")
+
+
+class DaylightError(Exception):
+    """A failed call: status 1 (usage: flags, names), 2 (runtime: a file, the GPU, 'cancelled'), and the core's message
+    (reference bindings/python/daylight/_native.py:44-55)."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+    def __reduce__(self):
+        return DaylightError, (self.status, str(self))
+
+
+def _check(status):
+    if status != _d.DL_OK:
+        raise DaylightError(status, (_d.lib().dl_last_error() or b"").decode(errors="replace"))
+
+
+def _tool(tool):
+    if tool not in ("daylight", "views"):
+        raise ValueError(f"tool: 'daylight' or 'views', not {tool!r}")
+    return _d.DL_DAYLIGHT if tool == "daylight" else _d.DL_VIEWS
+
+
+class _Handle:
+    def _own(self, handle, free, status):
+        if not handle:
+            _check(status)
+        self._h = handle
+        self._free = weakref.finalize(self, free, handle)
+        self._free.atexit = False
+
+    def close(self):
+        self._free()
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class SimulationMesh(_Handle):
+    """Meshes the core holds (dl_mesh), copied once and never changed; runs over the same objects reuse the GPU scene."""
+
+    def __init__(self, name, points, counts, indices, slots=None):
+        p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        c = np.ascontiguousarray(counts, np.int32).ravel()
+        i = np.ascontiguousarray(indices, np.int32).ravel()
+        s = None if slots is None else np.ascontiguousarray(slots, np.uint16).ravel()
+        if s is not None and len(s) != len(c):
+            raise ValueError(f"{name}: {len(s)} slots for {len(c)} faces")
+        L = _d.lib()
+        self._take(L.dl_mesh_create(name.encode(), p.ctypes.data, len(p), c.ctypes.data, len(c), i.ctypes.data, len(i),
+                                    None if s is None else s.ctypes.data), _d.DL_USAGE)
+
+    @classmethod
+    def load(cls, path):
+        m = cls.__new__(cls)
+        m._take(_d.lib().dl_mesh_load(os.fsencode(path)), _d.DL_RUNTIME)
+        return m
+
+    def _take(self, handle, status):
+        L = _d.lib()
+        self._own(handle, L.dl_mesh_free, status)
+        self.names = tuple(L.dl_mesh_name(handle, k).decode() for k in range(L.dl_mesh_count(handle)))
+
+
+class Control(_Handle):
+    """Cancel flag and progress of one run (dl_control)."""
+
+    def __init__(self):
+        self._own(_d.lib().dl_control_create(), _d.lib().dl_control_free, _d.DL_RUNTIME)
+
+    def cancel(self):
+        _d.lib().dl_control_cancel(self._h)
+
+    @property
+    def progress(self):
+        return _d.lib().dl_control_progress(self._h)
+
+
+class Result(NamedTuple):
+    values: np.ndarray
+    manifest: dict
+    stats: dict
+
+
+def usage(tool):
+    return _d.lib().dl_usage(_tool(tool)).decode()
+
+
+def _meshes(ms):
+    ms = [ms] if isinstance(ms, SimulationMesh) else list(ms)
+    if not all(isinstance(m, SimulationMesh) for m in ms):
+        raise TypeError("sensors / context: a SimulationMesh or SimulationMesh objects")
+    return ms
+
+
+def _handles(ms):
+    return (C.c_void_p * len(ms))(*[m._h for m in ms]) if ms else None
+
+
+def _argv(args):
+    argv = [os.fsencode(a) if isinstance(a, os.PathLike) else str(a).encode() for a in args]
+    return len(argv), (C.c_char_p * max(1, len(argv)))(*argv)
+
+
+def _take_result(out):
+    L = _d.lib()
+    try:
+        values = np.empty((L.dl_result_rows(out), L.dl_result_cols(out)))
+        st = np.empty(_d.DL_STATS)
+        _check(L.dl_result_values(out, values.ctypes.data, values.size))
+        _check(L.dl_result_stats(out, st.ctypes.data))
+        manifest = L.dl_result_manifest(out).decode()
+        return Result(values, json.loads(manifest), {k: f(x) for (k, f), x in zip(_STATS, st.tolist())})
+    finally:
+        L.dl_result_free(out)
+
+
+def run(tool, args, sensors, context=(), control=None):
+    """One run of `tool` ('daylight' or 'views') with its CLI flags `args`, sensor and context meshes as SimulationMesh
+    objects -> Result(values [N x C] float64, manifest dict, stats dict). DaylightError on failure."""
+    t = _tool(tool)
+    n, argv = _argv(args)
+    sens, ctx = _meshes(sensors), _meshes(context)
+    out = C.c_void_p()
+    _check(_d.lib().dl_run(t, n, argv, _handles(sens), len(sens), _handles(ctx), len(ctx), control._h if control else None,
+                           C.byref(out)))
+    return _take_result(out)
+
+
+def run_series_device(args, sensors, context=(), sink=None, control=None):
+    """A daylight run with --timeseries kept on the GPU. sink(row0, rows, cols, device_ptr) is called per chunk of
+    sensors (device_ptr: float32 [rows x cols] in device memory, valid during the call; 0 when there are no columns) and
+    returns None / 0 to go on; sink None drops the series. -> Result as run()."""
+    n, argv = _argv(args)
+    sens, ctx = _meshes(sensors), _meshes(context)
+    out = C.c_void_p()
+    cb = None
+    if sink is not None:
+        def _cb(user, row0, rows, cols, ptr):
+            try:
+                rc = sink(row0, rows, cols, ptr or 0)
+                return int(rc or 0)
+            except Exception:   # an exception cannot cross the C callback: stop the run
+                return 2
+        cb = _d.SERIES_SINK(_cb)
+    _check(_d.lib().dlx_run_series_device(n, argv, _handles(sens), len(sens), _handles(ctx), len(ctx),
+                                          control._h if control else None, C.cast(cb, C.c_void_p) if cb else None, None,
+                                          C.byref(out)))
+    return _take_result(out)
+
+
+def gpu_counters():
+    """The process GPU's counters (dlx_gpu_counters) as a dict."""
+    a = np.zeros(_d.DLX_COUNTERS)
+    _check(_d.lib().dlx_gpu_counters(a.ctypes.data, len(a)))
+    return dict(zip(_COUNTERS, a.tolist()))
+
+
+def count_fetches(on):
+    """Switch the traversal's counting variant on or off (dlx_count_fetches)."""
+    _d.lib().dlx_count_fetches(1 if on else 0)

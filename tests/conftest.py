@@ -6,6 +6,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "kit-sim_daylight_amd"))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
