@@ -165,6 +165,22 @@ def cpu_baseline(ctx, sens, dni, dhi, target_s=15.0):
                       "weather; oracle trace + FP64 sums + FP64 series, %d threads, %.1f s" % (nf, hours, threads, t)}
 
 
+# ---------------------------------------------------------------- the HBM traffic of the traversal (rocprofv3 --pmc)
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bits_urban_block_traffic.txt")
+
+
+def traffic_per_launch():
+    """HBM bytes per traversal launch as the committed counter summary states them (FETCH_SIZE + WRITE_SIZE of the
+    traversal, scaled by the counters' reading of a copy of known size), or None where there is none."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 # ---------------------------------------------------------------- main
 
 def main():
@@ -174,6 +190,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-methods", action="store_true", help="skip the side-by-side runs of the contraction methods")
+    ap.add_argument("--no-count", action="store_true", help="skip the traversal's counted run (roofline achieved = null)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path shards by sensor rows, "
@@ -181,9 +201,17 @@ def main():
         sys.exit(2)
 
     import torch
-    import native
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
+    if a.calibrate_copy:
+        src = torch.ones(a.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.calibrate_copy << 20, "copies": 5}), flush=True)
+        return
+    import native
 
     boxes, central = city()
     ctx = box_mesh(boxes)
@@ -231,22 +259,24 @@ def main():
           file=sys.stderr, flush=True)
 
     # the traversal's algorithmic fetches: one counted run (untimed)
-    native.count_fetches(True)
-    k0 = native.gpu_counters()
-    step()
-    k1 = native.gpu_counters()
-    native.count_fetches(False)
     launches = d["trace_launches"] / a.steps
     per_launch_ms = d["trace_ms"] / d["trace_launches"]
-    fetch_bytes = (k1["node_fetches"] - k0["node_fetches"]) * NODE_BYTES + (k1["tri_fetches"] - k0["tri_fetches"]) * TRI_BYTES
-    n_words = math.ceil((k1["counted_rays"] - k0["counted_rays"]) / n_faces / 32)
-    io_bytes = n_faces * n_words * (4 + 24)             # a visibility word written, the sensor's point + normal read per word
-    alg_bytes_launch = (fetch_bytes + io_bytes) / launches
-    achieved = alg_bytes_launch / (per_launch_ms * 1e-3) / 1e9
+    alg_bytes_launch = achieved = None
+    if not a.no_count:
+        native.count_fetches(True)
+        k0 = native.gpu_counters()
+        step()
+        k1 = native.gpu_counters()
+        native.count_fetches(False)
+        fetch_bytes = (k1["node_fetches"] - k0["node_fetches"]) * NODE_BYTES + (k1["tri_fetches"] - k0["tri_fetches"]) * TRI_BYTES
+        n_words = math.ceil((k1["counted_rays"] - k0["counted_rays"]) / n_faces / 32)
+        io_bytes = n_faces * n_words * (4 + 24)         # a visibility word written, the sensor's point + normal read per word
+        alg_bytes_launch = (fetch_bytes + io_bytes) / launches
+        achieved = alg_bytes_launch / (per_launch_ms * 1e-3) / 1e9
 
     # the contraction methods side by side (two runs each)
     methods = {}
-    for name in ("fp32", "bf16x3", "bf16x6"):
+    for name in () if a.no_methods else ("fp32", "bf16x3", "bf16x6"):
         os.environ["DAYLIGHT_SERIES"] = name
         step()
         m0 = native.gpu_counters()
@@ -259,7 +289,8 @@ def main():
         methods[name] = {"ms_per_step": round(1e3 * tm, 2), "series_ms_per_step": round((m1["series_ms"] - m0["series_ms"]) / 2, 3),
                          "sensor_hours_per_s": round(sensor_hours / tm, 1)}
     os.environ.pop("DAYLIGHT_SERIES", None)
-    print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
+    if methods:
+        print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
 
     cpu = None if a.no_cpu else cpu_baseline(ctx, sens, dni, dhi, a.cpu_seconds)
     lat_ms = np.array(lat) * 1e3
@@ -277,9 +308,11 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "contraction_methods": methods,
-        "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "launch_ms": round(per_launch_ms, 4), "algorithmic_bytes_per_launch": round(alg_bytes_launch)},
+        "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm",
+                     "achieved": None if achieved is None else round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
+                     "launch_ms": round(per_launch_ms, 4),
+                     "algorithmic_bytes_per_launch": None if alg_bytes_launch is None else round(alg_bytes_launch)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

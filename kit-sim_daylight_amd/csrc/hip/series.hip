@@ -37,10 +37,14 @@ __device__ __forceinline__ float coeff(const uint32_t* __restrict__ bits, uint32
 }
 
 // ---------------------------------------------------------------------------------------------- method 0: FP32
+// Each E[i, t]: the fma chain over the sky directions in ascending order, then fma(G_i, g_t, .), then the sun of step t
+// (its one direction, when visible): per element the reference kernel's order of operations (gpu.cu:128-148).
 constexpr int kTile = 64, kKt = 16;
 __global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__ bits, uint32_t nc, const float3* __restrict__ N,
                                                     const float3* __restrict__ D, uint32_t nSky, const int* __restrict__ row,
-                                                    const float* __restrict__ wsky, const float* __restrict__ S, int T, float* __restrict__ E) {
+                                                    const float* __restrict__ wsky, const float* __restrict__ S, int T,
+                                                    const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
+                                                    const int* __restrict__ sunOfCol, const float* __restrict__ sunW, float* __restrict__ E) {
   __shared__ float As[kKt][kTile];
   __shared__ float Bs[kKt][kTile + 4];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -67,26 +71,15 @@ __global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__
   for (int r = 0; r < 4; ++r) {
     const uint32_t i = i0 + ty * 4 + r;
     if (i >= nc) continue;
+    const float3 n = N[i];
     for (int c = 0; c < 4; ++c) {
       const int t = int(t0) + tx * 4 + c;
-      if (t < T) E[size_t(i) * T + t] = e[r][c];
+      if (t >= T) continue;
+      float v = fmaf(G[i], g[t], e[r][c]);
+      const int j = sunOfCol[t];
+      if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) v = fmaf(fmaxf(dlm::dot3(n, D[sun0 + uint32_t(j)]), 0.f) * 1.f, sunW[j], v);
+      E[size_t(i) * T + t] = v;
     }
-  }
-}
-
-// E += G g (fma), then the sun's term at its step (fma), per the reference's order
-__global__ void k_series_f32_tail(const uint32_t* __restrict__ bits, uint32_t nc, const float3* __restrict__ N, const float3* __restrict__ D,
-                                  const float* __restrict__ G, const float* __restrict__ g, int T, uint32_t sun0, uint32_t nSun,
-                                  const int* __restrict__ sunStep, const float* __restrict__ sunW, float* __restrict__ E) {
-  const uint32_t i = blockIdx.x;
-  if (i >= nc) return;
-  float* Ei = E + size_t(i) * T;
-  for (int t = threadIdx.x; t < T; t += blockDim.x) Ei[t] = fmaf(G[i], g[t], Ei[t]);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const float3 n = N[i];
-    for (uint32_t j = 0; j < nSun; ++j)
-      if (vis_bit(bits, nc, i, sun0 + j)) Ei[sunStep[j]] = fmaf(fmaxf(dlm::dot3(n, D[sun0 + j]), 0.f) * 1.f, sunW[j], Ei[sunStep[j]]);
   }
 }
 
@@ -125,61 +118,106 @@ __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ r
   for (int k = 0; k < NS; ++k) B[k * plane + size_t(t) * Kp + d] = p[k];
 }
 
-// Block: 4 waves, 64 sensors x 64 steps; wave w: rows 32 (w & 1), columns 32 (w >> 1). v_mfma_f32_32x32x8 bf16 operand
-// layout: lane l holds A[l % 32][4 (l / 32) + v] and B[4 (l / 32) + v][l % 32], v = 0..3; result acc[v] is
-// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
-template <int NS>
-__global__ __launch_bounds__(256) void k_series_mfma(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, uint32_t nc, uint32_t Kp,
-                                                     uint32_t Tp, int T, const uint32_t* __restrict__ bits, const float3* __restrict__ N,
-                                                     const float3* __restrict__ D, const float* __restrict__ G, const float* __restrict__ g,
-                                                     uint32_t sun0, const int* __restrict__ sunOfCol, const float* __restrict__ sunW,
-                                                     float* __restrict__ E) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t i0 = blockIdx.y * 64 + 32 * (w & 1), t0 = blockIdx.x * 64 + 32 * (w >> 1);
-  const uint32_t ia = i0 + (lane & 31), tb = t0 + (lane & 31), kq = 4 * (lane >> 5);
-  const bool aOk = ia < nc;
+// v_mfma_f32_32x32x8 bf16 operand layout: lane l holds A[l % 32][4 (l / 32) + v] and B[4 (l / 32) + v][l % 32], v = 0..3;
+// the result acc[v] is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+//
+// Block: WM x WN waves; wave (wm, wn) owns 32 sensors x 64 steps (two 32 x 32 MFMA tiles side by side, sharing the A
+// fragment). Per 32-direction slice the block stages A [NS][BM][32] and S^T [NS][BN][32] in LDS (rows padded to 36
+// bf16 so that the lanes' 8-byte reads spread over the banks), runs the products on MFMA into FP32 and adds the
+// slice's sums into FP64 accumulators. Every 32 x 32 output tile sees the same MFMA sequence as a one-tile wave would.
+// Blocks are numbered XCD-major: the blocks an XCD runs (hardware id % 8) take consecutive tiles, so the tiles that
+// share sensor rows share that XCD's L2.
+constexpr int kBK = 32, kLds = 36;
+template <int NS, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_series_mfma(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, uint32_t nc,
+                                                              uint32_t Kp, uint32_t Tp, int T, const uint32_t* __restrict__ bits,
+                                                              const float3* __restrict__ N, const float3* __restrict__ D,
+                                                              const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
+                                                              const int* __restrict__ sunOfCol, const float* __restrict__ sunW,
+                                                              float* __restrict__ E, uint32_t tilesT, uint32_t tiles) {
+  constexpr int BM = 32 * WM, BN = 64 * WN, NT = 64 * WM * WN;
+  __shared__ uint16_t As[NS][BM][kLds];
+  __shared__ uint16_t Bs[NS][BN][kLds];
+  // XCD-major tile order
+  const uint32_t id = blockIdx.x, per = (tiles + 7) / 8, xcd = id % 8, rank = id / 8;
+  const uint32_t full = tiles % 8 == 0 ? 8 : tiles % 8;   // XCDs that take `per` tiles; the rest take per - 1
+  const uint32_t tile = xcd < full ? xcd * per + rank : full * per + (xcd - full) * (per - 1) + rank;
+  if (tile >= tiles || (xcd >= full && rank >= per - 1)) return;
+  const uint32_t ti = tile / tilesT, tt = tile % tilesT;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w % WM, wn = w / WM;
+  const uint32_t iBlk = ti * BM, tBlk = tt * BN;
   const size_t aPlane = size_t(nc) * Kp, bPlane = size_t(Tp) * Kp;
-  const uint16_t* Ar = A + size_t(aOk ? ia : 0) * Kp + kq;
-  const uint16_t* Br = B + size_t(tb) * Kp + kq;
-  double acc64[16];
+  const int kq = 4 * (lane >> 5), r32 = lane & 31;
+  double acc64[2][16];
 #pragma unroll
-  for (int v = 0; v < 16; ++v) acc64[v] = 0.0;
-  for (uint32_t k0 = 0; k0 < Kp; k0 += 32) {
-    f32x16 acc = {};
+  for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (uint32_t kk = 0; kk < 32; kk += 8) {
-      bf16x4 a[NS], b[NS];
+    for (int v = 0; v < 16; ++v) acc64[c][v] = 0.0;
+  for (uint32_t k0 = 0; k0 < Kp; k0 += kBK) {
+    // stage the slice: rows of 32 bf16 = 4 x 16 bytes, written to LDS as 8-byte halves
+    for (int q = threadIdx.x; q < NS * (BM + BN) * 4; q += NT) {
+      const int s = q / ((BM + BN) * 4), rem = q % ((BM + BN) * 4), r = rem / 4, c = rem % 4;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      uint16_t* dst;
+      if (r < BM) {
+        const uint32_t i = iBlk + r;
+        if (i < nc) v = *reinterpret_cast<const uint4*>(A + s * aPlane + size_t(i) * Kp + k0 + 8 * c);
+        dst = &As[s][r][8 * c];
+      } else {
+        const uint32_t t = tBlk + (r - BM);
+        v = *reinterpret_cast<const uint4*>(B + s * bPlane + size_t(t) * Kp + k0 + 8 * c);
+        dst = &Bs[s][r - BM][8 * c];
+      }
+      reinterpret_cast<uint2*>(dst)[0] = make_uint2(v.x, v.y);
+      reinterpret_cast<uint2*>(dst)[1] = make_uint2(v.z, v.w);
+    }
+    __syncthreads();
+    f32x16 acc[2] = {f32x16{}, f32x16{}};
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 8) {
+      bf16x4 a[NS], b[2][NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        a[s] = aOk ? *reinterpret_cast<const bf16x4*>(Ar + s * aPlane + k0 + kk) : bf16x4{0, 0, 0, 0};
-        b[s] = *reinterpret_cast<const bf16x4*>(Br + s * bPlane + k0 + kk);
+        a[s] = *reinterpret_cast<const bf16x4*>(&As[s][32 * wm + r32][kk + kq]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) b[c][s] = *reinterpret_cast<const bf16x4*>(&Bs[s][64 * wn + 32 * c + r32][kk + kq]);
       }
       // products whose exponent sum is at most NS - 1 pieces below the leading one, smallest first
 #pragma unroll
       for (int o = NS - 1; o >= 0; --o)
 #pragma unroll
-        for (int s = 0; s <= o; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[s], b[o - s], acc, 0, 0, 0);
+        for (int s = 0; s <= o; ++s)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[s], b[c][o - s], acc[c], 0, 0, 0);
     }
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc64[v] += (double)acc[v];
-  }
-  const uint32_t t = t0 + (lane & 31);
-  if (int(t) >= T) return;
-  const int j = sunOfCol[t];
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const uint32_t i = i0 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
-    if (i >= nc) continue;
-    double e = acc64[v] + (double)G[i] * (double)g[t];
-    if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) e += (double)fmaxf(dlm::dot3(N[i], D[sun0 + uint32_t(j)]), 0.f) * (double)sunW[j];
-    E[size_t(i) * T + t] = (float)e;
+      for (int v = 0; v < 16; ++v) acc64[c][v] += (double)acc[c][v];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t t = tBlk + 64 * wn + 32 * c + r32;
+    if (int(t) >= T) continue;
+    const int j = sunOfCol[t];
+    const float gt = g[t];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const uint32_t i = iBlk + 32 * wm + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+      if (i >= nc) continue;
+      double e = acc64[c][v] + (double)G[i] * (double)gt;
+      if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) e += (double)fmaxf(dlm::dot3(N[i], D[sun0 + uint32_t(j)]), 0.f) * (double)sunW[j];
+      E[size_t(i) * T + t] = (float)e;
+    }
   }
 }
 
 template <int NS>
 int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* dG, const float* dg, const int* dRow, const float* dWsky,
                  const int* dSunOfCol, const float* dSunW, char* err, int32_t errLen) {
-  const uint32_t Kp = (in->nSky + 31) / 32 * 32, T = uint32_t(in->T), Tp = (T + 63) / 64 * 64;
+  constexpr int WM = 4, WN = 1, BM = 32 * WM, BN = 64 * WN;
+  const uint32_t Kp = (in->nSky + kBK - 1) / kBK * kBK, T = uint32_t(in->T), Tp = (T + BN - 1) / BN * BN;
   const size_t aBytes = size_t(NS) * nc * Kp * 2, bBytes = size_t(NS) * Tp * Kp * 2;
   DLG_CHECK(g->seriesWork.grow(aBytes + bBytes));
   uint16_t* A = g->seriesWork.as<uint16_t>();
@@ -187,8 +225,10 @@ int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* 
   const float3* N = g->N.as<float3>() + g->i0;
   k_split_a<NS><<<dim3((Kp + 255) / 256, nc), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, N, g->D.as<float3>(), in->nSky, Kp, dWsky, A);
   k_split_s<NS><<<dim3((Kp + 255) / 256, Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, in->nSky, Kp, in->T, Tp, B);
-  k_series_mfma<NS><<<dim3(Tp / 64, (nc + 63) / 64), 256, 0, g->stream>>>(A, B, nc, Kp, Tp, in->T, g->bits.as<uint32_t>(), N, g->D.as<float3>(),
-                                                                          dG, dg, in->sun0, dSunOfCol, dSunW, g->seriesE.as<float>());
+  const uint32_t tilesT = Tp / BN, tiles = tilesT * ((nc + BM - 1) / BM);
+  k_series_mfma<NS, WM, WN><<<(tiles + 7) / 8 * 8, 64 * WM * WN, 0, g->stream>>>(A, B, nc, Kp, Tp, in->T, g->bits.as<uint32_t>(), N,
+                                                                               g->D.as<float3>(), dG, dg, in->sun0, dSunOfCol, dSunW,
+                                                                               g->seriesE.as<float>(), tilesT, tiles);
   DLG_CHECK(hipGetLastError());
   return 0;
 }
@@ -204,7 +244,7 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
   size_t off = 0;
   const auto take = [&](size_t bytes) { const size_t at = off; off += (bytes + 255) & ~size_t(255); return at; };
   const size_t oRow = take(4 * size_t(in->nSky)), oW = take(4 * size_t(in->nSky)), oG = take(4 * size_t(nc)), oCol = take(4 * size_t(T)),
-               oSw = take(4 * size_t(in->nSun ? in->nSun : 1)), oSs = take(4 * size_t(in->nSun ? in->nSun : 1));
+               oSw = take(4 * size_t(in->nSun ? in->nSun : 1));
   DLG_CHECK(g->scratch.grow(off));
   char* b = g->scratch.as<char>();
   std::vector<int> sunOfCol(static_cast<size_t>(T), -1);
@@ -216,7 +256,6 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
   DLG_CHECK(hipMemcpyAsync(b + oCol, sunOfCol.data(), 4 * size_t(T), hipMemcpyHostToDevice, g->stream));
   if (in->nSun) {
     DLG_CHECK(hipMemcpyAsync(b + oSw, in->sunW, 4 * size_t(in->nSun), hipMemcpyHostToDevice, g->stream));
-    DLG_CHECK(hipMemcpyAsync(b + oSs, in->sunStep, 4 * size_t(in->nSun), hipMemcpyHostToDevice, g->stream));
   }
   DLG_CHECK(g->seriesE.grow(size_t(nc) * size_t(T) * 4));
   g->seriesN = nc;
@@ -226,11 +265,8 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
     const float3* N = g->N.as<float3>() + g->i0;
     k_series_f32<<<dim3((T + kTile - 1) / kTile, unsigned((nc + kTile - 1) / kTile)), 256, 0, g->stream>>>(
         g->bits.as<uint32_t>(), uint32_t(nc), N, g->D.as<float3>(), in->nSky, reinterpret_cast<const int*>(b + oRow),
-        reinterpret_cast<const float*>(b + oW), g->skyS.as<float>(), T, g->seriesE.as<float>());
-    k_series_f32_tail<<<unsigned(nc), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(nc), N, g->D.as<float3>(),
-                                                           reinterpret_cast<const float*>(b + oG), dg, T, in->sun0, in->nSun,
-                                                           reinterpret_cast<const int*>(b + oSs), reinterpret_cast<const float*>(b + oSw),
-                                                           g->seriesE.as<float>());
+        reinterpret_cast<const float*>(b + oW), g->skyS.as<float>(), T, reinterpret_cast<const float*>(b + oG), dg, in->sun0,
+        reinterpret_cast<const int*>(b + oCol), reinterpret_cast<const float*>(b + oSw), g->seriesE.as<float>());
     DLG_CHECK(hipGetLastError());
   } else if (method == 1 || method == 2) {
     const int32_t rc = (method == 1 ? run_mfma<2> : run_mfma<3>)(g, uint32_t(nc), in, reinterpret_cast<const float*>(b + oG), dg,

@@ -12,6 +12,8 @@
 // float rounding of the outputs except in rare last-bit cases.
 #include <vector>
 
+#include <algorithm>
+
 #include "dl_math.h"
 #include "gpu_internal.h"
 
@@ -107,7 +109,7 @@ __global__ void k_norm(int T, int P, StepOut* __restrict__ so, const float* __re
   o.sfVis = (o.q.difVis / norm) * (1. / 179.);
 }
 
-__global__ void k_apply(int T, int P, const StepOut* __restrict__ so, const float* __restrict__ L, int wantRad, int wantVis,
+__global__ void k_apply(int T, int P, const StepOut* __restrict__ so, const float* __restrict__ L, const double* __restrict__ dt, int wantRad, int wantVis,
                         const int* __restrict__ colOf, int Tup, int seriesMode, double* __restrict__ LpRad, double* __restrict__ LpVis,
                         float* __restrict__ S) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
@@ -120,42 +122,66 @@ __global__ void k_apply(int T, int P, const StepOut* __restrict__ so, const floa
     if (wantRad || seriesMode == 1) r = sky_bright(l, o.sfRad);
     if (wantVis || seriesMode == 2) v = 179.0 * sky_bright(l, o.sfVis);
   }
-  if (wantRad) LpRad[at] = r;
-  if (wantVis) LpVis[at] = v;
+  // the step's contribution to the cumulative weights (run.cpp:436-446), 0 at night
+  if (wantRad) LpRad[at] = r * dt[t] / 1000.0;
+  if (wantVis) LpVis[at] = v * dt[t];
   const int col = colOf[t];
   if (seriesMode && col >= 0) S[size_t(p) * Tup + col] = (float)(seriesMode == 1 ? r : v);
 }
 
-__global__ void k_accum(int T, int P, const StepOut* __restrict__ so, const double* __restrict__ dt, const double* __restrict__ LpRad,
-                        const double* __restrict__ LpVis, int wantRad, int wantVis, double* __restrict__ skyRad, double* __restrict__ skyVis) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  double r = 0.0, v = 0.0;
-  for (int t = 0; t < T; ++t) {
-    if (so[t].q.night) continue;
-    const size_t at = size_t(t) * P + p;
-    if (wantRad) r += LpRad[at] * dt[t] / 1000.0;
-    if (wantVis) v += LpVis[at] * dt[t];
+// skyRad[p] = sum over the steps, in step order, of the contributions c[t P + p] (a step at night adds 0.0, which
+// leaves the sum's bits unchanged): 64 patches a block, tiles of 64 steps staged through LDS so that the sequential sum
+// of each patch reads LDS, not HBM.
+constexpr int kAccP = 64, kAccT = 64;
+__global__ __launch_bounds__(256) void k_accum(int T, int P, const double* __restrict__ c, double* __restrict__ sum) {
+  __shared__ double tile[kAccT][kAccP + 1];
+  const int lp = threadIdx.x % kAccP, part = threadIdx.x / kAccP, p = blockIdx.x * kAccP + lp;
+  double r = 0.0;
+  for (int t0 = 0; t0 < T; t0 += kAccT) {
+    for (int k = part; k < kAccT; k += 256 / kAccP) {
+      const int t = t0 + k;
+      tile[k][lp] = (t < T && p < P) ? c[size_t(t) * P + p] : 0.0;
+    }
+    __syncthreads();
+    if (part == 0) {
+      const int n = T - t0 < kAccT ? T - t0 : kAccT;
+      for (int k = 0; k < n; ++k) r += tile[k][lp];
+    }
+    __syncthreads();
   }
-  skyRad[p] = r;
-  skyVis[p] = v;
+  if (part == 0 && p < P) sum[p] = r;
 }
 
-__global__ void k_ground(int T, const StepOut* __restrict__ so, const double* __restrict__ dt, const int* __restrict__ colOf, int seriesMode,
-                         double* __restrict__ gnd, float* __restrict__ gS, double* __restrict__ sunRad, double* __restrict__ sunVis) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// per step: the sun's weights, the ground's series value and the ground's contributions to the cumulative weights
+__global__ void k_ground_steps(int T, const StepOut* __restrict__ so, const double* __restrict__ dt, const int* __restrict__ colOf,
+                               int seriesMode, double* __restrict__ cR, double* __restrict__ cV, float* __restrict__ gS,
+                               double* __restrict__ sunRad, double* __restrict__ sunVis) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const StepOut& o = so[t];
+  sunRad[t] = o.sunRad;
+  sunVis[t] = o.sunVis;
+  cR[t] = o.q.night ? 0.0 : o.gRadB * dt[t] / 1000.0;
+  cV[t] = o.q.night ? 0.0 : o.gVisB * dt[t];
+  if (seriesMode && !o.q.night && colOf[t] >= 0) gS[colOf[t]] = (float)(seriesMode == 1 ? o.gRadB : o.gVisB);
+}
+
+// gnd[0] / gnd[1]: the ground contributions summed in step order (one block; tiles of 256 steps through LDS)
+__global__ __launch_bounds__(256) void k_ground_sum(int T, const double* __restrict__ cR, const double* __restrict__ cV, double* __restrict__ gnd) {
+  __shared__ double a[256], b[256];
   double r = 0.0, v = 0.0;
-  for (int t = 0; t < T; ++t) {
-    const StepOut& o = so[t];
-    sunRad[t] = o.sunRad;
-    sunVis[t] = o.sunVis;
-    if (o.q.night) continue;
-    r += o.gRadB * dt[t] / 1000.0;
-    v += o.gVisB * dt[t];
-    if (seriesMode && colOf[t] >= 0) gS[colOf[t]] = (float)(seriesMode == 1 ? o.gRadB : o.gVisB);
+  for (int t0 = 0; t0 < T; t0 += 256) {
+    const int t = t0 + int(threadIdx.x);
+    a[threadIdx.x] = t < T ? cR[t] : 0.0;
+    b[threadIdx.x] = t < T ? cV[t] : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int n = T - t0 < 256 ? T - t0 : 256;
+      for (int k = 0; k < n; ++k) { r += a[k]; v += b[k]; }
+    }
+    __syncthreads();
   }
-  gnd[0] = r;
-  gnd[1] = v;
+  if (threadIdx.x == 0) { gnd[0] = r; gnd[1] = v; }
 }
 
 inline unsigned blocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
@@ -197,7 +223,8 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
                oDhi = take(4 * T), oPalt = take(4 * P), oPazi = take(4 * P), oOm = take(4 * P), oTs = take(8 * P), oCol = take(4 * T),
                oSo = take(sizeof(StepOut) * T), oL = take(size_t(4) * T * P), oLr = take(in->wantRad ? size_t(8) * T * P : 8),
                oLv = take(in->wantVis ? size_t(8) * T * P : 8), oAlt = take(8 * T), oAzi = take(8 * T), oUp = take(4 * T),
-               oVec = take(12 * T), oSkyR = take(8 * P), oSkyV = take(8 * P), oGnd = take(16), oSunR = take(8 * T), oSunV = take(8 * T);
+               oVec = take(12 * T), oSkyR = take(8 * P), oSkyV = take(8 * P), oGnd = take(16), oSunR = take(8 * T), oSunV = take(8 * T),
+               oGr = take(8 * T), oGv = take(8 * T);
   DLG_CHECK(g->skyWork.grow(off));
   char* b = g->skyWork.as<char>();
   const auto up = [&](size_t at, const void* src, size_t bytes) { return hipMemcpyAsync(b + at, src, bytes, hipMemcpyHostToDevice, g->stream); };
@@ -220,13 +247,15 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
   if (T > 0) {
     k_lum<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oPalt), PTR(float, oPazi), PTR(float, oL));
     k_norm<<<blocks(T, 64), 64, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oTs), PTR(float, oOm));
-    k_apply<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), in->wantRad, in->wantVis,
-                                                                      PTR(int, oCol), Tup, seriesMode, PTR(double, oLr), PTR(double, oLv),
-                                                                      seriesMode ? g->skyS.as<float>() : nullptr);
-    k_accum<<<blocks(P, 64), 64, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(double, oDt), PTR(double, oLr), PTR(double, oLv), in->wantRad,
-                                                 in->wantVis, PTR(double, oSkyR), PTR(double, oSkyV));
-    k_ground<<<1, 64, 0, g->stream>>>(T, PTR(StepOut, oSo), PTR(double, oDt), PTR(int, oCol), seriesMode, PTR(double, oGnd),
-                                      seriesMode ? g->skyG.as<float>() : nullptr, PTR(double, oSunR), PTR(double, oSunV));
+    k_apply<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oDt), in->wantRad,
+                                                                      in->wantVis, PTR(int, oCol), Tup, seriesMode, PTR(double, oLr),
+                                                                      PTR(double, oLv), seriesMode ? g->skyS.as<float>() : nullptr);
+    if (in->wantRad) k_accum<<<blocks(P, kAccP), 256, 0, g->stream>>>(T, P, PTR(double, oLr), PTR(double, oSkyR));
+    if (in->wantVis) k_accum<<<blocks(P, kAccP), 256, 0, g->stream>>>(T, P, PTR(double, oLv), PTR(double, oSkyV));
+    k_ground_steps<<<blocks(T, 256), 256, 0, g->stream>>>(T, PTR(StepOut, oSo), PTR(double, oDt), PTR(int, oCol), seriesMode, PTR(double, oGr),
+                                                          PTR(double, oGv), seriesMode ? g->skyG.as<float>() : nullptr, PTR(double, oSunR),
+                                                          PTR(double, oSunV));
+    k_ground_sum<<<1, 256, 0, g->stream>>>(T, PTR(double, oGr), PTR(double, oGv), PTR(double, oGnd));
   }
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
@@ -235,8 +264,14 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
   const auto down = [&](void* dst, size_t at, size_t bytes) { return hipMemcpyAsync(dst, b + at, bytes, hipMemcpyDeviceToHost, g->stream); };
   DLG_CHECK(down(out->alt, oAlt, 8 * T)); DLG_CHECK(down(out->azi, oAzi, 8 * T)); DLG_CHECK(down(out->up, oUp, 4 * T));
   DLG_CHECK(down(out->vec, oVec, 12 * T));
-  if (out->skyRad) DLG_CHECK(down(out->skyRad, oSkyR, 8 * P));
-  if (out->skyVis) DLG_CHECK(down(out->skyVis, oSkyV, 8 * P));
+  if (out->skyRad) {
+    if (in->wantRad && T > 0) DLG_CHECK(down(out->skyRad, oSkyR, 8 * P));
+    else std::fill_n(out->skyRad, P, 0.0);
+  }
+  if (out->skyVis) {
+    if (in->wantVis && T > 0) DLG_CHECK(down(out->skyVis, oSkyV, 8 * P));
+    else std::fill_n(out->skyVis, P, 0.0);
+  }
   double gnd[2] = {0, 0};
   DLG_CHECK(down(gnd, oGnd, 16));
   if (out->sunRad) DLG_CHECK(down(out->sunRad, oSunR, 8 * T));

@@ -9,6 +9,8 @@
 // A ray is open when every hit on [1e-5, inf) is an own-plane hit (dl_math.h own_hit) and they lie at 3 distinct
 // distances at most (the reference's closest-hit walk past the sensor's own planes, trace_dev.cu:76-88); the first other
 // hit ends the traversal (any-hit, as TERMINATE_ON_FIRST_HIT).
+#include <type_traits>
+
 #include "dl_math.h"
 #include "gpu_internal.h"
 
@@ -140,14 +142,19 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     counts = g->counts.as<unsigned long long>();
     DLG_CHECK(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), g->stream));
   }
+  // the stack holds at most one entry per level below the root: the smallest variant that covers the tree's height
+  // (a smaller LDS stack lets more waves share a CU)
+  const auto launch = [&](auto count) {
+    constexpr bool C_ = decltype(count)::value;
+    if (s.height <= 24) launch_bits<24, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
+    else if (s.height <= 32) launch_bits<32, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
+    else if (s.height <= 48) launch_bits<48, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
+    else if (s.height <= 64) launch_bits<64, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
+    else launch_bits<160, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
+  };
   dlg_tbegin(g);
-  if (counts) {
-    if (s.height <= 64) launch_bits<64, true>(g, grid, P, N, uint32_t(nc), cull, counts);
-    else launch_bits<160, true>(g, grid, P, N, uint32_t(nc), cull, counts);
-  } else {
-    if (s.height <= 64) launch_bits<64, false>(g, grid, P, N, uint32_t(nc), cull, nullptr);
-    else launch_bits<160, false>(g, grid, P, N, uint32_t(nc), cull, nullptr);
-  }
+  if (counts) launch(std::true_type{});
+  else launch(std::false_type{});
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   if (counts) {

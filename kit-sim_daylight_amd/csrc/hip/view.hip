@@ -289,7 +289,11 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   a.cls = reinterpret_cast<uint8_t*>(b); a.dist = reinterpret_cast<float*>(b + oDist);
   const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(words));
   dlg_tbegin(g);
-  if (s.height <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
+  // the smallest LDS stack that covers the tree's height (at most one entry per level below the root)
+  if (s.height <= 24) k_view_rays<24><<<grid, kWave, 0, g->stream>>>(a);
+  else if (s.height <= 32) k_view_rays<32><<<grid, kWave, 0, g->stream>>>(a);
+  else if (s.height <= 48) k_view_rays<48><<<grid, kWave, 0, g->stream>>>(a);
+  else if (s.height <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
   else if (s.height <= 160) k_view_rays<160><<<grid, kWave, 0, g->stream>>>(a);
   else { dlg_fail(err, errLen, "dlg_view: the scene's tree is deeper than 160 levels"); return 1; }
   DLG_CHECK(hipGetLastError());

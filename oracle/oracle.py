@@ -313,3 +313,142 @@ def daylight(meshes, weather=None, quantities=("radiation",), mf=1, patchSamples
     if series:
         res["series"] = ser[: ne.value * nS * tup.value].reshape(ne.value * nS, tup.value)
     return res
+
+
+def views(meshes, fov=(120.0, -70.0, 50.0), cell=2.0, jitter=True, far=1000.0, reach=(), distance=None, offset=0.0,
+          sides="front", onVertices=False, selfOcclude=True, north=0.0, virtualGround=True, groundZ=math.nan,
+          cullBackfaces=False, targets=None, nthreads=None, raw=False):
+    """The oracle's views run: the host steps of reference src/views.cpp (field :28-66, cell weights :126-136, labels,
+    sensors, the launch frame :268-272, the columns :235-266) over the oracle's C kernels (orc_trace_view,
+    orc_reduce_view). targets: {mesh name: group}. distance: a set of 'mean', 'min', 'group', 'isovist' or None for
+    the default (isovist at 360, else mean and min). Returns dict(values [N x C], columns, cls, dist when raw)."""
+    L = lib()
+    fovH, lo, hi = fov
+    flat = lo == 0 and hi == 0
+    nAz = max(1, round(fovH / cell))
+    nEl = max(1, round((hi - lo) / cell))
+    dist = set(distance) if distance is not None else ({"isovist"} if fovH == 360 else {"mean", "min"})
+    if math.isnan(groundZ):
+        groundZ = min(float(m.points[2::3].min()) for m in meshes if len(m.points))
+    world = fovH == 360
+    dAz = fovH / nAz
+    dEl = (hi - lo) / nEl
+    kDeg = math.pi / 180
+    rowSin = np.zeros(2 * nEl, np.float32)
+    wRow = np.zeros(nEl)
+    tot = 0.0
+    for k in range(nEl):
+        top, bot = math.sin((hi - k * dEl) * kDeg), math.sin((hi - (k + 1) * dEl) * kDeg)
+        rowSin[2 * k], rowSin[2 * k + 1] = bot, top
+        wRow[k] = dAz * kDeg * (1 if flat else top - bot)
+        tot += wRow[k] * nAz
+    wRow /= tot
+    # groups in first-appearance order; the label of a triangle: 2, or 3 + its mesh's group
+    targets = targets or {}
+    groups = []
+    for m in meshes:
+        g = targets.get(m.name)
+        if g is not None and g not in groups:
+            groups.append(g)
+    nClass = 3 + len(groups)
+    # scene: the blocking meshes recentred (as orc_run.c), labels per triangle in build order
+    arr = (Mesh * len(meshes))(*[m.c() for m in meshes])
+    c = np.zeros(3)
+    L.orc_bbox_centre(C.cast(arr, C.c_void_p), len(meshes), c)
+    blocks = [not m.sensor or selfOcclude for m in meshes]
+    nv = sum(len(m.points) // 3 for m, b in zip(meshes, blocks) if b)
+    ntCap = sum(int(np.maximum(m.counts - 2, 0).sum()) for m, b in zip(meshes, blocks) if b)
+    V = np.zeros(3 * max(nv, 1), np.float32)
+    I = np.zeros(3 * max(ntCap, 1), np.uint32)
+    label = []
+    vb = nt = 0
+    skipped = C.c_int(0)
+    for k, m in enumerate(meshes):
+        if not blocks[k]:
+            continue
+        Vk = np.zeros(len(m.points), np.float32)
+        Ik = np.zeros(3 * max(1, int(np.maximum(m.counts - 2, 0).sum())), np.uint32)
+        n = L.orc_triangulate(C.byref(arr[k]), c, Vk if len(Vk) else np.zeros(3, np.float32), Ik, vb, C.byref(skipped))
+        V[3 * vb:3 * vb + len(Vk)] = Vk
+        I[3 * nt:3 * (nt + n)] = Ik[:3 * n]
+        g = targets.get(m.name)
+        label += [2 if g is None else 3 + groups.index(g)] * n
+        vb += len(m.points) // 3
+        nt += n
+    label = np.array(label if label else [2], np.uint8)
+    nS = 2 if sides == "both" else 1
+    sideCode = {"front": 0, "back": 1, "both": 2}[sides]
+    nElem = sum((len(m.points) // 3 if onVertices else len(m.counts)) for m in meshes if m.sensor)
+    nSens = nElem * nS
+    P = np.zeros(3 * nSens, np.float32)
+    N = np.zeros(3 * nSens, np.float32)
+    at = 0
+    for k, m in enumerate(meshes):
+        if m.sensor:
+            at += nS * L.orc_sensors(C.byref(arr[k]), int(onVertices), sideCode, offset, c, P[3 * at:], N[3 * at:])
+    sc = L.orc_scene_new(V, nv, I, nt)
+    rad = north * math.pi / 180
+    v = View(nAz, nEl, -fovH / 2 if not world else 0.0, dAz, rowSin.ctypes.data, int(world), int(world and offset == 0 and not flat),
+             int(jitter), int(sides == "back"), int(virtualGround), nS, groundZ - c[2], math.cos(rad), math.sin(rad),
+             label.ctypes.data)
+    K = nAz * nEl
+    cls = np.zeros(K * nSens, np.uint8)
+    dst = np.zeros(K * nSens, np.float32)
+    L.orc_trace_view(sc, P, N, nSens, 0, C.byref(v), int(cullBackfaces), cls, dst, nthreads or min(8, os.cpu_count() or 1))
+    L.orc_scene_free(sc)
+    iso = "isovist" in dist
+    nR = len(reach)
+    NA = 3 * nClass + 1 + nR + (1 if iso else 0)
+    sums = np.zeros(nSens * NA)
+    reachF = np.array(reach if nR else [0.0], np.float32)
+    L.orc_reduce_view(cls, dst, nSens, K, nAz, wRow, nClass, far, reachF, nR, 0, (2 if flat else 3) if iso else 0, C.byref(v), 0, sums)
+    sums = sums.reshape(nSens, NA)
+    # columns (views.cpp:235-266)
+    names = ["sky", "ground", "obstruction"] + groups
+    if iso:
+        names.append("area" if flat else "volume")
+    mean, mn_, grp = "mean" in dist, "min" in dist, "group" in dist
+    if mean:
+        names.append("distance")
+    if mn_:
+        names.append("distance_min")
+    for g in (groups if grp else []):
+        if mean:
+            names.append(g + "_distance")
+        if mn_:
+            names.append(g + "_distance_min")
+    names += ["reach_%g" % D for D in reach]
+    sfx = (["_front", "_back"] if nS == 2 else ["_back" if sides == "back" else "_front"])
+    columns = [n + s for n in names for s in sfx]
+    nearF, farF = float(np.float32(0.1)), float(np.float32(far))
+    least = lambda m: math.nan if math.isinf(m) else 0.1 if m <= nearF else far if m >= farF else m
+    logMean = lambda w, wl, m: math.nan if not w > 0 else far if m >= farF else math.exp(wl / w)
+    out = np.zeros((nElem, len(columns)))
+    for si in range(nSens):
+        a = sums[si]
+        e, col0 = si // nS, si % nS
+        vals = []
+        own = a[3 * nClass]
+        w = sum(a[3 * k] for k in range(nClass))
+        wl = sum(a[3 * k + 1] for k in range(nClass))
+        mn = min(a[3 * k + 2] for k in range(nClass))
+        total = w + own
+        vals += [a[0] / total, a[3] / total, (a[6] + own) / total] + [a[3 * (3 + k)] / total for k in range(len(groups))]
+        if iso:
+            vals.append(a[NA - 1])
+        if mean:
+            vals.append(logMean(w, wl, mn))
+        if mn_:
+            vals.append(least(mn))
+        for k in (range(len(groups)) if grp else []):
+            if mean:
+                vals.append(logMean(a[3 * (3 + k)], a[3 * (3 + k) + 1], a[3 * (3 + k) + 2]))
+            if mn_:
+                vals.append(least(a[3 * (3 + k) + 2]))
+        vals += [a[3 * nClass + 1 + r] / total for r in range(nR)]
+        for j, x in enumerate(vals):
+            out[e, col0 + j * nS] = x
+    res = dict(values=out, columns=columns, nSens=nSens, K=K)
+    if raw:
+        res.update(cls=cls.reshape(K, nSens).T.copy(), dist=dst.reshape(K, nSens).T.copy(), sums=sums)
+    return res

@@ -160,11 +160,12 @@ def test_scene_resident_across_runs(native, box):
 
 def test_progress_and_cancel(native, box):
     print("case: box, control", flush=True)
+    flags = ["--quantity", "sunhours"] + BERLIN + ["--tz", "1", "--from", "06-21T00:00", "--to", "06-22T00:00"]
     c = native.Control()
-    native.run("daylight", ["--quantity", "sunhours"] + BERLIN, box, control=c)
+    native.run("daylight", flags, box, control=c)
     assert c.progress == 1.0
     c2 = native.Control()
     c2.cancel()
     with pytest.raises(native.DaylightError) as e:
-        native.run("daylight", ["--quantity", "sunhours"] + BERLIN, box, control=c2)
+        native.run("daylight", flags, box, control=c2)
     assert e.value.status == 2 and "cancelled" in str(e.value)
