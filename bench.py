@@ -32,7 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "kit-sim_daylight_amd"))
 
 LAT, LON, TZ = 52.52, 13.41, 1.0
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES = 64, 48
+NODE_BYTES, TRI_BYTES = 112, 48   # a 4-wide record visit loads 7 x 16 bytes; a triangle 3 x 16
 
 
 # ---------------------------------------------------------------- synthetic inputs
@@ -261,7 +261,7 @@ def main():
     # the traversal's algorithmic fetches: one counted run (untimed)
     launches = d["trace_launches"] / a.steps
     per_launch_ms = d["trace_ms"] / d["trace_launches"]
-    alg_bytes_launch = achieved = None
+    alg_bytes_launch = achieved = simd_eff = None
     if not a.no_count:
         native.count_fetches(True)
         k0 = native.gpu_counters()
@@ -273,6 +273,8 @@ def main():
         io_bytes = n_faces * n_words * (4 + 24)         # a visibility word written, the sensor's point + normal read per word
         alg_bytes_launch = (fetch_bytes + io_bytes) / launches
         achieved = alg_bytes_launch / (per_launch_ms * 1e-3) / 1e9
+        waves = k1["wave_iters"] - k0["wave_iters"]
+        simd_eff = (k1["node_fetches"] - k0["node_fetches"]) / (64.0 * waves) if waves else None
 
     # the contraction methods side by side (two runs each)
     methods = {}
@@ -311,7 +313,7 @@ def main():
         "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm",
                      "achieved": None if achieved is None else round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
-                     "launch_ms": round(per_launch_ms, 4),
+                     "launch_ms": round(per_launch_ms, 4), "simd_efficiency": None if simd_eff is None else round(simd_eff, 3),
                      "algorithmic_bytes_per_launch": None if alg_bytes_launch is None else round(alg_bytes_launch)},
         "cpu_baseline": cpu,
     }

@@ -133,9 +133,9 @@ typedef struct {
   double buildMs, traceMs, reduceMs, seriesMs, skyMs, viewMs;
   int64_t rays, traced, launches, sceneBytes, nodes, height;
   double lastTraceMs, lastReduceMs, lastSeriesMs, lastSkyMs, lastViewMs, lastViewReduceMs;
-  /* the counting variant's totals (dlg_count_fetches): node visits and triangle tests of the visibility traversal, and
-   * the ray slots they were counted over */
-  int64_t nodeFetches, triFetches, countedRays;
+  /* the counting variant's totals (dlg_count_fetches): node visits and triangle tests of the visibility traversal, the
+   * ray slots they were counted over, and the traversal iterations the waves executed */
+  int64_t nodeFetches, triFetches, countedRays, waveIters;
 } dlg_stats_t;
 int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out);
 /* No counterpart in the reference (its OptiX traversal is opaque): on != 0 makes dlg_trace run the traversal's counting

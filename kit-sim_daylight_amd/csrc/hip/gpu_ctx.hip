@@ -28,7 +28,7 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
-  for (DevBuf* b : {&g->scene.nodes, &g->scene.leaves, &g->P, &g->N, &g->D, &g->bits, &g->scratch, &g->out, &g->skyS, &g->skyG,
+  for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->bits, &g->scratch, &g->out, &g->skyS, &g->skyG,
                     &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
@@ -62,16 +62,14 @@ dlg_scene* dlg_detach(dlg_gpu* g) {
 }
 
 void dlg_attach(dlg_gpu* g, dlg_scene* s) {
-  g->scene.nodes.release();
-  g->scene.leaves.release();
+  g->scene.release();
   g->scene = *s;
   delete s;
 }
 
 void dlg_free_scene(dlg_scene* s) {
   if (!s) return;
-  s->nodes.release();
-  s->leaves.release();
+  s->release();
   delete s;
 }
 

@@ -11,9 +11,11 @@
 // child & 0x7fffffff), 2 unused: one 64-byte line per node visit. Leaf k = 12 floats: v0, v1, v2 of the k-th triangle
 // in sorted order, its original index (as bits), 2 unused: three 16-byte loads.
 // The same arrays come out of the host build in the oracle, bit for bit (tests compare them).
+//   6. the 4-wide records the traversals walk (wide.h): one per inner node, from the node and its inner children
 #include <hipcub/hipcub.hpp>
 
 #include "gpu_internal.h"
+#include "wide.h"
 
 namespace {
 
@@ -161,6 +163,41 @@ __global__ void k_refit(int64_t n, const uint32_t* __restrict__ L, const uint32_
   }
 }
 
+// record n of the 4-wide tree (wide.h): n's children, an inner child replaced by its two children
+__global__ void k_wide(int64_t nn, const float* __restrict__ nodes, float4* __restrict__ wide) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const float inf = __int_as_float(0x7f800000);
+  float lo[3][4], hi[3][4];
+  uint32_t rf[4];
+  for (int k = 0; k < 4; ++k) {
+    for (int a = 0; a < 3; ++a) { lo[a][k] = inf; hi[a][k] = -inf; }
+    rf[k] = dlw::kEmpty;
+  }
+  const float* nd = nodes + 16 * n;
+  int k = 0;
+  for (int side = 0; side < 2; ++side) {
+    const uint32_t c = __float_as_uint(nd[12 + side]);
+    if (c & 0x80000000u) {
+      for (int a = 0; a < 3; ++a) { lo[a][k] = nd[6 * side + a]; hi[a][k] = nd[6 * side + 3 + a]; }
+      rf[k++] = c;
+    } else {
+      const float* cn = nodes + 16 * size_t(c);
+      for (int s2 = 0; s2 < 2; ++s2) {
+        for (int a = 0; a < 3; ++a) { lo[a][k] = cn[6 * s2 + a]; hi[a][k] = cn[6 * s2 + 3 + a]; }
+        rf[k++] = __float_as_uint(cn[12 + s2]);
+      }
+    }
+  }
+  float4* q = wide + 8 * n;
+  for (int a = 0; a < 3; ++a) {
+    q[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+    q[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+  }
+  q[6] = make_float4(__uint_as_float(rf[0]), __uint_as_float(rf[1]), __uint_as_float(rf[2]), __uint_as_float(rf[3]));
+  q[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 inline unsigned blocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
@@ -168,14 +205,13 @@ inline unsigned blocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 extern "C" int64_t dlg_build_bytes(int64_t nv, int64_t nt) {
   if (nt <= 0) return 0;
   const int64_t in = nv * 12 + nt * 12, keys = nt * (8 + 4) * 2, cen = nt * 12, tree = nt * (4 * 4 + 24 + 4 + 4);
-  const int64_t out = (nt > 1 ? nt - 1 : 1) * 64 + nt * 48;
+  const int64_t out = (nt > 1 ? nt - 1 : 1) * (64 + 128) + nt * 48;
   return in + keys + cen + tree + out + (int64_t(64) << 20);   // + the sort's temp storage (a few MB)
 }
 
 extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint32_t* I, int64_t nt, char* err, int32_t errLen) {
   dlg_scene& s = g->scene;
-  s.nodes.release();
-  s.leaves.release();
+  s.release();
   s.nNodes = s.nTris = 0;
   s.height = 0;
   float mx = 0.f;
@@ -232,6 +268,8 @@ extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint3
   k_refit<<<blocks(nt, 256), 256, 0, g->stream>>>(nt, L.as<uint32_t>(), R.as<uint32_t>(), pl.as<uint32_t>(), pi.as<uint32_t>(),
                                                   s.leaves.as<float>(), ibox.as<float>(), height.as<int>(), flags.as<uint32_t>(),
                                                   s.nodes.as<float>());
+  TRY(s.wide.grow(size_t(nn) * 128), "wide records");
+  k_wide<<<blocks(nn, 256), 256, 0, g->stream>>>(nn, s.nodes.as<float>(), s.wide.as<float4>());
   TRY(hipGetLastError(), "kernels");
   int h = 0;
   TRY(hipMemcpyAsync(&h, height.p, 4, hipMemcpyDeviceToHost, g->stream), "height");

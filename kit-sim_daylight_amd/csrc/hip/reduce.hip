@@ -1,45 +1,53 @@
 // SPDX-License-Identifier: Apache-2.0
 // The cumulative reduces of a daylight run in one pass over the visibility bits: replaces k_reduce<Bits> and gpu_reduce
 // (reference src/gpu.cu:99-114, 612-625), which ran once per quantity and re-read the bits each time.
-// A thread per sensor walks its words (word-major bits[w nc + i]: coalesced across the wave) and, for each visible
-// direction in ascending order, adds double(float(max(0, dotf(n, D_d)))) x double(W[q][d]) to every quantity q it
-// carries: the reference's sums, operation for operation, so the result is the same bits for any chunking.
-// HBM-bound: 4 nWords bytes of bits + 12 + 8 nQ per sensor; the weights and directions stay in L2.
+// Each sensor's sum is the reference's: acc_q += double(float(max(0, dotf(n, D_d)))) x double(W[q][d]) over its visible
+// directions in ascending order (sun hours: the float 1 in place of the cosine), in FP64, so the result is the same bits
+// for any chunking. The order makes the sum a dependent chain; the work around it is spread over a wave:
+//   a wave (one block) per sensor; per slice of 64 directions the lanes of the visible ones form their products and
+//   store them compacted in direction order (position = visible lanes below), then lane q adds the slice's products of
+//   quantity q in that order.
+// HBM: 4 nWords bytes of bits + 24 (P, N) + 8 nQ per sensor; directions and weights stay in L2.
 #include "dl_math.h"
 #include "gpu_internal.h"
 
 namespace {
 
-constexpr int kMaxQ = 8;
+constexpr int kMaxQ = 8, kWave = 64;
 
 template <int NQ>
-__global__ __launch_bounds__(256) void k_reduce(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
-                                                const float3* __restrict__ D, const float* __restrict__ W, uint32_t nDir, uint32_t cosMask,
-                                                double* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
+__global__ __launch_bounds__(kWave) void k_reduce(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
+                                                  const float3* __restrict__ D, const float* __restrict__ W, uint32_t nDir,
+                                                  uint32_t cosMask, double* __restrict__ out) {
+  __shared__ double prod[NQ][kWave];
+  const uint32_t lane = threadIdx.x, i = blockIdx.x;
   const float3 n = N[i];
-  double acc[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  for (uint32_t w = 0; w < nWords; ++w) {
-    uint32_t m = bits[size_t(w) * nc + i];
-    while (m) {
-      const uint32_t d = w * 32 + (__ffs(m) - 1);
-      m &= m - 1;
+  double acc = 0.0;          // lane q < NQ: quantity q
+  for (uint32_t base = 0; base < nDir; base += kWave) {
+    const uint32_t d = base + lane;
+    const bool vis = d < nDir && ((bits[size_t(d >> 5) * nc + i] >> (d & 31)) & 1u);
+    const uint64_t m = __ballot(vis);
+    if (vis) {
+      const uint32_t at = __popcll(m & ((uint64_t(1) << lane) - 1));
       const float c = fmaxf(dlm::dot3(n, D[d]), 0.f);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[q] += (double)((cosMask >> q) & 1u ? c : 1.f) * (double)W[size_t(q) * nDir + d];
+      for (int q = 0; q < NQ; ++q) prod[q][at] = (double)((cosMask >> q) & 1u ? c : 1.f) * (double)W[size_t(q) * nDir + d];
     }
+    __syncthreads();
+    const int cnt = __popcll(m);
+    if (lane < uint32_t(NQ)) {
+      const double* pq = prod[lane];
+      for (int k = 0; k < cnt; ++k) acc += pq[k];
+    }
+    __syncthreads();
   }
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) out[size_t(q) * nc + i] = acc[q];
+  if (lane < uint32_t(NQ)) out[size_t(lane) * nc + i] = acc;
 }
 
 template <int NQ>
 void launch(dlg_gpu* g, uint32_t nc, uint32_t mask) {
-  k_reduce<NQ><<<(nc + 255) / 256, 256, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0,
-                                                       g->D.as<float3>(), g->scratch.as<float>(), uint32_t(g->nDir), mask, g->out.as<double>());
+  k_reduce<NQ><<<nc, kWave, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0, g->D.as<float3>(),
+                                           g->scratch.as<float>(), uint32_t(g->nDir), mask, g->out.as<double>());
 }
 
 }  // namespace

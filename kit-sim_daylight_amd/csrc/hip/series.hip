@@ -16,7 +16,7 @@
 
 namespace {
 
-typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 __device__ __forceinline__ uint16_t bf16_rn(float f) {   // round to nearest even (finite inputs)
@@ -118,16 +118,18 @@ __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ r
   for (int k = 0; k < NS; ++k) B[k * plane + size_t(t) * Kp + d] = p[k];
 }
 
-// v_mfma_f32_32x32x8 bf16 operand layout: lane l holds A[l % 32][4 (l / 32) + v] and B[4 (l / 32) + v][l % 32], v = 0..3;
-// the result acc[v] is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// v_mfma_f32_32x32x16 bf16 (gfx950) operands: lane l holds 8 values of row l % 32 of A and of column l % 32 of B, in
+// the K slots of lane group l / 32 (the same slots for A and B, so the product sums every k once whichever slot order
+// the unit uses); the result acc[v] is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
 //
 // Block: WM x WN waves; wave (wm, wn) owns 32 sensors x 64 steps (two 32 x 32 MFMA tiles side by side, sharing the A
-// fragment). Per 32-direction slice the block stages A [NS][BM][32] and S^T [NS][BN][32] in LDS (rows padded to 36
-// bf16 so that the lanes' 8-byte reads spread over the banks), runs the products on MFMA into FP32 and adds the
-// slice's sums into FP64 accumulators. Every 32 x 32 output tile sees the same MFMA sequence as a one-tile wave would.
-// Blocks are numbered XCD-major: the blocks an XCD runs (hardware id % 8) take consecutive tiles, so the tiles that
-// share sensor rows share that XCD's L2.
-constexpr int kBK = 32, kLds = 36;
+// fragment). Per 32-direction slice the block stages A [NS][BM][32] and S^T [NS][BN][32] in LDS (rows padded to 40
+// bf16: 16-byte aligned lane reads spread over the banks), runs the products on MFMA into FP32 and adds the slice's
+// sums into FP64 accumulators.
+// Blocks are numbered XCD-major: the blocks an XCD runs (hardware id % 8) take consecutive tiles of a grouped order
+// (kGroup rows of sensor tiles per step column), so the tiles that share operand panels share that XCD's L2.
+constexpr int kBK = 32, kLds = 40;
+constexpr uint32_t kGroup = 4;
 template <int NS, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void k_series_mfma(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, uint32_t nc,
                                                               uint32_t Kp, uint32_t Tp, int T, const uint32_t* __restrict__ bits,
@@ -143,18 +145,22 @@ __global__ __launch_bounds__(64 * WM * WN) void k_series_mfma(const uint16_t* __
   const uint32_t full = tiles % 8 == 0 ? 8 : tiles % 8;   // XCDs that take `per` tiles; the rest take per - 1
   const uint32_t tile = xcd < full ? xcd * per + rank : full * per + (xcd - full) * (per - 1) + rank;
   if (tile >= tiles || (xcd >= full && rank >= per - 1)) return;
-  const uint32_t ti = tile / tilesT, tt = tile % tilesT;
+  // grouped order: kGroup sensor-tile rows walked down before the next step column, so that a group's A panels stay in
+  // L2 while each S^T tile serves kGroup rows
+  const uint32_t tilesI = tiles / tilesT, grp = tile / (kGroup * tilesT), first = grp * kGroup;
+  const uint32_t rows = tilesI - first < kGroup ? tilesI - first : kGroup, local = tile % (kGroup * tilesT);
+  const uint32_t ti = first + local % rows, tt = local / rows;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w % WM, wn = w / WM;
   const uint32_t iBlk = ti * BM, tBlk = tt * BN;
   const size_t aPlane = size_t(nc) * Kp, bPlane = size_t(Tp) * Kp;
-  const int kq = 4 * (lane >> 5), r32 = lane & 31;
+  const int k8 = 8 * (lane >> 5), r32 = lane & 31;
   double acc64[2][16];
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc64[c][v] = 0.0;
   for (uint32_t k0 = 0; k0 < Kp; k0 += kBK) {
-    // stage the slice: rows of 32 bf16 = 4 x 16 bytes, written to LDS as 8-byte halves
+    // stage the slice: rows of 32 bf16 = 4 x 16 bytes
     for (int q = threadIdx.x; q < NS * (BM + BN) * 4; q += NT) {
       const int s = q / ((BM + BN) * 4), rem = q % ((BM + BN) * 4), r = rem / 4, c = rem % 4;
       uint4 v = make_uint4(0, 0, 0, 0);
@@ -168,27 +174,26 @@ __global__ __launch_bounds__(64 * WM * WN) void k_series_mfma(const uint16_t* __
         v = *reinterpret_cast<const uint4*>(B + s * bPlane + size_t(t) * Kp + k0 + 8 * c);
         dst = &Bs[s][r - BM][8 * c];
       }
-      reinterpret_cast<uint2*>(dst)[0] = make_uint2(v.x, v.y);
-      reinterpret_cast<uint2*>(dst)[1] = make_uint2(v.z, v.w);
+      *reinterpret_cast<uint4*>(dst) = v;
     }
     __syncthreads();
     f32x16 acc[2] = {f32x16{}, f32x16{}};
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 8) {
-      bf16x4 a[NS], b[2][NS];
+    for (int kk = 0; kk < kBK; kk += 16) {
+      bf16x8 a[NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        a[s] = *reinterpret_cast<const bf16x4*>(&As[s][32 * wm + r32][kk + kq]);
+      for (int s = 0; s < NS; ++s) a[s] = *reinterpret_cast<const bf16x8*>(&As[s][32 * wm + r32][kk + k8]);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) b[c][s] = *reinterpret_cast<const bf16x4*>(&Bs[s][64 * wn + 32 * c + r32][kk + kq]);
+      for (int c = 0; c < 2; ++c) {   // one column tile's fragments live at a time
+        bf16x8 b[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) b[s] = *reinterpret_cast<const bf16x8*>(&Bs[s][64 * wn + 32 * c + r32][kk + k8]);
+        // products whose exponent sum is at most NS - 1 pieces below the leading one, smallest first
+#pragma unroll
+        for (int o = NS - 1; o >= 0; --o)
+#pragma unroll
+          for (int s = 0; s <= o; ++s) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], b[o - s], acc[c], 0, 0, 0);
       }
-      // products whose exponent sum is at most NS - 1 pieces below the leading one, smallest first
-#pragma unroll
-      for (int o = NS - 1; o >= 0; --o)
-#pragma unroll
-        for (int s = 0; s <= o; ++s)
-#pragma unroll
-          for (int c = 0; c < 2; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[s], b[c][o - s], acc[c], 0, 0, 0);
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c)

@@ -2,7 +2,7 @@
 // Views on CDNA4: replaces __raygen__view (reference src/trace_dev.cu:177-232), gpu_view (gpu.cu:572-610), k_view /
 // k_iso / gpu_reduce_view (gpu.cu:171-274, 671-697).
 //   k_view_rays  a wave per 64 consecutive sensors and one 32-cell word (grid y): frame, jittered cell, virtual ground,
-//                closest hit through the LBVH (exact ties by the lower triangle index), the sensor's own planes passed
+//                closest hit through the 4-wide records of the LBVH (exact ties by the lower triangle index), the sensor's own planes passed
 //                (at most 3; the 4th an obstruction there), the hit triangle's label. Direction-major cls / dist.
 //   k_view_sums  a wave per sensor, lane l sums directions l, l + 64, ... (class weights, weight x ln d, minima, own
 //                surface, reach, glazing), then a fixed shuffle tree (s = 32 .. 1): the same bits for any chunking.
@@ -13,22 +13,13 @@
 
 #include "dl_math.h"
 #include "gpu_internal.h"
+#include "wide.h"
 
 namespace {
 
 using namespace dlm;
 constexpr int kWave = 64;
 constexpr unsigned kClasses = 3 + 16;
-
-struct BoxHit { bool hit; float t; };
-__device__ __forceinline__ BoxHit box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
-  const float nx = r.inv.x >= 0.f ? lx : hx, fx = r.inv.x >= 0.f ? hx : lx;
-  const float ny = r.inv.y >= 0.f ? ly : hy, fy = r.inv.y >= 0.f ? hy : ly;
-  const float nz = r.inv.z >= 0.f ? lz : hz, fz = r.inv.z >= 0.f ? hz : lz;
-  const float t0 = fmaxf(fmaxf(fmaxf(tmin, (nx - r.oNear.x) * r.inv.x), (ny - r.oNear.y) * r.inv.y), (nz - r.oNear.z) * r.inv.z);
-  const float t1 = fminf(fminf(fminf(tmax, (fx - r.oFar.x) * r.inv.x), (fy - r.oFar.y) * r.inv.y), (fz - r.oFar.z) * r.inv.z);
-  return {t0 <= t1, t0};
-}
 
 struct Closest { float t; uint32_t leaf, prim; bool front, hit; };
 
@@ -42,37 +33,37 @@ __device__ __forceinline__ void leaf_test(const Ray& r, const float4* __restrict
   c.t = t; c.leaf = leaf; c.prim = prim; c.front = front; c.hit = true;
 }
 
+// closest hit through the 4-wide records (wide.h): children nearest first, those starting beyond the best hit dropped
 template <int STACK>
-__device__ Closest closest_hit(const float4* __restrict__ nodes, const float4* __restrict__ leaves, const Ray& r, float tmin, float tmax, bool cull,
+__device__ Closest closest_hit(const float4* __restrict__ wide, const float4* __restrict__ leaves, const Ray& r, float tmin, float tmax, bool cull,
                                uint32_t* __restrict__ stk, int lane) {
   Closest c{tmax, 0u, 0u, false, false};
+  const dlw::Slab sl = dlw::slab(r);
   uint32_t cur = 0;
   int sp = 0;
   for (;;) {
-    const float4 A = nodes[4 * cur], B = nodes[4 * cur + 1], C = nodes[4 * cur + 2], E = nodes[4 * cur + 3];
-    const BoxHit hl = box_hit(r, A.x, A.y, A.z, A.w, B.x, B.y, tmin, c.t);
-    const BoxHit hr = box_hit(r, B.z, B.w, C.x, C.y, C.z, C.w, tmin, c.t);
-    const uint32_t cl = __float_as_uint(E.x), cr = __float_as_uint(E.y);
-    if (hl.hit && (cl & 0x80000000u)) leaf_test(r, leaves, cl & 0x7fffffffu, tmin, cull, c);
-    if (hr.hit && (cr & 0x80000000u)) leaf_test(r, leaves, cr & 0x7fffffffu, tmin, cull, c);
-    const bool il = hl.hit && !(cl & 0x80000000u), ir = hr.hit && !(cr & 0x80000000u);
-    if (il && ir) {
-      const bool leftFirst = hl.t <= hr.t;
-      stk[sp * kWave + lane] = leftFirst ? cr : cl;
-      ++sp;
-      cur = leftFirst ? cl : cr;
-    } else if (il) cur = cl;
-    else if (ir) cur = cr;
-    else {
+    dlw::Visit v;
+    dlw::visit(wide, cur, sl, tmin, c.t, v);
+    for (uint32_t m = v.leafMask; m; m &= m - 1) {
+      const int k = __ffs(m) - 1;
+      leaf_test(r, leaves, k == 0 ? v.leaf[0] : k == 1 ? v.leaf[1] : k == 2 ? v.leaf[2] : v.leaf[3], tmin, cull, c);
+    }
+    // slot 0 holds the nearest hit inner child; a child is taken where it starts within the best hit so far
+    if (!(v.tInner[0] <= c.t)) {
       if (sp == 0) return c;
       --sp;
       cur = stk[sp * kWave + lane];
+      continue;
     }
+    if (v.tInner[3] <= c.t) stk[sp++ * kWave + lane] = v.inner[3];
+    if (v.tInner[2] <= c.t) stk[sp++ * kWave + lane] = v.inner[2];
+    if (v.tInner[1] <= c.t) stk[sp++ * kWave + lane] = v.inner[1];
+    cur = v.inner[0];
   }
 }
 
 struct ViewArgs {
-  const float4* nodes; const float4* leaves; int64_t nTris; float pad;
+  const float4* wide; const float4* leaves; int64_t nTris; float pad;
   const float3* P; const float3* N;
   const uint8_t* label;
   const float* rowSin;       // [2 nEl]
@@ -129,7 +120,7 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
       uint32_t own = 0, panes = 0;
       for (;;) {
         Closest c{};
-        if (a.nTris > 0) c = closest_hit<STACK>(a.nodes, a.leaves, ray, tmin, tmax, a.cullBack != 0, stk, int(lane));
+        if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, tmin, tmax, a.cullBack != 0, stk, int(lane));
         if (!c.hit) { cl = dir.z >= 0.f ? 0 : 1; t = tg; break; }
         t = c.t;
         const float4 A = a.leaves[3 * c.leaf], B = a.leaves[3 * c.leaf + 1], E = a.leaves[3 * c.leaf + 2];
@@ -277,7 +268,7 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   if (nc == 0 || K == 0) { DLG_CHECK(hipStreamSynchronize(g->stream)); return 0; }
   const dlg_scene& s = g->scene;
   ViewArgs a;
-  a.nodes = s.nodes.as<float4>(); a.leaves = s.leaves.as<float4>(); a.nTris = s.nTris; a.pad = s.pad;
+  a.wide = s.wide.as<float4>(); a.leaves = s.leaves.as<float4>(); a.nTris = s.nTris; a.pad = s.pad;
   a.P = g->P.as<float3>() + i0; a.N = g->N.as<float3>() + i0;
   a.label = g->labels.as<uint8_t>();
   a.rowSin = reinterpret_cast<const float*>(b + oSin);
@@ -289,13 +280,14 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   a.cls = reinterpret_cast<uint8_t*>(b); a.dist = reinterpret_cast<float*>(b + oDist);
   const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(words));
   dlg_tbegin(g);
-  // the smallest LDS stack that covers the tree's height (at most one entry per level below the root)
-  if (s.height <= 24) k_view_rays<24><<<grid, kWave, 0, g->stream>>>(a);
-  else if (s.height <= 32) k_view_rays<32><<<grid, kWave, 0, g->stream>>>(a);
-  else if (s.height <= 48) k_view_rays<48><<<grid, kWave, 0, g->stream>>>(a);
-  else if (s.height <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
-  else if (s.height <= 160) k_view_rays<160><<<grid, kWave, 0, g->stream>>>(a);
-  else { dlg_fail(err, errLen, "dlg_view: the scene's tree is deeper than 160 levels"); return 1; }
+  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU)
+  const int need = dlw::wide_stack(s.height);
+  if (need <= 24) k_view_rays<24><<<grid, kWave, 0, g->stream>>>(a);
+  else if (need <= 32) k_view_rays<32><<<grid, kWave, 0, g->stream>>>(a);
+  else if (need <= 48) k_view_rays<48><<<grid, kWave, 0, g->stream>>>(a);
+  else if (need <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
+  else if (need <= 160) k_view_rays<160><<<grid, kWave, 0, g->stream>>>(a);
+  else { dlg_fail(err, errLen, "dlg_view: the scene's tree is too deep for the traversal stack"); return 1; }
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   g->stats.viewMs += ms;

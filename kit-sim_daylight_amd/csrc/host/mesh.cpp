@@ -6,6 +6,7 @@
 // The reference triangulates n-gons with earcut.hpp (v2.2.4, fetched at build time, not in the reference tree); this
 // build's ear clipping gives the same area for simple polygons but may pick other diagonals.
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <charconv>
 #include <cmath>
@@ -325,17 +326,48 @@ size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const d
     if (opt.sides != Sides::Back) { pOut[j] = f3(q + nr * opt.offset - C); nOut[j] = f3(nr); ++j; }
     if (opt.sides != Sides::Front) { pOut[j] = f3(q - nr * opt.offset - C); nOut[j] = f3(nr * -1); ++j; }
   };
-  size_t off = 0;
-  for (size_t f = 0; f < m.counts.size(); ++f) {
-    const int n = m.counts[f];
-    const int* idx = m.indices.data() + off;
-    off += size_t(n);
-    V3 nrm, cen;
-    double area;
-    face_geom(m, idx, n, nrm, area, cen);
-    if (area < 1e-12) ++degenerate;
-    if (opt.onVertices) for (int k = 0; k < n; ++k) vn[size_t(idx[k])] = vn[size_t(idx[k])] + nrm * area;
-    else push(cen, nrm);
+  const size_t nF = m.counts.size();
+  constexpr size_t kBlock = 4096;
+  if (!opt.onVertices && nF > 2 * kBlock) {
+    // faces are independent: blocks of faces on the host threads, each writing its own sensors (face f: f x sides)
+    const size_t nS = opt.sides == Sides::Both ? 2 : 1, nB = (nF + kBlock - 1) / kBlock;
+    std::vector<size_t> start(nB + 1, 0);
+    for (size_t b = 0, off = 0, f = 0; b < nB; ++b) {
+      start[b] = off;
+      for (const size_t e = std::min(nF, f + kBlock); f < e; ++f) off += size_t(m.counts[f]);
+    }
+    std::atomic<int> deg{0};
+    par_for(nB, [&](size_t b) {
+      size_t off = start[b], jb = b * kBlock * nS;
+      int bad = 0;
+      for (size_t f = b * kBlock, e = std::min(nF, f + kBlock); f < e; ++f) {
+        const int n = m.counts[f];
+        const int* idx = m.indices.data() + off;
+        off += size_t(n);
+        V3 nrm, cen;
+        double area;
+        face_geom(m, idx, n, nrm, area, cen);
+        if (area < 1e-12) ++bad;
+        if (opt.sides != Sides::Back) { pOut[jb] = f3(cen + nrm * opt.offset - C); nOut[jb] = f3(nrm); ++jb; }
+        if (opt.sides != Sides::Front) { pOut[jb] = f3(cen - nrm * opt.offset - C); nOut[jb] = f3(nrm * -1); ++jb; }
+      }
+      deg += bad;
+    });
+    degenerate = deg;
+    j = nF * nS;
+  } else {
+    size_t off = 0;
+    for (size_t f = 0; f < nF; ++f) {
+      const int n = m.counts[f];
+      const int* idx = m.indices.data() + off;
+      off += size_t(n);
+      V3 nrm, cen;
+      double area;
+      face_geom(m, idx, n, nrm, area, cen);
+      if (area < 1e-12) ++degenerate;
+      if (opt.onVertices) for (int k = 0; k < n; ++k) vn[size_t(idx[k])] = vn[size_t(idx[k])] + nrm * area;
+      else push(cen, nrm);
+    }
   }
   for (size_t v = 0; v < vn.size(); ++v) {
     V3 n = vn[v];

@@ -4,7 +4,9 @@
 // (reference solar.cpp). The per-step sun and Perez sky of a run are computed on the GPU (sky.hip); clear_sky_weather,
 // which makes an input file's worth of weather, calls the same __host__ __device__ sun as the kernel.
 #include <algorithm>
+#include <charconv>
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -253,16 +255,50 @@ bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string
     }
   }
   w.dni.clear(); w.dhi.clear(); w.missing = 0; w.shiftH = 0;
+  w.dni.reserve(8784);
+  w.dhi.reserve(8784);
   std::vector<float> ghi;
-  while (std::getline(f, line)) {
-    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-    split(line);
-    if (cols.size() < 16) { err = "EPW: short data row " + std::to_string(w.dni.size() + 1) + ": " + label; return false; }
-    double dni = std::atof(cols[14].c_str()), dhi = std::atof(cols[15].c_str());
+  // the data rows in one buffer, scanned in place: columns 13-15 (GHI, DNI, DHI) read as atof reads a column
+  std::string rest;
+  {
+    char buf[1 << 16];
+    while (f.read(buf, sizeof buf) || f.gcount() > 0) rest.append(buf, size_t(f.gcount()));
+  }
+  const char* p = rest.data();
+  const char* const end = p + rest.size();
+  std::string field;
+  // atof of a field: from_chars (correctly rounded, as glibc's strtod) where it takes the whole number, else atof itself
+  const auto number = [&](const char* a, const char* b) {
+    const char* c = a;
+    while (c < b && (*c == ' ' || *c == '\t')) ++c;
+    if (c < b && *c == '+' && c + 1 < b && *(c + 1) != '-' && *(c + 1) != '+') ++c;
+    double x = 0;
+    const auto r = std::from_chars(c, b, x);
+    if (r.ec == std::errc() && (r.ptr == b || *r.ptr == '\r' || *r.ptr == ' ' || *r.ptr == '\t')) return x;
+    field.assign(a, b);
+    return std::atof(field.c_str());
+  };
+  while (p < end) {
+    const char* eol = static_cast<const char*>(std::memchr(p, '\n', size_t(end - p)));
+    if (!eol) eol = end;
+    const char* q = p;
+    while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
+    if (q == eol) { p = eol + 1; continue; }
+    // field starts; the field count is a getline split's: commas + 1, less a trailing empty field
+    const char* at[16];
+    size_t n = 1;
+    at[0] = p;
+    for (const char* c = p; c < eol; ++c)
+      if (*c == ',') { if (n < 16) at[n] = c + 1; ++n; }
+    if (eol[-1] == ',') --n;
+    if (n < 16) { err = "EPW: short data row " + std::to_string(w.dni.size() + 1) + ": " + label; return false; }
+    const auto colEnd = [&](int k) { const char* c = static_cast<const char*>(std::memchr(at[k], ',', size_t(eol - at[k]))); return c ? c : eol; };
+    double dni = number(at[14], colEnd(14)), dhi = number(at[15], colEnd(15));
     if (dni >= 9999 || dhi >= 9999) { ++w.missing; if (dni >= 9999) dni = 0; if (dhi >= 9999) dhi = 0; }
     w.dni.push_back(float(dni));
     w.dhi.push_back(float(dhi));
-    if (pvgis) { const double g = std::atof(cols[13].c_str()); ghi.push_back(g < 9999 ? float(g) : 0.f); }
+    if (pvgis) { const double g = number(at[13], colEnd(13)); ghi.push_back(g < 9999 ? float(g) : 0.f); }
+    p = eol + 1;
   }
   const size_t n = w.dni.size();
   w.leap = hdrLeap || n == 8784;

@@ -38,13 +38,15 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// A built occluder scene: LBVH nodes (16 floats each), leaf triangles (12 floats each), its padding and height.
+// A built occluder scene: LBVH nodes (16 floats each), their 4-wide records (32 floats each, wide.h), leaf triangles
+// (12 floats each), its padding and height.
 struct dlg_scene {
-  DevBuf nodes, leaves;
+  DevBuf nodes, wide, leaves;
   int64_t nNodes = 0, nTris = 0;
   int32_t height = 0;
   float pad = 0.f;
-  int64_t bytes() const { return int64_t(nodes.cap + leaves.cap); }
+  int64_t bytes() const { return int64_t(nodes.cap + wide.cap + leaves.cap); }
+  void release() { nodes.release(); wide.release(); leaves.release(); }
 };
 
 struct dlg_gpu {

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libdaylight_amd.so")
+# DAYLIGHT_AMD_LIB names another build of the library (as the reference binding's DAYLIGHT_DLL does)
+LIB_PATH = os.environ.get("DAYLIGHT_AMD_LIB") or os.path.join(HERE, "lib", "libdaylight_amd.so")
 
 F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -46,7 +47,7 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("buildMs", "traceMs", "reduceMs", "seriesMs", "skyMs", "viewMs")] + \
                [(n, C.c_int64) for n in ("rays", "traced", "launches", "sceneBytes", "nodes", "height")] + \
                [(n, C.c_double) for n in ("lastTraceMs", "lastReduceMs", "lastSeriesMs", "lastSkyMs", "lastViewMs", "lastViewReduceMs")] + \
-               [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays")]
+               [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays", "waveIters")]
 
 
 class ViewIn(C.Structure):
@@ -56,7 +57,7 @@ class ViewIn(C.Structure):
 
 
 SERIES_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p)
-DLX_COUNTERS = 15
+DLX_COUNTERS = 16
 
 
 # every function include/*.h declares: (name, restype, argtypes)

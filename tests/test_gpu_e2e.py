@@ -169,3 +169,61 @@ def test_progress_and_cancel(native, box):
     with pytest.raises(native.DaylightError) as e:
         native.run("daylight", flags, box, control=c2)
     assert e.value.status == 2 and "cancelled" in str(e.value)
+
+
+def write_epw(path, dni, dhi, lat=52.52, lon=13.41, tz=1.0):
+    rows = ["LOCATION,Test,-,DEU,test,000000,%.2f,%.2f,%.1f,0.0" % (lat, lon, tz), "DESIGN CONDITIONS,0",
+            "TYPICAL/EXTREME PERIODS,0", "GROUND TEMPERATURES,0", "HOLIDAYS/DAYLIGHT SAVINGS,No,0,0,0", "COMMENTS 1,",
+            "COMMENTS 2,", "DATA PERIODS,1,1,Data,Sunday, 1/ 1,12/31"]
+    r = 0
+    for m, nd in enumerate([31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31], 1):
+        for d in range(1, nd + 1):
+            for h in range(1, 25):
+                f = [1999, m, d, h, 0, "?", 10.0, 5.0, 70, 101325, 0, 0, 300, "%.1f" % (dni[r] + dhi[r]), "%.1f" % dni[r],
+                     "%.1f" % dhi[r]] + [0] * 19
+                rows.append(",".join(str(x) for x in f))
+                r += 1
+    with open(path, "w") as fh:
+        fh.write("\n".join(rows) + "\r\n")
+
+
+def test_epw_year_vs_oracle(native, box, orc, box_meshes, tmp_path):
+    """--epw: a cloudy year written as an EPW, read by the library's reader, against the oracle given the same numbers
+    (parsed here from the same text)."""
+    print("case: box, --epw cloudy year", flush=True)
+    dni, dhi = orc.clear_sky(52.52, 13.41, 1.0)
+    rng = np.random.default_rng(3)
+    c = rng.beta(2, 3, 8760).astype(np.float32)
+    dni, dhi = dni * (1 - c), dhi + 0.5 * c * dni
+    path = str(tmp_path / "cloudy.epw")
+    write_epw(path, dni, dhi)
+    back = np.array([[float(x) for x in l.split(",")[14:16]] for l in open(path).read().splitlines()[8:]], np.float32)
+    w = dict(lat=52.52, lon=13.41, tz=1.0, leap=False, dni=back[:, 0].copy(), dhi=back[:, 1].copy())
+    qs = ("radiation", "illuminance", "sunhours")
+    r = native.run("daylight", ["--epw", path, "--quantity", ",".join(qs)], box)
+    o = orc.daylight(box_meshes, w, quantities=qs)["values"]
+    assert np.array_equal(r.values[:, 2], o[:, 2])
+    for c_ in (0, 1):
+        assert rel_err(r.values[:, c_], o[:, c_]) <= 2e-6
+
+
+def test_large_grid_vs_oracle(native, box, orc, box_meshes):
+    """A sensor mesh large enough to be made on several host threads (9,600 quads; both sides, offset) over the box
+    scene as context: every value against the oracle, sun hours exactly."""
+    print("case: 120 x 80 grid over the box scene, both sides", flush=True)
+    nx, ny = 120, 80
+    xs, ys = np.meshgrid(np.linspace(-3, 5, nx + 1), np.linspace(-3, 5, ny + 1), indexing="ij")
+    pts = np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, 0.25)], 1)
+    i = np.arange(nx)[:, None] * (ny + 1) + np.arange(ny)[None, :]
+    quads = np.stack([i, i + ny + 1, i + ny + 2, i + 1], -1).reshape(-1, 4).astype(np.int32)
+    counts = np.full(len(quads), 4, np.int32)
+    grid = native.SimulationMesh("grid", pts, counts, quads.ravel())
+    qs = ("radiation", "sunhours")
+    r = native.run("daylight", ["--clear-sky"] + BERLIN + ["--tz", "1", "--quantity", ",".join(qs), "--sides", "both", "--offset",
+                                                         "0.01"], grid, context=box)
+    g = orc.MeshArrays("grid", pts, counts, quads.ravel(), sensor=True)
+    ctx = [orc.MeshArrays(m.name, m.points, m.counts, m.indices, sensor=False) for m in box_meshes]
+    o = orc.daylight([g] + ctx, clear_weather(orc), quantities=qs, sides="both", offset=0.01)["values"]
+    assert r.values.shape == o.shape == (nx * ny, 4)
+    assert np.array_equal(r.values[:, 2:], o[:, 2:])
+    assert rel_err(r.values[:, :2], o[:, :2]) <= 2e-6
