@@ -101,6 +101,72 @@ def sensor_mesh(boxes):
     return pts, np.full(len(idx), 4, np.int32), idx.ravel().astype(np.int32)
 
 
+def interior(seed=1, side=158.0, wall_h=3.0, tri_budget=2_000_000):
+    """The seeded interior: a rectilinear plan of rooms with sides U(3, 8) m, walls 3 m high tessellated into 0.25 m
+    quads with a door opening (0.9-1.5 m wide, 2.1 m high) in 70 % of the wall segments, no ceiling, and clutter boxes
+    with sides U(0.3, 2) m filling the rest of the triangle budget. -> (points, counts, indices) as quads."""
+    rng = np.random.default_rng(seed)
+
+    def cuts():
+        c = [0.0]
+        while c[-1] < side:
+            c.append(min(side, c[-1] + rng.uniform(3, 8)))
+        return np.array(c)
+
+    xs, ys = cuts(), cuts()
+    pts, quads = [], []
+    base = 0
+
+    def wall(p0, p1, lo, hi):   # vertical rectangle from p0 to p1 (xy), z in [lo, hi], 0.25 m quads
+        nonlocal base
+        length = math.hypot(p1[0] - p0[0], p1[1] - p0[1])
+        nu, nv = max(1, int(math.ceil(length / 0.25))), max(1, int(math.ceil((hi - lo) / 0.25)))
+        p, q = grid_quads(np.array([p0[0], p0[1], lo]), np.array([p1[0] - p0[0], p1[1] - p0[1], 0.0]), np.array([0, 0, hi - lo]), nu, nv)
+        pts.append(p)
+        quads.append(q + base)
+        base += len(p)
+
+    def segment(a, b, axis, at):   # a wall segment along x (axis 0) or y (axis 1) at the other coordinate `at`
+        def xy(u):
+            return (u, at) if axis == 0 else (at, u)
+        if rng.uniform() < 0.7 and b - a > 2.0:
+            w = rng.uniform(0.9, 1.5)
+            d0 = rng.uniform(a + 0.3, b - 0.3 - w)
+            wall(xy(a), xy(d0), 0.0, wall_h)
+            wall(xy(d0 + w), xy(b), 0.0, wall_h)
+            wall(xy(d0), xy(d0 + w), 2.1, wall_h)
+        else:
+            wall(xy(a), xy(b), 0.0, wall_h)
+
+    for y in ys:
+        for a, b in zip(xs[:-1], xs[1:]):
+            segment(a, b, 0, y)
+    for x in xs:
+        for a, b in zip(ys[:-1], ys[1:]):
+            segment(a, b, 1, x)
+    nq = sum(len(q) for q in quads)
+    nbox = max(0, (tri_budget - 2 * nq) // 12)
+    sz = rng.uniform(0.3, 2.0, (nbox, 3))
+    c = rng.uniform(0, side, (nbox, 2))
+    boxes = np.stack([c[:, 0] - sz[:, 0] / 2, c[:, 1] - sz[:, 1] / 2, c[:, 0] + sz[:, 0] / 2, c[:, 1] + sz[:, 1] / 2, sz[:, 2]], 1)
+    bp, bc, bi = box_mesh(boxes)
+    pts.append(bp)
+    quads.append(bi.reshape(-1, 4) + base)
+    P = np.concatenate(pts)
+    Q = np.concatenate(quads)
+    return P, np.full(len(Q), 4, np.int32), Q.ravel().astype(np.int32)
+
+
+def floor_grid(side=158.0, step=0.5, n=100_000):
+    """Sensor quads of `step` m on the floor (z = 0, facing up), n of them from the plan's corner row by row."""
+    per = int(side / step)
+    rows = (n + per - 1) // per
+    p, q = grid_quads(np.array([0.0, 0.0, 0.0]), np.array([per * step, 0, 0]), np.array([0, rows * step, 0]), per, rows)
+    # grid_quads runs the first axis slowest; keep the first n quads
+    q = q[:n]
+    return p, np.full(len(q), 4, np.int32), q.ravel().astype(np.int32)
+
+
 def write_epw(path, dni, dhi):
     rows = ["LOCATION,Synthetic,-,DEU,synthetic,000000,%.2f,%.2f,%.1f,0.0" % (LAT, LON, TZ),
             "DESIGN CONDITIONS,0", "TYPICAL/EXTREME PERIODS,0", "GROUND TEMPERATURES,0",
@@ -165,6 +231,41 @@ def cpu_baseline(ctx, sens, dni, dhi, target_s=15.0):
                       "weather; oracle trace + FP64 sums + FP64 series, %d threads, %.1f s" % (nf, hours, threads, t)}
 
 
+# ---------------------------------------------------------------- the views leg (closest-hit rays and their reductions)
+
+VIEW_ARGS = ["--fov", "360", "--offset", "1.2", "--cell", "2", "--far", "1000", "--jitter", "on", "--distance", "isovist,mean,min",
+             "--reach", "en17037", "--no-occlude", "grid"]
+
+
+def views_leg(native, torch, steps=3):
+    """The seeded interior (~2M triangles) seen all around from a 0.5 m floor grid of 100,000 sensors at eye height
+    1.2 m, 2-degree cells (10,800 rays each): whole views runs through the C ABI on the resident scene."""
+    P, C, I = interior()
+    gp, gc, gi = floor_grid()
+    room = native.SimulationMesh("interior", P, C, I)
+    grid = native.SimulationMesh("grid", gp, gc, gi)
+    r = native.run("views", VIEW_ARGS, grid, context=room)   # builds and keeps the scene
+    n = r.values.shape[0]
+    c0 = native.gpu_counters()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = native.run("views", VIEW_ARGS, grid, context=room)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    c1 = native.gpu_counters()
+    assert r.stats["scene_reused"]
+    rays = n * 10800
+    return {"workload": "seeded interior, %d triangles (rooms U(3,8) m, door openings, clutter U(0.3,2) m), %d floor sensors at "
+                        "0.5 m, eye 1.2 m, 360 x 120 degrees in 2-degree cells, jitter, isovist + mean/min + reach 6/20/50 m"
+                        % (2 * len(C), n),
+            "rays_per_run": rays, "runs": steps, "ms_per_run": round(1e3 * wall, 2),
+            "grays_per_s": round(rays / wall / 1e9, 3),
+            "trace_ms_per_run": round((c1["views_ms"] - c0["views_ms"]) / steps, 3),
+            "reduce_ms_per_run": round((c1["reduce_ms"] - c0["reduce_ms"]) / steps, 3),
+            "columns": r.manifest["columns"]}
+
+
 # ---------------------------------------------------------------- the HBM traffic of the traversal (rocprofv3 --pmc)
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bits_urban_block_traffic.txt")
@@ -192,6 +293,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-methods", action="store_true", help="skip the side-by-side runs of the contraction methods")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal's counted run (roofline achieved = null)")
+    ap.add_argument("--no-views", action="store_true", help="skip the views leg")
+    ap.add_argument("--only-views", action="store_true", help="run the views leg alone (prints its object as the line)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
@@ -212,6 +315,9 @@ def main():
         print(json.dumps({"copy_bytes": a.calibrate_copy << 20, "copies": 5}), flush=True)
         return
     import native
+    if a.only_views:
+        print(json.dumps({"views": views_leg(native, torch)}), flush=True)
+        return
 
     boxes, central = city()
     ctx = box_mesh(boxes)
@@ -294,6 +400,9 @@ def main():
     if methods:
         print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
 
+    views = None if a.no_views else views_leg(native, torch)
+    if views:
+        print("bench: views %s" % json.dumps({k: v for k, v in views.items() if k != "columns"}), file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline(ctx, sens, dni, dhi, a.cpu_seconds)
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -310,6 +419,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "contraction_methods": methods,
+        "views": views,
         "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm",
                      "achieved": None if achieved is None else round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),

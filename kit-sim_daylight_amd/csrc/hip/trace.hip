@@ -62,7 +62,7 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
       if (__builtin_amdgcn_readfirstlane(lane) == lane) ++f.waveIters;   // one per iteration the wave executes
     }
     dlw::Visit v;
-    dlw::visit(wide, cur, sl, 1e-5f, 3.40282347e38f, v);
+    dlw::visit<false>(wide, cur, sl, 1e-5f, 3.40282347e38f, v);
     // the lane's hit leaves, one triangle test per iteration (lanes with fewer wait less than with a block per slot)
     for (uint32_t m = v.leafMask; m; m &= m - 1) {
       const int c = __ffs(m) - 1;
@@ -150,6 +150,7 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     else if (need <= 32) launch_bits<32, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
     else if (need <= 48) launch_bits<48, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
     else if (need <= 64) launch_bits<64, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
+    else if (need <= 96) launch_bits<96, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
     else launch_bits<160, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
   };
   dlg_tbegin(g);

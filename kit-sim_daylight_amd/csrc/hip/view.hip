@@ -43,7 +43,7 @@ __device__ Closest closest_hit(const float4* __restrict__ wide, const float4* __
   int sp = 0;
   for (;;) {
     dlw::Visit v;
-    dlw::visit(wide, cur, sl, tmin, c.t, v);
+    dlw::visit<true>(wide, cur, sl, tmin, c.t, v);
     for (uint32_t m = v.leafMask; m; m &= m - 1) {
       const int k = __ffs(m) - 1;
       leaf_test(r, leaves, k == 0 ? v.leaf[0] : k == 1 ? v.leaf[1] : k == 2 ? v.leaf[2] : v.leaf[3], tmin, cull, c);
@@ -286,6 +286,7 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   else if (need <= 32) k_view_rays<32><<<grid, kWave, 0, g->stream>>>(a);
   else if (need <= 48) k_view_rays<48><<<grid, kWave, 0, g->stream>>>(a);
   else if (need <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
+  else if (need <= 96) k_view_rays<96><<<grid, kWave, 0, g->stream>>>(a);
   else if (need <= 160) k_view_rays<160><<<grid, kWave, 0, g->stream>>>(a);
   else { dlg_fail(err, errLen, "dlg_view: the scene's tree is too deep for the traversal stack"); return 1; }
   DLG_CHECK(hipGetLastError());
@@ -327,6 +328,7 @@ extern "C" int32_t dlg_reduce_view(dlg_gpu* g, int64_t nc, const double* wRow, u
   }
   DLG_CHECK(hipGetLastError());
   g->stats.lastViewReduceMs = dlg_tend(g);
+  g->stats.reduceMs += g->stats.lastViewReduceMs;
   DLG_CHECK(hipMemcpyAsync(out, g->out.p, size_t(nc) * NA * 8, hipMemcpyDeviceToHost, g->stream));
   DLG_CHECK(hipStreamSynchronize(g->stream));
   return 0;

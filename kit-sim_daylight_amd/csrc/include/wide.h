@@ -53,8 +53,8 @@ __device__ __forceinline__ BoxHit slab_hit(const Slab& s, float lx, float ly, fl
   return {t0 <= t1, t0};
 }
 
-// One record visit: leaf slots hit (bit c of leafMask, reference leaf[c]) and the inner slots sorted by entry distance
-// (+inf where not hit), so slot 0 is the nearest hit one.
+// One record visit: leaf slots hit (bit c of leafMask, reference leaf[c]) and the inner slots with their entry distance
+// (+inf where not hit), slot 0 the nearest hit one.
 struct Visit {
   uint32_t leafMask;
   uint32_t leaf[4];
@@ -70,6 +70,9 @@ __device__ __forceinline__ void cx(float& ka, uint32_t& ra, float& kb, uint32_t&
   ka = k0; kb = k1; ra = r0; rb = r1;
 }
 
+// FULL: all four inner slots sorted (closest hit: the pops come nearest first, which prunes); else only the nearest is
+// moved to slot 0 (any hit: the order of the rest does not matter, and three fewer compare-exchanges per visit)
+template <bool FULL>
 __device__ __forceinline__ void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
   const float4* q = wide + 8 * size_t(n);
   const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
@@ -90,8 +93,10 @@ __device__ __forceinline__ void visit(const float4* __restrict__ wide, uint32_t 
   cx(k0, r0, k1, r1);
   cx(k2, r2, k3, r3);
   cx(k0, r0, k2, r2);
-  cx(k1, r1, k3, r3);
-  cx(k1, r1, k2, r2);
+  if (FULL) {
+    cx(k1, r1, k3, r3);
+    cx(k1, r1, k2, r2);
+  }
   v.nInner = (k0 < inf) + (k1 < inf) + (k2 < inf) + (k3 < inf);
   v.inner[0] = r0; v.inner[1] = r1; v.inner[2] = r2; v.inner[3] = r3;
   v.tInner[0] = k0; v.tInner[1] = k1; v.tInner[2] = k2; v.tInner[3] = k3;
