@@ -8,8 +8,8 @@
  *
  * Conventions: status 0 ok, nonzero failed with the message in err[0..errLen) (NUL-terminated); one dlg_gpu per process
  * (the reference's Gpu), made on first use; calls on one dlg_gpu run one at a time. Device buffers belong to the
- * dlg_gpu and grow as needed. Layouts: visibility bits word-major bits[w nc + i]; views classes / distances
- * direction-major [d nc + i]; downloads are transposed to row-major as the reference's are.
+ * dlg_gpu and grow as needed. Layouts: visibility bits word-major bits[w nc + i] (downloads transposed to row-major as
+ * the reference's are); views classes / distances sensor-major [i K + d], the download's layout.
  */
 #ifndef DL_DEVICE_H
 #define DL_DEVICE_H
@@ -119,7 +119,7 @@ typedef struct {
 } dlg_view_in;
 /* gpu_set_labels (internal.h:197): the class of a hit per triangle (input order; 2 obstruction, 3 + g target group g). */
 int32_t dlg_set_labels(dlg_gpu* g, const uint8_t* label, int64_t n, char* err, int32_t errLen);
-/* gpu_view (internal.h:201): classes and distances of sensors [i0, i0 + nc) on the device, direction-major [K x nc]. */
+/* gpu_view (internal.h:201): classes and distances of sensors [i0, i0 + nc) on the device, sensor-major [nc x K]. */
 int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const dlg_view_in* v, char* err, int32_t errLen);
 /* gpu_reduce_view (internal.h:208-209): out[i NA + a], NA = 3 nClass + 1 + nReach + glazing + (isoDim > 0): per class
  * weight, weight x ln d, min d; own surface; reach sums; glazing; isovist area (isoDim 2) or volume (3). */

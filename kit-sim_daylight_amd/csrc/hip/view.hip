@@ -1,9 +1,9 @@
 // SPDX-License-Identifier: Apache-2.0
 // Views on CDNA4: replaces __raygen__view (reference src/trace_dev.cu:177-232), gpu_view (gpu.cu:572-610), k_view /
 // k_iso / gpu_reduce_view (gpu.cu:171-274, 671-697).
-//   k_view_rays  a wave per 64 consecutive sensors and one 32-cell word (grid y): frame, jittered cell, virtual ground,
+//   k_view_rays  a wave per sensor and 8 x 8 tile of cells (grid y, 4 tiles a block): frame, jittered cell, virtual ground,
 //                closest hit through the 4-wide records of the LBVH (exact ties by the lower triangle index), the sensor's own planes passed
-//                (at most 3; the 4th an obstruction there), the hit triangle's label. Direction-major cls / dist.
+//                (at most 3; the 4th an obstruction there), the hit triangle's label. Sensor-major cls / dist [nc x K].
 //   k_view_sums  a wave per sensor, lane l sums directions l, l + 64, ... (class weights, weight x ln d, minima, own
 //                surface, reach, glazing), then a fixed shuffle tree (s = 32 .. 1): the same bits for any chunking.
 //   k_view_iso   a wave per sensor: the isovist's area (one row) or volume (tetrahedra between adjacent rows and
@@ -72,11 +72,13 @@ struct ViewArgs {
   uint8_t* cls; float* dist;
 };
 
+// A block (one wave) per sensor and group of kTiles cell tiles: lane l takes cell (row, col) of an 8 x 8 tile (1 x 64 on
+// a flat field), so a wave's rays leave one origin in neighbouring directions and meet the same part of the tree.
+constexpr uint32_t kTiles = 4;
 template <int STACK>
 __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
   __shared__ uint32_t stk[STACK * kWave];
-  const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
-  if (i >= a.nc) return;
+  const uint32_t lane = threadIdx.x, i = blockIdx.x;
   const uint32_t gi = a.i0 + i, element = gi / a.nS, side = a.nS == 2 ? gi % 2 : a.back;
   const float3 n = a.N[i];
   const float3 o = offset_ray(a.P[i], n);
@@ -99,12 +101,12 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
     }
     u = make_float3(r.y * f.z - r.z * f.y, r.z * f.x - r.x * f.z, r.x * f.y - r.y * f.x);
   }
-  const uint32_t K = a.nAz * a.nEl;
-  for (uint32_t k = 0; k < 32; ++k) {
-    const uint32_t d = w * 32 + k;
-    if (d >= K) break;
-    const uint32_t row = d / a.nAz;
-    const float3 lc = view_cell(d, d % a.nAz, a.rowSin[2 * row], a.rowSin[2 * row + 1], a.az0, a.dAz, a.jitter != 0, element, side);
+  const uint32_t K = a.nAz * a.nEl, th = a.nEl > 1 ? 8u : 1u, tw = kWave / th, tilesAz = (a.nAz + tw - 1) / tw;
+  for (uint32_t tk = 0; tk < kTiles; ++tk) {
+    const uint32_t tile = blockIdx.y * kTiles + tk, row = (tile / tilesAz) * th + lane / tw, col = (tile % tilesAz) * tw + lane % tw;
+    if (row >= a.nEl || col >= a.nAz) continue;
+    const uint32_t d = row * a.nAz + col;
+    const float3 lc = view_cell(d, col, a.rowSin[2 * row], a.rowSin[2 * row + 1], a.az0, a.dAz, a.jitter != 0, element, side);
     const float3 dir = make_float3((lc.x * r.x + lc.y * f.x) + lc.z * u.x, (lc.x * r.y + lc.y * f.y) + lc.z * u.y, (lc.x * r.z + lc.y * f.z) + lc.z * u.z);
     uint8_t cl = 2;
     float t = inf;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
       }
       if (panes) cl |= 0x80;
     }
-    const size_t at = size_t(d) * a.nc + i;
+    const size_t at = size_t(i) * K + d;   // sensor-major
     a.cls[at] = cl;
     a.dist[at] = t;
   }
@@ -166,8 +168,8 @@ __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
   for (uint32_t c = 0; c < kClasses; ++c) { w[c] = 0; wl[c] = 0; mn[c] = __int_as_float(0x7f800000); }
   for (uint32_t r = 0; r < 8; ++r) rc[r] = 0;
   for (uint32_t d = lane; d < a.K; d += kWave) {
-    uint32_t c = a.cls[size_t(d) * a.nc + i];
-    const float t = a.dist[size_t(d) * a.nc + i];
+    uint32_t c = a.cls[size_t(i) * a.K + d];
+    const float t = a.dist[size_t(i) * a.K + d];
     const double wd = a.wRow[d / a.nAz];
     if (c == 254) { own += wd; continue; }
     if (c & 0x80) { gz += wd; c &= 0x7f; }
@@ -218,7 +220,8 @@ __global__ __launch_bounds__(256) void k_view_iso(IsoArgs a) {
   const uint32_t lane = threadIdx.x & (kWave - 1), i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (i >= a.nc) return;
   const uint32_t gi = a.i0 + i, element = gi / a.nS, side = a.nS == 2 ? gi % 2 : a.back;
-  const auto len = [&](uint32_t k) { const size_t at = size_t(k) * a.nc + i; return a.cls[at] == 254 ? 0.f : fminf(a.dist[at], a.farF); };
+  const size_t K = size_t(a.nAz) * a.nEl;
+  const auto len = [&](uint32_t k) { const size_t at = size_t(i) * K + k; return a.cls[at] == 254 ? 0.f : fminf(a.dist[at], a.farF); };
   double sum = 0;
   for (uint32_t base = 0; base < a.nAz; base += kWave) {
     const uint32_t col = base + lane < a.nAz ? base + lane : a.nAz - 1, c1 = col + 1 == a.nAz ? 0 : col + 1;
@@ -258,7 +261,7 @@ extern "C" int32_t dlg_set_labels(dlg_gpu* g, const uint8_t* label, int64_t n, c
 
 extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const dlg_view_in* v, char* err, int32_t errLen) {
   if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_view: sensor range exceeds the scene"); return 1; }
-  const size_t K = size_t(v->nAz) * v->nEl, words = (K + 31) / 32;
+  const size_t K = size_t(v->nAz) * v->nEl;
   const size_t oDist = align256(size_t(nc) * K), oSin = oDist + align256(size_t(nc) * K * 4);
   DLG_CHECK(g->view.grow(oSin + align256(size_t(v->nEl) * 8)));
   char* b = g->view.as<char>();
@@ -278,7 +281,9 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   const double rad = v->north * 3.14159265358979323846 / 180;
   a.cosN = float(std::cos(rad)); a.sinN = float(std::sin(rad));
   a.cls = reinterpret_cast<uint8_t*>(b); a.dist = reinterpret_cast<float*>(b + oDist);
-  const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(words));
+  const uint32_t th = v->nEl > 1 ? 8u : 1u, tw = uint32_t(kWave) / th;
+  const size_t tiles = size_t((v->nEl + th - 1) / th) * ((v->nAz + tw - 1) / tw);
+  const dim3 grid(unsigned(nc), unsigned((tiles + kTiles - 1) / kTiles));
   dlg_tbegin(g);
   // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU)
   const int need = dlw::wide_stack(s.height);
@@ -338,13 +343,9 @@ extern "C" int32_t dlg_download_view(dlg_gpu* g, int64_t nc, uint8_t* cls, float
   if (nc != g->viewN) { dlg_fail(err, errLen, "dlg_download_view: no dlg_view for this chunk"); return 1; }
   const size_t K = size_t(g->viewK), n = size_t(nc) * K;
   if (!n) return 0;
-  std::vector<uint8_t> c(n);
-  std::vector<float> t(n);
-  const char* b = g->view.as<char>();
-  DLG_CHECK(hipMemcpyAsync(c.data(), b, n, hipMemcpyDeviceToHost, g->stream));
-  DLG_CHECK(hipMemcpyAsync(t.data(), b + align256(n), n * 4, hipMemcpyDeviceToHost, g->stream));
+  const char* b = g->view.as<char>();   // sensor-major on the device, as the download is
+  DLG_CHECK(hipMemcpyAsync(cls, b, n, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipMemcpyAsync(dist, b + align256(n), n * 4, hipMemcpyDeviceToHost, g->stream));
   DLG_CHECK(hipStreamSynchronize(g->stream));
-  for (size_t d = 0; d < K; ++d)
-    for (size_t i = 0; i < size_t(nc); ++i) { cls[i * K + d] = c[d * size_t(nc) + i]; dist[i * K + d] = t[d * size_t(nc) + i]; }
   return 0;
 }
