@@ -266,6 +266,56 @@ def views_leg(native, torch, steps=3):
             "columns": r.manifest["columns"]}
 
 
+# ---------------------------------------------------------------- the reference's own --bench runs, through our CLI
+
+BOSTON = ["--clear-sky", "--lat", "42.35", "--lon", "-71.07", "--tz", "-5"]
+# (label, tool arguments, the reference's published figure on an RTX 2060: docs/performance.md)
+REFERENCE_BENCHES = [
+    ("daylight --bench 100000 300000", ["daylight", "--bench", "100000", "300000"] + BOSTON,
+     "2.0 GRays/s, trace 0.24 s, wall 0.6 s (performance.md:7)"),
+    ("daylight --bench 1000000 3000000", ["daylight", "--bench", "1000000", "3000000"] + BOSTON,
+     "2.8 GRays/s, trace 1.7 s, wall 2.6 s (performance.md:8)"),
+    ("daylight --bench 1000000 12000000", ["daylight", "--bench", "1000000", "12000000"] + BOSTON,
+     "1.9 GRays/s, trace 2.5 s, wall 4.2 s (performance.md:9)"),
+    ("daylight --bench 1000000 12000000 --sky-density 2", ["daylight", "--bench", "1000000", "12000000", "--sky-density", "2"] + BOSTON,
+     "1.9 GRays/s, trace 3.0 s, wall 4.8 s (performance.md:10)"),
+    ("views --bench 1000000 10000000", ["views", "--bench", "1000000", "10000000"],
+     "1.98 GRays/s, trace 1.84 s, reduce 0.66 s, run 3.58-3.66 s (performance.md:105)"),
+    ("views --bench 1000000 10000000 --fov 360", ["views", "--bench", "1000000", "10000000", "--fov", "360"],
+     "trace 4.3-4.7 s, reduce 2.35 s, run 8.3 s (performance.md:122-129)"),
+]
+
+
+def reference_benches():
+    """Each of the reference's published --bench runs with this build's CLI (a child process on the same GPU): its
+    printed rays, trace time and run total. The annual Boston EPW of the reference's table is not available here; the
+    Boston clear sky stands in (the same sun steps; radiation + sun hours)."""
+    import re
+    import subprocess
+    lib = os.path.join(ROOT, "kit-sim_daylight_amd", "lib")
+    out = []
+    for label, argv, ref in REFERENCE_BENCHES:
+        t0 = time.perf_counter()
+        p = subprocess.run([os.path.join(lib, argv[0])] + argv[1:], capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        err = p.stderr
+        row = {"command": label, "reference": ref, "exit": p.returncode, "process_s": round(wall, 2)}
+        m = re.search(r"([0-9.e+]+) rays in ([0-9.]+) ms \(([0-9.]+) GRays/s\)", err)
+        if m:
+            row.update(rays=float(m.group(1)), trace_s=round(float(m.group(2)) / 1e3, 3), grays_per_s=float(m.group(3)))
+        m = re.search(r"run ([0-9.]+) s total", err)
+        if m:
+            row["run_s"] = float(m.group(1))
+        m = re.search(r"reduce ([0-9.]+) ms", err)
+        if m:
+            row["reduce_s"] = round(float(m.group(1)) / 1e3, 3)
+        if p.returncode:
+            row["stderr"] = err[-400:]
+        print("bench: %s" % json.dumps(row), file=sys.stderr, flush=True)
+        out.append(row)
+    return out
+
+
 # ---------------------------------------------------------------- the HBM traffic of the traversal (rocprofv3 --pmc)
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bits_urban_block_traffic.txt")
@@ -295,6 +345,7 @@ def main():
     ap.add_argument("--no-count", action="store_true", help="skip the traversal's counted run (roofline achieved = null)")
     ap.add_argument("--no-views", action="store_true", help="skip the views leg")
     ap.add_argument("--only-views", action="store_true", help="run the views leg alone (prints its object as the line)")
+    ap.add_argument("--no-reference-benches", action="store_true", help="skip the reference's --bench runs through the CLI")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
@@ -403,6 +454,7 @@ def main():
     views = None if a.no_views else views_leg(native, torch)
     if views:
         print("bench: views %s" % json.dumps({k: v for k, v in views.items() if k != "columns"}), file=sys.stderr, flush=True)
+    refs = None if a.no_reference_benches else reference_benches()
     cpu = None if a.no_cpu else cpu_baseline(ctx, sens, dni, dhi, a.cpu_seconds)
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -420,6 +472,7 @@ def main():
         "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "contraction_methods": methods,
         "views": views,
+        "reference_benches": refs,
         "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm",
                      "achieved": None if achieved is None else round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),

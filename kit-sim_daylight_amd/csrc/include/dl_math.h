@@ -89,6 +89,12 @@ struct Ray {
   float3 inv, oNear, oFar;
 };
 DL_HD float comp(float3 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : v.z; }
+// the origin's part of the setup: the slab origins moved by the box padding towards the near / far planes
+DL_HD void ray_origin(Ray& r, float3 o, float pad) {
+  r.o = o;
+  r.oNear = make_float3(r.inv.x >= 0.f ? o.x + pad : o.x - pad, r.inv.y >= 0.f ? o.y + pad : o.y - pad, r.inv.z >= 0.f ? o.z + pad : o.z - pad);
+  r.oFar = make_float3(r.inv.x >= 0.f ? o.x - pad : o.x + pad, r.inv.y >= 0.f ? o.y - pad : o.y + pad, r.inv.z >= 0.f ? o.z - pad : o.z + pad);
+}
 DL_HD void ray_setup(Ray& r, float3 o, float3 d, float pad) {
   r.o = o; r.d = d;
   const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
@@ -104,8 +110,7 @@ DL_HD void ray_setup(Ray& r, float3 o, float3 d, float pad) {
   const float dy = d.y != 0.f ? d.y : ((as_uint(d.y) >> 31) ? -1e-30f : 1e-30f);
   const float dzz = d.z != 0.f ? d.z : ((as_uint(d.z) >> 31) ? -1e-30f : 1e-30f);
   r.inv = make_float3(1.f / dx, 1.f / dy, 1.f / dzz);
-  r.oNear = make_float3(r.inv.x >= 0.f ? o.x + pad : o.x - pad, r.inv.y >= 0.f ? o.y + pad : o.y - pad, r.inv.z >= 0.f ? o.z + pad : o.z - pad);
-  r.oFar = make_float3(r.inv.x >= 0.f ? o.x - pad : o.x + pad, r.inv.y >= 0.f ? o.y - pad : o.y + pad, r.inv.z >= 0.f ? o.z - pad : o.z + pad);
+  ray_origin(r, o, pad);
 }
 
 // Hit on [tmin, tmax]: t and front (det > 0: the ray meets the side the winding's normal faces). cull drops back faces.
