@@ -28,10 +28,17 @@ void dlg_destroy(dlg_gpu* g);
 /* the device in use: name, compute units, HBM bytes (for logs and the bench) */
 int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmBytes);
 
-/* gpu_build (internal.h:133): the occluder triangles (vertices v[3 nv], indices tri[3 nt]) as an LBVH (Karras 2012:
- * 63-bit Morton codes of the centroids, radix sort, hierarchy, bottom-up boxes) installed in g. nt == 0: an empty
- * scene (every ray misses). */
+/* gpu_build (internal.h:133): the occluder triangles (vertices v[3 nv], indices tri[3 nt]) as a BVH installed in g:
+ * 63-bit Morton codes of the centroids and a radix sort order the leaves; then either the binned-SAH tree over that
+ * order (DLG_TREE_SAH, the default: 16 centroid bins per axis, surface-area heuristic down to single triangles) or the
+ * LBVH of the codes (DLG_TREE_LBVH: Karras 2012 hierarchy, bottom-up boxes), as dlg_set_tree chose. Both are exact
+ * functions of the input (the oracle builds the same arrays) and give the same ray results. nt == 0: an empty scene
+ * (every ray misses). */
 int32_t dlg_build(dlg_gpu* g, const float* v, int64_t nv, const uint32_t* tri, int64_t nt, char* err, int32_t errLen);
+#define DLG_TREE_SAH 0
+#define DLG_TREE_LBVH 1
+/* The tree the next dlg_build makes (no reference counterpart: OptiX picks its own). */
+void dlg_set_tree(dlg_gpu* g, int32_t tree);
 /* gpu_build_bytes (internal.h:136): device bytes a build of nt triangles over nv vertices peaks at. */
 int64_t dlg_build_bytes(int64_t nv, int64_t nt);
 /* gpu_trim (internal.h:138): frees the per-run buffers (with sensors: the sensors too). */

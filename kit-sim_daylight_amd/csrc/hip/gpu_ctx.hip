@@ -116,6 +116,7 @@ int32_t dlg_scene_download(dlg_gpu* g, float* nodes, int64_t* nNodes, float* lea
 }
 
 void dlg_count_fetches(dlg_gpu* g, int32_t on) { g->countFetches = on ? 1 : 0; }
+void dlg_set_tree(dlg_gpu* g, int32_t tree) { g->tree = tree == DLG_TREE_LBVH ? DLG_TREE_LBVH : DLG_TREE_SAH; }
 
 int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out) {
   *out = g->stats;

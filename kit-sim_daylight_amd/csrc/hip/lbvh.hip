@@ -12,6 +12,8 @@
 // in sorted order, its original index (as bits), 2 unused: three 16-byte loads.
 // The same arrays come out of the host build in the oracle, bit for bit (tests compare them).
 //   6. the 4-wide records the traversals walk (wide.h): one per inner node, from the node and its inner children
+// By default steps 4-5 are replaced by the binned-SAH tree of sah.hip over the sorted leaves (same layout); the LBVH
+// stays for dlg_set_tree(DLG_TREE_LBVH) and for SAH trees deeper than the traversal stacks take.
 #include <hipcub/hipcub.hpp>
 
 #include "gpu_internal.h"
@@ -206,7 +208,8 @@ extern "C" int64_t dlg_build_bytes(int64_t nv, int64_t nt) {
   if (nt <= 0) return 0;
   const int64_t in = nv * 12 + nt * 12, keys = nt * (8 + 4) * 2, cen = nt * 12, tree = nt * (4 * 4 + 24 + 4 + 4);
   const int64_t out = (nt > 1 ? nt - 1 : 1) * (64 + 128) + nt * 48;
-  return in + keys + cen + tree + out + (int64_t(64) << 20);   // + the sort's temp storage (a few MB)
+  // + the sort's temp storage (a few MB) + the SAH build's temporaries (the larger tree build counted for both)
+  return in + keys + cen + tree + out + dlg_sah_bytes(nt) + (int64_t(64) << 20);
 }
 
 extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint32_t* I, int64_t nt, char* err, int32_t errLen) {
@@ -249,6 +252,26 @@ extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint3
                                          int(nt), 0, 64, g->stream), "sort");
   TRY(s.leaves.grow(size_t(nt) * 48), "leaves");
   k_gather<<<blocks(nt, 256), 256, 0, g->stream>>>(dV.as<float>(), dI.as<uint32_t>(), idx2.as<uint32_t>(), nt, s.leaves.as<float>());
+  if (g->tree == DLG_TREE_SAH && nt >= 2) {
+    // the binned-SAH tree over the Morton order (sah.hip); a tree deeper than the traversal stacks take (status 2,
+    // the leaves untouched) falls through to the LBVH
+    TRY(s.nodes.grow(size_t(nn) * 64), "nodes");
+    int32_t h = 0;
+    const int32_t rc = dlg_sah_tree(g, nt, s.leaves.as<float>(), s.nodes.as<float>(), h, err, errLen);
+    if (rc == 1) { cleanup(); return 1; }
+    if (rc == 0) {
+      TRY(s.wide.grow(size_t(nn) * 128), "wide records");
+      k_wide<<<blocks(nn, 256), 256, 0, g->stream>>>(nn, s.nodes.as<float>(), s.wide.as<float4>());
+      TRY(hipGetLastError(), "kernels");
+      TRY(hipStreamSynchronize(g->stream), "build");
+      s.nNodes = nn;
+      s.nTris = nt;
+      s.height = h;
+      g->stats.buildMs += dlg_tend(g);
+      cleanup();
+      return 0;
+    }
+  }
   TRY(L.grow(size_t(nn) * 4), "tree"); TRY(R.grow(size_t(nn) * 4), "tree");
   TRY(pl.grow(size_t(nt) * 4), "tree"); TRY(pi.grow(size_t(nn) * 4), "tree");
   if (nt == 1) {

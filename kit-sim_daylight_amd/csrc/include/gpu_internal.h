@@ -74,7 +74,14 @@ struct dlg_gpu {
   // the traversal's counting variant (dlg_count_fetches) and its two counters
   int32_t countFetches = 0;
   DevBuf counts;
+  // the tree dlg_build makes (dlg_set_tree): DLG_TREE_SAH or DLG_TREE_LBVH
+  int32_t tree = DLG_TREE_SAH;
 };
+
+// sah.hip: the binned-SAH tree over nt >= 2 leaves in Morton order (nodes [nt - 1][16] written, leaves reordered in
+// place), and the device bytes of its temporaries
+int32_t dlg_sah_tree(dlg_gpu* g, int64_t nt, float* leaves, float* nodes, int32_t& height, char* err, int32_t errLen);
+int64_t dlg_sah_bytes(int64_t nt);
 
 // event timing on the library's stream: ms of the work enqueued between begin and end
 inline void dlg_tbegin(dlg_gpu* g) { (void)hipEventRecord(g->ev[0], g->stream); }

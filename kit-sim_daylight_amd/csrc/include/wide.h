@@ -24,7 +24,7 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 struct BoxHit { bool hit; float t; };
 
 // slab test of one box on [tmin, tmax], robust to the box padding through the ray's near / far origins
-__device__ __forceinline__ BoxHit box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
+DL_HD BoxHit box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
   const float nx = r.inv.x >= 0.f ? lx : hx, fx = r.inv.x >= 0.f ? hx : lx;
   const float ny = r.inv.y >= 0.f ? ly : hy, fy = r.inv.y >= 0.f ? hy : ly;
   const float nz = r.inv.z >= 0.f ? lz : hz, fz = r.inv.z >= 0.f ? hz : lz;
@@ -39,12 +39,12 @@ __device__ __forceinline__ BoxHit box_hit(const Ray& r, float lx, float ly, floa
 // the ray) stays far inside the box padding (2^-18 (|coord| + 1)), so the test remains conservative. Per axis the
 // near distance is min(lo's, hi's) and the far one max: with lo <= hi that is the plane the sign selects.
 struct Slab { float3 inv, oLo, oHi; };
-__device__ __forceinline__ Slab slab(const Ray& r) {
+DL_HD Slab slab(const Ray& r) {
   const float3 lo = make_float3(r.inv.x >= 0.f ? r.oNear.x : r.oFar.x, r.inv.y >= 0.f ? r.oNear.y : r.oFar.y, r.inv.z >= 0.f ? r.oNear.z : r.oFar.z);
   const float3 hi = make_float3(r.inv.x >= 0.f ? r.oFar.x : r.oNear.x, r.inv.y >= 0.f ? r.oFar.y : r.oNear.y, r.inv.z >= 0.f ? r.oFar.z : r.oNear.z);
   return {r.inv, make_float3(lo.x * r.inv.x, lo.y * r.inv.y, lo.z * r.inv.z), make_float3(hi.x * r.inv.x, hi.y * r.inv.y, hi.z * r.inv.z)};
 }
-__device__ __forceinline__ BoxHit slab_hit(const Slab& s, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
+DL_HD BoxHit slab_hit(const Slab& s, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
   const float ax = fmaf(lx, s.inv.x, -s.oLo.x), bx = fmaf(hx, s.inv.x, -s.oHi.x);
   const float ay = fmaf(ly, s.inv.y, -s.oLo.y), by = fmaf(hy, s.inv.y, -s.oHi.y);
   const float az = fmaf(lz, s.inv.z, -s.oLo.z), bz = fmaf(hz, s.inv.z, -s.oHi.z);
@@ -63,7 +63,7 @@ struct Visit {
   float tInner[4];
 };
 
-__device__ __forceinline__ void cx(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+DL_HD void cx(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
   const bool s = kb < ka;
   const float k0 = s ? kb : ka, k1 = s ? ka : kb;
   const uint32_t r0 = s ? rb : ra, r1 = s ? ra : rb;
@@ -73,7 +73,7 @@ __device__ __forceinline__ void cx(float& ka, uint32_t& ra, float& kb, uint32_t&
 // FULL: all four inner slots sorted (closest hit: the pops come nearest first, which prunes); else only the nearest is
 // moved to slot 0 (any hit: the order of the rest does not matter, and three fewer compare-exchanges per visit)
 template <bool FULL>
-__device__ __forceinline__ void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
+DL_HD void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
   const float4* q = wide + 8 * size_t(n);
   const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
   const uint4 rf = *reinterpret_cast<const uint4*>(q + 6);
@@ -86,7 +86,7 @@ __device__ __forceinline__ void visit(const float4* __restrict__ wide, uint32_t 
   const bool l0 = (rf.x & 0x80000000u) != 0, l1 = (rf.y & 0x80000000u) != 0, l2 = (rf.z & 0x80000000u) != 0, l3 = (rf.w & 0x80000000u) != 0;
   v.leafMask = (h0.hit && l0 ? 1u : 0u) | (h1.hit && l1 ? 2u : 0u) | (h2.hit && l2 ? 4u : 0u) | (h3.hit && l3 ? 8u : 0u);
   v.leaf[0] = rf.x & 0x7fffffffu; v.leaf[1] = rf.y & 0x7fffffffu; v.leaf[2] = rf.z & 0x7fffffffu; v.leaf[3] = rf.w & 0x7fffffffu;
-  const float inf = __int_as_float(0x7f800000);
+  const float inf = dlm::as_float(0x7f800000u);
   // inner slots hit keep their entry distance (finite: t0 <= t1 <= tmax), the others +inf; then sort
   float k0 = h0.hit && !l0 ? h0.t : inf, k1 = h1.hit && !l1 ? h1.t : inf, k2 = h2.hit && !l2 ? h2.t : inf, k3 = h3.hit && !l3 ? h3.t : inf;
   uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;

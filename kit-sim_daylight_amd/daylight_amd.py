@@ -111,6 +111,7 @@ SEAM = [
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),
     ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
     ("dlg_count_fetches", None, [C.c_void_p, C.c_int32]),
+    ("dlg_set_tree", None, [C.c_void_p, C.c_int32]),
     ("dlg_set_labels", C.c_int32, [C.c_void_p, U8P, C.c_int64, C.c_char_p, C.c_int32]),
     ("dlg_view", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint32, C.POINTER(ViewIn), C.c_char_p, C.c_int32]),
     ("dlg_reduce_view", C.c_int32, [C.c_void_p, C.c_int64, F64P, C.c_uint32, C.c_double, C.c_void_p, C.c_uint32, C.c_int32, C.c_uint32,
@@ -166,6 +167,10 @@ class Gpu:
         if self.h:
             self.L.dlg_destroy(self.h)
             self.h = C.c_void_p()
+
+    def set_tree(self, tree):
+        """'sah' (the default) or 'lbvh': the tree the next build makes."""
+        self.L.dlg_set_tree(self.h, {"sah": 0, "lbvh": 1}[tree])
 
     def build(self, V, I):
         V = np.ascontiguousarray(V, np.float32).reshape(-1)

@@ -108,6 +108,10 @@ def lib():
         L.orc_scene_new.restype = C.c_void_p
         L.orc_scene_new.argtypes = [F32P, C.c_long, U32P, C.c_long]
         L.orc_scene_free.argtypes = [C.c_void_p]
+        L.orc_scene_sah.restype = C.c_int
+        L.orc_scene_sah.argtypes = [C.c_void_p]
+        L.orc_scene_height.restype = C.c_int
+        L.orc_scene_height.argtypes = [C.c_void_p]
         L.orc_scene_nodes.restype = C.c_long
         L.orc_scene_nodes.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
         L.orc_scene_pad.restype = C.c_float

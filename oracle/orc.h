@@ -90,6 +90,11 @@ int orc_visible_brute(const float* V, const uint32_t* I, long nt, const float p[
  * leaf triangles [nt][12] (v0, v1, v2, original index as bits), original index per leaf. Returns the node count. */
 long orc_scene_nodes(const orc_scene* s, const float** node, const float** tri, const uint32_t** prim);
 float orc_scene_pad(const orc_scene* s);
+/* The binned-SAH tree of the MI355X build (sah.hip) replacing the scene's LBVH, leaves reordered: returns its height.
+ * No reference counterpart (OptiX builds its own GAS, gpu.cu:386-457); traversals give the same results over either. */
+int orc_scene_sah(orc_scene* s);
+/* inner nodes on the longest root-to-leaf path (the LBVH's k_refit height) */
+int orc_scene_height(const orc_scene* s);
 /* Morton keys (21 bits per axis of the centroid in the centroid box) and Karras 2012 children of sorted keys. */
 void orc_lbvh_keys(const float* V, const uint32_t* I, long nt, uint64_t* keys);
 void orc_lbvh_topology(const uint64_t* sortedKeys, long n, uint32_t* left, uint32_t* right);

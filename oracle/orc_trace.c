@@ -310,6 +310,180 @@ orc_scene* orc_scene_new(const float* V, long nv, const uint32_t* I, long nt) {
   return s;
 }
 
+/* ---- the binned-SAH tree (the MI355X build's default, kit-sim_daylight_amd/csrc/hip/sah.hip), restated on the host.
+ * No reference counterpart: OptiX builds its own GAS (reference src/gpu.cu:386-457); any tree gives the same ray results.
+ * Rules: a triangle's box by ordered min / max of its vertices, centroid (lo + hi) * 0.5; a range's 16 bins per axis over
+ * its centroid bounds [l, h] (ext = h - l, scale = 16 / ext, axis used when ext > 0 and scale finite, bin = min(15,
+ * int((c - l) scale))); candidates (a, s), s = 0..14, valid when both sides hold triangles, cost A(L) n(L) + A(R) n(R)
+ * with A = (dx dy + dy dz) + dz dx, the least wins, ties to the lower a * 15 + s; none valid: halve (left = n / 2); the
+ * partition is stable; an inner node over leaf positions [b, e) split at m has id m - 1, and the root's id swaps with 0;
+ * leaf k is the k-th triangle of the final order. Starts from the Morton order of orc_scene_new. */
+static uint32_t okey(float f) { const uint32_t u = u_as(f); return (u & 0x80000000u) ? ~u : (u | 0x80000000u); }
+static float okf(uint32_t k) { return f_as((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+static float omin(float a, float b) { return okey(b) < okey(a) ? b : a; }
+static float omax(float a, float b) { return okey(b) > okey(a) ? b : a; }
+
+typedef struct { long b, e; uint32_t slot; int depth; } sah_job;
+
+int orc_scene_sah(orc_scene* s) {
+  const long nt = s->nt;
+  if (nt < 2) return 0;
+  const long nn = nt - 1;
+  float* pb = malloc(sizeof(float) * 6 * (size_t)nt);
+  for (long k = 0; k < nt; ++k) {
+    const float* t = s->tri + 12 * k;
+    for (int a = 0; a < 3; ++a) {
+      pb[6 * k + a] = omin(omin(t[a], t[3 + a]), t[6 + a]);
+      pb[6 * k + 3 + a] = omax(omax(t[a], t[3 + a]), t[6 + a]);
+    }
+  }
+  uint32_t* ref = malloc(sizeof(uint32_t) * (size_t)nt);
+  uint32_t* tmp = malloc(sizeof(uint32_t) * (size_t)nt);
+  for (long k = 0; k < nt; ++k) ref[k] = (uint32_t)k;
+  float* raw = calloc((size_t)nn * 16, sizeof(float));
+  sah_job* st = malloc(sizeof(sah_job) * (size_t)(nt + 1));
+  long sp = 0;
+  st[sp++] = (sah_job){0, nt, 0xffffffffu, 1};
+  uint32_t root = 0;
+  int height = 0;
+  while (sp) {
+    const sah_job J = st[--sp];
+    const long n = J.e - J.b;
+    uint32_t cb[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0, 0, 0};
+    for (long k = J.b; k < J.e; ++k)
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t c = okey((pb[6 * ref[k] + a] + pb[6 * ref[k] + 3 + a]) * 0.5f);
+        if (c < cb[a]) cb[a] = c;
+        if (c > cb[3 + a]) cb[3 + a] = c;
+      }
+    float lo[3], sc[3];
+    int use[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = okf(cb[a]);
+      const float ext = okf(cb[3 + a]) - lo[a];
+      sc[a] = 16.f / ext;
+      use[a] = ext > 0.f && sc[a] < INFINITY;
+    }
+    /* bins[a][q]: lo xyz, hi xyz keys, count */
+    uint32_t bins[3][16][7];
+    for (int a = 0; a < 3; ++a)
+      for (int q = 0; q < 16; ++q)
+        for (int j = 0; j < 7; ++j) bins[a][q][j] = j < 3 ? 0xffffffffu : 0u;
+    for (long k = J.b; k < J.e; ++k) {
+      const float* b = pb + 6 * ref[k];
+      for (int a = 0; a < 3; ++a) {
+        if (!use[a]) continue;
+        int q = (int)(((b[a] + b[3 + a]) * 0.5f - lo[a]) * sc[a]);
+        if (q > 15) q = 15;
+        uint32_t* B = bins[a][q];
+        for (int j = 0; j < 3; ++j) {
+          if (okey(b[j]) < B[j]) B[j] = okey(b[j]);
+          if (okey(b[3 + j]) > B[3 + j]) B[3 + j] = okey(b[3 + j]);
+        }
+        B[6] += 1;
+      }
+    }
+    int bestIdx = -1;
+    uint32_t bestCost = 0;
+    for (int idx = 0; idx < 45; ++idx) {
+      const int a = idx / 15, sIdx = idx % 15;
+      if (!use[a]) continue;
+      uint32_t L[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0, 0, 0}, R[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0, 0, 0};
+      uint32_t nl = 0, nr = 0;
+      for (int q = 0; q < 16; ++q) {
+        uint32_t* X = q <= sIdx ? L : R;
+        for (int j = 0; j < 3; ++j) {
+          if (bins[a][q][j] < X[j]) X[j] = bins[a][q][j];
+          if (bins[a][q][3 + j] > X[3 + j]) X[3 + j] = bins[a][q][3 + j];
+        }
+        if (q <= sIdx) nl += bins[a][q][6]; else nr += bins[a][q][6];
+      }
+      if (!nl || !nr) continue;
+      const float ldx = okf(L[3]) - okf(L[0]), ldy = okf(L[4]) - okf(L[1]), ldz = okf(L[5]) - okf(L[2]);
+      const float rdx = okf(R[3]) - okf(R[0]), rdy = okf(R[4]) - okf(R[1]), rdz = okf(R[5]) - okf(R[2]);
+      const float al = (ldx * ldy + ldy * ldz) + ldz * ldx, ar = (rdx * rdy + rdy * rdz) + rdz * rdx;
+      const float cost = al * (float)nl + ar * (float)nr;
+      if (bestIdx < 0 || u_as(cost) < bestCost) { bestIdx = idx; bestCost = u_as(cost); }
+    }
+    long nL = 0;
+    if (bestIdx < 0) {
+      nL = n / 2;
+    } else {
+      const int a = bestIdx / 15, sIdx = bestIdx % 15;
+      long l = J.b, r = 0;
+      for (long k = J.b; k < J.e; ++k) {
+        const float* b = pb + 6 * ref[k];
+        int q = (int)(((b[a] + b[3 + a]) * 0.5f - lo[a]) * sc[a]);
+        if (q > 15) q = 15;
+        if (q <= sIdx) ref[l++] = ref[k];
+        else tmp[r++] = ref[k];
+      }
+      nL = l - J.b;
+      memcpy(ref + l, tmp, sizeof(uint32_t) * (size_t)r);
+    }
+    const uint32_t id = (uint32_t)(J.b + nL - 1);
+    if (J.slot == 0xffffffffu) root = id;
+    else raw[16 * (size_t)(J.slot >> 1) + 12 + (J.slot & 1u)] = f_as(id);
+    if (J.depth > height) height = J.depth;
+    float* nd = raw + 16 * (size_t)id;
+    for (int side = 0; side < 2; ++side) {
+      const long b = side ? J.b + nL : J.b, e = side ? J.e : J.b + nL;
+      float bl[6] = {0};
+      for (long k = b; k < e; ++k)
+        for (int a = 0; a < 3; ++a) {
+          const float x = pb[6 * ref[k] + a], y = pb[6 * ref[k] + 3 + a];
+          bl[a] = k == b ? x : omin(bl[a], x);
+          bl[3 + a] = k == b ? y : omax(bl[3 + a], y);
+        }
+      memcpy(nd + 6 * side, bl, 24);
+      if (e - b == 1) nd[12 + side] = f_as(0x80000000u | (uint32_t)b);
+      else st[sp++] = (sah_job){b, e, id << 1 | (uint32_t)side, J.depth + 1};
+    }
+  }
+  /* ids to slots: the root and 0 swap */
+#define MAPID(x) ((x) == root ? 0u : (x) == 0u ? root : (x))
+  for (long i = 0; i < nn; ++i) {
+    float* dst = s->node + 16 * (size_t)MAPID((uint32_t)i);
+    memcpy(dst, raw + 16 * (size_t)i, 64);
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t c = u_as(dst[12 + side]);
+      if (!(c & 0x80000000u)) dst[12 + side] = f_as(MAPID(c));
+    }
+    dst[14] = 0.f;
+    dst[15] = 0.f;
+  }
+#undef MAPID
+  float* tri = malloc(sizeof(float) * 12 * (size_t)nt);
+  uint32_t* prim = malloc(sizeof(uint32_t) * (size_t)nt);
+  for (long k = 0; k < nt; ++k) {
+    memcpy(tri + 12 * k, s->tri + 12 * (size_t)ref[k], 48);
+    prim[k] = s->prim[ref[k]];
+  }
+  free(s->tri); free(s->prim);
+  s->tri = tri; s->prim = prim;
+  free(pb); free(ref); free(tmp); free(raw); free(st);
+  return height;
+}
+
+/* the tree's height: inner nodes on the longest root-to-leaf path */
+int orc_scene_height(const orc_scene* s) {
+  if (s->nt < 2) return s->nt == 1 ? 1 : 0;
+  uint32_t* st = malloc(sizeof(uint32_t) * 2 * (size_t)(s->nNode + 2));
+  long sp = 0;
+  int h = 0;
+  st[sp++] = 0; st[sp++] = 1;
+  while (sp) {
+    const uint32_t d = st[--sp], n = st[--sp];
+    if ((int)d > h) h = (int)d;
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t c = u_as(s->node[16 * (size_t)n + 12 + side]);
+      if (!(c & 0x80000000u)) { st[sp++] = c; st[sp++] = d + 1; }
+    }
+  }
+  free(st);
+  return h;
+}
+
 void orc_scene_free(orc_scene* s) {
   if (!s) return;
   free(s->tri); free(s->prim); free(s->node); free(s);

@@ -21,10 +21,12 @@ def gpu():
     g.close()
 
 
-def oracle_scene(orc, V, I):
+def oracle_scene(orc, V, I, tree="lbvh"):
     import ctypes as C
     L = orc.lib()
     sc = L.orc_scene_new(V, len(V) // 3, I, len(I) // 3)
+    if tree == "sah":
+        L.orc_scene_sah(sc)
     node, tri, prim = C.c_void_p(), C.c_void_p(), C.c_void_p()
     nn = L.orc_scene_nodes(sc, C.byref(node), C.byref(tri), C.byref(prim))
     nt = len(I) // 3
@@ -54,17 +56,79 @@ def bits_matrix(bits, nc, nDir):
 
 @pytest.mark.parametrize("nbox", [1, 7, 300, 20000])
 def test_lbvh_bit_exact_vs_host_build(gpu, orc, nbox):
+    print("case: LBVH, %d boxes" % nbox, flush=True)
     rng = np.random.default_rng(nbox)
     t = city(rng, nbox, 10.0 * math.sqrt(nbox) + 20)
     V, I = t.v, t.i
-    gpu.build(V, I)
+    gpu.set_tree("lbvh")
+    try:
+        gpu.build(V, I)
+    finally:
+        gpu.set_tree("sah")
     nodes, leaves, height = gpu.scene()
     sc, onodes, oleaves = oracle_scene(orc, V, I)
+    oh = orc.lib().orc_scene_height(sc)
     orc.lib().orc_scene_free(sc)
     assert nodes.shape == onodes.shape and leaves.shape == oleaves.shape
     assert np.array_equal(nodes.view(np.uint32), onodes.view(np.uint32))
     assert np.array_equal(leaves[:, :10].view(np.uint32), oleaves[:, :10].view(np.uint32))
-    assert 1 <= height <= 160
+    assert height == oh and 1 <= height <= 160
+
+
+def sah_scenes():
+    """Scenes for each path of the SAH build: a small task only (<= 64 triangles), medium tasks (<= 4096), big tasks
+    (chunked levels), and degenerate ranges (identical triangles: no valid plane, halved; a flat grid)."""
+    out = []
+    for nbox in (1, 5, 300, 2000):
+        rng = np.random.default_rng(nbox)
+        out.append(("city %d" % nbox, city(rng, nbox, 10.0 * math.sqrt(nbox) + 20)))
+    t = Tris()
+    for _ in range(150):
+        t.quad((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))   # 300 identical-pair triangles
+    out.append(("identical quads", t))
+    t = Tris()
+    for j in range(60):
+        for i in range(60):
+            t.quad((i, j, 0), (i + 1, j, 0), (i + 1, j + 1, 0), (i, j + 1, 0))
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        x0, y0 = rng.uniform(0, 55, 2)
+        t.box(x0, y0, 0.1, x0 + rng.uniform(1, 5), y0 + rng.uniform(1, 5), rng.uniform(1, 10))
+    out.append(("grid + boxes", t))
+    return out
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_sah_bit_exact_vs_host_build(gpu, orc, k):
+    """The binned-SAH tree (sah.hip) against the oracle's host restatement of the same rules: nodes, leaves and height
+    bit for bit."""
+    name, t = sah_scenes()[k]
+    print("case: SAH, %s (%d triangles)" % (name, len(t.i) // 3), flush=True)
+    V, I = t.v, t.i
+    gpu.build(V, I)
+    nodes, leaves, height = gpu.scene()
+    sc, onodes, oleaves = oracle_scene(orc, V, I, "sah")
+    oh = orc.lib().orc_scene_height(sc)
+    orc.lib().orc_scene_free(sc)
+    assert nodes.shape == onodes.shape and leaves.shape == oleaves.shape
+    assert np.array_equal(nodes.view(np.uint32), onodes.view(np.uint32))
+    assert np.array_equal(leaves[:, :10].view(np.uint32), oleaves[:, :10].view(np.uint32))
+    assert height == oh
+
+
+def test_sah_big_levels_bit_exact(gpu, orc):
+    """More than 4096 triangles per task at the top: the chunked level kernels (bin, choose, count, offsets, scatter)."""
+    print("case: SAH, 20000-box city", flush=True)
+    rng = np.random.default_rng(20000)
+    t = city(rng, 20000, 10.0 * math.sqrt(20000) + 20)
+    gpu.build(t.v, t.i)
+    nodes, leaves, height = gpu.scene()
+    sc, onodes, oleaves = oracle_scene(orc, t.v, t.i, "sah")
+    oh = orc.lib().orc_scene_height(sc)
+    orc.lib().orc_scene_free(sc)
+    assert np.array_equal(nodes.view(np.uint32), onodes.view(np.uint32))
+    assert np.array_equal(leaves[:, :10].view(np.uint32), oleaves[:, :10].view(np.uint32))
+    assert height == oh
 
 
 def test_single_triangle_and_empty_scene(gpu, orc):

@@ -180,3 +180,139 @@ def test_lbvh_structure(orc):
     leaves = kids[kids >= 0x80000000] & 0x7fffffff
     inner = kids[kids < 0x80000000]
     assert sorted(leaves.tolist()) == list(range(nt)) and sorted(inner.tolist()) == list(range(1, nn))
+
+
+def scene_arrays(orc, sc, nt):
+    import ctypes as C
+    node, tri, prim = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nn = orc.lib().orc_scene_nodes(sc, C.byref(node), C.byref(tri), C.byref(prim))
+    nodes = np.ctypeslib.as_array(C.cast(node, C.POINTER(C.c_float)), shape=(nn * 16,)).reshape(nn, 16).copy()
+    tris = np.ctypeslib.as_array(C.cast(tri, C.POINTER(C.c_float)), shape=(nt * 12,)).reshape(nt, 12).copy()
+    prims = np.ctypeslib.as_array(C.cast(prim, C.POINTER(C.c_uint32)), shape=(nt,)).copy()
+    return nodes, tris, prims
+
+
+def sah_scene(orc, t):
+    nt = len(t.i) // 3
+    sc = orc.lib().orc_scene_new(t.v, len(t.v) // 3, t.i, nt)
+    h = orc.lib().orc_scene_sah(sc)
+    return sc, nt, h
+
+
+def test_sah_traversal_equals_brute_force(orc):
+    """Any tree gives the same ray results: the SAH tree's traversal against every triangle, both cull modes."""
+    rng = np.random.default_rng(3)
+    t = random_scene(rng)
+    sc, nt, _ = sah_scene(orc, t)
+    mism = 0
+    for _ in range(4000):
+        p = np.array([rng.uniform(-60, 60), rng.uniform(-60, 60), rng.choice([0.0, rng.uniform(0, 10)])], np.float32)
+        n = np.array([0, 0, 1], np.float32) if rng.uniform() < 0.5 else rng.normal(size=3)
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        for cull in (0, 1):
+            mism += orc.lib().orc_visible(sc, p, n, d, cull) != orc.lib().orc_visible_brute(t.v, t.i, nt, p, n, d, cull)
+    orc.lib().orc_scene_free(sc)
+    assert mism == 0
+
+
+def subtree_leaves(nodes, root=0):
+    """{slot: sorted leaf positions} for every inner node, by a walk from the root."""
+    out, order = {}, []
+    st = [root]
+    while st:
+        x = st.pop()
+        order.append(x)
+        for c in nodes[x, 12:14].copy().view(np.uint32):
+            if c < 0x80000000:
+                st.append(int(c))
+    for x in reversed(order):
+        s = []
+        for c in nodes[x, 12:14].copy().view(np.uint32):
+            s += [int(c & 0x7fffffff)] if c >= 0x80000000 else out[int(c)]
+        out[x] = sorted(s)
+    return out
+
+
+def test_sah_structure(orc):
+    """Every leaf once; a node's two child boxes are exactly its children's boxes (a leaf's: its triangle's min / max);
+    every subtree holds a contiguous run of leaf positions, left before right; the node of a run split at m sits at slot
+    m - 1, the root and slot 0 swapped; the height the build reports is the tree's."""
+    rng = np.random.default_rng(5)
+    t = random_scene(rng, nbox=200)
+    sc, nt, h = sah_scene(orc, t)
+    nodes, tris, prims = scene_arrays(orc, sc, nt)
+    assert h == orc.lib().orc_scene_height(sc)
+    orc.lib().orc_scene_free(sc)
+    nn = nt - 1
+    assert len(nodes) == nn and sorted(prims.tolist()) == list(range(nt))
+    kids = nodes[:, 12:14].copy().view(np.uint32).reshape(-1)
+    assert sorted((kids[kids >= 0x80000000] & 0x7fffffff).tolist()) == list(range(nt))
+    assert sorted(kids[kids < 0x80000000].tolist()) == list(range(1, nn))
+    runs = subtree_leaves(nodes)
+    assert len(runs) == nn and runs[0] == list(range(nt))
+    splits = {}
+    for x, s in runs.items():
+        assert s == list(range(s[0], s[-1] + 1))   # contiguous
+        left = nodes[x, 12:13].copy().view(np.uint32)[0]
+        nl = 1 if left >= 0x80000000 else len(runs[int(left)])
+        splits[x] = s[0] + nl   # the right run starts here
+    root_id = splits[0] - 1
+    for x, m in splits.items():
+        want = 0 if m - 1 == root_id else (root_id if m - 1 == 0 else m - 1)
+        assert x == want
+    tv = tris[:, :9].reshape(nt, 3, 3)
+    for x in range(nn):
+        for side in (0, 1):
+            c = nodes[x, 12 + side:13 + side].copy().view(np.uint32)[0]
+            box = nodes[x, 6 * side:6 * side + 6]
+            if c >= 0x80000000:
+                k = int(c & 0x7fffffff)
+                assert np.array_equal(box[:3], tv[k].min(0)) and np.array_equal(box[3:], tv[k].max(0))
+            else:
+                cb = nodes[int(c)]
+                assert np.array_equal(box[:3], np.minimum(cb[0:3], cb[6:9])) and np.array_equal(box[3:], np.maximum(cb[3:6], cb[9:12]))
+    # the leaves carry their triangles: leaf k = input triangle prims[k]
+    V = t.v.reshape(-1, 3)
+    I = t.i.reshape(-1, 3)
+    assert np.array_equal(tv, V[I[prims]])
+
+
+def test_sah_known_cases(orc):
+    """Two clusters 1 km apart: the root separates them. Identical triangles: no plane splits them, so every range halves
+    (height ceil(log2 n)). Random boxes over a ground grid: less summed node area (the SAH's measure) than the LBVH."""
+    t = Tris()
+    rng = np.random.default_rng(1)
+    for off in (0.0, 1000.0):
+        for _ in range(40):
+            x, y = rng.uniform(0, 5, 2)
+            t.box(off + x, y, 0, off + x + 1, y + 1, 1)
+    sc, nt, _ = sah_scene(orc, t)
+    nodes, tris, _ = scene_arrays(orc, sc, nt)
+    orc.lib().orc_scene_free(sc)
+    runs = subtree_leaves(nodes)
+    left = nodes[0, 12:13].copy().view(np.uint32)[0]
+    lk = runs[int(left)]
+    xs = tris[:, [0, 3, 6]].mean(1)
+    assert len(lk) == nt // 2 and len(set((xs[lk] > 500).tolist())) == 1
+    t = Tris()
+    for _ in range(150):
+        t.quad((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))
+    sc, nt, h = sah_scene(orc, t)
+    orc.lib().orc_scene_free(sc)
+    assert nt == 300 and h == 9
+    t = random_scene(np.random.default_rng(7), nbox=300)
+
+    def area(nodes):
+        b = np.concatenate([nodes[:, 0:6], nodes[:, 6:12]]).astype(np.float64)
+        d = b[:, 3:] - b[:, :3]
+        return float((d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0]).sum())
+
+    nt = len(t.i) // 3
+    sc = orc.lib().orc_scene_new(t.v, len(t.v) // 3, t.i, nt)
+    a_lbvh = area(scene_arrays(orc, sc, nt)[0])
+    orc.lib().orc_scene_sah(sc)
+    a_sah = area(scene_arrays(orc, sc, nt)[0])
+    orc.lib().orc_scene_free(sc)
+    assert a_sah < 0.8 * a_lbvh
