@@ -398,11 +398,14 @@ def main():
     torch.cuda.synchronize()
     c0 = native.gpu_counters()
     lat = []
+    stages = dict.fromkeys(("prep_ms", "sky_ms", "trace_ms", "reduce_ms"), 0.0)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         ts = time.perf_counter()
         r = step()
         lat.append(time.perf_counter() - ts)
+        for k in stages:
+            stages[k] += r.stats[k]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     c1 = native.gpu_counters()
@@ -468,6 +471,7 @@ def main():
                    "sensors": n_faces, "context_triangles": 2 * len(ctx[1]), "patches": 577, "hours": hours,
                    "series_columns": None, "contraction": "bf16x6"},
         "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3), "rays_per_step": rays},
+        "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in stages.items()},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "contraction_methods": methods,

@@ -65,7 +65,10 @@ struct RunOptions {
   int residentScenes = 1;
   const std::atomic<bool>* cancel = nullptr;
   bool verbose = false;
-  size_t chunkBytes = size_t(256) << 20;
+  // caps each per-chunk device buffer (results do not depend on it). 4 GiB here, where the reference's 256 MiB
+  // (daylight.h:85) was sized for 6 GB cards: on a 288 GB MI355X fewer, larger launches leave fewer partly filled waves
+  // at each launch's end; admission still halves it down to 32 MiB when memory is short.
+  size_t chunkBytes = size_t(4) << 30;
   bool f32Columns = false;
   std::atomic<double>* progress = nullptr;
 };

@@ -88,6 +88,9 @@ DL_API int32_t dlx_run_series_device(int32_t argc, const char* const* argv, dl_m
 DL_API int32_t dlx_gpu_counters(double* out, int32_t n);
 /* on != 0: the traversal runs its counting variant (slower) and fills counters [3]-[5]. */
 DL_API void dlx_count_fetches(int32_t on);
+/* bytes > 0: the per-chunk device buffer cap (the C++ Options::chunkBytes) of every following dl_run; 0: the default.
+ * Results do not depend on it; tests use it to run one case in many chunks. */
+DL_API void dlx_chunk_bytes(int64_t bytes);
 
 #ifdef __cplusplus
 }

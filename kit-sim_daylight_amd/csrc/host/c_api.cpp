@@ -73,10 +73,13 @@ int32_t refused(const cli::CommonArgs& a) {
   return DL_OK;
 }
 
+std::atomic<int64_t> chunkOverride{0};   // dlx_chunk_bytes
+
 void set_run(RunOptions& o, const std::string& key, dl_control* control) {
   o.sceneKey = key + (o.selfOcclude ? " self" : " noself");
   o.cancel = control ? &control->cancel : nullptr;
   o.progress = control ? &control->progress : nullptr;
+  if (const int64_t c = chunkOverride.load()) o.chunkBytes = size_t(c);
 }
 
 int32_t finish(Result& r, std::string manifest, dl_result** out) {
@@ -274,6 +277,8 @@ int32_t dlx_gpu_counters(double* out, int32_t n) {
   for (int32_t k = 0; k < n && k < DLX_COUNTERS; ++k) out[k] = c[k];
   return DL_OK;
 }
+
+void dlx_chunk_bytes(int64_t bytes) { chunkOverride.store(bytes > 0 ? bytes : 0); }
 
 void dlx_count_fetches(int32_t on) {
   std::lock_guard<std::mutex> lock(runMu);

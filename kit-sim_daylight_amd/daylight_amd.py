@@ -86,6 +86,7 @@ C_ABI = [
                                           C.c_void_p, C.c_void_p]),
     ("dlx_gpu_counters", C.c_int32, [C.c_void_p, C.c_int32]),
     ("dlx_count_fetches", None, [C.c_int32]),
+    ("dlx_chunk_bytes", None, [C.c_int64]),
 ]
 SEAM = [
     ("dlg_init", C.c_int32, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int32]),

@@ -20,7 +20,7 @@ import numpy as np
 import daylight_amd as _d
 
 __all__ = ["Control", "DaylightError", "SimulationMesh", "Result", "run", "run_series_device", "usage", "gpu_counters",
-           "count_fetches"]
+           "count_fetches", "chunk_bytes"]
 
 # dl_result_stats order (reference bindings/python/daylight/native.py:24-25)
 _STATS = (("scene_reused", bool), ("prep_ms", float), ("sky_ms", float), ("build_ms", float), ("trace_ms", float),
@@ -195,3 +195,8 @@ def gpu_counters():
 def count_fetches(on):
     """Switch the traversal's counting variant on or off (dlx_count_fetches)."""
     _d.lib().dlx_count_fetches(1 if on else 0)
+
+
+def chunk_bytes(n):
+    """The per-chunk device buffer cap of the following runs (dlx_chunk_bytes; 0 restores the default)."""
+    _d.lib().dlx_chunk_bytes(int(n))

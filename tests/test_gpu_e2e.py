@@ -227,3 +227,47 @@ def test_large_grid_vs_oracle(native, box, orc, box_meshes):
     assert r.values.shape == o.shape == (nx * ny, 4)
     assert np.array_equal(r.values[:, 2:], o[:, 2:])
     assert rel_err(r.values[:, :2], o[:, :2]) <= 2e-6
+
+
+def test_chunking_invariance(native, box, tmp_path):
+    """check_gpu.cpp:778-797: the chunk size changes no bit. The 9,600-quad grid (both sides, 19,200 sensors) in ~11
+    chunks against one; the box's series handed to the device sink two sensors at a time against the default."""
+    print("case: chunking invariance (grid, series sink)", flush=True)
+    nx, ny = 120, 80
+    xs, ys = np.meshgrid(np.linspace(-3, 5, nx + 1), np.linspace(-3, 5, ny + 1), indexing="ij")
+    pts = np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, 0.25)], 1)
+    i = np.arange(nx)[:, None] * (ny + 1) + np.arange(ny)[None, :]
+    quads = np.stack([i, i + ny + 1, i + ny + 2, i + 1], -1).reshape(-1, 4).astype(np.int32)
+    grid = native.SimulationMesh("grid", pts, np.full(len(quads), 4, np.int32), quads.ravel())
+    flags = ["--clear-sky"] + BERLIN + ["--tz", "1", "--quantity", "radiation,irradiance,sunhours", "--sides", "both"]
+    one = native.run("daylight", flags, grid, context=box)
+    sflags = ["--clear-sky"] + BERLIN + ["--tz", "1", "--quantity", "radiation", "--from", "03-01T00:00", "--to", "03-08T00:00",
+                                         "--timeseries", str(tmp_path / "s.npy")]
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+
+    def series(calls):
+        out = {}
+
+        def sink(row0, rows, cols, ptr):
+            calls.append(rows)
+            buf = np.empty((rows, cols), np.float32)
+            assert hip.hipMemcpy(buf.ctypes.data, C.c_void_p(ptr), buf.nbytes, 2) == 0
+            out[row0] = buf
+            return 0
+
+        native.run_series_device(sflags, box, sink=sink)
+        return np.concatenate([out[k] for k in sorted(out)])
+
+    calls1, callsN = [], []
+    e1 = series(calls1)
+    try:
+        native.chunk_bytes(1 << 20)
+        many = native.run("daylight", flags, grid, context=box)
+        native.chunk_bytes(e1.shape[1] * 4 * 2)   # two sensors of series per chunk
+        eN = series(callsN)
+    finally:
+        native.chunk_bytes(0)
+    assert np.array_equal(one.values, many.values)
+    assert len(calls1) == 1 and len(callsN) == 8 and set(callsN) == {2}
+    assert np.array_equal(e1, eN)

@@ -90,3 +90,20 @@ def test_views_vs_oracle(native, box, orc, box_meshes, tmp_path, name, flags, kw
     assert np.array_equal(np.load(dump), o["cls"])
     assert np.array_equal(np.load(dump + ".depth.npy").view(np.uint32), o["dist"].view(np.uint32))
     compare(r.values, o["values"], o["columns"])
+
+
+def test_views_chunking_invariance(native, box, tmp_path):
+    """The chunk size changes no bit (views.cpp chunks by the same rule as daylight): the box's default views and the
+    360-degree isovist case in chunks of about two sensors against one chunk, values and every ray."""
+    print("case: views chunking invariance", flush=True)
+    for flags in ([], ["--fov", "360", "--offset", "0.3", "--cell", "4", "--distance", "isovist,mean,min", "--reach", "en17037"]):
+        a = native.run("views", flags + ["--dump-rays", str(tmp_path / "a.npy")], box)
+        try:
+            native.chunk_bytes(40000)
+            b = native.run("views", flags + ["--dump-rays", str(tmp_path / "b.npy")], box)
+        finally:
+            native.chunk_bytes(0)
+        assert np.array_equal(a.values, b.values, equal_nan=True)
+        assert np.array_equal(np.load(tmp_path / "a.npy"), np.load(tmp_path / "b.npy"))
+        assert np.array_equal(np.load(str(tmp_path / "a.npy") + ".depth.npy").view(np.uint32),
+                              np.load(str(tmp_path / "b.npy") + ".depth.npy").view(np.uint32))
