@@ -178,7 +178,7 @@ def write_epw(path, dni, dhi):
         for d in range(1, nd + 1):
             for hr in range(1, 25):
                 ghi = dni[r] + dhi[r]
-                f = [1999, m, d, hr, 0, "?", 10.0, 5.0, 70, 101325, 0, 0, 300, "%.1f" % ghi, "%.1f" % dni[r], "%.1f" % dhi[r]]
+                f = [1999, m, d, hr, 0, "?", 10.0, 5.0, 70, 101325, 0, 0, 300, "%.3f" % ghi, "%.3f" % dni[r], "%.3f" % dhi[r]]
                 f += [0] * 19
                 rows.append(",".join(str(x) for x in f))
                 r += 1
@@ -198,7 +198,9 @@ def weather(native, tmp, seed=2):
     c = np.random.default_rng(seed).beta(2, 3, 8760)
     dni = dni_cs * (1 - c)
     dhi = dhi_cs + 0.5 * c * dni_cs * np.maximum(0, np.sin(np.radians(alt)))
-    return dni.astype(np.float32), dhi.astype(np.float32)
+    # multiples of 1/8 W/m2: exact in binary and in the EPW's decimal text, so the library (reading the file) and the
+    # oracle (given the arrays) see the same numbers
+    return (np.round(dni * 8) / 8).astype(np.float32), (np.round(dhi * 8) / 8).astype(np.float32)
 
 
 # ---------------------------------------------------------------- the CPU baseline (the oracle, on a bounded sample)
@@ -229,6 +231,49 @@ def cpu_baseline(ctx, sens, dni, dhi, target_s=15.0):
     return {"value": nf * hours / t, "unit": "sensor-hours/s", "cores": threads, "kind": "port",
             "sample": "%d of the sensor faces (the first in mesh order), full year (%.0f h), 577 patches, same context and "
                       "weather; oracle trace + FP64 sums + FP64 series, %d threads, %.1f s" % (nf, hours, threads, t)}
+
+
+# ---------------------------------------------------------------- accuracy on the workload itself (the oracle, FP64)
+
+def oracle_sample(ctx, sens, dni, dhi, n=64):
+    """The oracle's FP64 run (annual radiation and hourly series) of the first n sensors of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.build()
+    pts, counts, idx = sens
+    sub = orc.MeshArrays("sensors", pts, counts[:n], idx[: 4 * n], sensor=True)
+    ctx_m = orc.MeshArrays("city", ctx[0], ctx[1], ctx[2], sensor=False)
+    o = orc.daylight([sub, ctx_m], weather=dict(lat=LAT, lon=LON, tz=TZ, leap=False, dni=dni, dhi=dhi), quantities=("radiation",),
+                     mf=2, selfOcclude=False, series=True, nthreads=max(1, min(16, os.cpu_count() or 1)))
+    return o["series"], o["values"][:, 0]
+
+
+def accuracy(native, args, S, Cx, ref, n=64):
+    """One more run of the workload: the device series of its first n sensors (copied out of the sink) and their annual
+    radiation against the oracle's FP64 values. Error = max |x - ref| / max(|ref|, 1e-3 max |ref|) over the sample (the
+    validation scale rule of the reference's validate/README.md:160)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    got = {}
+
+    def sink(row0, rows, cols, ptr):
+        if row0 == 0:
+            buf = np.empty((min(n, rows), cols), np.float32)
+            if hip.hipMemcpy(buf.ctypes.data, C.c_void_p(ptr), buf.nbytes, 2):
+                return 2
+            got["E"] = buf
+        return 0
+
+    r = native.run_series_device(args, sensors=S, context=Cx, sink=sink)
+
+    def rel(x, want):
+        scale = np.maximum(np.abs(want), 1e-3 * np.abs(want).max())
+        return float(np.max(np.abs(x - want) / np.where(scale > 0, scale, 1.0)))
+
+    e_series, e_rad = rel(got["E"].astype(np.float64), ref[0]), rel(r.values[:n, 0], ref[1])
+    return {"sensors": n, "series_max_rel_err": e_series, "radiation_max_rel_err": e_rad, "bar": 2e-6,
+            "within_bar": bool(e_series <= 2e-6 and e_rad <= 2e-6)}
 
 
 # ---------------------------------------------------------------- the views leg (closest-hit rays and their reductions)
@@ -436,6 +481,10 @@ def main():
         waves = k1["wave_iters"] - k0["wave_iters"]
         simd_eff = (k1["node_fetches"] - k0["node_fetches"]) / (64.0 * waves) if waves else None
 
+    ref = oracle_sample(ctx, sens, dni, dhi)
+    acc = accuracy(native, args, S, Cx, ref)
+    print("bench: accuracy %s" % json.dumps(acc), file=sys.stderr, flush=True)
+
     # the contraction methods side by side (two runs each)
     methods = {}
     for name in () if a.no_methods else ("fp32", "bf16x3", "bf16x6"):
@@ -449,7 +498,8 @@ def main():
         tm = (time.perf_counter() - tm) / 2
         m1 = native.gpu_counters()
         methods[name] = {"ms_per_step": round(1e3 * tm, 2), "series_ms_per_step": round((m1["series_ms"] - m0["series_ms"]) / 2, 3),
-                         "sensor_hours_per_s": round(sensor_hours / tm, 1)}
+                         "sensor_hours_per_s": round(sensor_hours / tm, 1),
+                         "series_max_rel_err": accuracy(native, args, S, Cx, ref)["series_max_rel_err"]}
     os.environ.pop("DAYLIGHT_SERIES", None)
     if methods:
         print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
@@ -472,6 +522,7 @@ def main():
                    "series_columns": None, "contraction": "bf16x6"},
         "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3), "rays_per_step": rays},
         "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in stages.items()},
+        "accuracy": acc,
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "contraction_methods": methods,

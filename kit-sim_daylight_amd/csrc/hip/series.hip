@@ -4,11 +4,10 @@
 // coefficients, exact 0 where a direction is blocked), S[row_d, t] the patch radiance of the non-night steps (dlg_sky).
 //   method 0, FP32: each E[i, t] an fma chain over d in ascending order, then fma(G_i, g_t, .), then the sun's fma at its
 //             step: the reference kernel's arithmetic (nvcc contracts it so), tiled 64 x 64 through LDS.
-//   method 1, BF16x3: A and S split into bf16 hi + lo; hi.hi + hi.lo + lo.hi on MFMA (v_mfma_f32_32x32x8 bf16),
-//             FP32 accumulation over blocks of 32 directions, the blocks summed in FP64; epilogue in FP64.
+//   method 1, BF16x3: A and S split into bf16 hi + lo; hi.hi + hi.lo + lo.hi on MFMA (v_mfma_f32_32x32x16 bf16),
+//             FP32 accumulation; epilogue in FP64.
 //   method 2, BF16x6: hi + mid + lo; the six products down to 2^-16 relative, as method 1 otherwise: FP32-level.
-// Layout: A planes [nc x Kp] bf16 row-major (Kp = nSky rounded up to 32, zero padded); S^T planes [Tp x Kp] so that a lane's
-// four k values of an MFMA operand are one 8-byte load for both operands.
+// Operands: fragment-major bf16 planes (Kp = nSky rounded up to 32, zero padded), described at k_split_a.
 #include <vector>
 
 #include "dl_math.h"
@@ -18,6 +17,7 @@ namespace {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes of operand (a native vector: kept in registers)
 
 __device__ __forceinline__ uint16_t bf16_rn(float f) {   // round to nearest even (finite inputs)
   const uint32_t u = __float_as_uint(f);
@@ -92,17 +92,25 @@ __device__ __forceinline__ void split(float x, uint16_t* out) {
   for (int s = 0; s < NS; ++s) { out[s] = bf16_rn(r); r = r - bf16_f(out[s]); }
 }
 
+// Operand layouts: fragment-major, so that an MFMA operand is one 16-byte load per lane and a wave's load one
+// contiguous kilobyte. Planes A [NS][nRG][nKS][64][8] (nRG row groups of 32 sensors, zero padded to the tile) and
+// B = S^T [NS][nCG][nKS][64][8] (nCG column groups of 32 steps); fragment (group g, k-step ks) lane l holds the 8 values
+// of row (or column) 32 g + l % 32 at k = 16 ks + 8 (l / 32) + 0..7, the v_mfma_f32_32x32x16_bf16 operand layout.
+__device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k, uint32_t nKS) {
+  return ((size_t(g) * nKS + k / 16) * 64 + r + 32 * ((k / 8) & 1)) * 8 + (k & 7);
+}
+
 template <int NS>
 __global__ void k_split_a(const uint32_t* __restrict__ bits, uint32_t nc, const float3* __restrict__ N, const float3* __restrict__ D,
-                          uint32_t nSky, uint32_t Kp, const float* __restrict__ wsky, uint16_t* __restrict__ A) {
+                          uint32_t nSky, uint32_t Kp, uint32_t nRG, const float* __restrict__ wsky, uint16_t* __restrict__ A) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-  if (d >= Kp || i >= nc) return;
-  const float c = d < nSky ? coeff(bits, nc, i, d, N[i], D, wsky) : 0.f;
+  if (d >= Kp) return;
+  const float c = (d < nSky && i < nc) ? coeff(bits, nc, i, d, N[i], D, wsky) : 0.f;
   uint16_t p[NS];
   split<NS>(c, p);
-  const size_t plane = size_t(nc) * Kp;
+  const size_t plane = size_t(nRG) * 32 * Kp, at = frag_index(i / 32, i % 32, d, Kp / 16);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) A[s * plane + size_t(i) * Kp + d] = p[s];
+  for (int s = 0; s < NS; ++s) A[s * plane + at] = p[s];
 }
 
 template <int NS>
@@ -113,127 +121,133 @@ __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ r
   const float s = (d < nSky && int(t) < T) ? S[size_t(row[d]) * T + t] : 0.f;
   uint16_t p[NS];
   split<NS>(s, p);
-  const size_t plane = size_t(Tp) * Kp;
+  const size_t plane = size_t(Tp) * Kp, at = frag_index(t / 32, t % 32, d, Kp / 16);
 #pragma unroll
-  for (int k = 0; k < NS; ++k) B[k * plane + size_t(t) * Kp + d] = p[k];
+  for (int k = 0; k < NS; ++k) B[k * plane + at] = p[k];
 }
 
-// v_mfma_f32_32x32x16 bf16 (gfx950) operands: lane l holds 8 values of row l % 32 of A and of column l % 32 of B, in
-// the K slots of lane group l / 32 (the same slots for A and B, so the product sums every k once whichever slot order
-// the unit uses); the result acc[v] is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// v_mfma_f32_32x32x16 bf16 (gfx950): lane l's A operand is row l % 32, B operand column l % 32, both at the k slots of
+// lane group l / 32 (the same slots for A and B, so the product sums every k once); acc[v] is C[8 (v / 4) + 4 (l / 32) +
+// v % 4][l % 32].
 //
-// Block: WM x WN waves; wave (wm, wn) owns 32 sensors x 64 steps (two 32 x 32 MFMA tiles side by side, sharing the A
-// fragment). Per 32-direction slice the block stages A [NS][BM][32] and S^T [NS][BN][32] in LDS (rows padded to 40
-// bf16: 16-byte aligned lane reads spread over the banks), runs the products on MFMA into FP32 and adds the slice's
-// sums into FP64 accumulators.
+// Block: 4 waves over 128 sensors x 128 steps, wave (wm, wn) 64 x 64 = 2 x 2 MFMA tiles (each A fragment serves two
+// tiles, each B fragment two). Per slice of 32 directions the block stages its fragments in LDS - NS x 4 row groups x
+// 2 k-steps of A and the same of B, 48 KB for NS = 3, laid out as in HBM so that both the copy and each wave's reads are
+// contiguous - and loads the next slice into registers while the MFMAs of this one run. FP32 accumulation over all
+// directions (the FP32 method's class of rounding: ~sqrt(nSky) ulps), the epilogue (G g, the sun) in FP64.
 // Blocks are numbered XCD-major: the blocks an XCD runs (hardware id % 8) take consecutive tiles of a grouped order
 // (kGroup rows of sensor tiles per step column), so the tiles that share operand panels share that XCD's L2.
-constexpr int kBK = 32, kLds = 40;
-constexpr uint32_t kGroup = 4;
-template <int NS, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void k_series_mfma(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, uint32_t nc,
-                                                              uint32_t Kp, uint32_t Tp, int T, const uint32_t* __restrict__ bits,
-                                                              const float3* __restrict__ N, const float3* __restrict__ D,
-                                                              const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
-                                                              const int* __restrict__ sunOfCol, const float* __restrict__ sunW,
-                                                              float* __restrict__ E, uint32_t tilesT, uint32_t tiles) {
-  constexpr int BM = 32 * WM, BN = 64 * WN, NT = 64 * WM * WN;
-  __shared__ uint16_t As[NS][BM][kLds];
-  __shared__ uint16_t Bs[NS][BN][kLds];
+constexpr uint32_t kGroup = 4, kTileM = 128, kTileN = 128, kSliceK = 32;
+template <int NS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_series_mfma(const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG,
+                                                     uint32_t nCG, uint32_t nKS, int T, const uint32_t* __restrict__ bits,
+                                                     const float3* __restrict__ N, const float3* __restrict__ D,
+                                                     const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
+                                                     const int* __restrict__ sunOfCol, const float* __restrict__ sunW,
+                                                     float* __restrict__ E, uint32_t tilesT, uint32_t tiles) {
+  constexpr int kFrag = NS * 4 * 2 * 64;          // uint4 per operand per slice: NS planes x 4 groups x 2 k-steps x 64 lanes
+  constexpr int kPer = kFrag / 256;               // per thread
+  __shared__ u32x4 sA[kFrag];
+  __shared__ u32x4 sB[kFrag];
   // XCD-major tile order
   const uint32_t id = blockIdx.x, per = (tiles + 7) / 8, xcd = id % 8, rank = id / 8;
   const uint32_t full = tiles % 8 == 0 ? 8 : tiles % 8;   // XCDs that take `per` tiles; the rest take per - 1
   const uint32_t tile = xcd < full ? xcd * per + rank : full * per + (xcd - full) * (per - 1) + rank;
   if (tile >= tiles || (xcd >= full && rank >= per - 1)) return;
-  // grouped order: kGroup sensor-tile rows walked down before the next step column, so that a group's A panels stay in
-  // L2 while each S^T tile serves kGroup rows
+  // grouped order: kGroup sensor-tile rows walked down before the next step column
   const uint32_t tilesI = tiles / tilesT, grp = tile / (kGroup * tilesT), first = grp * kGroup;
   const uint32_t rows = tilesI - first < kGroup ? tilesI - first : kGroup, local = tile % (kGroup * tilesT);
   const uint32_t ti = first + local % rows, tt = local / rows;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w % WM, wn = w / WM;
-  const uint32_t iBlk = ti * BM, tBlk = tt * BN;
-  const size_t aPlane = size_t(nc) * Kp, bPlane = size_t(Tp) * Kp;
-  const int k8 = 8 * (lane >> 5), r32 = lane & 31;
-  double acc64[2][16];
+  const uint32_t rg0 = ti * 4, cg0 = tt * 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
+  // this thread's share of a slice: entry j = tid + 256 m is plane m / 2, group 2 (m % 2) + tid / 128, uint4 tid % 128
+  // of the group's two k-steps (32-bit offsets: a plane holds fewer than 2^32 / 3 uint4 at any chunk size)
+  const uint32_t gStride = nKS * 64, tpart = (threadIdx.x >> 7) * gStride + (threadIdx.x & 127);
+  const uint32_t aBase = rg0 * gStride + tpart, bBase = cg0 * gStride + tpart, aPl = nRG * gStride, bPl = nCG * gStride;
+  u32x4 pa[kPer], pb[kPer];
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
+  for (int m = 0; m < kPer; ++m) {
+    pa[m] = A[(m / 2) * aPl + (m % 2) * 2 * gStride + aBase];
+    pb[m] = B[(m / 2) * bPl + (m % 2) * 2 * gStride + bBase];
+  }
+  f32x16 acc[2][2];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc64[c][v] = 0.0;
-  for (uint32_t k0 = 0; k0 < Kp; k0 += kBK) {
-    // stage the slice: rows of 32 bf16 = 4 x 16 bytes
-    for (int q = threadIdx.x; q < NS * (BM + BN) * 4; q += NT) {
-      const int s = q / ((BM + BN) * 4), rem = q % ((BM + BN) * 4), r = rem / 4, c = rem % 4;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      uint16_t* dst;
-      if (r < BM) {
-        const uint32_t i = iBlk + r;
-        if (i < nc) v = *reinterpret_cast<const uint4*>(A + s * aPlane + size_t(i) * Kp + k0 + 8 * c);
-        dst = &As[s][r][8 * c];
-      } else {
-        const uint32_t t = tBlk + (r - BM);
-        v = *reinterpret_cast<const uint4*>(B + s * bPlane + size_t(t) * Kp + k0 + 8 * c);
-        dst = &Bs[s][r - BM][8 * c];
-      }
-      *reinterpret_cast<uint4*>(dst) = v;
-    }
+  for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+    for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = f32x16{};
+  const uint32_t nSlices = nKS / 2;
+  for (uint32_t q = 0; q < nSlices; ++q) {
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) { sA[threadIdx.x + 256 * m] = pa[m]; sB[threadIdx.x + 256 * m] = pb[m]; }
     __syncthreads();
-    f32x16 acc[2] = {f32x16{}, f32x16{}};
+    {   // the next slice in flight while this one's MFMAs run (the last iteration reloads its own: no branch)
+      const uint32_t step = 128 * (q + 1 < nSlices ? q + 1 : q);
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 16) {
-      bf16x8 a[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) a[s] = *reinterpret_cast<const bf16x8*>(&As[s][32 * wm + r32][kk + k8]);
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {   // one column tile's fragments live at a time
-        bf16x8 b[NS];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) b[s] = *reinterpret_cast<const bf16x8*>(&Bs[s][64 * wn + 32 * c + r32][kk + k8]);
-        // products whose exponent sum is at most NS - 1 pieces below the leading one, smallest first
-#pragma unroll
-        for (int o = NS - 1; o >= 0; --o)
-#pragma unroll
-          for (int s = 0; s <= o; ++s) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], b[o - s], acc[c], 0, 0, 0);
+      for (int m = 0; m < kPer; ++m) {
+        pa[m] = A[(m / 2) * aPl + (m % 2) * 2 * gStride + aBase + step];
+        pb[m] = B[(m / 2) * bPl + (m % 2) * 2 * gStride + bBase + step];
       }
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int kk = 0; kk < 2; ++kk) {
+      bf16x8 a[2][NS], b[2][NS];
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc64[c][v] += (double)acc[c][v];
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          const u32x4 va = sA[((s * 4 + 2 * wm + f) * 2 + kk) * 64 + lane], vb = sB[((s * 4 + 2 * wn + f) * 2 + kk) * 64 + lane];
+          a[f][s] = __builtin_bit_cast(bf16x8, va);
+          b[f][s] = __builtin_bit_cast(bf16x8, vb);
+        }
+      // products whose exponent sum is at most NS - 1 pieces below the leading one, smallest first
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < 2; ++fn)
+#pragma unroll
+          for (int o = NS - 1; o >= 0; --o)
+#pragma unroll
+            for (int s = 0; s <= o; ++s) acc[fm][fn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[fm][s], b[fn][o - s], acc[fm][fn], 0, 0, 0);
+    }
     __syncthreads();
   }
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const uint32_t t = tBlk + 64 * wn + 32 * c + r32;
+  for (int fn = 0; fn < 2; ++fn) {
+    const uint32_t t = 32 * (cg0 + 2 * wn + fn) + (lane & 31);
     if (int(t) >= T) continue;
     const int j = sunOfCol[t];
     const float gt = g[t];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const uint32_t i = iBlk + 32 * wm + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
-      if (i >= nc) continue;
-      double e = acc64[c][v] + (double)G[i] * (double)gt;
-      if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) e += (double)fmaxf(dlm::dot3(N[i], D[sun0 + uint32_t(j)]), 0.f) * (double)sunW[j];
-      E[size_t(i) * T + t] = (float)e;
-    }
+    for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const uint32_t i = 32 * (rg0 + 2 * wm + fm) + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+        if (i >= nc) continue;
+        double e = (double)acc[fm][fn][v] + (double)G[i] * (double)gt;
+        if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) e += (double)fmaxf(dlm::dot3(N[i], D[sun0 + uint32_t(j)]), 0.f) * (double)sunW[j];
+        E[size_t(i) * T + t] = (float)e;
+      }
   }
 }
 
 template <int NS>
 int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* dG, const float* dg, const int* dRow, const float* dWsky,
                  const int* dSunOfCol, const float* dSunW, char* err, int32_t errLen) {
-  constexpr int WM = 4, WN = 1, BM = 32 * WM, BN = 64 * WN;
-  const uint32_t Kp = (in->nSky + kBK - 1) / kBK * kBK, T = uint32_t(in->T), Tp = (T + BN - 1) / BN * BN;
-  const size_t aBytes = size_t(NS) * nc * Kp * 2, bBytes = size_t(NS) * Tp * Kp * 2;
+  const uint32_t Kp = (in->nSky + kSliceK - 1) / kSliceK * kSliceK, T = uint32_t(in->T);
+  const uint32_t tilesI = (nc + kTileM - 1) / kTileM, tilesT = (T + kTileN - 1) / kTileN;
+  const uint32_t nRG = tilesI * (kTileM / 32), nCG = tilesT * (kTileN / 32), Tp = nCG * 32, nKS = Kp / 16;
+  const size_t aBytes = size_t(NS) * nRG * 32 * Kp * 2, bBytes = size_t(NS) * Tp * Kp * 2;
   DLG_CHECK(g->seriesWork.grow(aBytes + bBytes));
   uint16_t* A = g->seriesWork.as<uint16_t>();
   uint16_t* B = A + aBytes / 2;
   const float3* N = g->N.as<float3>() + g->i0;
-  k_split_a<NS><<<dim3((Kp + 255) / 256, nc), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, N, g->D.as<float3>(), in->nSky, Kp, dWsky, A);
+  k_split_a<NS><<<dim3((Kp + 255) / 256, nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, N, g->D.as<float3>(), in->nSky, Kp, nRG,
+                                                                          dWsky, A);
   k_split_s<NS><<<dim3((Kp + 255) / 256, Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, in->nSky, Kp, in->T, Tp, B);
-  const uint32_t tilesT = Tp / BN, tiles = tilesT * ((nc + BM - 1) / BM);
-  k_series_mfma<NS, WM, WN><<<(tiles + 7) / 8 * 8, 64 * WM * WN, 0, g->stream>>>(A, B, nc, Kp, Tp, in->T, g->bits.as<uint32_t>(), N,
-                                                                               g->D.as<float3>(), dG, dg, in->sun0, dSunOfCol, dSunW,
-                                                                               g->seriesE.as<float>(), tilesT, tiles);
+  const uint32_t tiles = tilesT * tilesI;
+  k_series_mfma<NS><<<(tiles + 7) / 8 * 8, 256, 0, g->stream>>>(reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), nc, nRG,
+                                                                 nCG, nKS, in->T, g->bits.as<uint32_t>(), N, g->D.as<float3>(), dG, dg,
+                                                                 in->sun0, dSunOfCol, dSunW, g->seriesE.as<float>(), tilesT, tiles);
   DLG_CHECK(hipGetLastError());
   return 0;
 }
