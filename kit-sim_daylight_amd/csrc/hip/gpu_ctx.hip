@@ -4,7 +4,21 @@
 #include <cstring>
 #include <string>
 
+#include "dl_math.h"
 #include "gpu_internal.h"
+
+namespace {
+
+// per direction the ray constants every ray along it shares (dl_math.h ray_dir, packed by dir_pack)
+__global__ void k_dir_table(const float3* __restrict__ D, int64_t n, float4* __restrict__ tab) {
+  const int64_t d = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  dlm::Ray r;
+  dlm::ray_dir(r, D[d]);
+  dlm::dir_pack(r, tab[2 * d], tab[2 * d + 1]);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -28,8 +42,8 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
-  for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->bits, &g->scratch, &g->out, &g->skyS, &g->skyG,
-                    &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels})
+  for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -91,7 +105,12 @@ int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens,
     DLG_CHECK(hipMemcpyAsync(g->P.p, P, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
     DLG_CHECK(hipMemcpyAsync(g->N.p, N, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
   }
-  if (nDir) DLG_CHECK(hipMemcpyAsync(g->D.p, D, size_t(nDir) * 12, hipMemcpyHostToDevice, g->stream));
+  if (nDir) {
+    DLG_CHECK(hipMemcpyAsync(g->D.p, D, size_t(nDir) * 12, hipMemcpyHostToDevice, g->stream));
+    DLG_CHECK(g->dirTab.grow(size_t(nDir) * 32));
+    k_dir_table<<<unsigned((nDir + 255) / 256), 256, 0, g->stream>>>(g->D.as<float3>(), nDir, g->dirTab.as<float4>());
+    DLG_CHECK(hipGetLastError());
+  }
   DLG_CHECK(hipStreamSynchronize(g->stream));
   return 0;
 }

@@ -41,7 +41,7 @@ constexpr int kBins = 16, kPlanes = kBins - 1, kCand = 3 * kPlanes;
 constexpr int kBinWords = 3 * kBins * 7;   // per axis and bin: lo x y z, hi x y z (ordered keys), count
 constexpr uint32_t kChunk = 4096, kSmall = 64, kRootSlot = 0xffffffffu;
 constexpr int kBlock = 256;
-constexpr int kMaxHeight = 100;   // the traversal stacks hold wide_stack(100) = 151 <= 160 entries
+constexpr int kMaxHeight = 100;   // the traversal stacks hold wide_stack(100) + 3 = 154 <= 160 entries
 
 struct Task {
   uint32_t b, e;        // leaf positions [b, e)

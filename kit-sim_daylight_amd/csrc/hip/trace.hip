@@ -49,9 +49,10 @@ struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
 // any hit through the 4-wide records (wide.h); the stack holds record indices, nearest child taken first
 template <int STACK, bool COUNT>
 __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restrict__ leaves, float pad, float3 o, float3 nrm, float3 d,
-                         bool cull, uint32_t* __restrict__ stk, int lane, Fetches& f) {
+                         float4 dirA, float4 dirB, bool cull, uint32_t* __restrict__ stk, int lane, Fetches& f) {
   Ray r;
-  ray_setup(r, o, d, pad);
+  dir_unpack(r, d, dirA, dirB);   // the direction's constants, made once per direction (k_dir_table)
+  ray_origin(r, o, pad);
   const dlw::Slab sl = dlw::slab(r);
   OwnSet own;
   uint32_t cur = 0;
@@ -76,11 +77,15 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
       cur = stk[sp * kWave + lane];
       continue;
     }
-    // slot 0 holds the nearest hit inner child; the others are pushed where hit (finite entry distance)
+    // slot 0 holds the nearest hit inner child; the other hit ones (finite entry distance) are pushed by unconditional
+    // writes at positions that skip the misses, and the top moves by the hits (a miss's entry lies above the top, dead;
+    // the stack holds two entries beyond the deepest top for them)
     const float inf = __int_as_float(0x7f800000);
-    if (v.tInner[3] < inf) stk[sp++ * kWave + lane] = v.inner[3];
-    if (v.tInner[2] < inf) stk[sp++ * kWave + lane] = v.inner[2];
-    if (v.tInner[1] < inf) stk[sp++ * kWave + lane] = v.inner[1];
+    const int h1 = v.tInner[1] < inf, h2 = v.tInner[2] < inf, h3 = v.tInner[3] < inf;
+    stk[sp * kWave + lane] = v.inner[1];
+    stk[(sp + h1) * kWave + lane] = v.inner[2];
+    stk[(sp + h1 + h2) * kWave + lane] = v.inner[3];
+    sp += h1 + h2 + h3;
     cur = v.inner[0];
   }
 }
@@ -90,7 +95,7 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
-                                               uint32_t nDir, uint32_t nc, uint32_t cull, uint32_t* __restrict__ bits,
+                                               const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull, uint32_t* __restrict__ bits,
                                                unsigned long long* __restrict__ counts) {
   __shared__ uint32_t stk[STACK * kWave];
   const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
@@ -104,7 +109,8 @@ __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide,
     if (d >= nDir) break;
     const float3 dir = D[d];
     if (n.x * dir.x + n.y * dir.y + n.z * dir.z <= 0.f) continue;   // hemisphere cull (trace_dev.cu:101)
-    if (nTris == 0 || ray_open<STACK, COUNT>(wide, leaves, pad, o, n, dir, cull != 0, stk, int(lane), f)) word |= 1u << k;
+    if (nTris == 0 || ray_open<STACK, COUNT>(wide, leaves, pad, o, n, dir, dirTab[2 * d], dirTab[2 * d + 1], cull != 0, stk, int(lane), f))
+      word |= 1u << k;
   }
   bits[size_t(w) * nc + i] = word;
   if (COUNT) {
@@ -118,7 +124,7 @@ template <int STACK, bool COUNT>
 void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32_t nc, uint32_t cull, unsigned long long* counts) {
   const dlg_scene& s = g->scene;
   k_bits<STACK, COUNT><<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(),
-                                                      uint32_t(g->nDir), nc, cull, g->bits.as<uint32_t>(), counts);
+                                                      g->dirTab.as<float4>(), uint32_t(g->nDir), nc, cull, g->bits.as<uint32_t>(), counts);
 }
 
 }  // namespace
@@ -141,8 +147,9 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     counts = g->counts.as<unsigned long long>();
     DLG_CHECK(hipMemsetAsync(counts, 0, 3 * sizeof(unsigned long long), g->stream));
   }
-  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU)
-  const int need = dlw::wide_stack(s.height);
+  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU), with the two entries
+  // above the top that the unconditional pushes write
+  const int need = dlw::wide_stack(s.height) + 2;
   if (need > 160) { dlg_fail(err, errLen, "dlg_trace: the scene's tree is too deep for the traversal stack"); return 1; }
   const auto launch = [&](auto count) {
     constexpr bool C_ = decltype(count)::value;

@@ -55,9 +55,14 @@ __device__ Closest closest_hit(const float4* __restrict__ wide, const float4* __
       cur = stk[sp * kWave + lane];
       continue;
     }
-    if (v.tInner[3] <= c.t) stk[sp++ * kWave + lane] = v.inner[3];
-    if (v.tInner[2] <= c.t) stk[sp++ * kWave + lane] = v.inner[2];
-    if (v.tInner[1] <= c.t) stk[sp++ * kWave + lane] = v.inner[1];
+    // the sorted slots 1..3 that start within the best hit are a prefix (n of them): pushed farthest first, so that
+    // the nearest is popped next, by unconditional writes; the ones not taken land above the new top, dead (at most
+    // two entries above the deepest top: the stack holds three beyond it)
+    const int n = (v.tInner[1] <= c.t) + (v.tInner[2] <= c.t) + (v.tInner[3] <= c.t);
+    stk[(sp + (n >= 1 ? n - 1 : 0)) * kWave + lane] = v.inner[1];
+    stk[(sp + (n >= 2 ? n - 2 : 1)) * kWave + lane] = v.inner[2];
+    stk[(sp + (n >= 3 ? 0 : 2)) * kWave + lane] = v.inner[3];
+    sp += n;
     cur = v.inner[0];
   }
 }
@@ -285,8 +290,9 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   const size_t tiles = size_t((v->nEl + th - 1) / th) * ((v->nAz + tw - 1) / tw);
   const dim3 grid(unsigned(nc), unsigned((tiles + kTiles - 1) / kTiles));
   dlg_tbegin(g);
-  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU)
-  const int need = dlw::wide_stack(s.height);
+  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU), with the entries above
+  // the top that the unconditional pushes write
+  const int need = dlw::wide_stack(s.height) + 3;
   if (need <= 24) k_view_rays<24><<<grid, kWave, 0, g->stream>>>(a);
   else if (need <= 32) k_view_rays<32><<<grid, kWave, 0, g->stream>>>(a);
   else if (need <= 48) k_view_rays<48><<<grid, kWave, 0, g->stream>>>(a);

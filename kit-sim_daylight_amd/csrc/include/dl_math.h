@@ -95,8 +95,9 @@ DL_HD void ray_origin(Ray& r, float3 o, float pad) {
   r.oNear = make_float3(r.inv.x >= 0.f ? o.x + pad : o.x - pad, r.inv.y >= 0.f ? o.y + pad : o.y - pad, r.inv.z >= 0.f ? o.z + pad : o.z - pad);
   r.oFar = make_float3(r.inv.x >= 0.f ? o.x - pad : o.x + pad, r.inv.y >= 0.f ? o.y - pad : o.y + pad, r.inv.z >= 0.f ? o.z - pad : o.z + pad);
 }
-DL_HD void ray_setup(Ray& r, float3 o, float3 d, float pad) {
-  r.o = o; r.d = d;
+// the direction's part of the setup (shared by every ray along d)
+DL_HD void ray_dir(Ray& r, float3 d) {
+  r.d = d;
   const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   r.kz = ax >= ay ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
   r.kx = r.kz == 2 ? 0 : r.kz + 1;
@@ -110,7 +111,23 @@ DL_HD void ray_setup(Ray& r, float3 o, float3 d, float pad) {
   const float dy = d.y != 0.f ? d.y : ((as_uint(d.y) >> 31) ? -1e-30f : 1e-30f);
   const float dzz = d.z != 0.f ? d.z : ((as_uint(d.z) >> 31) ? -1e-30f : 1e-30f);
   r.inv = make_float3(1.f / dx, 1.f / dy, 1.f / dzz);
+}
+DL_HD void ray_setup(Ray& r, float3 o, float3 d, float pad) {
+  ray_dir(r, d);
   ray_origin(r, o, pad);
+}
+// A direction's part as two float4 (a table built once per direction set): inv xyz, Sx | Sy, Sz, axes kx | ky << 2 |
+// kz << 4 as bits, 0. dir_unpack gives back exactly what ray_dir computed.
+DL_HD void dir_pack(const Ray& r, float4& a, float4& b) {
+  a = make_float4(r.inv.x, r.inv.y, r.inv.z, r.Sx);
+  b = make_float4(r.Sy, r.Sz, as_float(uint32_t(r.kx | r.ky << 2 | r.kz << 4)), 0.f);
+}
+DL_HD void dir_unpack(Ray& r, float3 d, float4 a, float4 b) {
+  r.d = d;
+  r.inv = make_float3(a.x, a.y, a.z);
+  r.Sx = a.w; r.Sy = b.x; r.Sz = b.y;
+  const uint32_t k = as_uint(b.z);
+  r.kx = int(k & 3u); r.ky = int((k >> 2) & 3u); r.kz = int((k >> 4) & 3u);
 }
 
 // Hit on [tmin, tmax]: t and front (det > 0: the ray meets the side the winding's normal faces). cull drops back faces.

@@ -53,8 +53,8 @@ struct dlg_gpu {
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = {};
   dlg_scene scene;
-  // resident sensors and directions
-  DevBuf P, N, D;
+  // resident sensors and directions; per direction its ray constants (dl_math.h dir_pack: 2 float4)
+  DevBuf P, N, D, dirTab;
   int64_t nSens = 0, nDir = 0, nWords = 0;
   int64_t i0 = 0, nc = 0;          // the last traced chunk
   DevBuf bits;                     // [nWords x nc]
