@@ -47,13 +47,15 @@ struct OwnSet {
 struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
 
 // any hit through the 4-wide records (wide.h); the stack holds record indices, nearest child taken first
-template <int STACK, bool COUNT>
+// OFF32: fewer than 2^25 records, walked through uniform bases and 32-bit offsets (wide.h visit32)
+template <int STACK, bool COUNT, bool OFF32>
 __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restrict__ leaves, float pad, float3 o, float3 nrm, float3 d,
                          float4 dirA, float4 dirB, bool cull, uint32_t* __restrict__ stk, int lane, Fetches& f) {
   Ray r;
   dir_unpack(r, d, dirA, dirB);   // the direction's constants, made once per direction (k_dir_table)
   ray_origin(r, o, pad);
   const dlw::Slab sl = dlw::slab(r);
+  const dlw::Bases bs = dlw::bases(wide, sl);
   OwnSet own;
   uint32_t cur = 0;
   int sp = 0;
@@ -63,7 +65,8 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
       if (__builtin_amdgcn_readfirstlane(lane) == lane) ++f.waveIters;   // one per iteration the wave executes
     }
     dlw::Visit v;
-    dlw::visit<false>(wide, cur, sl, 1e-5f, 3.40282347e38f, v);
+    if (OFF32) dlw::visit32<false>(bs, cur, sl, 1e-5f, 3.40282347e38f, v);
+    else dlw::visit<false>(wide, cur, sl, 1e-5f, 3.40282347e38f, v);
     // the lane's hit leaves, one triangle test per iteration (lanes with fewer wait less than with a block per slot)
     for (uint32_t m = v.leafMask; m; m &= m - 1) {
       const int c = __ffs(m) - 1;
@@ -92,7 +95,7 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
 
 // COUNT: also adds the lane's node visits and triangle tests to counts[0] / counts[1] (the algorithmic fetches) and the
 // traversal iterations the wave executed to counts[2] (SIMD efficiency = node visits / (64 x wave iterations))
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool OFF32>
 __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull, uint32_t* __restrict__ bits,
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide,
     if (d >= nDir) break;
     const float3 dir = D[d];
     if (n.x * dir.x + n.y * dir.y + n.z * dir.z <= 0.f) continue;   // hemisphere cull (trace_dev.cu:101)
-    if (nTris == 0 || ray_open<STACK, COUNT>(wide, leaves, pad, o, n, dir, dirTab[2 * d], dirTab[2 * d + 1], cull != 0, stk, int(lane), f))
+    if (nTris == 0 || ray_open<STACK, COUNT, OFF32>(wide, leaves, pad, o, n, dir, dirTab[2 * d], dirTab[2 * d + 1], cull != 0, stk, int(lane), f))
       word |= 1u << k;
   }
   bits[size_t(w) * nc + i] = word;
@@ -123,7 +126,8 @@ __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide,
 template <int STACK, bool COUNT>
 void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32_t nc, uint32_t cull, unsigned long long* counts) {
   const dlg_scene& s = g->scene;
-  k_bits<STACK, COUNT><<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(),
+  const auto k = s.nNodes < (int64_t(1) << 25) ? k_bits<STACK, COUNT, true> : k_bits<STACK, COUNT, false>;
+  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(),
                                                       g->dirTab.as<float4>(), uint32_t(g->nDir), nc, cull, g->bits.as<uint32_t>(), counts);
 }
 

@@ -23,33 +23,27 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 
 struct BoxHit { bool hit; float t; };
 
-// slab test of one box on [tmin, tmax], robust to the box padding through the ray's near / far origins
-DL_HD BoxHit box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
-  const float nx = r.inv.x >= 0.f ? lx : hx, fx = r.inv.x >= 0.f ? hx : lx;
-  const float ny = r.inv.y >= 0.f ? ly : hy, fy = r.inv.y >= 0.f ? hy : ly;
-  const float nz = r.inv.z >= 0.f ? lz : hz, fz = r.inv.z >= 0.f ? hz : lz;
-  const float t0 = fmaxf(fmaxf(fmaxf(tmin, (nx - r.oNear.x) * r.inv.x), (ny - r.oNear.y) * r.inv.y), (nz - r.oNear.z) * r.inv.z);
-  const float t1 = fminf(fminf(fminf(tmax, (fx - r.oFar.x) * r.inv.x), (fy - r.oFar.y) * r.inv.y), (fz - r.oFar.z) * r.inv.z);
-  return {t0 <= t1, t0};
-}
-
-// The ray as the slab test of a record uses it: per axis the inverse direction and the two origins' products with it,
-// the one meeting a slot's lo plane and the one meeting its hi plane (oNear for the near plane, oFar for the far one,
-// by the sign of the direction). A plane's distance is then one fma, (plane inv - o inv); its rounding (2^-24 |o| along
-// the ray) stays far inside the box padding (2^-18 (|coord| + 1)), so the test remains conservative. Per axis the
-// near distance is min(lo's, hi's) and the far one max: with lo <= hi that is the plane the sign selects.
-struct Slab { float3 inv, oLo, oHi; };
+// The ray as the slab test of a record uses it. Per axis the near plane of every box is its lo plane when the direction
+// is positive and its hi plane otherwise (and the far plane the other), so the record's six SoA float4 are read as
+// near x/y/z and far x/y/z by the direction's signs (nOff: the float4 index of axis a's near planes, a or a + 3), and
+// a plane's distance is one fma, plane inv - c: c = oNear inv for near planes, oFar inv for far ones (the ray's padded
+// origins, dl_math.h ray_origin). Its rounding (2^-24 |o| along the ray) stays far inside the box padding (2^-18
+// (|coord| + 1)), so the test remains conservative. Entry and exit are the same numbers as the min / max of the two
+// planes' distances would give (for a box, lo <= hi, the near plane's distance is the smaller one, fma being monotonic).
+struct Slab { float3 inv, cN, cF; int nOff[3]; };
 DL_HD Slab slab(const Ray& r) {
-  const float3 lo = make_float3(r.inv.x >= 0.f ? r.oNear.x : r.oFar.x, r.inv.y >= 0.f ? r.oNear.y : r.oFar.y, r.inv.z >= 0.f ? r.oNear.z : r.oFar.z);
-  const float3 hi = make_float3(r.inv.x >= 0.f ? r.oFar.x : r.oNear.x, r.inv.y >= 0.f ? r.oFar.y : r.oNear.y, r.inv.z >= 0.f ? r.oFar.z : r.oNear.z);
-  return {r.inv, make_float3(lo.x * r.inv.x, lo.y * r.inv.y, lo.z * r.inv.z), make_float3(hi.x * r.inv.x, hi.y * r.inv.y, hi.z * r.inv.z)};
+  Slab s;
+  s.inv = r.inv;
+  s.cN = make_float3(r.oNear.x * r.inv.x, r.oNear.y * r.inv.y, r.oNear.z * r.inv.z);
+  s.cF = make_float3(r.oFar.x * r.inv.x, r.oFar.y * r.inv.y, r.oFar.z * r.inv.z);
+  s.nOff[0] = r.inv.x >= 0.f ? 0 : 3;
+  s.nOff[1] = r.inv.y >= 0.f ? 1 : 4;
+  s.nOff[2] = r.inv.z >= 0.f ? 2 : 5;
+  return s;
 }
-DL_HD BoxHit slab_hit(const Slab& s, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax) {
-  const float ax = fmaf(lx, s.inv.x, -s.oLo.x), bx = fmaf(hx, s.inv.x, -s.oHi.x);
-  const float ay = fmaf(ly, s.inv.y, -s.oLo.y), by = fmaf(hy, s.inv.y, -s.oHi.y);
-  const float az = fmaf(lz, s.inv.z, -s.oLo.z), bz = fmaf(hz, s.inv.z, -s.oHi.z);
-  const float t0 = fmaxf(fmaxf(tmin, fminf(ax, bx)), fmaxf(fminf(ay, by), fminf(az, bz)));
-  const float t1 = fminf(fminf(tmax, fmaxf(ax, bx)), fminf(fmaxf(ay, by), fmaxf(az, bz)));
+DL_HD BoxHit slab_hit(const Slab& s, float nx, float ny, float nz, float fx, float fy, float fz, float tmin, float tmax) {
+  const float t0 = fmaxf(fmaxf(fmaf(nx, s.inv.x, -s.cN.x), fmaf(ny, s.inv.y, -s.cN.y)), fmaxf(fmaf(nz, s.inv.z, -s.cN.z), tmin));
+  const float t1 = fminf(fminf(fmaf(fx, s.inv.x, -s.cF.x), fmaf(fy, s.inv.y, -s.cF.y)), fminf(fmaf(fz, s.inv.z, -s.cF.z), tmax));
   return {t0 <= t1, t0};
 }
 
@@ -71,18 +65,16 @@ DL_HD void cx(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
 }
 
 // FULL: all four inner slots sorted (closest hit: the pops come nearest first, which prunes); else only the nearest is
-// moved to slot 0 (any hit: the order of the rest does not matter, and three fewer compare-exchanges per visit)
+// moved to slot 0 (any hit: the order of the rest does not matter, and three fewer compare-exchanges per visit).
+// nx..fz: the record's near and far plane float4s (slab), rf its references.
 template <bool FULL>
-DL_HD void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
-  const float4* q = wide + 8 * size_t(n);
-  const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
-  const uint4 rf = *reinterpret_cast<const uint4*>(q + 6);
-  BoxHit h0 = slab_hit(sl, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, tmax);
-  BoxHit h1 = slab_hit(sl, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, tmax);
-  BoxHit h2 = slab_hit(sl, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, tmax);
-  BoxHit h3 = slab_hit(sl, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tmin, tmax);
-  // an empty slot (lo +inf, hi -inf) is never followed: excluded by its reference
-  h0.hit = h0.hit && rf.x != kEmpty; h1.hit = h1.hit && rf.y != kEmpty; h2.hit = h2.hit && rf.z != kEmpty; h3.hit = h3.hit && rf.w != kEmpty;
+DL_HD void decide(float4 nx, float4 ny, float4 nz, float4 fx, float4 fy, float4 fz, uint4 rf, const Slab& sl, float tmin, float tmax,
+                  Visit& v) {
+  // an empty slot (lo +inf, hi -inf) is never hit: its entry is +inf and its exit -inf whatever the direction
+  const BoxHit h0 = slab_hit(sl, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, tmin, tmax);
+  const BoxHit h1 = slab_hit(sl, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, tmin, tmax);
+  const BoxHit h2 = slab_hit(sl, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, tmin, tmax);
+  const BoxHit h3 = slab_hit(sl, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, tmin, tmax);
   const bool l0 = (rf.x & 0x80000000u) != 0, l1 = (rf.y & 0x80000000u) != 0, l2 = (rf.z & 0x80000000u) != 0, l3 = (rf.w & 0x80000000u) != 0;
   v.leafMask = (h0.hit && l0 ? 1u : 0u) | (h1.hit && l1 ? 2u : 0u) | (h2.hit && l2 ? 4u : 0u) | (h3.hit && l3 ? 8u : 0u);
   v.leaf[0] = rf.x & 0x7fffffffu; v.leaf[1] = rf.y & 0x7fffffffu; v.leaf[2] = rf.z & 0x7fffffffu; v.leaf[3] = rf.w & 0x7fffffffu;
@@ -100,6 +92,34 @@ DL_HD void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, fl
   v.nInner = (k0 < inf) + (k1 < inf) + (k2 < inf) + (k3 < inf);
   v.inner[0] = r0; v.inner[1] = r1; v.inner[2] = r2; v.inner[3] = r3;
   v.tInner[0] = k0; v.tInner[1] = k1; v.tInner[2] = k2; v.tInner[3] = k3;
+}
+
+// record n, its planes read by the ray's own signs (any lane its own direction)
+template <bool FULL>
+DL_HD void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
+  const float4* q = wide + 8 * size_t(n);
+  decide<FULL>(q[sl.nOff[0]], q[sl.nOff[1]], q[sl.nOff[2]], q[3 - sl.nOff[0]], q[5 - sl.nOff[1]], q[7 - sl.nOff[2]],
+               *reinterpret_cast<const uint4*>(q + 6), sl, tmin, tmax, v);   // axis a's far planes: 2a + 3 - nOff[a]
+}
+
+// The same where every lane of the wave traces one direction: the seven arrays' bases are uniform (scalar registers)
+// and a record is a 32-bit byte offset n * 128 from each (fewer than 2^25 records), so each load is one instruction
+// with no address arithmetic per lane.
+struct Bases { const char* p[7]; };   // near x, y, z, far x, y, z, references
+DL_HD Bases bases(const float4* wide, const Slab& sl) {
+  Bases b;
+  for (int a = 0; a < 3; ++a) {
+    b.p[a] = reinterpret_cast<const char*>(wide + sl.nOff[a]);
+    b.p[3 + a] = reinterpret_cast<const char*>(wide + (2 * a + 3 - sl.nOff[a]));
+  }
+  b.p[6] = reinterpret_cast<const char*>(wide + 6);
+  return b;
+}
+template <bool FULL>
+DL_HD void visit32(const Bases& b, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
+  const uint32_t off = n * 128u;
+  const auto f4 = [&](int k) { return *reinterpret_cast<const float4*>(b.p[k] + off); };
+  decide<FULL>(f4(0), f4(1), f4(2), f4(3), f4(4), f4(5), *reinterpret_cast<const uint4*>(b.p[6] + off), sl, tmin, tmax, v);
 }
 
 // Stack entries a traversal of a tree of the given LBVH height can hold at once: records sit at even depths (at most
