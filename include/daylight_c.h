@@ -91,6 +91,11 @@ DL_API void dlx_count_fetches(int32_t on);
 /* bytes > 0: the per-chunk device buffer cap (the C++ Options::chunkBytes) of every following dl_run; 0: the default.
  * Results do not depend on it; tests use it to run one case in many chunks. */
 DL_API void dlx_chunk_bytes(int64_t bytes);
+/* The weather file a run reads with --epw, as it reads it (no GPU involved): *rows data rows (8760 or 8784), their
+ * DNI / DHI in W/m2 (missing values 0; the first min(rows, cap) written where the arrays are not NULL), and
+ * latLonTz[3] = latitude, longitude, time zone from the LOCATION line when not NULL. DL_RUNTIME with the message on a
+ * file it cannot use. */
+DL_API int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t cap, int64_t* rows, double* latLonTz);
 
 #ifdef __cplusplus
 }

@@ -280,6 +280,23 @@ int32_t dlx_gpu_counters(double* out, int32_t n) {
 
 void dlx_chunk_bytes(int64_t bytes) { chunkOverride.store(bytes > 0 ? bytes : 0); }
 
+int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t cap, int64_t* rows, double* latLonTz) {
+  return guard([&]() -> int32_t {
+    if (!path || !rows) return fail(DL_USAGE, "dlx_read_epw: a NULL path or rows");
+    const CLocale cLocale;
+    Weather w;
+    std::string err;
+    if (!read_epw(path, w, err)) return fail(DL_RUNTIME, err.c_str());
+    *rows = int64_t(w.dni.size());
+    if (latLonTz) { latLonTz[0] = w.lat; latLonTz[1] = w.lon; latLonTz[2] = w.tz; }
+    for (int64_t k = 0; k < *rows && k < cap; ++k) {
+      if (dni) dni[k] = w.dni[size_t(k)];
+      if (dhi) dhi[k] = w.dhi[size_t(k)];
+    }
+    return DL_OK;
+  });
+}
+
 void dlx_count_fetches(int32_t on) {
   std::lock_guard<std::mutex> lock(runMu);
   std::string err;

@@ -7,6 +7,10 @@
 #include <charconv>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#if defined(__SSE2__)
+#include <emmintrin.h>
+#endif
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -225,13 +229,30 @@ bool clear_sky_weather(double lat, double lon, double tz, double turbidity, doub
   return true;
 }
 
-bool read_epw(const std::string& path, Weather& w, std::string& err) {
-  std::ifstream f(path);
-  if (!f) { err = "cannot open EPW " + path; return false; }
-  return read_epw(f, path, w, err);
+namespace {
+
+// The first k commas of [p, eol) (their addresses into out): 16 bytes at a time where the line has them, so that no
+// character costs a branch. Returns how many were found (<= k).
+size_t first_commas(const char* p, const char* eol, const char** out, size_t k) {
+  size_t n = 0;
+  const char* c = p;
+#if defined(__SSE2__)
+  const __m128i comma = _mm_set1_epi8(',');
+  for (; c + 16 <= eol && n < k; c += 16) {
+    unsigned m = unsigned(_mm_movemask_epi8(_mm_cmpeq_epi8(_mm_loadu_si128(reinterpret_cast<const __m128i*>(c)), comma)));
+    for (; m && n < k; m &= m - 1) out[n++] = c + __builtin_ctz(m);
+  }
+#endif
+  for (; c < eol && n < k; ++c)
+    if (*c == ',') out[n++] = c;
+  return n;
 }
 
-bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string& err) {
+// The EPW text in memory: the 8 header lines, then the data rows, scanned in place (columns 13-15: GHI, DNI, DHI, read
+// as atof reads a column).
+bool parse_epw(const char* data, size_t size, const std::string& label, Weather& w, std::string& err) {
+  const char* p = data;
+  const char* const end = data + size;
   std::string line;
   std::vector<std::string> cols;
   const auto split = [&](const std::string& l) {
@@ -241,37 +262,53 @@ bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string
   };
   bool hdrLeap = false, pvgis = false;
   double declared = NAN;
-  for (int i = 0; i < 8; ++i) {
-    if (!std::getline(f, line)) { err = "EPW header truncated: " + label; return false; }
+  for (int i = 0; i < 8; ++i) {   // getline's lines: up to '\n' (not kept); the last one may end at the buffer's end
+    if (p >= end) { err = "EPW header truncated: " + label; return false; }
+    const char* eol = static_cast<const char*>(std::memchr(p, '\n', size_t(end - p)));
+    line.assign(p, eol ? eol : end);
+    p = eol ? eol + 1 : end;
     split(line);
     if (i == 0) {
       if (cols.size() < 10 || cols[0] != "LOCATION") { err = "EPW: bad LOCATION line: " + label; return false; }
       w.lat = std::atof(cols[6].c_str()); w.lon = std::atof(cols[7].c_str()); w.tz = std::atof(cols[8].c_str());
     }
     if (i == 4 && cols.size() > 1 && cols[1] == "Yes") hdrLeap = true;
-    if (const size_t p = line.find("Irradiance Time Offset"); p != std::string::npos) {
+    if (const size_t q = line.find("Irradiance Time Offset"); q != std::string::npos) {
       pvgis = true;
-      if (const size_t c = line.find(':', p); c != std::string::npos) declared = std::atof(line.c_str() + c + 1);
+      if (const size_t c = line.find(':', q); c != std::string::npos) declared = std::atof(line.c_str() + c + 1);
     }
   }
   w.dni.clear(); w.dhi.clear(); w.missing = 0; w.shiftH = 0;
   w.dni.reserve(8784);
   w.dhi.reserve(8784);
   std::vector<float> ghi;
-  // the data rows in one buffer, scanned in place: columns 13-15 (GHI, DNI, DHI) read as atof reads a column
-  std::string rest;
-  {
-    char buf[1 << 16];
-    while (f.read(buf, sizeof buf) || f.gcount() > 0) rest.append(buf, size_t(f.gcount()));
-  }
-  const char* p = rest.data();
-  const char* const end = p + rest.size();
   std::string field;
-  // atof of a field: from_chars (correctly rounded, as glibc's strtod) where it takes the whole number, else atof itself
+  // atof of a field. A plain decimal ([-]digits[.digits], at most 15 digits, then the field's end) is exactly
+  // mantissa / 10^k with both parts exact in double, so one correctly rounded division gives what strtod gives (Clinger
+  // 1990's fast path); anything else goes to from_chars (correctly rounded, as glibc's strtod) where it takes the whole
+  // number, else to atof itself.
+  static const double kPow10[16] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10, 1e11, 1e12, 1e13, 1e14, 1e15};
   const auto number = [&](const char* a, const char* b) {
     const char* c = a;
     while (c < b && (*c == ' ' || *c == '\t')) ++c;
     if (c < b && *c == '+' && c + 1 < b && *(c + 1) != '-' && *(c + 1) != '+') ++c;
+    {
+      const char* d = c;
+      const bool neg = d < b && *d == '-';
+      if (neg) ++d;
+      uint64_t m = 0;
+      int digits = 0, frac = 0;
+      bool dot = false, ok = d < b;
+      for (; d < b && *d != '\r'; ++d) {
+        if (*d >= '0' && *d <= '9') { m = m * 10 + uint64_t(*d - '0'); ++digits; frac += dot; }
+        else if (*d == '.' && !dot) dot = true;
+        else { ok = false; break; }
+      }
+      if (ok && digits > 0 && digits <= 15 && (d == b || *d == '\r')) {
+        const double x = double(m) / kPow10[frac];
+        return neg ? -x : x;
+      }
+    }
     double x = 0;
     const auto r = std::from_chars(c, b, x);
     if (r.ec == std::errc() && (r.ptr == b || *r.ptr == '\r' || *r.ptr == ' ' || *r.ptr == '\t')) return x;
@@ -284,15 +321,16 @@ bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string
     const char* q = p;
     while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
     if (q == eol) { p = eol + 1; continue; }
-    // field starts; the field count is a getline split's: commas + 1, less a trailing empty field
-    const char* at[16];
-    size_t n = 1;
+    // field starts up to the 17th: the field count (a getline split's: commas + 1, less a trailing empty field) only
+    // has to reach 16, which 16 commas settle; short rows are counted out in full
+    const char* cm[16];
+    const char* at[17];
+    size_t n = 1 + first_commas(p, eol, cm, 16);
     at[0] = p;
-    for (const char* c = p; c < eol; ++c)
-      if (*c == ',') { if (n < 16) at[n] = c + 1; ++n; }
-    if (eol[-1] == ',') --n;
+    for (size_t k = 1; k < n; ++k) at[k] = cm[k - 1] + 1;
+    if (n < 17 && eol > p && eol[-1] == ',') --n;
     if (n < 16) { err = "EPW: short data row " + std::to_string(w.dni.size() + 1) + ": " + label; return false; }
-    const auto colEnd = [&](int k) { const char* c = static_cast<const char*>(std::memchr(at[k], ',', size_t(eol - at[k]))); return c ? c : eol; };
+    const auto colEnd = [&](size_t k) { return k + 1 < n ? at[k + 1] - 1 : eol; };
     double dni = number(at[14], colEnd(14)), dhi = number(at[15], colEnd(15));
     if (dni >= 9999 || dhi >= 9999) { ++w.missing; if (dni >= 9999) dni = 0; if (dhi >= 9999) dhi = 0; }
     w.dni.push_back(float(dni));
@@ -308,6 +346,34 @@ bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string
   }
   if (pvgis) pvgis_align(w, ghi, declared);
   return true;
+}
+
+}  // namespace
+
+bool read_epw(const std::string& path, Weather& w, std::string& err) {
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path.c_str(), "rb"), std::fclose);
+  if (!f) { err = "cannot open EPW " + path; return false; }
+  std::string data;
+  if (std::fseek(f.get(), 0, SEEK_END) == 0) {   // one read of the whole file
+    const long size = std::ftell(f.get());
+    if (size > 0 && std::fseek(f.get(), 0, SEEK_SET) == 0) {
+      data.resize(size_t(size));
+      data.resize(std::fread(data.data(), 1, data.size(), f.get()));
+    }
+  }
+  if (data.empty()) {   // not seekable: read it in pieces
+    std::rewind(f.get());
+    char buf[1 << 16];
+    for (size_t k; (k = std::fread(buf, 1, sizeof buf, f.get())) > 0;) data.append(buf, k);
+  }
+  return parse_epw(data.data(), data.size(), path, w, err);
+}
+
+bool read_epw(std::istream& f, const std::string& label, Weather& w, std::string& err) {
+  std::string data;
+  char buf[1 << 16];
+  while (f.read(buf, sizeof buf) || f.gcount() > 0) data.append(buf, size_t(f.gcount()));
+  return parse_epw(data.data(), data.size(), label, w, err);
 }
 
 }  // namespace daylight

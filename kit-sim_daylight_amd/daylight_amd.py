@@ -87,6 +87,7 @@ C_ABI = [
     ("dlx_gpu_counters", C.c_int32, [C.c_void_p, C.c_int32]),
     ("dlx_count_fetches", None, [C.c_int32]),
     ("dlx_chunk_bytes", None, [C.c_int64]),
+    ("dlx_read_epw", C.c_int32, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
 ]
 SEAM = [
     ("dlg_init", C.c_int32, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int32]),

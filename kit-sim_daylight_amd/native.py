@@ -20,7 +20,7 @@ import numpy as np
 import daylight_amd as _d
 
 __all__ = ["Control", "DaylightError", "SimulationMesh", "Result", "run", "run_series_device", "usage", "gpu_counters",
-           "count_fetches", "chunk_bytes"]
+           "count_fetches", "chunk_bytes", "read_epw"]
 
 # dl_result_stats order (reference bindings/python/daylight/native.py:24-25)
 _STATS = (("scene_reused", bool), ("prep_ms", float), ("sky_ms", float), ("build_ms", float), ("trace_ms", float),
@@ -200,3 +200,13 @@ def count_fetches(on):
 def chunk_bytes(n):
     """The per-chunk device buffer cap of the following runs (dlx_chunk_bytes; 0 restores the default)."""
     _d.lib().dlx_chunk_bytes(int(n))
+
+
+def read_epw(path):
+    """The weather file as a --epw run reads it (dlx_read_epw): (dni, dhi) float32 arrays and (lat, lon, tz)."""
+    L = _d.lib()
+    n = C.c_int64()
+    _check(L.dlx_read_epw(os.fsencode(path), None, None, 0, C.byref(n), None))
+    dni, dhi, loc = np.zeros(n.value, np.float32), np.zeros(n.value, np.float32), np.zeros(3)
+    _check(L.dlx_read_epw(os.fsencode(path), dni.ctypes.data, dhi.ctypes.data, n.value, C.byref(n), loc.ctypes.data))
+    return dni, dhi, tuple(loc.tolist())
