@@ -3,50 +3,71 @@
 // (reference src/gpu.cu:99-114, 612-625), which ran once per quantity and re-read the bits each time.
 // Each sensor's sum is the reference's: acc_q += double(float(max(0, dotf(n, D_d)))) x double(W[q][d]) over its visible
 // directions in ascending order (sun hours: the float 1 in place of the cosine), in FP64, so the result is the same bits
-// for any chunking. The order makes the sum a dependent chain; the work around it is spread over a wave:
-//   a wave (one block) per sensor; per slice of 64 directions the lanes of the visible ones form their products and
-//   store them compacted in direction order (position = visible lanes below), then lane q adds the slice's products of
-//   quantity q in that order.
-// HBM: 4 nWords bytes of bits + 24 (P, N) + 8 nQ per sensor; directions and weights stay in L2.
+// for any chunking. The order makes each sum a dependent chain; the chains of different sensors run side by side:
+//   a lane per sensor; every lane of a wave walks the directions together (the direction and its weights are the same
+//   for all: staged in LDS per chunk of 256 directions and read as broadcasts), reads its visibility word once per 32
+//   directions (one coalesced load per wave) and adds the product where its bit is set. Words that no lane of the wave
+//   sees through are skipped.
+// HBM: 4 nWords bytes of bits + 12 (N) + 8 nQ per sensor; directions and weights stay in the scalar cache / L2.
 #include "dl_math.h"
 #include "gpu_internal.h"
 
 namespace {
 
-constexpr int kMaxQ = 8, kWave = 64;
+constexpr int kMaxQ = 8, kBlock = 256;
 
 template <int NQ>
-__global__ __launch_bounds__(kWave) void k_reduce(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
-                                                  const float3* __restrict__ D, const float* __restrict__ W, uint32_t nDir,
-                                                  uint32_t cosMask, double* __restrict__ out) {
-  __shared__ double prod[NQ][kWave];
-  const uint32_t lane = threadIdx.x, i = blockIdx.x;
-  const float3 n = N[i];
-  double acc = 0.0;          // lane q < NQ: quantity q
-  for (uint32_t base = 0; base < nDir; base += kWave) {
-    const uint32_t d = base + lane;
-    const bool vis = d < nDir && ((bits[size_t(d >> 5) * nc + i] >> (d & 31)) & 1u);
-    const uint64_t m = __ballot(vis);
-    if (vis) {
-      const uint32_t at = __popcll(m & ((uint64_t(1) << lane) - 1));
-      const float c = fmaxf(dlm::dot3(n, D[d]), 0.f);
+__global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
+                                                   const float3* __restrict__ D, const float* __restrict__ W, uint32_t nDir,
+                                                   uint32_t cosMask, double* __restrict__ out) {
+  // chunks of 8 words: their 256 directions and weights staged in LDS (every lane reads the same entry: a broadcast),
+  // the 8 visibility words of the lane loaded up front
+  constexpr uint32_t kChunkW = 8, kChunkD = kChunkW * 32;
+  __shared__ float sD[kChunkD * 3];
+  __shared__ float sW[NQ][kChunkD];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < nc;
+  const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+  double acc[NQ];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) prod[q][at] = (double)((cosMask >> q) & 1u ? c : 1.f) * (double)W[size_t(q) * nDir + d];
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  for (uint32_t w0 = 0; w0 < nWords; w0 += kChunkW) {
+    const uint32_t d0 = w0 * 32, dn = nDir - d0 < kChunkD ? nDir - d0 : kChunkD;
+    uint32_t words[kChunkW];
+#pragma unroll
+    for (uint32_t j = 0; j < kChunkW; ++j) words[j] = (live && w0 + j < nWords) ? bits[size_t(w0 + j) * nc + i] : 0u;
+    __syncthreads();   // the previous chunk's reads are done
+    for (uint32_t k = threadIdx.x; k < dn; k += kBlock) {
+      const float3 v = D[d0 + k];
+      sD[3 * k] = v.x; sD[3 * k + 1] = v.y; sD[3 * k + 2] = v.z;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sW[q][k] = W[size_t(q) * nDir + d0 + k];
     }
     __syncthreads();
-    const int cnt = __popcll(m);
-    if (lane < uint32_t(NQ)) {
-      const double* pq = prod[lane];
-      for (int k = 0; k < cnt; ++k) acc += pq[k];
+#pragma unroll
+    for (uint32_t j = 0; j < kChunkW; ++j) {
+      const uint32_t word = words[j], base = 32 * j;
+      if (base >= dn || !__any(word != 0u)) continue;
+      const uint32_t cnt = dn - base < 32 ? dn - base : 32;
+#pragma unroll 4
+      for (uint32_t k = 0; k < cnt; ++k) {
+        if (!((word >> k) & 1u)) continue;
+        const uint32_t e = base + k;
+        const float c = fmaxf(dlm::dot3(n, make_float3(sD[3 * e], sD[3 * e + 1], sD[3 * e + 2])), 0.f);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] += (double)((cosMask >> q) & 1u ? c : 1.f) * (double)sW[q][e];
+      }
     }
-    __syncthreads();
   }
-  if (lane < uint32_t(NQ)) out[size_t(lane) * nc + i] = acc;
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[size_t(q) * nc + i] = acc[q];
+  }
 }
 
 template <int NQ>
 void launch(dlg_gpu* g, uint32_t nc, uint32_t mask) {
-  k_reduce<NQ><<<nc, kWave, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0, g->D.as<float3>(),
+  k_reduce<NQ><<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0, g->D.as<float3>(),
                                            g->scratch.as<float>(), uint32_t(g->nDir), mask, g->out.as<double>());
 }
 

@@ -130,26 +130,36 @@ __global__ void k_apply(int T, int P, const StepOut* __restrict__ so, const floa
 }
 
 // skyRad[p] = sum over the steps, in step order, of the contributions c[t P + p] (a step at night adds 0.0, which
-// leaves the sum's bits unchanged): 64 patches a block, tiles of 64 steps staged through LDS so that the sequential sum
-// of each patch reads LDS, not HBM.
-constexpr int kAccP = 64, kAccT = 64;
+// leaves the sum's bits unchanged): 4 patches a block (145 blocks at P = 577), tiles of 256 steps staged through LDS so
+// that the sequential sum of each patch reads LDS, not HBM, while the next tile's loads are in flight.
+constexpr int kAccP = 4, kAccT = 256;
 __global__ __launch_bounds__(256) void k_accum(int T, int P, const double* __restrict__ c, double* __restrict__ sum) {
-  __shared__ double tile[kAccT][kAccP + 1];
-  const int lp = threadIdx.x % kAccP, part = threadIdx.x / kAccP, p = blockIdx.x * kAccP + lp;
+  __shared__ double tile[kAccT][kAccP];
+  // thread (k, lp): patch p0 + lp at steps t0 + k + 64 j, j < 4; the next tile is loaded into registers while the
+  // lanes lp < kAccP add this one's kAccT values in step order
+  const int lp = threadIdx.x % kAccP, k = threadIdx.x / kAccP, p = blockIdx.x * kAccP + lp;
+  double nxt[4];
+  const auto load = [&](int t0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = t0 + k + 64 * j;
+      nxt[j] = (t < T && p < P) ? c[size_t(t) * P + p] : 0.0;
+    }
+  };
+  load(0);
   double r = 0.0;
   for (int t0 = 0; t0 < T; t0 += kAccT) {
-    for (int k = part; k < kAccT; k += 256 / kAccP) {
-      const int t = t0 + k;
-      tile[k][lp] = (t < T && p < P) ? c[size_t(t) * P + p] : 0.0;
-    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tile[k + 64 * j][lp] = nxt[j];
     __syncthreads();
-    if (part == 0) {
+    if (t0 + kAccT < T) load(t0 + kAccT);
+    if (threadIdx.x < kAccP) {
       const int n = T - t0 < kAccT ? T - t0 : kAccT;
-      for (int k = 0; k < n; ++k) r += tile[k][lp];
+      for (int q = 0; q < n; ++q) r += tile[q][lp];
     }
     __syncthreads();
   }
-  if (part == 0 && p < P) sum[p] = r;
+  if (threadIdx.x < kAccP && p < P) sum[p] = r;
 }
 
 // per step: the sun's weights, the ground's series value and the ground's contributions to the cumulative weights

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cctype>
+#include <chrono>
 #include <clocale>
 #include <cstdio>
 #include <cstring>
@@ -199,6 +200,7 @@ int32_t run_tool(int32_t tool, int32_t argc, const char* const* argv, dl_mesh* c
     std::string err;
     Result r;
     if (tool == DL_DAYLIGHT) {
+      const auto t0 = std::chrono::steady_clock::now();
       cli::Args a;
       if (cli::parse(int(av.size()), av.data(), a, err)) return fail(DL_USAGE, err.c_str());
       if (const int32_t rc = refused(a)) return rc;
@@ -211,13 +213,21 @@ int32_t run_tool(int32_t tool, int32_t argc, const char* const* argv, dl_mesh* c
       if (!resolve_optics(mv, a.opt.optics, slots, err) || !resolve_no_occlude(mv, a.opt.noOcclude, open, err)) return fail(DL_USAGE, err.c_str());
       set_run(a.opt, key, control);
       if (sink && a.opt.seriesPath.empty()) return fail(DL_USAGE, "dlx_run_series_device: the flags name no --timeseries");
+      const auto t1 = std::chrono::steady_clock::now();
       {
         std::lock_guard<std::mutex> lock(runMu);
         const bool ok = sink ? run_series_device(mv, haveWeather ? &w : nullptr, a.opt, *sink, r, err) : run(mv, haveWeather ? &w : nullptr, a.opt, r, err);
         if (!ok) return fail(DL_RUNTIME, err.c_str());
       }
       if (control) control->progress = 1;
-      return finish(r, cli::manifest_json(r), out);
+      const auto t2 = std::chrono::steady_clock::now();
+      const int32_t rc = finish(r, cli::manifest_json(r), out);
+      if (a.opt.verbose) {
+        const auto ms = [](auto x, auto y) { return std::chrono::duration<double, std::milli>(y - x).count(); };
+        std::fprintf(stderr, "info: dl_run (ms): flags and weather %.3f, run %.3f, result %.3f\n", ms(t0, t1), ms(t1, t2),
+                     ms(t2, std::chrono::steady_clock::now()));
+      }
+      return rc;
     }
     views::cli::Args a;
     if (views::cli::parse(int(av.size()), av.data(), a, err)) return fail(DL_USAGE, err.c_str());

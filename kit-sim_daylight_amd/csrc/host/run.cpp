@@ -213,6 +213,9 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
                          const char*& stage, const SeriesSink* sink) {
   const auto cancelled = [&] { if (opt.cancel && opt.cancel->load()) { err = "cancelled"; return true; } return false; };
   out.stats = RunStats{};
+  // -v: where the run's host time goes (stage boundaries, printed at the end)
+  std::vector<std::pair<const char*, Clk::time_point>> marks{{"start", Clk::now()}};
+  const auto mark = [&](const char* what) { if (opt.verbose) marks.emplace_back(what, Clk::now()); };
   // ---- validation (run.cpp:219-242)
   if (!optics_ok(opt, err)) return false;
   bool needWeather = false, wantVis = false, wantRad = false;
@@ -245,6 +248,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   if (!make_steps(opt, w, steps, err, shift)) return false;
   if (steps.empty()) { err = "no timesteps selected"; return false; }
   const size_t T = steps.size();
+  mark("steps");
   double durationHours = 0;
   for (const Step& s : steps) durationHours += s.dt;
   out.durationHours = durationHours;
@@ -346,6 +350,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
     for (size_t d = nSky; d < 2 * nSky; ++d) Wg[d] = pa.omega[size_t(rowOf[d - nSky])] / float(K);
   }
   out.stats.skyMs = ms(tSky);
+  mark("sky, directions, weights");
   if (opt.verbose && droppedDni > 0)
     std::fprintf(stderr, "warn: %.1f Wh/m2 of DNI at steps with the sun below the horizon dropped (gendaymtx would bin it)\n", droppedDni);
 
@@ -367,6 +372,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   dlg_gpu* gp = nullptr;
   Scene sc;
   if (!scene_acquire(meshes, blocks, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
+  mark("scene");
   dlg_stats_t st0{};
   if (gp) {
     dlg_stats(gp, &st0);
@@ -379,6 +385,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   std::string warn;
   if (!sensors_of(meshes, opt, sc.c, nSens, sens, out, warn, err)) return false;
   out.stats.prepMs += ms(tSens);
+  mark("sensors");
   if (opt.verbose) {
     std::fprintf(stderr, "info: %zu sensors, %zu triangles%s, %zu steps (%.1f h)\n", nSens, sc.nTris, out.sceneReused ? " (scene reused)" : "", T, durationHours);
     if (sc.skipped) std::fprintf(stderr, "warn: %d degenerate faces skipped in occluders\n", sc.skipped);
@@ -390,6 +397,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
                       reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e)) { err = e; return false; }
   }
   sens = Sensors{};
+  mark("upload");
 
   stage = "allocating the result";
   const size_t C = nQ * size_t(nS);
@@ -466,6 +474,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
         if (std::fwrite(E.data(), sizeof(float), E.size(), sf.get()) != E.size()) { err = "write failed: " + opt.seriesPath; return false; }
       }
     }
+    mark("trace, reduce, series");
     dlg_stats_t st1{};
     dlg_stats(gp, &st1);
     out.stats.traceMs = st1.traceMs - st0.traceMs;
@@ -480,6 +489,13 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   if (sf) {
     File hf(std::fopen((opt.seriesPath + ".hoys.txt").c_str(), "w"), std::fclose);
     if (hf) for (const double h : seriesHoys) std::fprintf(hf.get(), "%.4f\n", h);
+  }
+  mark("end");
+  if (opt.verbose) {
+    std::string line = "info: host stages (ms):";
+    for (size_t k = 1; k < marks.size(); ++k)
+      line += std::string(" ") + marks[k].first + " " + std::to_string(std::chrono::duration<double, std::milli>(marks[k].second - marks[k - 1].second).count()).substr(0, 6) + ";";
+    std::fprintf(stderr, "%s\n", line.c_str());
   }
   if (!opt.dumpSun.empty()) {   // the NOAA sun per step (columns of the reference's dump; Radiance's sun.c columns repeat it)
     File f(std::fopen(opt.dumpSun.c_str(), "w"), std::fclose);
