@@ -156,11 +156,12 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
 bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
                    std::string& err) {
   Resident& R = resident();
-  dlg_trim(g, 1);
   const auto need = [&] {
     const size_t nc = std::min(nSens, std::max<size_t>(1, chunkBytes / perSensor));
     return nSens * 2 * 12 + nc * chunkSensor + fixedBytes + kVramMargin;
   };
+  // the last run's buffers are kept for this one (they grow in place) unless the room they hold is needed
+  if (size_t(dlg_free_bytes()) < need()) dlg_trim(g, 1);
   while (!R.parked.empty() && (R.parked.size() + 1 > size_t(std::max(1, residentScenes)) || size_t(dlg_free_bytes()) < need())) {
     dlg_free_scene(R.parked.back().parked);
     R.parked.pop_back();
@@ -185,6 +186,43 @@ void scene_forget(const std::string& key) {
   if (R.cur.key == key) R.cur.key.clear();
   for (auto it = R.parked.begin(); it != R.parked.end();)
     if (it->key == key) { dlg_free_scene(it->parked); it = R.parked.erase(it); } else ++it;
+}
+
+namespace {
+struct SensorCache {
+  std::string key;
+  Sensors s;
+  std::vector<std::pair<std::string, size_t>> ranges;
+  std::string warn;
+};
+SensorCache& sensor_cache() {
+  static SensorCache c;
+  return c;
+}
+}  // namespace
+
+const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
+                              Sensors& scratch, Result& out, std::string& warn, std::string& err) {
+  std::string key;
+  if (!opt.sceneKey.empty()) {   // what shapes the sensors besides the meshes (their roles are part of the scene key)
+    char b[96];
+    std::snprintf(b, sizeof b, "\nsensors v%d s%d o%a n%zu", opt.onVertices ? 1 : 0, int(opt.sides), opt.offset, nSens);
+    key = opt.sceneKey + b;
+  }
+  SensorCache& C = sensor_cache();
+  if (!key.empty() && C.key == key) {
+    out.ranges = C.ranges;
+    warn = C.warn;
+    return &C.s;
+  }
+  C.key.clear();
+  if (!sensors_of(meshes, opt, c, nSens, scratch, out, warn, err)) return nullptr;
+  if (key.empty()) return &scratch;
+  C.s = std::move(scratch);
+  C.ranges = out.ranges;
+  C.warn = warn;
+  C.key = key;
+  return &C.s;
 }
 
 bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
@@ -381,9 +419,10 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   }
   stage = "making the sensors";
   const auto tSens = Clk::now();
-  Sensors sens;
+  Sensors scratch;
   std::string warn;
-  if (!sensors_of(meshes, opt, sc.c, nSens, sens, out, warn, err)) return false;
+  const Sensors* sens = sensors_cached(meshes, opt, sc.c, nSens, scratch, out, warn, err);
+  if (!sens) return false;
   out.stats.prepMs += ms(tSens);
   mark("sensors");
   if (opt.verbose) {
@@ -393,10 +432,10 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   }
   if (gp) {
     char e[512] = {};
-    if (dlg_set_scene(gp, reinterpret_cast<const float*>(sens.p.data()), reinterpret_cast<const float*>(sens.n.data()), int64_t(nSens),
+    if (dlg_set_scene(gp, reinterpret_cast<const float*>(sens->p.data()), reinterpret_cast<const float*>(sens->n.data()), int64_t(nSens),
                       reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e)) { err = e; return false; }
   }
-  sens = Sensors{};
+  scratch = Sensors{};
   mark("upload");
 
   stage = "allocating the result";

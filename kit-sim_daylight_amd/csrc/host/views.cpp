@@ -153,9 +153,10 @@ static bool run_views(const std::vector<SimulationMeshView>& meshes, const Optio
   if (!run_admission(g, nSens, perSensor, 5 * K + NA * sizeof(double), label.size(), opt.residentScenes, chunkBytes, err)) return false;
   stage = "making the sensors";
   const auto tSens = Clk::now();
-  Sensors sens;
+  Sensors scratch;
   std::string warn;
-  if (!sensors_of(meshes, opt, sc.c, nSens, sens, out, warn, err)) return false;
+  const Sensors* sens = sensors_cached(meshes, opt, sc.c, nSens, scratch, out, warn, err);
+  if (!sens) return false;
   out.stats.prepMs += ms(tSens);
   const float plane = float(fd.groundZ - sc.c[2]);
   if (opt.verbose) {
@@ -167,12 +168,12 @@ static bool run_views(const std::vector<SimulationMeshView>& meshes, const Optio
     if (!warn.empty()) std::fprintf(stderr, "warn: %s\n", warn.c_str());
   }
   char e[512] = {};
-  if (dlg_set_scene(g, reinterpret_cast<const float*>(sens.p.data()), reinterpret_cast<const float*>(sens.n.data()), int64_t(nSens), nullptr, 0, e, sizeof e) ||
+  if (dlg_set_scene(g, reinterpret_cast<const float*>(sens->p.data()), reinterpret_cast<const float*>(sens->n.data()), int64_t(nSens), nullptr, 0, e, sizeof e) ||
       dlg_set_labels(g, label.data(), int64_t(label.size()), e, sizeof e)) {
     err = e;
     return false;
   }
-  sens = Sensors{};
+  scratch = Sensors{};
   label = {};
 
   stage = "allocating the result";

@@ -81,6 +81,11 @@ std::string open_key(const std::vector<char>& open, bool selfOcclude);
 void scene_forget(const std::string& key);
 bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
                 std::string& warn, std::string& err);
+// The run's sensors: those of the last run when it had the same meshes (opt.sceneKey: the mesh handles, which never
+// change) and the same sensor options, else made by sensors_of and kept for the next run. Without a scene key, made
+// into `scratch`. The pointer stays valid until the next run.
+const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
+                              Sensors& scratch, Result& out, std::string& warn, std::string& err);
 double ms_since(const void* clockStart);
 // daylight::run with the --timeseries output kept on the device: per chunk of sensors, sink(i0, nc, T, E) with E the
 // device array float [nc x T] (null when the run has no series columns); an empty sink drops the series.

@@ -271,3 +271,22 @@ def test_chunking_invariance(native, box, tmp_path):
     assert np.array_equal(one.values, many.values)
     assert len(calls1) == 1 and len(callsN) == 8 and set(callsN) == {2}
     assert np.array_equal(e1, eN)
+
+
+def test_runs_in_turn_keep_their_own_sensors(native, box):
+    """Sensors kept between runs over the same mesh handles follow each run's own sensor options (offset, sides,
+    vertices), also when views runs sit in between: every run gives the values it gives alone."""
+    print("case: sensors kept between runs", flush=True)
+    base = ["--quantity", "sunhours"] + BERLIN + ["--tz", "1", "--from", "06-21T00:00", "--to", "06-22T00:00"]
+    variants = [[], ["--offset", "0.5"], ["--sides", "both"], ["--on", "vertices"], []]
+    alone = {}
+    for v in variants[:4]:
+        alone[tuple(v)] = native.run("daylight", base + v, box).values
+    native.run("views", [], box)
+    for v in variants:
+        got = native.run("daylight", base + v, box).values
+        assert np.array_equal(got, alone[tuple(v)]), v
+        native.run("views", ["--offset", "0.3"], box)
+    # the variants differ, so a run handed another run's sensors would show
+    assert not np.array_equal(alone[()], alone[("--offset", "0.5")])
+    assert alone[()].shape != alone[("--sides", "both")].shape and alone[()].shape != alone[("--on", "vertices")].shape
