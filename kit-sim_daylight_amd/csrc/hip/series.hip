@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   constexpr int kPer = kFrag / 256;               // per thread
   __shared__ u32x4 sA[kFrag];
   __shared__ u32x4 sB[kFrag];
+  static_assert(kFrag * 16 >= 128 * 10 * 4 && kFrag * 16 >= 128 * 128, "the epilogue's staging fits the operand buffers");
   // XCD-major tile order
   const uint32_t id = blockIdx.x, per = (tiles + 7) / 8, xcd = id % 8, rank = id / 8;
   const uint32_t full = tiles % 8 == 0 ? 8 : tiles % 8;   // XCDs that take `per` tiles; the rest take per - 1
@@ -180,7 +181,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (int m = 0; m < kPer; ++m) { sA[threadIdx.x + 256 * m] = pa[m]; sB[threadIdx.x + 256 * m] = pb[m]; }
     __syncthreads();
-    {   // the next slice in flight while this one's MFMAs run (the last iteration reloads its own: no branch)
+    {   // the next slice in flight while this one's MFMAs run (the last iteration reloads its own: no branch); the
+        // scheduling barriers keep the loads here, ahead of the MFMAs, and each k-step's fragments after the last's
+        // products, so that the two do not share registers
       const uint32_t step = 128 * (q + 1 < nSlices ? q + 1 : q);
 #pragma unroll
       for (int m = 0; m < kPer; ++m) {
@@ -188,6 +191,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         pb[m] = B[(m / 2) * bPl + (m % 2) * 2 * gStride + bBase + step];
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
       bf16x8 a[2][NS], b[2][NS];
@@ -208,23 +212,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           for (int o = NS - 1; o >= 0; --o)
 #pragma unroll
             for (int s = 0; s <= o; ++s) acc[fm][fn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[fm][s], b[fn][o - s], acc[fm][fn], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
   }
+  // Epilogue: the tile's rows (N, G), columns (g, the step's sun: D, weight) and the sun visibility of every (row,
+  // column) staged in the operand buffers (free after the loop's last barrier), so that the 64 elements of a lane read
+  // LDS instead of gathering from HBM; E = acc + G g + the sun's term, in FP64.
+  float* sRow = reinterpret_cast<float*>(sA);       // [128][4]: N x y z, G
+  float* sCol = sRow + 128 * 4;                      // [128][6]: g, D x y z, sun weight, sun index (bits; -1: none)
+  uint8_t* sVis = reinterpret_cast<uint8_t*>(sB);   // [128 rows][128 columns]
+  const uint32_t i0 = 32 * rg0, t0 = 32 * cg0;
+  if (threadIdx.x < 128) {
+    const uint32_t i = i0 + threadIdx.x;
+    const float3 n = i < nc ? N[i] : make_float3(0.f, 0.f, 0.f);
+    float* r = sRow + 4 * threadIdx.x;
+    r[0] = n.x; r[1] = n.y; r[2] = n.z; r[3] = i < nc ? G[i] : 0.f;
+  } else {
+    const uint32_t c = threadIdx.x - 128, t = t0 + c;
+    const int j = int(t) < T ? sunOfCol[t] : -1;
+    const float3 d = j >= 0 ? D[sun0 + uint32_t(j)] : make_float3(0.f, 0.f, 0.f);
+    float* q = sCol + 6 * c;
+    q[0] = int(t) < T ? g[t] : 0.f; q[1] = d.x; q[2] = d.y; q[3] = d.z; q[4] = j >= 0 ? sunW[j] : 0.f; q[5] = __int_as_float(j);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < 128 * 128; k += 256) {   // consecutive threads: consecutive rows of one column
+    const uint32_t r = k & 127, c = k >> 7, i = i0 + r;
+    const int j = __float_as_int(sCol[6 * c + 5]);
+    sVis[r * 128 + c] = (j >= 0 && i < nc) ? uint8_t(vis_bit(bits, nc, i, sun0 + uint32_t(j))) : uint8_t(0);
+  }
+  __syncthreads();
 #pragma unroll
   for (int fn = 0; fn < 2; ++fn) {
-    const uint32_t t = 32 * (cg0 + 2 * wn + fn) + (lane & 31);
+    const uint32_t c = 32 * (2 * wn + fn) + (lane & 31), t = t0 + c;
     if (int(t) >= T) continue;
-    const int j = sunOfCol[t];
-    const float gt = g[t];
+    const float* q = sCol + 6 * c;
+    const double gt = (double)q[0], sw = (double)q[4];
+    const float3 dj = make_float3(q[1], q[2], q[3]);
 #pragma unroll
     for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const uint32_t i = 32 * (rg0 + 2 * wm + fm) + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+        const uint32_t r = 32 * (2 * wm + fm) + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4), i = i0 + r;
         if (i >= nc) continue;
-        double e = (double)acc[fm][fn][v] + (double)G[i] * (double)gt;
-        if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) e += (double)fmaxf(dlm::dot3(N[i], D[sun0 + uint32_t(j)]), 0.f) * (double)sunW[j];
+        const float* rr = sRow + 4 * r;
+        double e = (double)acc[fm][fn][v] + (double)rr[3] * gt;
+        if (sVis[r * 128 + c]) e += (double)fmaxf(dlm::dot3(make_float3(rr[0], rr[1], rr[2]), dj), 0.f) * sw;
         E[size_t(i) * T + t] = (float)e;
       }
   }
