@@ -97,6 +97,14 @@ DL_API void dlx_chunk_bytes(int64_t bytes);
  * file it cannot use. */
 DL_API int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t cap, int64_t* rows, double* latLonTz);
 
+/* Host checks of the scene's optics (no GPU involved; the reference's tests call its internal functions the same way):
+ * dlx_thin_solid: the thin-solid flags of slot `slot` of a one-mesh handle (0, or DLG_SLOT_THIN | DLG_SLOT_INWARD /
+ * DLG_SLOT_TWIN of dl_device.h: 2 | 4 / 8) with the mesh recentred by centre[3] (NULL: the origin), reference
+ * mesh.cpp:275-304. dlx_triangle_slots: the global slot (slotBase + the face's slot) of every triangle the mesh draws,
+ * in build order, reference mesh.cpp:137-168 (Tris::slot); *n triangles, the first min(n, cap) written. */
+DL_API int32_t dlx_thin_solid(const dl_mesh* mesh, int32_t slot, const double* centre, uint32_t* flags);
+DL_API int32_t dlx_triangle_slots(const dl_mesh* mesh, uint32_t slotBase, uint32_t* slots, int64_t cap, int64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,9 @@
  * Conventions: status 0 ok, nonzero failed with the message in err[0..errLen) (NUL-terminated); one dlg_gpu per process
  * (the reference's Gpu), made on first use; calls on one dlg_gpu run one at a time. Device buffers belong to the
  * dlg_gpu and grow as needed. Layouts: visibility bits word-major bits[w nc + i] (downloads transposed to row-major as
- * the reference's are); views classes / distances sensor-major [i K + d], the download's layout.
+ * the reference's are); byte mode's transmittance in quads of directions, the 32-bit word [(d / 4) nc + i] holding
+ * directions 4 (d / 4) .. + 3 of sensor i in its bytes (the download is direction-major, as the reference's); views
+ * classes / distances sensor-major [i K + d], the download's layout.
  */
 #ifndef DL_DEVICE_H
 #define DL_DEVICE_H
@@ -55,18 +57,38 @@ int64_t dlg_free_bytes(void);
  * original index as bits, 2 unused), tree height. Sizes in *nNodes / *nTris when the pointers are null. */
 int32_t dlg_scene_download(dlg_gpu* g, float* nodes, int64_t* nNodes, float* leaves, int64_t* nTris, int32_t* height);
 
+/* Slot optics (the reference's Slot, params.h:7-19): rho (the bounce pass, unused here), tau the transmittance at normal
+ * incidence (> 0: the slot transmits), tn the glass transmissivity (tn_from_Tn(tau)), flags DLG_SLOT_*. */
+typedef struct { float rho, tau, tn; uint32_t flags; } dlg_slot;
+#define DLG_SLOT_GLASS 1u   /* transmittance by angle (Radiance glass); else tau at every angle */
+#define DLG_SLOT_THIN 2u    /* a closed thin solid: its exit face passes everything */
+#define DLG_SLOT_INWARD 4u  /* ... wound inward: entry is the back face */
+#define DLG_SLOT_TWIN 8u    /* ... of coincident face pairs (the bounce pass only) */
+/* gpu_set_slots (internal.h:139): the optics of every global slot, read by a byte-mode trace (every run). */
+int32_t dlg_set_slots(dlg_gpu* g, const dlg_slot* slots, int64_t n, char* err, int32_t errLen);
+/* The global slot of every triangle of the installed scene (input order, nt = the build's triangle count), kept with the
+ * scene's leaves: the reference passes them with the build (Tris::slot, internal.h:58, gpu_build's per-triangle SBT
+ * offsets). A scene built without them has every triangle in slot 0. */
+int32_t dlg_set_tri_slots(dlg_gpu* g, const uint32_t* slot, int64_t nt, char* err, int32_t errLen);
+
 /* gpu_set_scene (internal.h:146): resident sensors (points P[3 n], normals N[3 n], recentred) and directions D[3 nDir]. */
 int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens, const float* D, int64_t nDir, char* err, int32_t errLen);
 /* gpu_ray_flags (internal.h:218): the reference's OptiX flag values (TERMINATE_ON_FIRST_HIT 4 | DISABLE_ANYHIT 1 |
  * CULL_BACK_FACING_TRIANGLES 16), kept so callers and tests read the same numbers. */
 uint32_t dlg_ray_flags(int32_t cullBackfaces, int32_t translucent);
-/* gpu_trace (internal.h:151): visibility bits of sensors [i0, i0 + nc) against every direction, on the device. */
+/* gpu_trace (internal.h:151): visibility bits of sensors [i0, i0 + nc) against every direction, on the device. Ray
+ * flags without DISABLE_ANYHIT (dlg_ray_flags(., 1): a slot transmits): byte mode, the transmittance of every direction
+ * as a byte (x 255, 0 = culled or blocked) instead of a bit, through the slots of dlg_set_slots / dlg_set_tri_slots
+ * (the reference's __raygen__bytes + __anyhit__tr, trace_dev.cu:115-165). */
 int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, char* err, int32_t errLen);
-/* gpu_reduce (internal.h:154), every quantity of a run in one pass over the bits: out[q nc + i] = sum over the visible
- * directions d (ascending) of double(float(useCos[q] ? max(0, N_i . D_d) : 1)) * double(W[q nDir + d]). */
+/* gpu_reduce (internal.h:154), every quantity of a run in one pass over the visibility: out[q nc + i] = sum over the
+ * visible directions d (ascending) of double(float(useCos[q] ? max(0, N_i . D_d) : 1) * T) * double(W[q nDir + d]),
+ * T = 1 (bits) or byte / 255.f (byte mode). */
 int32_t dlg_reduce(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, double* out, char* err, int32_t errLen);
 /* gpu_download_bits (internal.h:182): [nWords x nc] as on the device. */
 int32_t dlg_download_bits(dlg_gpu* g, int64_t nc, uint32_t* out, char* err, int32_t errLen);
+/* gpu_download_bytes (internal.h:183): byte mode's transmittance, direction-major [nDir x nc]. */
+int32_t dlg_download_bytes(dlg_gpu* g, int64_t nc, uint8_t* out, char* err, int32_t errLen);
 
 /* The sky of a run on the GPU, FP64 (replaces the host loops of reference run.cpp:311-446 and sky.cpp's
  * gendaymtx_step / solar.cpp's sun_noaa per step): the sun of every step, the Perez all-weather sky per step and patch,

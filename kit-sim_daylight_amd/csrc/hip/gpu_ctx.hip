@@ -3,6 +3,7 @@
 // and directions, scene residency. Replaces the OptiX context / pipeline / SBT host code of reference src/gpu.cu:277-497.
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "dl_math.h"
 #include "gpu_internal.h"
@@ -16,6 +17,13 @@ __global__ void k_dir_table(const float3* __restrict__ D, int64_t n, float4* __r
   dlm::Ray r;
   dlm::ray_dir(r, D[d]);
   dlm::dir_pack(r, tab[2 * d], tab[2 * d + 1]);
+}
+
+// leaf k's global slot (float 10, as bits) from the per-triangle slots in input order (its original index, float 9)
+__global__ void k_leaf_slots(float* __restrict__ leaves, int64_t nt, const uint32_t* __restrict__ slot) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  leaves[12 * k + 10] = __uint_as_float(slot[__float_as_uint(leaves[12 * k + 9])]);
 }
 
 }  // namespace
@@ -43,7 +51,7 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
   for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts})
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->slots})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -117,8 +125,56 @@ int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens,
 
 uint32_t dlg_ray_flags(int32_t cullBackfaces, int32_t translucent) { return 4u | (translucent ? 0u : 1u) | (cullBackfaces ? 16u : 0u); }
 
+int32_t dlg_set_slots(dlg_gpu* g, const dlg_slot* slots, int64_t n, char* err, int32_t errLen) {
+  g->nSlots = 0;
+  if (n <= 0) return 0;
+  DLG_CHECK(g->slots.grow(size_t(n) * sizeof(dlg_slot)));
+  DLG_CHECK(hipMemcpyAsync(g->slots.p, slots, size_t(n) * sizeof(dlg_slot), hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  g->nSlots = n;
+  return 0;
+}
+
+int32_t dlg_set_tri_slots(dlg_gpu* g, const uint32_t* slot, int64_t nt, char* err, int32_t errLen) {
+  dlg_scene& s = g->scene;
+  if (nt != s.nTris) { dlg_fail(err, errLen, "dlg_set_tri_slots: " + std::to_string(nt) + " slots for a scene of " + std::to_string(s.nTris) + " triangles"); return 1; }
+  uint32_t mx = 0;
+  for (int64_t t = 0; t < nt; ++t) mx = slot[t] > mx ? slot[t] : mx;
+  if (nt == 0) { s.slotMax = 0; return 0; }
+  DevBuf d;
+  hipError_t e = d.grow(size_t(nt) * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(d.p, slot, size_t(nt) * 4, hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) {
+    k_leaf_slots<<<unsigned((nt + 255) / 256), 256, 0, g->stream>>>(s.leaves.as<float>(), nt, d.as<uint32_t>());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  d.release();
+  if (e != hipSuccess) { dlg_fail(err, errLen, std::string("dlg_set_tri_slots: ") + hipGetErrorString(e)); return 1; }
+  s.slotMax = mx;
+  return 0;
+}
+
+int32_t dlg_download_bytes(dlg_gpu* g, int64_t nc, uint8_t* out, char* err, int32_t errLen) {
+  if (nc != g->nc) { dlg_fail(err, errLen, "dlg_download_bytes: no trace of this chunk"); return 1; }
+  if (!g->bytes) { dlg_fail(err, errLen, "dlg_download_bytes: the last trace wrote bits"); return 1; }
+  const size_t nQuads = size_t(g->nWords) * 8, n = nQuads * size_t(nc);
+  if (n == 0) return 0;
+  std::vector<uint32_t> q(n);
+  DLG_CHECK(hipMemcpyAsync(q.data(), g->bits.p, n * 4, hipMemcpyDeviceToHost, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  for (size_t d = 0; d < size_t(g->nDir); ++d) {   // quad d / 4, byte d % 4 -> direction-major [d nc + i]
+    const uint32_t* row = q.data() + (d / 4) * size_t(nc);
+    const uint32_t sh = 8 * uint32_t(d % 4);
+    uint8_t* o = out + d * size_t(nc);
+    for (size_t i = 0; i < size_t(nc); ++i) o[i] = uint8_t(row[i] >> sh);
+  }
+  return 0;
+}
+
 int32_t dlg_download_bits(dlg_gpu* g, int64_t nc, uint32_t* out, char* err, int32_t errLen) {
   if (nc != g->nc) { dlg_fail(err, errLen, "dlg_download_bits: no trace of this chunk"); return 1; }
+  if (g->bytes) { dlg_fail(err, errLen, "dlg_download_bits: the last trace wrote bytes"); return 1; }
   const size_t n = size_t(g->nWords) * size_t(nc);
   if (n) DLG_CHECK(hipMemcpyAsync(out, g->bits.p, n * 4, hipMemcpyDeviceToHost, g->stream));
   DLG_CHECK(hipStreamSynchronize(g->stream));

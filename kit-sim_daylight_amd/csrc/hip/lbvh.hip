@@ -9,7 +9,8 @@
 //   5. bottom-up boxes: a thread per leaf climbs, the second thread to reach a node writes it (exact, so any order)
 // Layout (HBM): node n = 16 floats: left child box lo xyz hi xyz, right child box, left, right (child >= 2^31: leaf
 // child & 0x7fffffff), 2 unused: one 64-byte line per node visit. Leaf k = 12 floats: v0, v1, v2 of the k-th triangle
-// in sorted order, its original index (as bits), 2 unused: three 16-byte loads.
+// in sorted order, its original index (as bits), its global slot (as bits, 0 until dlg_set_tri_slots), 1 unused: three
+// 16-byte loads.
 // The same arrays come out of the host build in the oracle, bit for bit (tests compare them).
 //   6. the 4-wide records the traversals walk (wide.h): one per inner node, from the node and its inner children
 // By default steps 4-5 are replaced by the binned-SAH tree of sah.hip over the sorted leaves (same layout); the LBVH
@@ -217,6 +218,7 @@ extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint3
   s.release();
   s.nNodes = s.nTris = 0;
   s.height = 0;
+  s.slotMax = 0;
   float mx = 0.f;
   for (int64_t i = 0; i < 3 * nv; ++i) mx = fmaxf(mx, fabsf(V[i]));
   s.pad = (mx + 1.f) * 3.81469727e-6f;   // 2^-18 (|coordinate| + 1): box slack beyond the triangle test's rounding

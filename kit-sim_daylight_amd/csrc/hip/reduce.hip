@@ -1,14 +1,15 @@
 // SPDX-License-Identifier: Apache-2.0
-// The cumulative reduces of a daylight run in one pass over the visibility bits: replaces k_reduce<Bits> and gpu_reduce
-// (reference src/gpu.cu:99-114, 612-625), which ran once per quantity and re-read the bits each time.
-// Each sensor's sum is the reference's: acc_q += double(float(max(0, dotf(n, D_d)))) x double(W[q][d]) over its visible
-// directions in ascending order (sun hours: the float 1 in place of the cosine), in FP64, so the result is the same bits
-// for any chunking. The order makes each sum a dependent chain; the chains of different sensors run side by side:
+// The cumulative reduces of a daylight run in one pass over the visibility: replaces k_reduce<Bits> / k_reduce<Bytes> and
+// gpu_reduce (reference src/gpu.cu:76-114, 612-625), which ran once per quantity and re-read the visibility each time.
+// Each sensor's sum is the reference's: acc_q += double(float(max(0, dotf(n, D_d)) x T)) x double(W[q][d]) over its
+// visible directions in ascending order (sun hours: the float 1 in place of the cosine; T = 1 for bits, byte / 255.f in
+// byte mode), in FP64, so the result is the same bits for any chunking. The order makes each sum a dependent chain; the
+// chains of different sensors run side by side:
 //   a lane per sensor; every lane of a wave walks the directions together (the direction and its weights are the same
 //   for all: staged in LDS per chunk of 256 directions and read as broadcasts), reads its visibility word once per 32
 //   directions (one coalesced load per wave) and adds the product where its bit is set. Words that no lane of the wave
 //   sees through are skipped.
-// HBM: 4 nWords bytes of bits + 12 (N) + 8 nQ per sensor; directions and weights stay in the scalar cache / L2.
+// HBM: 4 nWords bytes of bits (byte mode: 32 nWords) + 12 (N) + 8 nQ per sensor; directions and weights stay in L2.
 #include "dl_math.h"
 #include "gpu_internal.h"
 
@@ -16,26 +17,36 @@ namespace {
 
 constexpr int kMaxQ = 8, kBlock = 256;
 
-template <int NQ>
-__global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
+template <int NQ, bool BYTES>
+__global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ vis, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
                                                    const float3* __restrict__ D, const float* __restrict__ W, uint32_t nDir,
                                                    uint32_t cosMask, double* __restrict__ out) {
-  // chunks of 8 words: their 256 directions and weights staged in LDS (every lane reads the same entry: a broadcast),
-  // the 8 visibility words of the lane loaded up front
+  // chunks of 8 words: their 256 directions and weights staged in LDS (every lane reads the same entry: a broadcast);
+  // bits: the 8 visibility words of the lane loaded up front; bytes: per word its 8 quads, the fraction of a byte from a
+  // table in LDS
   constexpr uint32_t kChunkW = 8, kChunkD = kChunkW * 32;
   __shared__ float sD[kChunkD * 3];
   __shared__ float sW[NQ][kChunkD];
+  __shared__ float sT[BYTES ? 256 : 1];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const bool live = i < nc;
   const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+  if (BYTES) sT[threadIdx.x] = dlm::byte_T(threadIdx.x);   // ordered before its reads by the chunk loop's barriers
   double acc[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  const auto term = [&](uint32_t e, float T) {
+    const float c = fmaxf(dlm::dot3(n, make_float3(sD[3 * e], sD[3 * e + 1], sD[3 * e + 2])), 0.f);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] += (double)(((cosMask >> q) & 1u ? c : 1.f) * T) * (double)sW[q][e];
+  };
   for (uint32_t w0 = 0; w0 < nWords; w0 += kChunkW) {
     const uint32_t d0 = w0 * 32, dn = nDir - d0 < kChunkD ? nDir - d0 : kChunkD;
     uint32_t words[kChunkW];
+    if (!BYTES) {
 #pragma unroll
-    for (uint32_t j = 0; j < kChunkW; ++j) words[j] = (live && w0 + j < nWords) ? bits[size_t(w0 + j) * nc + i] : 0u;
+      for (uint32_t j = 0; j < kChunkW; ++j) words[j] = (live && w0 + j < nWords) ? vis[size_t(w0 + j) * nc + i] : 0u;
+    }
     __syncthreads();   // the previous chunk's reads are done
     for (uint32_t k = threadIdx.x; k < dn; k += kBlock) {
       const float3 v = D[d0 + k];
@@ -44,6 +55,25 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
       for (int q = 0; q < NQ; ++q) sW[q][k] = W[size_t(q) * nDir + d0 + k];
     }
     __syncthreads();
+    if (BYTES) {
+      for (uint32_t j = 0; j < kChunkW && 32 * j < dn; ++j) {
+        uint32_t quad[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) quad[u] = live ? vis[size_t(8 * (w0 + j) + u) * nc + i] : 0u;
+        uint32_t any = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) any |= quad[u];
+        if (!__any(any != 0u)) continue;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+#pragma unroll
+          for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t v = (quad[u] >> (8 * b)) & 255u;   // 0 beyond the last direction
+            if (v) term(32 * j + 4 * u + b, sT[v]);
+          }
+      }
+      continue;
+    }
 #pragma unroll
     for (uint32_t j = 0; j < kChunkW; ++j) {
       const uint32_t word = words[j], base = 32 * j;
@@ -52,10 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
 #pragma unroll 4
       for (uint32_t k = 0; k < cnt; ++k) {
         if (!((word >> k) & 1u)) continue;
-        const uint32_t e = base + k;
-        const float c = fmaxf(dlm::dot3(n, make_float3(sD[3 * e], sD[3 * e + 1], sD[3 * e + 2])), 0.f);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += (double)((cosMask >> q) & 1u ? c : 1.f) * (double)sW[q][e];
+        term(base + k, 1.f);
       }
     }
   }
@@ -67,8 +94,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
 
 template <int NQ>
 void launch(dlg_gpu* g, uint32_t nc, uint32_t mask) {
-  k_reduce<NQ><<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0, g->D.as<float3>(),
-                                           g->scratch.as<float>(), uint32_t(g->nDir), mask, g->out.as<double>());
+  const auto k = g->bytes ? k_reduce<NQ, true> : k_reduce<NQ, false>;
+  k<<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0, g->D.as<float3>(),
+                                                          g->scratch.as<float>(), uint32_t(g->nDir), mask, g->out.as<double>());
 }
 
 }  // namespace

@@ -1,7 +1,8 @@
 // SPDX-License-Identifier: Apache-2.0
 // The time-series contraction E = A S + G g + sun of a daylight chunk: replaces k_series / gpu_series (reference
 // src/gpu.cu:119-149, 639-669). A[i, d] = max(0, dotf(n_i, D_d)) wsky_d on the visible sky directions (the daylight
-// coefficients, exact 0 where a direction is blocked), S[row_d, t] the patch radiance of the non-night steps (dlg_sky).
+// coefficients, exact 0 where a direction is blocked; byte mode: x the transmittance byte / 255.f, gpu.cu:83-92), S[row_d,
+// t] the patch radiance of the non-night steps (dlg_sky).
 //   method 0, FP32: each E[i, t] an fma chain over d in ascending order, then fma(G_i, g_t, .), then the sun's fma at its
 //             step: the reference kernel's arithmetic (nvcc contracts it so), tiled 64 x 64 through LDS.
 //   method 1, BF16x3: A and S split into bf16 hi + lo; hi.hi + hi.lo + lo.hi on MFMA (v_mfma_f32_32x32x16 bf16),
@@ -25,22 +26,27 @@ __device__ __forceinline__ uint16_t bf16_rn(float f) {   // round to nearest eve
 }
 __device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-__device__ __forceinline__ bool vis_bit(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t i, uint32_t d) {
-  return (bits[size_t(d >> 5) * nc + i] >> (d & 31)) & 1u;
+// the visibility of (sensor i, direction d) as a transmittance byte: a set bit is 255; byte mode's quads hold the byte
+__device__ __forceinline__ uint32_t vis_byte(const uint32_t* __restrict__ vis, uint32_t nc, uint32_t i, uint32_t d, uint32_t bytes) {
+  if (bytes) return (vis[size_t(d >> 2) * nc + i] >> (8 * (d & 3))) & 255u;
+  return ((vis[size_t(d >> 5) * nc + i] >> (d & 31)) & 1u) * 255u;
 }
+// the fraction a byte stands for (a bit's 255: exactly 1)
+__device__ __forceinline__ float vis_T(uint32_t v) { return v == 255u ? 1.f : dlm::byte_T(v); }
 
-// the daylight coefficient of (sensor i, sky direction d), as k_series forms it (gpu.cu:137)
-__device__ __forceinline__ float coeff(const uint32_t* __restrict__ bits, uint32_t nc, uint32_t i, uint32_t d, float3 n,
+// the daylight coefficient of (sensor i, sky direction d), as k_series forms it (gpu.cu:137): cos x T x wsky in float
+__device__ __forceinline__ float coeff(const uint32_t* __restrict__ vis, uint32_t bytes, uint32_t nc, uint32_t i, uint32_t d, float3 n,
                                        const float3* __restrict__ D, const float* __restrict__ wsky) {
-  if (!vis_bit(bits, nc, i, d)) return 0.f;
-  return (fmaxf(dlm::dot3(n, D[d]), 0.f) * 1.f) * wsky[d];
+  const uint32_t v = vis_byte(vis, nc, i, d, bytes);
+  if (!v) return 0.f;
+  return (fmaxf(dlm::dot3(n, D[d]), 0.f) * vis_T(v)) * wsky[d];
 }
 
 // ---------------------------------------------------------------------------------------------- method 0: FP32
 // Each E[i, t]: the fma chain over the sky directions in ascending order, then fma(G_i, g_t, .), then the sun of step t
 // (its one direction, when visible): per element the reference kernel's order of operations (gpu.cu:128-148).
 constexpr int kTile = 64, kKt = 16;
-__global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__ bits, uint32_t nc, const float3* __restrict__ N,
+__global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N,
                                                     const float3* __restrict__ D, uint32_t nSky, const int* __restrict__ row,
                                                     const float* __restrict__ wsky, const float* __restrict__ S, int T,
                                                     const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
@@ -54,7 +60,7 @@ __global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__
     for (int k = threadIdx.x; k < kKt * kTile; k += 256) {   // A tile: 16 directions x 64 sensors, from the bits
       const int dd = k / kTile, ii = k % kTile;
       const uint32_t d = d0 + dd, i = i0 + ii;
-      As[dd][ii] = (d < nSky && i < nc) ? coeff(bits, nc, i, d, N[i], D, wsky) : 0.f;
+      As[dd][ii] = (d < nSky && i < nc) ? coeff(bits, bytes, nc, i, d, N[i], D, wsky) : 0.f;
       const int t = int(t0) + ii;
       Bs[dd][ii] = (d < nSky && t < T) ? S[size_t(row[d]) * T + t] : 0.f;
     }
@@ -77,7 +83,8 @@ __global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__
       if (t >= T) continue;
       float v = fmaf(G[i], g[t], e[r][c]);
       const int j = sunOfCol[t];
-      if (j >= 0 && vis_bit(bits, nc, i, sun0 + uint32_t(j))) v = fmaf(fmaxf(dlm::dot3(n, D[sun0 + uint32_t(j)]), 0.f) * 1.f, sunW[j], v);
+      const uint32_t b = j >= 0 ? vis_byte(bits, nc, i, sun0 + uint32_t(j), bytes) : 0u;
+      if (b) v = fmaf(fmaxf(dlm::dot3(n, D[sun0 + uint32_t(j)]), 0.f) * vis_T(b), sunW[j], v);
       E[size_t(i) * T + t] = v;
     }
   }
@@ -101,11 +108,11 @@ __device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k,
 }
 
 template <int NS>
-__global__ void k_split_a(const uint32_t* __restrict__ bits, uint32_t nc, const float3* __restrict__ N, const float3* __restrict__ D,
+__global__ void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N, const float3* __restrict__ D,
                           uint32_t nSky, uint32_t Kp, uint32_t nRG, const float* __restrict__ wsky, uint16_t* __restrict__ A) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (d >= Kp) return;
-  const float c = (d < nSky && i < nc) ? coeff(bits, nc, i, d, N[i], D, wsky) : 0.f;
+  const float c = (d < nSky && i < nc) ? coeff(bits, bytes, nc, i, d, N[i], D, wsky) : 0.f;
   uint16_t p[NS];
   split<NS>(c, p);
   const size_t plane = size_t(nRG) * 32 * Kp, at = frag_index(i / 32, i % 32, d, Kp / 16);
@@ -140,7 +147,7 @@ __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ r
 constexpr uint32_t kGroup = 4, kTileM = 128, kTileN = 128, kSliceK = 32;
 template <int NS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_series_mfma(const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG,
-                                                     uint32_t nCG, uint32_t nKS, int T, const uint32_t* __restrict__ bits,
+                                                     uint32_t nCG, uint32_t nKS, int T, const uint32_t* __restrict__ bits, uint32_t bytes,
                                                      const float3* __restrict__ N, const float3* __restrict__ D,
                                                      const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
                                                      const int* __restrict__ sunOfCol, const float* __restrict__ sunW,
@@ -221,7 +228,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // LDS instead of gathering from HBM; E = acc + G g + the sun's term, in FP64.
   float* sRow = reinterpret_cast<float*>(sA);       // [128][4]: N x y z, G
   float* sCol = sRow + 128 * 4;                      // [128][6]: g, D x y z, sun weight, sun index (bits; -1: none)
-  uint8_t* sVis = reinterpret_cast<uint8_t*>(sB);   // [128 rows][128 columns]
+  uint8_t* sVis = reinterpret_cast<uint8_t*>(sB);   // [128 rows][128 columns]: the sun's transmittance byte
   const uint32_t i0 = 32 * rg0, t0 = 32 * cg0;
   if (threadIdx.x < 128) {
     const uint32_t i = i0 + threadIdx.x;
@@ -239,7 +246,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   for (uint32_t k = threadIdx.x; k < 128 * 128; k += 256) {   // consecutive threads: consecutive rows of one column
     const uint32_t r = k & 127, c = k >> 7, i = i0 + r;
     const int j = __float_as_int(sCol[6 * c + 5]);
-    sVis[r * 128 + c] = (j >= 0 && i < nc) ? uint8_t(vis_bit(bits, nc, i, sun0 + uint32_t(j))) : uint8_t(0);
+    sVis[r * 128 + c] = (j >= 0 && i < nc) ? uint8_t(vis_byte(bits, nc, i, sun0 + uint32_t(j), bytes)) : uint8_t(0);
   }
   __syncthreads();
 #pragma unroll
@@ -257,7 +264,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (i >= nc) continue;
         const float* rr = sRow + 4 * r;
         double e = (double)acc[fm][fn][v] + (double)rr[3] * gt;
-        if (sVis[r * 128 + c]) e += (double)fmaxf(dlm::dot3(make_float3(rr[0], rr[1], rr[2]), dj), 0.f) * sw;
+        const uint32_t sv = sVis[r * 128 + c];
+        if (sv) e += (double)(fmaxf(dlm::dot3(make_float3(rr[0], rr[1], rr[2]), dj), 0.f) * vis_T(sv)) * sw;
         E[size_t(i) * T + t] = (float)e;
       }
   }
@@ -274,12 +282,12 @@ int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* 
   uint16_t* A = g->seriesWork.as<uint16_t>();
   uint16_t* B = A + aBytes / 2;
   const float3* N = g->N.as<float3>() + g->i0;
-  k_split_a<NS><<<dim3((Kp + 255) / 256, nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, N, g->D.as<float3>(), in->nSky, Kp, nRG,
-                                                                          dWsky, A);
+  k_split_a<NS><<<dim3((Kp + 255) / 256, nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), nc, N, g->D.as<float3>(),
+                                                                          in->nSky, Kp, nRG, dWsky, A);
   k_split_s<NS><<<dim3((Kp + 255) / 256, Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, in->nSky, Kp, in->T, Tp, B);
   const uint32_t tiles = tilesT * tilesI;
   k_series_mfma<NS><<<(tiles + 7) / 8 * 8, 256, 0, g->stream>>>(reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), nc, nRG,
-                                                                 nCG, nKS, in->T, g->bits.as<uint32_t>(), N, g->D.as<float3>(), dG, dg,
+                                                                 nCG, nKS, in->T, g->bits.as<uint32_t>(), uint32_t(g->bytes), N, g->D.as<float3>(), dG, dg,
                                                                  in->sun0, dSunOfCol, dSunW, g->seriesE.as<float>(), tilesT, tiles);
   DLG_CHECK(hipGetLastError());
   return 0;
@@ -316,7 +324,7 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
   if (method == 0) {
     const float3* N = g->N.as<float3>() + g->i0;
     k_series_f32<<<dim3((T + kTile - 1) / kTile, unsigned((nc + kTile - 1) / kTile)), 256, 0, g->stream>>>(
-        g->bits.as<uint32_t>(), uint32_t(nc), N, g->D.as<float3>(), in->nSky, reinterpret_cast<const int*>(b + oRow),
+        g->bits.as<uint32_t>(), uint32_t(g->bytes), uint32_t(nc), N, g->D.as<float3>(), in->nSky, reinterpret_cast<const int*>(b + oRow),
         reinterpret_cast<const float*>(b + oW), g->skyS.as<float>(), T, reinterpret_cast<const float*>(b + oG), dg, in->sun0,
         reinterpret_cast<const int*>(b + oCol), reinterpret_cast<const float*>(b + oSw), g->seriesE.as<float>());
     DLG_CHECK(hipGetLastError());

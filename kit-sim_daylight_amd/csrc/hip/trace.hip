@@ -1,6 +1,6 @@
 // SPDX-License-Identifier: Apache-2.0
-// Visibility bits on CDNA4: replaces OptiX __raygen__bits (reference src/trace_dev.cu:91-108) and gpu_trace's launch loop
-// (src/gpu.cu:499-531). No RT cores: a software traversal per ray of the 4-wide records built from the LBVH (lbvh.hip,
+// Visibility bits on CDNA4: replaces OptiX __raygen__bits (reference src/trace_dev.cu:91-108), byte mode's
+// __raygen__bytes + __anyhit__tr (:115-165) and gpu_trace's launch loop (src/gpu.cu:499-531). No RT cores: a software traversal per ray of the 4-wide records built from the LBVH (lbvh.hip,
 // wide.h), each visit two levels of the binary tree.
 //
 // Launch: a wave64 per 64 consecutive sensors and one 32-direction word (grid y). All lanes of a wave trace the same
@@ -10,6 +10,7 @@
 // A ray is open when every hit on [1e-5, inf) is an own-plane hit (dl_math.h own_hit) and they lie at 3 distinct
 // distances at most (the reference's closest-hit walk past the sensor's own planes, trace_dev.cu:76-88); the first other
 // hit ends the traversal (any-hit, as TERMINATE_ON_FIRST_HIT).
+#include <string>
 #include <type_traits>
 
 #include "dl_math.h"
@@ -46,17 +47,12 @@ struct OwnSet {
 // iterations its wave executed
 struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
 
-// any hit through the 4-wide records (wide.h); the stack holds record indices, nearest child taken first
+// every hit through the 4-wide records (wide.h) until hit(leaf) returns false (then false: the ray is blocked); true when
+// the walk ends. The stack holds record indices, nearest child taken first.
 // OFF32: fewer than 2^25 records, walked through uniform bases and 32-bit offsets (wide.h visit32)
-template <int STACK, bool COUNT, bool OFF32>
-__device__ bool ray_open(const float4* __restrict__ wide, const float4* __restrict__ leaves, float pad, float3 o, float3 nrm, float3 d,
-                         float4 dirA, float4 dirB, bool cull, uint32_t* __restrict__ stk, int lane, Fetches& f) {
-  Ray r;
-  dir_unpack(r, d, dirA, dirB);   // the direction's constants, made once per direction (k_dir_table)
-  ray_origin(r, o, pad);
-  const dlw::Slab sl = dlw::slab(r);
-  const dlw::Bases bs = dlw::bases(wide, sl);
-  OwnSet own;
+template <int STACK, bool COUNT, bool OFF32, class Hit>
+__device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw::Slab& sl, const dlw::Bases& bs, uint32_t* __restrict__ stk,
+                                     int lane, Fetches& f, Hit&& hit) {
   uint32_t cur = 0;
   int sp = 0;
   for (;;) {
@@ -72,7 +68,7 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
       const int c = __ffs(m) - 1;
       const uint32_t leaf = c == 0 ? v.leaf[0] : c == 1 ? v.leaf[1] : c == 2 ? v.leaf[2] : v.leaf[3];
       if (COUNT) ++f.tris;
-      if (!own.add(r, nrm, leaves, leaf, cull)) return false;
+      if (!hit(leaf)) return false;
     }
     if (v.nInner == 0) {
       if (sp == 0) return true;
@@ -91,6 +87,19 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
     sp += h1 + h2 + h3;
     cur = v.inner[0];
   }
+}
+
+// any hit of a visibility ray: open when every hit is an own-plane hit at 3 distinct distances at most
+template <int STACK, bool COUNT, bool OFF32>
+__device__ bool ray_open(const float4* __restrict__ wide, const float4* __restrict__ leaves, float pad, float3 o, float3 nrm, float3 d,
+                         float4 dirA, float4 dirB, bool cull, uint32_t* __restrict__ stk, int lane, Fetches& f) {
+  Ray r;
+  dir_unpack(r, d, dirA, dirB);   // the direction's constants, made once per direction (k_dir_table)
+  ray_origin(r, o, pad);
+  const dlw::Slab sl = dlw::slab(r);
+  const dlw::Bases bs = dlw::bases(wide, sl);
+  OwnSet own;
+  return walk<STACK, COUNT, OFF32>(wide, sl, bs, stk, lane, f, [&](uint32_t leaf) { return own.add(r, nrm, leaves, leaf, cull); });
 }
 
 // COUNT: also adds the lane's node visits and triangle tests to counts[0] / counts[1] (the algorithmic fetches) and the
@@ -123,6 +132,121 @@ __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide,
   }
 }
 
+// Byte mode (reference __raygen__bytes + __anyhit__tr, trace_dev.cu:115-165): what a ray passes, one walk over every hit
+// on [1e-5, inf):
+//  - a hit that is not an own plane: opaque blocks the ray; translucent multiplies its factor (dl_math.h pass_factor)
+//    into pNon, which only shrinks (below 1/510 the ray is blocked: its byte would be 0), and m is the nearest such hit;
+//  - own-plane hits (dl_math.h own_hit) are kept by distance (4 distinct at most; a fifth blocks): whether an opaque one
+//    lies there and the product of the translucent ones' factors.
+// Then without an opaque own plane the ray passes the product of every factor, the reference's any-hit product (its
+// terminate below 1/510 blocks it). With one, the reference reports that hit and walks past the sensor's own planes by
+// closest hits (past_own_plane, :76-88): the own planes nearer than m are passed (3 distances at most, else blocked)
+// and the ray from there passes the product of what lies beyond them, any opaque hit there blocking it.
+// Factors multiply in double, so any hit order rounds to the same float (the reference's float product in OptiX's hit
+// order agrees to an ulp). Where the reference's result depends on its hit order (a translucent own plane in a ray
+// whose product falls below 1/510), the rule above blocks the ray.
+struct ByteRay {
+  double pNon = 1.0, f0 = 1.0, f1 = 1.0, f2 = 1.0, f3 = 1.0;   // f_k: the translucent factors of own planes at t_k
+  float m = 3.40282347e38f, t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+  int n = 0;
+  uint32_t opq = 0;   // bit k: an opaque own plane at t_k
+  static constexpr double kMin = double(1.f / 510.f);
+
+  __device__ __forceinline__ bool own(float tt, bool opaque, double fac) {
+    int k = (n > 0 && t0 == tt) ? 0 : (n > 1 && t1 == tt) ? 1 : (n > 2 && t2 == tt) ? 2 : (n > 3 && t3 == tt) ? 3 : -1;
+    if (k < 0) {
+      if (n == 4) return false;
+      k = n++;
+      if (k == 0) t0 = tt; else if (k == 1) t1 = tt; else if (k == 2) t2 = tt; else t3 = tt;
+    }
+    if (opaque) opq |= 1u << k;
+    else if (k == 0) f0 *= fac; else if (k == 1) f1 *= fac; else if (k == 2) f2 *= fac; else f3 *= fac;
+    return true;
+  }
+  __device__ __forceinline__ bool add(const Ray& r, float3 nrm, const float4* __restrict__ leaves, const float4* __restrict__ slots,
+                                      uint32_t slotLast, uint32_t leaf, bool cull) {
+    const float4 a = leaves[3 * leaf], b = leaves[3 * leaf + 1], c = leaves[3 * leaf + 2];
+    const float3 v0 = make_float3(a.x, a.y, a.z), v1 = make_float3(a.w, b.x, b.y), v2 = make_float3(b.z, b.w, c.x);
+    float tt;
+    bool front;
+    if (!tri_hit(r, v0, v1, v2, 1e-5f, 3.40282347e38f, cull, tt, front)) return true;
+    const uint32_t si = __float_as_uint(c.z);
+    const float4 s = slots[si < slotLast ? si : slotLast];   // rho, tau, tn, flags
+    const bool isOwn = own_hit(r.d, nrm, tt, front, v0, v1, v2);
+    if (!(s.y > 0.f)) return isOwn && own(tt, true, 1.0);
+    const double fac = pass_factor(s.y, s.z, __float_as_uint(s.w), front, v0, v1, v2, r.d);
+    if (isOwn) return own(tt, false, fac);
+    pNon *= fac;
+    m = fminf(m, tt);
+    return pNon >= kMin;
+  }
+  // the own plane k at t with factor f into the product p (false: blocked)
+  __device__ __forceinline__ bool take(int k, float t, double f, double& p, int& below) const {
+    if (k >= n) return true;
+    if (opq && t < m) { ++below; return true; }   // walked past (an opaque own plane was reported)
+    if ((opq >> k) & 1u) return false;
+    p *= f;
+    return true;
+  }
+  __device__ __forceinline__ float value() const {
+    double p = pNon;
+    int below = 0;
+    if (!take(0, t0, f0, p, below) || !take(1, t1, f1, p, below) || !take(2, t2, f2, p, below) || !take(3, t3, f3, p, below)) return 0.f;
+    if (below >= 4 || !(p >= kMin)) return 0.f;
+    return float(p);
+  }
+};
+
+// bytes in quads: word (32 w + 4 q) / 4 of sensor i at vis[(8 w + q) nc + i], direction 32 w + 4 q + j in byte j; every
+// quad of the word is written (0 beyond the last direction)
+template <int STACK, bool OFF32>
+__global__ __launch_bounds__(kWave) void k_bytes(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
+                                                const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
+                                                const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull,
+                                                const float4* __restrict__ slots, uint32_t slotLast, uint32_t* __restrict__ vis) {
+  __shared__ uint32_t stk[STACK * kWave];
+  const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
+  if (i >= nc) return;
+  const float3 n = N[i];
+  const float3 o = offset_ray(P[i], n);
+  Fetches f;
+  uint32_t quad = 0;
+  for (uint32_t k = 0; k < 32; ++k) {
+    const uint32_t d = w * 32 + k;
+    uint32_t b = 0;
+    if (d < nDir) {
+      const float3 dir = D[d];
+      if (n.x * dir.x + n.y * dir.y + n.z * dir.z > 0.f) {   // hemisphere cull (trace_dev.cu:131)
+        if (nTris == 0) b = 255;
+        else {
+          Ray r;
+          dir_unpack(r, dir, dirTab[2 * d], dirTab[2 * d + 1]);
+          ray_origin(r, o, pad);
+          const dlw::Slab sl = dlw::slab(r);
+          const dlw::Bases bs = dlw::bases(wide, sl);
+          ByteRay br;
+          if (walk<STACK, false, OFF32>(wide, sl, bs, stk, int(lane), f,
+                                        [&](uint32_t leaf) { return br.add(r, n, leaves, slots, slotLast, leaf, cull != 0); }))
+            b = T_byte(br.value());
+        }
+      }
+    }
+    quad |= b << (8 * (k & 3));
+    if ((k & 3) == 3) {
+      vis[size_t(w * 8 + k / 4) * nc + i] = quad;
+      quad = 0;
+    }
+  }
+}
+
+template <int STACK>
+void launch_bytes(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32_t nc, uint32_t cull) {
+  const dlg_scene& s = g->scene;
+  const auto k = s.nNodes < (int64_t(1) << 25) ? k_bytes<STACK, true> : k_bytes<STACK, false>;
+  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(), g->dirTab.as<float4>(),
+                                   uint32_t(g->nDir), nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), g->bits.as<uint32_t>());
+}
+
 template <int STACK, bool COUNT>
 void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32_t nc, uint32_t cull, unsigned long long* counts) {
   const dlg_scene& s = g->scene;
@@ -135,18 +259,24 @@ void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32
 
 extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, char* err, int32_t errLen) {
   if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_trace: sensor range exceeds the scene"); return 1; }
-  if (!(rayFlags & 1u)) { dlg_fail(err, errLen, "dlg_trace: transmitting slots (any-hit) are not part of this build"); return 1; }
-  DLG_CHECK(g->bits.grow(size_t(g->nWords) * size_t(nc) * 4));
+  const bool bytes = !(rayFlags & 1u);   // any-hit on: a slot transmits
+  const dlg_scene& s = g->scene;
+  if (bytes && s.nTris > 0 && (g->nSlots == 0 || int64_t(s.slotMax) >= g->nSlots)) {
+    dlg_fail(err, errLen, "dlg_trace: the scene names slot " + std::to_string(s.slotMax) + " and " + std::to_string(g->nSlots) +
+                              " slots are set (dlg_set_slots)");
+    return 1;
+  }
+  DLG_CHECK(g->bits.grow(size_t(g->nWords) * size_t(nc) * (bytes ? 32 : 4)));
   g->i0 = i0;
   g->nc = nc;
+  g->bytes = bytes ? 1 : 0;
   if (nc == 0 || g->nWords == 0) return 0;
-  const dlg_scene& s = g->scene;
   const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
   const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(g->nWords));
   const float3* P = g->P.as<float3>() + i0;
   const float3* N = g->N.as<float3>() + i0;
   unsigned long long* counts = nullptr;
-  if (g->countFetches) {
+  if (g->countFetches && !bytes) {
     DLG_CHECK(g->counts.grow(3 * sizeof(unsigned long long)));
     counts = g->counts.as<unsigned long long>();
     DLG_CHECK(hipMemsetAsync(counts, 0, 3 * sizeof(unsigned long long), g->stream));
@@ -165,7 +295,14 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     else launch_bits<160, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
   };
   dlg_tbegin(g);
-  if (counts) launch(std::true_type{});
+  if (bytes) {
+    if (need <= 24) launch_bytes<24>(g, grid, P, N, uint32_t(nc), cull);
+    else if (need <= 32) launch_bytes<32>(g, grid, P, N, uint32_t(nc), cull);
+    else if (need <= 48) launch_bytes<48>(g, grid, P, N, uint32_t(nc), cull);
+    else if (need <= 64) launch_bytes<64>(g, grid, P, N, uint32_t(nc), cull);
+    else if (need <= 96) launch_bytes<96>(g, grid, P, N, uint32_t(nc), cull);
+    else launch_bytes<160>(g, grid, P, N, uint32_t(nc), cull);
+  } else if (counts) launch(std::true_type{});
   else launch(std::false_type{});
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);

@@ -290,6 +290,30 @@ int32_t dlx_gpu_counters(double* out, int32_t n) {
 
 void dlx_chunk_bytes(int64_t bytes) { chunkOverride.store(bytes > 0 ? bytes : 0); }
 
+int32_t dlx_thin_solid(const dl_mesh* mesh, int32_t slot, const double* centre, uint32_t* flags) {
+  return guard([&]() -> int32_t {
+    if (!mesh || !flags || mesh->meshes.size() != 1 || slot < 0) return fail(DL_USAGE, "dlx_thin_solid: one mesh, a slot >= 0 and flags");
+    const double zero[3] = {0, 0, 0};
+    *flags = thin_solid(mesh->meshes[0], uint32_t(slot), centre ? centre : zero);
+    return DL_OK;
+  });
+}
+
+int32_t dlx_triangle_slots(const dl_mesh* mesh, uint32_t slotBase, uint32_t* slots, int64_t cap, int64_t* n) {
+  return guard([&]() -> int32_t {
+    if (!mesh || !n || mesh->meshes.size() != 1) return fail(DL_USAGE, "dlx_triangle_slots: one mesh and n");
+    const SimulationMesh& m = mesh->meshes[0];
+    std::vector<float3> v(m.numPoints());
+    std::vector<uint3> tri(tri_cap(m));
+    std::vector<uint32_t> ts(tri.size());
+    const double zero[3] = {0, 0, 0};
+    int skipped = 0;
+    *n = int64_t(triangulate_into(m, zero, v.data(), tri.data(), ts.data(), 0, slotBase, skipped));
+    for (int64_t k = 0; slots && k < *n && k < cap; ++k) slots[k] = ts[size_t(k)];
+    return DL_OK;
+  });
+}
+
 int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t cap, int64_t* rows, double* latLonTz) {
   return guard([&]() -> int32_t {
     if (!path || !rows) return fail(DL_USAGE, "dlx_read_epw: a NULL path or rows");

@@ -34,7 +34,7 @@ const char* const kUsage =
     "  --sky-density 1|2|4  --patch-samples K  --ground RHO  --north DEG  --offset M  --no-self-occlude\n"
     "  --no-occlude NAME[;NAME...]  sensor meshes analysed but casting no shade\n"
     "  --virtual-ground on|off  ground below the horizon reflecting --ground RHO (default on)\n"
-    "  --optics SPEC[;SPEC...]  NAME[/SLOT]=RHO[,TAU[,glass|porous]] (transmitting slots: not in this build)\n"
+    "  --optics SPEC[;SPEC...]  NAME[/SLOT]=RHO[,TAU[,glass|porous]]\n"
     "  --transmission off|on  --fresnel on|off  --bounces N (0 only in this build)  --bounce-samples M\n"
     "  --out FILE[.npy]         values (text rows, or float64 [N x C] NumPy when .npy) + FILE.json manifest\n"
     "  --timeseries FILE.npy  --dump-bits FILE.npy  --dump-sun FILE\n"
@@ -287,7 +287,7 @@ const char* const kUsage =
     "  --virtual-ground on|off  a ground plane ends downward rays (default on)   --ground-z Z  its height\n"
     "  --reach D[,D...]|en17037 share of the field that sees at least D metres (at most 8)\n"
     "  --on faces|vertices  --sides front|back|both  --cull-backfaces  --north DEG  --offset M  --no-self-occlude\n"
-    "  --no-occlude NAME[;NAME...]  --optics SPEC[;SPEC...]  --transmission off|on (glass panes: not in this build)\n"
+    "  --no-occlude NAME[;NAME...]  --optics SPEC[;SPEC...]  --transmission off|on\n"
     "  --out FILE[.npy]         values + FILE.json manifest    --dump-rays F.npy  classes uint8 + F.npy.depth.npy\n"
     "  --bench [NSENS NTRIS]  -v          exit code: 1 usage, 2 runtime error\n";
 

@@ -18,6 +18,7 @@
 #include <unordered_map>
 
 #include "core.h"
+#include "dl_math.h"
 
 namespace daylight {
 namespace {
@@ -162,17 +163,23 @@ size_t tri_cap(const SimulationMesh& m) {
   return n;
 }
 
-size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* vOut, uint3* tri, unsigned base, int& skipped) {
+size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* vOut, uint3* tri, uint32_t* slots, unsigned base, uint32_t slotBase,
+                        int& skipped) {
   const V3 C{c[0], c[1], c[2]};
   for (size_t i = 0; i < m.numPoints(); ++i) vOut[i] = f3(pt(m, int(i)) - C);
   size_t made = 0, off = 0;
-  const auto emit = [&](int a, int b, int d) { tri[made++] = make_uint3(base + unsigned(a), base + unsigned(b), base + unsigned(d)); };
+  uint32_t slot = slotBase;   // the current face's global slot
+  const auto emit = [&](int a, int b, int d) {
+    if (slots) slots[made] = slot;
+    tri[made++] = make_uint3(base + unsigned(a), base + unsigned(b), base + unsigned(d));
+  };
   std::vector<int> ear;
   std::vector<std::array<double, 2>> poly;
   for (size_t f = 0; f < m.counts.size(); ++f) {
     const int n = m.counts[f];
     const int* idx = m.indices.data() + off;
     off += size_t(n);
+    slot = slotBase + (m.slot.empty() ? 0u : uint32_t(m.slot[f]));
     if (n == 3) emit(idx[0], idx[1], idx[2]);
     else if (n == 4) { emit(idx[0], idx[1], idx[2]); emit(idx[0], idx[2], idx[3]); }
     else if (n >= 5) {
@@ -195,6 +202,50 @@ size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* vOut
 }
 
 size_t slot_count(const SimulationMesh& m) { return m.slot.empty() ? 1 : size_t(*std::max_element(m.slot.begin(), m.slot.end())) + 1; }
+
+uint32_t thin_solid(const SimulationMesh& m, uint32_t s, const double c[3]) {
+  const V3 C{c[0], c[1], c[2]};
+  // the corners of the slot's faces welded to a 1e-5 m grid (ids in order of first appearance), the faces' directed
+  // edges between the welded ids (degenerate ones dropped), the area and the signed volume (sum N . p0 / 6 over planar
+  // faces, |N| = twice the area)
+  std::unordered_map<uint64_t, std::vector<std::pair<std::array<long long, 3>, uint32_t>>> grid;
+  uint32_t nIds = 0;
+  const auto weld = [&](V3 p) {
+    const std::array<long long, 3> k{std::llround(p.x * 1e5), std::llround(p.y * 1e5), std::llround(p.z * 1e5)};
+    const uint64_t h = uint64_t(k[0]) * 0x9E3779B97F4A7C15ull ^ uint64_t(k[1]) * 0xC2B2AE3D27D4EB4Full ^ uint64_t(k[2]) * 0x165667B19E3779F9ull;
+    auto& bucket = grid[h];
+    for (const auto& e : bucket) if (e.first == k) return e.second;
+    bucket.emplace_back(k, nIds);
+    return nIds++;
+  };
+  std::vector<uint64_t> fwd, rev;
+  std::vector<uint32_t> ids;
+  double area = 0, vol = 0;
+  size_t off = 0;
+  for (size_t f = 0; f < m.counts.size(); ++f) {
+    const int n = m.counts[f];
+    const int* idx = m.indices.data() + off;
+    off += size_t(n);
+    if ((m.slot.empty() ? 0u : uint32_t(m.slot[f])) != s) continue;
+    const V3 N = newell(m, idx, n);
+    area += len(N) / 2;
+    vol += dot(N, pt(m, idx[0]) - C) / 6;
+    ids.resize(size_t(n));
+    for (int j = 0; j < n; ++j) ids[size_t(j)] = weld(pt(m, idx[j]) - C);
+    for (int j = 0; j < n; ++j) {
+      const uint64_t a = ids[size_t(j)], b = ids[size_t((j + 1) % n)];
+      if (a == b) continue;
+      fwd.push_back(a << 32 | b);
+      rev.push_back(b << 32 | a);
+    }
+  }
+  if (!(area > 0) || 2 * std::fabs(vol) / area >= 0.1) return 0;
+  std::sort(fwd.begin(), fwd.end());
+  std::sort(rev.begin(), rev.end());
+  if (fwd != rev) return 0;   // open, or wound against itself
+  const double th = 2 * std::fabs(vol) / area;
+  return DLG_SLOT_THIN | (th < 1e-5 ? DLG_SLOT_TWIN : vol < 0 ? DLG_SLOT_INWARD : 0u);
+}
 
 bool mesh_ok(const SimulationMesh& m, const std::string& label, std::string& err) {
   uint64_t sum = 0;
@@ -261,7 +312,8 @@ bool resolve_optics(const std::vector<SimulationMeshView>& meshes, const std::ve
         if (rank == 2 && size_t(o.slot) >= e - b) { err = row + "slot out of range (the mesh has slots 0.." + std::to_string(e - b - 1) + ")"; return false; }
         if (rank == 2) { b += size_t(o.slot); e = b + 1; }
       }
-      std::fill(out.begin() + std::ptrdiff_t(b), out.begin() + std::ptrdiff_t(e), SlotOptics{float(o.rho), float(o.tau), o.glass});
+      const SlotOptics so{float(o.rho), float(o.tau), o.glass ? float(dlm::tn_from_Tn(o.tau)) : 0.f, o.glass ? DLG_SLOT_GLASS : 0u};
+      std::fill(out.begin() + std::ptrdiff_t(b), out.begin() + std::ptrdiff_t(e), so);
     }
   return true;
 }

@@ -4,8 +4,10 @@
 // loop: trace -> every quantity reduced in one pass -> (series) the contraction on MFMA, streamed to .npy.
 // Host work here is set-up of size O(steps + patches + directions) and the file I/O; everything per ray, per sensor
 // or per (step, patch) runs on the GPU.
-// Not in this build (the reference's next rows): transmitting slots (--transmission on with tau > 0),
-// reflected light (--bounces), the Ladybug emulation mode. A run that asks for them fails with a message saying so.
+// Transmitting slots (--transmission on, tau > 0) trace in byte mode: a transmittance byte per direction instead of a bit,
+// which the reduces and the contraction weight by.
+// Not in this build: reflected light (--bounces), the Ladybug emulation mode. A run that asks for them fails with a
+// message saying so.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -72,7 +74,9 @@ dlg_gpu* process_gpu(std::string& err) {
   return R.gpu;
 }
 
-bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, const std::string& key,
+Scene& installed_scene() { return resident().cur; }
+
+bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
                    int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage,
                    std::string& err) {
   Resident& R = resident();
@@ -104,10 +108,11 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
   stage = "triangulating the scene";
   const auto tPrep = Clk::now();
   bbox_centre(meshes, s.c);
-  std::vector<size_t> vBase(meshes.size() + 1, 0), tBase(meshes.size() + 1, 0);
+  std::vector<size_t> vBase(meshes.size() + 1, 0), tBase(meshes.size() + 1, 0), sBase(meshes.size() + 1, 0);
   for (size_t k = 0; k < meshes.size(); ++k) {
     vBase[k + 1] = vBase[k] + (blocks(k) ? meshes[k].mesh->numPoints() : 0);
     tBase[k + 1] = tBase[k] + (blocks(k) ? tri_cap(*meshes[k].mesh) : 0);
+    sBase[k + 1] = sBase[k] + slot_count(*meshes[k].mesh);   // global slots: every mesh's, in input order
   }
   const size_t nPts = vBase.back(), nTri = tBase.back();
   stage = "making room on the GPU";
@@ -123,26 +128,33 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
   Tris tris;
   tris.v.resize(nPts);
   tris.i.resize(nTri);
+  std::vector<uint32_t> triSlot(slotted ? nTri : 0);
   std::vector<size_t> made(meshes.size(), 0);
   std::vector<int> skip(meshes.size(), 0);
   par_for(meshes.size(), [&](size_t k) {
     if (!blocks(k)) return;
-    made[k] = triangulate_into(*meshes[k].mesh, s.c, tris.v.data() + vBase[k], tris.i.data() + tBase[k], unsigned(vBase[k]), skip[k]);
+    made[k] = triangulate_into(*meshes[k].mesh, s.c, tris.v.data() + vBase[k], tris.i.data() + tBase[k], slotted ? triSlot.data() + tBase[k] : nullptr,
+                               unsigned(vBase[k]), uint32_t(sBase[k]), skip[k]);
   });
   size_t at = 0;   // close the gaps of faces that drew fewer than n - 2 triangles
   for (size_t k = 0; k < meshes.size(); ++k) {
-    if (at != tBase[k] && made[k]) std::memmove(tris.i.data() + at, tris.i.data() + tBase[k], made[k] * sizeof(uint3));
+    if (at != tBase[k] && made[k]) {
+      std::memmove(tris.i.data() + at, tris.i.data() + tBase[k], made[k] * sizeof(uint3));
+      if (slotted) std::memmove(triSlot.data() + at, triSlot.data() + tBase[k], made[k] * sizeof(uint32_t));
+    }
     if (blocks(k)) s.order.push_back({meshes[k].name, at, made[k], meshes[k].mesh});
     at += made[k];
     s.skipped += skip[k];
   }
   tris.i.resize(at);
+  if (slotted) triSlot.resize(at);
   out.stats.prepMs += ms(tPrep);
   stage = "building the scene";
   const auto tBuild = Clk::now();
   char e[512] = {};
   if (dlg_build(g, reinterpret_cast<const float*>(tris.v.data()), int64_t(tris.v.size()), reinterpret_cast<const uint32_t*>(tris.i.data()),
-                int64_t(tris.i.size()), e, sizeof e)) {
+                int64_t(tris.i.size()), e, sizeof e) ||
+      (slotted && dlg_set_tri_slots(g, triSlot.data(), int64_t(triSlot.size()), e, sizeof e))) {
     err = e;
     return false;
   }
@@ -291,20 +303,28 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   for (const Step& s : steps) durationHours += s.dt;
   out.durationHours = durationHours;
 
-  // ---- geometry roles
+  // ---- geometry roles and optics (run.cpp:263-296). --transmission off: every slot opaque (bits); --fresnel off: glass
+  // passes tau at every angle. Only a slot with faces in the scene transmits (not a sensor mesh's under
+  // --no-self-occlude or --no-occlude, nor an id no face uses).
   std::vector<SlotOptics> slots;
   std::vector<char> open;
   if (!resolve_optics(meshes, opt.optics, slots, err) || !resolve_no_occlude(meshes, opt.noOcclude, open, err)) return false;
   const auto blocks = [&](size_t k) { return !meshes[k].sensor || (opt.selfOcclude && !open[k]); };
-  if (opt.transmission) {   // a slot with faces in the scene that lets light through
-    for (size_t k = 0, b = 0; k < meshes.size(); b += slot_count(*meshes[k++].mesh)) {
-      const SimulationMesh& m = *meshes[k].mesh;
-      if (!blocks(k)) continue;
-      for (size_t f = 0; f < m.numFaces(); ++f)
-        if (slots[b + (m.slot.empty() ? 0 : m.slot[f])].tau > 0) { err = "--transmission on with transmitting slots is not part of this build"; return false; }
-    }
+  std::vector<char> inGas(opt.transmission ? slots.size() : 0, 0);
+  for (size_t k = 0, b = 0; !inGas.empty() && k < meshes.size(); b += slot_count(*meshes[k++].mesh)) {
+    const SimulationMesh& m = *meshes[k].mesh;
+    if (!blocks(k)) continue;
+    if (m.slot.empty()) inGas[b] = !m.counts.empty();
+    else for (const auto f : m.slot) inGas[b + size_t(f)] = 1;
   }
-  const std::string key = opt.sceneKey.empty() ? "" : opt.sceneKey + open_key(open, opt.selfOcclude);
+  std::string lit;   // the transmitting slots: their triangles' slots are kept with the scene, so part of its identity
+  for (size_t s = 0; s < slots.size(); ++s) {
+    if (!opt.transmission || !inGas[s]) slots[s].tau = 0;
+    if (!opt.fresnel) slots[s].flags &= ~DLG_SLOT_GLASS;
+    if (slots[s].tau > 0) lit += (lit.empty() ? "" : ",") + std::to_string(s);
+  }
+  const bool translucent = !lit.empty();
+  const std::string key = opt.sceneKey.empty() ? "" : opt.sceneKey + (translucent ? "\ntransmitting " + lit : "") + open_key(open, opt.selfOcclude);
   const int nS = opt.sides == Sides::Both ? 2 : 1;
   size_t nElem = 0;
   for (const SimulationMeshView& v : meshes) if (v.sensor) nElem += opt.onVertices ? v.mesh->numPoints() : v.mesh->numFaces();
@@ -394,7 +414,8 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
 
   // ---- the scene, the sensors, the result
   const size_t nWords = (nDir + 31) / 32;
-  const size_t perSensor = std::max<size_t>(1, std::max(4 * nWords, series ? 4 * size_t(Tup) : size_t(0)));
+  // per sensor of a chunk: its visibility (bits, or a byte per direction in whole words) or its series row
+  const size_t perSensor = std::max<size_t>(1, std::max((translucent ? 32 : 4) * nWords, series ? 4 * size_t(Tup) : size_t(0)));
   size_t chunkBytes = opt.chunkBytes;
   File sf(nullptr, std::fclose), bf(nullptr, std::fclose);
   std::vector<double> seriesHoys;
@@ -405,18 +426,40 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   }
   if (!opt.dumpBitsPath.empty()) {
     bf.reset(std::fopen(opt.dumpBitsPath.c_str(), "wb"));
-    if (!bf || !npy_write_header(bf.get(), nSens, nWords, "<u4")) { err = "cannot write " + opt.dumpBitsPath; return false; }
+    if (!bf || !npy_write_header(bf.get(), nSens, translucent ? nDir : nWords, translucent ? "<u1" : "<u4")) { err = "cannot write " + opt.dumpBitsPath; return false; }
   }
   dlg_gpu* gp = nullptr;
   Scene sc;
-  if (!scene_acquire(meshes, blocks, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
+  if (!scene_acquire(meshes, blocks, translucent, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
   mark("scene");
   dlg_stats_t st0{};
+  std::vector<uint32_t>& thin = sc.thin;
   if (gp) {
     dlg_stats(gp, &st0);
+    if (translucent && !out.sceneReused) {   // a fresh build's thin solids, from the same meshes and centre, kept with it
+      stage = "finding thin solids";
+      const auto tThin = Clk::now();
+      Scene& cur = installed_scene();
+      cur.key.clear();   // no run reuses the scene before they are known
+      thin.assign(slots.size(), 0);
+      std::vector<size_t> sBase(meshes.size() + 1, 0);
+      for (size_t k = 0; k < meshes.size(); ++k) sBase[k + 1] = sBase[k] + slot_count(*meshes[k].mesh);
+      par_for(meshes.size(), [&](size_t k) {
+        if (!blocks(k)) return;
+        const SimulationMesh& m = *meshes[k].mesh;
+        for (size_t s = 0; s < slot_count(m); ++s)
+          if (slots[sBase[k] + s].tau > 0) thin[sBase[k] + s] = thin_solid(m, uint32_t(s), sc.c);
+      });
+      cur.thin = thin;
+      cur.key = key;
+      out.stats.prepMs += ms(tThin);
+    }
     const size_t buffers = 1 + (series ? 3 : 0);   // the bits; the series' output and its split operands
     if (!run_admission(gp, nSens, perSensor, buffers * perSensor + 8 * (nQ + 1), 0, opt.residentScenes, chunkBytes, err)) return false;
   }
+  // A thin solid's exit face passes everything. With --cull-backfaces a closed solid is crossed through one front face
+  // anyway (the other is culled): tau once without the rule.
+  for (size_t s = 0; !opt.cullBackfaces && s < slots.size() && s < thin.size(); ++s) slots[s].flags |= thin[s];
   stage = "making the sensors";
   const auto tSens = Clk::now();
   Sensors scratch;
@@ -429,11 +472,17 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
     std::fprintf(stderr, "info: %zu sensors, %zu triangles%s, %zu steps (%.1f h)\n", nSens, sc.nTris, out.sceneReused ? " (scene reused)" : "", T, durationHours);
     if (sc.skipped) std::fprintf(stderr, "warn: %d degenerate faces skipped in occluders\n", sc.skipped);
     if (!warn.empty()) std::fprintf(stderr, "warn: %s\n", warn.c_str());
+    if (translucent) {
+      size_t nLit = 0, nThin = 0;
+      for (const SlotOptics& s : slots) { nLit += s.tau > 0; nThin += (s.flags & DLG_SLOT_THIN) != 0; }
+      std::fprintf(stderr, "info: %zu of %zu optics slots transmit (%zu thin solids)\n", nLit, slots.size(), nThin);
+    }
   }
   if (gp) {
     char e[512] = {};
     if (dlg_set_scene(gp, reinterpret_cast<const float*>(sens->p.data()), reinterpret_cast<const float*>(sens->n.data()), int64_t(nSens),
                       reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e)) { err = e; return false; }
+    if (translucent && dlg_set_slots(gp, reinterpret_cast<const dlg_slot*>(slots.data()), int64_t(slots.size()), e, sizeof e)) { err = e; return false; }
   }
   scratch = Sensors{};
   mark("upload");
@@ -461,13 +510,14 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   if (gp) {
     stage = "tracing";
     const size_t ncMax = std::max<size_t>(1, chunkBytes / perSensor);
-    const uint32_t flags = dlg_ray_flags(opt.cullBackfaces ? 1 : 0, 0);
+    const uint32_t flags = dlg_ray_flags(opt.cullBackfaces ? 1 : 0, translucent ? 1 : 0);
     std::vector<int32_t> useCos(nQ + 1, 1);
     for (size_t qi = 0; qi < nQ; ++qi) useCos[qi] = opt.quantities[qi] != Quantity::SunHours;
     const int nR = int(nQ) + (series ? 1 : 0);
     std::vector<double> tmp;
     std::vector<float> G, E;
     std::vector<uint32_t> bitsBuf, bitRows;
+    std::vector<uint8_t> byteBuf, byteRows;
     std::vector<float> wsky(nSky), sunW(nSun, 0.f);
     std::vector<int32_t> sunCol(nSun, 0);
     for (size_t d = 0; d < nSky; ++d) wsky[d] = pa.omega[size_t(rowOf[d])] / float(K);
@@ -490,7 +540,14 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
         for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, tmp[qi * nc + i]);
       out.stats.reduceMs += ms(tRed);
       if (opt.progress) opt.progress->store(double(i0 + nc) / double(nSens));
-      if (bf) {
+      if (bf && translucent) {   // direction-major bytes -> row-major [nc x nDir] rows
+        byteBuf.resize(nDir * nc);
+        if (dlg_download_bytes(gp, int64_t(nc), byteBuf.data(), e, sizeof e)) { err = e; return false; }
+        byteRows.resize(nc * nDir);
+        for (size_t d = 0; d < nDir; ++d)
+          for (size_t i = 0; i < nc; ++i) byteRows[i * nDir + d] = byteBuf[d * nc + i];
+        if (std::fwrite(byteRows.data(), 1, byteRows.size(), bf.get()) != byteRows.size()) { err = "write failed: " + opt.dumpBitsPath; return false; }
+      } else if (bf) {
         bitsBuf.resize(nWords * nc);
         if (dlg_download_bits(gp, int64_t(nc), bitsBuf.data(), e, sizeof e)) { err = e; return false; }
         bitRows.resize(nc * nWords);

@@ -1,7 +1,7 @@
 // SPDX-License-Identifier: Apache-2.0
 // views::field / views::run (reference src/views.cpp): the field and its checks, the cell weights, the target labels on
 // the shared resident scene, the chunk loop over the GPU (dlg_view -> dlg_reduce_view), the columns and --dump-rays.
-// Glass panes that rays pass (--transmission on with glass slots) are not part of this build: such a run fails saying so.
+// With --transmission on, rays pass glass slots (label 255, at most 16 panes) and count toward the glazing column.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -76,16 +76,26 @@ static bool run_views(const std::vector<SimulationMeshView>& meshes, const Optio
   if (!field(meshes, opt, fd, err) || !resolve_no_occlude(meshes, opt.noOcclude, open, err) ||
       !resolve_targets(meshes, opt.targets, opt.selfOcclude, open, groups, clsOf, err)) return false;
   const auto blocks = [&](size_t k) { return !meshes[k].sensor || (opt.selfOcclude && !open[k]); };
+  // glass: with --transmission on rays pass the global slots that are glass with tau > 0 (numbered as daylight numbers
+  // them); per blocking mesh all or some of its faces (views.cpp:86-106)
   std::vector<SlotOptics> slots;
   if (!resolve_optics(meshes, opt.optics, slots, err)) return false;
-  if (opt.transmission) {
-    for (size_t k = 0, b = 0; k < meshes.size(); b += slot_count(*meshes[k++].mesh)) {
-      if (!blocks(k)) continue;
-      const SimulationMesh& m = *meshes[k].mesh;
-      for (size_t f = 0; f < m.numFaces(); ++f) {
-        const SlotOptics& s = slots[b + (m.slot.empty() ? 0 : m.slot[f])];
-        if (s.glass && s.tau > 0) { err = "views: rays passing glass (--transmission on with glass slots) are not part of this build"; return false; }
-      }
+  std::vector<size_t> base(meshes.size() + 1, 0);
+  for (size_t k = 0; k < meshes.size(); ++k) base[k + 1] = base[k] + slot_count(*meshes[k].mesh);
+  const auto passes = [&](size_t s) { return opt.transmission && (slots[s].flags & DLG_SLOT_GLASS) && slots[s].tau > 0; };
+  std::vector<char> passAll(meshes.size(), 0), passAny(meshes.size(), 0);
+  bool anyPass = false;
+  for (size_t k = 0; opt.transmission && k < meshes.size(); ++k) {
+    if (!blocks(k)) continue;
+    const SimulationMesh& m = *meshes[k].mesh;
+    size_t n = 0;
+    for (size_t f = 0; f < m.numFaces(); ++f) n += passes(base[k] + (m.slot.empty() ? 0 : m.slot[f])) ? 1 : 0;
+    passAny[k] = n > 0;
+    passAll[k] = n > 0 && n == m.numFaces();
+    anyPass = anyPass || n > 0;
+    if (clsOf[k] != 2 && passAll[k]) {
+      err = "--target " + meshes[k].name + ": rays pass through it (glass with --transmission on), none can end on it";
+      return false;
     }
   }
   const std::string key = opt.sceneKey.empty() ? "" : opt.sceneKey + open_key(open, opt.selfOcclude);
@@ -125,27 +135,41 @@ static bool run_views(const std::vector<SimulationMeshView>& meshes, const Optio
   }
   dlg_gpu* g = nullptr;
   Scene sc;
-  if (!scene_acquire(meshes, blocks, key, opt.residentScenes, opt.cancel, &g, sc, out, stage, err)) return false;
-  // labels per triangle of the scene (build order): 2, or 3 + the group of a target mesh
+  if (!scene_acquire(meshes, blocks, false, key, opt.residentScenes, opt.cancel, &g, sc, out, stage, err)) return false;
+  // labels per triangle of the scene (build order): 2, or 3 + the group of a target mesh, or kLabelPass (255) for glass
+  // that rays pass, a mixed mesh's triangles found by triangulating it again as the scene did (same centre, same faces
+  // kept); a target or glass mesh the scene lacks (built under other names): a fresh build under the key
   std::vector<uint8_t> label;
   const auto labelled = [&] {
     label.assign(sc.nTris, 2);
     std::unordered_map<std::string, const SceneMesh*> byName;
     for (const SceneMesh& s : sc.order) if (!byName.emplace(s.name, &s).second) byName[s.name] = nullptr;
     for (size_t k = 0; k < meshes.size(); ++k) {
-      if (clsOf[k] == 2) continue;
+      if (clsOf[k] == 2 && !passAny[k]) continue;
       const auto it = byName.find(meshes[k].name);
       if (it == byName.end() || !it->second || it->second->mesh != meshes[k].mesh) return false;
-      std::fill_n(label.data() + it->second->tri0, it->second->nTri, clsOf[k]);
+      const SceneMesh& s = *it->second;
+      std::fill_n(label.data() + s.tri0, s.nTri, passAll[k] ? uint8_t(255) : clsOf[k]);
+      if (passAny[k] && !passAll[k]) {
+        const SimulationMesh& m = *meshes[k].mesh;
+        std::vector<float3> v(m.numPoints());
+        std::vector<uint3> tri(tri_cap(m));
+        std::vector<uint32_t> ts(tri.size());
+        int skipped = 0;
+        const size_t n = triangulate_into(m, sc.c, v.data(), tri.data(), ts.data(), 0, uint32_t(base[k]), skipped);
+        if (n != s.nTri) return false;
+        for (size_t t = 0; t < n; ++t) if (passes(ts[t])) label[s.tri0 + t] = 255;
+      }
     }
     return true;
   };
-  stage = "labelling the targets";
-  if (!groups.empty() && !labelled()) {
+  stage = "labelling the targets and the glass";
+  if ((!groups.empty() || anyPass) && !labelled()) {
     scene_forget(key);
-    if (!scene_acquire(meshes, blocks, key, opt.residentScenes, opt.cancel, &g, sc, out, stage, err)) return false;
-    if (!labelled()) { err = "internal: a target mesh is not in the scene as built"; return false; }
+    if (!scene_acquire(meshes, blocks, false, key, opt.residentScenes, opt.cancel, &g, sc, out, stage, err)) return false;
+    if (!labelled()) { err = "internal: a target or glass mesh is not in the scene as built"; return false; }
   }
+  const size_t nPass = size_t(std::count(label.begin(), label.end(), uint8_t(255)));
   dlg_stats_t st0{};
   dlg_stats(g, &st0);
   size_t chunkBytes = opt.chunkBytes;
@@ -166,6 +190,7 @@ static bool run_views(const std::vector<SimulationMeshView>& meshes, const Optio
                  out.sceneReused ? " (scene reused)" : "", fd.nEl, fd.nAz, K, groups.size(), vg);
     if (sc.skipped) std::fprintf(stderr, "warn: %d degenerate faces skipped in occluders\n", sc.skipped);
     if (!warn.empty()) std::fprintf(stderr, "warn: %s\n", warn.c_str());
+    if (opt.transmission) std::fprintf(stderr, "info: --transmission on: rays pass %zu glass triangles\n", nPass);
   }
   char e[512] = {};
   if (dlg_set_scene(g, reinterpret_cast<const float*>(sens->p.data()), reinterpret_cast<const float*>(sens->n.data()), int64_t(nSens), nullptr, 0, e, sizeof e) ||

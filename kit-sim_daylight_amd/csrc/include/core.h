@@ -47,13 +47,23 @@ inline std::vector<SimulationMeshView> views_of(const std::vector<SimulationMesh
 void par_for(size_t n, const std::function<void(size_t)>& f);
 void bbox_centre(const std::vector<SimulationMeshView>& meshes, double c[3]);
 size_t tri_cap(const SimulationMesh& m);
-size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* v, uint3* tri, unsigned base, int& skipped);
+// m's triangles (points recentred by c, indices from base) into v / tri; with slots non-null, each triangle's global slot
+// (slotBase + its face's slot) into slots. Returns the triangles made (degenerate n-gons skipped and counted).
+size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* v, uint3* tri, uint32_t* slots, unsigned base, uint32_t slotBase,
+                        int& skipped);
 size_t slot_count(const SimulationMesh& m);
 bool mesh_ok(const SimulationMesh& m, const std::string& label, std::string& err);
 bool optics_ok(const Options& opt, std::string& err);
 bool optics_rows_ok(const std::vector<Optics>& rows, std::string& err);
-// per global slot: does the slot transmit (tau > 0; glass with tau > 0) once resolved by precedence NAME/SLOT > NAME > *
-struct SlotOptics { float rho = 0.2f, tau = 0.f; bool glass = false; };
+// per global slot its optics once resolved by precedence NAME/SLOT > NAME > *: the device's record (dl_device.h dlg_slot):
+// rho, tau, tn (glass: tn_from_Tn(tau)), flags (DLG_SLOT_GLASS; a run adds DLG_SLOT_THIN / INWARD / TWIN)
+struct SlotOptics { float rho = 0.2f, tau = 0.f, tn = 0.f; uint32_t flags = 0; };
+static_assert(sizeof(SlotOptics) == sizeof(dlg_slot), "a slot uploads as the device's record");
+// Faces of slot s of m (recentred by c) as one closed thin solid (reference mesh.cpp:275-304): DLG_SLOT_THIN when the
+// corners welded on a 1e-5 m grid close it (every directed edge matched by its reverse), 2 |V| / S < 0.1 m, with
+// DLG_SLOT_INWARD when its signed volume is negative or DLG_SLOT_TWIN instead when 2 |V| / S is under 1e-5 m (coincident
+// face pairs: a double-faced sheet); else 0.
+uint32_t thin_solid(const SimulationMesh& m, uint32_t s, const double c[3]);
 bool resolve_optics(const std::vector<SimulationMeshView>& meshes, const std::vector<Optics>& rows, std::vector<SlotOptics>& out, std::string& err);
 bool resolve_no_occlude(const std::vector<SimulationMeshView>& meshes, const std::vector<std::string>& names, std::vector<char>& open, std::string& err);
 bool resolve_targets(const std::vector<SimulationMeshView>& meshes, const std::vector<views::Target>& targets, bool selfOcclude,
@@ -69,12 +79,17 @@ struct Scene {
   size_t nTris = 0;
   int skipped = 0;
   std::vector<SceneMesh> order;
+  std::vector<uint32_t> thin;   // per global slot: thin_solid of a transmitting slot (a slotted build's)
   dlg_scene* parked = nullptr;
 };
 // The process's GPU (made on first use, never destroyed), or null with err.
 dlg_gpu* process_gpu(std::string& err);
-bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, const std::string& key,
+// The run's scene: the installed or a parked one under key, else built from the blocking meshes (slotted: each
+// triangle's global slot kept with the leaves, dlg_set_tri_slots).
+bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
                    int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err);
+// the process's installed scene (a run sets the thin solids of a fresh slotted build on it)
+Scene& installed_scene();
 bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
                    std::string& err);
 std::string open_key(const std::vector<char>& open, bool selfOcclude);

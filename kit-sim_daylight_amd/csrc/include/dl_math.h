@@ -140,15 +140,25 @@ DL_HD bool tri_hit(const Ray& r, float3 v0, float3 v1, float3 v2, float tmin, fl
   const float Bx = comp(B, r.kx) - r.Sx * Bz0, By = comp(B, r.ky) - r.Sy * Bz0;
   const float Cx = comp(C, r.kx) - r.Sx * Cz0, Cy = comp(C, r.ky) - r.Sy * Cz0;
   float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  bool edge = false;
   if (U == 0.f || V == 0.f || W == 0.f) {   // on an edge in float: decide with exact products in double
     U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
     V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
     W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    edge = U == 0.f || V == 0.f || W == 0.f;
   }
   if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
   const float det = U + V + W;
   if (det == 0.f) return false;
   if (cull && det < 0.f) return false;
+  if (edge) {
+    // the ray meets an edge (or a vertex): a top-left rule gives it to one triangle of those that share it, as a
+    // hardware triangle test does (each edge's direction in the winding, turned by the facing: two triangles on either
+    // side of an edge see it in opposite directions)
+    const float sg = det > 0.f ? 1.f : -1.f;
+    const auto mine = [sg](float ex, float ey) { ex *= sg; ey *= sg; return ey > 0.f || (ey == 0.f && ex > 0.f); };
+    if ((U == 0.f && !mine(Cx - Bx, Cy - By)) || (V == 0.f && !mine(Ax - Cx, Ay - Cy)) || (W == 0.f && !mine(Bx - Ax, By - Ay))) return false;
+  }
   const float T = U * (r.Sz * Az0) + V * (r.Sz * Bz0) + W * (r.Sz * Cz0);
   const float t = T / det;
   if (!(t >= tmin && t <= tmax)) return false;
@@ -169,6 +179,50 @@ DL_HD bool own_hit(float3 d, float3 n, float t, bool front, float3 v0, float3 v1
   const float3 hn = make_float3(s * ng.x, s * ng.y, s * ng.z);
   const float ext = fmaxf(fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)), fmaxf(fabsf(e1.z), fabsf(e2.x))), fmaxf(fabsf(e2.y), fabsf(e2.z)));
   return t * fabsf(dot3(d, hn)) < ext * 2.3841858e-7f && dot3(n, hn) <= 0.f;
+}
+
+// ------------------------------------------------------------------------------------------------ transmission
+// Slot flags (reference src/params.h:14-19; include/dl_device.h DLG_SLOT_*)
+constexpr uint32_t kSlotGlass = 1, kSlotThin = 2, kSlotInward = 4;
+
+// A transmittance byte back as the fraction it stands for: b / 255.f, correctly rounded (the quotient in double rounds to
+// the same float: b / 255 repeats an 8-bit pattern, so it never lies within 2^-53 of a float midpoint).
+DL_HD float byte_T(uint32_t b) { return float(double(b) / 255.0); }
+// The byte a transmittance is stored as (trace_dev.cu:141): round to nearest of T x 255 in float.
+DL_HD uint32_t T_byte(float T) { return uint32_t(rintf(T * 255.f)); }
+
+// Glass transmissivity tn from the transmittance at normal incidence Tn (Radiance rad5R4 src/rt/m_glass.c, the form of
+// reference src/glass.h:11-13), at least 0.
+DL_HD double tn_from_Tn(double Tn) {
+  return fmax(0., (sqrt(.8402528435 + .0072522239 * Tn * Tn) - .9166530661) / .0036261119 / Tn);
+}
+
+// Transmittance of a glass pane at incidence cosine c (0..1) for transmissivity tn (reference src/glass.h:17-25, after
+// m_glass.c): refractive index 1.52, the cosine inside c2 by Snell's law, the path's transmissivity tn^(1 / c2), the
+// Fresnel reflectances of the s and p polarisations at each face and the pane's internal reflections summed,
+// (1 - r)^2 d / (1 - r^2 d^2), averaged over the two. In double; the reference's float formula agrees to a few float ulps.
+DL_HD float glass_T(double c, double tn) {
+  const double n = 1.52;
+  const double c2 = sqrt((1. - 1. / (n * n)) + c * c / (n * n));
+  const double d = pow(tn, 1. / c2), d2 = d * d;
+  double rs = (c - n * c2) / (c + n * c2), rp = (c2 - n * c) / (c2 + n * c);
+  rs *= rs;
+  rp *= rp;
+  return float(.5 * (1. - rs) * (1. - rs) * d / (1. - rs * rs * d2) + .5 * (1. - rp) * (1. - rp) * d / (1. - rp * rp * d2));
+}
+
+// What a translucent slot's triangle passes (reference __anyhit__tr, trace_dev.cu:149-165): 1 on a thin solid's exit
+// face (the back face, or the front face when it is wound inward), Radiance glass at the hit's incidence (the geometric
+// normal of v0 v1 v2 in double from the float edges, against the unit direction d), else tau.
+DL_HD float pass_factor(float tau, float tn, uint32_t flags, bool front, float3 v0, float3 v1, float3 v2, float3 d) {
+  if ((flags & kSlotThin) && front == ((flags & kSlotInward) != 0)) return 1.f;
+  if (!(flags & kSlotGlass)) return tau;
+  const double ax = v1.x - v0.x, ay = v1.y - v0.y, az = v1.z - v0.z;
+  const double bx = v2.x - v0.x, by = v2.y - v0.y, bz = v2.z - v0.z;
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double l = sqrt(nx * nx + ny * ny + nz * nz);
+  if (!(l > 0.)) return 0.f;
+  return glass_T(fabs(nx * d.x + ny * d.y + nz * d.z) / l, tn);
 }
 
 // ------------------------------------------------------------------------------------------------ the sun (FP64)

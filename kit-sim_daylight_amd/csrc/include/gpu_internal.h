@@ -39,12 +39,14 @@ struct DevBuf {
 };
 
 // A built occluder scene: LBVH nodes (16 floats each), their 4-wide records (32 floats each, wide.h), leaf triangles
-// (12 floats each), its padding and height.
+// (12 floats each: v0, v1, v2, the original index and the global slot as bits, 1 unused), its padding and height, and
+// the largest slot its leaves name.
 struct dlg_scene {
   DevBuf nodes, wide, leaves;
   int64_t nNodes = 0, nTris = 0;
   int32_t height = 0;
   float pad = 0.f;
+  uint32_t slotMax = 0;
   int64_t bytes() const { return int64_t(nodes.cap + wide.cap + leaves.cap); }
   void release() { nodes.release(); wide.release(); leaves.release(); }
 };
@@ -57,7 +59,10 @@ struct dlg_gpu {
   DevBuf P, N, D, dirTab;
   int64_t nSens = 0, nDir = 0, nWords = 0;
   int64_t i0 = 0, nc = 0;          // the last traced chunk
-  DevBuf bits;                     // [nWords x nc]
+  DevBuf bits;                     // [nWords x nc] bits, or byte mode's [8 nWords x nc] quads of transmittance bytes
+  int32_t bytes = 0;               // the last trace wrote bytes
+  DevBuf slots;                    // dlg_set_slots: [nSlots] dlg_slot
+  int64_t nSlots = 0;
   DevBuf scratch, out;             // per-call inputs / outputs of the reduces
   // sky (dlg_sky): S [P x Tup] and g [Tup] of the last run, the tcos table
   DevBuf skyS, skyG, skyWork;

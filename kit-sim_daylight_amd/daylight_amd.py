@@ -88,6 +88,8 @@ C_ABI = [
     ("dlx_count_fetches", None, [C.c_int32]),
     ("dlx_chunk_bytes", None, [C.c_int64]),
     ("dlx_read_epw", C.c_int32, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("dlx_thin_solid", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_uint32)]),
+    ("dlx_triangle_slots", C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
 ]
 SEAM = [
     ("dlg_init", C.c_int32, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int32]),
@@ -107,6 +109,9 @@ SEAM = [
     ("dlg_trace", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint32, C.c_char_p, C.c_int32]),
     ("dlg_reduce", C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, F32P, I32P, F64P, C.c_char_p, C.c_int32]),
     ("dlg_download_bits", C.c_int32, [C.c_void_p, C.c_int64, U32P, C.c_char_p, C.c_int32]),
+    ("dlg_download_bytes", C.c_int32, [C.c_void_p, C.c_int64, U8P, C.c_char_p, C.c_int32]),
+    ("dlg_set_slots", C.c_int32, [C.c_void_p, F32P, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_set_tri_slots", C.c_int32, [C.c_void_p, U32P, C.c_int64, C.c_char_p, C.c_int32]),
     ("dlg_sky", C.c_int32, [C.c_void_p, C.POINTER(SkyIn), C.POINTER(SkyOut), C.c_char_p, C.c_int32]),
     ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_char_p, C.c_int32]),
     ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
@@ -201,13 +206,32 @@ class Gpu:
         self.nSens = P.size // 3
         _check(self.L.dlg_set_scene(self.h, P, N, P.size // 3, D if D.size else np.zeros(3, np.float32), D.size // 3, self.err, 1024), self.err)
 
-    def trace(self, i0, nc, cull=False):
-        _check(self.L.dlg_trace(self.h, i0, nc, self.L.dlg_ray_flags(int(cull), 0), self.err, 1024), self.err)
+    def trace(self, i0, nc, cull=False, translucent=False):
+        """translucent: byte mode (the slots of set_slots / set_tri_slots)."""
+        _check(self.L.dlg_trace(self.h, i0, nc, self.L.dlg_ray_flags(int(cull), int(translucent)), self.err, 1024), self.err)
 
     def bits(self, nc):
         out = np.zeros(max(1, ((self.nDir + 31) // 32) * nc), np.uint32)
         _check(self.L.dlg_download_bits(self.h, nc, out, self.err, 1024), self.err)
         return out[: ((self.nDir + 31) // 32) * nc]
+
+    def bytes(self, nc):
+        """byte mode's transmittance, direction-major [nDir x nc]"""
+        out = np.zeros(max(1, self.nDir * nc), np.uint8)
+        _check(self.L.dlg_download_bytes(self.h, nc, out, self.err, 1024), self.err)
+        return out[: self.nDir * nc].reshape(self.nDir, nc)
+
+    def set_slots(self, slots):
+        """slots: [n][4] rows rho, tau, tn, flags (flags as an integer: 1 glass, 2 thin, 4 inward, 8 twin)"""
+        a = np.zeros((len(slots), 4), np.float32)
+        for k, (rho, tau, tn, flags) in enumerate(slots):
+            a[k, :3] = (rho, tau, tn)
+            a[k, 3] = np.array([flags], np.uint32).view(np.float32)[0]
+        _check(self.L.dlg_set_slots(self.h, a.reshape(-1), len(slots), self.err, 1024), self.err)
+
+    def set_tri_slots(self, slot):
+        slot = np.ascontiguousarray(slot, np.uint32)
+        _check(self.L.dlg_set_tri_slots(self.h, slot if slot.size else np.zeros(1, np.uint32), slot.size, self.err, 1024), self.err)
 
     def reduce(self, nc, W, useCos):
         W = np.ascontiguousarray(W, np.float32).reshape(len(useCos), -1)

@@ -132,6 +132,16 @@ def lib():
                                    C.c_uint, I32P, F32P, F32P, F64P, C.c_int]
         L.orc_series32.argtypes = [U32P, C.c_long, F32P, F32P, C.c_uint, I32P, F32P, F32P, C.c_int, F32P, F32P, C.c_uint,
                                    C.c_uint, I32P, F32P, F32P, F32P]
+        L.orc_trace_bytes.argtypes = [C.c_void_p, F32P, F32P, C.c_long, F32P, C.c_int, C.c_int, C.c_void_p, F32P, C.c_long, U8P, C.c_int]
+        L.orc_tn_from_Tn.restype = C.c_double
+        L.orc_tn_from_Tn.argtypes = [C.c_double]
+        L.orc_glass_T.restype = C.c_float
+        L.orc_glass_T.argtypes = [C.c_double, C.c_double]
+        L.orc_reduce_bytes.argtypes = [U8P, C.c_long, C.c_int, F32P, F32P, F32P, C.c_int, F64P]
+        L.orc_series64_bytes.argtypes = [U8P, C.c_long, F32P, F32P, C.c_uint, I32P, F32P, F32P, C.c_int, F32P, F32P, C.c_uint,
+                                         C.c_uint, I32P, F32P, F32P, F64P, C.c_int]
+        L.orc_series32_bytes.argtypes = [U8P, C.c_long, F32P, F32P, C.c_uint, I32P, F32P, F32P, C.c_int, F32P, F32P, C.c_uint,
+                                         C.c_uint, I32P, F32P, F32P, F32P]
         L.orc_reduce_view.argtypes = [U8P, F32P, C.c_long, C.c_uint, C.c_uint, F64P, C.c_uint, C.c_double, F32P, C.c_uint,
                                       C.c_int, C.c_uint, C.POINTER(View), C.c_uint, F64P]
         L.orc_daylight_size.restype = C.c_int
@@ -321,11 +331,13 @@ def daylight(meshes, weather=None, quantities=("radiation",), mf=1, patchSamples
 
 def views(meshes, fov=(120.0, -70.0, 50.0), cell=2.0, jitter=True, far=1000.0, reach=(), distance=None, offset=0.0,
           sides="front", onVertices=False, selfOcclude=True, north=0.0, virtualGround=True, groundZ=math.nan,
-          cullBackfaces=False, targets=None, nthreads=None, raw=False):
+          cullBackfaces=False, targets=None, nthreads=None, raw=False, transmission=False, glass=()):
     """The oracle's views run: the host steps of reference src/views.cpp (field :28-66, cell weights :126-136, labels,
     sensors, the launch frame :268-272, the columns :235-266) over the oracle's C kernels (orc_trace_view,
     orc_reduce_view). targets: {mesh name: group}. distance: a set of 'mean', 'min', 'group', 'isovist' or None for
-    the default (isovist at 360, else mean and min). Returns dict(values [N x C], columns, cls, dist when raw)."""
+    the default (isovist at 360, else mean and min). transmission: the glazing column, and rays pass the meshes named
+    in glass (every face glass with tau > 0: label 255, views.cpp:86-106). Returns dict(values [N x C], columns, cls,
+    dist when raw)."""
     L = lib()
     fovH, lo, hi = fov
     flat = lo == 0 and hi == 0
@@ -376,7 +388,7 @@ def views(meshes, fov=(120.0, -70.0, 50.0), cell=2.0, jitter=True, far=1000.0, r
         V[3 * vb:3 * vb + len(Vk)] = Vk
         I[3 * nt:3 * (nt + n)] = Ik[:3 * n]
         g = targets.get(m.name)
-        label += [2 if g is None else 3 + groups.index(g)] * n
+        label += [255 if transmission and m.name in glass else 2 if g is None else 3 + groups.index(g)] * n
         vb += len(m.points) // 3
         nt += n
     label = np.array(label if label else [2], np.uint8)
@@ -402,13 +414,14 @@ def views(meshes, fov=(120.0, -70.0, 50.0), cell=2.0, jitter=True, far=1000.0, r
     L.orc_scene_free(sc)
     iso = "isovist" in dist
     nR = len(reach)
-    NA = 3 * nClass + 1 + nR + (1 if iso else 0)
+    gz = 1 if transmission else 0
+    NA = 3 * nClass + 1 + nR + gz + (1 if iso else 0)
     sums = np.zeros(nSens * NA)
     reachF = np.array(reach if nR else [0.0], np.float32)
-    L.orc_reduce_view(cls, dst, nSens, K, nAz, wRow, nClass, far, reachF, nR, 0, (2 if flat else 3) if iso else 0, C.byref(v), 0, sums)
+    L.orc_reduce_view(cls, dst, nSens, K, nAz, wRow, nClass, far, reachF, nR, gz, (2 if flat else 3) if iso else 0, C.byref(v), 0, sums)
     sums = sums.reshape(nSens, NA)
     # columns (views.cpp:235-266)
-    names = ["sky", "ground", "obstruction"] + groups
+    names = ["sky", "ground", "obstruction"] + groups + (["glazing"] if transmission else [])
     if iso:
         names.append("area" if flat else "volume")
     mean, mn_, grp = "mean" in dist, "min" in dist, "group" in dist
@@ -438,6 +451,8 @@ def views(meshes, fov=(120.0, -70.0, 50.0), cell=2.0, jitter=True, far=1000.0, r
         mn = min(a[3 * k + 2] for k in range(nClass))
         total = w + own
         vals += [a[0] / total, a[3] / total, (a[6] + own) / total] + [a[3 * (3 + k)] / total for k in range(len(groups))]
+        if transmission:
+            vals.append(a[3 * nClass + 1 + nR] / total)
         if iso:
             vals.append(a[NA - 1])
         if mean:

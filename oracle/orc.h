@@ -82,6 +82,15 @@ void orc_scene_free(orc_scene* s);
 /* __raygen__bits semantics (trace_dev.cu:91-108): bits[w nc + i], word-major; cull = back faces culled. */
 void orc_trace_bits(const orc_scene* s, const float* P, const float* N, long nc, const float* D, int nDir, int cull,
                     uint32_t* bits, int nthreads);
+/* __raygen__bytes + __anyhit__tr semantics (trace_dev.cu:115-165): the transmittance byte of every (sensor, direction),
+ * direction-major bytes[d nc + i]; triSlot: the global slot of each triangle (input order; NULL: all slot 0), slots
+ * [nSlots][4] = rho, tau, tn, flags (as bits: 1 glass, 2 thin, 4 inward). The any-hit calls in the order of the sorted
+ * hits, the transmittance a product in double. */
+void orc_trace_bytes(const orc_scene* s, const float* P, const float* N, long nc, const float* D, int nDir, int cull, const uint32_t* triSlot,
+                     const float* slots, long nSlots, uint8_t* bytes, int nthreads);
+/* glass.h: tn_from_Tn, and glass_TR's transmittance (in double, rounded to float) */
+double orc_tn_from_Tn(double Tn);
+float orc_glass_T(double c, double tn);
 /* One ray as the bits pass classifies it: 1 visible, 0 blocked or culled. */
 int orc_visible(const orc_scene* s, const float p[3], const float n[3], const float d[3], int cull);
 /* The same against every triangle (no BVH): the hit set the traversal must reproduce. */
@@ -131,6 +140,15 @@ void orc_series64(const uint32_t* bits, long nc, const float* N, const float* D,
 void orc_series32(const uint32_t* bits, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
                   const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
                   const float* sunVec, const float* sunW, float* E);
+/* The same three over byte mode's direction-major bytes[d nc + i] (k_reduce<Bytes> / k_series<Bytes>, gpu.cu:83-92):
+ * each product's first factor times byte / 255.f. */
+void orc_reduce_bytes(const uint8_t* bytes, long nc, int nDir, const float* N, const float* D, const float* W, int useCos, double* out);
+void orc_series64_bytes(const uint8_t* bytes, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                        const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                        const float* sunVec, const float* sunW, double* E, int nthreads);
+void orc_series32_bytes(const uint8_t* bytes, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                        const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                        const float* sunVec, const float* sunW, float* E);
 /* k_view + k_iso (gpu.cu:171-274): out[i NA + a]. isoDim 0/2/3. */
 void orc_reduce_view(const uint8_t* cls, const float* dist, long nc, unsigned K, unsigned nAz, const double* wRow, unsigned nClass,
                      double farDist, const float* reach, unsigned nReach, int glazing, unsigned isoDim, const orc_view* v,

@@ -1,6 +1,7 @@
 /*
  * Oracle (test infrastructure only): the accumulation kernels of reference src/gpu.cu restated on the CPU.
- *  - k_reduce<Bits> (gpu.cu:99-114): double sums in ascending direction order of float products (dotf without fma).
+ *  - k_reduce<Bits> / k_reduce<Bytes> (gpu.cu:76-114): double sums in ascending direction order of float products
+ *    (dotf without fma; byte mode: x byte / 255.f).
  *  - k_series (gpu.cu:119-149): in FP32 as the reference's kernel accumulates (fma per direction, in direction order),
  *    and the FP64 contraction every GPU form is measured against.
  *  - k_view / k_iso (gpu.cu:171-274): per sensor the class weights, log distances, minima, own surface, reach, glazing
@@ -17,28 +18,47 @@
 #define LANES 64
 
 static float dotf(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-static int bit_of(const uint32_t* bits, long nc, long i, unsigned d) { return (bits[(size_t)(d >> 5) * nc + i] >> (d & 31)) & 1u; }
+/* the visibility of (sensor i, direction d) as the fraction that arrives (gpu.cu:76-92): Bits word-major, 1 for a set
+ * bit; Bytes direction-major (byte mode), byte / 255.f; 0 = blocked or culled */
+typedef struct { const uint32_t* bits; const uint8_t* bytes; } vis_t;
+static float vis_at(const vis_t* v, long nc, long i, unsigned d) {
+  if (v->bytes) { const uint8_t b = v->bytes[(size_t)d * nc + i]; return b ? (float)b / 255.f : 0.f; }
+  return ((v->bits[(size_t)(d >> 5) * nc + i] >> (d & 31)) & 1u) ? 1.f : 0.f;
+}
 
-void orc_reduce(const uint32_t* bits, long nc, int nDir, const float* N, const float* D, const float* W, int useCos, double* out) {
-  const int nWords = (nDir + 31) / 32;
+static void reduce(const vis_t* v, long nc, int nDir, const float* N, const float* D, const float* W, int useCos, double* out) {
   for (long i = 0; i < nc; ++i) {
     const float* n = N + 3 * i;
     double acc = 0;
-    for (int w = 0; w < nWords; ++w) {
-      uint32_t m = bits[(size_t)w * nc + i];
-      while (m) {
-        const int d = w * 32 + __builtin_ctz(m);
-        m &= m - 1;
+    if (v->bits) {   /* the set bits, ascending */
+      for (int w = 0; w < (nDir + 31) / 32; ++w)
+        for (uint32_t m = v->bits[(size_t)w * nc + i]; m; m &= m - 1) {
+          const int d = w * 32 + __builtin_ctz(m);
+          const float c = useCos ? fmaxf(dotf(n, D + 3 * d), 0.f) : 1.f;
+          acc += (double)(c * 1.f) * (double)W[d];
+        }
+    } else {
+      for (int d = 0; d < nDir; ++d) {
+        const float T = vis_at(v, nc, i, (unsigned)d);
+        if (T == 0.f) continue;
         const float c = useCos ? fmaxf(dotf(n, D + 3 * d), 0.f) : 1.f;
-        acc += (double)(c * 1.f) * (double)W[d];
+        acc += (double)(c * T) * (double)W[d];
       }
     }
     out[i] = acc;
   }
 }
+void orc_reduce(const uint32_t* bits, long nc, int nDir, const float* N, const float* D, const float* W, int useCos, double* out) {
+  const vis_t v = {bits, NULL};
+  reduce(&v, nc, nDir, N, D, W, useCos, out);
+}
+void orc_reduce_bytes(const uint8_t* bytes, long nc, int nDir, const float* N, const float* D, const float* W, int useCos, double* out) {
+  const vis_t v = {NULL, bytes};
+  reduce(&v, nc, nDir, N, D, W, useCos, out);
+}
 
 typedef struct {
-  const uint32_t* bits; long nc; const float *N, *D; unsigned nSky; const int* row; const float* wsky; const float* S; int T;
+  vis_t vis; long nc; const float *N, *D; unsigned nSky; const int* row; const float* wsky; const float* S; int T;
   const float *g, *G; unsigned sun0, nSun; const int* sunStep; const float *sunVec, *sunW; double* E;
   long next; pthread_mutex_t mu;
 } s64_job;
@@ -55,25 +75,27 @@ static void* s64_worker(void* arg) {
     const float* n = j->N + 3 * i;
     for (int t = 0; t < T; ++t) acc[t] = 0;
     for (unsigned d = 0; d < j->nSky; ++d) {
-      if (!bit_of(j->bits, j->nc, i, d)) continue;
-      const double c = (double)((fmaxf(dotf(n, j->D + 3 * d), 0.f) * 1.f) * j->wsky[d]);
+      const float Tv = vis_at(&j->vis, j->nc, i, d);
+      if (Tv == 0.f) continue;
+      const double c = (double)((fmaxf(dotf(n, j->D + 3 * d), 0.f) * Tv) * j->wsky[d]);
       const float* s = j->S + (size_t)j->row[d] * T;
       for (int t = 0; t < T; ++t) acc[t] += c * (double)s[t];
     }
     double* Ei = j->E + (size_t)i * T;
     for (int t = 0; t < T; ++t) Ei[t] = acc[t] + (double)j->G[i] * (double)j->g[t];
-    for (unsigned q = 0; q < j->nSun; ++q)
-      if (bit_of(j->bits, j->nc, i, j->sun0 + q))
-        Ei[j->sunStep[q]] += (double)fmaxf(dotf(n, j->sunVec + 3 * q), 0.f) * (double)j->sunW[q];
+    for (unsigned q = 0; q < j->nSun; ++q) {
+      const float Tv = vis_at(&j->vis, j->nc, i, j->sun0 + q);
+      if (Tv != 0.f) Ei[j->sunStep[q]] += (double)(fmaxf(dotf(n, j->sunVec + 3 * q), 0.f) * Tv) * (double)j->sunW[q];
+    }
   }
   free(acc);
   return NULL;
 }
 
-void orc_series64(const uint32_t* bits, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
-                  const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
-                  const float* sunVec, const float* sunW, double* E, int nthreads) {
-  s64_job j = {bits, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E, 0};
+static void series64(const vis_t* v, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                     const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                     const float* sunVec, const float* sunW, double* E, int nthreads) {
+  s64_job j = {*v, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E, 0, PTHREAD_MUTEX_INITIALIZER};
   pthread_mutex_init(&j.mu, NULL);
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
@@ -84,23 +106,51 @@ void orc_series64(const uint32_t* bits, long nc, const float* N, const float* D,
   pthread_mutex_destroy(&j.mu);
 }
 
-void orc_series32(const uint32_t* bits, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+void orc_series64(const uint32_t* bits, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
                   const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
-                  const float* sunVec, const float* sunW, float* E) {
+                  const float* sunVec, const float* sunW, double* E, int nthreads) {
+  const vis_t v = {bits, NULL};
+  series64(&v, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E, nthreads);
+}
+void orc_series64_bytes(const uint8_t* bytes, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                        const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                        const float* sunVec, const float* sunW, double* E, int nthreads) {
+  const vis_t v = {NULL, bytes};
+  series64(&v, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E, nthreads);
+}
+
+static void series32(const vis_t* v, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                     const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                     const float* sunVec, const float* sunW, float* E) {
   for (long i = 0; i < nc; ++i) {
     const float* n = N + 3 * i;
     float* Ei = E + (size_t)i * T;
     for (int t = 0; t < T; ++t) Ei[t] = 0.f;
     for (unsigned d = 0; d < nSky; ++d) {
-      if (!bit_of(bits, nc, i, d)) continue;
-      const float c = (fmaxf(dotf(n, D + 3 * d), 0.f) * 1.f) * wsky[d];
+      const float Tv = vis_at(v, nc, i, d);
+      if (Tv == 0.f) continue;
+      const float c = (fmaxf(dotf(n, D + 3 * d), 0.f) * Tv) * wsky[d];
       const float* s = S + (size_t)row[d] * T;
       for (int t = 0; t < T; ++t) Ei[t] = fmaf(c, s[t], Ei[t]);
     }
     for (int t = 0; t < T; ++t) Ei[t] = fmaf(G[i], g[t], Ei[t]);
-    for (unsigned q = 0; q < nSun; ++q)
-      if (bit_of(bits, nc, i, sun0 + q)) Ei[sunStep[q]] = fmaf(fmaxf(dotf(n, sunVec + 3 * q), 0.f) * 1.f, sunW[q], Ei[sunStep[q]]);
+    for (unsigned q = 0; q < nSun; ++q) {
+      const float Tv = vis_at(v, nc, i, sun0 + q);
+      if (Tv != 0.f) Ei[sunStep[q]] = fmaf(fmaxf(dotf(n, sunVec + 3 * q), 0.f) * Tv, sunW[q], Ei[sunStep[q]]);
+    }
   }
+}
+void orc_series32(const uint32_t* bits, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                  const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                  const float* sunVec, const float* sunW, float* E) {
+  const vis_t v = {bits, NULL};
+  series32(&v, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E);
+}
+void orc_series32_bytes(const uint8_t* bytes, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
+                        const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
+                        const float* sunVec, const float* sunW, float* E) {
+  const vis_t v = {NULL, bytes};
+  series32(&v, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E);
 }
 
 /* lane values summed as the wave64 tree: v[l] += v[l + s] for s = 32 .. 1, result in lane 0 */

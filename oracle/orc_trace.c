@@ -7,7 +7,9 @@
  *  - offset_ray (trace_dev.cu:16-25), tmin 1e-5, the hemisphere cull (:101), the own-plane rule (:53-88) with its
  *    constants (2^-22 x the triangle's extent, the 1/32 m gate, at most 3 own planes passed), views' frame, cells,
  *    jitter hash, virtual ground and closest-hit loop (:177-232) with labels and panes.
- *  - The triangle test is the watertight test of Woop, Benthin & Wald (JCGT 2(1), 2013), not OptiX's hardware test.
+ *  - The triangle test is the watertight test of Woop, Benthin & Wald (JCGT 2(1), 2013), not OptiX's hardware test; a
+ *    ray through an edge or a vertex goes to one of the triangles that share it by a top-left rule, as hardware
+ *    tests do (OptiX's own choice of triangle is not documented).
  *  - The first hit of a visibility ray: OptiX reports any hit; here a ray is open when every hit on [1e-5, inf) is an
  *    own-plane hit (gate, plane, facing) and they lie at 3 distinct distances at most. That is the reference's walk for
  *    every order OptiX may report hits in, except exact ties of an own-plane hit with a blocker.
@@ -126,15 +128,26 @@ static int tri_hit(const ray_t* r, const float* v0, const float* v1, const float
   const float Bx = B[r->kx] - r->Sx * B[r->kz], By = B[r->ky] - r->Sy * B[r->kz];
   const float Cx = C[r->kx] - r->Sx * C[r->kz], Cy = C[r->ky] - r->Sy * C[r->kz];
   float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  int edge = 0;
   if (U == 0.f || V == 0.f || W == 0.f) {   /* on an edge in float: decide in double (exact products) */
     U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
     V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
     W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    edge = U == 0.f || V == 0.f || W == 0.f;
   }
   if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return 0;
   const float det = U + V + W;
   if (det == 0.f) return 0;
   if (cull && det < 0.f) return 0;
+  /* on an edge or a vertex: the top-left rule of each zero edge function (the edge's direction in the winding, times
+   * the facing's sign: y > 0, or y == 0 and x > 0) gives it to one of the triangles that share it */
+  if (edge) {
+    const float sg = det > 0.f ? 1.f : -1.f;
+    const float ex[3] = {sg * (Cx - Bx), sg * (Ax - Cx), sg * (Bx - Ax)}, ey[3] = {sg * (Cy - By), sg * (Ay - Cy), sg * (By - Ay)};
+    const float e[3] = {U, V, W};
+    for (int k = 0; k < 3; ++k)
+      if (e[k] == 0.f && !(ey[k] > 0.f || (ey[k] == 0.f && ex[k] > 0.f))) return 0;
+  }
   const float Az = r->Sz * A[r->kz], Bz = r->Sz * B[r->kz], Cz = r->Sz * C[r->kz];
   const float T = U * Az + V * Bz + W * Cz;
   const float t = T / det;
@@ -638,6 +651,153 @@ void orc_trace_bits(const orc_scene* s, const float* P, const float* N, long nc,
   memset(&j, 0, sizeof j);
   j.s = s; j.P = P; j.N = N; j.nc = nc; j.D = D; j.nDir = nDir; j.cull = cull; j.bits = bits;
   run_workers(&j, bits_worker, nthreads);
+}
+
+/* ---------------------------------------------------------------- byte mode (transmission) */
+/* tn_from_Tn (glass.h:11-13, Radiance m_glass.c): glass transmissivity from the transmittance at normal incidence */
+double orc_tn_from_Tn(double Tn) {
+  return fmax(0., (sqrt(.8402528435 + .0072522239 * Tn * Tn) - .9166530661) / .0036261119 / Tn);
+}
+/* glass_TR's T (glass.h:17-25): the pane at incidence cosine c, n = 1.52, both polarisations' two-face series, in double,
+ * rounded to float */
+float orc_glass_T(double c, double tn) {
+  const double n = 1.52;
+  const double ci = sqrt((1. - 1. / (n * n)) + c * c / (n * n));   /* cosine inside the glass (Snell) */
+  const double a = pow(tn, 1. / ci), a2 = a * a;                   /* the slanted path's transmissivity */
+  const double s0 = (c - n * ci) / (c + n * ci), p0 = (ci - n * c) / (ci + n * c);
+  const double rs = s0 * s0, rp = p0 * p0;
+  return (float)(.5 * (1. - rs) * (1. - rs) * a / (1. - rs * rs * a2) + .5 * (1. - rp) * (1. - rp) * a / (1. - rp * rp * a2));
+}
+/* __anyhit__tr's factor (trace_dev.cu:149-165): a thin solid's exit face 1; glass by the angle of the hit triangle's
+ * geometric normal (float edges, double products) to d; else tau. slot: rho, tau, tn, flags (as bits). */
+static float pass_factor(const float* slot, int front, const float* v0, const float* v1, const float* v2, const float d[3]) {
+  uint32_t flags;
+  memcpy(&flags, slot + 3, 4);
+  if ((flags & 2u) && front == ((flags & 4u) != 0)) return 1.f;
+  if (!(flags & 1u)) return slot[1];
+  const float e1f[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]}, e2f[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+  const double ax = e1f[0], ay = e1f[1], az = e1f[2], bx = e2f[0], by = e2f[1], bz = e2f[2];
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double l = sqrt(nx * nx + ny * ny + nz * nz);
+  if (!(l > 0.)) return 0.f;
+  return orc_glass_T(fabs(nx * d[0] + ny * d[1] + nz * d[2]) / l, slot[2]);
+}
+
+typedef struct { float t; uint32_t prim; long leaf; int front; } hit_t;
+typedef struct { const orc_scene* s; const ray_t* r; int cull; hit_t* h; long n, cap; } all_ctx;
+static int all_leaf(void* c, long leaf) {
+  all_ctx* a = c;
+  const float* T = a->s->tri + 12 * leaf;
+  float t;
+  int front;
+  if (!tri_hit(a->r, T, T + 3, T + 6, 1e-5f, FLT_MAX, a->cull, &t, &front)) return 0;
+  if (a->n == a->cap) { a->cap = a->cap ? 2 * a->cap : 16; a->h = realloc(a->h, (size_t)a->cap * sizeof(hit_t)); }
+  a->h[a->n].t = t; a->h[a->n].prim = a->s->prim[leaf]; a->h[a->n].leaf = leaf; a->h[a->n].front = front;
+  ++a->n;
+  return 0;
+}
+static int hit_cmp(const void* x, const void* y) {
+  const hit_t *a = x, *b = y;
+  if (a->t != b->t) return a->t < b->t ? -1 : 1;
+  return a->prim < b->prim ? -1 : a->prim > b->prim;
+}
+
+typedef struct {
+  const orc_scene* s;
+  const uint32_t* triSlot;   /* global slot per triangle (original index) */
+  const float* slots;        /* [nSlots][4] */
+  long nSlots;
+} optics_t;
+static const float* slot_of(const optics_t* op, uint32_t prim) {
+  const uint32_t si = op->triSlot ? op->triSlot[prim] : 0;
+  return op->slots + 4 * (si < (uint32_t)op->nSlots ? si : (uint32_t)op->nSlots - 1);
+}
+/* the any-hit traversal on [tmin, inf) with payload 1 (trace_dev.cu:133-146 / :140-141), its any-hit calls in the order of
+ * the sorted hits: an opaque hit, or a translucent one that takes the product below 1/510, is reported (index, or -1
+ * with the product in *T); products in double */
+static long anyhit_pass(const optics_t* op, const hit_t* h, long n, float tmin, const float d[3], double* T) {
+  double p = 1.0;
+  for (long k = 0; k < n; ++k) {
+    if (!(h[k].t >= tmin)) continue;
+    const float* sl = slot_of(op, h[k].prim);
+    if (!(sl[1] > 0.f)) return k;
+    const float* V = op->s->tri + 12 * h[k].leaf;
+    p *= pass_factor(sl, h[k].front, V, V + 3, V + 6, d);
+    if (!(p >= (double)(1.f / 510.f))) return k;
+  }
+  *T = p;
+  return -1;
+}
+/* past_own_plane (trace_dev.cu:76-88) with translucent: closest hits from 1e-5 past the sensor's own planes, 4 segments
+ * at most; the tmin the rest of the ray goes on from, 0 = blocked */
+static float past_own(const optics_t* op, const hit_t* h, long n, const float d[3], const float nrm[3]) {
+  float tmin = 1e-5f;
+  for (int seg = 0; seg < 4; ++seg) {
+    long k = 0;
+    while (k < n && !(h[k].t >= tmin)) ++k;
+    if (k == n) return tmin;
+    const float* V = op->s->tri + 12 * h[k].leaf;
+    if (!own_hit(d, nrm, h[k].t, h[k].front, V, V + 3, V + 6)) return slot_of(op, h[k].prim)[1] > 0.f ? tmin : 0.f;
+    tmin = f_as(u_as(h[k].t) + 1);
+  }
+  return 0.f;
+}
+/* __raygen__bytes for one ray (trace_dev.cu:119-145), already past the hemisphere cull: the transmittance byte */
+static uint8_t byte_of(const optics_t* op, const float o[3], const float nrm[3], const float d[3], int cull, all_ctx* a) {
+  if (op->s->nt == 0) return 255;
+  ray_t r;
+  ray_setup(&r, o, d, op->s->pad);
+  a->s = op->s; a->r = &r; a->cull = cull; a->n = 0;
+  const float tmax = FLT_MAX;
+  traverse(op->s, &r, 1e-5f, &tmax, all_leaf, a);
+  qsort(a->h, (size_t)a->n, sizeof(hit_t), hit_cmp);
+  double T = 0.0;
+  const long rep = anyhit_pass(op, a->h, a->n, 1e-5f, d, &T);
+  if (rep >= 0) {
+    const float* V = op->s->tri + 12 * a->h[rep].leaf;
+    if (!own_hit(d, nrm, a->h[rep].t, a->h[rep].front, V, V + 3, V + 6)) return 0;
+    const float t0 = past_own(op, a->h, a->n, d, nrm);
+    if (!(t0 > 0.f) || anyhit_pass(op, a->h, a->n, t0, d, &T) >= 0) return 0;
+  }
+  return (uint8_t)rintf((float)T * 255.f);
+}
+
+typedef struct {
+  job_t j;
+  optics_t op;
+  uint8_t* bytes;
+} bytes_job;
+static void* bytes_worker(void* arg) {
+  bytes_job* b = arg;
+  job_t* j = &b->j;
+  all_ctx a = {0};
+  for (;;) {
+    const long i0 = take(j, 16);
+    if (i0 >= j->nc) break;
+    const long i1 = i0 + 16 < j->nc ? i0 + 16 : j->nc;
+    for (long i = i0; i < i1; ++i) {
+      const float* n = j->N + 3 * i;
+      float o[3];
+      orc_offset_ray(j->P + 3 * i, n, o);
+      for (int d = 0; d < j->nDir; ++d) {
+        const float* dir = j->D + 3 * d;
+        const uint8_t v = n[0] * dir[0] + n[1] * dir[1] + n[2] * dir[2] > 0.f ? byte_of(&b->op, o, n, dir, j->cull, &a) : 0;
+        b->bytes[(size_t)d * j->nc + i] = v;
+      }
+    }
+  }
+  free(a.h);
+  return NULL;
+}
+
+void orc_trace_bytes(const orc_scene* s, const float* P, const float* N, long nc, const float* D, int nDir, int cull, const uint32_t* triSlot,
+                     const float* slots, long nSlots, uint8_t* bytes, int nthreads) {
+  bytes_job b;
+  memset(&b, 0, sizeof b);
+  b.j.s = s; b.j.P = P; b.j.N = N; b.j.nc = nc; b.j.D = D; b.j.nDir = nDir; b.j.cull = cull;
+  b.op.s = s; b.op.triSlot = triSlot; b.op.slots = slots; b.op.nSlots = nSlots;
+  b.bytes = bytes;
+  run_workers(&b.j, bytes_worker, nthreads);
 }
 
 /* ---------------------------------------------------------------- views (closest hits) */

@@ -8,24 +8,30 @@ class Tris:
     """A triangle soup: v float32 [3 nv], i uint32 [3 nt]."""
 
     def __init__(self):
-        self._v, self._i = [], []
+        self._v, self._i, self._s = [], [], []
 
-    def quad(self, a, b, c, d):   # CCW seen from outside: (0, 1, 2), (0, 2, 3)
+    def quad(self, a, b, c, d, slot=0):   # CCW seen from outside: (0, 1, 2), (0, 2, 3); slot: both triangles' global slot
         k = len(self._v) // 3
         for p in (a, b, c, d):
             self._v += [float(x) for x in p]
         self._i += [k, k + 1, k + 2, k, k + 2, k + 3]
+        self._s += [slot, slot]
 
-    def box(self, x0, y0, z0, x1, y1, z1):
+    def box(self, x0, y0, z0, x1, y1, z1, slot=0):   # faces wound outward
         lo, hi = (x0, y0, z0), (x1, y1, z1)
         c = [(hi[0] if k & 1 else lo[0], hi[1] if k & 2 else lo[1], hi[2] if k & 4 else lo[2]) for k in range(8)]
         for q in ((0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (1, 3, 7, 5), (3, 2, 6, 7), (2, 0, 4, 6)):
-            self.quad(*[c[j] for j in q])
+            self.quad(*[c[j] for j in q], slot=slot)
 
     def copy(self):
         t = Tris()
-        t._v, t._i = list(self._v), list(self._i)
+        t._v, t._i, t._s = list(self._v), list(self._i), list(self._s)
         return t
+
+    @property
+    def s(self):
+        """the global slot of every triangle"""
+        return np.array(self._s, np.uint32)
 
     @property
     def v(self):
