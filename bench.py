@@ -391,6 +391,7 @@ def main():
     ap.add_argument("--no-views", action="store_true", help="skip the views leg")
     ap.add_argument("--only-views", action="store_true", help="run the views leg alone (prints its object as the line)")
     ap.add_argument("--no-reference-benches", action="store_true", help="skip the reference's --bench runs through the CLI")
+    ap.add_argument("--no-transmission", action="store_true", help="skip the byte-mode leg (the context as glass)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
@@ -504,6 +505,29 @@ def main():
     if methods:
         print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
 
+    # byte mode: the same workload with every context face glass (tau 0.64, Radiance glass) under --transmission on, so
+    # every ray walks every crossing; the first run builds the slotted scene (its own key), two timed runs follow
+    trans = None
+    if not a.no_transmission:
+        targs = args + ["--transmission", "on", "--optics", "city=0,0.64,glass"]
+        native.run_series_device(targs, sensors=S, context=Cx)
+        t0t = native.gpu_counters()
+        tt = time.perf_counter()
+        for _ in range(2):
+            rt = native.run_series_device(targs, sensors=S, context=Cx)
+        torch.cuda.synchronize()
+        tt = (time.perf_counter() - tt) / 2
+        t1t = native.gpu_counters()
+        assert rt.stats["scene_reused"]
+        trans = {"workload": "the urban block with every context face glass (tau 0.64), --transmission on",
+                 "ms_per_step": round(1e3 * tt, 2), "sensor_hours_per_s": round(sensor_hours / tt, 1),
+                 "trace_ms_per_step": round((t1t["trace_ms"] - t0t["trace_ms"]) / 2, 3),
+                 "series_ms_per_step": round((t1t["series_ms"] - t0t["series_ms"]) / 2, 3),
+                 "reduce_ms_per_step": round((t1t["reduce_ms"] - t0t["reduce_ms"]) / 2, 3),
+                 "grays_per_s": round((t1t["rays"] - t0t["rays"]) / ((t1t["trace_ms"] - t0t["trace_ms"]) * 1e-3) / 1e9, 3),
+                 "mean_radiation_vs_opaque": round(float(rt.values.mean() / r.values.mean()), 4)}
+        print("bench: transmission %s" % json.dumps(trans), file=sys.stderr, flush=True)
+
     views = None if a.no_views else views_leg(native, torch)
     if views:
         print("bench: views %s" % json.dumps({k: v for k, v in views.items() if k != "columns"}), file=sys.stderr, flush=True)
@@ -526,6 +550,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "contraction_methods": methods,
+        "transmission": trans,
         "views": views,
         "reference_benches": refs,
         "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm",

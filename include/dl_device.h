@@ -57,14 +57,14 @@ int64_t dlg_free_bytes(void);
  * original index as bits, 2 unused), tree height. Sizes in *nNodes / *nTris when the pointers are null. */
 int32_t dlg_scene_download(dlg_gpu* g, float* nodes, int64_t* nNodes, float* leaves, int64_t* nTris, int32_t* height);
 
-/* Slot optics (the reference's Slot, params.h:7-19): rho (the bounce pass, unused here), tau the transmittance at normal
+/* Slot optics (the reference's Slot, params.h:7-19): rho the diffuse reflectance (the bounce pass), tau the transmittance at normal
  * incidence (> 0: the slot transmits), tn the glass transmissivity (tn_from_Tn(tau)), flags DLG_SLOT_*. */
 typedef struct { float rho, tau, tn; uint32_t flags; } dlg_slot;
 #define DLG_SLOT_GLASS 1u   /* transmittance by angle (Radiance glass); else tau at every angle */
 #define DLG_SLOT_THIN 2u    /* a closed thin solid: its exit face passes everything */
 #define DLG_SLOT_INWARD 4u  /* ... wound inward: entry is the back face */
 #define DLG_SLOT_TWIN 8u    /* ... of coincident face pairs (the bounce pass only) */
-/* gpu_set_slots (internal.h:139): the optics of every global slot, read by a byte-mode trace (every run). */
+/* gpu_set_slots (internal.h:139): the optics of every global slot, read by a byte-mode trace and the bounce pass. */
 int32_t dlg_set_slots(dlg_gpu* g, const dlg_slot* slots, int64_t n, char* err, int32_t errLen);
 /* The global slot of every triangle of the installed scene (input order, nt = the build's triangle count), kept with the
  * scene's leaves: the reference passes them with the build (Tris::slot, internal.h:58, gpu_build's per-triangle SBT
@@ -90,6 +90,25 @@ int32_t dlg_download_bits(dlg_gpu* g, int64_t nc, uint32_t* out, char* err, int3
 /* gpu_download_bytes (internal.h:183): byte mode's transmittance, direction-major [nDir x nc]. */
 int32_t dlg_download_bytes(dlg_gpu* g, int64_t nc, uint8_t* out, char* err, int32_t errLen);
 
+/* gpu_bounce (internal.h:175-178) + __raygen__bounce (trace_dev.cu:260-311): reflected light of sensors [i0, i0 + nc),
+ * M paths per sensor with up to `bounces` diffuse reflections by the slots' rho (glass mirrors, porous and glass
+ * transmit, thin solids' exit faces pass), binned where they escape into rows dc[b nc + i] kept on the device: bin 0 the
+ * ground (below the horizon), 1 + p the Reinhart patches of sky density mf in gendaymtx order, true north = the model's
+ * turned by -north; pi / M per path, so sum_b dc(b, i) L_b is the reflected irradiance under bin radiances L. The rows
+ * do not depend on the chunking (seeds from the global sensor index, sums in sample order). rayFlags: dlg_bounce_flags. */
+typedef struct {
+  uint32_t bounces, M;          /* diffuse reflections a path may have; paths per sensor */
+  int32_t mf;                   /* sky density 1, 2, 4 */
+  double north;                 /* degrees */
+} dlg_bounce_in;
+/* gpu_bounce_flags (internal.h:220): DISABLE_ANYHIT (1) [| CULL_BACK_FACING_TRIANGLES (16)] */
+uint32_t dlg_bounce_flags(int32_t cullBackfaces);
+int32_t dlg_bounce(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const dlg_bounce_in* b, char* err, int32_t errLen);
+/* gpu_reduce_dc (internal.h:179): out[i] = sum over bins b (ascending) of double(dc(b, i)) * double(W[b]), W [P + 1] (P the sky patches). */
+int32_t dlg_reduce_dc(dlg_gpu* g, int64_t nc, const float* W, double* out, char* err, int32_t errLen);
+/* gpu_download_dc (internal.h:180): the rows [(P + 1) x nc] as on the device. */
+int32_t dlg_download_dc(dlg_gpu* g, int64_t nc, float* out, char* err, int32_t errLen);
+
 /* The sky of a run on the GPU, FP64 (replaces the host loops of reference run.cpp:311-446 and sky.cpp's
  * gendaymtx_step / solar.cpp's sun_noaa per step): the sun of every step, the Perez all-weather sky per step and patch,
  * the cumulative per-patch weights and, for a series, the patch radiance matrix S [P x Tup] kept on the device. */
@@ -103,6 +122,8 @@ typedef struct {
   int32_t wantRad, wantVis;     /* radiometric / visible cumulative weights */
   int32_t series, seriesVis;    /* keep S (radiometric, or visible when seriesVis) and g for the non-night steps */
   const double* tcos;           /* [1025] the table cosine (Radiance tcos.c): cos(k pi / 2048) */
+  int32_t reflect;              /* reflected light: the sun binned as gendaymtx -d (AddDirect) per patch, summed into
+                                   sunBinRad / sunBinVis, and with series the bin radiance matrix Sdc kept on the device */
 } dlg_sky_in;
 typedef struct {
   double* alt; double* azi; int32_t* up; float* vec;             /* [T], vec [3 T] turned by north */
@@ -110,10 +131,12 @@ typedef struct {
   double* gndRad; double* gndVis;                                /* [1] */
   double* sunRad; double* sunVis;                                /* [T]: the step's sun weight (used where up) */
   int32_t* Tup;                                                  /* non-night steps (series columns) */
+  double* sunBinRad; double* sunBinVis;                          /* [P] reflect: the binned sun's cumulative weights, or null */
 } dlg_sky_out;
 int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, char* err, int32_t errLen);
-/* The last dlg_sky's series inputs: S [P x Tup] and g [Tup] (floats), as the contraction reads them. */
-int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, char* err, int32_t errLen);
+/* The last dlg_sky's series inputs: S [P x Tup] and g [Tup] (floats), as the contraction reads them, and with reflect the
+ * bins' radiance Sdc [(P + 1) x Tup] (row 0 g, row 1 + p S plus the binned sun; null: not wanted). */
+int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, float* Sdc, char* err, int32_t errLen);
 
 /* gpu_series (internal.h:155-171): E[i T + t] for the chunk's sensors, S from the last dlg_sky (P x T, T = Tup).
  * method 0: FP32 on the vector units, each sum in direction order by fma (the reference kernel's arithmetic);
@@ -129,6 +152,7 @@ typedef struct {
   uint32_t sun0, nSun;          /* sun directions */
   const int32_t* sunStep;       /* [nSun] column of each sun direction */
   const float* sunW;            /* [nSun] */
+  int32_t reflect;              /* add the chunk's dlg_bounce rows times the last sky's Sdc: sum_b dc(b, i) Sdc[b, t] */
 } dlg_series_in;
 int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, float* E, char* err, int32_t errLen);
 /* the device copy of the last series (float [nc x T]) */
@@ -165,6 +189,9 @@ typedef struct {
   /* the counting variant's totals (dlg_count_fetches): node visits and triangle tests of the visibility traversal, the
    * ray slots they were counted over, and the traversal iterations the waves executed */
   int64_t nodeFetches, triFetches, countedRays, waveIters;
+  /* reflected light: ms, paths, launches */
+  double bounceMs;
+  int64_t paths, bounceLaunches;
 } dlg_stats_t;
 int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out);
 /* No counterpart in the reference (its OptiX traversal is opaque): on != 0 makes dlg_trace run the traversal's counting

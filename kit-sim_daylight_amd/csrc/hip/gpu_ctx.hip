@@ -51,7 +51,7 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
   for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->slots})
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->slots, &g->dc, &g->skySdc})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -70,7 +70,7 @@ int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmB
 }
 
 void dlg_trim(dlg_gpu* g, int32_t sensors) {
-  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->view}) b->release();
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->view, &g->dc}) b->release();
   if (sensors) {
     g->P.release(); g->N.release(); g->labels.release();
     g->nSens = 0;

@@ -8,7 +8,9 @@
 //   method 1, BF16x3: A and S split into bf16 hi + lo; hi.hi + hi.lo + lo.hi on MFMA (v_mfma_f32_32x32x16 bf16),
 //             FP32 accumulation; epilogue in FP64.
 //   method 2, BF16x6: hi + mid + lo; the six products down to 2^-16 relative, as method 1 otherwise: FP32-level.
-// Operands: fragment-major bf16 planes (Kp = nSky rounded up to 32, zero padded), described at k_split_a.
+// Operands: fragment-major bf16 planes (Kp = nSky rounded up to 32, zero padded), described at k_split_a. Reflected light
+// (dlg_bounce rows dc and the sky's bin radiance Sdc) extends K: E += dc Sdc in the same products (methods 1 / 2) or a
+// second FP32 pass after the reference's (method 0).
 #include <vector>
 
 #include "dl_math.h"
@@ -90,6 +92,43 @@ __global__ __launch_bounds__(256) void k_series_f32(const uint32_t* __restrict__
   }
 }
 
+// reflected light in FP32 (k_series_dc, gpu.cu:153-169): E[i, t] += (an fma chain over the bins in order of dc(b, i)
+// Sdc[b, t]), the reference kernel's arithmetic, tiled 64 x 64 through LDS as k_series_f32
+__global__ __launch_bounds__(256) void k_series_dc_f32(const float* __restrict__ dc, uint32_t nc, uint32_t nBins, const float* __restrict__ Sdc,
+                                                       int T, float* __restrict__ E) {
+  __shared__ float As[kKt][kTile];
+  __shared__ float Bs[kKt][kTile + 4];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const uint32_t i0 = blockIdx.y * kTile, t0 = blockIdx.x * kTile;
+  float e[4][4] = {};
+  for (uint32_t b0 = 0; b0 < nBins; b0 += kKt) {
+    for (int k = threadIdx.x; k < kKt * kTile; k += 256) {
+      const int bb = k / kTile, ii = k % kTile;
+      const uint32_t b = b0 + bb, i = i0 + ii;
+      As[bb][ii] = (b < nBins && i < nc) ? dc[size_t(b) * nc + i] : 0.f;
+      const int t = int(t0) + ii;
+      Bs[bb][ii] = (b < nBins && t < T) ? Sdc[size_t(b) * T + t] : 0.f;
+    }
+    __syncthreads();
+    const int kk = int(nBins - b0) < kKt ? int(nBins - b0) : kKt;
+    for (int k = 0; k < kk; ++k) {
+      float a[4], bv[4];
+      for (int r = 0; r < 4; ++r) { a[r] = As[k][ty * 4 + r]; bv[r] = Bs[k][tx * 4 + r]; }
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) e[r][c] = fmaf(a[r], bv[c], e[r][c]);
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t i = i0 + ty * 4 + r;
+    if (i >= nc) continue;
+    for (int c = 0; c < 4; ++c) {
+      const int t = int(t0) + tx * 4 + c;
+      if (t < T) E[size_t(i) * T + t] += e[r][c];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------- methods 1 / 2: MFMA
 // split x into NS bf16 pieces, each the rounding of what the previous ones leave
 template <int NS>
@@ -107,12 +146,14 @@ __device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k,
   return ((size_t(g) * nKS + k / 16) * 64 + r + 32 * ((k / 8) & 1)) * 8 + (k & 7);
 }
 
+// A's columns: the sky directions' coefficients, then (reflected light) the chunk's bounce rows dc[b nc + i]
 template <int NS>
 __global__ void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N, const float3* __restrict__ D,
-                          uint32_t nSky, uint32_t Kp, uint32_t nRG, const float* __restrict__ wsky, uint16_t* __restrict__ A) {
+                          uint32_t nSky, uint32_t Kp, uint32_t nRG, const float* __restrict__ wsky, const float* __restrict__ dc, uint32_t nBins,
+                          uint16_t* __restrict__ A) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (d >= Kp) return;
-  const float c = (d < nSky && i < nc) ? coeff(bits, bytes, nc, i, d, N[i], D, wsky) : 0.f;
+  const float c = i >= nc ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, N[i], D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
   uint16_t p[NS];
   split<NS>(c, p);
   const size_t plane = size_t(nRG) * 32 * Kp, at = frag_index(i / 32, i % 32, d, Kp / 16);
@@ -120,12 +161,13 @@ __global__ void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uin
   for (int s = 0; s < NS; ++s) A[s * plane + at] = p[s];
 }
 
+// B's rows: S of each sky direction's patch, then (reflected light) the bins' radiance Sdc [b T + t]
 template <int NS>
 __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ row, uint32_t nSky, uint32_t Kp, int T, uint32_t Tp,
-                          uint16_t* __restrict__ B) {
+                          const float* __restrict__ Sdc, uint32_t nBins, uint16_t* __restrict__ B) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
   if (d >= Kp || t >= Tp) return;
-  const float s = (d < nSky && int(t) < T) ? S[size_t(row[d]) * T + t] : 0.f;
+  const float s = int(t) >= T ? 0.f : d < nSky ? S[size_t(row[d]) * T + t] : d - nSky < nBins ? Sdc[size_t(d - nSky) * T + t] : 0.f;
   uint16_t p[NS];
   split<NS>(s, p);
   const size_t plane = size_t(Tp) * Kp, at = frag_index(t / 32, t % 32, d, Kp / 16);
@@ -274,7 +316,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 template <int NS>
 int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* dG, const float* dg, const int* dRow, const float* dWsky,
                  const int* dSunOfCol, const float* dSunW, char* err, int32_t errLen) {
-  const uint32_t Kp = (in->nSky + kSliceK - 1) / kSliceK * kSliceK, T = uint32_t(in->T);
+  const uint32_t nBins = in->reflect ? g->dcBins : 0u, Kp = (in->nSky + nBins + kSliceK - 1) / kSliceK * kSliceK, T = uint32_t(in->T);
   const uint32_t tilesI = (nc + kTileM - 1) / kTileM, tilesT = (T + kTileN - 1) / kTileN;
   const uint32_t nRG = tilesI * (kTileM / 32), nCG = tilesT * (kTileN / 32), Tp = nCG * 32, nKS = Kp / 16;
   const size_t aBytes = size_t(NS) * nRG * 32 * Kp * 2, bBytes = size_t(NS) * Tp * Kp * 2;
@@ -283,8 +325,9 @@ int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* 
   uint16_t* B = A + aBytes / 2;
   const float3* N = g->N.as<float3>() + g->i0;
   k_split_a<NS><<<dim3((Kp + 255) / 256, nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), nc, N, g->D.as<float3>(),
-                                                                          in->nSky, Kp, nRG, dWsky, A);
-  k_split_s<NS><<<dim3((Kp + 255) / 256, Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, in->nSky, Kp, in->T, Tp, B);
+                                                                          in->nSky, Kp, nRG, dWsky, g->dc.as<float>(), nBins, A);
+  k_split_s<NS><<<dim3((Kp + 255) / 256, Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, in->nSky, Kp, in->T, Tp, g->skySdc.as<float>(),
+                                                                   nBins, B);
   const uint32_t tiles = tilesT * tilesI;
   k_series_mfma<NS><<<(tiles + 7) / 8 * 8, 256, 0, g->stream>>>(reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), nc, nRG,
                                                                  nCG, nKS, in->T, g->bits.as<uint32_t>(), uint32_t(g->bytes), N, g->D.as<float3>(), dG, dg,
@@ -299,6 +342,10 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
   if (nc == 0 || in->T <= 0) return 0;
   if (nc != g->nc) { dlg_fail(err, errLen, "dlg_series: no trace of this chunk"); return 1; }
   if (in->P != g->skyP || in->T != g->skyT) { dlg_fail(err, errLen, "dlg_series: S of another sky (run dlg_sky with series on)"); return 1; }
+  if (in->reflect && (!g->skyReflect || g->dcN != nc || g->dcI0 != g->i0 || g->dcBins != uint32_t(g->skyP + 1))) {
+    dlg_fail(err, errLen, "dlg_series: reflected light needs the chunk's dlg_bounce rows and a sky with reflect on");
+    return 1;
+  }
   const int T = in->T;
   // inputs: row | wsky | G | sunStep-by-column | sunW (each 256-byte aligned), in the scratch buffer
   size_t off = 0;
@@ -327,6 +374,9 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
         g->bits.as<uint32_t>(), uint32_t(g->bytes), uint32_t(nc), N, g->D.as<float3>(), in->nSky, reinterpret_cast<const int*>(b + oRow),
         reinterpret_cast<const float*>(b + oW), g->skyS.as<float>(), T, reinterpret_cast<const float*>(b + oG), dg, in->sun0,
         reinterpret_cast<const int*>(b + oCol), reinterpret_cast<const float*>(b + oSw), g->seriesE.as<float>());
+    if (in->reflect)
+      k_series_dc_f32<<<dim3((T + kTile - 1) / kTile, unsigned((nc + kTile - 1) / kTile)), 256, 0, g->stream>>>(
+          g->dc.as<float>(), uint32_t(nc), g->dcBins, g->skySdc.as<float>(), T, g->seriesE.as<float>());
     DLG_CHECK(hipGetLastError());
   } else if (method == 1 || method == 2) {
     const int32_t rc = (method == 1 ? run_mfma<2> : run_mfma<3>)(g, uint32_t(nc), in, reinterpret_cast<const float*>(b + oG), dg,

@@ -28,12 +28,16 @@ struct StepOut {
   int uniform;                 // zero sky: every patch 1
   double gRadB, gVisB;         // ground bright (radiometric W/m2/sr, visible x179)
   double sunVis, sunRad;       // the sun's weights of this step
+  // the sun as AddDirect spreads it (gendaymtx -d) over its 4 nearest patches: the patches and each one's value, visible
+  // and radiometric (0: not spread, e.g. at night)
+  int binP[4];
+  float binVis[4], binRad[4];
 };
 
 __global__ void k_steps(int T, const int* __restrict__ month, const int* __restrict__ day, const int* __restrict__ jd,
                         const double* __restrict__ hour, const double* __restrict__ dt, const float* __restrict__ dni,
                         const float* __restrict__ dhi, double lat, double lon, double tz, double north, int leap, float grefl, int wantVis,
-                        int P, const float* __restrict__ palt, const float* __restrict__ pazi, const float* __restrict__ omega,
+                        int reflect, int P, const float* __restrict__ palt, const float* __restrict__ pazi, const float* __restrict__ omega,
                         StepOut* __restrict__ so, double* __restrict__ alt, double* __restrict__ azi, int* __restrict__ up,
                         float* __restrict__ vec) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -51,7 +55,8 @@ __global__ void k_steps(int T, const int* __restrict__ month, const int* __restr
   o.gRadB = o.q.night ? 0.0 : ground_bright(o.q.gRad, grefl);
   o.gVisB = o.q.night ? 0.0 : 179.0 * ground_bright(o.q.gVis, grefl);
   o.sunRad = dni[t] * dt[t] / 1000.0;
-  if (wantVis && !o.q.night && o.q.dirVis > 1e-4) {   // AddDirect (-d, -O0): the 4 patch centres nearest the sun
+  const bool vis = (wantVis || reflect) && o.q.dirVis > 1e-4, rad = reflect && o.q.dirRad > 1e-4;
+  if (!o.q.night && (vis || rad)) {   // AddDirect (-d): the 4 patch centres nearest the sun (sky.cpp:435-473)
     double best[4] = {-1., -1., -1., -1.};
     int bestP[4] = {0, 0, 0, 0};
     const double ca = cos(o.altRad), sv0 = ca * sin(o.aziRad), sv1 = ca * cos(o.aziRad), sv2 = sin(o.altRad);
@@ -68,20 +73,27 @@ __global__ void k_steps(int T, const int* __restrict__ month, const int* __restr
     }
     double w[4], wsum = 0;
     for (int i = 3; i >= 0; --i) wsum += w[i] = 1. / (1.002 - best[i]);
-    // the patches' green channel after AddDirect, summed x omega in patch order (zeros elsewhere)
-    int ord[4] = {0, 1, 2, 3};
-    for (int a = 0; a < 4; ++a)
-      for (int b = a + 1; b < 4; ++b)
-        if (bestP[ord[b]] < bestP[ord[a]]) { const int x = ord[a]; ord[a] = ord[b]; ord[b] = x; }
-    double dirIll = 0;
-    for (int k = 0; k < 4; ++k) {
-      const int i = ord[k];
-      float add = (float)(w[i] * o.q.dirVis / (179. * wsum));
-      add /= omega[bestP[i]];
-      const float val = 0.f + add * 1.f;
-      dirIll += (double)(val * omega[bestP[i]]);   // float x float, as run.cpp:462 multiplies the float row by omega
+    // each patch's value after AddDirect (the row zeroed first, -d): the added float, per mode
+    for (int i = 0; i < 4; ++i) {
+      o.binP[i] = bestP[i];
+      float addV = (float)(w[i] * o.q.dirVis / (179. * wsum)), addR = (float)(w[i] * o.q.dirRad / (179. * wsum));
+      addV /= omega[bestP[i]];
+      addR /= omega[bestP[i]];
+      o.binVis[i] = vis ? 0.f + addV * 1.f : 0.f;
+      o.binRad[i] = rad ? 0.f + addR * 1.f : 0.f;
     }
-    o.sunVis = 179.0 * dirIll * dt[t];
+    if (vis && wantVis) {   // the direct illuminance: the patches' green channel x omega summed in patch order
+      int ord[4] = {0, 1, 2, 3};
+      for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b)
+          if (bestP[ord[b]] < bestP[ord[a]]) { const int x = ord[a]; ord[a] = ord[b]; ord[b] = x; }
+      double dirIll = 0;
+      for (int k = 0; k < 4; ++k) {
+        const int i = ord[k];
+        dirIll += (double)(o.binVis[i] * omega[bestP[i]]);   // float x float, as run.cpp:462 multiplies the float row by omega
+      }
+      o.sunVis = 179.0 * dirIll * dt[t];
+    }
   }
   so[t] = o;
 }
@@ -194,6 +206,27 @@ __global__ __launch_bounds__(256) void k_ground_sum(int T, const double* __restr
   if (threadIdx.x == 0) { gnd[0] = r; gnd[1] = v; }
 }
 
+// reflected light's bin radiance per step (run.cpp:412-442): row 0 the ground g, row 1 + p the patch's sky S plus the sun
+// binned by AddDirect (float S + float(bright of the sun's value)); a thread per (bin, column)
+__global__ void k_sdc_sky(int P, int Tup, const float* __restrict__ S, const float* __restrict__ gS, float* __restrict__ Sdc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (c >= Tup) return;
+  Sdc[size_t(b) * Tup + c] = b == 0 ? gS[c] : S[size_t(b - 1) * Tup + c];
+}
+__global__ void k_sdc_sun(int T, int Tup, const StepOut* __restrict__ so, const int* __restrict__ colOf, int seriesVis, const float* __restrict__ S,
+                          float* __restrict__ Sdc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T || colOf[t] < 0) return;
+  const StepOut& o = so[t];
+  for (int i = 0; i < 4; ++i) {
+    const float v = seriesVis ? o.binVis[i] : o.binRad[i];
+    if (v == 0.f) continue;   // not spread: S + 0 = S
+    const double Ls = (seriesVis ? 179.0 : 1.0) * sun_bright(v);
+    const int p = o.binP[i];
+    Sdc[size_t(p + 1) * Tup + colOf[t]] = S[size_t(p) * Tup + colOf[t]] + (float)Ls;
+  }
+}
+
 inline unsigned blocks(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
@@ -247,12 +280,14 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
     DLG_CHECK(g->skyG.grow(size_t(Tup > 0 ? Tup : 1) * 4));
     g->skyP = P;
     g->skyT = Tup;
+    g->skyReflect = in->reflect ? 1 : 0;
+    if (in->reflect) DLG_CHECK(g->skySdc.grow(size_t(P + 1) * size_t(Tup > 0 ? Tup : 1) * 4));
   }
 #define PTR(T_, o) reinterpret_cast<T_*>(b + (o))
   dlg_tbegin(g);
   k_steps<<<blocks(T, 64), 64, 0, g->stream>>>(T, PTR(int, oMonth), PTR(int, oDay), PTR(int, oJd), PTR(double, oHour), PTR(double, oDt),
                                                PTR(float, oDni), PTR(float, oDhi), in->lat, in->lon, in->tz, in->north, in->leap,
-                                               (float)in->grefl, in->wantVis, P, PTR(float, oPalt), PTR(float, oPazi), PTR(float, oOm),
+                                               (float)in->grefl, in->wantVis, in->reflect, P, PTR(float, oPalt), PTR(float, oPazi), PTR(float, oOm),
                                                PTR(StepOut, oSo), PTR(double, oAlt), PTR(double, oAzi), PTR(int, oUp), PTR(float, oVec));
   if (T > 0) {
     k_lum<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oPalt), PTR(float, oPazi), PTR(float, oL));
@@ -266,6 +301,11 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
                                                           PTR(double, oGv), seriesMode ? g->skyG.as<float>() : nullptr, PTR(double, oSunR),
                                                           PTR(double, oSunV));
     k_ground_sum<<<1, 256, 0, g->stream>>>(T, PTR(double, oGr), PTR(double, oGv), PTR(double, oGnd));
+    if (in->reflect && seriesMode && Tup > 0) {
+      k_sdc_sky<<<dim3(blocks(Tup, 256), unsigned(P + 1)), 256, 0, g->stream>>>(P, Tup, g->skyS.as<float>(), g->skyG.as<float>(), g->skySdc.as<float>());
+      k_sdc_sun<<<blocks(T, 256), 256, 0, g->stream>>>(T, Tup, PTR(StepOut, oSo), PTR(int, oCol), seriesMode == 2, g->skyS.as<float>(),
+                                                        g->skySdc.as<float>());
+    }
   }
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
@@ -286,17 +326,32 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
   DLG_CHECK(down(gnd, oGnd, 16));
   if (out->sunRad) DLG_CHECK(down(out->sunRad, oSunR, 8 * T));
   if (out->sunVis) DLG_CHECK(down(out->sunVis, oSunV, 8 * T));
+  std::vector<StepOut> steps(in->reflect ? size_t(T) : 0);
+  if (!steps.empty()) DLG_CHECK(down(steps.data(), oSo, sizeof(StepOut) * T));
   DLG_CHECK(hipStreamSynchronize(g->stream));
   if (out->gndRad) *out->gndRad = gnd[0];
   if (out->gndVis) *out->gndVis = gnd[1];
+  // reflected light: the sun binned (AddDirect) summed per patch in step order, as the sky's weights (run.cpp:414-441)
+  if (out->sunBinRad) std::fill_n(out->sunBinRad, P, 0.0);
+  if (out->sunBinVis) std::fill_n(out->sunBinVis, P, 0.0);
+  for (int t = 0; t < int(steps.size()); ++t) {
+    const StepOut& o = steps[size_t(t)];
+    if (o.q.night) continue;
+    for (int i = 0; i < 4; ++i) {
+      if (out->sunBinRad && in->wantRad && o.binRad[i] != 0.f) out->sunBinRad[o.binP[i]] += sun_bright(o.binRad[i]) * in->dt[t] / 1000.0;
+      if (out->sunBinVis && in->wantVis && o.binVis[i] != 0.f) out->sunBinVis[o.binP[i]] += 179.0 * sun_bright(o.binVis[i]) * in->dt[t];
+    }
+  }
 #undef PTR
   return 0;
 }
 
-extern "C" int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, char* err, int32_t errLen) {
+extern "C" int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, float* Sdc, char* err, int32_t errLen) {
   if (!g->skyT) { dlg_fail(err, errLen, "dlg_sky_series: the last dlg_sky kept no series"); return 1; }
+  if (Sdc && !g->skyReflect) { dlg_fail(err, errLen, "dlg_sky_series: the last dlg_sky ran without reflect"); return 1; }
   DLG_CHECK(hipMemcpyAsync(S, g->skyS.p, size_t(g->skyP) * size_t(g->skyT) * 4, hipMemcpyDeviceToHost, g->stream));
   DLG_CHECK(hipMemcpyAsync(gS, g->skyG.p, size_t(g->skyT) * 4, hipMemcpyDeviceToHost, g->stream));
+  if (Sdc) DLG_CHECK(hipMemcpyAsync(Sdc, g->skySdc.p, size_t(g->skyP + 1) * size_t(g->skyT) * 4, hipMemcpyDeviceToHost, g->stream));
   DLG_CHECK(hipStreamSynchronize(g->stream));
   return 0;
 }

@@ -35,7 +35,7 @@ const char* const kUsage =
     "  --no-occlude NAME[;NAME...]  sensor meshes analysed but casting no shade\n"
     "  --virtual-ground on|off  ground below the horizon reflecting --ground RHO (default on)\n"
     "  --optics SPEC[;SPEC...]  NAME[/SLOT]=RHO[,TAU[,glass|porous]]\n"
-    "  --transmission off|on  --fresnel on|off  --bounces N (0 only in this build)  --bounce-samples M\n"
+    "  --transmission off|on  --fresnel on|off  --bounces N  --bounce-samples M\n"
     "  --out FILE[.npy]         values (text rows, or float64 [N x C] NumPy when .npy) + FILE.json manifest\n"
     "  --timeseries FILE.npy  --dump-bits FILE.npy  --dump-sun FILE\n"
     "  --bench [NSENS NTRIS]  -v          exit code: 1 usage, 2 runtime error\n";

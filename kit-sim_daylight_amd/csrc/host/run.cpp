@@ -5,9 +5,9 @@
 // Host work here is set-up of size O(steps + patches + directions) and the file I/O; everything per ray, per sensor
 // or per (step, patch) runs on the GPU.
 // Transmitting slots (--transmission on, tau > 0) trace in byte mode: a transmittance byte per direction instead of a bit,
-// which the reduces and the contraction weight by.
-// Not in this build: reflected light (--bounces), the Ladybug emulation mode. A run that asks for them fails with a
-// message saying so.
+// which the reduces and the contraction weight by. Reflected light (--bounces) adds the bounce pass's rows (dlg_bounce)
+// weighted by the bins' radiance to the cumulative columns and the series.
+// Not in this build: the Ladybug emulation mode. A run that asks for it fails with a message saying so.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -283,7 +283,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   const bool series = !opt.seriesPath.empty();
   bool wantDc = series && needWeather;
   for (const Quantity q : opt.quantities) wantDc = wantDc || (q != Quantity::Direct && q != Quantity::SunHours);
-  if (opt.bounces > 0 && wantDc) { err = "--bounces (reflected light) is not part of this build"; return false; }
+  const bool reflect = opt.bounces > 0 && wantDc;   // reflected light: radiation, diffuse, irradiance, illuminance, the series
   double lat = w ? w->lat : opt.lat, lon = w ? w->lon : opt.lon, tz = w ? w->tz : opt.tz;
   if (!std::isnan(opt.lat)) lat = opt.lat;
   if (!std::isnan(opt.lon)) lon = opt.lon;
@@ -310,21 +310,28 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   std::vector<char> open;
   if (!resolve_optics(meshes, opt.optics, slots, err) || !resolve_no_occlude(meshes, opt.noOcclude, open, err)) return false;
   const auto blocks = [&](size_t k) { return !meshes[k].sensor || (opt.selfOcclude && !open[k]); };
-  std::vector<char> inGas(opt.transmission ? slots.size() : 0, 0);
+  std::vector<char> inGas(opt.transmission || reflect ? slots.size() : 0, 0);
   for (size_t k = 0, b = 0; !inGas.empty() && k < meshes.size(); b += slot_count(*meshes[k++].mesh)) {
     const SimulationMesh& m = *meshes[k].mesh;
     if (!blocks(k)) continue;
     if (m.slot.empty()) inGas[b] = !m.counts.empty();
     else for (const auto f : m.slot) inGas[b + size_t(f)] = 1;
   }
-  std::string lit;   // the transmitting slots: their triangles' slots are kept with the scene, so part of its identity
+  std::string lit;          // the transmitting slots: their triangles' slots are kept with the scene, so part of its identity
+  size_t first = SIZE_MAX;  // reflect: the first slot in the scene
+  bool differs = false;     // ... and another's rho or glass differs: the bounce pass reads each triangle's slot
   for (size_t s = 0; s < slots.size(); ++s) {
     if (!opt.transmission || !inGas[s]) slots[s].tau = 0;
     if (!opt.fresnel) slots[s].flags &= ~DLG_SLOT_GLASS;
     if (slots[s].tau > 0) lit += (lit.empty() ? "" : ",") + std::to_string(s);
+    if (!reflect || !inGas[s]) continue;
+    if (first == SIZE_MAX) first = s;
+    const SlotOptics &a = slots[s], &b = slots[first];
+    differs = differs || a.rho != b.rho || a.flags != b.flags || ((a.flags & DLG_SLOT_GLASS) && a.tn != b.tn);
   }
-  const bool translucent = !lit.empty();
-  const std::string key = opt.sceneKey.empty() ? "" : opt.sceneKey + (translucent ? "\ntransmitting " + lit : "") + open_key(open, opt.selfOcclude);
+  const bool translucent = !lit.empty(), slotted = translucent || differs;
+  const std::string key = opt.sceneKey.empty() ? "" : opt.sceneKey + (translucent ? "\ntransmitting " + lit : slotted ? "\nslot records" : "") +
+                                                   open_key(open, opt.selfOcclude);
   const int nS = opt.sides == Sides::Both ? 2 : 1;
   size_t nElem = 0;
   for (const SimulationMeshView& v : meshes) if (v.sensor) nElem += opt.onVertices ? v.mesh->numPoints() : v.mesh->numFaces();
@@ -341,7 +348,8 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   const int P = pa.P;
   const size_t Pz = size_t(P);
   std::vector<int> month(T), day(T), jd(T);
-  std::vector<double> hour(T), dtv(T), alt(T), azi(T), sunRadT(T), sunVisT(T), skyRad(Pz, 0.0), skyVis(Pz, 0.0);
+  std::vector<double> hour(T), dtv(T), alt(T), azi(T), sunRadT(T), sunVisT(T), skyRad(Pz, 0.0), skyVis(Pz, 0.0), sunBinRad(Pz, 0.0),
+      sunBinVis(Pz, 0.0);
   std::vector<float> dni(T), dhi(T), vec(3 * T);
   std::vector<int> up(T);
   for (size_t t = 0; t < T; ++t) {
@@ -355,8 +363,9 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   {
     dlg_sky_in in{int32_t(T), month.data(), day.data(), jd.data(), hour.data(), dtv.data(), dni.data(), dhi.data(), lat, lon, tz, opt.north, grefl,
                   leap ? 1 : 0, opt.mf, needWeather && wantRad ? 1 : 0, needWeather && wantVis ? 1 : 0, series && needWeather ? 1 : 0,
-                  seriesVis ? 1 : 0, tcos_table()};
-    dlg_sky_out so{alt.data(), azi.data(), up.data(), vec.data(), skyRad.data(), skyVis.data(), &gndRad, &gndVis, sunRadT.data(), sunVisT.data(), &Tup};
+                  seriesVis ? 1 : 0, tcos_table(), reflect ? 1 : 0};
+    dlg_sky_out so{alt.data(), azi.data(), up.data(), vec.data(), skyRad.data(), skyVis.data(), &gndRad, &gndVis, sunRadT.data(), sunVisT.data(), &Tup,
+                   reflect ? sunBinRad.data() : nullptr, reflect ? sunBinVis.data() : nullptr};
     char e[512] = {};
     if (dlg_sky(g, &in, &so, e, sizeof e)) { err = e; return false; }
   }
@@ -407,6 +416,20 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
     float* Wg = Wq.data() + nQ * nDir;
     for (size_t d = nSky; d < 2 * nSky; ++d) Wg[d] = pa.omega[size_t(rowOf[d - nSky])] / float(K);
   }
+  // reflected light (run.cpp:506-515): radiance per bounce bin (0 the ground, 1 + p the sky and the sun binned), no cosine
+  // (the rows hold it); radiation / diffuse / irradiance radiometric, illuminance visible, direct and sun hours none
+  const size_t nBins = Pz + 1;
+  std::vector<float> Wdc(reflect ? nQ * nBins : 0, 0.f);
+  std::vector<char> hasDc(nQ, 0);
+  for (size_t qi = 0; reflect && qi < nQ; ++qi) {
+    const Quantity q = opt.quantities[qi];
+    if (q == Quantity::Direct || q == Quantity::SunHours) continue;
+    const bool vis = q == Quantity::Illuminance;
+    float* W = Wdc.data() + qi * nBins;
+    hasDc[qi] = 1;
+    W[0] = float(vis ? gndVis : gndRad);
+    for (size_t p = 0; p < Pz; ++p) W[p + 1] = float(vis ? skyVis[p] + sunBinVis[p] : skyRad[p] + sunBinRad[p]);
+  }
   out.stats.skyMs = ms(tSky);
   mark("sky, directions, weights");
   if (opt.verbose && droppedDni > 0)
@@ -415,7 +438,8 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   // ---- the scene, the sensors, the result
   const size_t nWords = (nDir + 31) / 32;
   // per sensor of a chunk: its visibility (bits, or a byte per direction in whole words) or its series row
-  const size_t perSensor = std::max<size_t>(1, std::max((translucent ? 32 : 4) * nWords, series ? 4 * size_t(Tup) : size_t(0)));
+  const size_t perSensor =
+      std::max<size_t>(1, std::max({(translucent ? 32 : 4) * nWords, series ? 4 * size_t(Tup) : size_t(0), reflect ? 4 * nBins : size_t(0)}));
   size_t chunkBytes = opt.chunkBytes;
   File sf(nullptr, std::fclose), bf(nullptr, std::fclose);
   std::vector<double> seriesHoys;
@@ -430,7 +454,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   }
   dlg_gpu* gp = nullptr;
   Scene sc;
-  if (!scene_acquire(meshes, blocks, translucent, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
+  if (!scene_acquire(meshes, blocks, slotted, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
   mark("scene");
   dlg_stats_t st0{};
   std::vector<uint32_t>& thin = sc.thin;
@@ -454,7 +478,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
       cur.key = key;
       out.stats.prepMs += ms(tThin);
     }
-    const size_t buffers = 1 + (series ? 3 : 0);   // the bits; the series' output and its split operands
+    const size_t buffers = 1 + (reflect ? 1 : 0) + (series ? 3 : 0);   // the visibility; the bounce rows; the series' output and operands
     if (!run_admission(gp, nSens, perSensor, buffers * perSensor + 8 * (nQ + 1), 0, opt.residentScenes, chunkBytes, err)) return false;
   }
   // A thin solid's exit face passes everything. With --cull-backfaces a closed solid is crossed through one front face
@@ -482,7 +506,10 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
     char e[512] = {};
     if (dlg_set_scene(gp, reinterpret_cast<const float*>(sens->p.data()), reinterpret_cast<const float*>(sens->n.data()), int64_t(nSens),
                       reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e)) { err = e; return false; }
-    if (translucent && dlg_set_slots(gp, reinterpret_cast<const dlg_slot*>(slots.data()), int64_t(slots.size()), e, sizeof e)) { err = e; return false; }
+    // a scene without per-triangle slots reads slot 0: the one record of its slots' common optics (the reference's one SBT record)
+    const std::vector<SlotOptics> one{first == SIZE_MAX ? SlotOptics{} : slots[first]};
+    const std::vector<SlotOptics>& up = slotted ? slots : one;
+    if ((translucent || reflect) && dlg_set_slots(gp, reinterpret_cast<const dlg_slot*>(up.data()), int64_t(up.size()), e, sizeof e)) { err = e; return false; }
   }
   scratch = Sensors{};
   mark("upload");
@@ -514,8 +541,9 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
     std::vector<int32_t> useCos(nQ + 1, 1);
     for (size_t qi = 0; qi < nQ; ++qi) useCos[qi] = opt.quantities[qi] != Quantity::SunHours;
     const int nR = int(nQ) + (series ? 1 : 0);
-    std::vector<double> tmp;
+    std::vector<double> tmp, tmp2;
     std::vector<float> G, E;
+    const dlg_bounce_in bin{uint32_t(opt.bounces), uint32_t(opt.bounceSamples), opt.mf, opt.north};
     std::vector<uint32_t> bitsBuf, bitRows;
     std::vector<uint8_t> byteBuf, byteRows;
     std::vector<float> wsky(nSky), sunW(nSun, 0.f);
@@ -533,11 +561,19 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
       const size_t nc = std::min(ncMax, nSens - i0);
       char e[512] = {};
       if (dlg_trace(gp, int64_t(i0), int64_t(nc), flags, e, sizeof e)) { err = e; return false; }
+      // the reflected light next: it writes only its own rows, so the reduces see the trace's visibility
+      if (reflect && dlg_bounce(gp, int64_t(i0), int64_t(nc), dlg_bounce_flags(opt.cullBackfaces ? 1 : 0), &bin, e, sizeof e)) { err = e; return false; }
       const auto tRed = Clk::now();
       tmp.assign(size_t(nR) * nc, 0.0);
       if (dlg_reduce(gp, int64_t(nc), nR, Wq.data(), useCos.data(), tmp.data(), e, sizeof e)) { err = e; return false; }
-      for (size_t qi = 0; qi < nQ; ++qi)
-        for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, tmp[qi * nc + i]);
+      for (size_t qi = 0; qi < nQ; ++qi) {
+        if (hasDc[qi]) {
+          tmp2.assign(nc, 0.0);
+          if (dlg_reduce_dc(gp, int64_t(nc), Wdc.data() + qi * nBins, tmp2.data(), e, sizeof e)) { err = e; return false; }
+          for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, tmp[qi * nc + i] + tmp2[i]);   // the same double sum as the reference's
+        } else
+          for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, tmp[qi * nc + i]);
+      }
       out.stats.reduceMs += ms(tRed);
       if (opt.progress) opt.progress->store(double(i0 + nc) / double(nSens));
       if (bf && translucent) {   // direction-major bytes -> row-major [nc x nDir] rows
@@ -559,7 +595,8 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
         G.resize(nc);
         for (size_t i = 0; i < nc; ++i) G[i] = float(tmp[nQ * nc + i]);
         const bool compute = Tup > 0 && needWeather;
-        dlg_series_in in{uint32_t(nSky), rowOf.data(), wsky.data(), P, Tup, G.data(), uint32_t(sun0), uint32_t(nSun), sunCol.data(), sunW.data()};
+        dlg_series_in in{uint32_t(nSky), rowOf.data(), wsky.data(), P, Tup, G.data(), uint32_t(sun0), uint32_t(nSun), sunCol.data(), sunW.data(),
+                         reflect ? 1 : 0};
         if (sink) {   // the series stays on the device: the sink reads each chunk there
           if (compute && dlg_series(gp, int64_t(nc), &in, method, nullptr, e, sizeof e)) { err = e; return false; }
           if (*sink && !(*sink)(i0, nc, size_t(Tup), compute ? dlg_series_device(gp) : nullptr, err)) return false;
@@ -574,13 +611,19 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
     dlg_stats_t st1{};
     dlg_stats(gp, &st1);
     out.stats.traceMs = st1.traceMs - st0.traceMs;
+    out.stats.bounceMs = st1.bounceMs - st0.bounceMs;
     out.stats.rays = size_t(st1.rays - st0.rays);
     out.stats.launches = size_t(st1.launches - st0.launches);
     out.stats.gasBytes = size_t(st1.sceneBytes);
-    if (opt.verbose)
+    if (opt.verbose) {
       std::fprintf(stderr, "info: scene %.1f MB (LBVH height %lld) build %.0f ms, %zu launches, %.3g rays in %.0f ms (%.2f GRays/s)\n",
                    st1.sceneBytes / 1048576.0, (long long)st1.height, st1.buildMs, out.stats.launches, double(out.stats.rays), out.stats.traceMs,
                    out.stats.traceMs > 0 ? double(out.stats.rays) / out.stats.traceMs / 1e6 : 0.0);
+      if (reflect)
+        std::fprintf(stderr, "info: reflected light: %d bounces, %d samples per sensor, %.3g paths in %.0f ms (%lld launches)%s\n", opt.bounces,
+                     opt.bounceSamples, double(st1.paths - st0.paths), out.stats.bounceMs, (long long)(st1.bounceLaunches - st0.bounceLaunches),
+                     differs ? ", a slot per triangle" : "");
+    }
   }
   if (sf) {
     File hf(std::fopen((opt.seriesPath + ".hoys.txt").c_str(), "w"), std::fclose);

@@ -131,7 +131,9 @@ DL_HD void dir_unpack(Ray& r, float3 d, float4 a, float4 b) {
 }
 
 // Hit on [tmin, tmax]: t and front (det > 0: the ray meets the side the winding's normal faces). cull drops back faces.
-DL_HD bool tri_hit(const Ray& r, float3 v0, float3 v1, float3 v2, float tmin, float tmax, bool cull, float& tOut, bool& front) {
+// bc (optional): the hit's barycentric weights of v1 and v2 (V / det, W / det).
+DL_HD bool tri_hit(const Ray& r, float3 v0, float3 v1, float3 v2, float tmin, float tmax, bool cull, float& tOut, bool& front,
+                   float* bc = nullptr) {
   const float3 A = make_float3(v0.x - r.o.x, v0.y - r.o.y, v0.z - r.o.z);
   const float3 B = make_float3(v1.x - r.o.x, v1.y - r.o.y, v1.z - r.o.z);
   const float3 C = make_float3(v2.x - r.o.x, v2.y - r.o.y, v2.z - r.o.z);
@@ -164,13 +166,13 @@ DL_HD bool tri_hit(const Ray& r, float3 v0, float3 v1, float3 v2, float tmin, fl
   if (!(t >= tmin && t <= tmax)) return false;
   tOut = t;
   front = det > 0.f;
+  if (bc) { bc[0] = V / det; bc[1] = W / det; }
   return true;
 }
 
-// own_hit (trace_dev.cu:60-72): the hit lies under 1/32 m above the sensor's plane, its triangle is not degenerate, its
-// plane passes within 2^-22 x its extent of the origin and faces away from the sensor's normal n.
-DL_HD bool own_hit(float3 d, float3 n, float t, bool front, float3 v0, float3 v1, float3 v2) {
-  if (!(t * dot3(n, d) < 1.f / 32.f)) return false;
+// own_plane (trace_dev.cu:63-66): the hit's triangle is not degenerate, its plane passes within 2^-22 x its extent of
+// the origin, and it faces away from n (the sensor's side)
+DL_HD bool own_plane(float3 d, float3 n, float t, bool front, float3 v0, float3 v1, float3 v2) {
   const float3 e1 = make_float3(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z), e2 = make_float3(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z);
   const float3 ng = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
   const float l2 = dot3(ng, ng);
@@ -180,10 +182,14 @@ DL_HD bool own_hit(float3 d, float3 n, float t, bool front, float3 v0, float3 v1
   const float ext = fmaxf(fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)), fmaxf(fabsf(e1.z), fabsf(e2.x))), fmaxf(fabsf(e2.y), fabsf(e2.z)));
   return t * fabsf(dot3(d, hn)) < ext * 2.3841858e-7f && dot3(n, hn) <= 0.f;
 }
+// own_hit (trace_dev.cu:60-72): the hit lies under 1/32 m above the sensor's plane (the gate), and own_plane.
+DL_HD bool own_hit(float3 d, float3 n, float t, bool front, float3 v0, float3 v1, float3 v2) {
+  return t * dot3(n, d) < 1.f / 32.f && own_plane(d, n, t, front, v0, v1, v2);
+}
 
 // ------------------------------------------------------------------------------------------------ transmission
 // Slot flags (reference src/params.h:14-19; include/dl_device.h DLG_SLOT_*)
-constexpr uint32_t kSlotGlass = 1, kSlotThin = 2, kSlotInward = 4;
+constexpr uint32_t kSlotGlass = 1, kSlotThin = 2, kSlotInward = 4, kSlotTwin = 8;
 
 // A transmittance byte back as the fraction it stands for: b / 255.f, correctly rounded (the quotient in double rounds to
 // the same float: b / 255 repeats an 8-bit pattern, so it never lies within 2^-53 of a float midpoint).
@@ -197,18 +203,26 @@ DL_HD double tn_from_Tn(double Tn) {
   return fmax(0., (sqrt(.8402528435 + .0072522239 * Tn * Tn) - .9166530661) / .0036261119 / Tn);
 }
 
-// Transmittance of a glass pane at incidence cosine c (0..1) for transmissivity tn (reference src/glass.h:17-25, after
+// Transmittance T of a glass pane at incidence cosine c (0..1) for transmissivity tn (reference src/glass.h:17-25, after
 // m_glass.c): refractive index 1.52, the cosine inside c2 by Snell's law, the path's transmissivity tn^(1 / c2), the
 // Fresnel reflectances of the s and p polarisations at each face and the pane's internal reflections summed,
 // (1 - r)^2 d / (1 - r^2 d^2), averaged over the two. In double; the reference's float formula agrees to a few float ulps.
-DL_HD float glass_T(double c, double tn) {
+// The reflectance R likewise: the first face's Fresnel reflection plus what the pane's internal reflections send back,
+// rs (1 + (1 - 2 rs) d^2) / (1 - rs^2 d^2) per polarisation, averaged.
+DL_HD void glass_TR(double c, double tn, float& T, float& R) {
   const double n = 1.52;
   const double c2 = sqrt((1. - 1. / (n * n)) + c * c / (n * n));
   const double d = pow(tn, 1. / c2), d2 = d * d;
   double rs = (c - n * c2) / (c + n * c2), rp = (c2 - n * c) / (c2 + n * c);
   rs *= rs;
   rp *= rp;
-  return float(.5 * (1. - rs) * (1. - rs) * d / (1. - rs * rs * d2) + .5 * (1. - rp) * (1. - rp) * d / (1. - rp * rp * d2));
+  T = float(.5 * (1. - rs) * (1. - rs) * d / (1. - rs * rs * d2) + .5 * (1. - rp) * (1. - rp) * d / (1. - rp * rp * d2));
+  R = float(.5 * rs * (1. + (1. - 2. * rs) * d2) / (1. - rs * rs * d2) + .5 * rp * (1. + (1. - 2. * rp) * d2) / (1. - rp * rp * d2));
+}
+DL_HD float glass_T(double c, double tn) {
+  float T, R;
+  glass_TR(c, tn, T, R);
+  return T;
 }
 
 // What a translucent slot's triangle passes (reference __anyhit__tr, trace_dev.cu:149-165): 1 on a thin solid's exit
@@ -223,6 +237,42 @@ DL_HD float pass_factor(float tau, float tn, uint32_t flags, bool front, float3 
   const double l = sqrt(nx * nx + ny * ny + nz * nz);
   if (!(l > 0.)) return 0.f;
   return glass_T(fabs(nx * d.x + ny * d.y + nz * d.z) / l, tn);
+}
+
+// ------------------------------------------------------------------------------------------------ reflected light
+// 32-bit reversal (the Hammersley point's radical inverse)
+DL_HD uint32_t brev32(uint32_t x) {
+  x = (x >> 16) | (x << 16);
+  x = ((x & 0xff00ff00u) >> 8) | ((x & 0x00ff00ffu) << 8);
+  x = ((x & 0xf0f0f0f0u) >> 4) | ((x & 0x0f0f0f0fu) << 4);
+  x = ((x & 0xccccccccu) >> 2) | ((x & 0x33333333u) << 2);
+  return ((x & 0xaaaaaaaau) >> 1) | ((x & 0x55555555u) << 1);
+}
+// A cosine-weighted direction about unit n (trace_dev.cu:235-242): sin^2 theta = u, azimuth 2 pi v, in the orthonormal
+// basis of Duff et al. 2017 (JCGT 6(1), "Building an orthonormal basis, revisited").
+DL_HD float3 cosine_dir(float3 n, float u, float v) {
+  const float sg = copysignf(1.f, n.z), a = -1.f / (sg + n.z), b = n.x * n.y * a;
+  const float3 t = make_float3(1.f + sg * n.x * n.x * a, sg * b, -sg * n.x), w = make_float3(b, sg + n.y * n.y * a, -n.y);
+  float s, c;
+  sincospi(2.f * v, &s, &c);
+  const float r = sqrtf(u), z = sqrtf(fmaxf(0.f, 1.f - u)), rc = r * c, rs = r * s;
+  return make_float3((rc * t.x + rs * w.x) + z * n.x, (rc * t.y + rs * w.y) + z * n.y, (rc * t.z + rs * w.z) + z * n.z);
+}
+// The bin of an escaping direction (trace_dev.cu:244-258, Radiance reinhart.cal's rbin): 0 below the horizon (the
+// ground), else 1 + its Reinhart patch of sky density mf in gendaymtx order, the direction turned back to true north
+// (cosN, sinN of --north); alpha the patch rows' height (radians); the zenith cap is the last bin. The angles in double.
+DL_HD uint32_t sky_bin(float3 d, uint32_t mf, float alpha, uint32_t nBins, float cosN, float sinN) {
+  if (!(d.z >= 0.f)) return 0;
+  const uint32_t tnaz[7] = {30, 30, 24, 24, 18, 12, 6};   // Tregenza patches per row (x mf rows of x mf patches)
+  const double el = asin(double(fminf(d.z, 1.f)));
+  uint32_t r = uint32_t(el / double(alpha));
+  if (r >= 7 * mf) return nBins - 1;
+  const uint32_t band = r / mf, n = tnaz[band] * mf;
+  uint32_t b = 1 + (r % mf) * n;
+  for (uint32_t q = 0; q < band; ++q) b += tnaz[q] * mf * mf;
+  double az = atan2(double(d.x * cosN + d.y * sinN), double(d.y * cosN - d.x * sinN));   // clockwise from north
+  if (az < 0.) az += 2. * kPi;
+  return b + uint32_t(az * (double(n) / (2. * kPi)) + .5) % n;
 }
 
 // ------------------------------------------------------------------------------------------------ the sun (FP64)
@@ -390,6 +440,8 @@ DL_HD double sky_bright(float L, double sf) {
   const float r = l * .96f, g = l * 1.004f, b = l * 1.118f;
   return 0.265074126 * r + 0.670114631 * g + 0.064811243 * b;
 }
+// bright() of a grey value v (r = g = b, as AddDirect adds the sun): the CIE weights summed in channel order (run.cpp:27)
+DL_HD double sun_bright(float v) { return 0.265074126 * v + 0.670114631 * v + 0.064811243 * v; }
 DL_HD double ground_bright(float g, float grefl) {
   const float v = g * grefl;
   return 0.265074126 * v + 0.670114631 * v + 0.064811243 * v;

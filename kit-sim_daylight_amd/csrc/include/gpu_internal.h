@@ -63,10 +63,14 @@ struct dlg_gpu {
   int32_t bytes = 0;               // the last trace wrote bytes
   DevBuf slots;                    // dlg_set_slots: [nSlots] dlg_slot
   int64_t nSlots = 0;
+  // reflected light (dlg_bounce): rows dc[b nc + i] of the chunk, and the sky's bin radiance matrix Sdc [(P + 1) x Tup]
+  DevBuf dc, skySdc;
+  uint32_t dcBins = 0;
+  int64_t dcN = 0, dcI0 = 0;
   DevBuf scratch, out;             // per-call inputs / outputs of the reduces
   // sky (dlg_sky): S [P x Tup] and g [Tup] of the last run, the tcos table
   DevBuf skyS, skyG, skyWork;
-  int32_t skyP = 0, skyT = 0;
+  int32_t skyP = 0, skyT = 0, skyReflect = 0;
   // series
   DevBuf seriesE, seriesWork;
   int64_t seriesN = 0;

@@ -28,26 +28,32 @@ class SkyIn(C.Structure):
     _fields_ = [("T", C.c_int32), ("month", C.c_void_p), ("day", C.c_void_p), ("jd", C.c_void_p), ("hour", C.c_void_p),
                 ("dt", C.c_void_p), ("dni", C.c_void_p), ("dhi", C.c_void_p), ("lat", C.c_double), ("lon", C.c_double),
                 ("tz", C.c_double), ("north", C.c_double), ("grefl", C.c_double), ("leap", C.c_int32), ("mf", C.c_int32),
-                ("wantRad", C.c_int32), ("wantVis", C.c_int32), ("series", C.c_int32), ("seriesVis", C.c_int32), ("tcos", C.c_void_p)]
+                ("wantRad", C.c_int32), ("wantVis", C.c_int32), ("series", C.c_int32), ("seriesVis", C.c_int32), ("tcos", C.c_void_p),
+                ("reflect", C.c_int32)]
 
 
 class SkyOut(C.Structure):
     _fields_ = [("alt", C.c_void_p), ("azi", C.c_void_p), ("up", C.c_void_p), ("vec", C.c_void_p), ("skyRad", C.c_void_p),
                 ("skyVis", C.c_void_p), ("gndRad", C.c_void_p), ("gndVis", C.c_void_p), ("sunRad", C.c_void_p),
-                ("sunVis", C.c_void_p), ("Tup", C.c_void_p)]
+                ("sunVis", C.c_void_p), ("Tup", C.c_void_p), ("sunBinRad", C.c_void_p), ("sunBinVis", C.c_void_p)]
 
 
 class SeriesIn(C.Structure):
     _fields_ = [("nSky", C.c_uint32), ("row", C.c_void_p), ("wsky", C.c_void_p), ("P", C.c_int32), ("T", C.c_int32),
                 ("G", C.c_void_p), ("sun0", C.c_uint32), ("nSun", C.c_uint32), ("sunStep", C.c_void_p),
-                ("sunW", C.c_void_p)]
+                ("sunW", C.c_void_p), ("reflect", C.c_int32)]
 
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("buildMs", "traceMs", "reduceMs", "seriesMs", "skyMs", "viewMs")] + \
                [(n, C.c_int64) for n in ("rays", "traced", "launches", "sceneBytes", "nodes", "height")] + \
                [(n, C.c_double) for n in ("lastTraceMs", "lastReduceMs", "lastSeriesMs", "lastSkyMs", "lastViewMs", "lastViewReduceMs")] + \
-               [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays", "waveIters")]
+               [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays", "waveIters")] + \
+               [("bounceMs", C.c_double), ("paths", C.c_int64), ("bounceLaunches", C.c_int64)]
+
+
+class BounceIn(C.Structure):
+    _fields_ = [("bounces", C.c_uint32), ("M", C.c_uint32), ("mf", C.c_int32), ("north", C.c_double)]
 
 
 class ViewIn(C.Structure):
@@ -112,8 +118,12 @@ SEAM = [
     ("dlg_download_bytes", C.c_int32, [C.c_void_p, C.c_int64, U8P, C.c_char_p, C.c_int32]),
     ("dlg_set_slots", C.c_int32, [C.c_void_p, F32P, C.c_int64, C.c_char_p, C.c_int32]),
     ("dlg_set_tri_slots", C.c_int32, [C.c_void_p, U32P, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_bounce_flags", C.c_uint32, [C.c_int32]),
+    ("dlg_bounce", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint32, C.POINTER(BounceIn), C.c_char_p, C.c_int32]),
+    ("dlg_reduce_dc", C.c_int32, [C.c_void_p, C.c_int64, F32P, F64P, C.c_char_p, C.c_int32]),
+    ("dlg_download_dc", C.c_int32, [C.c_void_p, C.c_int64, F32P, C.c_char_p, C.c_int32]),
     ("dlg_sky", C.c_int32, [C.c_void_p, C.POINTER(SkyIn), C.POINTER(SkyOut), C.c_char_p, C.c_int32]),
-    ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_char_p, C.c_int32]),
+    ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),
     ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
@@ -239,39 +249,57 @@ class Gpu:
         _check(self.L.dlg_reduce(self.h, nc, len(useCos), W.reshape(-1), np.array(useCos, np.int32), out, self.err, 1024), self.err)
         return out.reshape(len(useCos), nc)
 
-    def sky(self, steps, lat, lon, tz, leap, north, grefl, mf, tcos, wantRad=True, wantVis=False, series=False, seriesVis=False):
+    def sky(self, steps, lat, lon, tz, leap, north, grefl, mf, tcos, wantRad=True, wantVis=False, series=False, seriesVis=False, reflect=False):
         T = len(steps["month"])
         arr = {k: np.ascontiguousarray(v) for k, v in steps.items()}
         keep = [arr, tcos]
         sin = SkyIn(T, arr["month"].ctypes.data, arr["day"].ctypes.data, arr["jd"].ctypes.data, arr["hour"].ctypes.data,
                     arr["dt"].ctypes.data, arr["dni"].ctypes.data, arr["dhi"].ctypes.data, lat, lon, tz, north, grefl, int(leap), mf,
-                    int(wantRad), int(wantVis), int(series), int(seriesVis), tcos.ctypes.data)
+                    int(wantRad), int(wantVis), int(series), int(seriesVis), tcos.ctypes.data, int(reflect))
         P = {1: 145, 2: 577, 4: 2305}[mf]
         out = dict(alt=np.zeros(T), azi=np.zeros(T), up=np.zeros(T, np.int32), vec=np.zeros(3 * T, np.float32), skyRad=np.zeros(P),
                    skyVis=np.zeros(P), gndRad=np.zeros(1), gndVis=np.zeros(1), sunRad=np.zeros(T), sunVis=np.zeros(T),
-                   Tup=np.zeros(1, np.int32))
-        so = SkyOut(*[out[k].ctypes.data for k in ("alt", "azi", "up", "vec", "skyRad", "skyVis", "gndRad", "gndVis", "sunRad", "sunVis", "Tup")])
+                   Tup=np.zeros(1, np.int32), sunBinRad=np.zeros(P), sunBinVis=np.zeros(P))
+        so = SkyOut(*[out[k].ctypes.data for k in ("alt", "azi", "up", "vec", "skyRad", "skyVis", "gndRad", "gndVis", "sunRad", "sunVis", "Tup",
+                                                   "sunBinRad", "sunBinVis")])
         _check(self.L.dlg_sky(self.h, C.byref(sin), C.byref(so), self.err, 1024), self.err)
         del keep
         out["vec"] = out["vec"].reshape(T, 3)
         out["Tup"] = int(out["Tup"][0])
         if series:
             S, g = np.zeros(max(1, P * out["Tup"]), np.float32), np.zeros(max(1, out["Tup"]), np.float32)
-            _check(self.L.dlg_sky_series(self.h, S, g, self.err, 1024), self.err)
+            Sdc = np.zeros(max(1, (P + 1) * out["Tup"]), np.float32) if reflect else None
+            _check(self.L.dlg_sky_series(self.h, S, g, Sdc.ctypes.data if reflect else None, self.err, 1024), self.err)
             out["S"], out["g"] = S[: P * out["Tup"]].reshape(P, out["Tup"]), g[: out["Tup"]]
+            if reflect:
+                out["Sdc"] = Sdc[: (P + 1) * out["Tup"]].reshape(P + 1, out["Tup"])
         return out
 
-    def series(self, nc, nSky, row, wsky, P, T, G, sun0, sunStep, sunW, method, download=True):
+    def series(self, nc, nSky, row, wsky, P, T, G, sun0, sunStep, sunW, method, download=True, reflect=False):
         row = np.ascontiguousarray(row, np.int32)
         wsky = np.ascontiguousarray(wsky, np.float32)
         G = np.ascontiguousarray(G, np.float32)
         sunStep = np.ascontiguousarray(sunStep, np.int32)
         sunW = np.ascontiguousarray(sunW, np.float32)
         si = SeriesIn(nSky, row.ctypes.data, wsky.ctypes.data, P, T, G.ctypes.data, sun0, len(sunW), sunStep.ctypes.data,
-                      sunW.ctypes.data)
+                      sunW.ctypes.data, int(reflect))
         E = np.zeros(nc * T, np.float32) if download else None
         _check(self.L.dlg_series(self.h, nc, C.byref(si), method, E.ctypes.data if download else None, self.err, 1024), self.err)
         return E.reshape(nc, T) if download else None
+
+    def bounce(self, i0, nc, bounces, M, mf=1, north=0.0, cull=False):
+        """reflected light of sensors [i0, i0 + nc): the rows [(P + 1) x nc] (bin 0 the ground, 1 + p the patches)"""
+        b = BounceIn(bounces, M, mf, north)
+        _check(self.L.dlg_bounce(self.h, i0, nc, self.L.dlg_bounce_flags(int(cull)), C.byref(b), self.err, 1024), self.err)
+        nb = 144 * mf * mf + 2
+        out = np.zeros(max(1, nb * nc), np.float32)
+        _check(self.L.dlg_download_dc(self.h, nc, out, self.err, 1024), self.err)
+        return out[: nb * nc].reshape(nb, nc)
+
+    def reduce_dc(self, nc, W):
+        out = np.zeros(nc, np.float64)
+        _check(self.L.dlg_reduce_dc(self.h, nc, np.ascontiguousarray(W, np.float32), out, self.err, 1024), self.err)
+        return out
 
     def stats(self):
         s = Stats()

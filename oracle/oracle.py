@@ -137,6 +137,10 @@ def lib():
         L.orc_tn_from_Tn.argtypes = [C.c_double]
         L.orc_glass_T.restype = C.c_float
         L.orc_glass_T.argtypes = [C.c_double, C.c_double]
+        L.orc_series_dc.argtypes = [F32P, C.c_long, C.c_uint, F32P, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_glass_TR.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_bounce.argtypes = [C.c_void_p, F32P, F32P, C.c_long, C.c_uint, C.c_void_p, F32P, C.c_long, C.c_uint, C.c_uint, C.c_int,
+                                 C.c_double, C.c_int, F32P, C.c_int]
         L.orc_reduce_bytes.argtypes = [U8P, C.c_long, C.c_int, F32P, F32P, F32P, C.c_int, F64P]
         L.orc_series64_bytes.argtypes = [U8P, C.c_long, F32P, F32P, C.c_uint, I32P, F32P, F32P, C.c_int, F32P, F32P, C.c_uint,
                                          C.c_uint, I32P, F32P, F32P, F64P, C.c_int]

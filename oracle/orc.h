@@ -88,9 +88,15 @@ void orc_trace_bits(const orc_scene* s, const float* P, const float* N, long nc,
  * hits, the transmittance a product in double. */
 void orc_trace_bytes(const orc_scene* s, const float* P, const float* N, long nc, const float* D, int nDir, int cull, const uint32_t* triSlot,
                      const float* slots, long nSlots, uint8_t* bytes, int nthreads);
-/* glass.h: tn_from_Tn, and glass_TR's transmittance (in double, rounded to float) */
+/* glass.h: tn_from_Tn, and glass_TR's transmittance and reflectance (in double, rounded to float) */
 double orc_tn_from_Tn(double Tn);
 float orc_glass_T(double c, double tn);
+void orc_glass_TR(double c, double tn, float* T, float* R);
+/* __raygen__bounce semantics (trace_dev.cu:260-311): reflected light of sensors i0 .. i0 + nc - 1 (global indices seed
+ * the paths), M paths each with up to `bounces` diffuse reflections, rows dc[b nc + i] (bin 0 the ground, 1 + p the
+ * Reinhart patches of sky density mf), slots as orc_trace_bytes; cull: back faces culled. */
+void orc_bounce(const orc_scene* s, const float* P, const float* N, long nc, unsigned i0, const uint32_t* triSlot, const float* slots, long nSlots,
+                unsigned bounces, unsigned M, int mf, double north, int cull, float* dc, int nthreads);
 /* One ray as the bits pass classifies it: 1 visible, 0 blocked or culled. */
 int orc_visible(const orc_scene* s, const float p[3], const float n[3], const float d[3], int cull);
 /* The same against every triangle (no BVH): the hit set the traversal must reproduce. */
@@ -149,6 +155,8 @@ void orc_series64_bytes(const uint8_t* bytes, long nc, const float* N, const flo
 void orc_series32_bytes(const uint8_t* bytes, long nc, const float* N, const float* D, unsigned nSky, const int* row, const float* wsky,
                         const float* S, int T, const float* g, const float* G, unsigned sun0, unsigned nSun, const int* sunStep,
                         const float* sunVec, const float* sunW, float* E);
+/* k_series_dc (gpu.cu:153-169): E += sum_b dc(b, i) Sdc[b, t], E32 as the reference's FP32 kernel, E64 in double */
+void orc_series_dc(const float* dc, long nc, unsigned nBins, const float* Sdc, int T, float* E32, double* E64);
 /* k_view + k_iso (gpu.cu:171-274): out[i NA + a]. isoDim 0/2/3. */
 void orc_reduce_view(const uint8_t* cls, const float* dist, long nc, unsigned K, unsigned nAz, const double* wRow, unsigned nClass,
                      double farDist, const float* reach, unsigned nReach, int glazing, unsigned isoDim, const orc_view* v,

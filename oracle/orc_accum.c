@@ -153,6 +153,23 @@ void orc_series32_bytes(const uint8_t* bytes, long nc, const float* N, const flo
   series32(&v, nc, N, D, nSky, row, wsky, S, T, g, G, sun0, nSun, sunStep, sunVec, sunW, E);
 }
 
+/* k_series_dc (gpu.cu:153-169): reflected light added to a series, E[i T + t] += sum_b dc[b nc + i] Sdc[b T + t]: E32 in
+ * FP32 as the reference's kernel (an fma chain over the bins, then added), E64 in double (either may be NULL) */
+void orc_series_dc(const float* dc, long nc, unsigned nBins, const float* Sdc, int T, float* E32, double* E64) {
+  for (long i = 0; i < nc; ++i)
+    for (int t = 0; t < T; ++t) {
+      float e = 0.f;
+      double e64 = 0.0;
+      for (unsigned b = 0; b < nBins; ++b) {
+        const float c = dc[(size_t)b * nc + i], s = Sdc[(size_t)b * T + t];
+        e = fmaf(c, s, e);
+        e64 += (double)c * (double)s;
+      }
+      if (E32) E32[(size_t)i * T + t] += e;
+      if (E64) E64[(size_t)i * T + t] += e64;
+    }
+}
+
 /* lane values summed as the wave64 tree: v[l] += v[l + s] for s = 32 .. 1, result in lane 0 */
 static double tree64(double* v) {
   for (int s = LANES / 2; s; s >>= 1)

@@ -119,8 +119,8 @@ static void ray_setup(ray_t* r, const float o[3], const float d[3], float pad) {
 }
 
 /* Hit with t in [tmin, tmax]: 1 and t, front (the ray meets the side the winding's normal faces: det > 0). */
-static int tri_hit(const ray_t* r, const float* v0, const float* v1, const float* v2, float tmin, float tmax, int cull, float* tOut,
-                   int* front) {
+static int tri_hit_bc(const ray_t* r, const float* v0, const float* v1, const float* v2, float tmin, float tmax, int cull, float* tOut,
+                      int* front, float* bc) {
   const float A[3] = {v0[0] - r->o[0], v0[1] - r->o[1], v0[2] - r->o[2]};
   const float B[3] = {v1[0] - r->o[0], v1[1] - r->o[1], v1[2] - r->o[2]};
   const float C[3] = {v2[0] - r->o[0], v2[1] - r->o[1], v2[2] - r->o[2]};
@@ -154,13 +154,22 @@ static int tri_hit(const ray_t* r, const float* v0, const float* v1, const float
   if (!(t >= tmin && t <= tmax)) return 0;
   *tOut = t;
   *front = det > 0.f;
+  if (bc) { bc[0] = V / det; bc[1] = W / det; }
   return 1;
+}
+static int tri_hit(const ray_t* r, const float* v0, const float* v1, const float* v2, float tmin, float tmax, int cull, float* tOut,
+                   int* front) {
+  return tri_hit_bc(r, v0, v1, v2, tmin, tmax, cull, tOut, front, NULL);
 }
 
 /* own_hit (trace_dev.cu:60-72): under the 1/32 m gate, non-degenerate, its plane within 2^-22 x its extent of the
  * origin and facing away from the sensor's normal n */
+static int own_plane(const float d[3], const float n[3], float t, int front, const float* v0, const float* v1, const float* v2);
 static int own_hit(const float d[3], const float n[3], float t, int front, const float* v0, const float* v1, const float* v2) {
-  if (!(t * dot3(n, d) < 1.f / 32.f)) return 0;
+  return t * dot3(n, d) < 1.f / 32.f && own_plane(d, n, t, front, v0, v1, v2);
+}
+/* own_plane (trace_dev.cu:63-66): non-degenerate, its plane within 2^-22 x its extent of the origin, facing away from n */
+static int own_plane(const float d[3], const float n[3], float t, int front, const float* v0, const float* v1, const float* v2) {
   const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
   const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
   const float ng[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
@@ -660,13 +669,20 @@ double orc_tn_from_Tn(double Tn) {
 }
 /* glass_TR's T (glass.h:17-25): the pane at incidence cosine c, n = 1.52, both polarisations' two-face series, in double,
  * rounded to float */
-float orc_glass_T(double c, double tn) {
+void orc_glass_TR(double c, double tn, float* T, float* R) {
   const double n = 1.52;
   const double ci = sqrt((1. - 1. / (n * n)) + c * c / (n * n));   /* cosine inside the glass (Snell) */
   const double a = pow(tn, 1. / ci), a2 = a * a;                   /* the slanted path's transmissivity */
   const double s0 = (c - n * ci) / (c + n * ci), p0 = (ci - n * c) / (ci + n * c);
   const double rs = s0 * s0, rp = p0 * p0;
-  return (float)(.5 * (1. - rs) * (1. - rs) * a / (1. - rs * rs * a2) + .5 * (1. - rp) * (1. - rp) * a / (1. - rp * rp * a2));
+  *T = (float)(.5 * (1. - rs) * (1. - rs) * a / (1. - rs * rs * a2) + .5 * (1. - rp) * (1. - rp) * a / (1. - rp * rp * a2));
+  /* reflected: the first face plus what the internal reflections send back */
+  *R = (float)(.5 * rs * (1. + (1. - 2. * rs) * a2) / (1. - rs * rs * a2) + .5 * rp * (1. + (1. - 2. * rp) * a2) / (1. - rp * rp * a2));
+}
+float orc_glass_T(double c, double tn) {
+  float T, R;
+  orc_glass_TR(c, tn, &T, &R);
+  return T;
 }
 /* __anyhit__tr's factor (trace_dev.cu:149-165): a thin solid's exit face 1; glass by the angle of the hit triangle's
  * geometric normal (float edges, double products) to d; else tau. slot: rho, tau, tn, flags (as bits). */
@@ -798,6 +814,153 @@ void orc_trace_bytes(const orc_scene* s, const float* P, const float* N, long nc
   b.op.s = s; b.op.triSlot = triSlot; b.op.slots = slots; b.op.nSlots = nSlots;
   b.bytes = bytes;
   run_workers(&b.j, bytes_worker, nthreads);
+}
+
+/* ---------------------------------------------------------------- reflected light (the bounce pass) */
+static int closest(const orc_scene* s, const ray_t* r, float tmin, float tmax, int cull, float* t, long* leaf, int* front);
+static uint32_t brev32(uint32_t x) {
+  uint32_t r = 0;
+  for (int k = 0; k < 32; ++k) r |= ((x >> k) & 1u) << (31 - k);
+  return r;
+}
+/* cosine_dir (trace_dev.cu:235-242): sin^2 theta = u, azimuth 2 pi v, Duff et al.'s basis about unit n */
+static void cosine_dir(const float n[3], float u, float v, float out[3]) {
+  const float sg = copysignf(1.f, n[2]), a = -1.f / (sg + n[2]), b = n[0] * n[1] * a;
+  const float t[3] = {1.f + sg * n[0] * n[0] * a, sg * b, -sg * n[0]}, w[3] = {b, sg + n[1] * n[1] * a, -n[1]};
+  float sn, cs;
+  sincospi_f(2.f * v, &sn, &cs);
+  const float r = sqrtf(u), z = sqrtf(fmaxf(0.f, 1.f - u)), rc = r * cs, rs = r * sn;
+  for (int k = 0; k < 3; ++k) out[k] = (rc * t[k] + rs * w[k]) + z * n[k];
+}
+/* sky_bin (trace_dev.cu:246-258, reinhart.cal rbin): 0 below the horizon, else 1 + the Reinhart patch of the direction
+ * turned back to true north; angles in double */
+static uint32_t sky_bin(const float d[3], uint32_t mf, float alpha, uint32_t nBins, float cosN, float sinN) {
+  static const uint32_t tnaz[7] = {30, 30, 24, 24, 18, 12, 6};
+  if (!(d[2] >= 0.f)) return 0;
+  const double el = asin((double)fminf(d[2], 1.f));
+  const uint32_t r = (uint32_t)(el / (double)alpha);
+  if (r >= 7 * mf) return nBins - 1;
+  const uint32_t band = r / mf, n = tnaz[band] * mf;
+  uint32_t b = 1 + (r % mf) * n;
+  for (uint32_t q = 0; q < band; ++q) b += tnaz[q] * mf * mf;
+  double az = atan2((double)(d[0] * cosN + d[1] * sinN), (double)(d[1] * cosN - d[0] * sinN));
+  if (az < 0.) az += 2. * 3.14159265358979323846;
+  return b + (uint32_t)(az * ((double)n / (2. * 3.14159265358979323846)) + .5) % n;
+}
+
+typedef struct {
+  job_t j;
+  optics_t op;
+  uint32_t i0, M, bounces, mf, nBins;
+  float alpha, cosN, sinN;
+  float* dc;
+} bounce_job;
+
+/* __raygen__bounce (trace_dev.cu:271-311) for sensor i: its M paths, in sample order, into row i of dc [nBins x nc] */
+static void bounce_one(const bounce_job* b, long i) {
+  const job_t* j = &b->j;
+  const orc_scene* sc = j->s;
+  const float* n = j->N + 3 * i;
+  const uint32_t gi = b->i0 + (uint32_t)i;
+  float o0[3];
+  orc_offset_ray(j->P + 3 * i, n, o0);
+  uint32_t rot[3];
+  orc_pcg3d(gi, ~0u, ~0u, rot);
+  const float r0 = u01(rot[0]), r1 = u01(rot[1]), weight = 3.14159265f / (float)b->M;
+  for (uint32_t s = 0; s < b->M; ++s) {
+    const float u = (float)s / (float)b->M + r0, v = u01(brev32(s)) + r1;
+    float o[3], d[3];
+    memcpy(o, o0, 12);
+    cosine_dir(n, u - floorf(u), v - floorf(v), d);
+    float w = 1.f;
+    uint32_t diffuse = 0, k = 0;
+    int reflected = 0;
+    for (uint32_t seg = 0; seg < b->bounces + 16; ++seg) {
+      ray_t r;
+      ray_setup(&r, o, d, sc->pad);
+      float t;
+      long leaf;
+      int front;
+      if (sc->nt == 0 || !closest(sc, &r, 1e-5f, FLT_MAX, j->cull, &t, &leaf, &front)) {
+        if (reflected) b->dc[(size_t)sky_bin(d, b->mf, b->alpha, b->nBins, b->cosN, b->sinN) * j->nc + i] += w * weight;
+        break;
+      }
+      const float* V = sc->tri + 12 * leaf;
+      const float e1[3] = {V[3] - V[0], V[4] - V[1], V[5] - V[2]}, e2[3] = {V[6] - V[0], V[7] - V[1], V[8] - V[2]};
+      const float ng[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const float l2 = dot3(ng, ng);
+      if (!(l2 > 0.f)) break;
+      const float scl = front ? 1.f / sqrtf(l2) : -(1.f / sqrtf(l2));
+      const float hn[3] = {scl * ng[0], scl * ng[1], scl * ng[2]};
+      float bc[2] = {0.f, 0.f}, tt;
+      int fr;
+      tri_hit_bc(&r, V, V + 3, V + 6, t, t, j->cull, &tt, &fr, bc);
+      float p[3], lift[3], thr[3], nh[3];
+      const float ext = fmaxf(fmaxf(fmaxf(fabsf(e1[0]), fabsf(e1[1])), fmaxf(fabsf(e1[2]), fabsf(e2[0]))), fmaxf(fabsf(e2[1]), fabsf(e2[2])));
+      for (int a = 0; a < 3; ++a) {
+        p[a] = (V[a] + bc[0] * e1[a]) + bc[1] * e2[a];
+        lift[a] = (ext * 9.5367432e-7f) * hn[a];
+        nh[a] = -1.f * hn[a];
+      }
+      orc_offset_ray(p, nh, thr);
+      for (int a = 0; a < 3; ++a) thr[a] = thr[a] - lift[a];   /* through(h): past the triangle */
+      if (seg == 0 && own_plane(d, n, t, front, V, V + 3, V + 6)) { memcpy(o, thr, 12); continue; }
+      const float* sl = slot_of(&b->op, sc->prim[leaf]);
+      uint32_t fl;
+      memcpy(&fl, sl + 3, 4);
+      if ((fl & (2u | 8u)) == 2u && front == ((fl & 4u) != 0)) { memcpy(o, thr, 12); continue; }   /* a thin solid's exit face */
+      const int mirror = (fl & 1u) != 0;
+      float T = sl[1], R = sl[0];
+      if (mirror) {
+        orc_glass_TR((double)fabsf(dot3(d, hn)), (double)sl[2], &T, &R);
+        if (!(sl[1] > 0.f)) T = 0.f;
+      } else if (diffuse == b->bounces) R = 0.f;
+      if (!(T + R > 0.f)) break;
+      uint32_t rr[3];
+      orc_pcg3d(gi, s, k++, rr);
+      w *= T + R;
+      if (u01(rr[0]) * (T + R) < T) { memcpy(o, thr, 12); continue; }   /* transmitted */
+      reflected = 1;
+      if (mirror) {
+        const float k2 = 2.f * dot3(d, hn);
+        for (int a = 0; a < 3; ++a) d[a] = d[a] - k2 * hn[a];
+      } else {
+        cosine_dir(hn, u01(rr[1]), u01(rr[2]), d);
+        if (++diffuse >= 2) {   /* Russian roulette, survival = w */
+          uint32_t q[3];
+          orc_pcg3d(gi, s, k++, q);
+          if (u01(q[0]) >= w) break;
+          w = 1.f;
+        }
+      }
+      orc_offset_ray(p, hn, o);
+      for (int a = 0; a < 3; ++a) o[a] = o[a] + lift[a];
+    }
+  }
+}
+static void* bounce_worker(void* arg) {
+  bounce_job* b = arg;
+  for (;;) {
+    const long i0 = take(&b->j, 1);
+    if (i0 >= b->j.nc) break;
+    bounce_one(b, i0);
+  }
+  return NULL;
+}
+
+void orc_bounce(const orc_scene* s, const float* P, const float* N, long nc, unsigned i0, const uint32_t* triSlot, const float* slots, long nSlots,
+                unsigned bounces, unsigned M, int mf, double north, int cull, float* dc, int nthreads) {
+  bounce_job b;
+  memset(&b, 0, sizeof b);
+  b.j.s = s; b.j.P = P; b.j.N = N; b.j.nc = nc; b.j.cull = cull;
+  b.op.s = s; b.op.triSlot = triSlot; b.op.slots = slots; b.op.nSlots = nSlots;
+  b.i0 = i0; b.M = M; b.bounces = bounces; b.mf = (uint32_t)mf; b.nBins = (uint32_t)(144 * mf * mf + 2);
+  b.alpha = (float)((3.14159265358979323846 / 2.) / (7 * mf + .5));
+  const double rad = north * 3.14159265358979323846 / 180;
+  b.cosN = (float)cos(rad); b.sinN = (float)sin(rad);
+  b.dc = dc;
+  memset(dc, 0, sizeof(float) * (size_t)b.nBins * (size_t)nc);
+  run_workers(&b.j, bounce_worker, nthreads);
 }
 
 /* ---------------------------------------------------------------- views (closest hits) */
