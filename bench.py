@@ -324,6 +324,19 @@ REFERENCE_BENCHES = [
      "1.9 GRays/s, trace 2.5 s, wall 4.2 s (performance.md:9)"),
     ("daylight --bench 1000000 12000000 --sky-density 2", ["daylight", "--bench", "1000000", "12000000", "--sky-density", "2"] + BOSTON,
      "1.9 GRays/s, trace 3.0 s, wall 4.8 s (performance.md:10)"),
+    ("daylight --bench 1000000 12000000 --transmission on --optics bench_boxes/1=0.2,0.5,porous",
+     ["daylight", "--bench", "1000000", "12000000", "--transmission", "on", "--optics", "bench_boxes/1=0.2,0.5,porous"] + BOSTON,
+     "1.65 GRays/s, trace 2.9 s, wall 5.2 s (performance.md:15-19)"),
+    ("daylight --bench 100000 300000 --transmission on --optics bench_boxes=0.2,0.5,porous",
+     ["daylight", "--bench", "100000", "300000", "--transmission", "on", "--optics", "bench_boxes=0.2,0.5,porous"] + BOSTON,
+     "0.5 GRays/s, every box porous (performance.md:19-20)"),
+    ("daylight --bench 1000000 120000 --bounces 1", ["daylight", "--bench", "1000000", "120000", "--bounces", "1"] + BOSTON,
+     "reflected-light pass 2.1 s, 494 M paths/s, wall 5.6 s (performance.md:29)"),
+    ("daylight --bench 1000000 120000 --bounces 2", ["daylight", "--bench", "1000000", "120000", "--bounces", "2"] + BOSTON,
+     "reflected-light pass 2.2 s, 455 M paths/s, wall 5.5 s (performance.md:30)"),
+    ("daylight --bench 1000000 120000 --sky-density 4 --bounces 1",
+     ["daylight", "--bench", "1000000", "120000", "--sky-density", "4", "--bounces", "1"] + BOSTON,
+     "reflected-light pass 2.1 s, 485 M paths/s, wall 7.9 s (performance.md:31)"),
     ("views --bench 1000000 10000000", ["views", "--bench", "1000000", "10000000"],
      "1.98 GRays/s, trace 1.84 s, reduce 0.66 s, run 3.58-3.66 s (performance.md:105)"),
     ("views --bench 1000000 10000000 --fov 360", ["views", "--bench", "1000000", "10000000", "--fov", "360"],
@@ -354,6 +367,10 @@ def reference_benches():
         m = re.search(r"reduce ([0-9.]+) ms", err)
         if m:
             row["reduce_s"] = round(float(m.group(1)) / 1e3, 3)
+        m = re.search(r"([0-9.e+]+) paths in ([0-9.]+) ms", err)
+        if m:
+            row.update(paths=float(m.group(1)), bounce_s=round(float(m.group(2)) / 1e3, 3),
+                       paths_per_s=round(float(m.group(1)) / (float(m.group(2)) * 1e-3), 1))
         if p.returncode:
             row["stderr"] = err[-400:]
         print("bench: %s" % json.dumps(row), file=sys.stderr, flush=True)
