@@ -1,7 +1,7 @@
 // SPDX-License-Identifier: Apache-2.0
 // Reflected light on CDNA4: replaces __raygen__bounce (reference src/trace_dev.cu:260-311), gpu_bounce (gpu.cu:533-568),
 // gpu_reduce_dc (gpu.cu:627-637) and gpu_download_dc (gpu.cu:700-705).
-//   k_bounce     a thread per sensor of the chunk walks its samples [s0, s1) of M in order. Sample s leaves the sensor
+//   k_bounce     a wave per sensor of the chunk walks its M samples, 64 at a time, in order. Sample s leaves the sensor
 //                cosine-weighted (Hammersley point s, rotated by a hash of the global sensor index) and goes from
 //                closest hit to closest hit (closest.h): the sensor's own plane (first segment) and a thin solid's exit
 //                face pass it unchanged; elsewhere it is transmitted with probability T / (T + R) (porous tau, Radiance
@@ -9,7 +9,7 @@
 //                (these count toward the bounces; Russian roulette from the second on, survival = the weight). A path
 //                that reflected and escapes adds weight x pi / M to its bin (dl_math.h sky_bin) of the sensor's row.
 //                Random numbers pcg3d(global sensor, sample, event): rows do not depend on chunks or launches; a row is
-//                summed by its own thread in sample order (no atomics).
+//                summed in sample order (no atomics).
 //   k_reduce_dc  a lane per sensor: out[i] = sum over bins b (ascending) of double(dc[b nc + i]) x double(W[b]).
 // Rows dc[b nc + i] float, bin 0 the ground, 1 + p the Reinhart patches of the sky density (gendaymtx order).
 #include <algorithm>
@@ -36,7 +36,7 @@ struct BounceArgs {
   const float4* slots;   // rho, tau, tn, flags (bits)
   uint32_t slotLast;
   float* dc;
-  uint32_t nc, i0, s0, s1, M, bounces, mf, nBins, cull;
+  uint32_t nc, i0, M, bounces, mf, nBins, cull;
   float alpha, cosN, sinN;
 };
 
@@ -44,81 +44,109 @@ DL_HD float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y,
 DL_HD float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 DL_HD float3 mul3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
 
+// One path: sample s of global sensor gi (normal n, origin o0), the rotation (r0, r1) of its sample pattern. True when
+// it reflected and escaped, with its bin and weight.
 template <int STACK>
-__global__ __launch_bounds__(kWave) void k_bounce(BounceArgs a) {
+__device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3 o0, uint32_t gi, uint32_t s, float r0, float r1,
+                                           uint32_t* __restrict__ stk, int lane, uint32_t& bin, float& val) {
+  const bool cull = a.cull != 0;
+  const float u = float(s) / float(a.M) + r0, v = u01(brev32(s)) + r1;
+  float3 o = o0, d = cosine_dir(n, u - floorf(u), v - floorf(v));
+  float w = 1.f;
+  uint32_t diffuse = 0, k = 0;   // diffuse reflections, random draws (events)
+  bool reflected = false;
+  for (uint32_t seg = 0; seg < a.bounces + 16; ++seg) {
+    Ray ray;
+    ray_setup(ray, o, d, a.pad);
+    Closest c{};
+    if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, 1e-5f, 3.40282347e38f, cull, stk, lane);
+    if (!c.hit) {
+      if (!reflected) return false;
+      bin = sky_bin(d, a.mf, a.alpha, a.nBins, a.cosN, a.sinN);
+      val = w * (3.14159265f / float(a.M));
+      return true;
+    }
+    // the hit triangle (trace_dev.cu:43-56): its point by barycentrics, unit normal on the side the ray came from,
+    // extent, and the lift new origins leave it by
+    const float4 A = a.leaves[3 * c.leaf], B = a.leaves[3 * c.leaf + 1], E = a.leaves[3 * c.leaf + 2];
+    const float3 v0 = make_float3(A.x, A.y, A.z), v1 = make_float3(A.w, B.x, B.y), v2 = make_float3(B.z, B.w, E.x);
+    const float3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
+    const float3 ng = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+    const float l2 = dot3(ng, ng);
+    if (!(l2 > 0.f)) return false;
+    const float sc = c.front ? 1.f / sqrtf(l2) : -(1.f / sqrtf(l2));
+    const float3 hn = mul3(sc, ng);
+    float bc[2] = {0.f, 0.f}, tt;
+    bool fr;
+    tri_hit(ray, v0, v1, v2, c.t, c.t, cull, tt, fr, bc);   // the closest hit again, for its barycentrics
+    const float3 p = add3(add3(v0, mul3(bc[0], e1)), mul3(bc[1], e2));
+    const float ext = fmaxf(fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)), fmaxf(fabsf(e1.z), fabsf(e2.x))), fmaxf(fabsf(e2.y), fabsf(e2.z)));
+    const float3 lift = mul3(ext * 9.5367432e-7f, hn);
+    const float3 through = sub3(offset_ray(p, mul3(-1.f, hn)), lift);   // the origin past the triangle
+    if (seg == 0 && own_plane(d, n, c.t, c.front, v0, v1, v2)) { o = through; continue; }
+    const uint32_t si = __float_as_uint(E.z);
+    const float4 sl = a.slots[si < a.slotLast ? si : a.slotLast];
+    const uint32_t fl = __float_as_uint(sl.w);
+    if ((fl & (kSlotThin | kSlotTwin)) == kSlotThin && c.front == ((fl & kSlotInward) != 0)) { o = through; continue; }   // exit face
+    const bool mirror = (fl & kSlotGlass) != 0;
+    float T = sl.y, R = sl.x;   // opaque: tau 0 (every slot with --transmission off)
+    if (mirror) {
+      glass_TR(double(fabsf(dot3(d, hn))), double(sl.z), T, R);
+      if (!(sl.y > 0.f)) T = 0.f;   // glass with --transmission off mirrors R and absorbs T
+    } else if (diffuse == a.bounces) R = 0.f;   // no diffuse reflection left: transmitted or absorbed
+    if (!(T + R > 0.f)) return false;
+    const uint3 rr = pcg3d(gi, s, k++);
+    w *= T + R;
+    if (u01(rr.x) * (T + R) < T) { o = through; continue; }   // transmitted
+    reflected = true;
+    if (mirror) d = sub3(d, mul3(2.f * dot3(d, hn), hn));
+    else {
+      d = cosine_dir(hn, u01(rr.y), u01(rr.z));
+      if (++diffuse >= 2) {   // Russian roulette, survival = w (<= 1)
+        if (u01(pcg3d(gi, s, k++).x) >= w) return false;
+        w = 1.f;
+      }
+    }
+    o = add3(offset_ray(p, hn), lift);
+  }
+  return false;
+}
+
+// A wave per sensor of [c0, c0 + gridDim.x): lane l traces samples l, l + 64, ... (rounds of 64 from one origin, so the
+// top of the tree is shared); each round's escaping paths add to the sensor's row in LDS in lane order, i.e. in sample
+// order, which keeps every bin's float sum that of one thread walking the samples in turn; the row is written at the end.
+template <int STACK>
+__global__ __launch_bounds__(kWave) void k_bounce(BounceArgs a, uint32_t c0) {
   __shared__ uint32_t stk[STACK * kWave];
-  const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane;
-  if (i >= a.nc) return;   // no barriers below: each lane walks its own stack
+  extern __shared__ float sRow[];   // [nBins]
+  const uint32_t lane = threadIdx.x, i = c0 + blockIdx.x;
+  if (i >= a.nc) return;   // whole blocks
+  for (uint32_t b = lane; b < a.nBins; b += kWave) sRow[b] = 0.f;
+  __syncthreads();
   const uint32_t gi = a.i0 + i;
   const float3 n = a.N[i], o0 = offset_ray(a.P[i], n);
   const uint3 rot = pcg3d(gi, ~0u, ~0u);
-  const float r0 = u01(rot.x), r1 = u01(rot.y), weight = 3.14159265f / float(a.M);
-  float* row = a.dc + i;
-  const bool cull = a.cull != 0;
-  for (uint32_t s = a.s0; s < a.s1; ++s) {
-    const float u = float(s) / float(a.M) + r0, v = u01(brev32(s)) + r1;
-    float3 o = o0, d = cosine_dir(n, u - floorf(u), v - floorf(v));
-    float w = 1.f;
-    uint32_t diffuse = 0, k = 0;   // diffuse reflections, random draws (events)
-    bool reflected = false;
-    for (uint32_t seg = 0; seg < a.bounces + 16; ++seg) {
-      Ray ray;
-      ray_setup(ray, o, d, a.pad);
-      Closest c{};
-      if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, 1e-5f, 3.40282347e38f, cull, stk, int(lane));
-      if (!c.hit) {
-        if (reflected) row[size_t(sky_bin(d, a.mf, a.alpha, a.nBins, a.cosN, a.sinN)) * a.nc] += w * weight;
-        break;
-      }
-      // the hit triangle (trace_dev.cu:43-56): its point by barycentrics, unit normal on the side the ray came from,
-      // extent, and the lift new origins leave it by
-      const float4 A = a.leaves[3 * c.leaf], B = a.leaves[3 * c.leaf + 1], E = a.leaves[3 * c.leaf + 2];
-      const float3 v0 = make_float3(A.x, A.y, A.z), v1 = make_float3(A.w, B.x, B.y), v2 = make_float3(B.z, B.w, E.x);
-      const float3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
-      const float3 ng = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
-      const float l2 = dot3(ng, ng);
-      if (!(l2 > 0.f)) break;
-      const float sc = c.front ? 1.f / sqrtf(l2) : -(1.f / sqrtf(l2));
-      const float3 hn = mul3(sc, ng);
-      float bc[2] = {0.f, 0.f}, tt;
-      bool fr;
-      tri_hit(ray, v0, v1, v2, c.t, c.t, cull, tt, fr, bc);   // the closest hit again, for its barycentrics
-      const float3 p = add3(add3(v0, mul3(bc[0], e1)), mul3(bc[1], e2));
-      const float ext = fmaxf(fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)), fmaxf(fabsf(e1.z), fabsf(e2.x))), fmaxf(fabsf(e2.y), fabsf(e2.z)));
-      const float3 lift = mul3(ext * 9.5367432e-7f, hn);
-      const float3 through = sub3(offset_ray(p, mul3(-1.f, hn)), lift);   // the origin past the triangle
-      if (seg == 0 && own_plane(d, n, c.t, c.front, v0, v1, v2)) { o = through; continue; }
-      const uint32_t si = __float_as_uint(E.z);
-      const float4 sl = a.slots[si < a.slotLast ? si : a.slotLast];
-      const uint32_t fl = __float_as_uint(sl.w);
-      if ((fl & (kSlotThin | kSlotTwin)) == kSlotThin && c.front == ((fl & kSlotInward) != 0)) { o = through; continue; }   // exit face
-      const bool mirror = (fl & kSlotGlass) != 0;
-      float T = sl.y, R = sl.x;   // opaque: tau 0 (every slot with --transmission off)
-      if (mirror) {
-        glass_TR(double(fabsf(dot3(d, hn))), double(sl.z), T, R);
-        if (!(sl.y > 0.f)) T = 0.f;   // glass with --transmission off mirrors R and absorbs T
-      } else if (diffuse == a.bounces) R = 0.f;   // no diffuse reflection left: transmitted or absorbed
-      if (!(T + R > 0.f)) break;
-      const uint3 rr = pcg3d(gi, s, k++);
-      w *= T + R;
-      if (u01(rr.x) * (T + R) < T) { o = through; continue; }   // transmitted
-      reflected = true;
-      if (mirror) d = sub3(d, mul3(2.f * dot3(d, hn), hn));
-      else {
-        d = cosine_dir(hn, u01(rr.y), u01(rr.z));
-        if (++diffuse >= 2) {   // Russian roulette, survival = w (<= 1)
-          if (u01(pcg3d(gi, s, k++).x) >= w) break;
-          w = 1.f;
-        }
-      }
-      o = add3(offset_ray(p, hn), lift);
+  const float r0 = u01(rot.x), r1 = u01(rot.y);
+  for (uint32_t s0 = 0; s0 < a.M; s0 += kWave) {
+    const uint32_t s = s0 + lane;
+    uint32_t bin = 0;
+    float val = 0.f;
+    const bool has = s < a.M && trace_path<STACK>(a, n, o0, gi, s, r0, r1, stk, int(lane), bin, val);
+    // in lane order: one lane at a time adds its value (a bin's sum is the sequential one)
+    for (uint64_t m = __ballot(has); m; m &= m - 1) {
+      const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);
+      if (lane == l) sRow[bin] += val;
+      __syncthreads();   // one wave per block: orders the LDS updates
     }
   }
+  __syncthreads();
+  float* row = a.dc + i;
+  for (uint32_t b = lane; b < a.nBins; b += kWave) row[size_t(b) * a.nc] = sRow[b];
 }
 
 template <int STACK>
-void launch_bounce(dlg_gpu* g, const BounceArgs& a) {
-  k_bounce<STACK><<<(a.nc + kWave - 1) / kWave, kWave, 0, g->stream>>>(a);
+void launch_bounce(dlg_gpu* g, const BounceArgs& a, uint32_t c0, uint32_t n) {
+  k_bounce<STACK><<<n, kWave, a.nBins * 4, g->stream>>>(a, c0);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_dc(const float* __restrict__ dc, uint32_t nc, uint32_t nBins, const float* __restrict__ W,
@@ -166,24 +194,22 @@ int32_t dlg_bounce(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const 
   a.nBins = nBins; a.cull = (rayFlags & 16u) ? 1u : 0u;
   a.alpha = float((3.14159265358979323846 / 2.) / (7 * b->mf + .5));   // the patch rows' height (the sky's Patches::alpha)
   a.cosN = float(std::cos(rad)); a.sinN = float(std::sin(rad));
-  // the samples in launches of at most 64 per sensor (a launch's runtime bounded; rows keep the sample order)
-  for (uint32_t done = 0; done < b->M;) {
-    const uint32_t m = std::min<uint32_t>(64, b->M - done);
-    a.s0 = done;
-    a.s1 = done + m;
+  // sensors in launches of about 2^24 paths (a launch's runtime bounded; each sensor's row is one block's)
+  const uint32_t per = std::max<uint32_t>(1, (1u << 24) / std::max<uint32_t>(1, b->M));
+  for (uint32_t c0 = 0; c0 < uint32_t(nc); c0 += per) {
+    const uint32_t m = std::min<uint32_t>(per, uint32_t(nc) - c0);
     dlg_tbegin(g);
-    if (need <= 24) launch_bounce<24>(g, a);
-    else if (need <= 32) launch_bounce<32>(g, a);
-    else if (need <= 48) launch_bounce<48>(g, a);
-    else if (need <= 64) launch_bounce<64>(g, a);
-    else if (need <= 96) launch_bounce<96>(g, a);
-    else launch_bounce<160>(g, a);
+    if (need <= 24) launch_bounce<24>(g, a, c0, m);
+    else if (need <= 32) launch_bounce<32>(g, a, c0, m);
+    else if (need <= 48) launch_bounce<48>(g, a, c0, m);
+    else if (need <= 64) launch_bounce<64>(g, a, c0, m);
+    else if (need <= 96) launch_bounce<96>(g, a, c0, m);
+    else launch_bounce<160>(g, a, c0, m);
     DLG_CHECK(hipGetLastError());
     const float ms = dlg_tend(g);
     g->stats.bounceMs += ms;
-    g->stats.paths += int64_t(m) * nc;
+    g->stats.paths += int64_t(m) * b->M;
     g->stats.bounceLaunches += 1;
-    done += m;
   }
   return 0;
 }
