@@ -205,41 +205,48 @@ size_t slot_count(const SimulationMesh& m) { return m.slot.empty() ? 1 : size_t(
 
 uint32_t thin_solid(const SimulationMesh& m, uint32_t s, const double c[3]) {
   const V3 C{c[0], c[1], c[2]};
-  // the corners of the slot's faces welded to a 1e-5 m grid (ids in order of first appearance), the faces' directed
-  // edges between the welded ids (degenerate ones dropped), the area and the signed volume (sum N . p0 / 6 over planar
-  // faces, |N| = twice the area)
-  std::unordered_map<uint64_t, std::vector<std::pair<std::array<long long, 3>, uint32_t>>> grid;
-  uint32_t nIds = 0;
-  const auto weld = [&](V3 p) {
-    const std::array<long long, 3> k{std::llround(p.x * 1e5), std::llround(p.y * 1e5), std::llround(p.z * 1e5)};
-    const uint64_t h = uint64_t(k[0]) * 0x9E3779B97F4A7C15ull ^ uint64_t(k[1]) * 0xC2B2AE3D27D4EB4Full ^ uint64_t(k[2]) * 0x165667B19E3779F9ull;
-    auto& bucket = grid[h];
-    for (const auto& e : bucket) if (e.first == k) return e.second;
-    bucket.emplace_back(k, nIds);
-    return nIds++;
-  };
-  std::vector<uint64_t> fwd, rev;
-  std::vector<uint32_t> ids;
+  // the corners of the slot's faces welded to a 1e-5 m grid, face after face; the area and the signed volume (sum N . p0
+  // / 6 over planar faces, |N| = twice the area)
+  using Key = std::array<long long, 3>;
+  std::vector<Key> keys;
+  std::vector<int> n;
   double area = 0, vol = 0;
   size_t off = 0;
   for (size_t f = 0; f < m.counts.size(); ++f) {
-    const int n = m.counts[f];
+    const int k = m.counts[f];
     const int* idx = m.indices.data() + off;
-    off += size_t(n);
+    off += size_t(k);
     if ((m.slot.empty() ? 0u : uint32_t(m.slot[f])) != s) continue;
-    const V3 N = newell(m, idx, n);
+    const V3 N = newell(m, idx, k);
     area += len(N) / 2;
     vol += dot(N, pt(m, idx[0]) - C) / 6;
-    ids.resize(size_t(n));
-    for (int j = 0; j < n; ++j) ids[size_t(j)] = weld(pt(m, idx[j]) - C);
-    for (int j = 0; j < n; ++j) {
-      const uint64_t a = ids[size_t(j)], b = ids[size_t((j + 1) % n)];
+    for (int j = 0; j < k; ++j) {
+      const V3 p = pt(m, idx[j]) - C;
+      keys.push_back({std::llround(p.x * 1e5), std::llround(p.y * 1e5), std::llround(p.z * 1e5)});
+    }
+    n.push_back(k);
+  }
+  if (!(area > 0) || 2 * std::fabs(vol) / area >= 0.1) return 0;
+  // ids of the welded positions: the corners sorted by key, equal keys one id
+  std::vector<uint32_t> order(keys.size()), id(keys.size());
+  for (uint32_t j = 0; j < uint32_t(order.size()); ++j) order[j] = j;
+  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return keys[x] < keys[y]; });
+  for (size_t j = 0, cur = 0; j < order.size(); ++j) {
+    if (j > 0 && keys[order[j]] != keys[order[j - 1]]) ++cur;
+    id[order[j]] = uint32_t(cur);
+  }
+  // the faces' directed edges between welded ids (degenerate ones dropped) and the same reversed: equal multisets when
+  // the slot is closed and consistently wound
+  std::vector<uint64_t> fwd, rev;
+  fwd.reserve(keys.size());
+  rev.reserve(keys.size());
+  for (size_t f = 0, j0 = 0; f < n.size(); j0 += size_t(n[f++]))
+    for (int j = 0; j < n[f]; ++j) {
+      const uint64_t a = id[j0 + size_t(j)], b = id[j0 + size_t((j + 1) % n[f])];
       if (a == b) continue;
       fwd.push_back(a << 32 | b);
       rev.push_back(b << 32 | a);
     }
-  }
-  if (!(area > 0) || 2 * std::fabs(vol) / area >= 0.1) return 0;
   std::sort(fwd.begin(), fwd.end());
   std::sort(rev.begin(), rev.end());
   if (fwd != rev) return 0;   // open, or wound against itself
