@@ -45,6 +45,28 @@ void dlg_set_tree(dlg_gpu* g, int32_t tree);
 int64_t dlg_build_bytes(int64_t nv, int64_t nt);
 /* gpu_trim (internal.h:138): frees the per-run buffers (with sensors: the sensors too). */
 void dlg_trim(dlg_gpu* g, int32_t sensors);
+/* Mesh ingest on the GPU: the scene's triangles made where they are built (the reference triangulates on the host,
+ * run.cpp:120-147 over mesh.cpp:137-168 triangulate_into, one mesh per host thread). dlg_mesh_begin stages room for
+ * nPts vertices and up to nTriCap triangles (sum of n - 2 over the faces), with per-triangle global slots when slotted;
+ * dlg_mesh_add appends one mesh: its points xyz[3 nPts] (double) recentred on centre and rounded to float, its faces
+ * (counts[nFaces], corner indices idx[nIdx], mesh-local) fanned in face order — a triangle (a, b, c), a quad (a, b, c)
+ * (a, c, d) — at the running triangle count, with the face's slot slotBase + faceSlot[f] (faceSlot may be null: 0).
+ * Faces of 5 or more corners come triangulated by the host (the ear clipping of triangulate_into): ngonFace[nNgon]
+ * their face indices ascending, ngonMade[nNgon] the triangles each made (0: skipped), ngonTri[3 sum(made)] the corners
+ * (mesh-local) in face order. *made = the mesh's triangles. dlg_mesh_build builds the staged triangles as dlg_build
+ * would the same arrays from the host (and the per-triangle slots as dlg_set_tri_slots), then frees the staging. The
+ * points and faces must pass the host's mesh checks (counts >= 3 summing to nIdx, indices in range): a violation found
+ * on the device fails the call and nothing out of range is read. */
+int32_t dlg_mesh_begin(dlg_gpu* g, int64_t nPts, int64_t nTriCap, int32_t slotted, const double* centre, char* err, int32_t errLen);
+int32_t dlg_mesh_add(dlg_gpu* g, const double* xyz, int64_t nPts, const int32_t* counts, int64_t nFaces, const int32_t* idx, int64_t nIdx,
+                     const uint16_t* faceSlot, uint32_t slotBase, const int64_t* ngonFace, const int32_t* ngonMade, const uint32_t* ngonTri,
+                     int64_t nNgon, int64_t* made, char* err, int32_t errLen);
+int32_t dlg_mesh_build(dlg_gpu* g, char* err, int32_t errLen);
+/* The staged arrays before dlg_mesh_build (for the check against the host triangulation): v[3 nv], tri[3 nt] (global
+ * vertex ids), slot[nt] when slotted (may be null). Sizes in *nv / *nt when v and tri are null. */
+int32_t dlg_mesh_staged(dlg_gpu* g, float* v, int64_t* nv, uint32_t* tri, int64_t* nt, uint32_t* slot, char* err, int32_t errLen);
+/* device bytes one dlg_mesh_add of a mesh this large peaks at, beside the staging (for the scene's admission) */
+int64_t dlg_mesh_bytes(int64_t nPts, int64_t nFaces, int64_t nIdx);
 /* gpu_detach / gpu_attach / gpu_free (internal.h:141-144): a built scene moved out of g and kept resident. */
 dlg_scene* dlg_detach(dlg_gpu* g);
 void dlg_attach(dlg_gpu* g, dlg_scene* s);

@@ -28,6 +28,20 @@ __global__ void k_leaf_slots(float* __restrict__ leaves, int64_t nt, const uint3
 
 }  // namespace
 
+int32_t dlg_leaf_slots_device(dlg_gpu* g, const uint32_t* slot, int64_t nt, uint32_t mx, char* err, int32_t errLen) {
+  dlg_scene& s = g->scene;
+  if (nt != s.nTris) { dlg_fail(err, errLen, "dlg_set_tri_slots: " + std::to_string(nt) + " slots for a scene of " + std::to_string(s.nTris) + " triangles"); return 1; }
+  hipError_t e = hipSuccess;
+  if (nt > 0) {
+    k_leaf_slots<<<unsigned((nt + 255) / 256), 256, 0, g->stream>>>(s.leaves.as<float>(), nt, slot);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  }
+  if (e != hipSuccess) { dlg_fail(err, errLen, std::string("dlg_set_tri_slots: ") + hipGetErrorString(e)); return 1; }
+  s.slotMax = mx;
+  return 0;
+}
+
 extern "C" {
 
 int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
@@ -51,7 +65,8 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
   for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->slots, &g->dc, &g->skySdc})
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->slots, &g->dc, &g->skySdc,
+                    &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -70,7 +85,9 @@ int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmB
 }
 
 void dlg_trim(dlg_gpu* g, int32_t sensors) {
-  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->view, &g->dc}) b->release();
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->view, &g->dc, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
+  g->meshOpen = 0;
+  g->meshNv = g->meshNt = 0;
   if (sensors) {
     g->P.release(); g->N.release(); g->labels.release();
     g->nSens = 0;
@@ -144,15 +161,10 @@ int32_t dlg_set_tri_slots(dlg_gpu* g, const uint32_t* slot, int64_t nt, char* er
   DevBuf d;
   hipError_t e = d.grow(size_t(nt) * 4);
   if (e == hipSuccess) e = hipMemcpyAsync(d.p, slot, size_t(nt) * 4, hipMemcpyHostToDevice, g->stream);
-  if (e == hipSuccess) {
-    k_leaf_slots<<<unsigned((nt + 255) / 256), 256, 0, g->stream>>>(s.leaves.as<float>(), nt, d.as<uint32_t>());
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) { d.release(); dlg_fail(err, errLen, std::string("dlg_set_tri_slots: ") + hipGetErrorString(e)); return 1; }
+  const int32_t rc = dlg_leaf_slots_device(g, d.as<uint32_t>(), nt, mx, err, errLen);
   d.release();
-  if (e != hipSuccess) { dlg_fail(err, errLen, std::string("dlg_set_tri_slots: ") + hipGetErrorString(e)); return 1; }
-  s.slotMax = mx;
-  return 0;
+  return rc;
 }
 
 int32_t dlg_download_bytes(dlg_gpu* g, int64_t nc, uint8_t* out, char* err, int32_t errLen) {

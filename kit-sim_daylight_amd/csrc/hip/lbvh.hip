@@ -213,20 +213,19 @@ extern "C" int64_t dlg_build_bytes(int64_t nv, int64_t nt) {
   return in + keys + cen + tree + out + dlg_sah_bytes(nt) + (int64_t(64) << 20);
 }
 
-extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint32_t* I, int64_t nt, char* err, int32_t errLen) {
+int32_t dlg_build_device(dlg_gpu* g, const float* dV, int64_t nv, const uint32_t* dI, int64_t nt, float mx, char* err, int32_t errLen) {
+  (void)nv;
   dlg_scene& s = g->scene;
   s.release();
   s.nNodes = s.nTris = 0;
   s.height = 0;
   s.slotMax = 0;
-  float mx = 0.f;
-  for (int64_t i = 0; i < 3 * nv; ++i) mx = fmaxf(mx, fabsf(V[i]));
   s.pad = (mx + 1.f) * 3.81469727e-6f;   // 2^-18 (|coordinate| + 1): box slack beyond the triangle test's rounding
   if (nt <= 0) return 0;
   dlg_tbegin(g);
-  DevBuf dV, dI, cen, box, keys, keys2, idx, idx2, tmp, L, R, pl, pi, ibox, height, flags;
+  DevBuf cen, box, keys, keys2, idx, idx2, tmp, L, R, pl, pi, ibox, height, flags;
   const int64_t nn = nt > 1 ? nt - 1 : 1;
-  const auto cleanup = [&] { for (DevBuf* b : {&dV, &dI, &cen, &box, &keys, &keys2, &idx, &idx2, &tmp, &L, &R, &pl, &pi, &ibox, &height, &flags}) b->release(); };
+  const auto cleanup = [&] { for (DevBuf* b : {&cen, &box, &keys, &keys2, &idx, &idx2, &tmp, &L, &R, &pl, &pi, &ibox, &height, &flags}) b->release(); };
   const auto fail = [&](hipError_t e, const char* what) {
     dlg_fail(err, errLen, std::string("dlg_build: ") + what + ": " + hipGetErrorString(e));
     cleanup();
@@ -234,15 +233,11 @@ extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint3
   };
   hipError_t e;
 #define TRY(x, what) if ((e = (x)) != hipSuccess) return fail(e, what)
-  TRY(dV.grow(size_t(nv) * 12), "vertices");
-  TRY(dI.grow(size_t(nt) * 12), "indices");
-  TRY(hipMemcpyAsync(dV.p, V, size_t(nv) * 12, hipMemcpyHostToDevice, g->stream), "upload");
-  TRY(hipMemcpyAsync(dI.p, I, size_t(nt) * 12, hipMemcpyHostToDevice, g->stream), "upload");
   TRY(cen.grow(size_t(nt) * 12), "centroids");
   TRY(box.grow(24), "box");
   const uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
   TRY(hipMemcpyAsync(box.p, init, 24, hipMemcpyHostToDevice, g->stream), "box init");
-  k_centroids<<<blocks(nt, 256), 256, 0, g->stream>>>(dV.as<float>(), dI.as<uint32_t>(), nt, cen.as<float>(), box.as<uint32_t>());
+  k_centroids<<<blocks(nt, 256), 256, 0, g->stream>>>(dV, dI, nt, cen.as<float>(), box.as<uint32_t>());
   TRY(keys.grow(size_t(nt) * 8), "keys"); TRY(keys2.grow(size_t(nt) * 8), "keys");
   TRY(idx.grow(size_t(nt) * 4), "index"); TRY(idx2.grow(size_t(nt) * 4), "index");
   k_morton<<<blocks(nt, 256), 256, 0, g->stream>>>(cen.as<float>(), nt, box.as<uint32_t>(), keys.as<uint64_t>(), idx.as<uint32_t>());
@@ -253,7 +248,7 @@ extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint3
   TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), idx.as<uint32_t>(), idx2.as<uint32_t>(),
                                          int(nt), 0, 64, g->stream), "sort");
   TRY(s.leaves.grow(size_t(nt) * 48), "leaves");
-  k_gather<<<blocks(nt, 256), 256, 0, g->stream>>>(dV.as<float>(), dI.as<uint32_t>(), idx2.as<uint32_t>(), nt, s.leaves.as<float>());
+  k_gather<<<blocks(nt, 256), 256, 0, g->stream>>>(dV, dI, idx2.as<uint32_t>(), nt, s.leaves.as<float>());
   if (g->tree == DLG_TREE_SAH && nt >= 2) {
     // the binned-SAH tree over the Morton order (sah.hip); a tree deeper than the traversal stacks take (status 2,
     // the leaves untouched) falls through to the LBVH
@@ -306,4 +301,26 @@ extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint3
   g->stats.buildMs += dlg_tend(g);
   cleanup();
   return 0;
+}
+
+extern "C" int32_t dlg_build(dlg_gpu* g, const float* V, int64_t nv, const uint32_t* I, int64_t nt, char* err, int32_t errLen) {
+  float mx = 0.f;
+  for (int64_t i = 0; i < 3 * nv; ++i) mx = fmaxf(mx, fabsf(V[i]));
+  if (nt <= 0) return dlg_build_device(g, nullptr, nv, nullptr, 0, mx, err, errLen);
+  g->scene.release();   // the old scene's room first: the upload and the build share it
+  DevBuf dV, dI;
+  hipError_t e = dV.grow(size_t(nv) * 12);
+  if (e == hipSuccess) e = dI.grow(size_t(nt) * 12);
+  if (e == hipSuccess) e = hipMemcpyAsync(dV.p, V, size_t(nv) * 12, hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dI.p, I, size_t(nt) * 12, hipMemcpyHostToDevice, g->stream);
+  if (e != hipSuccess) {
+    dlg_fail(err, errLen, std::string("dlg_build: upload: ") + hipGetErrorString(e));
+    g->scene.release();
+    g->scene.nNodes = g->scene.nTris = 0;
+    return 1;
+  }
+  const int32_t rc = dlg_build_device(g, dV.as<float>(), nv, dI.as<uint32_t>(), nt, mx, err, errLen);
+  dV.release();
+  dI.release();
+  return rc;
 }

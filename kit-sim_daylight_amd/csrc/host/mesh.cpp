@@ -163,6 +163,27 @@ size_t tri_cap(const SimulationMesh& m) {
   return n;
 }
 
+// A face of n >= 5 corners: its Newell plane, the corners projected on it and ear-clipped; out[3 k ..] the corner
+// indices of its triangles (at most n - 2). -1: degenerate (no normal, or no ear found), the face is skipped.
+static int ngon_into(const SimulationMesh& m, const int* idx, int n, std::vector<std::array<double, 2>>& poly, std::vector<int>& ear, int* out) {
+  V3 N = newell(m, idx, n);
+  if (len(N) == 0) return -1;
+  N = unit(N);
+  const V3 ax = std::fabs(N.x) <= std::fabs(N.y) && std::fabs(N.x) <= std::fabs(N.z) ? V3{1, 0, 0}
+                : std::fabs(N.y) <= std::fabs(N.z)                                     ? V3{0, 1, 0}
+                                                                                         : V3{0, 0, 1};
+  const V3 u = unit(cross(N, ax)), v = cross(N, u);   // (u, v, N) right-handed: the winding is counterclockwise in (u, v)
+  poly.clear();
+  for (int k = 0; k < n; ++k) { const V3 p = pt(m, idx[k]); poly.push_back({dot(p, u), dot(p, v)}); }
+  ear.clear();
+  ear_clip(poly, ear);
+  if (ear.empty()) return -1;
+  int made = 0;
+  for (size_t k = 0; k + 2 < ear.size() && k / 3 < size_t(n - 2); k += 3, ++made)
+    for (int q = 0; q < 3; ++q) out[3 * made + q] = idx[ear[k + size_t(q)]];
+  return made;
+}
+
 size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* vOut, uint3* tri, uint32_t* slots, unsigned base, uint32_t slotBase,
                         int& skipped) {
   const V3 C{c[0], c[1], c[2]};
@@ -173,7 +194,7 @@ size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* vOut
     if (slots) slots[made] = slot;
     tri[made++] = make_uint3(base + unsigned(a), base + unsigned(b), base + unsigned(d));
   };
-  std::vector<int> ear;
+  std::vector<int> ear, out;
   std::vector<std::array<double, 2>> poly;
   for (size_t f = 0; f < m.counts.size(); ++f) {
     const int n = m.counts[f];
@@ -183,22 +204,34 @@ size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* vOut
     if (n == 3) emit(idx[0], idx[1], idx[2]);
     else if (n == 4) { emit(idx[0], idx[1], idx[2]); emit(idx[0], idx[2], idx[3]); }
     else if (n >= 5) {
-      V3 N = newell(m, idx, n);
-      if (len(N) == 0) { ++skipped; continue; }
-      N = unit(N);
-      const V3 ax = std::fabs(N.x) <= std::fabs(N.y) && std::fabs(N.x) <= std::fabs(N.z) ? V3{1, 0, 0}
-                    : std::fabs(N.y) <= std::fabs(N.z)                                     ? V3{0, 1, 0}
-                                                                                             : V3{0, 0, 1};
-      const V3 u = unit(cross(N, ax)), v = cross(N, u);   // (u, v, N) right-handed: the winding is counterclockwise in (u, v)
-      poly.clear();
-      for (int k = 0; k < n; ++k) { const V3 p = pt(m, idx[k]); poly.push_back({dot(p, u), dot(p, v)}); }
-      ear.clear();
-      ear_clip(poly, ear);
-      if (ear.empty()) { ++skipped; continue; }
-      for (size_t k = 0; k + 2 < ear.size() && k / 3 < size_t(n - 2); k += 3) emit(idx[ear[k]], idx[ear[k + 1]], idx[ear[k + 2]]);
+      out.resize(3 * size_t(n - 2));
+      const int k = ngon_into(m, idx, n, poly, ear, out.data());
+      if (k < 0) { ++skipped; continue; }
+      for (int j = 0; j < k; ++j) emit(out[3 * size_t(j)], out[3 * size_t(j) + 1], out[3 * size_t(j) + 2]);
     } else ++skipped;
   }
   return made;
+}
+
+void ngon_faces(const SimulationMesh& m, std::vector<int64_t>& face, std::vector<int32_t>& made, std::vector<uint32_t>& tri, int& skipped) {
+  face.clear();
+  made.clear();
+  tri.clear();
+  std::vector<int> ear, out;
+  std::vector<std::array<double, 2>> poly;
+  size_t off = 0;
+  for (size_t f = 0; f < m.counts.size(); ++f) {
+    const int n = m.counts[f];
+    const int* idx = m.indices.data() + off;
+    off += size_t(n);
+    if (n < 5) { if (n < 3) ++skipped; continue; }
+    out.resize(3 * size_t(n - 2));
+    int k = ngon_into(m, idx, n, poly, ear, out.data());
+    if (k < 0) { ++skipped; k = 0; }
+    face.push_back(int64_t(f));
+    made.push_back(k);
+    for (int j = 0; j < 3 * k; ++j) tri.push_back(uint32_t(out[size_t(j)]));
+  }
 }
 
 size_t slot_count(const SimulationMesh& m) { return m.slot.empty() ? 1 : size_t(*std::max_element(m.slot.begin(), m.slot.end())) + 1; }
@@ -261,6 +294,7 @@ bool mesh_ok(const SimulationMesh& m, const std::string& label, std::string& err
     sum += uint64_t(c);
   }
   if (sum != m.indices.size()) { err = label + ": sum(counts) != n_indices"; return false; }
+  if (!m.slot.empty() && m.slot.size() != m.counts.size()) { err = label + ": one optics slot per face needed"; return false; }
   const uint64_t n = m.numPoints();
   for (const int i : m.indices) if (i < 0 || uint64_t(i) >= n) { err = label + ": vertex index out of range"; return false; }
   return true;

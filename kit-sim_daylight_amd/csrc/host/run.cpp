@@ -117,49 +117,54 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
   const size_t nPts = vBase.back(), nTri = tBase.back();
   stage = "making room on the GPU";
   dlg_trim(g, 1);
-  const size_t need = size_t(dlg_build_bytes(int64_t(nPts), int64_t(nTri))) + kVramMargin;
+  // the peak: the staged triangles beside one mesh's ingest temporaries, or beside the tree's build
+  int64_t work = 0;
+  for (size_t k = 0; k < meshes.size(); ++k)
+    if (blocks(k)) {
+      const SimulationMesh& m = *meshes[k].mesh;
+      work = std::max(work, dlg_mesh_bytes(int64_t(m.numPoints()), int64_t(m.counts.size()), int64_t(m.indices.size())));
+    }
+  const size_t staged = nPts * 12 + nTri * (slotted ? 16 : 12);
+  const size_t need = std::max(size_t(dlg_build_bytes(int64_t(nPts), int64_t(nTri))) + (slotted ? nTri * 4 : 0), staged + size_t(work)) + kVramMargin;
   while (!R.parked.empty() && size_t(dlg_free_bytes()) < need) { dlg_free_scene(R.parked.back().parked); R.parked.pop_back(); }
   if (const size_t free = size_t(dlg_free_bytes()); free < need) {
     err = "out of memory: building the scene (" + std::to_string(nTri) + " triangles) needs " + gib(need) + " of GPU memory, " + gib(free) +
           " free (close other programs that use the GPU, or run fewer meshes)";
     return false;
   }
+  // the faces of 5 or more corners ear-clipped on the host threads; the rest fanned on the GPU, mesh after mesh
   stage = "triangulating the scene";
-  Tris tris;
-  tris.v.resize(nPts);
-  tris.i.resize(nTri);
-  std::vector<uint32_t> triSlot(slotted ? nTri : 0);
-  std::vector<size_t> made(meshes.size(), 0);
+  std::vector<std::vector<int64_t>> ngFace(meshes.size());
+  std::vector<std::vector<int32_t>> ngMade(meshes.size());
+  std::vector<std::vector<uint32_t>> ngTri(meshes.size());
   std::vector<int> skip(meshes.size(), 0);
-  par_for(meshes.size(), [&](size_t k) {
-    if (!blocks(k)) return;
-    made[k] = triangulate_into(*meshes[k].mesh, s.c, tris.v.data() + vBase[k], tris.i.data() + tBase[k], slotted ? triSlot.data() + tBase[k] : nullptr,
-                               unsigned(vBase[k]), uint32_t(sBase[k]), skip[k]);
-  });
-  size_t at = 0;   // close the gaps of faces that drew fewer than n - 2 triangles
+  par_for(meshes.size(), [&](size_t k) { if (blocks(k)) ngon_faces(*meshes[k].mesh, ngFace[k], ngMade[k], ngTri[k], skip[k]); });
+  char e[512] = {};
+  if (dlg_mesh_begin(g, int64_t(nPts), int64_t(nTri), slotted ? 1 : 0, s.c, e, sizeof e)) { err = e; return false; }
+  size_t at = 0;
   for (size_t k = 0; k < meshes.size(); ++k) {
-    if (at != tBase[k] && made[k]) {
-      std::memmove(tris.i.data() + at, tris.i.data() + tBase[k], made[k] * sizeof(uint3));
-      if (slotted) std::memmove(triSlot.data() + at, triSlot.data() + tBase[k], made[k] * sizeof(uint32_t));
+    if (!blocks(k)) continue;
+    const SimulationMesh& m = *meshes[k].mesh;
+    int64_t made = 0;
+    if (dlg_mesh_add(g, m.points.data(), int64_t(m.numPoints()), m.counts.data(), int64_t(m.counts.size()), m.indices.data(), int64_t(m.indices.size()),
+                     m.slot.empty() ? nullptr : m.slot.data(), uint32_t(sBase[k]), ngFace[k].data(), ngMade[k].data(), ngTri[k].data(),
+                     int64_t(ngFace[k].size()), &made, e, sizeof e)) {
+      err = e;
+      return false;
     }
-    if (blocks(k)) s.order.push_back({meshes[k].name, at, made[k], meshes[k].mesh});
-    at += made[k];
+    s.order.push_back({meshes[k].name, at, size_t(made), meshes[k].mesh});
+    at += size_t(made);
     s.skipped += skip[k];
   }
-  tris.i.resize(at);
-  if (slotted) triSlot.resize(at);
   out.stats.prepMs += ms(tPrep);
   stage = "building the scene";
   const auto tBuild = Clk::now();
-  char e[512] = {};
-  if (dlg_build(g, reinterpret_cast<const float*>(tris.v.data()), int64_t(tris.v.size()), reinterpret_cast<const uint32_t*>(tris.i.data()),
-                int64_t(tris.i.size()), e, sizeof e) ||
-      (slotted && dlg_set_tri_slots(g, triSlot.data(), int64_t(triSlot.size()), e, sizeof e))) {
+  if (dlg_mesh_build(g, e, sizeof e)) {
     err = e;
     return false;
   }
   out.stats.buildMs += ms(tBuild);
-  s.nTris = tris.i.size();
+  s.nTris = at;
   s.key = key;
   R.cur = s;
   return true;

@@ -51,6 +51,10 @@ size_t tri_cap(const SimulationMesh& m);
 // (slotBase + its face's slot) into slots. Returns the triangles made (degenerate n-gons skipped and counted).
 size_t triangulate_into(const SimulationMesh& m, const double c[3], float3* v, uint3* tri, uint32_t* slots, unsigned base, uint32_t slotBase,
                         int& skipped);
+// The faces of 5 or more corners of m, ear-clipped as triangulate_into does: face[k] their indices ascending, made[k]
+// the triangles each made (0: skipped, counted in skipped), tri[3 ..] the corners (mesh-local) in face order. What the
+// GPU ingest (dlg_mesh_add) takes beside the faces it fans itself.
+void ngon_faces(const SimulationMesh& m, std::vector<int64_t>& face, std::vector<int32_t>& made, std::vector<uint32_t>& tri, int& skipped);
 size_t slot_count(const SimulationMesh& m);
 bool mesh_ok(const SimulationMesh& m, const std::string& label, std::string& err);
 bool optics_ok(const Options& opt, std::string& err);

@@ -85,7 +85,18 @@ struct dlg_gpu {
   DevBuf counts;
   // the tree dlg_build makes (dlg_set_tree): DLG_TREE_SAH or DLG_TREE_LBVH
   int32_t tree = DLG_TREE_SAH;
+  // mesh ingest (dlg_mesh_*): staged vertices [meshNv] float3, triangles [meshNt] uint3, their global slots, the
+  // largest |coordinate| as float bits and a fault word; the per-mesh temporaries
+  DevBuf meshV, meshI, meshS, meshWord, meshWork;
+  int64_t meshNv = 0, meshNt = 0, meshCapV = 0, meshCapT = 0;
+  int32_t meshSlotted = 0, meshOpen = 0;
+  double meshC[3] = {0, 0, 0};
 };
+
+// lbvh.hip: the build over triangles already on the device (v [nv] float3, tri [nt] uint3), mx the largest |coordinate|
+int32_t dlg_build_device(dlg_gpu* g, const float* v, int64_t nv, const uint32_t* tri, int64_t nt, float mx, char* err, int32_t errLen);
+// gpu_ctx.hip: leaf slots from per-triangle slots (input order) already on the device, largest slot mx
+int32_t dlg_leaf_slots_device(dlg_gpu* g, const uint32_t* slot, int64_t nt, uint32_t mx, char* err, int32_t errLen);
 
 // sah.hip: the binned-SAH tree over nt >= 2 leaves in Morton order (nodes [nt - 1][16] written, leaves reordered in
 // place), and the device bytes of its temporaries

@@ -118,6 +118,13 @@ SEAM = [
     ("dlg_download_bytes", C.c_int32, [C.c_void_p, C.c_int64, U8P, C.c_char_p, C.c_int32]),
     ("dlg_set_slots", C.c_int32, [C.c_void_p, F32P, C.c_int64, C.c_char_p, C.c_int32]),
     ("dlg_set_tri_slots", C.c_int32, [C.c_void_p, U32P, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_mesh_begin", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("dlg_mesh_add", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_uint32,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_char_p, C.c_int32]),
+    ("dlg_mesh_build", C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32]),
+    ("dlg_mesh_staged", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_char_p,
+                                    C.c_int32]),
+    ("dlg_mesh_bytes", C.c_int64, [C.c_int64, C.c_int64, C.c_int64]),
     ("dlg_bounce_flags", C.c_uint32, [C.c_int32]),
     ("dlg_bounce", C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint32, C.POINTER(BounceIn), C.c_char_p, C.c_int32]),
     ("dlg_reduce_dc", C.c_int32, [C.c_void_p, C.c_int64, F32P, F64P, C.c_char_p, C.c_int32]),
@@ -242,6 +249,37 @@ class Gpu:
     def set_tri_slots(self, slot):
         slot = np.ascontiguousarray(slot, np.uint32)
         _check(self.L.dlg_set_tri_slots(self.h, slot if slot.size else np.zeros(1, np.uint32), slot.size, self.err, 1024), self.err)
+
+    def mesh_begin(self, nPts, nTriCap, slotted, centre):
+        c = np.ascontiguousarray(centre, np.float64)
+        _check(self.L.dlg_mesh_begin(self.h, nPts, nTriCap, int(slotted), c.ctypes.data, self.err, 1024), self.err)
+
+    def mesh_add(self, points, counts, indices, face_slot=None, slot_base=0, ngon=None):
+        """One mesh into the staged scene; ngon = (faces int64, made int32, corners uint32 [3 sum made]) of its faces
+        of 5 or more corners, ear-clipped by the host. Returns the triangles made."""
+        keep = [np.ascontiguousarray(points, np.float64).reshape(-1), np.ascontiguousarray(counts, np.int32),
+                np.ascontiguousarray(indices, np.int32)]
+        fs = None if face_slot is None else np.ascontiguousarray(face_slot, np.uint16)
+        ng = ngon or (np.zeros(0, np.int64), np.zeros(0, np.int32), np.zeros(0, np.uint32))
+        ng = [np.ascontiguousarray(ng[0], np.int64), np.ascontiguousarray(ng[1], np.int32), np.ascontiguousarray(ng[2], np.uint32)]
+        ptr = lambda a: a.ctypes.data if a is not None and a.size else None
+        made = C.c_int64()
+        _check(self.L.dlg_mesh_add(self.h, ptr(keep[0]), keep[0].size // 3, ptr(keep[1]), keep[1].size, ptr(keep[2]), keep[2].size, ptr(fs),
+                                   slot_base, ptr(ng[0]), ptr(ng[1]), ptr(ng[2]), ng[0].size, C.byref(made), self.err, 1024), self.err)
+        return made.value
+
+    def mesh_staged(self, slotted=False):
+        nv, nt = C.c_int64(), C.c_int64()
+        _check(self.L.dlg_mesh_staged(self.h, None, C.byref(nv), None, C.byref(nt), None, self.err, 1024), self.err)
+        V = np.zeros((max(nv.value, 1), 3), np.float32)
+        I = np.zeros((max(nt.value, 1), 3), np.uint32)
+        S = np.zeros(max(nt.value, 1), np.uint32) if slotted else None
+        _check(self.L.dlg_mesh_staged(self.h, V.ctypes.data, C.byref(nv), I.ctypes.data, C.byref(nt), None if S is None else S.ctypes.data,
+                                      self.err, 1024), self.err)
+        return V[:nv.value], I[:nt.value], (S[:nt.value] if slotted else None)
+
+    def mesh_build(self):
+        _check(self.L.dlg_mesh_build(self.h, self.err, 1024), self.err)
 
     def reduce(self, nc, W, useCos):
         W = np.ascontiguousarray(W, np.float32).reshape(len(useCos), -1)
