@@ -104,6 +104,14 @@ DL_API int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t ca
  * in build order, reference mesh.cpp:137-168 (Tris::slot); *n triangles, the first min(n, cap) written. */
 DL_API int32_t dlx_thin_solid(const dl_mesh* mesh, int32_t slot, const double* centre, uint32_t* flags);
 DL_API int32_t dlx_triangle_slots(const dl_mesh* mesh, uint32_t slotBase, uint32_t* slots, int64_t cap, int64_t* n);
+/* Ladybug mode's sky, the host emulation of gendaymtx -A the reference runs inline (run.cpp:362-470; no GPU involved),
+ * over T steps (jd, hour, dni, dhi) at a site (lat, lon east-positive, fileTz the weather file's zone in hours) with the
+ * ground reflectance grefl and the run's hours dur: per patch of sky density mf (1 or 2) dir / dif / rad [P] (kWh/m2 x
+ * sr-weights), the ground's *gnd, and dump [2 x 3 (P + 1)] the two averaged blocks --dump-sky prints (-s then -d, ground
+ * row first). */
+DL_API int32_t dlx_ladybug_sky(int32_t mf, int64_t T, const int32_t* jd, const double* hour, const float* dni, const float* dhi, double lat,
+                               double lon, double fileTz, double grefl, double dur, double* dir, double* dif, double* rad, double* gnd,
+                               float* dump);
 
 #ifdef __cplusplus
 }

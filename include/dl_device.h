@@ -159,6 +159,9 @@ int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, char* err, i
 /* The last dlg_sky's series inputs: S [P x Tup] and g [Tup] (floats), as the contraction reads them, and with reflect the
  * bins' radiance Sdc [(P + 1) x Tup] (row 0 g, row 1 + p S plus the binned sun; null: not wanted). */
 int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, float* Sdc, char* err, int32_t errLen);
+/* The series inputs from the host instead of dlg_sky: S [P x Tup], g [Tup] (ladybug mode's gendaymtx -A columns, the
+ * host loop of reference run.cpp:377-396). */
+int32_t dlg_set_sky_series(dlg_gpu* g, const float* S, const float* gS, int32_t P, int32_t Tup, char* err, int32_t errLen);
 
 /* gpu_series (internal.h:155-171): E[i T + t] for the chunk's sensors, S from the last dlg_sky (P x T, T = Tup).
  * method 0: FP32 on the vector units, each sum in direction order by fma (the reference kernel's arithmetic);

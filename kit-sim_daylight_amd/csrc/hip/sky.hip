@@ -346,6 +346,19 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
   return 0;
 }
 
+extern "C" int32_t dlg_set_sky_series(dlg_gpu* g, const float* S, const float* gS, int32_t P, int32_t Tup, char* err, int32_t errLen) {
+  if (P <= 0 || Tup <= 0) { dlg_fail(err, errLen, "dlg_set_sky_series: empty series"); return 1; }
+  DLG_CHECK(g->skyS.grow(size_t(P) * size_t(Tup) * 4));
+  DLG_CHECK(g->skyG.grow(size_t(Tup) * 4));
+  DLG_CHECK(hipMemcpyAsync(g->skyS.p, S, size_t(P) * size_t(Tup) * 4, hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipMemcpyAsync(g->skyG.p, gS, size_t(Tup) * 4, hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipStreamSynchronize(g->stream));
+  g->skyP = P;
+  g->skyT = Tup;
+  g->skyReflect = 0;
+  return 0;
+}
+
 extern "C" int32_t dlg_sky_series(dlg_gpu* g, float* S, float* gS, float* Sdc, char* err, int32_t errLen) {
   if (!g->skyT) { dlg_fail(err, errLen, "dlg_sky_series: the last dlg_sky kept no series"); return 1; }
   if (Sdc && !g->skyReflect) { dlg_fail(err, errLen, "dlg_sky_series: the last dlg_sky ran without reflect"); return 1; }

@@ -314,6 +314,23 @@ int32_t dlx_triangle_slots(const dl_mesh* mesh, uint32_t slotBase, uint32_t* slo
   });
 }
 
+int32_t dlx_ladybug_sky(int32_t mf, int64_t T, const int32_t* jd, const double* hour, const float* dni, const float* dhi, double lat,
+                        double lon, double fileTz, double grefl, double dur, double* dir, double* dif, double* rad, double* gnd, float* dump) {
+  return guard([&]() -> int32_t {
+    if ((mf != 1 && mf != 2) || T <= 0 || !jd || !hour || !dni || !dhi || !dir || !dif || !rad || !gnd || !dump)
+      return fail(DL_USAGE, "dlx_ladybug_sky: sky density 1 or 2, T > 0 steps and every array");
+    std::vector<Step> steps(static_cast<size_t>(T));
+    for (size_t t = 0; t < steps.size(); ++t) { steps[t].jd = jd[t]; steps[t].hour = hour[t]; steps[t].dni = dni[t]; steps[t].dhi = dhi[t]; }
+    const Patches pa = make_patches(mf);
+    LadybugSky lb;
+    ladybug_sky(pa, steps, lat, lon, fileTz, grefl, dur, {}, nullptr, nullptr, 0, lb);
+    for (size_t p = 0; p < size_t(pa.P); ++p) { dir[p] = lb.dir[p]; dif[p] = lb.dif[p]; rad[p] = lb.rad[p]; }
+    *gnd = lb.gnd;
+    std::copy(lb.dump.begin(), lb.dump.end(), dump);
+    return DL_OK;
+  });
+}
+
 int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t cap, int64_t* rows, double* latLonTz) {
   return guard([&]() -> int32_t {
     if (!path || !rows) return fail(DL_USAGE, "dlx_read_epw: a NULL path or rows");

@@ -21,7 +21,7 @@ const char* const kUsage =
     "  meshes: .obj (one o/g group per mesh) or .dmesh (raw arrays); world metres, Z up; positional = sensor meshes\n"
     "  --context FILE           occluder-only mesh (repeatable)\n"
     "  --on faces|vertices      sensor placement (default faces)\n"
-    "  --mode physical          (ladybug: not in this build)\n"
+    "  --mode physical|ladybug  default physical\n"
     "  --quantity Q[,Q...]      radiation|direct|diffuse|irradiance|illuminance|sunhours (default radiation)\n"
     "  --sides front|back|both  receiving side(s) of each element (default front)\n"
     "  --cull-backfaces         occluders block only from their front side (default two-sided)\n"

@@ -46,11 +46,27 @@ V3 newell(const SimulationMesh& m, const int* idx, int n) {
   return N;
 }
 
-// physical face geometry: unit Newell normal (0 under 1e-12 area), signed-area-weighted fan centroid
-void face_geom(const SimulationMesh& m, const int* idx, int n, V3& nrm, double& area, V3& cen) {
+// Face geometry by the mode's rules (reference mesh.cpp:45-69). Physical: unit Newell normal (0 under 1e-12 area),
+// signed-area-weighted fan centroid. Ladybug (RadiationStudy's, tri / quad faces): the corners' mean; a triangle's
+// cross product; a quad's two triangle normals (0 1 2) and (2 3 1) summed, the area their half-lengths summed.
+void face_geom(const SimulationMesh& m, const int* idx, int n, bool ladybug, V3& nrm, double& area, V3& cen) {
   nrm = {0, 0, 0}; area = 0; cen = {0, 0, 0};
   if (n < 3) return;
   const V3 P0 = pt(m, idx[0]);
+  if (ladybug) {
+    for (int i = 0; i < n; ++i) cen = cen + pt(m, idx[i]);
+    cen = cen * (1.0 / n);
+    V3 N{0, 0, 0};
+    if (n == 3) { N = cross(pt(m, idx[1]) - P0, pt(m, idx[2]) - P0); area = len(N) / 2; }
+    else if (n == 4) {
+      const V3 P1 = pt(m, idx[1]), P2 = pt(m, idx[2]), P3 = pt(m, idx[3]);
+      const V3 n1 = cross(P1 - P0, P2 - P0), n2 = cross(P3 - P2, P1 - P2);
+      N = n1 + n2;
+      area = (len(n1) + len(n2)) / 2;
+    }
+    nrm = area < 1e-12 ? V3{0, 0, 0} : unit(N);
+    return;
+  }
   const V3 N = newell(m, idx, n), Nu = unit(N);
   area = len(N) / 2;
   double sum = 0;
@@ -149,11 +165,23 @@ void par_for(size_t n, const std::function<void(size_t)>& f) {
 }
 
 void bbox_centre(const std::vector<SimulationMeshView>& meshes, double c[3]) {
+  // blocks of points on the host threads, each with its own bounds (min and max are exact in any order)
+  constexpr size_t kBlock = size_t(1) << 18;
+  struct Job { const double* p; size_t n; };
+  std::vector<Job> jobs;
+  for (const SimulationMeshView& v : meshes)
+    for (size_t i = 0, n = v.mesh->numPoints(); i < n; i += kBlock) jobs.push_back({v.mesh->points.data() + 3 * i, std::min(kBlock, n - i)});
+  std::vector<std::array<double, 6>> b(jobs.size());
+  par_for(jobs.size(), [&](size_t j) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const double* p = jobs[j].p;
+    for (size_t i = 0; i < jobs[j].n; ++i, p += 3)
+      for (int k = 0; k < 3; ++k) { lo[k] = std::fmin(lo[k], p[k]); hi[k] = std::fmax(hi[k], p[k]); }
+    b[j] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+  });
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (const SimulationMeshView& v : meshes) {
-    const std::vector<double>& p = v.mesh->points;
-    for (size_t i = 0; i < p.size(); ++i) { lo[i % 3] = std::fmin(lo[i % 3], p[i]); hi[i % 3] = std::fmax(hi[i % 3], p[i]); }
-  }
+  for (const std::array<double, 6>& x : b)
+    for (int k = 0; k < 3; ++k) { lo[k] = std::fmin(lo[k], x[size_t(k)]); hi[k] = std::fmax(hi[k], x[size_t(3 + k)]); }
   for (int k = 0; k < 3; ++k) c[k] = lo[k] <= hi[k] ? (lo[k] + hi[k]) / 2 : 0;
 }
 
@@ -409,8 +437,15 @@ bool resolve_targets(const std::vector<SimulationMeshView>& meshes, const std::v
   return true;
 }
 
-size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* pOut, float3* nOut, std::string& warn) {
+size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* pOut, float3* nOut, std::string& warn,
+                         bool ladybug) {
   const V3 C{c[0], c[1], c[2]};
+  if (ladybug)   // RadiationStudy's rule: tri / quad faces only (nothing made)
+    for (size_t f = 0; f < m.counts.size(); ++f)
+      if (m.counts[f] > 4) {
+        warn = "error: ladybug mode supports tri/quad faces only (mesh " + m.name + ", face " + std::to_string(f) + ")";
+        return 0;
+      }
   int degenerate = 0, zeroVn = 0;
   std::vector<V3> vn;
   if (opt.onVertices) vn.assign(m.numPoints(), V3{0, 0, 0});
@@ -439,7 +474,7 @@ size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const d
         off += size_t(n);
         V3 nrm, cen;
         double area;
-        face_geom(m, idx, n, nrm, area, cen);
+        face_geom(m, idx, n, ladybug, nrm, area, cen);
         if (area < 1e-12) ++bad;
         if (opt.sides != Sides::Back) { pOut[jb] = f3(cen + nrm * opt.offset - C); nOut[jb] = f3(nrm); ++jb; }
         if (opt.sides != Sides::Front) { pOut[jb] = f3(cen - nrm * opt.offset - C); nOut[jb] = f3(nrm * -1); ++jb; }
@@ -456,7 +491,7 @@ size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const d
       off += size_t(n);
       V3 nrm, cen;
       double area;
-      face_geom(m, idx, n, nrm, area, cen);
+      face_geom(m, idx, n, ladybug, nrm, area, cen);
       if (area < 1e-12) ++degenerate;
       if (opt.onVertices) for (int k = 0; k < n; ++k) vn[size_t(idx[k])] = vn[size_t(idx[k])] + nrm * area;
       else push(cen, nrm);

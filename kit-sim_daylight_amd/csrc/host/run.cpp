@@ -78,7 +78,8 @@ Scene& installed_scene() { return resident().cur; }
 
 bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
                    int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage,
-                   std::string& err) {
+                   std::string& err, const std::function<void(const char*)>& mark) {
+  const auto at_mark = [&](const char* what) { if (mark) mark(what); };
   Resident& R = resident();
   const bool installed = !key.empty() && R.gpu && key == R.cur.key;
   const auto parkedAt = [&] { return std::find_if(R.parked.begin(), R.parked.end(), [&](const Scene& p) { return p.key == key; }); };
@@ -115,6 +116,7 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
     sBase[k + 1] = sBase[k] + slot_count(*meshes[k].mesh);   // global slots: every mesh's, in input order
   }
   const size_t nPts = vBase.back(), nTri = tBase.back();
+  at_mark("centre");
   stage = "making room on the GPU";
   dlg_trim(g, 1);
   // the peak: the staged triangles beside one mesh's ingest temporaries, or beside the tree's build
@@ -139,6 +141,7 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
   std::vector<std::vector<uint32_t>> ngTri(meshes.size());
   std::vector<int> skip(meshes.size(), 0);
   par_for(meshes.size(), [&](size_t k) { if (blocks(k)) ngon_faces(*meshes[k].mesh, ngFace[k], ngMade[k], ngTri[k], skip[k]); });
+  at_mark("ear-clip");
   char e[512] = {};
   if (dlg_mesh_begin(g, int64_t(nPts), int64_t(nTri), slotted ? 1 : 0, s.c, e, sizeof e)) { err = e; return false; }
   size_t at = 0;
@@ -157,6 +160,7 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
     s.skipped += skip[k];
   }
   out.stats.prepMs += ms(tPrep);
+  at_mark("ingest");
   stage = "building the scene";
   const auto tBuild = Clk::now();
   if (dlg_mesh_build(g, e, sizeof e)) {
@@ -164,6 +168,7 @@ bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::fun
     return false;
   }
   out.stats.buildMs += ms(tBuild);
+  at_mark("build");
   s.nTris = at;
   s.key = key;
   R.cur = s;
@@ -219,11 +224,11 @@ SensorCache& sensor_cache() {
 }  // namespace
 
 const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
-                              Sensors& scratch, Result& out, std::string& warn, std::string& err) {
+                              Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug) {
   std::string key;
   if (!opt.sceneKey.empty()) {   // what shapes the sensors besides the meshes (their roles are part of the scene key)
     char b[96];
-    std::snprintf(b, sizeof b, "\nsensors v%d s%d o%a n%zu", opt.onVertices ? 1 : 0, int(opt.sides), opt.offset, nSens);
+    std::snprintf(b, sizeof b, "\nsensors v%d s%d o%a n%zu m%d", opt.onVertices ? 1 : 0, int(opt.sides), opt.offset, nSens, ladybug ? 1 : 0);
     key = opt.sceneKey + b;
   }
   SensorCache& C = sensor_cache();
@@ -233,7 +238,7 @@ const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, con
     return &C.s;
   }
   C.key.clear();
-  if (!sensors_of(meshes, opt, c, nSens, scratch, out, warn, err)) return nullptr;
+  if (!sensors_of(meshes, opt, c, nSens, scratch, out, warn, err, ladybug)) return nullptr;
   if (key.empty()) return &scratch;
   C.s = std::move(scratch);
   C.ranges = out.ranges;
@@ -243,7 +248,7 @@ const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, con
 }
 
 bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
-                std::string& warn, std::string& err) {
+                std::string& warn, std::string& err, bool ladybug) {
   const size_t nS = opt.sides == Sides::Both ? 2 : 1;
   s.p.resize(nSens);
   s.n.resize(nSens);
@@ -252,11 +257,12 @@ bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions&
     sAt[k + 1] = sAt[k] + (meshes[k].sensor ? (opt.onVertices ? meshes[k].mesh->numPoints() : meshes[k].mesh->numFaces()) * nS : 0);
   std::vector<std::string> warns(meshes.size());
   par_for(meshes.size(), [&](size_t k) {
-    if (meshes[k].sensor) elems[k] = make_sensors_into(*meshes[k].mesh, opt, c, s.p.data() + sAt[k], s.n.data() + sAt[k], warns[k]);
+    if (meshes[k].sensor) elems[k] = make_sensors_into(*meshes[k].mesh, opt, c, s.p.data() + sAt[k], s.n.data() + sAt[k], warns[k], ladybug);
   });
   out.ranges.clear();
   for (size_t k = 0; k < meshes.size(); ++k) {
     if (!meshes[k].sensor) continue;
+    if (warns[k].rfind("error:", 0) == 0) { err = warns[k]; return false; }   // the ladybug mode's n-gon error
     if (!warns[k].empty()) warn += (warn.empty() ? "" : "; ") + warns[k];
     out.ranges.emplace_back(meshes[k].name, elems[k]);
   }
@@ -277,10 +283,12 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   for (const Quantity q : opt.quantities)
     if (q != Quantity::SunHours) { needWeather = true; if (q == Quantity::Illuminance) wantVis = true; else wantRad = true; }
   if (needWeather && !w) { err = "weather (--epw) required for radiation/irradiance/illuminance"; return false; }
-  if (opt.mode == Mode::Ladybug) { err = "--mode ladybug is not part of this build (physical mode only)"; return false; }
-  if (!opt.dumpSky.empty()) { err = "--dump-sky is ladybug-mode only (with weather)"; return false; }
+  const bool ladybug = opt.mode == Mode::Ladybug;
+  if (ladybug && wantVis) { err = "illuminance is not available in ladybug mode (RadiationStudy is radiometric)"; return false; }
+  if (ladybug && opt.mf != 1 && opt.mf != 2) { err = "ladybug mode supports --sky-density 1 or 2 only"; return false; }
+  if (!opt.dumpSky.empty() && !(ladybug && needWeather)) { err = "--dump-sky is ladybug-mode only (with weather)"; return false; }
   if (opt.mf != 1 && opt.mf != 2 && opt.mf != 4) { err = "--sky-density must be 1, 2 or 4"; return false; }
-  const int K = std::max(1, opt.patchSamples);
+  const int K = ladybug ? 1 : std::max(1, opt.patchSamples);
   {
     const int m = int(std::lround(std::sqrt(double(K))));
     if (m * m != K) { err = "--patch-samples must be a square number (1, 4, 9, 16, ...)"; return false; }
@@ -342,6 +350,10 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   for (const SimulationMeshView& v : meshes) if (v.sensor) nElem += opt.onVertices ? v.mesh->numPoints() : v.mesh->numFaces();
   const size_t nSens = nElem * size_t(nS);
   if (nSens == 0) { err = "no sensor elements (no sensor meshes, or all faces degenerate)"; return false; }
+  if (ladybug)   // RadiationStudy's rule, checked before the scene is built (run.cpp:304-307)
+    for (const SimulationMeshView& v : meshes)
+      for (size_t f = 0; v.sensor && f < v.mesh->counts.size(); ++f)
+        if (v.mesh->counts[f] > 4) { err = "error: ladybug mode supports tri/quad faces only (mesh " + v.name + ", face " + std::to_string(f) + ")"; return false; }
 
   // ---- the sky of every step, on the GPU (sun, Perez, cumulative weights, series matrix)
   if (cancelled()) return false;
@@ -349,6 +361,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   stage = "computing the sky";
   dlg_gpu* g = process_gpu(err);
   if (!g) return false;
+  mark("gpu");
   const Patches pa = make_patches(opt.mf);
   const int P = pa.P;
   const size_t Pz = size_t(P);
@@ -367,8 +380,8 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   int Tup = 0;
   {
     dlg_sky_in in{int32_t(T), month.data(), day.data(), jd.data(), hour.data(), dtv.data(), dni.data(), dhi.data(), lat, lon, tz, opt.north, grefl,
-                  leap ? 1 : 0, opt.mf, needWeather && wantRad ? 1 : 0, needWeather && wantVis ? 1 : 0, series && needWeather ? 1 : 0,
-                  seriesVis ? 1 : 0, tcos_table(), reflect ? 1 : 0};
+                  leap ? 1 : 0, opt.mf, needWeather && wantRad && !ladybug ? 1 : 0, needWeather && wantVis ? 1 : 0,
+                  series && needWeather && !ladybug ? 1 : 0, seriesVis ? 1 : 0, tcos_table(), reflect ? 1 : 0};
     dlg_sky_out so{alt.data(), azi.data(), up.data(), vec.data(), skyRad.data(), skyVis.data(), &gndRad, &gndVis, sunRadT.data(), sunVisT.data(), &Tup,
                    reflect ? sunBinRad.data() : nullptr, reflect ? sunBinVis.data() : nullptr};
     char e[512] = {};
@@ -380,7 +393,10 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   size_t nSky = 0;
   if (needWeather) {
     for (int p = 0; p < P; ++p)
-      for (int k = 0; k < K; ++k) { D.push_back(dlm::rotate_z(patch_sample(pa, p, k, K), opt.north)); rowOf.push_back(p); }
+      for (int k = 0; k < K; ++k) {
+        D.push_back(dlm::rotate_z(ladybug ? pa.lb[size_t(p)] : patch_sample(pa, p, k, K), opt.north));
+        rowOf.push_back(p);
+      }
     nSky = D.size();
     for (size_t d = 0; d < nSky; ++d) D.push_back(make_float3(-D[d].x, -D[d].y, -D[d].z));
   }
@@ -394,13 +410,20 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   std::vector<int> colOf(T, -1);
   for (size_t t = 0, c = 0; t < T; ++t) if (steps[t].dni + steps[t].dhi > kNight) colOf[t] = int(c++);
   if (!series) Tup = 0;
+  // ladybug: gendaymtx -A on the host (its series columns replace the device sky's)
+  LadybugSky lb;
+  std::vector<float> lbS, lbG;
+  if (ladybug && needWeather) {
+    if (series) { lbS.assign(Pz * size_t(Tup), 0.f); lbG.assign(size_t(std::max(Tup, 0)), 0.f); }
+    ladybug_sky(pa, steps, lat, lon, w ? w->tz : tz, grefl, durationHours, colOf, series ? lbS.data() : nullptr, series ? lbG.data() : nullptr, Tup, lb);
+  }
   // ---- per-quantity weights (run.cpp:475-505); the ground visibility weights for the series
   const size_t nQ = opt.quantities.size();
   std::vector<float> Wq(nQ * nDir + (series ? nDir : 0), 0.f);
   std::vector<double> sunRad(nSun, 0.0), sunVis(nSun, 0.0);
   double droppedDni = 0;
   for (size_t j = 0; j < nSun; ++j) { sunRad[j] = needWeather ? sunRadT[size_t(sunStep[j])] : 0.0; sunVis[j] = needWeather ? sunVisT[size_t(sunStep[j])] : 0.0; }
-  for (size_t t = 0; t < T && needWeather; ++t) if (!up[t] && steps[t].dni > 0) droppedDni += double(steps[t].dni) * steps[t].dt;
+  for (size_t t = 0; t < T && needWeather && !ladybug; ++t) if (!up[t] && steps[t].dni > 0) droppedDni += double(steps[t].dni) * steps[t].dt;
   for (size_t qi = 0; qi < nQ; ++qi) {
     const Quantity q = opt.quantities[qi];
     float* W = Wq.data() + qi * nDir;
@@ -409,6 +432,14 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
       continue;
     }
     const bool vis = q == Quantity::Illuminance, wantSky = q != Quantity::Direct, wantSun = q != Quantity::Diffuse;
+    if (ladybug) {   // the sun is inside the patches (gendaymtx -d): sky and ground only (run.cpp:489-502)
+      for (size_t d = 0; d < nSky; ++d) {
+        const size_t p = size_t(rowOf[d]);
+        W[d] = float(q == Quantity::Direct ? lb.dir[p] : q == Quantity::Diffuse ? lb.dif[p] : lb.rad[p]);
+      }
+      if (q != Quantity::Direct) for (size_t d = nSky; d < 2 * nSky; ++d) W[d] = float(lb.gnd);
+      continue;
+    }
     for (size_t d = 0; d < nSky; ++d) {
       const size_t p = size_t(rowOf[d]);
       W[d] = float(wantSky ? (vis ? skyVis[p] : skyRad[p]) * pa.omega[p] / K : 0.0);
@@ -419,7 +450,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   }
   if (series) {
     float* Wg = Wq.data() + nQ * nDir;
-    for (size_t d = nSky; d < 2 * nSky; ++d) Wg[d] = pa.omega[size_t(rowOf[d - nSky])] / float(K);
+    for (size_t d = nSky; d < 2 * nSky; ++d) Wg[d] = ladybug ? 1.f : pa.omega[size_t(rowOf[d - nSky])] / float(K);
   }
   // reflected light (run.cpp:506-515): radiance per bounce bin (0 the ground, 1 + p the sky and the sun binned), no cosine
   // (the rows hold it); radiation / diffuse / irradiance radiometric, illuminance visible, direct and sun hours none
@@ -459,7 +490,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   }
   dlg_gpu* gp = nullptr;
   Scene sc;
-  if (!scene_acquire(meshes, blocks, slotted, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err)) return false;
+  if (!scene_acquire(meshes, blocks, slotted, key, opt.residentScenes, opt.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err, mark)) return false;
   mark("scene");
   dlg_stats_t st0{};
   std::vector<uint32_t>& thin = sc.thin;
@@ -493,7 +524,7 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
   const auto tSens = Clk::now();
   Sensors scratch;
   std::string warn;
-  const Sensors* sens = sensors_cached(meshes, opt, sc.c, nSens, scratch, out, warn, err);
+  const Sensors* sens = sensors_cached(meshes, opt, sc.c, nSens, scratch, out, warn, err, ladybug);
   if (!sens) return false;
   out.stats.prepMs += ms(tSens);
   mark("sensors");
@@ -558,7 +589,11 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
       const int c = colOf[size_t(sunStep[j])];
       sunCol[j] = c < 0 ? 0 : c;
       const double dtj = steps[size_t(sunStep[j])].dt;
-      sunW[j] = (c < 0 || dtj <= 0) ? 0.f : float((seriesVis ? sunVis[j] : sunRad[j] * 1000.0) / dtj);
+      sunW[j] = (ladybug || c < 0 || dtj <= 0) ? 0.f : float((seriesVis ? sunVis[j] : sunRad[j] * 1000.0) / dtj);   // ladybug: in the patches
+    }
+    if (ladybug && series && Tup > 0) {   // the host's gendaymtx -A columns as the contraction's sky
+      char e[512] = {};
+      if (dlg_set_sky_series(gp, lbS.data(), lbG.data(), P, Tup, e, sizeof e)) { err = e; return false; }
     }
     const int method = series_method();
     for (size_t i0 = 0; i0 < nSens; i0 += ncMax) {
@@ -641,13 +676,28 @@ static bool run_daylight(const std::vector<SimulationMeshView>& meshes, const We
       line += std::string(" ") + marks[k].first + " " + std::to_string(std::chrono::duration<double, std::milli>(marks[k].second - marks[k - 1].second).count()).substr(0, 6) + ";";
     std::fprintf(stderr, "%s\n", line.c_str());
   }
-  if (!opt.dumpSun.empty()) {   // the NOAA sun per step (columns of the reference's dump; Radiance's sun.c columns repeat it)
+  if (!opt.dumpSun.empty()) {   // columns 7-9: Radiance sun.c at the exact location, for the gendaymtx -D comparison (run.cpp:713-723)
     File f(std::fopen(opt.dumpSun.c_str(), "w"), std::fclose);
     if (!f) { err = "cannot write " + opt.dumpSun; return false; }
-    std::fprintf(f.get(), "# hoy month day hour alt_deg azi_deg up dni dhi   (north %.3f)\n", opt.north);
-    for (size_t t = 0; t < T; ++t)
-      std::fprintf(f.get(), "%.4f %d %d %.4f %.6f %.6f %d %.4f %.4f\n", steps[t].hoy, steps[t].month, steps[t].day, steps[t].hour, alt[t], azi[t],
-                   up[t], steps[t].dni, steps[t].dhi);
+    std::fprintf(f.get(), "# hoy month day hour alt_deg azi_deg up alt_radiance_deg azi_radiance_deg up_radiance dni dhi   (north %.3f)\n", opt.north);
+    for (size_t t = 0; t < T; ++t) {
+      double altR, aziR;
+      sun_radiance(lat, -lon, -15.0 * tz, steps[t].jd, steps[t].hour, altR, aziR);
+      std::fprintf(f.get(), "%.4f %d %d %.4f %.6f %.6f %d %.6f %.6f %d %.4f %.4f\n", steps[t].hoy, steps[t].month, steps[t].day, steps[t].hour, alt[t],
+                   azi[t], up[t], altR / (kPi / 180.), aziR / (kPi / 180.), altR > -0.2665 * (kPi / 180.) ? 1 : 0, steps[t].dni, steps[t].dhi);
+    }
+  }
+  if (!opt.dumpSky.empty()) {   // the two gendaymtx -A ASCII blocks, -s then -d, ground row first (run.cpp:725-736)
+    File f(std::fopen(opt.dumpSky.c_str(), "w"), std::fclose);
+    if (!f) { err = "cannot write " + opt.dumpSky; return false; }
+    const char* head[2] = {"# gendaymtx -s -A ascii rows (ground row first), %.3g\n", "# gendaymtx -d -A ascii rows (ground row first), %.3g\n"};
+    for (size_t b = 0; b < 2; ++b) {
+      std::fputs(head[b], f.get());
+      for (size_t r = 0; r <= Pz; ++r) {
+        const float* v = &lb.dump[b * 3 * (Pz + 1) + 3 * r];
+        std::fprintf(f.get(), "%.3g %.3g %.3g\n", v[0], v[1], v[2]);
+      }
+    }
   }
   (void)kPi; (void)kRf; (void)kGf; (void)kBf;
   return true;

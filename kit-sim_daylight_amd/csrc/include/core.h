@@ -27,6 +27,23 @@ struct Patches {
 Patches make_patches(int mf);
 float3 patch_sample(const Patches& pa, int p, int k, int K);
 const double* tcos_table();     // [1025] cos(k pi / 2048), Radiance tcos.c's table
+// Ladybug mode's sky (ladybug.cpp; reference run.cpp:362-470): per patch the -d and -s gendaymtx -A averages printed
+// %.3g as broadband x omega x hours / 1000 (dir, dif, their sum rad), the ground's gnd = ground x mean rad; dump the
+// two averaged blocks (-s then -d, ground row first, rgb) for --dump-sky. S / gS (may be null): the series columns
+// (bright(-s) + -d per patch, the ground's reflectance x the omega-weighted mean over P).
+struct LadybugSky {
+  std::vector<double> dir, dif, rad;
+  double gnd = 0;
+  std::vector<float> dump;
+};
+void ladybug_sky(const Patches& pa, const std::vector<Step>& steps, double lat, double lon, double fileTz, double grefl, double durationHours,
+                 const std::vector<int>& colOf, float* S, float* gS, int Tup, LadybugSky& out);
+// Radiance sun.c at a site (degrees; longitude and meridian west-positive): altitude and azimuth in radians
+void sun_radiance(double latDeg, double lonWestDeg, double meridianWestDeg, int jd, double hour, double& alt, double& azi);
+double round_fmt(double x, const char* fmt);
+void gendaymtx_rgb(const Patches& pa, const double* tsinP, int jd, double alt, double azi, double dni, double dhi, double grefl, bool sun, bool sky,
+                   float* out);
+std::vector<double> patch_table_sine(const Patches& pa);
 
 // ---------------------------------------------------------------- solar (reference solar.cpp)
 bool make_steps(const Options& opt, const Weather* w, std::vector<Step>& out, std::string& err, double shift = 0);
@@ -72,7 +89,8 @@ bool resolve_optics(const std::vector<SimulationMeshView>& meshes, const std::ve
 bool resolve_no_occlude(const std::vector<SimulationMeshView>& meshes, const std::vector<std::string>& names, std::vector<char>& open, std::string& err);
 bool resolve_targets(const std::vector<SimulationMeshView>& meshes, const std::vector<views::Target>& targets, bool selfOcclude,
                      const std::vector<char>& open, std::vector<std::string>& groups, std::vector<uint8_t>& cls, std::string& err);
-size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* p, float3* n, std::string& warn);
+size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* p, float3* n, std::string& warn,
+                         bool ladybug = false);
 
 // ---------------------------------------------------------------- scene residency and runs (reference run.cpp:38-213)
 bool npy_write_header(std::FILE* f, size_t rows, size_t cols, const char* descr);
@@ -89,9 +107,11 @@ struct Scene {
 // The process's GPU (made on first use, never destroyed), or null with err.
 dlg_gpu* process_gpu(std::string& err);
 // The run's scene: the installed or a parked one under key, else built from the blocking meshes (slotted: each
-// triangle's global slot kept with the leaves, dlg_set_tri_slots).
+// triangle's global slot kept with the leaves, dlg_set_tri_slots). mark (may be empty) is called at the build's stage
+// boundaries ("centre", "ear-clip", "ingest", "build") for the run's -v timing line.
 bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
-                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err);
+                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err,
+                   const std::function<void(const char*)>& mark = {});
 // the process's installed scene (a run sets the thin solids of a fresh slotted build on it)
 Scene& installed_scene();
 bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
@@ -99,12 +119,12 @@ bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSenso
 std::string open_key(const std::vector<char>& open, bool selfOcclude);
 void scene_forget(const std::string& key);
 bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
-                std::string& warn, std::string& err);
+                std::string& warn, std::string& err, bool ladybug = false);
 // The run's sensors: those of the last run when it had the same meshes (opt.sceneKey: the mesh handles, which never
 // change) and the same sensor options, else made by sensors_of and kept for the next run. Without a scene key, made
 // into `scratch`. The pointer stays valid until the next run.
 const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
-                              Sensors& scratch, Result& out, std::string& warn, std::string& err);
+                              Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug = false);
 double ms_since(const void* clockStart);
 // daylight::run with the --timeseries output kept on the device: per chunk of sensors, sink(i0, nc, T, E) with E the
 // device array float [nc x T] (null when the run has no series columns); an empty sink drops the series.

@@ -96,6 +96,8 @@ C_ABI = [
     ("dlx_read_epw", C.c_int32, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("dlx_thin_solid", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_uint32)]),
     ("dlx_triangle_slots", C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    ("dlx_ladybug_sky", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
 ]
 SEAM = [
     ("dlg_init", C.c_int32, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int32]),
@@ -130,6 +132,7 @@ SEAM = [
     ("dlg_reduce_dc", C.c_int32, [C.c_void_p, C.c_int64, F32P, F64P, C.c_char_p, C.c_int32]),
     ("dlg_download_dc", C.c_int32, [C.c_void_p, C.c_int64, F32P, C.c_char_p, C.c_int32]),
     ("dlg_sky", C.c_int32, [C.c_void_p, C.POINTER(SkyIn), C.POINTER(SkyOut), C.c_char_p, C.c_int32]),
+    ("dlg_set_sky_series", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
     ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),

@@ -47,7 +47,7 @@ class DaylightOpt(C.Structure):
                 ("ground", C.c_double), ("north", C.c_double), ("offset", C.c_double), ("sides", C.c_int),
                 ("onVertices", C.c_int), ("selfOcclude", C.c_int), ("cullBackfaces", C.c_int), ("virtualGround", C.c_int),
                 ("lat", C.c_double), ("lon", C.c_double), ("tz", C.c_double), ("time", Time), ("wantSeries", C.c_int),
-                ("nthreads", C.c_int)]
+                ("nthreads", C.c_int), ("ladybug", C.c_int)]
 
 
 class DaylightOut(C.Structure):
@@ -103,6 +103,11 @@ def lib():
         L.orc_bbox_centre.argtypes = [C.c_void_p, C.c_int, F64P]
         L.orc_sensors.restype = C.c_long
         L.orc_sensors.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_double, F64P, F32P, F32P]
+        L.orc_sensors_mode.restype = C.c_long
+        L.orc_sensors_mode.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_double, C.c_int, F64P, F32P, F32P]
+        L.orc_ladybug_sky.argtypes = [C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
         L.orc_triangulate.restype = C.c_long
         L.orc_triangulate.argtypes = [C.POINTER(Mesh), F64P, F32P, U32P, C.c_uint32, C.POINTER(C.c_int)]
         L.orc_scene_new.restype = C.c_void_p
@@ -293,7 +298,7 @@ def read_obj(path):
 def daylight(meshes, weather=None, quantities=("radiation",), mf=1, patchSamples=1, ground=0.2, north=0.0, offset=0.0,
              sides="front", onVertices=False, selfOcclude=True, cullBackfaces=False, virtualGround=True, lat=math.nan,
              lon=math.nan, tz=math.nan, t0=math.nan, t1=math.nan, stepMin=60, dayFrom=0.0, dayTo=1440.0, series=False,
-             nthreads=None):
+             nthreads=None, ladybug=False):
     """The oracle's daylight run. weather: dict(lat, lon, tz, leap, dni, dhi) or None. Returns dict(values, series, ...)."""
     L = lib()
     q = [QUANTITIES.index(x) for x in quantities]
@@ -307,6 +312,7 @@ def daylight(meshes, weather=None, quantities=("radiation",), mf=1, patchSamples
     o.time = Time(t0, t1, stepMin, dayFrom, dayTo)
     o.wantSeries = int(series)
     o.nthreads = nthreads or min(8, os.cpu_count() or 1)
+    o.ladybug = int(ladybug)
     arr = (Mesh * len(meshes))(*[m.c() for m in meshes])
     if weather is not None:
         dni = np.ascontiguousarray(weather["dni"], np.float32)
@@ -475,3 +481,41 @@ def views(meshes, fov=(120.0, -70.0, 50.0), cell=2.0, jitter=True, far=1000.0, r
     if raw:
         res.update(cls=cls.reshape(K, nSens).T.copy(), dist=dst.reshape(K, nSens).T.copy(), sums=sums)
     return res
+
+
+def sensors(mesh, ladybug=False, onVertices=False, sides="front", offset=0.0, centre=(0.0, 0.0, 0.0)):
+    """make_sensors_into by the mode's rules: (P [n, 3], N [n, 3]) float32, or None for ladybug's n-gon error."""
+    nS = 2 if sides == "both" else 1
+    n = (len(mesh.points) // 3 if onVertices else len(mesh.counts)) * nS
+    P = np.zeros(3 * max(n, 1), np.float32)
+    N = np.zeros(3 * max(n, 1), np.float32)
+    m = mesh.c()
+    made = lib().orc_sensors_mode(C.byref(m), int(onVertices), {"front": 0, "back": 1, "both": 2}[sides], offset, int(ladybug),
+                                  np.ascontiguousarray(centre, np.float64), P, N)
+    if made < 0:
+        return None
+    return P[:3 * n].reshape(-1, 3), N[:3 * n].reshape(-1, 3)
+
+
+def ladybug_sky(mf, jd, hour, dni, dhi, lat, lon, fileTz, grefl, dur, colOf=None, Tup=0):
+    """Ladybug mode's gendaymtx -A emulation over the steps: dict(dir, dif, rad, gnd, dump [2, P + 1, 3], S, g)."""
+    P = lib().orc_patch_count(mf)
+    T = len(jd)
+    jd = np.ascontiguousarray(jd, np.int32)
+    hour = np.ascontiguousarray(hour, np.float64)
+    dni = np.ascontiguousarray(dni, np.float32)
+    dhi = np.ascontiguousarray(dhi, np.float32)
+    d, f, r = np.zeros(P), np.zeros(P), np.zeros(P)
+    g = C.c_double()
+    dump = np.zeros(6 * (P + 1), np.float32)
+    S = np.zeros(max(1, P * Tup), np.float32) if colOf is not None else None
+    gS = np.zeros(max(1, Tup), np.float32) if colOf is not None else None
+    col = np.ascontiguousarray(colOf, np.int32) if colOf is not None else None
+    lib().orc_ladybug_sky(mf, T, jd.ctypes.data, hour.ctypes.data, dni.ctypes.data, dhi.ctypes.data, lat, lon, fileTz, grefl, dur,
+                          None if col is None else col.ctypes.data, Tup, None if S is None else S.ctypes.data,
+                          None if gS is None else gS.ctypes.data, d.ctypes.data, f.ctypes.data, r.ctypes.data, C.byref(g), dump.ctypes.data)
+    out = dict(dir=d, dif=f, rad=r, gnd=g.value, dump=dump.reshape(2, P + 1, 3))
+    if S is not None:
+        out["S"] = S[:P * Tup].reshape(P, Tup)
+        out["g"] = gS[:Tup]
+    return out

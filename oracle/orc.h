@@ -71,6 +71,7 @@ typedef struct {
 void orc_bbox_centre(const orc_mesh* m, int nMesh, double c[3]);           /* mesh.cpp:107-118 */
 /* make_sensors_into (mesh.cpp:315-349), physical rules; sides 0 front, 1 back, 2 both. Returns elements. */
 long orc_sensors(const orc_mesh* m, int onVertices, int sides, double offset, const double c[3], float* p, float* n);
+long orc_sensors_mode(const orc_mesh* m, int onVertices, int sides, double offset, int ladybug, const double c[3], float* p, float* n);
 /* triangulate_into (mesh.cpp:137-168) for faces of 3 or 4 corners (n-gons: skipped and counted). Returns triangles. */
 long orc_triangulate(const orc_mesh* m, const double c[3], float* v, uint32_t* tri, uint32_t base, int* skipped);
 
@@ -173,6 +174,7 @@ typedef struct {
   orc_time time;
   int wantSeries;           /* also the FP64 series (Tup columns) */
   int nthreads;
+  int ladybug;              /* --mode ladybug */
 } orc_daylight_opt;
 typedef struct {
   long nElem, nSens, nCols;
@@ -185,7 +187,12 @@ typedef struct {
 /* Sizes of a run before it runs: elements, columns, Tup, directions. 0 ok. */
 int orc_daylight_size(const orc_mesh* m, int nMesh, const float* dni, const float* dhi, long nRows, double wlat, double wlon,
                       double wtz, int leap, const orc_daylight_opt* o, long* nElem, long* nCols, int* Tup, long* nDir);
-/* run_daylight (run.cpp:216-738), physical mode, opaque occluders. Weather arrays may be null (sun hours only). 0 ok. */
+/* Ladybug mode's sky (run.cpp:362-470) over T steps: dir / dif / rad [P], *gnd, dump [2 x 3 (P + 1)] (-s then -d
+ * averages, %.3g), and when S: the series columns S [P x Tup], gS [Tup] of the steps with colOf >= 0. */
+void orc_ladybug_sky(int mf, long T, const int* jd, const double* hour, const float* dni, const float* dhi, double lat, double lon,
+                     double fileTz, double grefl, double dur, const int* colOf, int Tup, float* S, float* gS, double* dir, double* dif,
+                     double* rad, double* gnd, float* dump);
+/* run_daylight (run.cpp:216-738), physical or ladybug mode, opaque occluders. Weather arrays may be null (sun hours only). 0 ok. */
 int orc_daylight(const orc_mesh* m, int nMesh, const float* dni, const float* dhi, long nRows, double wlat, double wlon,
                  double wtz, int leap, const orc_daylight_opt* o, orc_daylight_out* out, char* err, int errLen);
 

@@ -42,11 +42,25 @@ static v3 newell(const orc_mesh* m, const int* idx, int n) {
 
 /* face_geom, physical rules (mesh.cpp:45-71): unit Newell normal (0 when the area is under 1e-12), the signed-area
  * weighted centroid of the fan from corner 0 (the corner mean when that area vanishes) */
-static void face_geom(const orc_mesh* m, const int* idx, int n, v3* nrm, double* area, v3* cen) {
+static void face_geom(const orc_mesh* m, const int* idx, int n, int ladybug, v3* nrm, double* area, v3* cen) {
   v3 zero = {0, 0, 0};
   *nrm = zero; *area = 0; *cen = zero;
   if (n < 3) return;
   const v3 P0 = pt(m, idx[0]);
+  if (ladybug) {   /* mesh.cpp:49-55: the corners' mean; a triangle's cross product; a quad's (0 1 2) + (2 3 1) normals */
+    v3 c = zero, N = zero;
+    for (int i = 0; i < n; ++i) c = vadd(c, pt(m, idx[i]));
+    *cen = vmul(c, 1.0 / n);
+    if (n == 3) { N = vcross(vsub(pt(m, idx[1]), P0), vsub(pt(m, idx[2]), P0)); *area = vlen(N) / 2; }
+    else if (n == 4) {
+      const v3 P1 = pt(m, idx[1]), P2 = pt(m, idx[2]), P3 = pt(m, idx[3]);
+      const v3 n1 = vcross(vsub(P1, P0), vsub(P2, P0)), n2 = vcross(vsub(P3, P2), vsub(P1, P2));
+      N = vadd(n1, n2);
+      *area = (vlen(n1) + vlen(n2)) / 2;
+    }
+    *nrm = *area < 1e-12 ? zero : vunit(N);
+    return;
+  }
   const v3 N = newell(m, idx, n), Nu = vunit(N);
   *area = vlen(N) / 2;
   double sum = 0;
@@ -83,6 +97,12 @@ static void put(float* p, float* n, long* j, v3 q, v3 nr, int sides, double offs
 }
 
 long orc_sensors(const orc_mesh* m, int onVertices, int sides, double offset, const double c[3], float* p, float* n) {
+  return orc_sensors_mode(m, onVertices, sides, offset, 0, c, p, n);
+}
+
+/* make_sensors_into (mesh.cpp:315-349) by the mode's rules; ladybug: a face of more than 4 corners makes nothing (-1) */
+long orc_sensors_mode(const orc_mesh* m, int onVertices, int sides, double offset, int ladybug, const double c[3], float* p, float* n) {
+  for (long f = 0; ladybug && f < m->nFaces; ++f) if (m->counts[f] > 4) return -1;
   const v3 C = {c[0], c[1], c[2]};
   long j = 0, off = 0;
   v3* vn = onVertices ? calloc((size_t)m->nPoints, sizeof(v3)) : NULL;
@@ -92,7 +112,7 @@ long orc_sensors(const orc_mesh* m, int onVertices, int sides, double offset, co
     off += k;
     v3 nrm, cen;
     double area;
-    face_geom(m, idx, k, &nrm, &area, &cen);
+    face_geom(m, idx, k, ladybug, &nrm, &area, &cen);
     if (onVertices) for (int q = 0; q < k; ++q) vn[idx[q]] = vadd(vn[idx[q]], vmul(nrm, area));
     else put(p, n, &j, cen, nrm, sides, offset, C);
   }
