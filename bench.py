@@ -193,7 +193,7 @@ def weather(native, tmp, seed=2):
         native.run("daylight", ["--clear-sky", "--lat", LAT, "--lon", LON, "--tz", TZ, "--quantity", "sunhours",
                                 "--dump-sun", sun], m)
     d = np.loadtxt(sun, comments="#")
-    alt, dni_cs, dhi_cs = d[:, 4], d[:, 7], d[:, 8]
+    alt, dni_cs, dhi_cs = d[:, 4], d[:, 10], d[:, 11]   # the reference's dump columns: ... up, sun.c alt azi up, dni, dhi
     assert len(d) == 8760, len(d)
     c = np.random.default_rng(seed).beta(2, 3, 8760)
     dni = dni_cs * (1 - c)

@@ -89,6 +89,8 @@ def test_dump_sun_radiance_columns(native, box, orc, tmp_path):
         L.orc_sun_radiance(52.52, -13.41, -15.0, 172, r[3], C.byref(a), C.byref(z))
         assert abs(r[7] - np.degrees(a.value)) < 1e-5 and abs(r[8] - np.degrees(z.value)) < 1e-5
         assert int(r[9]) == int(a.value > -0.2665 * np.pi / 180)
+    w = clear_weather(orc)   # the weather columns stay last, as the reference's dump has them
+    assert np.allclose(rows[:, 10], w["dni"][24 * 171:24 * 172], atol=5e-5) and np.allclose(rows[:, 11], w["dhi"][24 * 171:24 * 172], atol=5e-5)
 
 
 def test_ladybug_series_vs_oracle_fp64(native, box, orc, box_meshes, tmp_path, monkeypatch):
