@@ -500,6 +500,7 @@ __global__ __launch_bounds__(64) void k_small(const Task* __restrict__ tasks, co
   }
   __syncthreads();
   uint32_t root = T.slot == kRootSlot ? 0u : *o.rootId;
+  int maxDepth = 0;   // lane 0's deepest node: one atomic per task, not one per node on the one height word
   while (sSp > 0) {
     const int sp0 = sSp - 1;
     const uint32_t rb = sStack[sp0][0], re = sStack[sp0][1], slotP = sStack[sp0][2], depth = sStack[sp0][3];
@@ -550,7 +551,7 @@ __global__ __launch_bounds__(64) void k_small(const Task* __restrict__ tasks, co
       nd[14] = 0.f;
       nd[15] = 0.f;
       if (slotP != kRootSlot) o.nodes[16 * size_t(map_id(slotP >> 1, root)) + 12 + (slotP & 1u)] = __uint_as_float(slot);
-      atomicMax(o.height, int(depth));
+      maxDepth = max(maxDepth, int(depth));
       int spn = sp0;
       for (int s = 1; s >= 0; --s) {   // right pushed first: the left range is taken next
         const uint32_t b = s ? rb + nL : rb, e = s ? re : rb + nL;
@@ -564,6 +565,7 @@ __global__ __launch_bounds__(64) void k_small(const Task* __restrict__ tasks, co
     __syncthreads();
   }
   if (lane < n) o.finalRef[T.b + lane] = sRef[lane];
+  if (lane == 0) atomicMax(o.height, maxDepth);
 }
 
 // ---------------------------------------------------------------------------------------------------- output
