@@ -104,6 +104,11 @@ DL_API int32_t dlx_read_epw(const char* path, float* dni, float* dhi, int64_t ca
  * in build order, reference mesh.cpp:137-168 (Tris::slot); *n triangles, the first min(n, cap) written. */
 DL_API int32_t dlx_thin_solid(const dl_mesh* mesh, int32_t slot, const double* centre, uint32_t* flags);
 DL_API int32_t dlx_triangle_slots(const dl_mesh* mesh, uint32_t slotBase, uint32_t* slots, int64_t cap, int64_t* n);
+/* The triangles the scene build draws from a one-mesh handle (reference mesh.cpp:137-168 triangulate_into: a triangle as
+ * it is, a quad (0 1 2) (0 2 3), 5 or more corners ear-clipped in the face's Newell plane): v[3 nPoints] the points
+ * recentred on centre[3] (NULL: the origin) as floats, tri[3 n] the corners, *n triangles (the first min(n, cap)
+ * written), *skipped the degenerate faces left out. No GPU involved. */
+DL_API int32_t dlx_triangulate(const dl_mesh* mesh, const double* centre, float* v, uint32_t* tri, int64_t cap, int64_t* n, int32_t* skipped);
 /* Ladybug mode's sky, the host emulation of gendaymtx -A the reference runs inline (run.cpp:362-470; no GPU involved),
  * over T steps (jd, hour, dni, dhi) at a site (lat, lon east-positive, fileTz the weather file's zone in hours) with the
  * ground reflectance grefl and the run's hours dur: per patch of sky density mf (1 or 2) dir / dif / rad [P] (kWh/m2 x

@@ -314,6 +314,24 @@ int32_t dlx_triangle_slots(const dl_mesh* mesh, uint32_t slotBase, uint32_t* slo
   });
 }
 
+int32_t dlx_triangulate(const dl_mesh* mesh, const double* centre, float* v, uint32_t* tri, int64_t cap, int64_t* n, int32_t* skipped) {
+  return guard([&]() -> int32_t {
+    if (!mesh || !n || mesh->meshes.size() != 1) return fail(DL_USAGE, "dlx_triangulate: one mesh and n");
+    const SimulationMesh& m = mesh->meshes[0];
+    std::vector<float3> vv(m.numPoints());
+    std::vector<uint3> tt(tri_cap(m));
+    const double zero[3] = {0, 0, 0};
+    int sk = 0;
+    *n = int64_t(triangulate_into(m, centre ? centre : zero, vv.data(), tt.data(), nullptr, 0, 0, sk));
+    if (skipped) *skipped = sk;
+    for (size_t i = 0; v && i < vv.size(); ++i) { v[3 * i] = vv[i].x; v[3 * i + 1] = vv[i].y; v[3 * i + 2] = vv[i].z; }
+    for (int64_t k = 0; tri && k < *n && k < cap; ++k) {
+      tri[3 * k] = tt[size_t(k)].x; tri[3 * k + 1] = tt[size_t(k)].y; tri[3 * k + 2] = tt[size_t(k)].z;
+    }
+    return DL_OK;
+  });
+}
+
 int32_t dlx_ladybug_sky(int32_t mf, int64_t T, const int32_t* jd, const double* hour, const float* dni, const float* dhi, double lat,
                         double lon, double fileTz, double grefl, double dur, double* dir, double* dif, double* rad, double* gnd, float* dump) {
   return guard([&]() -> int32_t {

@@ -152,3 +152,35 @@ def test_bad_faces_fail_without_reading_out_of_range(gpu):
     gpu.mesh_begin(4, 2, False, np.zeros(3))
     assert gpu.mesh_add(P + 1, [4], [0, 1, 2, 3]) == 2
     gpu.mesh_build()
+
+
+def test_ngon_context_run_equals_its_triangles(orc):
+    """A run whose context has faces of 5 to 12 corners (ear-clipped on the host, placed by the GPU ingest) gives the
+    values of the same run over those faces' triangles given as triangles (dlx_triangulate): the same scene, bit for
+    bit."""
+    import os
+    import native
+    from test_ngon_cpu import triangulate
+    rng = np.random.default_rng(11)
+    pts, counts, idx = [], [], []
+    for k in range(60):   # star-shaped roofs over the box, 5 to 12 corners
+        n = int(rng.integers(5, 13))
+        ang = (np.arange(n) + rng.uniform(0, 0.8, n)) * (2 * np.pi / n)   # gaps under 180 degrees: star-shaped, simple
+        r = rng.uniform(0.2, 0.8, n)
+        c = rng.uniform(-4, 4, 2)
+        z = rng.uniform(2.5, 6.0)
+        base = len(pts)
+        pts += [[c[0] + r[j] * np.cos(ang[j]), c[1] + r[j] * np.sin(ang[j]), z] for j in range(n)]
+        counts.append(n)
+        idx += list(range(base, base + n))
+    V, T, sk = triangulate(pts, counts, idx)
+    assert sk == 0 and len(T) == sum(c - 2 for c in counts)
+    box = native.SimulationMesh.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "box.obj"))
+    args = ["--clear-sky", "--lat", "52.52", "--lon", "13.41", "--tz", "1", "--quantity", "radiation,sunhours"]
+    with native.SimulationMesh("roofs", pts, counts, idx) as ng, native.SimulationMesh("roofs", pts, [3] * len(T), T.ravel()) as tr:
+        a = native.run("daylight", args, box, context=[ng])
+        b = native.run("daylight", args, box, context=[tr])
+    box.close()
+    assert a.values.shape == b.values.shape
+    assert np.array_equal(a.values, b.values)
+    assert a.stats["rays"] == b.stats["rays"]
