@@ -200,7 +200,8 @@ struct ByteRay {
 // bytes in quads: word (32 w + 4 q) / 4 of sensor i at vis[(8 w + q) nc + i], direction 32 w + 4 q + j in byte j; every
 // quad of the word is written (0 beyond the last direction)
 template <int STACK, bool OFF32>
-__global__ __launch_bounds__(kWave) void k_bytes(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
+// 4 waves per SIMD where the stack's LDS allows (128 VGPRs, 8 B spilled): 9 % faster than 137 VGPRs at 3
+__global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                 const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                 const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull,
                                                 const float4* __restrict__ slots, uint32_t slotLast, uint32_t* __restrict__ vis) {
