@@ -122,12 +122,15 @@ struct SumArgs {
   double* out;
 };
 
+// NC >= the run's classes: the per-class sums live in registers, so the common runs (sky, ground, obstruction and a
+// target group or two) get a small NC and the occupancy its few registers allow
+template <uint32_t NC>
 __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
   const uint32_t lane = threadIdx.x & (kWave - 1), i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (i >= a.nc) return;   // whole waves
-  double w[kClasses], wl[kClasses], rc[8], own = 0, gz = 0;
-  float mn[kClasses];
-  for (uint32_t c = 0; c < kClasses; ++c) { w[c] = 0; wl[c] = 0; mn[c] = __int_as_float(0x7f800000); }
+  double w[NC], wl[NC], rc[8], own = 0, gz = 0;
+  float mn[NC];
+  for (uint32_t c = 0; c < NC; ++c) { w[c] = 0; wl[c] = 0; mn[c] = __int_as_float(0x7f800000); }
   for (uint32_t r = 0; r < 8; ++r) rc[r] = 0;
   for (uint32_t d = lane; d < a.K; d += kWave) {
     uint32_t c = a.cls[size_t(i) * a.K + d];
@@ -139,14 +142,14 @@ __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
     const double ld = dd == a.farF ? a.lnFar : dd == 0.1f ? -2.302585092994045684 : log((double)dd);
     // registers indexed by class: a small switch keeps them out of scratch
 #pragma unroll
-    for (uint32_t q = 0; q < kClasses; ++q)
+    for (uint32_t q = 0; q < NC; ++q)
       if (q == c) { w[q] += wd; wl[q] += wd * ld; mn[q] = fminf(mn[q], dd); }
 #pragma unroll
     for (uint32_t r = 0; r < 8; ++r) if (r < a.nReach && t >= a.reach[r]) rc[r] += wd;
   }
   double* o = a.out + size_t(i) * a.NA;
 #pragma unroll
-  for (uint32_t c = 0; c < kClasses; ++c) {
+  for (uint32_t c = 0; c < NC; ++c) {
     if (c >= a.nClass) break;
     const double s0 = tree_sum(w[c]), s1 = tree_sum(wl[c]);
     const float m = tree_min(mn[c]);
@@ -286,7 +289,10 @@ extern "C" int32_t dlg_reduce_view(dlg_gpu* g, int64_t nc, const double* wRow, u
   for (uint32_t r = 0; r < 8; ++r) s.reach[r] = r < nReach ? reach[r] : 0.f;
   s.out = g->out.as<double>();
   dlg_tbegin(g);
-  k_view_sums<<<unsigned((nc + 3) / 4), 256, 0, g->stream>>>(s);
+  const unsigned grid = unsigned((nc + 3) / 4);
+  if (nClass <= 4) k_view_sums<4><<<grid, 256, 0, g->stream>>>(s);
+  else if (nClass <= 8) k_view_sums<8><<<grid, 256, 0, g->stream>>>(s);
+  else k_view_sums<kClasses><<<grid, 256, 0, g->stream>>>(s);
   if (isoDim) {
     IsoArgs a;
     a.cls = s.cls; a.dist = s.dist; a.rowSin = reinterpret_cast<const float*>(b + oSin);
