@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The command-line contract of both tools (reference src/cli.h): flag parsing, the weather / time rules after parsing,
 // and the output manifests. Return codes are the exit codes: 0 ok, 1 usage error, 2 runtime error.
 #pragma once

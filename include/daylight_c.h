@@ -1,4 +1,4 @@
-/* SPDX-License-Identifier: Apache-2.0 */
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * daylight_c.h - the C ABI of the MI355X build: the daylight and views tools in-process, with the same symbols, types,
  * constants and semantics as reference src/daylight_c.h (what bindings/python, bindings/csharp and bindings/ts bind), so

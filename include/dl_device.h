@@ -1,4 +1,4 @@
-/* SPDX-License-Identifier: Apache-2.0 */
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * dl_device.h - the GPU seam of the MI355X build: plain C entry points over HIP, with host pointers and sizes only.
  *

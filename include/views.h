@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // views: view analysis on the daylight library, on MI355X. The declarations of reference src/views.h (same types,
 // defaults and functions); the rays and reductions run as HIP kernels (dl_device.h).
 #pragma once

@@ -1,3 +1,4 @@
+# SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 """Builds the MI355X library in-tree: kit-sim_daylight_amd/lib/libdaylight_amd.so (HIP kernels for gfx950 + the C++ host
 layer + the C ABI) and the two command-line tools lib/daylight and lib/views.
 

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Reflected light on CDNA4: replaces __raygen__bounce (reference src/trace_dev.cu:260-311), gpu_bounce (gpu.cu:533-568),
 // gpu_reduce_dc (gpu.cu:627-637) and gpu_download_dc (gpu.cu:700-705).
 //   k_bounce     a wave per sensor of the chunk walks its M samples, 64 at a time, in order. Sample s leaves the sensor

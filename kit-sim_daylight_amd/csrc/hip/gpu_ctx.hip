@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The device context of dl_device.h: one stream (default priority) and two timing events on device 0, resident sensors
 // and directions, scene residency. Replaces the OptiX context / pipeline / SBT host code of reference src/gpu.cu:277-497.
 #include <cstring>

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Mesh ingest on the GPU: the scene's points recentred and rounded to float, its faces fanned into triangles, where the
 // scene is then built. The host's triangulate_into (reference mesh.cpp:137-168, called per mesh from run.cpp:120-147)
 // restated as three passes per mesh: corner counts and triangles per face, two exclusive scans (each face's first

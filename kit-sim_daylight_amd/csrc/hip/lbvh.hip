@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The occluder acceleration structure on the GPU: a linear BVH (Karras 2012, "Maximizing parallelism in the construction
 // of BVHs, octrees and k-d trees"). Replaces the OptiX GAS build of reference src/gpu.cu:386-457 (RT cores have no
 // CDNA4 counterpart; the traversal kernels read this layout).

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The cumulative reduces of a daylight run in one pass over the visibility: replaces k_reduce<Bits> / k_reduce<Bytes> and
 // gpu_reduce (reference src/gpu.cu:76-114, 612-625), which ran once per quantity and re-read the visibility each time.
 // Each sensor's sum is the reference's: acc_q += double(float(max(0, dotf(n, D_d)) x T)) x double(W[q][d]) over its

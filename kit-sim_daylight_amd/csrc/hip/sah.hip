@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The default acceleration structure: a binned-SAH tree (16 centroid bins per axis, surface-area heuristic down to
 // single-triangle leaves), built top-down on the GPU over the Morton-ordered leaves of lbvh.hip and stored in the LBVH's
 // node layout, so that the 4-wide records (wide.h) and every traversal are shared. Replaces, as lbvh.hip does, the OptiX

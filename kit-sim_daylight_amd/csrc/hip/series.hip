@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The time-series contraction E = A S + G g + sun of a daylight chunk: replaces k_series / gpu_series (reference
 // src/gpu.cu:119-149, 639-669). A[i, d] = max(0, dotf(n_i, D_d)) wsky_d on the visible sky directions (the daylight
 // coefficients, exact 0 where a direction is blocked; byte mode: x the transmittance byte / 255.f, gpu.cu:83-92), S[row_d,

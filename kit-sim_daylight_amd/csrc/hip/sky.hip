@@ -1,4 +1,7 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
+// Radiance notice: the Perez tables, tcos and glass formulas here follow Radiance 5.4 (gendaymtx.c, tcos.c, m_glass.c),
+// Copyright (c) 1990 to 2022 The Regents of the University of California, through LBNL; Radiance Software License 2.0,
+// reproduced with its conditions and disclaimer in NOTICES.md.
 // The sky of a daylight run on the GPU in FP64: replaces the host loops of reference src/run.cpp:311-446 (sun_noaa per
 // step, gendaymtx_step per step, the cumulative per-patch weights) with kernels over steps and (step, patch).
 //   k_steps   a thread per step: sun position (NOAA, Ladybug conventions), Perez parameters, ground row, the sun's

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Visibility bits on CDNA4: replaces OptiX __raygen__bits (reference src/trace_dev.cu:91-108), byte mode's
 // __raygen__bytes + __anyhit__tr (:115-165) and gpu_trace's launch loop (src/gpu.cu:499-531). No RT cores: a software traversal per ray of the 4-wide records built from the LBVH (lbvh.hip,
 // wide.h), each visit two levels of the binary tree.

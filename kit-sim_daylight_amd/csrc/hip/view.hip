@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Views on CDNA4: replaces __raygen__view (reference src/trace_dev.cu:177-232), gpu_view (gpu.cu:572-610), k_view /
 // k_iso / gpu_reduce_view (gpu.cu:171-274, 671-697).
 //   k_view_rays  a wave per sensor and 8 x 8 tile of cells (grid y, 4 tiles a block): frame, jittered cell, virtual ground,

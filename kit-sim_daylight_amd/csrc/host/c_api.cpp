@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The C ABI of include/daylight_c.h (reference src/c_api.cpp's contract): the CLI's own parser, rules and manifests on
 // argv, meshes as immutable handles with never-reused serials (the resident scene's identity), one lock around the GPU
 // runs, a per-thread error buffer, and no exception across the boundary.

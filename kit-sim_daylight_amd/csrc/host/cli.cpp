@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The command-line contract of both tools (reference src/cli.cpp): the same flags, values, defaults, checks, exit codes
 // and manifest JSON, so scripts and the C ABI see the reference's behaviour. The manifest layout has to be alike byte
 // for byte (consumers parse it; the reference's C ABI test compares it with the CLI's).

@@ -1,4 +1,7 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
+// Radiance notice: the Perez tables, tcos and glass formulas here follow Radiance 5.4 (gendaymtx.c, tcos.c, m_glass.c),
+// Copyright (c) 1990 to 2022 The Regents of the University of California, through LBNL; Radiance Software License 2.0,
+// reproduced with its conditions and disclaimer in NOTICES.md.
 // Ladybug mode's sky (reference run.cpp:362-470): gendaymtx -A as Ladybug's RadiationStudy runs it, computed on the host
 // in double with the host's libm, since the emulation is exact only where its operations are the reference's: per step
 // the sun of Radiance's sun.c at the site as Ladybug writes it into the .wea (latitude and west longitude %.2f, the

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The command-line tools (reference src/main.cpp): `daylight` (meshes + place + time in, one value row per element out)
 // and, built with VIEWS_CLI, `views` (meshes + field of view in). Flags, rules and manifests are cli.cpp's; the runs are
 // daylight::run / views::run on the GPU. Outputs: stdout or --out FILE.txt ("%.10g", one row per element, columns

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Mesh inputs of the path (reference src/mesh.cpp): OBJ / .dmesh readers, the occluder triangles (quads split
 // (0,1,2)/(0,2,3); n-gons by ear clipping in the polygon's plane), sensor points and normals by the physical rules
 // (Newell normal, signed-area fan centroid, area-weighted vertex normals, sides, offset, recentring), and the option rows

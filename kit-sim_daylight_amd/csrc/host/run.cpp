@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // daylight::run (reference src/run.cpp:216-758): validation, clocks, steps, the sky of every step on the GPU (dlg_sky),
 // the direction table and per-quantity weights, the resident scene (LBVH in HBM, keyed), the sensors, and the chunk
 // loop: trace -> every quantity reduced in one pass -> (series) the contraction on MFMA, streamed to .npy.

@@ -1,4 +1,7 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
+// Radiance notice: the Perez tables, tcos and glass formulas here follow Radiance 5.4 (gendaymtx.c, tcos.c, m_glass.c),
+// Copyright (c) 1990 to 2022 The Regents of the University of California, through LBNL; Radiance Software License 2.0,
+// reproduced with its conditions and disclaimer in NOTICES.md.
 // Host inputs of the sky and the sun: the Reinhart patch table and its samples (reference sky.cpp:240-294), Radiance's
 // tcos table, evaluation steps, timestamps, the EPW reader with its PVGIS realignment, Spencer / Ineichen clear sky
 // (reference solar.cpp). The per-step sun and Perez sky of a run are computed on the GPU (sky.hip); clear_sky_weather,

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // views::field / views::run (reference src/views.cpp): the field and its checks, the cell weights, the target labels on
 // the shared resident scene, the chunk loop over the GPU (dlg_view -> dlg_reduce_view), the columns and --dump-rays.
 // With --transmission on, rays pass glass slots (label 255, at most 16 panes) and count toward the glazing column.

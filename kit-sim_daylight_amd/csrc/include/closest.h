@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The closest hit of a ray through the 4-wide records (wide.h), for the kernels that walk closest hits: views' rays
 // (view.hip) and the paths of reflected light (bounce.hip). Exact ties of t go to the lower triangle index.
 #pragma once

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Internal contracts of the host layer (the reference's src/internal.h, minus its OptiX seam, which dl_device.h
 // replaces). The host prepares inputs (steps, patches, directions, triangles, sensors) and orchestrates the device;
 // every per-ray, per-sensor or per-(step, patch) loop of the hot path runs on the GPU.

@@ -1,4 +1,7 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
+// Radiance notice: the Perez tables, tcos and glass formulas here follow Radiance 5.4 (gendaymtx.c, tcos.c, m_glass.c),
+// Copyright (c) 1990 to 2022 The Regents of the University of California, through LBNL; Radiance Software License 2.0,
+// reproduced with its conditions and disclaimer in NOTICES.md.
 // The per-ray and per-step arithmetic of the trace-and-accumulate path, as __host__ __device__ functions: the HIP kernels
 // run them, and host code that must agree with a kernel bit for bit (the scene's padding, the sun table) calls the same
 // functions. Everything here is compiled without floating-point contraction (-ffp-contract=off); fma is written out where

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The device state behind dl_device.h, shared by the HIP translation units of the library.
 #pragma once
 #include <hip/hip_runtime.h>

@@ -1,4 +1,4 @@
-// SPDX-License-Identifier: Apache-2.0
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // The 4-wide records the traversal kernels walk (built from the LBVH by k_wide, lbvh.hip).
 //
 // Record n, one per LBVH inner node: the boxes and references of n's children, an inner child replaced by its own two

@@ -1,3 +1,4 @@
+# SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 """ctypes binding of the MI355X library (lib/libdaylight_amd.so): the public C ABI of include/daylight_c.h (the binding the
 reference's bindings/python/daylight/_native.py makes over daylight_c.dll) and the GPU seam of include/dl_device.h.
 

@@ -1,3 +1,4 @@
+# SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 """The daylight and views tools in-process through libdaylight_amd.so: the interface of the reference's Python binding
 (bindings/python/daylight/native.py: SimulationMesh, Control, Result, run, usage; same names, argument meaning, errors
 and stats keys) over this build's C ABI (include/daylight_c.h)::

@@ -1,3 +1,4 @@
+# SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 """ctypes access to the oracle (liborc.so) and, where it was built, the reference's own sky / sun model (_ref/libdlref.so).
 
 Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this module, as the checker

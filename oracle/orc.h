@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle: a plain-C, FP64-where-the-reference-is restatement of daylightcore's trace-and-accumulate path.
  *

@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle (test infrastructure only): the accumulation kernels of reference src/gpu.cu restated on the CPU.
  *  - k_reduce<Bits> / k_reduce<Bytes> (gpu.cu:76-114): double sums in ascending direction order of float products

@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle (test infrastructure only): reference src/mesh.cpp restated in C for the path's inputs: the bounding-box
  * centre every coordinate is recentred by, sensor points/normals by the physical rules, and the occluder triangles of

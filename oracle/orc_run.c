@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle (test infrastructure only): run_daylight of reference src/run.cpp:216-738 restated for the physical mode
  * and the ladybug mode with opaque occluders (no transmission, no reflected light, no --no-occlude): steps, suns,

@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle (test infrastructure only): the sky model of reference src/sky.cpp restated in C. That file ports Radiance
  * rad5R4 gendaymtx.c / sun.c / tcos.c (Radiance Software License 2.0, LBNL); the Perez coefficient tables below are the

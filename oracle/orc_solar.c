@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle (test infrastructure only): reference src/solar.cpp restated in C: the NOAA/Meeus solar position with the
  * Ladybug Sunpath conventions (fixed year, minute resolution, 0.01-day rounding, NOAA refraction), Spencer's

@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 /*
  * Oracle (test infrastructure only): the ray semantics of reference src/trace_dev.cu (__raygen__bits, __raygen__view)
  * and src/params.h restated on the CPU, over a host LBVH (Karras 2012) whose build is the one the GPU build must

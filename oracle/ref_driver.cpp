@@ -1,3 +1,4 @@
+/* SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0 */
 // Oracle build helper (test infrastructure only): a C entry point per reference host function of the sky / sun model,
 // compiled together with the reference's own src/sky.cpp and src/solar.cpp (read in place, never copied) into
 // _ref/libdlref.so. Tests call these beside the C restatement (liborc.so) to pin it bit for bit.
