@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
+# SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 """Throughput of the daylight trace-and-accumulate path on one MI355X: sensor-hours/s of an annual run with its series.
 
-Workload (seeded urban block): 400 extruded boxes (footprint sides U(10, 40) m, heights lognormal with median 15 m and
-sigma 0.5) on a 20 x 20 grid of 50 m plots are the context; the roofs and facades of the 40 buildings nearest the centre,
-cut into quads of at most 1 m, are the sensors (they sit on the context boxes' faces and do not block themselves,
---no-self-occlude). Weather: the library's own clear sky (Ineichen, turbidity 3) at lat 52.52 lon 13.41 tz 1, with an
-hourly cloud cover c ~ Beta(2, 3) taking (1 - c) of the beam and moving half of what it took into the diffuse part,
-written to an EPW. Run: --sky-density 2 (577 patches), 8760 hourly steps, radiation plus its --timeseries; the series
-stays on the GPU (dlx_run_series_device), file output is excluded from the timing.
+Headline workload (the batched precision-form run): the seeded 3M-triangle city (250,000 extruded boxes on a 500 x 500
+grid of 50 m plots, footprint sides U(10, 40) m, heights lognormal with median 15 m and sigma 0.5) as context, and three
+sensor meshes of 1,000 / 150,000 / 850,000 faces (the roofs and facades of the central buildings in 1 m quads:
+1,001,000 sensors) in ONE run, so one launch of each kernel covers all three. Weather: the library's clear sky
+(Ineichen, turbidity 3) at lat 52.52 lon 13.41 tz 1 with an hourly cloud cover c ~ Beta(2, 3), written to an EPW. Run:
+--sky-density 4 (2305 patches), 8760 hourly steps, radiation, direct, diffuse and illuminance, plus the radiometric
+hourly series (1,001,000 x 4,456 non-night steps) by the BF16x6 contraction on MFMA, kept on the GPU
+(dlx_run_series_device; file output excluded from the timing). The sensors cast no shade of their own
+(--no-self-occlude): they lie on the context boxes' faces.
 
 A step is one such run through the C ABI with the context scene already resident on the GPU: the sky and sun of every
-step (FP64), the visibility traversal, the per-sensor reductions and the series contraction. The timed region holds
-exactly --steps runs, bracketed by torch.cuda.synchronize(). The line also carries the traversal's roofline (HIP-event
-times of its launches, counted node and triangle fetches), the three contraction methods side by side (FP32, BF16x3,
-BF16x6) and the oracle's CPU time on a bounded sample of the same workload.
+step (FP64), the visibility traversal, the per-sensor reductions and the series contraction, plus the host work of a
+run. The timed region holds exactly --steps runs, bracketed by torch.cuda.synchronize().
+
+The line also carries: the contraction methods side by side on the same workload (FP32 = the reference's arithmetic,
+BF16x3, BF16x6) with their series error; the per-column error against the oracle's FP64 run of a sensor sample; the
+traversal's roofline (HBM bytes from the committed rocprofv3 counters over the live launch time, and its limiter) and the
+contraction's MFMA roofline (executed and algorithmic flops over the live launch time); traced and slot Grays/s; the
+urban block (577 patches), the interior views and the byte-mode legs; the reference's own --bench commands through
+this build's CLI; and the oracle's CPU time on a bounded sample of the headline workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -21,6 +29,7 @@ import argparse
 import json
 import math
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -30,288 +39,225 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "kit-sim_daylight_amd"))
 
-LAT, LON, TZ = 52.52, 13.41, 1.0
+import workloads as W  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
 NODE_BYTES, TRI_BYTES = 112, 48   # a 4-wide record visit loads 7 x 16 bytes; a triangle 3 x 16
+PROFILES = os.path.join(ROOT, "profiles")
 
 
-# ---------------------------------------------------------------- synthetic inputs
-
-def city(seed=1, n_side=20, pitch=50.0, n_sensor_buildings=40):
-    """Boxes (x0, y0, x1, y1, h) of the block and the indices of the central sensor buildings."""
-    rng = np.random.default_rng(seed)
-    n = n_side * n_side
-    a = rng.uniform(10, 40, n)
-    b = rng.uniform(10, 40, n)
-    h = rng.lognormal(math.log(15.0), 0.5, n)
-    gx, gy = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")
-    cx = (gx.ravel() + 0.5 - n_side / 2) * pitch
-    cy = (gy.ravel() + 0.5 - n_side / 2) * pitch
-    # a 5 m street at least: the box moves within its plot
-    cx = cx + rng.uniform(-1, 1, n) * (pitch - 5 - a) / 2
-    cy = cy + rng.uniform(-1, 1, n) * (pitch - 5 - b) / 2
-    boxes = np.stack([cx - a / 2, cy - b / 2, cx + a / 2, cy + b / 2, h], 1)
-    central = np.argsort(np.hypot(cx, cy), kind="stable")[:n_sensor_buildings]
-    return boxes, np.sort(central)
+def rel_err(x, want):
+    """max |x - want| / max(|want|, 1e-3 max |want|) over the sample (the validation scale rule, reference
+    validate/README.md:160)"""
+    want = np.asarray(want, np.float64)
+    scale = np.maximum(np.abs(want), 1e-3 * np.abs(want).max())
+    return float(np.max(np.abs(np.asarray(x, np.float64) - want) / np.where(scale > 0, scale, 1.0)))
 
 
-def box_mesh(boxes):
-    """The boxes as quads (bottom, top, 4 sides), outward normals."""
-    pts, idx = [], []
-    for k, (x0, y0, x1, y1, h) in enumerate(boxes):
-        v = np.array([[x0, y0, 0], [x1, y0, 0], [x1, y1, 0], [x0, y1, 0], [x0, y0, h], [x1, y0, h], [x1, y1, h], [x0, y1, h]])
-        pts.append(v)
-        q = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]]) + 8 * k
-        idx.append(q)
-    pts, idx = np.concatenate(pts), np.concatenate(idx)
-    return pts, np.full(len(idx), 4, np.int32), idx.ravel().astype(np.int32)
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return model, n
 
 
-def grid_quads(origin, u, v, nu, nv):
-    """nu x nv quads over the parallelogram origin + s u + t v (s, t in [0, 1]); normal u x v."""
-    s = np.linspace(0, 1, nu + 1)
-    t = np.linspace(0, 1, nv + 1)
-    S, T = np.meshgrid(s, t, indexing="ij")
-    P = origin + S[..., None] * u + T[..., None] * v
-    pts = P.reshape(-1, 3)
-    i = np.arange(nu)[:, None]
-    j = np.arange(nv)[None, :]
-    a = i * (nv + 1) + j
-    quads = np.stack([a, a + nv + 1, a + nv + 2, a + 1], -1).reshape(-1, 4)
-    return pts, quads
+def profile_value(name, key):
+    """A 'key: value' line of a committed profile summary, or None."""
+    path = os.path.join(PROFILES, name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1].split()[0])
+    return None
 
 
-def sensor_mesh(boxes):
-    """Roofs and facades of the boxes in quads of at most 1 m, facing out."""
-    pts, idx, base = [], [], 0
-    for x0, y0, x1, y1, h in boxes:
-        a, b = x1 - x0, y1 - y0
-        na, nb, nh = int(math.ceil(a)), int(math.ceil(b)), int(math.ceil(h))
-        faces = [((x0, y0, h), (a, 0, 0), (0, b, 0), na, nb),           # roof, +z
-                 ((x0, y0, 0), (a, 0, 0), (0, 0, h), na, nh),           # y0 side, -y
-                 ((x1, y0, 0), (0, b, 0), (0, 0, h), nb, nh),           # x1 side, +x
-                 ((x1, y1, 0), (-a, 0, 0), (0, 0, h), na, nh),          # y1 side, +y
-                 ((x0, y1, 0), (0, -b, 0), (0, 0, h), nb, nh)]          # x0 side, -x
-        for o, u, v, nu, nv in faces:
-            p, q = grid_quads(np.array(o, float), np.array(u, float), np.array(v, float), nu, nv)
-            pts.append(p)
-            idx.append(q + base)
-            base += len(p)
-    pts, idx = np.concatenate(pts), np.concatenate(idx)
-    return pts, np.full(len(idx), 4, np.int32), idx.ravel().astype(np.int32)
+_hip = None
 
 
-def interior(seed=1, side=158.0, wall_h=3.0, tri_budget=2_000_000):
-    """The seeded interior: a rectilinear plan of rooms with sides U(3, 8) m, walls 3 m high tessellated into 0.25 m
-    quads with a door opening (0.9-1.5 m wide, 2.1 m high) in 70 % of the wall segments, no ceiling, and clutter boxes
-    with sides U(0.3, 2) m filling the rest of the triangle budget. -> (points, counts, indices) as quads."""
-    rng = np.random.default_rng(seed)
-
-    def cuts():
-        c = [0.0]
-        while c[-1] < side:
-            c.append(min(side, c[-1] + rng.uniform(3, 8)))
-        return np.array(c)
-
-    xs, ys = cuts(), cuts()
-    pts, quads = [], []
-    base = 0
-
-    def wall(p0, p1, lo, hi):   # vertical rectangle from p0 to p1 (xy), z in [lo, hi], 0.25 m quads
-        nonlocal base
-        length = math.hypot(p1[0] - p0[0], p1[1] - p0[1])
-        nu, nv = max(1, int(math.ceil(length / 0.25))), max(1, int(math.ceil((hi - lo) / 0.25)))
-        p, q = grid_quads(np.array([p0[0], p0[1], lo]), np.array([p1[0] - p0[0], p1[1] - p0[1], 0.0]), np.array([0, 0, hi - lo]), nu, nv)
-        pts.append(p)
-        quads.append(q + base)
-        base += len(p)
-
-    def segment(a, b, axis, at):   # a wall segment along x (axis 0) or y (axis 1) at the other coordinate `at`
-        def xy(u):
-            return (u, at) if axis == 0 else (at, u)
-        if rng.uniform() < 0.7 and b - a > 2.0:
-            w = rng.uniform(0.9, 1.5)
-            d0 = rng.uniform(a + 0.3, b - 0.3 - w)
-            wall(xy(a), xy(d0), 0.0, wall_h)
-            wall(xy(d0 + w), xy(b), 0.0, wall_h)
-            wall(xy(d0), xy(d0 + w), 2.1, wall_h)
-        else:
-            wall(xy(a), xy(b), 0.0, wall_h)
-
-    for y in ys:
-        for a, b in zip(xs[:-1], xs[1:]):
-            segment(a, b, 0, y)
-    for x in xs:
-        for a, b in zip(ys[:-1], ys[1:]):
-            segment(a, b, 1, x)
-    nq = sum(len(q) for q in quads)
-    nbox = max(0, (tri_budget - 2 * nq) // 12)
-    sz = rng.uniform(0.3, 2.0, (nbox, 3))
-    c = rng.uniform(0, side, (nbox, 2))
-    boxes = np.stack([c[:, 0] - sz[:, 0] / 2, c[:, 1] - sz[:, 1] / 2, c[:, 0] + sz[:, 0] / 2, c[:, 1] + sz[:, 1] / 2, sz[:, 2]], 1)
-    bp, bc, bi = box_mesh(boxes)
-    pts.append(bp)
-    quads.append(bi.reshape(-1, 4) + base)
-    P = np.concatenate(pts)
-    Q = np.concatenate(quads)
-    return P, np.full(len(Q), 4, np.int32), Q.ravel().astype(np.int32)
+def device_rows(ptr, rows, cols):
+    import ctypes as C
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    buf = np.empty((rows, cols), np.float32)
+    if _hip.hipMemcpy(buf.ctypes.data, C.c_void_p(ptr), buf.nbytes, 2):
+        raise RuntimeError("hipMemcpy of the series failed")
+    return buf
 
 
-def floor_grid(side=158.0, step=0.5, n=100_000):
-    """Sensor quads of `step` m on the floor (z = 0, facing up), n of them from the plan's corner row by row."""
-    per = int(side / step)
-    rows = (n + per - 1) // per
-    p, q = grid_quads(np.array([0.0, 0.0, 0.0]), np.array([per * step, 0, 0]), np.array([0, rows * step, 0]), per, rows)
-    # grid_quads runs the first axis slowest; keep the first n quads
-    q = q[:n]
-    return p, np.full(len(q), 4, np.int32), q.ravel().astype(np.int32)
+def series_of(native, args, sensors, context, rows):
+    """A run with the series on the device -> (Result, its first `rows` series rows)."""
+    got = []
+
+    def sink(row0, n, cols, ptr):
+        take = min(n, rows - row0)
+        if take > 0:
+            got.append(device_rows(ptr, take, cols))
+        return 0
+
+    r = native.run_series_device(args, sensors=sensors, context=context, sink=sink)
+    return r, np.concatenate(got)
 
 
-def write_epw(path, dni, dhi):
-    rows = ["LOCATION,Synthetic,-,DEU,synthetic,000000,%.2f,%.2f,%.1f,0.0" % (LAT, LON, TZ),
-            "DESIGN CONDITIONS,0", "TYPICAL/EXTREME PERIODS,0", "GROUND TEMPERATURES,0",
-            "HOLIDAYS/DAYLIGHT SAVINGS,No,0,0,0", "COMMENTS 1,clear sky x Beta(2 3) cloud cover",
-            "COMMENTS 2,", "DATA PERIODS,1,1,Data,Sunday, 1/ 1,12/31"]
-    days = [31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31]
-    r = 0
-    for m, nd in enumerate(days, 1):
-        for d in range(1, nd + 1):
-            for hr in range(1, 25):
-                ghi = dni[r] + dhi[r]
-                f = [1999, m, d, hr, 0, "?", 10.0, 5.0, 70, 101325, 0, 0, 300, "%.3f" % ghi, "%.3f" % dni[r], "%.3f" % dhi[r]]
-                f += [0] * 19
-                rows.append(",".join(str(x) for x in f))
-                r += 1
-    with open(path, "w") as fh:
-        fh.write("\n".join(rows) + "\n")
+def oracle_run(meshes, dni, dhi, nthreads, **kw):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.build()
+    wx = dict(lat=W.LAT, lon=W.LON, tz=W.TZ, leap=False, dni=dni, dhi=dhi)
+    return orc.daylight([orc.MeshArrays(n, *m, sensor=s) for n, m, s in meshes], weather=wx, selfOcclude=False, nthreads=nthreads, **kw)
 
 
-def weather(native, tmp, seed=2):
-    """Clear-sky DNI / DHI from the library (--clear-sky, --dump-sun), cloud-modulated; -> (dni, dhi) [8760] float32."""
-    sun = os.path.join(tmp, "sun.txt")
-    with native.SimulationMesh("probe", [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], [4], [0, 1, 2, 3]) as m:
-        native.run("daylight", ["--clear-sky", "--lat", LAT, "--lon", LON, "--tz", TZ, "--quantity", "sunhours",
-                                "--dump-sun", sun], m)
-    d = np.loadtxt(sun, comments="#")
-    alt, dni_cs, dhi_cs = d[:, 4], d[:, 10], d[:, 11]   # the reference's dump columns: ... up, sun.c alt azi up, dni, dhi
-    assert len(d) == 8760, len(d)
-    c = np.random.default_rng(seed).beta(2, 3, 8760)
-    dni = dni_cs * (1 - c)
-    dhi = dhi_cs + 0.5 * c * dni_cs * np.maximum(0, np.sin(np.radians(alt)))
-    # multiples of 1/8 W/m2: exact in binary and in the EPW's decimal text, so the library (reading the file) and the
-    # oracle (given the arrays) see the same numbers
-    return (np.round(dni * 8) / 8).astype(np.float32), (np.round(dhi * 8) / 8).astype(np.float32)
+def set_method(name):
+    if name is None:
+        os.environ.pop("DAYLIGHT_SERIES", None)
+    else:
+        os.environ["DAYLIGHT_SERIES"] = name
+
+
+def timed_runs(native, torch, fn, n):
+    torch.cuda.synchronize()
+    c0 = native.gpu_counters()
+    t0 = time.perf_counter()
+    lat = []
+    for _ in range(n):
+        ts = time.perf_counter()
+        r = fn()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = native.gpu_counters()
+    return r, wall, {k: c1[k] - c0[k] for k in c1}, lat
 
 
 # ---------------------------------------------------------------- the CPU baseline (the oracle, on a bounded sample)
 
-def cpu_baseline(ctx, sens, dni, dhi, target_s=15.0):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-    orc.build()
-    threads = max(1, min(16, os.cpu_count() or 1))
-    wx = dict(lat=LAT, lon=LON, tz=TZ, leap=False, dni=dni, dhi=dhi)
-    ctx_m = orc.MeshArrays("city", ctx[0], ctx[1], ctx[2], sensor=False)
-
+def cpu_baseline(ctx, mesh, dni, dhi, threads, model, target_s):
+    """The oracle (host BVH, FP64 sums and series, the same ray rules) on the first faces of the 1,000-face mesh, the
+    full context, 2305 patches and the full year; two probes give its fixed cost (the year's sky, the host BVH of the
+    3M triangles) and its cost per sensor, and the sample fills target_s."""
     def sample(nf):
-        pts, counts, idx = sens
-        sub = orc.MeshArrays("sensors", pts, counts[:nf], idx[: 4 * nf], sensor=True)
+        sub = W.submesh(mesh, 0, nf)
         t0 = time.perf_counter()
-        r = orc.daylight([sub, ctx_m], weather=wx, quantities=("radiation",), mf=2, selfOcclude=False, series=True,
-                         nthreads=threads)
+        r = oracle_run([("sensors", sub, True), ("city", ctx, False)], dni, dhi, threads, quantities=("radiation",), mf=4, series=True)
         return time.perf_counter() - t0, r["durationHours"]
 
-    # two probes give the fixed cost (the sky of the year) and the cost per sensor; the sample fills target_s
-    n0, n1 = 64, 512
+    n0, n1 = 16, 128
     t0, _ = sample(n0)
     t1, _ = sample(n1)
     per = max((t1 - t0) / (n1 - n0), 1e-6)
-    nf = int(min(len(sens[1]), max(n1, (target_s - max(t0 - n0 * per, 0.0)) / per)))
+    nf = int(min(len(mesh[1]), max(n1, (target_s - max(t0 - n0 * per, 0.0)) / per)))
     t, hours = sample(nf)
-    return {"value": nf * hours / t, "unit": "sensor-hours/s", "cores": threads, "kind": "port",
-            "sample": "%d of the sensor faces (the first in mesh order), full year (%.0f h), 577 patches, same context and "
-                      "weather; oracle trace + FP64 sums + FP64 series, %d threads, %.1f s" % (nf, hours, threads, t)}
+    return {"value": round(nf * hours / t, 1), "unit": "sensor-hours/s", "cores": threads, "kind": "port",
+            "cpu": model, "nproc": threads,
+            "sample": "%d sensor faces of the 1,000-face mesh, the full 3M-triangle context, 2305 patches, full year (%.0f h); "
+                      "the oracle: host BVH + trace + FP64 sums + FP64 series, %d threads (all the host's), %.1f s including "
+                      "the host BVH build and the year's sky (%.1f s at %d sensors)" % (nf, hours, threads, t, t0, n0)}
 
 
-# ---------------------------------------------------------------- accuracy on the workload itself (the oracle, FP64)
+# ---------------------------------------------------------------- legs
 
-def oracle_sample(ctx, sens, dni, dhi, n=64):
-    """The oracle's FP64 run (annual radiation and hourly series) of the first n sensors of the workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-    orc.build()
-    pts, counts, idx = sens
-    sub = orc.MeshArrays("sensors", pts, counts[:n], idx[: 4 * n], sensor=True)
-    ctx_m = orc.MeshArrays("city", ctx[0], ctx[1], ctx[2], sensor=False)
-    o = orc.daylight([sub, ctx_m], weather=dict(lat=LAT, lon=LON, tz=TZ, leap=False, dni=dni, dhi=dhi), quantities=("radiation",),
-                     mf=2, selfOcclude=False, series=True, nthreads=max(1, min(16, os.cpu_count() or 1)))
-    return o["series"], o["values"][:, 0]
+VIEW_ARGS = W.INTERIOR_VIEW_ARGS
 
 
-def accuracy(native, args, S, Cx, ref, n=64):
-    """One more run of the workload: the device series of its first n sensors (copied out of the sink) and their annual
-    radiation against the oracle's FP64 values. Error = max |x - ref| / max(|ref|, 1e-3 max |ref|) over the sample (the
-    validation scale rule of the reference's validate/README.md:160)."""
-    import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    got = {}
-
-    def sink(row0, rows, cols, ptr):
-        if row0 == 0:
-            buf = np.empty((min(n, rows), cols), np.float32)
-            if hip.hipMemcpy(buf.ctypes.data, C.c_void_p(ptr), buf.nbytes, 2):
-                return 2
-            got["E"] = buf
-        return 0
-
-    r = native.run_series_device(args, sensors=S, context=Cx, sink=sink)
-
-    def rel(x, want):
-        scale = np.maximum(np.abs(want), 1e-3 * np.abs(want).max())
-        return float(np.max(np.abs(x - want) / np.where(scale > 0, scale, 1.0)))
-
-    e_series, e_rad = rel(got["E"].astype(np.float64), ref[0]), rel(r.values[:n, 0], ref[1])
-    return {"sensors": n, "series_max_rel_err": e_series, "radiation_max_rel_err": e_rad, "bar": 2e-6,
-            "within_bar": bool(e_series <= 2e-6 and e_rad <= 2e-6)}
-
-
-# ---------------------------------------------------------------- the views leg (closest-hit rays and their reductions)
-
-VIEW_ARGS = ["--fov", "360", "--offset", "1.2", "--cell", "2", "--far", "1000", "--jitter", "on", "--distance", "isovist,mean,min",
-             "--reach", "en17037", "--no-occlude", "grid"]
-
-
-def views_leg(native, torch, steps=3):
+def views_leg(native, torch, threads, steps=3, n_check=64):
     """The seeded interior (~2M triangles) seen all around from a 0.5 m floor grid of 100,000 sensors at eye height
-    1.2 m, 2-degree cells (10,800 rays each): whole views runs through the C ABI on the resident scene."""
-    P, C, I = interior()
-    gp, gc, gi = floor_grid()
+    1.2 m, 2-degree cells (10,800 rays each): whole views runs through the C ABI on the resident scene; the first
+    n_check sensors' columns checked against the oracle's run of them."""
+    P, C, I = W.interior()
+    gp, gc, gi = W.floor_grid()
     room = native.SimulationMesh("interior", P, C, I)
     grid = native.SimulationMesh("grid", gp, gc, gi)
     r = native.run("views", VIEW_ARGS, grid, context=room)   # builds and keeps the scene
     n = r.values.shape[0]
-    c0 = native.gpu_counters()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        r = native.run("views", VIEW_ARGS, grid, context=room)
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / steps
-    c1 = native.gpu_counters()
+    r, wall, d, _ = timed_runs(native, torch, lambda: native.run("views", VIEW_ARGS, grid, context=room), steps)
     assert r.stats["scene_reused"]
+    wall /= steps
     rays = n * 10800
+    # accuracy: the first n_check sensors as a run of their own (same element indices: the jitter seeds agree)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    sub = W.floor_grid(n=n_check)
+    rs = native.run("views", VIEW_ARGS, native.SimulationMesh("grid", *sub), context=room)
+    o = orc.views([orc.MeshArrays("grid", *sub, sensor=True), orc.MeshArrays("interior", P, C, I, sensor=False)], fov=(360.0, -70.0, 50.0),
+                  cell=2.0, jitter=True, far=1000.0, reach=(6.0, 20.0, 50.0), distance={"isovist", "mean", "min"}, offset=1.2,
+                  selfOcclude=False, nthreads=threads)
+    same_rows = bool(np.array_equal(rs.values, r.values[:n_check], equal_nan=True))
+    errs = {}
+    for c, name in enumerate(o["columns"]):
+        g, w = rs.values[:, c], o["values"][:, c]
+        ok = ~np.isnan(w)
+        errs[name] = 0.0 if not ok.any() else float(np.max(np.abs(g[ok] - w[ok]) / np.maximum(np.abs(w[ok]), 1e-300)))
     return {"workload": "seeded interior, %d triangles (rooms U(3,8) m, door openings, clutter U(0.3,2) m), %d floor sensors at "
                         "0.5 m, eye 1.2 m, 360 x 120 degrees in 2-degree cells, jitter, isovist + mean/min + reach 6/20/50 m"
                         % (2 * len(C), n),
             "rays_per_run": rays, "runs": steps, "ms_per_run": round(1e3 * wall, 2),
             "grays_per_s": round(rays / wall / 1e9, 3),
-            "trace_ms_per_run": round((c1["views_ms"] - c0["views_ms"]) / steps, 3),
-            "reduce_ms_per_run": round((c1["reduce_ms"] - c0["reduce_ms"]) / steps, 3),
-            "columns": r.manifest["columns"]}
+            "trace_ms_per_run": round(d["views_ms"] / steps, 3),
+            "reduce_ms_per_run": round(d["reduce_ms"] / steps, 3),
+            "columns": r.manifest["columns"],
+            "accuracy": {"sensors": n_check, "max_rel_err_per_column": errs, "rows_equal_in_the_full_run": same_rows,
+                         "bar": "shares exact, distances 1e-12 (device vs host log)"}}
 
 
-# ---------------------------------------------------------------- the reference's own --bench runs, through our CLI
+def urban_block_leg(native, torch, tmp, epw, dni, dhi, threads, steps=5):
+    """The urban block (400 boxes, 4,800 triangles; the 40 central buildings' roofs and facades in 1 m quads: 107,344
+    sensors), 577 patches, the same cloudy year: each contraction method, with the series error of 64 sensors."""
+    boxes, central = W.city()
+    ctx = W.box_mesh(boxes)
+    sens = W.sensor_mesh(boxes[central])
+    S = native.SimulationMesh("sensors", *sens)
+    Cx = native.SimulationMesh("city", *ctx)
+    args = ["--epw", epw, "--sky-density", "2", "--quantity", "radiation", "--timeseries", os.path.join(tmp, "block.npy"), "--no-self-occlude"]
+    o = oracle_run([("sensors", W.submesh(sens, 0, 64), True), ("city", ctx, False)], dni, dhi, threads, quantities=("radiation",), mf=2,
+                   series=True)
+    out = {"workload": "seeded urban block: 400 boxes (4,800 triangles), roofs + facades of the 40 central buildings in <=1 m quads "
+                       "(%d sensors), 577 patches, 8760 h, the cloudy year, radiation + device series" % len(sens[1]), "methods": {}}
+    hours = 8760.0
+    for name in ("fp32", "bf16x3", "bf16x6"):
+        set_method(name)
+        native.run_series_device(args, sensors=S, context=Cx)
+        _, wall, d, _ = timed_runs(native, torch, lambda: native.run_series_device(args, sensors=S, context=Cx), steps)
+        _, E = series_of(native, args, S, Cx, 64)
+        out["methods"][name] = {"ms_per_step": round(1e3 * wall / steps, 2), "series_ms_per_step": round(d["series_ms"] / steps, 3),
+                                "sensor_hours_per_s": round(len(sens[1]) * hours * steps / wall, 1),
+                                "series_max_rel_err": rel_err(E, o["series"])}
+    set_method(None)
+    m = out["methods"]
+    out["bf16x6_over_fp32"] = {"step": round(m["fp32"]["ms_per_step"] / m["bf16x6"]["ms_per_step"], 2),
+                               "series": round(m["fp32"]["series_ms_per_step"] / m["bf16x6"]["series_ms_per_step"], 2)}
+    return out
+
+
+def transmission_leg(native, torch, epw, tmp):
+    """The urban block with every context face glass (tau 0.64, Radiance glass) under --transmission on: every ray walks
+    every crossing (byte mode)."""
+    boxes, central = W.city()
+    S = native.SimulationMesh("sensors", *W.sensor_mesh(boxes[central]))
+    Cx = native.SimulationMesh("city", *W.box_mesh(boxes))
+    targs = ["--epw", epw, "--sky-density", "2", "--quantity", "radiation", "--timeseries", os.path.join(tmp, "t.npy"), "--no-self-occlude",
+             "--transmission", "on", "--optics", "city=0,0.64,glass"]
+    native.run_series_device(targs, sensors=S, context=Cx)
+    rt, wall, d, _ = timed_runs(native, torch, lambda: native.run_series_device(targs, sensors=S, context=Cx), 2)
+    assert rt.stats["scene_reused"]
+    return {"workload": "the urban block with every context face glass (tau 0.64), --transmission on",
+            "ms_per_step": round(1e3 * wall / 2, 2), "trace_ms_per_step": round(d["trace_ms"] / 2, 3),
+            "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3),
+                            "traced": round(d["traced"] / (d["trace_ms"] * 1e-3) / 1e9, 3)}}
+
 
 BOSTON = ["--clear-sky", "--lat", "42.35", "--lon", "-71.07", "--tz", "-5"]
 # (label, tool arguments, the reference's published figure on an RTX 2060: docs/performance.md)
@@ -361,6 +307,9 @@ def reference_benches():
         m = re.search(r"([0-9.e+]+) rays in ([0-9.]+) ms \(([0-9.]+) GRays/s\)", err)
         if m:
             row.update(rays=float(m.group(1)), trace_s=round(float(m.group(2)) / 1e3, 3), grays_per_s=float(m.group(3)))
+        m = re.search(r"([0-9.e+]+) traced \(([0-9.]+) GRays/s traced\)", err)
+        if m:
+            row.update(traced=float(m.group(1)), traced_grays_per_s=float(m.group(2)))
         m = re.search(r"run ([0-9.]+) s total", err)
         if m:
             row["run_s"] = float(m.group(1))
@@ -378,22 +327,6 @@ def reference_benches():
     return out
 
 
-# ---------------------------------------------------------------- the HBM traffic of the traversal (rocprofv3 --pmc)
-
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bits_urban_block_traffic.txt")
-
-
-def traffic_per_launch():
-    """HBM bytes per traversal launch as the committed counter summary states them (FETCH_SIZE + WRITE_SIZE of the
-    traversal, scaled by the counters' reading of a copy of known size), or None where there is none."""
-    if not os.path.exists(TRAFFIC_FILE):
-        return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
 # ---------------------------------------------------------------- main
 
 def main():
@@ -402,13 +335,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-methods", action="store_true", help="skip the side-by-side runs of the contraction methods")
-    ap.add_argument("--no-count", action="store_true", help="skip the traversal's counted run (roofline achieved = null)")
-    ap.add_argument("--no-views", action="store_true", help="skip the views leg")
-    ap.add_argument("--only-views", action="store_true", help="run the views leg alone (prints its object as the line)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-methods", action="store_true", help="skip the contraction methods side by side")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the oracle's check of a sensor sample")
+    ap.add_argument("--no-count", action="store_true", help="skip the traversal's counted run")
+    ap.add_argument("--no-legs", action="store_true", help="skip the urban-block, views and byte-mode legs")
     ap.add_argument("--no-reference-benches", action="store_true", help="skip the reference's --bench runs through the CLI")
-    ap.add_argument("--no-transmission", action="store_true", help="skip the byte-mode leg (the context as glass)")
+    ap.add_argument("--only", choices=("headline", "views", "block", "transmission"), help="run one leg alone (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
@@ -429,62 +362,66 @@ def main():
         print(json.dumps({"copy_bytes": a.calibrate_copy << 20, "copies": 5}), flush=True)
         return
     import native
-    if a.only_views:
-        print(json.dumps({"views": views_leg(native, torch)}), flush=True)
+    model, threads = cpu_info()
+    tmp = tempfile.mkdtemp(prefix="dlbench")
+    epw, dni, dhi = W.weather_epw(native, tmp)
+    if a.only == "views":
+        print(json.dumps({"views": views_leg(native, torch, threads)}), flush=True)
+        return
+    if a.only == "block":
+        print(json.dumps({"urban_block": urban_block_leg(native, torch, tmp, epw, dni, dhi, threads, steps=a.steps)}), flush=True)
+        return
+    if a.only == "transmission":
+        print(json.dumps({"transmission": transmission_leg(native, torch, epw, tmp)}), flush=True)
         return
 
-    boxes, central = city()
-    ctx = box_mesh(boxes)
-    sens = sensor_mesh(boxes[central])
-    n_faces = len(sens[1])
-    tmp = tempfile.mkdtemp(prefix="dlbench")
-    dni, dhi = weather(native, tmp)
-    epw = os.path.join(tmp, "block.epw")
-    write_epw(epw, dni, dhi)
-    args = ["--epw", epw, "--sky-density", "2", "--quantity", "radiation", "--timeseries", os.path.join(tmp, "series.npy"),
+    boxes, central = W.city_3m()
+    ctx = W.box_mesh(boxes)
+    meshes = W.city_sensor_meshes(boxes, central)
+    n_sens = sum(len(m[1]) for _, m in meshes)
+    quantities = ["radiation", "direct", "diffuse", "illuminance"]
+    args = ["--epw", epw, "--sky-density", "4", "--quantity", ",".join(quantities), "--timeseries", os.path.join(tmp, "series.npy"),
             "--no-self-occlude"]
-    print("bench: %d context triangles, %d sensor faces; warmup %d, steps %d" % (2 * len(ctx[1]), n_faces, a.warmup, a.steps),
-          file=sys.stderr, flush=True)
-
-    def step(ms_series=None):
-        r = native.run_series_device(args, sensors=S, context=Cx)
-        return r
-
-    S = native.SimulationMesh("sensors", *sens)
+    print("bench: %d context triangles, sensor meshes %s; warmup %d, steps %d"
+          % (2 * len(ctx[1]), [(n, len(m[1])) for n, m in meshes], a.warmup, a.steps), file=sys.stderr, flush=True)
     Cx = native.SimulationMesh("city", *ctx)
-    os.environ.pop("DAYLIGHT_SERIES", None)            # the default contraction (BF16x6 on MFMA)
+    S = [native.SimulationMesh(n, *m) for n, m in meshes]
+
+    def step():
+        return native.run_series_device(args, sensors=S, context=Cx)
+
+    set_method(None)                  # the default contraction (BF16x6 on MFMA)
     r0 = step()
-    assert r0.values.shape == (n_faces, 1), r0.values.shape
+    assert r0.values.shape == (n_sens, len(quantities)), r0.values.shape
     hours = float(r0.manifest["duration_h"])
     for _ in range(a.warmup):
         step()
-    torch.cuda.synchronize()
-    c0 = native.gpu_counters()
-    lat = []
-    stages = dict.fromkeys(("prep_ms", "sky_ms", "trace_ms", "reduce_ms"), 0.0)
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        ts = time.perf_counter()
-        r = step()
-        lat.append(time.perf_counter() - ts)
-        for k in stages:
-            stages[k] += r.stats[k]
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    c1 = native.gpu_counters()
+    r, wall, d, lat = timed_runs(native, torch, step, a.steps)
     assert r.stats["scene_reused"], "the context scene was rebuilt inside the timed region"
-    d = {k: c1[k] - c0[k] for k in c1}
-    sensor_hours = n_faces * hours
+    sensor_hours = n_sens * hours
     value = sensor_hours * a.steps / wall
-    rays = d["rays"] / a.steps
-    print("bench: %.1f ms/step, %.3g sensor-hours/s, trace %.1f ms/step over %d launches, series %.1f ms/step"
-          % (1e3 * wall / a.steps, value, d["trace_ms"] / a.steps, d["trace_launches"] / a.steps, d["series_ms"] / a.steps),
-          file=sys.stderr, flush=True)
-
-    # the traversal's algorithmic fetches: one counted run (untimed)
     launches = d["trace_launches"] / a.steps
-    per_launch_ms = d["trace_ms"] / d["trace_launches"]
-    alg_bytes_launch = achieved = simd_eff = None
+    trace_launch_ms = d["trace_ms"] / max(d["trace_launches"], 1)
+    series_ms = d["series_ms"] / a.steps
+    print("bench: %.1f ms/step, %.4g sensor-hours/s, trace %.1f ms/step over %d launches, series %.1f ms/step"
+          % (1e3 * wall / a.steps, value, d["trace_ms"] / a.steps, launches, series_ms), file=sys.stderr, flush=True)
+
+    # the contraction's shape: K = 2305 sky directions (one sample per patch), T = the non-night steps
+    _, E0 = series_of(native, args, S[:1], Cx, 1)
+    Tup = E0.shape[1]
+    P = 2305
+    alg_flops = 2.0 * n_sens * P * Tup
+    kp, tp, np_ = (P + 31) // 32 * 32, (Tup + 127) // 128 * 128, (n_sens + 127) // 128 * 128
+    exec_flops = 6 * 2.0 * np_ * kp * tp                 # BF16x6: six products over the padded tiles
+    contraction = {"kernel": "k_series_mfma<3> (BF16x6, v_mfma_f32_32x32x16_bf16)", "bound": "mfma",
+                   "achieved": round(exec_flops / (series_ms * 1e-3) / 1e12, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "frac": round(exec_flops / (series_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                   "algorithmic_tflops": round(alg_flops / (series_ms * 1e-3) / 1e12, 1),
+                   "algorithmic_frac": round(alg_flops / (series_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                   "series_ms_per_step": round(series_ms, 3), "shape": [n_sens, P, Tup]}
+
+    # the traversal's counted fetches (one untimed run of the counting variant): its cache-level traffic
+    cache = None
     if not a.no_count:
         native.count_fetches(True)
         k0 = native.gpu_counters()
@@ -492,89 +429,95 @@ def main():
         k1 = native.gpu_counters()
         native.count_fetches(False)
         fetch_bytes = (k1["node_fetches"] - k0["node_fetches"]) * NODE_BYTES + (k1["tri_fetches"] - k0["tri_fetches"]) * TRI_BYTES
-        n_words = math.ceil((k1["counted_rays"] - k0["counted_rays"]) / n_faces / 32)
-        io_bytes = n_faces * n_words * (4 + 24)         # a visibility word written, the sensor's point + normal read per word
-        alg_bytes_launch = (fetch_bytes + io_bytes) / launches
-        achieved = alg_bytes_launch / (per_launch_ms * 1e-3) / 1e9
         waves = k1["wave_iters"] - k0["wave_iters"]
-        simd_eff = (k1["node_fetches"] - k0["node_fetches"]) / (64.0 * waves) if waves else None
+        cache = {"bytes_per_launch": round(fetch_bytes / launches), "gbs": round(fetch_bytes / launches / (trace_launch_ms * 1e-3) / 1e9, 1),
+                 "simd_efficiency": None if not waves else round((k1["node_fetches"] - k0["node_fetches"]) / (64.0 * waves), 3),
+                 "note": "the counted 4-wide record (112 B) and triangle (48 B) fetches: served by L1/L2, not HBM"}
+    traffic = profile_value("k_bits_city3m_traffic.txt", "traffic_bytes_per_launch")
+    hbm = None if traffic is None else traffic / (trace_launch_ms * 1e-3) / 1e9
+    roofline = {"kernel": "k_bits (visibility traversal, the step's largest kernel)", "bound": "valu",
+                "achieved": None if hbm is None else round(hbm, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None if hbm is None else round(hbm / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "launch_ms": round(trace_launch_ms, 3),
+                "limiter": {"valu_busy": profile_value("k_bits_city3m_counters.txt", "valu_busy"),
+                            "valu_lanes_active": profile_value("k_bits_city3m_counters.txt", "valu_lanes_active"),
+                            "source": "profiles/k_bits_city3m_counters.txt"},
+                "note": "achieved = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size: "
+                        "profiles/k_bits_city3m_traffic.txt) / the live mean launch time; the tree is served from L2 / MALL, "
+                        "so HBM is far from binding: the VALU issue is (counters above)",
+                "cache": cache}
 
-    ref = oracle_sample(ctx, sens, dni, dhi)
-    acc = accuracy(native, args, S, Cx, ref)
-    print("bench: accuracy %s" % json.dumps(acc), file=sys.stderr, flush=True)
-
-    # the contraction methods side by side (two runs each)
+    # accuracy: the first sensors of the 1,000-face mesh as a run of their own, against the oracle's FP64 run
+    acc = None
     methods = {}
-    for name in () if a.no_methods else ("fp32", "bf16x3", "bf16x6"):
-        os.environ["DAYLIGHT_SERIES"] = name
-        step()
-        m0 = native.gpu_counters()
-        tm = time.perf_counter()
-        for _ in range(2):
+    n_acc = 256
+    sub = W.submesh(meshes[0][1], 0, n_acc)
+    Ssub = native.SimulationMesh(meshes[0][0], *sub)
+    if not a.no_accuracy:
+        o = oracle_run([(meshes[0][0], sub, True), ("city", ctx, False)], dni, dhi, threads, quantities=tuple(quantities), mf=4, series=True)
+        rs, Es = series_of(native, args, [Ssub], Cx, n_acc)
+        acc = {"sensors": n_acc, "columns": {q: rel_err(rs.values[:, c], o["values"][:, c]) for c, q in enumerate(quantities)},
+               "series_bf16x6": rel_err(Es, o["series"]), "bar": 2e-6}
+        acc["within_bar"] = bool(max(list(acc["columns"].values()) + [acc["series_bf16x6"]]) <= 2e-6)
+        print("bench: accuracy %s" % json.dumps(acc), file=sys.stderr, flush=True)
+    # the contraction methods side by side on the headline workload
+    if not a.no_methods:
+        for name in ("fp32", "bf16x3", "bf16x6"):
+            set_method(name)
             step()
-        torch.cuda.synchronize()
-        tm = (time.perf_counter() - tm) / 2
-        m1 = native.gpu_counters()
-        methods[name] = {"ms_per_step": round(1e3 * tm, 2), "series_ms_per_step": round((m1["series_ms"] - m0["series_ms"]) / 2, 3),
-                         "sensor_hours_per_s": round(sensor_hours / tm, 1),
-                         "series_max_rel_err": accuracy(native, args, S, Cx, ref)["series_max_rel_err"]}
-    os.environ.pop("DAYLIGHT_SERIES", None)
-    if methods:
+            _, wm, dm, _ = timed_runs(native, torch, step, 2)
+            sm = dm["series_ms"] / 2
+            row = {"ms_per_step": round(1e3 * wm / 2, 2), "series_ms_per_step": round(sm, 3),
+                   "sensor_hours_per_s": round(sensor_hours * 2 / wm, 1), "series_sensor_hours_per_s": round(sensor_hours / (sm * 1e-3), 1)}
+            if name == "fp32":
+                row["fp32_peak_frac"] = round(alg_flops / (sm * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
+            else:
+                ns = 2 if name == "bf16x3" else 3
+                row["mfma_executed_frac"] = round(ns * (ns + 1) // 2 * 2.0 * np_ * kp * tp / (sm * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
+            if acc is not None:
+                _, Em = series_of(native, args, [Ssub], Cx, n_acc)
+                row["series_max_rel_err"] = rel_err(Em, o["series"])
+            methods[name] = row
+        set_method(None)
+        methods["bf16x6_over_fp32"] = {"step": round(methods["fp32"]["ms_per_step"] / methods["bf16x6"]["ms_per_step"], 2),
+                                       "series": round(methods["fp32"]["series_ms_per_step"] / methods["bf16x6"]["series_ms_per_step"], 2)}
         print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
 
-    # byte mode: the same workload with every context face glass (tau 0.64, Radiance glass) under --transmission on, so
-    # every ray walks every crossing; the first run builds the slotted scene (its own key), two timed runs follow
-    trans = None
-    if not a.no_transmission:
-        targs = args + ["--transmission", "on", "--optics", "city=0,0.64,glass"]
-        native.run_series_device(targs, sensors=S, context=Cx)
-        t0t = native.gpu_counters()
-        tt = time.perf_counter()
-        for _ in range(2):
-            rt = native.run_series_device(targs, sensors=S, context=Cx)
-        torch.cuda.synchronize()
-        tt = (time.perf_counter() - tt) / 2
-        t1t = native.gpu_counters()
-        assert rt.stats["scene_reused"]
-        trans = {"workload": "the urban block with every context face glass (tau 0.64), --transmission on",
-                 "ms_per_step": round(1e3 * tt, 2), "sensor_hours_per_s": round(sensor_hours / tt, 1),
-                 "trace_ms_per_step": round((t1t["trace_ms"] - t0t["trace_ms"]) / 2, 3),
-                 "series_ms_per_step": round((t1t["series_ms"] - t0t["series_ms"]) / 2, 3),
-                 "reduce_ms_per_step": round((t1t["reduce_ms"] - t0t["reduce_ms"]) / 2, 3),
-                 "grays_per_s": round((t1t["rays"] - t0t["rays"]) / ((t1t["trace_ms"] - t0t["trace_ms"]) * 1e-3) / 1e9, 3),
-                 "mean_radiation_vs_opaque": round(float(rt.values.mean() / r.values.mean()), 4)}
-        print("bench: transmission %s" % json.dumps(trans), file=sys.stderr, flush=True)
-
-    views = None if a.no_views else views_leg(native, torch)
-    if views:
-        print("bench: views %s" % json.dumps({k: v for k, v in views.items() if k != "columns"}), file=sys.stderr, flush=True)
+    legs = {}
+    if not a.no_legs:
+        legs["urban_block"] = urban_block_leg(native, torch, tmp, epw, dni, dhi, threads)
+        print("bench: urban block %s" % json.dumps(legs["urban_block"]), file=sys.stderr, flush=True)
+        legs["views"] = views_leg(native, torch, threads)
+        print("bench: views %s" % json.dumps({k: v for k, v in legs["views"].items() if k != "columns"}), file=sys.stderr, flush=True)
+        legs["transmission"] = transmission_leg(native, torch, epw, tmp)
+        print("bench: transmission %s" % json.dumps(legs["transmission"]), file=sys.stderr, flush=True)
     refs = None if a.no_reference_benches else reference_benches()
-    cpu = None if a.no_cpu else cpu_baseline(ctx, sens, dni, dhi, a.cpu_seconds)
+    cpu = None if a.no_cpu else cpu_baseline(ctx, meshes[0][1], dni, dhi, threads, model, a.cpu_seconds)
     lat_ms = np.array(lat) * 1e3
     line = {
-        "metric": "sensor-hours/s at <=2e-6 rel vs FP64 (annual radiation + hourly series)",
+        "metric": "sensor-hours/s at <=2e-6 rel vs FP64 (annual radiation, direct, diffuse, illuminance + hourly series)",
         "value": round(value, 1), "unit": "sensor-hours/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 trace, f64 sky and sums, bf16x6 mfma series", "data": "synthetic",
-        "config": {"workload": "seeded urban block: 400 boxes (4800 context triangles), roofs + facades of the 40 central "
-                               "buildings in <=1 m quads, 577 patches, 8760 h, Beta(2,3) cloudy clear-sky weather, "
-                               "radiation + device series",
-                   "sensors": n_faces, "context_triangles": 2 * len(ctx[1]), "patches": 577, "hours": hours,
-                   "series_columns": None, "contraction": "bf16x6"},
-        "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3), "rays_per_step": rays},
-        "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in stages.items()},
+        "dtype": "f32 trace, f64 sky and sums, bf16x6 mfma series (fp32 accumulation in 256-direction blocks)", "data": "synthetic",
+        "config": {"workload": "batched precision-form daylight: seeded 3M-triangle city (250,000 boxes), three sensor meshes of "
+                               "1,000 / 150,000 / 850,000 faces in one run, 2305 patches, 8760 h, Beta(2,3) cloudy clear-sky "
+                               "weather, radiation + direct + diffuse + illuminance + the radiometric device series",
+                   "sensors": n_sens, "context_triangles": 2 * len(ctx[1]), "patches": P, "hours": hours, "series_columns": Tup,
+                   "contraction": "bf16x6", "trace_launches_per_step": launches},
+        "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3),
+                        "traced": round(d["traced"] / (d["trace_ms"] * 1e-3) / 1e9, 3),
+                        "slots_per_step": d["rays"] / a.steps, "traced_per_step": d["traced"] / a.steps,
+                        "note": "slots count every sensor x direction pair as the reference does (daylight.h:110); traced "
+                                "counts the rays past the hemisphere cull"},
+        "stages_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
         "accuracy": acc,
-        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2)},
-        "kernels_ms_per_step": {k: round(d[k] / a.steps, 3) for k in ("trace_ms", "reduce_ms", "series_ms", "sky_ms")},
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2), "p99": round(float(np.percentile(lat_ms, 99)), 2),
+                       "note": "of back-to-back runs (one request at a time)"},
         "contraction_methods": methods,
-        "transmission": trans,
-        "views": views,
+        "roofline": roofline,
+        "contraction_roofline": contraction,
+        "legs": legs,
         "reference_benches": refs,
-        "roofline": {"kernel": "k_bits (visibility traversal)", "bound": "hbm",
-                     "achieved": None if achieved is None else round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
-                     "launch_ms": round(per_launch_ms, 4), "simd_efficiency": None if simd_eff is None else round(simd_eff, 3),
-                     "algorithmic_bytes_per_launch": None if alg_bytes_launch is None else round(alg_bytes_launch)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

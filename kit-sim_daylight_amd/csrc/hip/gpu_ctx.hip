@@ -65,7 +65,7 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
   for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->slots, &g->dc, &g->skySdc,
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->traceCount, &g->slots, &g->dc, &g->skySdc,
                     &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);

@@ -6,7 +6,7 @@
 //   method 0, FP32: each E[i, t] an fma chain over d in ascending order, then fma(G_i, g_t, .), then the sun's fma at its
 //             step: the reference kernel's arithmetic (nvcc contracts it so), tiled 64 x 64 through LDS.
 //   method 1, BF16x3: A and S split into bf16 hi + lo; hi.hi + hi.lo + lo.hi on MFMA (v_mfma_f32_32x32x16 bf16),
-//             FP32 accumulation; epilogue in FP64.
+//             FP32 accumulation in blocks of 256 directions summed into a running total; epilogue in FP64.
 //   method 2, BF16x6: hi + mid + lo; the six products down to 2^-16 relative, as method 1 otherwise: FP32-level.
 // Operands: fragment-major bf16 planes (Kp = nSky rounded up to 32, zero padded), described at k_split_a. Reflected light
 // (dlg_bounce rows dc and the sky's bin radiance Sdc) extends K: E += dc Sdc in the same products (methods 1 / 2) or a
@@ -182,13 +182,19 @@ __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ r
 // Block: 4 waves over 128 sensors x 128 steps, wave (wm, wn) 64 x 64 = 2 x 2 MFMA tiles (each A fragment serves two
 // tiles, each B fragment two). Per slice of 32 directions the block stages its fragments in LDS - NS x 4 row groups x
 // 2 k-steps of A and the same of B, 48 KB for NS = 3, laid out as in HBM so that both the copy and each wave's reads are
-// contiguous - and loads the next slice into registers while the MFMAs of this one run. FP32 accumulation over all
-// directions (the FP32 method's class of rounding: ~sqrt(nSky) ulps), the epilogue (G g, the sun) in FP64.
+// contiguous - and loads the next slice into registers while the MFMAs of this one run. FP32 accumulation in blocks of
+// kBlockSlices slices added to a running total, the epilogue (G g, the sun) in FP64.
 // Blocks are numbered XCD-major: the blocks an XCD runs (hardware id % 8) take consecutive tiles of a grouped order
 // (kGroup rows of sensor tiles per step column), so the tiles that share operand panels share that XCD's L2.
 constexpr uint32_t kGroup = 4, kTileM = 128, kTileN = 128, kSliceK = 32;
+// Accumulation in blocks: the MFMAs of kBlockSlices slices (256 directions) accumulate in FP32 from zero, and each block's
+// sum is added to a running FP32 total. A sum over K directions then carries about K / 256 + 6 x 16 roundings instead of
+// the 6 K / 16 of one chain (6 products per k-step at NS = 3). Measured at P = 2305 on the urban block: BF16x6 1.9e-6 ->
+// 3.9e-7, BF16x3 2.5e-6 -> 1.3e-6 relative to FP64. The totals take the registers of a third wave per SIMD (2 waves, 252
+// VGPRs, no spills; 2 % slower than the single chain at 3 waves; 3 waves with the totals spilled: 30 % slower).
+constexpr uint32_t kBlockSlices = 8;
 template <int NS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_series_mfma(const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_series_mfma(const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG,
                                                      uint32_t nCG, uint32_t nKS, int T, const uint32_t* __restrict__ bits, uint32_t bytes,
                                                      const float3* __restrict__ N, const float3* __restrict__ D,
                                                      const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
@@ -220,11 +226,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     pa[m] = A[(m / 2) * aPl + (m % 2) * 2 * gStride + aBase];
     pb[m] = B[(m / 2) * bPl + (m % 2) * 2 * gStride + bBase];
   }
-  f32x16 acc[2][2];
+  // acc: the products of the current block of kBlockSlices slices; tot: the blocks summed
+  f32x16 acc[2][2], tot[2][2];
 #pragma unroll
   for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
-    for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = f32x16{};
+    for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = tot[fm][fn] = f32x16{};
   const uint32_t nSlices = nKS / 2;
   for (uint32_t q = 0; q < nSlices; ++q) {
 #pragma unroll
@@ -264,6 +271,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
+    if ((q + 1) % kBlockSlices == 0 || q + 1 == nSlices) {   // a block of directions done: into the running sum
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < 2; ++fn) { tot[fm][fn] += acc[fm][fn]; acc[fm][fn] = f32x16{}; }
+    }
   }
   // Epilogue: the tile's rows (N, G), columns (g, the step's sun: D, weight) and the sun visibility of every (row,
   // column) staged in the operand buffers (free after the loop's last barrier), so that the 64 elements of a lane read
@@ -305,7 +318,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const uint32_t r = 32 * (2 * wm + fm) + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4), i = i0 + r;
         if (i >= nc) continue;
         const float* rr = sRow + 4 * r;
-        double e = (double)acc[fm][fn][v] + (double)rr[3] * gt;
+        double e = (double)tot[fm][fn][v] + (double)rr[3] * gt;
         const uint32_t sv = sVis[r * 128 + c];
         if (sv) e += (double)(fmaxf(dlm::dot3(make_float3(rr[0], rr[1], rr[2]), dj), 0.f) * vis_T(sv)) * sw;
         E[size_t(i) * T + t] = (float)e;

@@ -102,29 +102,38 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
   return walk<STACK, COUNT, OFF32>(wide, sl, bs, stk, lane, f, [&](uint32_t leaf) { return own.add(r, nrm, leaves, leaf, cull); });
 }
 
+// the rays a wave traced (past the hemisphere cull) added to *traced: a full-wave sum, one atomic per wave
+__device__ __forceinline__ void add_traced(uint32_t n, uint32_t lane, unsigned long long* __restrict__ traced) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) n += uint32_t(__shfl_xor(int(n), m));
+  if (lane == 0 && n) atomicAdd(traced, (unsigned long long)n);
+}
+
 // COUNT: also adds the lane's node visits and triangle tests to counts[0] / counts[1] (the algorithmic fetches) and the
 // traversal iterations the wave executed to counts[2] (SIMD efficiency = node visits / (64 x wave iterations))
 template <int STACK, bool COUNT, bool OFF32>
 __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull, uint32_t* __restrict__ bits,
-                                               unsigned long long* __restrict__ counts) {
+                                               unsigned long long* __restrict__ counts, unsigned long long* __restrict__ traced) {
   __shared__ uint32_t stk[STACK * kWave];
   const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
-  if (i >= nc) return;
-  const float3 n = N[i];
-  const float3 o = offset_ray(P[i], n);
-  uint32_t word = 0;
+  const bool live = i < nc;   // a lane past the chunk keeps a zero normal: every direction culled, nothing stored
+  const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+  const float3 o = live ? offset_ray(P[i], n) : n;
+  uint32_t word = 0, nTraced = 0;
   Fetches f;
   for (uint32_t k = 0; k < 32; ++k) {
     const uint32_t d = w * 32 + k;
     if (d >= nDir) break;
     const float3 dir = D[d];
     if (n.x * dir.x + n.y * dir.y + n.z * dir.z <= 0.f) continue;   // hemisphere cull (trace_dev.cu:101)
+    ++nTraced;
     if (nTris == 0 || ray_open<STACK, COUNT, OFF32>(wide, leaves, pad, o, n, dir, dirTab[2 * d], dirTab[2 * d + 1], cull != 0, stk, int(lane), f))
       word |= 1u << k;
   }
-  bits[size_t(w) * nc + i] = word;
+  if (live) bits[size_t(w) * nc + i] = word;
+  add_traced(nTraced, lane, traced);
   if (COUNT) {
     atomicAdd(&counts[0], (unsigned long long)f.nodes);
     atomicAdd(&counts[1], (unsigned long long)f.tris);
@@ -204,20 +213,22 @@ template <int STACK, bool OFF32>
 __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                 const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                 const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull,
-                                                const float4* __restrict__ slots, uint32_t slotLast, uint32_t* __restrict__ vis) {
+                                                const float4* __restrict__ slots, uint32_t slotLast, uint32_t* __restrict__ vis,
+                                                unsigned long long* __restrict__ traced) {
   __shared__ uint32_t stk[STACK * kWave];
   const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
-  if (i >= nc) return;
-  const float3 n = N[i];
-  const float3 o = offset_ray(P[i], n);
+  const bool live = i < nc;   // as k_bits: a zero normal past the chunk, nothing stored
+  const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+  const float3 o = live ? offset_ray(P[i], n) : n;
   Fetches f;
-  uint32_t quad = 0;
+  uint32_t quad = 0, nTraced = 0;
   for (uint32_t k = 0; k < 32; ++k) {
     const uint32_t d = w * 32 + k;
     uint32_t b = 0;
     if (d < nDir) {
       const float3 dir = D[d];
       if (n.x * dir.x + n.y * dir.y + n.z * dir.z > 0.f) {   // hemisphere cull (trace_dev.cu:131)
+        ++nTraced;
         if (nTris == 0) b = 255;
         else {
           Ray r;
@@ -234,10 +245,11 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
     }
     quad |= b << (8 * (k & 3));
     if ((k & 3) == 3) {
-      vis[size_t(w * 8 + k / 4) * nc + i] = quad;
+      if (live) vis[size_t(w * 8 + k / 4) * nc + i] = quad;
       quad = 0;
     }
   }
+  add_traced(nTraced, lane, traced);
 }
 
 template <int STACK>
@@ -245,7 +257,8 @@ void launch_bytes(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint3
   const dlg_scene& s = g->scene;
   const auto k = s.nNodes < (int64_t(1) << 25) ? k_bytes<STACK, true> : k_bytes<STACK, false>;
   k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(), g->dirTab.as<float4>(),
-                                   uint32_t(g->nDir), nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), g->bits.as<uint32_t>());
+                                   uint32_t(g->nDir), nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), g->bits.as<uint32_t>(),
+                                   g->traceCount.as<unsigned long long>());
 }
 
 template <int STACK, bool COUNT>
@@ -253,7 +266,8 @@ void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32
   const dlg_scene& s = g->scene;
   const auto k = s.nNodes < (int64_t(1) << 25) ? k_bits<STACK, COUNT, true> : k_bits<STACK, COUNT, false>;
   k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(),
-                                                      g->dirTab.as<float4>(), uint32_t(g->nDir), nc, cull, g->bits.as<uint32_t>(), counts);
+                                                      g->dirTab.as<float4>(), uint32_t(g->nDir), nc, cull, g->bits.as<uint32_t>(), counts,
+                                                      g->traceCount.as<unsigned long long>());
 }
 
 }  // namespace
@@ -276,6 +290,10 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
   const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(g->nWords));
   const float3* P = g->P.as<float3>() + i0;
   const float3* N = g->N.as<float3>() + i0;
+  if (!g->traceCount.p) {   // the running count of traced rays, read after each launch
+    DLG_CHECK(g->traceCount.grow(sizeof(unsigned long long)));
+    DLG_CHECK(hipMemsetAsync(g->traceCount.p, 0, sizeof(unsigned long long), g->stream));
+  }
   unsigned long long* counts = nullptr;
   if (g->countFetches && !bytes) {
     DLG_CHECK(g->counts.grow(3 * sizeof(unsigned long long)));
@@ -315,6 +333,9 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     g->stats.waveIters += int64_t(c[2]);
     g->stats.countedRays += nc * g->nDir;
   }
+  unsigned long long traced = 0;
+  DLG_CHECK(hipMemcpy(&traced, g->traceCount.p, sizeof traced, hipMemcpyDeviceToHost));
+  g->stats.traced = int64_t(traced);
   g->stats.traceMs += ms;
   g->stats.lastTraceMs = ms;
   g->stats.rays += nc * g->nDir;

@@ -283,7 +283,7 @@ int32_t dlx_gpu_counters(double* out, int32_t n) {
   dlg_stats(g, &st);
   const double c[DLX_COUNTERS] = {st.traceMs, double(st.launches), double(st.rays), double(st.nodeFetches), double(st.triFetches),
                                   double(st.countedRays), st.reduceMs, st.seriesMs, st.skyMs, st.buildMs, st.viewMs, st.lastTraceMs,
-                                  st.lastSeriesMs, double(st.sceneBytes), double(st.height), double(st.waveIters)};
+                                  st.lastSeriesMs, double(st.sceneBytes), double(st.height), double(st.waveIters), double(st.traced)};
   for (int32_t k = 0; k < n && k < DLX_COUNTERS; ++k) out[k] = c[k];
   return DL_OK;
 }

@@ -83,6 +83,8 @@ struct dlg_gpu {
   // the traversal's counting variant (dlg_count_fetches) and its two counters
   int32_t countFetches = 0;
   DevBuf counts;
+  // the rays traced past the hemisphere cull since the context was made (k_bits / k_bytes add to it)
+  DevBuf traceCount;
   // the tree dlg_build makes (dlg_set_tree): DLG_TREE_SAH or DLG_TREE_LBVH
   int32_t tree = DLG_TREE_SAH;
   // mesh ingest (dlg_mesh_*): staged vertices [meshNv] float3, triangles [meshNt] uint3, their global slots, the

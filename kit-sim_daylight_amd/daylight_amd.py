@@ -64,7 +64,7 @@ class ViewIn(C.Structure):
 
 
 SERIES_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p)
-DLX_COUNTERS = 16
+DLX_COUNTERS = 17
 
 
 # every function include/*.h declares: (name, restype, argtypes)

@@ -253,7 +253,8 @@ def test_sky_suns_weights_and_series_matrix(gpu, orc):
     weights to 1e-12 relative, the series matrix S to one float rounding."""
     dni, dhi = orc.clear_sky(52.52, 13.41, 1)
     steps, st = steps_of(orc, dni, dhi)
-    for mf in (1, 2):
+    for mf in (1, 2, 4):
+        print("case: GPU sky, sky density %d" % mf, flush=True)
         out = gpu.sky(steps, 52.52, 13.41, 1.0, False, 0.0, 0.2, mf, tcos_table(), wantRad=True, wantVis=True, series=True)
         pa = orc.patches(mf)
         for t in range(0, 8760, 7):
@@ -278,7 +279,9 @@ def test_sky_suns_weights_and_series_matrix(gpu, orc):
         assert abs(out["gndRad"][0] - gnd) <= 1e-12 * gnd
 
 
-def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc):
+@pytest.mark.parametrize("mf", [2, 4])
+def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc, mf):
+    print("case: series methods, sky density %d" % mf, flush=True)
     rng = np.random.default_rng(5)
     t = city(rng, 200, 120.0)
     nc = 1500
@@ -286,7 +289,6 @@ def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc):
     P[:, :2] = rng.uniform(-150, 150, (nc, 2))
     P[:, 2] = np.where(rng.uniform(size=nc) < 0.5, 0.0, rng.uniform(0, 20, nc))
     N = np.tile(np.array([0, 0, 1], np.float32), (nc, 1))
-    mf = 2
     pa = orc.patches(mf)
     Pp = pa["P"]
     dni, dhi = orc.clear_sky(52.52, 13.41, 1)
