@@ -72,6 +72,10 @@ dlg_scene* dlg_detach(dlg_gpu* g);
 void dlg_attach(dlg_gpu* g, dlg_scene* s);
 void dlg_free_scene(dlg_scene* s);
 int64_t dlg_scene_bytes(const dlg_scene* s);
+/* No counterpart in the reference (one context per process there): dst's scene becomes src's installed scene, borrowed
+ * (the arrays stay src's: dst never frees or replaces them; dlg_attach / dlg_detach on dst drop the loan). For lanes
+ * that trace one resident scene from several contexts, each on its own stream. */
+void dlg_borrow_scene(dlg_gpu* dst, const dlg_gpu* src);
 /* gpu_free_bytes (internal.h:145) */
 int64_t dlg_free_bytes(void);
 /* The installed scene as built, for the bit-exact check against a host build: nodes [n][16] floats (left box lo xyz hi

@@ -64,7 +64,8 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
 
 void dlg_destroy(dlg_gpu* g) {
   if (!g) return;
-  for (DevBuf* b : {&g->scene.nodes, &g->scene.wide, &g->scene.leaves, &g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
+  g->scene.release();
+  for (DevBuf* b : {&g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
                     &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->view, &g->labels, &g->counts, &g->traceCount, &g->slots, &g->dc, &g->skySdc,
                     &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
     b->release();
@@ -95,6 +96,7 @@ void dlg_trim(dlg_gpu* g, int32_t sensors) {
 }
 
 dlg_scene* dlg_detach(dlg_gpu* g) {
+  if (g->scene.borrowed) { g->scene = dlg_scene{}; return new dlg_scene; }   // a loan is dropped, not handed on
   dlg_scene* s = new dlg_scene(g->scene);
   g->scene = dlg_scene{};
   return s;
@@ -104,6 +106,12 @@ void dlg_attach(dlg_gpu* g, dlg_scene* s) {
   g->scene.release();
   g->scene = *s;
   delete s;
+}
+
+void dlg_borrow_scene(dlg_gpu* dst, const dlg_gpu* src) {
+  dst->scene.release();
+  dst->scene = src->scene;
+  dst->scene.borrowed = true;
 }
 
 void dlg_free_scene(dlg_scene* s) {

@@ -1,0 +1,250 @@
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
+// The engine: lanes (device contexts) and the store of resident scenes (engine.h). Replaces the reference's process-wide
+// Gpu and its parked GAS slots (reference src/run.cpp:30-43, 68-190) with an object that more than one lane can run
+// against at a time.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+
+namespace daylight {
+namespace {
+
+constexpr size_t kVramMargin = size_t(128) << 20;   // left free beside every admission
+constexpr size_t kMinChunk = size_t(32) << 20;
+using Clk = std::chrono::steady_clock;
+double ms_since(Clk::time_point t0) { return std::chrono::duration<double, std::milli>(Clk::now() - t0).count(); }
+std::string gb(size_t b) { char s[32]; std::snprintf(s, sizeof s, "%.2f GB", b / 1073741824.0); return s; }
+size_t free_bytes() { return size_t(dlg_free_bytes()); }
+
+}  // namespace
+
+Engine& Engine::get() {
+  static Engine e;
+  return e;
+}
+
+Lane* Engine::primary(std::string& err) {
+  if (!primary_.gpu) {
+    char e[512] = {};
+    if (dlg_init(&primary_.gpu, e, sizeof e)) { err = e; primary_.gpu = nullptr; return nullptr; }
+  }
+  return &primary_;
+}
+
+Lane* Engine::new_lane(std::string& err) {
+  Lane* p = primary(err);
+  if (!p) return nullptr;
+  auto* l = new Lane;
+  char e[512] = {};
+  if (dlg_init(&l->gpu, e, sizeof e)) { err = e; delete l; return nullptr; }
+  std::lock_guard<std::mutex> lk(lanesMu_);
+  dlg_borrow_scene(l->gpu, p->gpu);
+  lanes_.push_back(l);
+  ++borrowers_;
+  return l;
+}
+
+void Engine::release_lane(Lane* l) {
+  if (!l) return;
+  {
+    std::lock_guard<std::mutex> lk(lanesMu_);
+    lanes_.erase(std::remove(lanes_.begin(), lanes_.end(), l), lanes_.end());
+    --borrowers_;
+  }
+  dlg_destroy(l->gpu);   // a borrowed scene is left alone
+  delete l;
+}
+
+bool Engine::room(size_t need) const { return free_bytes() >= need; }
+
+// The installed scene leaves the primary lane: parked under its key when more scenes may stay resident (or a lane
+// borrows it), else freed.
+void Engine::park_or_free_installed(int residentScenes) {
+  dlg_scene* s = dlg_detach(primary_.gpu);
+  const bool keep = !installed_.key.empty() && (residentScenes > 1 || borrowers_ > 0);
+  if (keep) {
+    installed_.parked = s;
+    parked_.push_front(installed_);
+  } else dlg_free_scene(s);
+  installed_ = Scene{};
+}
+
+bool Engine::acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
+                     int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage,
+                     std::string& err, const std::function<void(const char*)>& mark) {
+  const auto note = [&](const char* what) { if (mark) mark(what); };
+  const bool keyed = !key.empty() && primary_.gpu;
+  const bool isInstalled = keyed && key == installed_.key;
+  auto parkedAt = std::find_if(parked_.begin(), parked_.end(), [&](const Scene& p) { return keyed && p.key == key; });
+  out.sceneReused = isInstalled || parkedAt != parked_.end();
+  s = Scene{};
+  if (out.sceneReused) { s = isInstalled ? installed_ : *parkedAt; s.parked = nullptr; }
+  if (!gpu) return true;   // no rays: nothing to install
+  if (cancel && cancel->load()) { err = "cancelled"; return false; }
+  Lane* lane = primary(err);
+  if (!lane) return false;
+  *gpu = lane->gpu;
+  if (isInstalled) return true;
+  park_or_free_installed(residentScenes);
+  if (parkedAt != parked_.end()) {   // back from the parked ones
+    installed_ = *parkedAt;
+    parked_.erase(parkedAt);
+    dlg_attach(lane->gpu, installed_.parked);
+    installed_.parked = nullptr;
+    return true;
+  }
+
+  // a fresh build: the blocking meshes' points recentred and their faces fanned on the GPU (n-gons ear-clipped here)
+  stage = "triangulating the scene";
+  const auto tPrep = Clk::now();
+  bbox_centre(meshes, s.c);
+  const size_t n = meshes.size();
+  std::vector<size_t> slotBase(n + 1, 0);
+  size_t nPts = 0, nTri = 0;
+  int64_t meshPeak = 0;   // the largest mesh's ingest temporaries
+  for (size_t k = 0; k < n; ++k) {
+    const SimulationMesh& m = *meshes[k].mesh;
+    slotBase[k + 1] = slotBase[k] + slot_count(m);   // global slots number every mesh's, blocking or not
+    if (!blocks(k)) continue;
+    nPts += m.numPoints();
+    nTri += tri_cap(m);
+    meshPeak = std::max(meshPeak, dlg_mesh_bytes(int64_t(m.numPoints()), int64_t(m.counts.size()), int64_t(m.indices.size())));
+  }
+  note("centre");
+  stage = "making room on the GPU";
+  dlg_trim(lane->gpu, 1);
+  const size_t staged = nPts * 12 + nTri * (slotted ? 16 : 12);
+  const size_t build = size_t(dlg_build_bytes(int64_t(nPts), int64_t(nTri))) + (slotted ? nTri * 4 : 0);
+  const size_t need = std::max(build, staged + size_t(meshPeak)) + kVramMargin;
+  while (!room(need) && !parked_.empty()) {   // the least recently used parked scenes go first
+    dlg_free_scene(parked_.back().parked);
+    parked_.pop_back();
+  }
+  if (!room(need)) {
+    err = "out of memory: building the scene (" + std::to_string(nTri) + " triangles) needs " + gb(need) + " of GPU memory, " + gb(free_bytes()) +
+          " free (close other programs that use the GPU, or run fewer meshes)";
+    return false;
+  }
+  stage = "triangulating the scene";
+  struct Ngons { std::vector<int64_t> face; std::vector<int32_t> made; std::vector<uint32_t> tri; int skipped = 0; };
+  std::vector<Ngons> ng(n);
+  par_for(n, [&](size_t k) { if (blocks(k)) ngon_faces(*meshes[k].mesh, ng[k].face, ng[k].made, ng[k].tri, ng[k].skipped); });
+  note("ear-clip");
+  char e[512] = {};
+  if (dlg_mesh_begin(lane->gpu, int64_t(nPts), int64_t(nTri), slotted ? 1 : 0, s.c, e, sizeof e)) { err = e; return false; }
+  size_t tri0 = 0;
+  for (size_t k = 0; k < n; ++k) {
+    if (!blocks(k)) continue;
+    const SimulationMesh& m = *meshes[k].mesh;
+    int64_t made = 0;
+    if (dlg_mesh_add(lane->gpu, m.points.data(), int64_t(m.numPoints()), m.counts.data(), int64_t(m.counts.size()), m.indices.data(),
+                     int64_t(m.indices.size()), m.slot.empty() ? nullptr : m.slot.data(), uint32_t(slotBase[k]), ng[k].face.data(), ng[k].made.data(),
+                     ng[k].tri.data(), int64_t(ng[k].face.size()), &made, e, sizeof e)) { err = e; return false; }
+    s.order.push_back({meshes[k].name, tri0, size_t(made), meshes[k].mesh});
+    tri0 += size_t(made);
+    s.skipped += ng[k].skipped;
+  }
+  out.stats.prepMs += ms_since(tPrep);
+  note("ingest");
+  stage = "building the scene";
+  const auto tBuild = Clk::now();
+  if (dlg_mesh_build(lane->gpu, e, sizeof e)) { err = e; return false; }
+  out.stats.buildMs += ms_since(tBuild);
+  note("build");
+  s.nTris = tri0;
+  s.key = key;
+  installed_ = s;
+  return true;
+}
+
+bool Engine::admit(Lane& lane, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
+                   std::string& err) {
+  const auto need = [&] {
+    const size_t perChunk = std::min(nSens, std::max<size_t>(1, chunkBytes / perSensor));
+    return nSens * 24 + perChunk * chunkSensor + fixedBytes + kVramMargin;
+  };
+  if (!room(need())) dlg_trim(lane.gpu, 1);   // the lane's last buffers are kept for the next run unless the room is needed
+  const size_t keep = size_t(std::max(1, residentScenes)) - 1;
+  while (!parked_.empty() && (parked_.size() > keep || !room(need()))) {
+    dlg_free_scene(parked_.back().parked);
+    parked_.pop_back();
+  }
+  while (!room(need()) && chunkBytes > kMinChunk) chunkBytes /= 2;
+  if (room(need())) return true;
+  err = "out of memory: the run's " + std::to_string(nSens) + " sensors need " + gb(need()) + " of GPU memory beside the scene, " + gb(free_bytes()) +
+        " free (close other programs that use the GPU, or use fewer receivers)";
+  return false;
+}
+
+void Engine::forget(const std::string& key) {
+  if (installed_.key == key) installed_.key.clear();
+  for (auto it = parked_.begin(); it != parked_.end();)
+    if (it->key == key) { dlg_free_scene(it->parked); it = parked_.erase(it); } else ++it;
+}
+
+const Sensors* lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
+                            Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug) {
+  std::string key;
+  if (!opt.sceneKey.empty()) {   // the sensors depend on the meshes (named by the scene key) and these options
+    char b[96];
+    std::snprintf(b, sizeof b, "\nsensors v%d s%d o%a n%zu m%d", opt.onVertices ? 1 : 0, int(opt.sides), opt.offset, nSens, ladybug ? 1 : 0);
+    key = opt.sceneKey + b;
+  }
+  if (!key.empty() && key == lane.sensorKey) {
+    out.ranges = lane.ranges;
+    warn = lane.sensorWarn;
+    return &lane.sensors;
+  }
+  lane.sensorKey.clear();
+  if (!sensors_of(meshes, opt, c, nSens, scratch, out, warn, err, ladybug)) return nullptr;
+  if (key.empty()) return &scratch;
+  lane.sensors = std::move(scratch);
+  lane.ranges = out.ranges;
+  lane.sensorWarn = warn;
+  lane.sensorKey = key;
+  return &lane.sensors;
+}
+
+// ------------------------------------------------------------------------------------------------ the core.h forms
+dlg_gpu* process_gpu(std::string& err) {
+  Lane* l = Engine::get().primary(err);
+  return l ? l->gpu : nullptr;
+}
+
+Scene& installed_scene() { return Engine::get().installed(); }
+
+bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
+                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err,
+                   const std::function<void(const char*)>& mark) {
+  return Engine::get().acquire(meshes, blocks, slotted, key, residentScenes, cancel, gpu, s, out, stage, err, mark);
+}
+
+bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
+                   std::string& err) {
+  std::string e;
+  Lane* l = Engine::get().primary(e);
+  if (!l || l->gpu != g) { err = "internal: admission of a context that is not the primary lane's"; return false; }
+  return Engine::get().admit(*l, nSens, perSensor, chunkSensor, fixedBytes, residentScenes, chunkBytes, err);
+}
+
+void scene_forget(const std::string& key) { Engine::get().forget(key); }
+
+const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
+                              Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug) {
+  Lane* l = Engine::get().primary(err);
+  if (!l) return nullptr;
+  return lane_sensors(*l, meshes, opt, c, nSens, scratch, out, warn, err, ladybug);
+}
+
+std::string open_key(const std::vector<char>& open, bool selfOcclude) {
+  // sensor meshes left out of the scene by --no-occlude change the scene: their indices are part of its key
+  std::string s;
+  if (!selfOcclude) return s;
+  for (size_t k = 0; k < open.size(); ++k)
+    if (open[k]) s += (s.empty() ? "\nopen " : ",") + std::to_string(k);
+  return s;
+}
+
+}  // namespace daylight

@@ -1,0 +1,79 @@
+// SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
+// The process's GPU as the host layer sees it: lanes and a store of resident scenes.
+//
+// A lane is one device context (dlg_gpu: a HIP stream, the per-run device buffers, the resident sensors and direction
+// table) and the sensors it made last. Runs on one lane take turns; runs on different lanes overlap on the device. The
+// primary lane serves daylight::run / views::run and the C ABI (reference src/run.cpp:30-35 keeps one Gpu for the
+// process); a server adds lanes of its own.
+//
+// The scene store keeps built scenes in HBM under their keys (the mesh handles, the roles and the transmitting slots):
+// the one installed in the primary lane and up to residentScenes - 1 parked ones (reference run.cpp:38-43, 68-156,
+// 179-190). Other lanes borrow the installed scene's arrays; while a lane borrows it the store neither replaces nor frees it.
+#pragma once
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <list>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "core.h"
+
+namespace daylight {
+
+struct Lane {
+  dlg_gpu* gpu = nullptr;
+  // the sensors of the last run whose meshes and sensor options gave `key` (empty: none kept)
+  std::string sensorKey;
+  Sensors sensors;
+  std::vector<std::pair<std::string, size_t>> ranges;
+  std::string sensorWarn;
+};
+
+class Engine {
+ public:
+  static Engine& get();
+
+  // the primary lane, its device context made on first use (null with err when there is no device)
+  Lane* primary(std::string& err);
+  // a new lane with a context of its own borrowing the installed scene (null with err); release_lane gives it back
+  Lane* new_lane(std::string& err);
+  void release_lane(Lane* l);
+
+  // The scene a run over `meshes` needs, installed in the primary lane: the installed one or a parked one under `key`,
+  // else built from the meshes that block (slotted: each triangle's global slot kept with the leaves). gpu null: a run
+  // without rays (nothing is installed). mark is called at the build's stage boundaries.
+  bool acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
+               int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err,
+               const std::function<void(const char*)>& mark);
+  // Room on the device for a run of nSens sensors beside the scene: per sensor 24 B resident plus chunkSensor bytes per
+  // chunk of chunkBytes / perSensor sensors, plus fixedBytes. Frees the lane's last buffers and parked scenes as needed
+  // and halves chunkBytes down to 32 MiB; false with err when even that does not fit.
+  bool admit(Lane& lane, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
+             std::string& err);
+  void forget(const std::string& key);
+  Scene& installed() { return installed_; }
+  // the run lock of the primary lane (the C ABI's runs take turns on it)
+  std::mutex& primary_lock() { return primaryMu_; }
+
+ private:
+  Engine() = default;
+  void park_or_free_installed(int residentScenes);
+  bool room(size_t need) const;
+
+  Lane primary_;
+  Scene installed_;
+  std::list<Scene> parked_;   // most recently used first
+  std::mutex primaryMu_;
+  std::mutex lanesMu_;        // the lanes' bookkeeping below
+  std::vector<Lane*> lanes_;
+  int borrowers_ = 0;         // lanes borrowing the installed scene
+};
+
+// The sensors of a run on a lane: those kept from its last run when the meshes (the scene key names the handles) and
+// the sensor options are the same, else made (and kept when the run has a key). Valid until the lane's next run.
+const Sensors* lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
+                            Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug = false);
+
+}  // namespace daylight

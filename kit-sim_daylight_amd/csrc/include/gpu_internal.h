@@ -47,8 +47,12 @@ struct dlg_scene {
   int32_t height = 0;
   float pad = 0.f;
   uint32_t slotMax = 0;
+  bool borrowed = false;   // another context's arrays (dlg_borrow_scene): never freed here
   int64_t bytes() const { return int64_t(nodes.cap + wide.cap + leaves.cap); }
-  void release() { nodes.release(); wide.release(); leaves.release(); }
+  void release() {
+    if (!borrowed) { nodes.release(); wide.release(); leaves.release(); }
+    *this = dlg_scene{};
+  }
 };
 
 struct dlg_gpu {

@@ -112,6 +112,7 @@ SEAM = [
     ("dlg_attach", None, [C.c_void_p, C.c_void_p]),
     ("dlg_free_scene", None, [C.c_void_p]),
     ("dlg_scene_bytes", C.c_int64, [C.c_void_p]),
+    ("dlg_borrow_scene", None, [C.c_void_p, C.c_void_p]),
     ("dlg_free_bytes", C.c_int64, []),
     ("dlg_scene_download", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     ("dlg_set_scene", C.c_int32, [C.c_void_p, F32P, F32P, C.c_int64, F32P, C.c_int64, C.c_char_p, C.c_int32]),
