@@ -65,10 +65,11 @@ struct RunOptions {
   int residentScenes = 1;
   const std::atomic<bool>* cancel = nullptr;
   bool verbose = false;
-  // caps each per-chunk device buffer (results do not depend on it). 4 GiB here, where the reference's 256 MiB
-  // (daylight.h:85) was sized for 6 GB cards: on a 288 GB MI355X fewer, larger launches leave fewer partly filled waves
-  // at each launch's end; admission still halves it down to 32 MiB when memory is short.
-  size_t chunkBytes = size_t(4) << 30;
+  // caps each per-chunk device buffer (results do not depend on it). 64 GiB here, where the reference's 256 MiB
+  // (daylight.h:85) was sized for 6 GB cards: on a 288 GB MI355X a run of a million sensors with its hourly series is
+  // one chunk, so each kernel is one launch over every sensor mesh of the run; admission still halves the cap down to
+  // 32 MiB when memory is short.
+  size_t chunkBytes = size_t(64) << 30;
   bool f32Columns = false;
   std::atomic<double>* progress = nullptr;
 };

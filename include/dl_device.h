@@ -210,6 +210,24 @@ int32_t dlg_reduce_view(dlg_gpu* g, int64_t nc, const double* wRow, uint32_t nCl
 /* gpu_download_view (internal.h:210): classes and distances, row-major [nc x K] */
 int32_t dlg_download_view(dlg_gpu* g, int64_t nc, uint8_t* cls, float* dist, char* err, int32_t errLen);
 
+/* No counterpart in the reference (its chunk loop launches and waits kernel by kernel, run.cpp:625-688): the device
+ * work of one daylight chunk as one HIP graph on the library's stream - dlg_trace of sensors [i0, i0 + nc), dlg_reduce
+ * of nQ rows of weights W [nQ x nDir] (useCos as there), and with series (T > 0) dlg_series by `method` with the
+ * ground view factor G taken on the device from the last row of the reduce (series->G is not read). The graph is
+ * captured on the first chunk of a shape and replayed while the buffers, sizes and flags stay the same. sums [nQ x nc]
+ * (as dlg_reduce's out), E [nc x T] (may be null: the series stays on the device, dlg_series_device). Not for the
+ * counting variant or reflected light (use the calls one by one). */
+typedef struct {
+  int64_t i0, nc;
+  uint32_t rayFlags;
+  int32_t nQ;
+  const float* W;
+  const int32_t* useCos;
+  const dlg_series_in* series;  /* null: no series */
+  int32_t method;
+} dlg_chunk_in;
+int32_t dlg_chunk(dlg_gpu* g, const dlg_chunk_in* in, double* sums, float* E, char* err, int32_t errLen);
+
 /* gpu_stats (internal.h:211-216), plus per-kernel device times of the last calls (HIP events on the library's stream). */
 typedef struct {
   double buildMs, traceMs, reduceMs, seriesMs, skyMs, viewMs;
@@ -221,6 +239,8 @@ typedef struct {
   /* reflected light: ms, paths, launches */
   double bounceMs;
   int64_t paths, bounceLaunches;
+  /* dlg_chunk: graphs captured, graph launches */
+  int64_t graphCaptures, graphLaunches;
 } dlg_stats_t;
 int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out);
 /* No counterpart in the reference (its OptiX traversal is opaque): on != 0 makes dlg_trace run the traversal's counting

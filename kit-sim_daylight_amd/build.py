@@ -46,7 +46,7 @@ def compile_one(src):
 def build(jobs=8, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
-    tools = {os.path.join(HERE, "csrc", "host", "main.cpp")}
+    tools = {os.path.join(HERE, "csrc", "host", "tools.cpp")}
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "hip", "*.hip")) + glob.glob(os.path.join(HERE, "csrc", "host", "*.cpp")))
     srcs = [s for s in srcs if s not in tools]
     lib_srcs = srcs
@@ -67,7 +67,7 @@ def build(jobs=8, verbose=False):
         raise RuntimeError("link failed\n" + r.stdout + r.stderr)
     os.replace(tmp, so)
     for tool, define in (("daylight", []), ("views", ["-DVIEWS_CLI"])):
-        main = os.path.join(HERE, "csrc", "host", "main.cpp")
+        main = os.path.join(HERE, "csrc", "host", "tools.cpp")
         if not os.path.exists(main):
             continue
         exe = os.path.join(LIB, tool)

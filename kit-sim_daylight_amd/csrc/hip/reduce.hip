@@ -101,26 +101,36 @@ void launch(dlg_gpu* g, uint32_t nc, uint32_t mask) {
 
 }  // namespace
 
-extern "C" int32_t dlg_reduce(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, double* out, char* err, int32_t errLen) {
-  if (nc == 0 || nQ == 0) return 0;
+int32_t dlg_reduce_prepare(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, uint32_t& mask, char* err, int32_t errLen) {
   if (nc != g->nc) { dlg_fail(err, errLen, "dlg_reduce: no trace of this chunk"); return 1; }
   if (nQ < 0 || nQ > kMaxQ) { dlg_fail(err, errLen, "dlg_reduce: 1 to 8 quantities"); return 1; }
   DLG_CHECK(g->scratch.grow(size_t(nQ) * size_t(g->nDir) * 4));
   DLG_CHECK(g->out.grow(size_t(nQ) * size_t(nc) * 8));
   DLG_CHECK(hipMemcpyAsync(g->scratch.p, W, size_t(nQ) * size_t(g->nDir) * 4, hipMemcpyHostToDevice, g->stream));
-  uint32_t mask = 0;
+  mask = 0;
   for (int q = 0; q < nQ; ++q) mask |= useCos[q] ? 1u << q : 0u;
-  dlg_tbegin(g);
+  return 0;
+}
+
+void dlg_reduce_enqueue(dlg_gpu* g, uint32_t nc, int32_t nQ, uint32_t mask) {
   switch (nQ) {
-    case 1: launch<1>(g, uint32_t(nc), mask); break;
-    case 2: launch<2>(g, uint32_t(nc), mask); break;
-    case 3: launch<3>(g, uint32_t(nc), mask); break;
-    case 4: launch<4>(g, uint32_t(nc), mask); break;
-    case 5: launch<5>(g, uint32_t(nc), mask); break;
-    case 6: launch<6>(g, uint32_t(nc), mask); break;
-    case 7: launch<7>(g, uint32_t(nc), mask); break;
-    default: launch<8>(g, uint32_t(nc), mask); break;
+    case 1: launch<1>(g, nc, mask); break;
+    case 2: launch<2>(g, nc, mask); break;
+    case 3: launch<3>(g, nc, mask); break;
+    case 4: launch<4>(g, nc, mask); break;
+    case 5: launch<5>(g, nc, mask); break;
+    case 6: launch<6>(g, nc, mask); break;
+    case 7: launch<7>(g, nc, mask); break;
+    default: launch<8>(g, nc, mask); break;
   }
+}
+
+extern "C" int32_t dlg_reduce(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, double* out, char* err, int32_t errLen) {
+  if (nc == 0 || nQ == 0) return 0;
+  uint32_t mask = 0;
+  if (dlg_reduce_prepare(g, nc, nQ, W, useCos, mask, err, errLen)) return 1;
+  dlg_tbegin(g);
+  dlg_reduce_enqueue(g, uint32_t(nc), nQ, mask);
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   g->stats.reduceMs += ms;

@@ -11,6 +11,9 @@
 // Operands: fragment-major bf16 planes (Kp = nSky rounded up to 32, zero padded), described at k_split_a. Reflected light
 // (dlg_bounce rows dc and the sky's bin radiance Sdc) extends K: E += dc Sdc in the same products (methods 1 / 2) or a
 // second FP32 pass after the reference's (method 0).
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "dl_math.h"
@@ -179,53 +182,62 @@ __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ r
 // lane group l / 32 (the same slots for A and B, so the product sums every k once); acc[v] is C[8 (v / 4) + 4 (l / 32) +
 // v % 4][l % 32].
 //
-// Block: 4 waves over 128 sensors x 128 steps, wave (wm, wn) 64 x 64 = 2 x 2 MFMA tiles (each A fragment serves two
-// tiles, each B fragment two). Per slice of 32 directions the block stages its fragments in LDS - NS x 4 row groups x
-// 2 k-steps of A and the same of B, 48 KB for NS = 3, laid out as in HBM so that both the copy and each wave's reads are
-// contiguous - and loads the next slice into registers while the MFMAs of this one run. FP32 accumulation in blocks of
-// kBlockSlices slices added to a running total, the epilogue (G g, the sun) in FP64.
+// Block: 2 WM waves over 64 WM sensors x 128 steps, wave (wm, wn) 64 x 64 = 2 x 2 MFMA tiles (each A fragment serves two
+// tiles, each B fragment two; WM = 4 shares each B fragment among four waves, at twice the block's A). Per slice of 32
+// directions the block stages its fragments in LDS - NS x 2 WM row groups x 2 k-steps of A and NS x 4 column groups of
+// B, laid out as in HBM so that both the copy and each wave's reads are contiguous - and loads the next slice into
+// registers while the MFMAs of this one run. FP32 accumulation in blocks of kBlockSlices slices added to a running
+// total, the epilogue (G g, the sun) in FP64. Every tile shape and order gives the same bits: each element's products
+// are summed in the same order.
 // Blocks are numbered XCD-major: the blocks an XCD runs (hardware id % 8) take consecutive tiles of a grouped order
-// (kGroup rows of sensor tiles per step column), so the tiles that share operand panels share that XCD's L2.
-constexpr uint32_t kGroup = 4, kTileM = 128, kTileN = 128, kSliceK = 32;
+// (`group` rows of sensor tiles per step column), so the tiles that share operand panels share that XCD's L2.
+constexpr uint32_t kTileN = 128, kSliceK = 32, kMaxTileM = 256;
 // Accumulation in blocks: the MFMAs of kBlockSlices slices (256 directions) accumulate in FP32 from zero, and each block's
 // sum is added to a running FP32 total. A sum over K directions then carries about K / 256 + 6 x 16 roundings instead of
 // the 6 K / 16 of one chain (6 products per k-step at NS = 3). Measured at P = 2305 on the urban block: BF16x6 1.9e-6 ->
 // 3.9e-7, BF16x3 2.5e-6 -> 1.3e-6 relative to FP64. The totals take the registers of a third wave per SIMD (2 waves, 252
 // VGPRs, no spills; 2 % slower than the single chain at 3 waves; 3 waves with the totals spilled: 30 % slower).
 constexpr uint32_t kBlockSlices = 8;
-template <int NS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_series_mfma(const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG,
-                                                     uint32_t nCG, uint32_t nKS, int T, const uint32_t* __restrict__ bits, uint32_t bytes,
-                                                     const float3* __restrict__ N, const float3* __restrict__ D,
-                                                     const float* __restrict__ G, const float* __restrict__ g, uint32_t sun0,
-                                                     const int* __restrict__ sunOfCol, const float* __restrict__ sunW,
-                                                     float* __restrict__ E, uint32_t tilesT, uint32_t tiles) {
-  constexpr int kFrag = NS * 4 * 2 * 64;          // uint4 per operand per slice: NS planes x 4 groups x 2 k-steps x 64 lanes
-  constexpr int kPer = kFrag / 256;               // per thread
-  __shared__ u32x4 sA[kFrag];
-  __shared__ u32x4 sB[kFrag];
-  static_assert(kFrag * 16 >= 128 * 10 * 4 && kFrag * 16 >= 128 * 128, "the epilogue's staging fits the operand buffers");
+template <int NS, int WM>
+__global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) void k_series_mfma(
+    const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG, uint32_t nCG, uint32_t nKS, int T,
+    const uint32_t* __restrict__ bits, uint32_t bytes, const float3* __restrict__ N, const float3* __restrict__ D, const float* __restrict__ G,
+    const float* __restrict__ g, uint32_t sun0, const int* __restrict__ sunOfCol, const float* __restrict__ sunW, float* __restrict__ E,
+    uint32_t tilesT, uint32_t tiles, uint32_t group) {
+  constexpr int kThreads = 128 * WM, kRows = 64 * WM, kGA = 2 * WM, kGB = 4;
+  constexpr int kFragA = NS * kGA * 128, kFragB = NS * kGB * 128;   // uint4 per slice: planes x groups x 2 k-steps x 64 lanes
+  constexpr int kPerA = kFragA / kThreads, kPerB = kFragB / kThreads;
+  static_assert(kFragA % kThreads == 0 && kFragB % kThreads == 0, "each thread stages whole entries");
+  __shared__ u32x4 sA[kFragA];
+  __shared__ u32x4 sB[kFragB];
+  static_assert(kFragA * 16 >= kRows * 128 && kFragB * 16 >= (kRows * 4 + 128 * 6) * 4, "the epilogue's staging fits the operand buffers");
   // XCD-major tile order
   const uint32_t id = blockIdx.x, per = (tiles + 7) / 8, xcd = id % 8, rank = id / 8;
   const uint32_t full = tiles % 8 == 0 ? 8 : tiles % 8;   // XCDs that take `per` tiles; the rest take per - 1
   const uint32_t tile = xcd < full ? xcd * per + rank : full * per + (xcd - full) * (per - 1) + rank;
   if (tile >= tiles || (xcd >= full && rank >= per - 1)) return;
-  // grouped order: kGroup sensor-tile rows walked down before the next step column
-  const uint32_t tilesI = tiles / tilesT, grp = tile / (kGroup * tilesT), first = grp * kGroup;
-  const uint32_t rows = tilesI - first < kGroup ? tilesI - first : kGroup, local = tile % (kGroup * tilesT);
+  // grouped order: `group` sensor-tile rows walked down before the next step column
+  const uint32_t tilesI = tiles / tilesT, grp = tile / (group * tilesT), first = grp * group;
+  const uint32_t rows = tilesI - first < group ? tilesI - first : group, local = tile % (group * tilesT);
   const uint32_t ti = first + local % rows, tt = local / rows;
-  const uint32_t rg0 = ti * 4, cg0 = tt * 4;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w & 1, wn = w >> 1;
-  // this thread's share of a slice: entry j = tid + 256 m is plane m / 2, group 2 (m % 2) + tid / 128, uint4 tid % 128
-  // of the group's two k-steps (32-bit offsets: a plane holds fewer than 2^32 / 3 uint4 at any chunk size)
-  const uint32_t gStride = nKS * 64, tpart = (threadIdx.x >> 7) * gStride + (threadIdx.x & 127);
-  const uint32_t aBase = rg0 * gStride + tpart, bBase = cg0 * gStride + tpart, aPl = nRG * gStride, bPl = nCG * gStride;
-  u32x4 pa[kPer], pb[kPer];
+  const uint32_t rg0 = ti * kGA, cg0 = tt * kGB;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w % WM, wn = w / WM;
+  // entry j = tid + kThreads m of a slice: plane j / (G 128), group (j / 128) % G, uint4 j % 128 of the group's two
+  // k-steps (32-bit offsets: a plane holds fewer than 2^32 / 3 uint4 at any chunk size)
+  const uint32_t gStride = nKS * 64, aPl = nRG * gStride, bPl = nCG * gStride;
+  const auto aAt = [&](int m) {
+    const uint32_t j = threadIdx.x + kThreads * m;
+    return (j / (kGA * 128)) * aPl + (rg0 + (j / 128) % kGA) * gStride + j % 128;
+  };
+  const auto bAt = [&](int m) {
+    const uint32_t j = threadIdx.x + kThreads * m;
+    return (j / (kGB * 128)) * bPl + (cg0 + (j / 128) % kGB) * gStride + j % 128;
+  };
+  u32x4 pa[kPerA], pb[kPerB];
 #pragma unroll
-  for (int m = 0; m < kPer; ++m) {
-    pa[m] = A[(m / 2) * aPl + (m % 2) * 2 * gStride + aBase];
-    pb[m] = B[(m / 2) * bPl + (m % 2) * 2 * gStride + bBase];
-  }
+  for (int m = 0; m < kPerA; ++m) pa[m] = A[aAt(m)];
+#pragma unroll
+  for (int m = 0; m < kPerB; ++m) pb[m] = B[bAt(m)];
   // acc: the products of the current block of kBlockSlices slices; tot: the blocks summed
   f32x16 acc[2][2], tot[2][2];
 #pragma unroll
@@ -235,17 +247,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const uint32_t nSlices = nKS / 2;
   for (uint32_t q = 0; q < nSlices; ++q) {
 #pragma unroll
-    for (int m = 0; m < kPer; ++m) { sA[threadIdx.x + 256 * m] = pa[m]; sB[threadIdx.x + 256 * m] = pb[m]; }
+    for (int m = 0; m < kPerA; ++m) sA[threadIdx.x + kThreads * m] = pa[m];
+#pragma unroll
+    for (int m = 0; m < kPerB; ++m) sB[threadIdx.x + kThreads * m] = pb[m];
     __syncthreads();
     {   // the next slice in flight while this one's MFMAs run (the last iteration reloads its own: no branch); the
         // scheduling barriers keep the loads here, ahead of the MFMAs, and each k-step's fragments after the last's
         // products, so that the two do not share registers
       const uint32_t step = 128 * (q + 1 < nSlices ? q + 1 : q);
 #pragma unroll
-      for (int m = 0; m < kPer; ++m) {
-        pa[m] = A[(m / 2) * aPl + (m % 2) * 2 * gStride + aBase + step];
-        pb[m] = B[(m / 2) * bPl + (m % 2) * 2 * gStride + bBase + step];
-      }
+      for (int m = 0; m < kPerA; ++m) pa[m] = A[aAt(m) + step];
+#pragma unroll
+      for (int m = 0; m < kPerB; ++m) pb[m] = B[bAt(m) + step];
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -255,7 +268,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-          const u32x4 va = sA[((s * 4 + 2 * wm + f) * 2 + kk) * 64 + lane], vb = sB[((s * 4 + 2 * wn + f) * 2 + kk) * 64 + lane];
+          const u32x4 va = sA[((s * kGA + 2 * wm + f) * 2 + kk) * 64 + lane], vb = sB[((s * kGB + 2 * wn + f) * 2 + kk) * 64 + lane];
           a[f][s] = __builtin_bit_cast(bf16x8, va);
           b[f][s] = __builtin_bit_cast(bf16x8, vb);
         }
@@ -281,25 +294,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   // Epilogue: the tile's rows (N, G), columns (g, the step's sun: D, weight) and the sun visibility of every (row,
   // column) staged in the operand buffers (free after the loop's last barrier), so that the 64 elements of a lane read
   // LDS instead of gathering from HBM; E = acc + G g + the sun's term, in FP64.
-  float* sRow = reinterpret_cast<float*>(sA);       // [128][4]: N x y z, G
-  float* sCol = sRow + 128 * 4;                      // [128][6]: g, D x y z, sun weight, sun index (bits; -1: none)
-  uint8_t* sVis = reinterpret_cast<uint8_t*>(sB);   // [128 rows][128 columns]: the sun's transmittance byte
+  float* sRow = reinterpret_cast<float*>(sB);       // [kRows][4]: N x y z, G
+  float* sCol = sRow + kRows * 4;                    // [128][6]: g, D x y z, sun weight, sun index (bits; -1: none)
+  uint8_t* sVis = reinterpret_cast<uint8_t*>(sA);   // [kRows][128 columns]: the sun's transmittance byte
   const uint32_t i0 = 32 * rg0, t0 = 32 * cg0;
-  if (threadIdx.x < 128) {
+  if (int(threadIdx.x) < kRows) {
     const uint32_t i = i0 + threadIdx.x;
     const float3 n = i < nc ? N[i] : make_float3(0.f, 0.f, 0.f);
     float* r = sRow + 4 * threadIdx.x;
     r[0] = n.x; r[1] = n.y; r[2] = n.z; r[3] = i < nc ? G[i] : 0.f;
-  } else {
-    const uint32_t c = threadIdx.x - 128, t = t0 + c;
+  } else if (threadIdx.x - kRows < 128) {
+    const uint32_t c = threadIdx.x - kRows, t = t0 + c;
     const int j = int(t) < T ? sunOfCol[t] : -1;
     const float3 d = j >= 0 ? D[sun0 + uint32_t(j)] : make_float3(0.f, 0.f, 0.f);
     float* q = sCol + 6 * c;
     q[0] = int(t) < T ? g[t] : 0.f; q[1] = d.x; q[2] = d.y; q[3] = d.z; q[4] = j >= 0 ? sunW[j] : 0.f; q[5] = __int_as_float(j);
   }
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k < 128 * 128; k += 256) {   // consecutive threads: consecutive rows of one column
-    const uint32_t r = k & 127, c = k >> 7, i = i0 + r;
+  for (uint32_t k = threadIdx.x; k < uint32_t(kRows) * 128; k += kThreads) {   // consecutive threads: consecutive rows of one column
+    const uint32_t r = k % kRows, c = k / kRows, i = i0 + r;
     const int j = __float_as_int(sCol[6 * c + 5]);
     sVis[r * 128 + c] = (j >= 0 && i < nc) ? uint8_t(vis_byte(bits, nc, i, sun0 + uint32_t(j), bytes)) : uint8_t(0);
   }
@@ -326,83 +339,223 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
 }
 
+// the ground view factor of each sensor of the chunk as the series reads it: float of the reduce's FP64 row
+__global__ void k_ground_rows(const double* __restrict__ sums, uint32_t nc, float* __restrict__ G) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) G[i] = float(sums[i]);
+}
+
+// the MFMA methods' operand sizes of a chunk: Kp the directions (and bins) padded to a slice, the row groups padded to
+// the tallest tile (so that one split serves every tile shape), the column groups to a tile
+struct MfmaShape {
+  uint32_t Kp, tilesT, nRG, nCG, Tp, nKS;
+  size_t aBytes, bBytes;
+  MfmaShape(uint32_t nc, uint32_t K, uint32_t T, int NS) {
+    Kp = (K + kSliceK - 1) / kSliceK * kSliceK;
+    tilesT = (T + kTileN - 1) / kTileN;
+    nRG = (nc + kMaxTileM - 1) / kMaxTileM * (kMaxTileM / 32);
+    nCG = tilesT * (kTileN / 32);
+    Tp = nCG * 32;
+    nKS = Kp / 16;
+    aBytes = size_t(NS) * nRG * 32 * Kp * 2;
+    bBytes = size_t(NS) * Tp * Kp * 2;
+  }
+};
+
+// The contraction's launch shapes: WM waves along the sensors (a tile of 64 WM sensors x 128 steps) and the tile-group
+// order. A shape is picked per problem by timing each on its first chunk (the tuner below), or by DAYLIGHT_SERIES_TILE
+// = "WMxGROUP".
+struct TileShape { int wm; uint32_t group; };
+constexpr TileShape kShapes[] = {{2, 4}, {2, 8}, {2, 16}, {4, 2}, {4, 4}, {4, 8}};
+constexpr int kNumShapes = int(sizeof kShapes / sizeof kShapes[0]);
+
+template <int NS, int WM>
+void launch_mfma(dlg_gpu* g, const MfmaShape& sh, const u32x4* A, const u32x4* B, uint32_t group) {
+  const dlg_series_plan& q = g->seriesPlan;
+  const char* b = g->seriesIn.as<char>();
+  const uint32_t tilesI = (q.nc + 64 * WM - 1) / (64 * WM), tiles = sh.tilesT * tilesI;
+  k_series_mfma<NS, WM><<<(tiles + 7) / 8 * 8, 128 * WM, 0, g->stream>>>(
+      A, B, q.nc, sh.nRG, sh.nCG, sh.nKS, q.T, g->bits.as<uint32_t>(), uint32_t(g->bytes), g->N.as<float3>() + g->i0, g->D.as<float3>(),
+      reinterpret_cast<const float*>(b + q.oG), g->skyG.as<float>(), q.sun0, reinterpret_cast<const int*>(b + q.oCol),
+      reinterpret_cast<const float*>(b + q.oSw), g->seriesE.as<float>(), sh.tilesT, tiles, group);
+}
+
 template <int NS>
-int32_t run_mfma(dlg_gpu* g, uint32_t nc, const dlg_series_in* in, const float* dG, const float* dg, const int* dRow, const float* dWsky,
-                 const int* dSunOfCol, const float* dSunW, char* err, int32_t errLen) {
-  const uint32_t nBins = in->reflect ? g->dcBins : 0u, Kp = (in->nSky + nBins + kSliceK - 1) / kSliceK * kSliceK, T = uint32_t(in->T);
-  const uint32_t tilesI = (nc + kTileM - 1) / kTileM, tilesT = (T + kTileN - 1) / kTileN;
-  const uint32_t nRG = tilesI * (kTileM / 32), nCG = tilesT * (kTileN / 32), Tp = nCG * 32, nKS = Kp / 16;
-  const size_t aBytes = size_t(NS) * nRG * 32 * Kp * 2, bBytes = size_t(NS) * Tp * Kp * 2;
-  DLG_CHECK(g->seriesWork.grow(aBytes + bBytes));
+void launch_shape(dlg_gpu* g, const MfmaShape& sh, const u32x4* A, const u32x4* B, int shape) {
+  const TileShape t = kShapes[shape];
+  if (t.wm == 4) launch_mfma<NS, 4>(g, sh, A, B, t.group);
+  else launch_mfma<NS, 2>(g, sh, A, B, t.group);
+}
+
+template <int NS>
+void enqueue_mfma(dlg_gpu* g) {
+  const dlg_series_plan& q = g->seriesPlan;
+  const char* b = g->seriesIn.as<char>();
+  const MfmaShape sh(q.nc, q.nSky + q.nBins, uint32_t(q.T), NS);
   uint16_t* A = g->seriesWork.as<uint16_t>();
-  uint16_t* B = A + aBytes / 2;
+  uint16_t* B = A + sh.aBytes / 2;
   const float3* N = g->N.as<float3>() + g->i0;
-  k_split_a<NS><<<dim3((Kp + 255) / 256, nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), nc, N, g->D.as<float3>(),
-                                                                          in->nSky, Kp, nRG, dWsky, g->dc.as<float>(), nBins, A);
-  k_split_s<NS><<<dim3((Kp + 255) / 256, Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, in->nSky, Kp, in->T, Tp, g->skySdc.as<float>(),
-                                                                   nBins, B);
-  const uint32_t tiles = tilesT * tilesI;
-  k_series_mfma<NS><<<(tiles + 7) / 8 * 8, 256, 0, g->stream>>>(reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), nc, nRG,
-                                                                 nCG, nKS, in->T, g->bits.as<uint32_t>(), uint32_t(g->bytes), N, g->D.as<float3>(), dG, dg,
-                                                                 in->sun0, dSunOfCol, dSunW, g->seriesE.as<float>(), tilesT, tiles);
-  DLG_CHECK(hipGetLastError());
-  return 0;
+  const int* dRow = reinterpret_cast<const int*>(b + q.oRow);
+  k_split_a<NS><<<dim3((sh.Kp + 255) / 256, sh.nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(),
+                                                                               q.nSky, sh.Kp, sh.nRG, reinterpret_cast<const float*>(b + q.oW),
+                                                                               g->dc.as<float>(), q.nBins, A);
+  k_split_s<NS><<<dim3((sh.Kp + 255) / 256, sh.Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, q.nSky, sh.Kp, q.T, sh.Tp, g->skySdc.as<float>(),
+                                                                        q.nBins, B);
+  launch_shape<NS>(g, sh, reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), q.shape);
+}
+
+// ------------------------------------------------------------------------------------------------ the tuner
+// The shape chosen per problem (products, sensors to the next power of two, padded directions, steps), for the process.
+struct TuneKey {
+  int ns;
+  uint32_t ncPow2, Kp, T;
+  bool operator==(const TuneKey& o) const { return ns == o.ns && ncPow2 == o.ncPow2 && Kp == o.Kp && T == o.T; }
+};
+std::mutex tuneMu;
+std::vector<std::pair<TuneKey, int>> tuned;
+
+TuneKey tune_key(const dlg_series_plan& q, int ns) {
+  uint32_t p = 1;
+  while (p < q.nc) p <<= 1;
+  return {ns, p, MfmaShape(q.nc, q.nSky + q.nBins, uint32_t(q.T), ns).Kp, uint32_t(q.T)};
+}
+
+// the forced shape of DAYLIGHT_SERIES_TILE ("WMxGROUP"), or -1
+int forced_shape() {
+  const char* e = std::getenv("DAYLIGHT_SERIES_TILE");
+  if (!e) return -1;
+  int wm = 0;
+  unsigned grp = 0;
+  if (std::sscanf(e, "%dx%u", &wm, &grp) != 2) return -1;
+  for (int k = 0; k < kNumShapes; ++k)
+    if (kShapes[k].wm == wm && kShapes[k].group == grp) return k;
+  return -1;
+}
+
+// the tuned shape of the plan's problem, or -1 when it has not been timed
+int known_shape(const dlg_series_plan& q, int ns) {
+  if (const int f = forced_shape(); f >= 0) return f;
+  const TuneKey k = tune_key(q, ns);
+  std::lock_guard<std::mutex> lk(tuneMu);
+  for (const auto& [key, shape] : tuned)
+    if (key == k) return shape;
+  return -1;
+}
+
+// Times every shape on the chunk whose operands are split (each launch writes the same E) and records the fastest.
+template <int NS>
+int tune(dlg_gpu* g) {
+  const dlg_series_plan& q = g->seriesPlan;
+  const MfmaShape sh(q.nc, q.nSky + q.nBins, uint32_t(q.T), NS);
+  const u32x4* A = g->seriesWork.as<u32x4>();
+  const u32x4* B = reinterpret_cast<const u32x4*>(g->seriesWork.as<char>() + sh.aBytes);
+  int best = 0;
+  float bestMs = 0;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return 0;
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 0; }
+  for (int k = 0; k < kNumShapes; ++k) {
+    (void)hipEventRecord(e0, g->stream);
+    launch_shape<NS>(g, sh, A, B, k);
+    (void)hipEventRecord(e1, g->stream);
+    float ms = 0;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+    if (k == 0 || ms < bestMs) { best = k; bestMs = ms; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(tuneMu);
+  tuned.push_back({tune_key(q, NS), best});
+  return best;
 }
 
 }  // namespace
 
-extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, float* E, char* err, int32_t errLen) {
-  if (nc == 0 || in->T <= 0) return 0;
+int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, int32_t groundRow, char* err, int32_t errLen) {
   if (nc != g->nc) { dlg_fail(err, errLen, "dlg_series: no trace of this chunk"); return 1; }
   if (in->P != g->skyP || in->T != g->skyT) { dlg_fail(err, errLen, "dlg_series: S of another sky (run dlg_sky with series on)"); return 1; }
   if (in->reflect && (!g->skyReflect || g->dcN != nc || g->dcI0 != g->i0 || g->dcBins != uint32_t(g->skyP + 1))) {
     dlg_fail(err, errLen, "dlg_series: reflected light needs the chunk's dlg_bounce rows and a sky with reflect on");
     return 1;
   }
+  if (method < 0 || method > 2) { dlg_fail(err, errLen, "dlg_series: method 0 (FP32), 1 (BF16x3) or 2 (BF16x6)"); return 1; }
   const int T = in->T;
-  // inputs: row | wsky | G | sunStep-by-column | sunW (each 256-byte aligned), in the scratch buffer
+  dlg_series_plan& q = g->seriesPlan;
+  q = dlg_series_plan{};
+  q.method = method;
+  q.groundRow = groundRow;
+  q.reflect = in->reflect;
+  q.nc = uint32_t(nc);
+  q.nSky = in->nSky;
+  q.nBins = in->reflect ? g->dcBins : 0u;
+  q.sun0 = in->sun0;
+  q.T = T;
+  // inputs: row | wsky | G | the sun of each column | sun weights, each 256-byte aligned
   size_t off = 0;
   const auto take = [&](size_t bytes) { const size_t at = off; off += (bytes + 255) & ~size_t(255); return at; };
-  const size_t oRow = take(4 * size_t(in->nSky)), oW = take(4 * size_t(in->nSky)), oG = take(4 * size_t(nc)), oCol = take(4 * size_t(T)),
-               oSw = take(4 * size_t(in->nSun ? in->nSun : 1));
-  DLG_CHECK(g->scratch.grow(off));
-  char* b = g->scratch.as<char>();
+  q.oRow = take(4 * size_t(in->nSky));
+  q.oW = take(4 * size_t(in->nSky));
+  q.oG = take(4 * size_t(nc));
+  q.oCol = take(4 * size_t(T));
+  q.oSw = take(4 * size_t(in->nSun ? in->nSun : 1));
+  DLG_CHECK(g->seriesIn.grow(off));
+  char* b = g->seriesIn.as<char>();
   std::vector<int> sunOfCol(static_cast<size_t>(T), -1);
   for (uint32_t j = 0; j < in->nSun; ++j)   // suns of night steps carry weight 0 and column 0: they add nothing
     if (in->sunW[j] != 0.f) sunOfCol[size_t(in->sunStep[j])] = int(j);
-  DLG_CHECK(hipMemcpyAsync(b + oRow, in->row, 4 * size_t(in->nSky), hipMemcpyHostToDevice, g->stream));
-  DLG_CHECK(hipMemcpyAsync(b + oW, in->wsky, 4 * size_t(in->nSky), hipMemcpyHostToDevice, g->stream));
-  DLG_CHECK(hipMemcpyAsync(b + oG, in->G, 4 * size_t(nc), hipMemcpyHostToDevice, g->stream));
-  DLG_CHECK(hipMemcpyAsync(b + oCol, sunOfCol.data(), 4 * size_t(T), hipMemcpyHostToDevice, g->stream));
-  if (in->nSun) {
-    DLG_CHECK(hipMemcpyAsync(b + oSw, in->sunW, 4 * size_t(in->nSun), hipMemcpyHostToDevice, g->stream));
-  }
+  g->seriesCols = sunOfCol;   // kept until the copies below have run (the caller waits before the next prepare)
+  DLG_CHECK(hipMemcpyAsync(b + q.oRow, in->row, 4 * size_t(in->nSky), hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipMemcpyAsync(b + q.oW, in->wsky, 4 * size_t(in->nSky), hipMemcpyHostToDevice, g->stream));
+  if (groundRow < 0) DLG_CHECK(hipMemcpyAsync(b + q.oG, in->G, 4 * size_t(nc), hipMemcpyHostToDevice, g->stream));
+  DLG_CHECK(hipMemcpyAsync(b + q.oCol, g->seriesCols.data(), 4 * size_t(T), hipMemcpyHostToDevice, g->stream));
+  if (in->nSun) DLG_CHECK(hipMemcpyAsync(b + q.oSw, in->sunW, 4 * size_t(in->nSun), hipMemcpyHostToDevice, g->stream));
   DLG_CHECK(g->seriesE.grow(size_t(nc) * size_t(T) * 4));
+  if (method) {
+    const MfmaShape sh(uint32_t(nc), in->nSky + q.nBins, uint32_t(T), method == 1 ? 2 : 3);
+    DLG_CHECK(g->seriesWork.grow(sh.aBytes + sh.bBytes));
+    q.shape = known_shape(q, method == 1 ? 2 : 3);
+  }
   g->seriesN = nc;
-  const float* dg = g->skyG.as<float>();
+  return 0;
+}
+
+void dlg_series_enqueue(dlg_gpu* g) {
+  const dlg_series_plan& q = g->seriesPlan;
+  const char* b = g->seriesIn.as<char>();
+  float* G = reinterpret_cast<float*>(g->seriesIn.as<char>() + q.oG);
+  if (q.groundRow >= 0) k_ground_rows<<<(q.nc + 255) / 256, 256, 0, g->stream>>>(g->out.as<double>() + size_t(q.groundRow) * q.nc, q.nc, G);
+  if (q.method == 1) { enqueue_mfma<2>(g); return; }
+  if (q.method == 2) { enqueue_mfma<3>(g); return; }
+  const float3* N = g->N.as<float3>() + g->i0;
+  const int T = q.T;
+  k_series_f32<<<dim3((T + kTile - 1) / kTile, (q.nc + kTile - 1) / kTile), 256, 0, g->stream>>>(
+      g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(), q.nSky, reinterpret_cast<const int*>(b + q.oRow),
+      reinterpret_cast<const float*>(b + q.oW), g->skyS.as<float>(), T, G, g->skyG.as<float>(), q.sun0, reinterpret_cast<const int*>(b + q.oCol),
+      reinterpret_cast<const float*>(b + q.oSw), g->seriesE.as<float>());
+  if (q.reflect)
+    k_series_dc_f32<<<dim3((T + kTile - 1) / kTile, (q.nc + kTile - 1) / kTile), 256, 0, g->stream>>>(g->dc.as<float>(), q.nc, g->dcBins,
+                                                                                                       g->skySdc.as<float>(), T, g->seriesE.as<float>());
+}
+
+bool dlg_series_tuned(const dlg_gpu* g) { return g->seriesPlan.method == 0 || g->seriesPlan.shape >= 0; }
+
+extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, float* E, char* err, int32_t errLen) {
+  if (nc == 0 || in->T <= 0) return 0;
+  if (dlg_series_prepare(g, nc, in, method, -1, err, errLen)) return 1;
+  dlg_series_plan& q = g->seriesPlan;
+  if (method && q.shape < 0) {   // the first chunk of this problem: split its operands and time every tile shape on it
+    q.shape = 0;
+    dlg_series_enqueue(g);
+    q.shape = method == 1 ? tune<2>(g) : tune<3>(g);
+  }
   dlg_tbegin(g);
-  if (method == 0) {
-    const float3* N = g->N.as<float3>() + g->i0;
-    k_series_f32<<<dim3((T + kTile - 1) / kTile, unsigned((nc + kTile - 1) / kTile)), 256, 0, g->stream>>>(
-        g->bits.as<uint32_t>(), uint32_t(g->bytes), uint32_t(nc), N, g->D.as<float3>(), in->nSky, reinterpret_cast<const int*>(b + oRow),
-        reinterpret_cast<const float*>(b + oW), g->skyS.as<float>(), T, reinterpret_cast<const float*>(b + oG), dg, in->sun0,
-        reinterpret_cast<const int*>(b + oCol), reinterpret_cast<const float*>(b + oSw), g->seriesE.as<float>());
-    if (in->reflect)
-      k_series_dc_f32<<<dim3((T + kTile - 1) / kTile, unsigned((nc + kTile - 1) / kTile)), 256, 0, g->stream>>>(
-          g->dc.as<float>(), uint32_t(nc), g->dcBins, g->skySdc.as<float>(), T, g->seriesE.as<float>());
-    DLG_CHECK(hipGetLastError());
-  } else if (method == 1 || method == 2) {
-    const int32_t rc = (method == 1 ? run_mfma<2> : run_mfma<3>)(g, uint32_t(nc), in, reinterpret_cast<const float*>(b + oG), dg,
-                                                              reinterpret_cast<const int*>(b + oRow), reinterpret_cast<const float*>(b + oW),
-                                                              reinterpret_cast<const int*>(b + oCol), reinterpret_cast<const float*>(b + oSw),
-                                                              err, errLen);
-    if (rc) return rc;
-  } else { dlg_fail(err, errLen, "dlg_series: method 0 (FP32), 1 (BF16x3) or 2 (BF16x6)"); return 1; }
+  dlg_series_enqueue(g);
+  DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   g->stats.seriesMs += ms;
   g->stats.lastSeriesMs = ms;
   if (E) {
-    DLG_CHECK(hipMemcpyAsync(E, g->seriesE.p, size_t(nc) * size_t(T) * 4, hipMemcpyDeviceToHost, g->stream));
+    DLG_CHECK(hipMemcpyAsync(E, g->seriesE.p, size_t(nc) * size_t(in->T) * 4, hipMemcpyDeviceToHost, g->stream));
     DLG_CHECK(hipStreamSynchronize(g->stream));
   }
   return 0;

@@ -272,7 +272,10 @@ void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32
 
 }  // namespace
 
-extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, char* err, int32_t errLen) {
+// The checks and buffers of a trace of sensors [i0, i0 + nc): 0, or 1 with err. stack: the LDS stack entries the
+// scene's tree needs (the smallest instantiation that covers it, with the two entries above the top that the
+// unconditional pushes write).
+int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen) {
   if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_trace: sensor range exceeds the scene"); return 1; }
   const bool bytes = !(rayFlags & 1u);   // any-hit on: a slot transmits
   const dlg_scene& s = g->scene;
@@ -282,47 +285,76 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     return 1;
   }
   DLG_CHECK(g->bits.grow(size_t(g->nWords) * size_t(nc) * (bytes ? 32 : 4)));
-  g->i0 = i0;
-  g->nc = nc;
-  g->bytes = bytes ? 1 : 0;
-  if (nc == 0 || g->nWords == 0) return 0;
-  const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
-  const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(g->nWords));
-  const float3* P = g->P.as<float3>() + i0;
-  const float3* N = g->N.as<float3>() + i0;
   if (!g->traceCount.p) {   // the running count of traced rays, read after each launch
     DLG_CHECK(g->traceCount.grow(sizeof(unsigned long long)));
     DLG_CHECK(hipMemsetAsync(g->traceCount.p, 0, sizeof(unsigned long long), g->stream));
   }
+  g->i0 = i0;
+  g->nc = nc;
+  g->bytes = bytes ? 1 : 0;
+  const int need = dlw::wide_stack(s.height) + 2;
+  if (need > 160) { dlg_fail(err, errLen, "dlg_trace: the scene's tree is too deep for the traversal stack"); return 1; }
+  stack = need <= 24 ? 24 : need <= 32 ? 32 : need <= 48 ? 48 : need <= 64 ? 64 : need <= 96 ? 96 : 160;
+  return 0;
+}
+
+// The trace's kernel enqueued on the library's stream (no wait: a graph may capture it); counts non-null runs the
+// counting variant.
+void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts) {
+  const int64_t nc = g->nc;
+  if (nc == 0 || g->nWords == 0) return;
+  const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
+  const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(g->nWords));
+  const float3* P = g->P.as<float3>() + g->i0;
+  const float3* N = g->N.as<float3>() + g->i0;
+  const auto launch = [&](auto count) {
+    constexpr bool C_ = decltype(count)::value;
+    switch (stack) {
+      case 24: launch_bits<24, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
+      case 32: launch_bits<32, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
+      case 48: launch_bits<48, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
+      case 64: launch_bits<64, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
+      case 96: launch_bits<96, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
+      default: launch_bits<160, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
+    }
+  };
+  if (g->bytes) {
+    switch (stack) {
+      case 24: launch_bytes<24>(g, grid, P, N, uint32_t(nc), cull); break;
+      case 32: launch_bytes<32>(g, grid, P, N, uint32_t(nc), cull); break;
+      case 48: launch_bytes<48>(g, grid, P, N, uint32_t(nc), cull); break;
+      case 64: launch_bytes<64>(g, grid, P, N, uint32_t(nc), cull); break;
+      case 96: launch_bytes<96>(g, grid, P, N, uint32_t(nc), cull); break;
+      default: launch_bytes<160>(g, grid, P, N, uint32_t(nc), cull); break;
+    }
+  } else if (counts) launch(std::true_type{});
+  else launch(std::false_type{});
+}
+
+// the statistics of a finished trace launch of ms milliseconds
+int32_t dlg_trace_account(dlg_gpu* g, float ms, char* err, int32_t errLen) {
+  unsigned long long traced = 0;
+  DLG_CHECK(hipMemcpy(&traced, g->traceCount.p, sizeof traced, hipMemcpyDeviceToHost));
+  g->stats.traced = int64_t(traced);
+  g->stats.traceMs += ms;
+  g->stats.lastTraceMs = ms;
+  g->stats.rays += g->nc * g->nDir;
+  g->stats.launches += 1;
+  return 0;
+}
+
+extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, char* err, int32_t errLen) {
+  int stack = 0;
+  if (dlg_trace_prepare(g, i0, nc, rayFlags, stack, err, errLen)) return 1;
+  if (nc == 0 || g->nWords == 0) return 0;
   unsigned long long* counts = nullptr;
-  if (g->countFetches && !bytes) {
+  if (g->countFetches && !g->bytes) {
     DLG_CHECK(g->counts.grow(3 * sizeof(unsigned long long)));
     counts = g->counts.as<unsigned long long>();
     DLG_CHECK(hipMemsetAsync(counts, 0, 3 * sizeof(unsigned long long), g->stream));
   }
-  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU), with the two entries
-  // above the top that the unconditional pushes write
-  const int need = dlw::wide_stack(s.height) + 2;
-  if (need > 160) { dlg_fail(err, errLen, "dlg_trace: the scene's tree is too deep for the traversal stack"); return 1; }
-  const auto launch = [&](auto count) {
-    constexpr bool C_ = decltype(count)::value;
-    if (need <= 24) launch_bits<24, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
-    else if (need <= 32) launch_bits<32, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
-    else if (need <= 48) launch_bits<48, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
-    else if (need <= 64) launch_bits<64, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
-    else if (need <= 96) launch_bits<96, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
-    else launch_bits<160, C_>(g, grid, P, N, uint32_t(nc), cull, counts);
-  };
   dlg_tbegin(g);
-  if (bytes) {
-    if (need <= 24) launch_bytes<24>(g, grid, P, N, uint32_t(nc), cull);
-    else if (need <= 32) launch_bytes<32>(g, grid, P, N, uint32_t(nc), cull);
-    else if (need <= 48) launch_bytes<48>(g, grid, P, N, uint32_t(nc), cull);
-    else if (need <= 64) launch_bytes<64>(g, grid, P, N, uint32_t(nc), cull);
-    else if (need <= 96) launch_bytes<96>(g, grid, P, N, uint32_t(nc), cull);
-    else launch_bytes<160>(g, grid, P, N, uint32_t(nc), cull);
-  } else if (counts) launch(std::true_type{});
-  else launch(std::false_type{});
+  dlg_trace_enqueue(g, rayFlags, stack, counts);
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   if (counts) {
@@ -333,12 +365,5 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
     g->stats.waveIters += int64_t(c[2]);
     g->stats.countedRays += nc * g->nDir;
   }
-  unsigned long long traced = 0;
-  DLG_CHECK(hipMemcpy(&traced, g->traceCount.p, sizeof traced, hipMemcpyDeviceToHost));
-  g->stats.traced = int64_t(traced);
-  g->stats.traceMs += ms;
-  g->stats.lastTraceMs = ms;
-  g->stats.rays += nc * g->nDir;
-  g->stats.launches += 1;
-  return 0;
+  return dlg_trace_account(g, ms, err, errLen);
 }

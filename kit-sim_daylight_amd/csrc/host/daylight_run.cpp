@@ -470,6 +470,10 @@ class DaylightRun {
     for (size_t i0 = 0; i0 < nSens; i0 += ncMax) {
       if (cancelled()) return false;
       const size_t nc = std::min(ncMax, nSens - i0);
+      if (!reflect) {   // one graph: trace, reduce, series (dlg_chunk)
+        if (!graph_chunk(i0, nc, flags, useCos, nR, wsky, sunCol, sunW, method, sums, E)) return false;
+        continue;
+      }
       if (dlg_trace(g, int64_t(i0), int64_t(nc), flags, e, sizeof e)) return fail(e);
       // the bounce pass writes only its own rows: the reduces still see the trace's visibility
       if (reflect && dlg_bounce(g, int64_t(i0), int64_t(nc), dlg_bounce_flags(O.cullBackfaces ? 1 : 0), &bounce, e, sizeof e)) return fail(e);
@@ -503,6 +507,35 @@ class DaylightRun {
     }
     mark("trace, reduce, series");
     stats();
+    return true;
+  }
+
+  // A chunk as one graph on the device (dlg_chunk): the trace, every quantity's reduce and the series, whose ground
+  // view factor comes from the reduce's last row on the device; then the columns, the dump and the series' output.
+  bool graph_chunk(size_t i0, size_t nc, uint32_t flags, const std::vector<int32_t>& useCos, int nR, const std::vector<float>& wsky,
+                   const std::vector<int32_t>& sunCol, const std::vector<float>& sunW, int method, std::vector<double>& sums, std::vector<float>& E) {
+    const bool compute = series && Tup > 0 && needWeather;
+    const dlg_series_in in{uint32_t(nSky), rowOf.data(), wsky.data(), pa.P, Tup, nullptr, uint32_t(sun0), uint32_t(nSun), sunCol.data(), sunW.data(), 0};
+    const dlg_chunk_in ci{int64_t(i0), int64_t(nc), flags, nR, Wq.data(), useCos.data(), compute ? &in : nullptr, method};
+    const auto t0 = Clk::now();
+    dlg_stats_t a{}, b{};
+    dlg_stats(g, &a);
+    sums.assign(size_t(nR) * nc, 0.0);
+    const bool toFile = series && !sink;
+    E.assign(toFile ? nc * size_t(Tup) : 0, 0.f);
+    char e[512] = {};
+    if (dlg_chunk(g, &ci, sums.data(), toFile && compute ? E.data() : nullptr, e, sizeof e)) return fail(e);
+    dlg_stats(g, &b);
+    const auto tStore = Clk::now();
+    for (size_t qi = 0; qi < nQ(); ++qi)
+      for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, sums[qi * nc + i]);
+    out.stats.reduceMs += (b.reduceMs - a.reduceMs) + ms_since(tStore);
+    (void)t0;
+    if (O.progress) O.progress->store(double(i0 + nc) / double(nSens));
+    if (bitsFile && !dump_visibility(nc, (nDir + 31) / 32)) return false;
+    if (!series) return true;
+    if (sink) return !*sink || (*sink)(i0, nc, size_t(Tup), compute ? dlg_series_device(g) : nullptr, err);
+    if (std::fwrite(E.data(), sizeof(float), E.size(), seriesFile.get()) != E.size()) return fail("write failed: " + O.seriesPath);
     return true;
   }
 

@@ -7,6 +7,7 @@
 // (reference solar.cpp). The per-step sun and Perez sky of a run are computed on the GPU (sky.hip); clear_sky_weather,
 // which makes an input file's worth of weather, calls the same __host__ __device__ sun as the kernel.
 #include <algorithm>
+#include <array>
 #include <charconv>
 #include <cmath>
 #include <cstring>
@@ -32,39 +33,48 @@ int month_days(int m, bool leap) { return dlm::month_days(m, leap); }
 
 float3 altaz(double alt, double azi) { return dlm::altaz_vec(alt, azi); }
 
-// weather between the centres of hour-ending rows of a cyclic year: row r at r + 0.5
+// The weather at hour t of a cyclic year of hour-ending rows (row r stands for r + 0.5): linear between the two rows
+// whose centres bracket t.
 float lerp_rows(const std::vector<float>& v, double t) {
   const long n = long(v.size());
-  const double x = t - 0.5, f = x - std::floor(x);
-  const long a = ((long(std::floor(x)) % n) + n) % n, b = (a + 1) % n;
-  return float(v[size_t(a)] * (1 - f) + v[size_t(b)] * f);
+  const double x = t - 0.5, below = std::floor(x), f = x - below;
+  const long lo = ((long(below) % n) + n) % n, hi = (lo + 1) % n;
+  return float(v[size_t(lo)] * (1 - f) + v[size_t(hi)] * f);
 }
 
-// PVGIS's satellite irradiance runs ahead of its hour labels; measure the lead from each day's first and last lit rows
-// against solar noon and move DNI / DHI later by it when it is clearly off (reference solar.cpp:36-63).
+// PVGIS labels satellite irradiance ahead of its hours: the mean offset of each day's lit midpoint (first and last rows
+// with GHI > 0, days lit at midnight left out) from solar noon is its lead; over 30 days or more and beyond 0.3 h, DNI
+// and DHI move later by it (reference solar.cpp:36-63). A file that declares its -0.5 h shift is left as it is.
 void pvgis_align(Weather& w, const std::vector<float>& ghi, double declared) {
   if (std::fabs(declared + 0.5) < 0.05) return;
   const double noon = 12 - (w.lon / 15 - w.tz);
-  double sum = 0;
-  int n = 0;
-  for (size_t day = 0; (day + 1) * 24 <= ghi.size(); ++day) {
-    int first = -1, last = -1;
-    for (int h = 0; h < 24; ++h)
-      if (ghi[day * 24 + size_t(h)] > 0) { if (first < 0) first = h; last = h; }
-    if (first <= 0 || last >= 23) continue;
-    sum += pmod((first + last) / 2.0 + 0.5 - noon + 12, 24) - 12;
-    ++n;
+  std::vector<double> leads;
+  for (size_t d0 = 0; d0 + 24 <= ghi.size(); d0 += 24) {
+    const auto lit = [&](int h) { return ghi[d0 + size_t(h)] > 0; };
+    int first = 0, last = 23;
+    while (first < 24 && !lit(first)) ++first;
+    while (last >= 0 && !lit(last)) --last;
+    if (first >= 24 || first == 0 || last == 23) continue;   // dark, or lit at either end of the day
+    leads.push_back(pmod((first + last) / 2.0 + 0.5 - noon + 12, 24) - 12);
   }
-  if (n < 30) return;
-  const double lead = sum / n;
+  if (leads.size() < 30) return;
+  double sum = 0;
+  for (const double l : leads) sum += l;
+  const double lead = sum / int(leads.size());
   if (std::fabs(lead) <= 0.3) return;
   const std::vector<float> dni = w.dni, dhi = w.dhi;
   for (size_t r = 0; r < dni.size(); ++r) {
-    const double t = double(r) + 0.5 + lead;
-    w.dni[r] = lerp_rows(dni, t);
-    w.dhi[r] = lerp_rows(dhi, t);
+    w.dni[r] = lerp_rows(dni, double(r) + 0.5 + lead);
+    w.dhi[r] = lerp_rows(dhi, double(r) + 0.5 + lead);
   }
   w.shiftH = -lead;
+}
+
+// the day of the year each month starts on (0-based), and the year's length in days after them
+std::array<int, 13> month_starts(bool leap) {
+  std::array<int, 13> s{};
+  for (int m = 1; m <= 12; ++m) s[size_t(m)] = s[size_t(m - 1)] + month_days(m, leap);
+  return s;
 }
 }  // namespace
 
@@ -121,73 +131,92 @@ const double* tcos_table() {
 }
 
 void hoy_to_date(double hoy, bool leap, int& month, int& day, double& hour) {
-  int d = int(std::floor(hoy / 24));
-  hour = hoy - d * 24.0;
-  month = 1;
-  while (month < 12 && d >= month_days(month, leap)) d -= month_days(month++, leap);
-  day = d + 1;
+  const int doy = int(std::floor(hoy / 24));
+  hour = hoy - doy * 24.0;
+  const std::array<int, 13> start = month_starts(leap);
+  month = 1;   // the last month that starts on or before doy (December takes the days past the year)
+  while (month < 12 && doy >= start[size_t(month)]) ++month;
+  day = doy - start[size_t(month - 1)] + 1;
 }
 
 bool parse_hoy(const std::string& s, bool leap, double& hoy, std::string& err) {
-  int y, m, d, h, mi;
-  if (std::sscanf(s.c_str(), "%d-%d-%dT%d:%d", &y, &m, &d, &h, &mi) != 5 && std::sscanf(s.c_str(), "%d-%dT%d:%d", &m, &d, &h, &mi) != 4) {
+  int year, m, d, h, mi;
+  const bool dated = std::sscanf(s.c_str(), "%d-%d-%dT%d:%d", &year, &m, &d, &h, &mi) == 5;
+  if (!dated && std::sscanf(s.c_str(), "%d-%dT%d:%d", &m, &d, &h, &mi) != 4) {
     err = "bad timestamp '" + s + "', expected [YYYY-]MM-DDTHH:MM";
     return false;
   }
-  if (m < 1 || m > 12 || d < 1 || d > month_days(m, leap) || h < 0 || h > 23 || mi < 0 || mi > 59) {
+  const bool inRange = m >= 1 && m <= 12 && d >= 1 && d <= month_days(m, leap) && h >= 0 && h <= 23 && mi >= 0 && mi <= 59;
+  if (!inRange) {
     err = "timestamp out of range '" + s + "'";
     return false;
   }
   hoy = 24.0 * (d - 1) + h + mi / 60.0;
-  for (int k = 1; k < m; ++k) hoy += 24 * month_days(k, leap);
+  for (int k = 1; k < m; ++k) hoy += 24 * month_days(k, leap);   // whole months before, added one by one
   return true;
 }
 
-bool make_steps(const Options& opt, const Weather* w, std::vector<Step>& out, std::string& err, double shift) {
-  out.clear();
-  const bool leap = w && w->leap;
-  const double nh = leap ? 8784 : 8760;
-  const auto wrap = [&](double t) { return t - nh * std::floor(t / nh); };
-  if (std::isnan(opt.t0)) {   // annual: a step per weather row, at its midpoint
-    if (!w) { err = "no weather and no time range: nothing to evaluate"; return false; }
-    out.reserve(w->dni.size());
-    for (size_t r = 0; r < w->dni.size(); ++r) {
-      Step s;
-      s.hoy = wrap(double(r) + 0.5 + shift);
-      s.jd = int(r / 24) + 1;
-      s.dt = 1;
-      s.dni = w->dni[r];
-      s.dhi = w->dhi[r];
-      hoy_to_date(double(r) + 0.5, leap, s.month, s.day, s.hour);
-      out.push_back(s);
-    }
-    return true;
+namespace {
+
+// One step per weather row, evaluated at the row's middle (moved by `shift` into the weather's clock).
+void annual_steps(const Weather& w, double shift, double yearH, std::vector<Step>& out) {
+  out.resize(w.dni.size());
+  for (size_t r = 0; r < out.size(); ++r) {
+    Step& s = out[r];
+    const double mid = double(r) + 0.5, at = mid + shift;
+    s.hoy = at - yearH * std::floor(at / yearH);
+    s.jd = int(r / 24) + 1;
+    s.dt = 1;
+    s.dni = w.dni[r];
+    s.dhi = w.dhi[r];
+    hoy_to_date(mid, w.leap, s.month, s.day, s.hour);
   }
+}
+
+// Steps of stepMin minutes from t0 up to t1 (one step at t0 when t1 is NaN), inside the [dayFrom, dayTo) minutes of
+// every day: a step lasts to the next one, the range's end or the day window's end, whichever is first, and is
+// evaluated at its instant (--time) or its middle; the weather is interpolated at that time in the weather's clock.
+bool ranged_steps(const Options& opt, const Weather* w, double shift, double yearH, std::vector<Step>& out, std::string& err) {
+  const bool leap = w && w->leap;
   if (opt.stepMin <= 0) { err = "step must be a positive number of minutes"; return false; }
   const bool instant = std::isnan(opt.t1), window = opt.dayFrom > 0 || opt.dayTo < 1440;
-  const double step = opt.stepMin / 60.0;
-  double t1 = instant ? opt.t0 + step : opt.t1;
-  if (t1 > nh) t1 = nh;
-  if (opt.t0 < 0 || opt.t0 >= nh || t1 <= opt.t0) { err = "time range must satisfy 0 <= from < to <= end of year"; return false; }
+  const double step = opt.stepMin / 60.0, end = std::min(instant ? opt.t0 + step : opt.t1, yearH);
+  if (opt.t0 < 0 || opt.t0 >= yearH || end <= opt.t0) { err = "time range must satisfy 0 <= from < to <= end of year"; return false; }
   for (int k = 0;; ++k) {
-    const double hoy = opt.t0 + k * step;
-    if (hoy >= t1 - 1e-9) break;
-    double hod = (hoy - 24 * std::floor(hoy / 24)) * 60;
-    if (hod >= 1440 - 1e-6) hod -= 1440;
-    if (hod < opt.dayFrom - 1e-6 || hod >= opt.dayTo - 1e-6) continue;
+    const double start = opt.t0 + k * step;
+    if (start >= end - 1e-9) break;
+    double minute = (start - 24 * std::floor(start / 24)) * 60;   // into the day (the --hours window)
+    if (minute >= 1440 - 1e-6) minute -= 1440;                     // a step at 24 n - eps is midnight's
+    if (minute < opt.dayFrom - 1e-6 || minute >= opt.dayTo - 1e-6) continue;
     double dt = step;
-    if (!instant) dt = std::min(dt, t1 - hoy);
-    if (!instant && window) dt = std::min(dt, (opt.dayTo - hod) / 60);
+    if (!instant) {
+      dt = std::min(dt, end - start);
+      if (window) dt = std::min(dt, (opt.dayTo - minute) / 60);
+    }
     Step s;
-    s.hoy = instant ? hoy : hoy + dt / 2;
-    const double at = wrap(s.hoy - shift);
+    s.hoy = instant ? start : start + dt / 2;
+    const double at = (s.hoy - shift) - yearH * std::floor((s.hoy - shift) / yearH);
     s.jd = int(at / 24) + 1;
     s.dt = dt;
     hoy_to_date(at, leap, s.month, s.day, s.hour);
-    if (w && !w->dni.empty()) { s.dni = lerp_rows(w->dni, at); s.dhi = lerp_rows(w->dhi, at); }
+    if (w && !w->dni.empty()) {
+      s.dni = lerp_rows(w->dni, at);
+      s.dhi = lerp_rows(w->dhi, at);
+    }
     out.push_back(s);
   }
   if (out.empty()) { err = "no step falls inside the --hours window"; return false; }
+  return true;
+}
+
+}  // namespace
+
+bool make_steps(const Options& opt, const Weather* w, std::vector<Step>& out, std::string& err, double shift) {
+  out.clear();
+  const double yearH = w && w->leap ? 8784 : 8760;
+  if (!std::isnan(opt.t0)) return ranged_steps(opt, w, shift, yearH, out, err);
+  if (!w) { err = "no weather and no time range: nothing to evaluate"; return false; }
+  annual_steps(*w, shift, yearH, out);
   return true;
 }
 

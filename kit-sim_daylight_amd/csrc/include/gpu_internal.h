@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "dl_device.h"
 
@@ -55,8 +56,20 @@ struct dlg_scene {
   }
 };
 
+// the series of the current chunk as dlg_series_prepare set it up (offsets into dlg_gpu::seriesIn)
+struct dlg_series_plan {
+  int32_t method = 0, groundRow = -1, reflect = 0, T = 0;
+  int32_t shape = -1;   // the MFMA methods' tile shape (series.hip kShapes), -1 until tuned
+  uint32_t nc = 0, nSky = 0, nBins = 0, sun0 = 0;
+  size_t oRow = 0, oW = 0, oG = 0, oCol = 0, oSw = 0;
+};
+
+struct dlg_graph_cache;   // chunk.hip: the chunk's captured graph and its events
+
 struct dlg_gpu {
   hipStream_t stream = nullptr;
+  dlg_graph_cache* graph = nullptr;
+  int32_t graphBroken = 0;   // a capture failed: chunks run their stages one by one
   hipEvent_t ev[2] = {};
   dlg_scene scene;
   // resident sensors and directions; per direction its ray constants (dl_math.h dir_pack: 2 float4)
@@ -75,9 +88,11 @@ struct dlg_gpu {
   // sky (dlg_sky): S [P x Tup] and g [Tup] of the last run, the tcos table
   DevBuf skyS, skyG, skyWork;
   int32_t skyP = 0, skyT = 0, skyReflect = 0;
-  // series
-  DevBuf seriesE, seriesWork;
+  // series: the output, the split operands, the inputs (seriesPlan) and the host copy of the sun per column
+  DevBuf seriesE, seriesWork, seriesIn;
   int64_t seriesN = 0;
+  dlg_series_plan seriesPlan;
+  std::vector<int> seriesCols;
   // views: classes | distances | row sines of the last dlg_view
   DevBuf view, labels;
   int64_t viewN = 0, viewK = 0, viewI0 = 0;
@@ -99,6 +114,9 @@ struct dlg_gpu {
   double meshC[3] = {0, 0, 0};
 };
 
+// chunk.hip: frees the chunk's graph
+void dlg_graph_free(dlg_gpu* g);
+
 // lbvh.hip: the build over triangles already on the device (v [nv] float3, tri [nt] uint3), mx the largest |coordinate|
 int32_t dlg_build_device(dlg_gpu* g, const float* v, int64_t nv, const uint32_t* tri, int64_t nt, float mx, char* err, int32_t errLen);
 // gpu_ctx.hip: leaf slots from per-triangle slots (input order) already on the device, largest slot mx
@@ -108,6 +126,21 @@ int32_t dlg_leaf_slots_device(dlg_gpu* g, const uint32_t* slot, int64_t nt, uint
 // place), and the device bytes of its temporaries
 int32_t dlg_sah_tree(dlg_gpu* g, int64_t nt, float* leaves, float* nodes, int32_t& height, char* err, int32_t errLen);
 int64_t dlg_sah_bytes(int64_t nt);
+
+// The stages of a chunk split into what must run outside a graph capture (checks, buffer growth, host uploads) and
+// what a capture can hold (kernel launches on g->stream, no waits). trace.hip:
+int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen);
+void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts);
+int32_t dlg_trace_account(dlg_gpu* g, float ms, char* err, int32_t errLen);
+// reduce.hip: W [nQ x nDir] uploaded, the output grown; the quantities' cosine mask
+int32_t dlg_reduce_prepare(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, uint32_t& mask, char* err, int32_t errLen);
+void dlg_reduce_enqueue(dlg_gpu* g, uint32_t nc, int32_t nQ, uint32_t mask);
+// series.hip: the series inputs uploaded (groundRow >= 0: G taken on the device from row groundRow of the reduce's
+// output instead of in->G), the output and operands grown
+int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, int32_t groundRow, char* err, int32_t errLen);
+void dlg_series_enqueue(dlg_gpu* g);
+// whether the prepared series has its tile shape (the MFMA methods time every shape on a problem's first chunk)
+bool dlg_series_tuned(const dlg_gpu* g);
 
 // event timing on the library's stream: ms of the work enqueued between begin and end
 inline void dlg_tbegin(dlg_gpu* g) { (void)hipEventRecord(g->ev[0], g->stream); }

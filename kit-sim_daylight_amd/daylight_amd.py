@@ -50,7 +50,8 @@ class Stats(C.Structure):
                [(n, C.c_int64) for n in ("rays", "traced", "launches", "sceneBytes", "nodes", "height")] + \
                [(n, C.c_double) for n in ("lastTraceMs", "lastReduceMs", "lastSeriesMs", "lastSkyMs", "lastViewMs", "lastViewReduceMs")] + \
                [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays", "waveIters")] + \
-               [("bounceMs", C.c_double), ("paths", C.c_int64), ("bounceLaunches", C.c_int64)]
+               [("bounceMs", C.c_double), ("paths", C.c_int64), ("bounceLaunches", C.c_int64)] + \
+               [("graphCaptures", C.c_int64), ("graphLaunches", C.c_int64)]
 
 
 class BounceIn(C.Structure):
@@ -64,7 +65,7 @@ class ViewIn(C.Structure):
 
 
 SERIES_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p)
-DLX_COUNTERS = 17
+DLX_COUNTERS = 19
 
 
 # every function include/*.h declares: (name, restype, argtypes)
@@ -138,6 +139,7 @@ SEAM = [
     ("dlg_set_sky_series", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
     ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("dlg_chunk", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),
     ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
     ("dlg_count_fetches", None, [C.c_void_p, C.c_int32]),
