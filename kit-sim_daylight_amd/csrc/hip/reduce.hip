@@ -93,10 +93,22 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
 }
 
 template <int NQ>
-void launch(dlg_gpu* g, uint32_t nc, uint32_t mask) {
-  const auto k = g->bytes ? k_reduce<NQ, true> : k_reduce<NQ, false>;
-  k<<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(g->bits.as<uint32_t>(), nc, uint32_t(g->nWords), g->N.as<float3>() + g->i0, g->D.as<float3>(),
-                                                          g->scratch.as<float>(), uint32_t(g->nDir), mask, g->out.as<double>());
+void launch(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float3* N, const float* W, uint32_t mask, double* out) {
+  const auto k = bytes ? k_reduce<NQ, true> : k_reduce<NQ, false>;
+  k<<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(vis, nc, uint32_t(g->nWords), N, g->D.as<float3>(), W, uint32_t(g->nDir), mask, out);
+}
+
+void launch_nq(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float3* N, const float* W, int32_t nQ, uint32_t mask, double* out) {
+  switch (nQ) {
+    case 1: launch<1>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 2: launch<2>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 3: launch<3>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 4: launch<4>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 5: launch<5>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 6: launch<6>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 7: launch<7>(g, vis, bytes, nc, N, W, mask, out); break;
+    default: launch<8>(g, vis, bytes, nc, N, W, mask, out); break;
+  }
 }
 
 }  // namespace
@@ -113,16 +125,12 @@ int32_t dlg_reduce_prepare(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, c
 }
 
 void dlg_reduce_enqueue(dlg_gpu* g, uint32_t nc, int32_t nQ, uint32_t mask) {
-  switch (nQ) {
-    case 1: launch<1>(g, nc, mask); break;
-    case 2: launch<2>(g, nc, mask); break;
-    case 3: launch<3>(g, nc, mask); break;
-    case 4: launch<4>(g, nc, mask); break;
-    case 5: launch<5>(g, nc, mask); break;
-    case 6: launch<6>(g, nc, mask); break;
-    case 7: launch<7>(g, nc, mask); break;
-    default: launch<8>(g, nc, mask); break;
-  }
+  launch_nq(g, g->bits.as<uint32_t>(), g->bytes != 0, nc, g->N.as<float3>() + g->i0, g->scratch.as<float>(), nQ, mask, g->out.as<double>());
+}
+
+void dlg_reduce_enqueue_from(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float* N, const float* W, int32_t nQ, uint32_t mask,
+                             double* out) {
+  launch_nq(g, vis, bytes, nc, reinterpret_cast<const float3*>(N), W, nQ, mask, out);
 }
 
 extern "C" int32_t dlg_reduce(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, double* out, char* err, int32_t errLen) {

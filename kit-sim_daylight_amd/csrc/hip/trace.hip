@@ -252,22 +252,59 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
   add_traced(nTraced, lane, traced);
 }
 
+// the launch's buffers: sensors, the visibility out, the traced-ray count (added to) and the counting variant's counters
+struct TraceBufs {
+  const float3* P;
+  const float3* N;
+  uint32_t nc;
+  uint32_t* vis;
+  unsigned long long* traced;
+  unsigned long long* counts;
+};
+
 template <int STACK>
-void launch_bytes(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32_t nc, uint32_t cull) {
+void launch_bytes(dlg_gpu* g, dim3 grid, const TraceBufs& b, uint32_t cull) {
   const dlg_scene& s = g->scene;
   const auto k = s.nNodes < (int64_t(1) << 25) ? k_bytes<STACK, true> : k_bytes<STACK, false>;
-  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(), g->dirTab.as<float4>(),
-                                   uint32_t(g->nDir), nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), g->bits.as<uint32_t>(),
-                                   g->traceCount.as<unsigned long long>());
+  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, b.P, b.N, g->D.as<float3>(), g->dirTab.as<float4>(),
+                                   uint32_t(g->nDir), b.nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), b.vis, b.traced);
 }
 
 template <int STACK, bool COUNT>
-void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32_t nc, uint32_t cull, unsigned long long* counts) {
+void launch_bits(dlg_gpu* g, dim3 grid, const TraceBufs& b, uint32_t cull) {
   const dlg_scene& s = g->scene;
   const auto k = s.nNodes < (int64_t(1) << 25) ? k_bits<STACK, COUNT, true> : k_bits<STACK, COUNT, false>;
-  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, P, N, g->D.as<float3>(),
-                                                      g->dirTab.as<float4>(), uint32_t(g->nDir), nc, cull, g->bits.as<uint32_t>(), counts,
-                                                      g->traceCount.as<unsigned long long>());
+  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, b.P, b.N, g->D.as<float3>(), g->dirTab.as<float4>(),
+                                   uint32_t(g->nDir), b.nc, cull, b.vis, b.counts, b.traced);
+}
+
+// the trace kernel of the stack size and the mode, over the buffers b
+void launch_trace(dlg_gpu* g, const TraceBufs& b, uint32_t rayFlags, int stack, bool bytes) {
+  if (b.nc == 0 || g->nWords == 0) return;
+  const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
+  const dim3 grid(unsigned((b.nc + kWave - 1) / kWave), unsigned(g->nWords));
+  const auto launch = [&](auto count) {
+    constexpr bool C_ = decltype(count)::value;
+    switch (stack) {
+      case 24: launch_bits<24, C_>(g, grid, b, cull); break;
+      case 32: launch_bits<32, C_>(g, grid, b, cull); break;
+      case 48: launch_bits<48, C_>(g, grid, b, cull); break;
+      case 64: launch_bits<64, C_>(g, grid, b, cull); break;
+      case 96: launch_bits<96, C_>(g, grid, b, cull); break;
+      default: launch_bits<160, C_>(g, grid, b, cull); break;
+    }
+  };
+  if (bytes) {
+    switch (stack) {
+      case 24: launch_bytes<24>(g, grid, b, cull); break;
+      case 32: launch_bytes<32>(g, grid, b, cull); break;
+      case 48: launch_bytes<48>(g, grid, b, cull); break;
+      case 64: launch_bytes<64>(g, grid, b, cull); break;
+      case 96: launch_bytes<96>(g, grid, b, cull); break;
+      default: launch_bytes<160>(g, grid, b, cull); break;
+    }
+  } else if (b.counts) launch(std::true_type{});
+  else launch(std::false_type{});
 }
 
 }  // namespace
@@ -275,8 +312,7 @@ void launch_bits(dlg_gpu* g, dim3 grid, const float3* P, const float3* N, uint32
 // The checks and buffers of a trace of sensors [i0, i0 + nc): 0, or 1 with err. stack: the LDS stack entries the
 // scene's tree needs (the smallest instantiation that covers it, with the two entries above the top that the
 // unconditional pushes write).
-int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen) {
-  if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_trace: sensor range exceeds the scene"); return 1; }
+int32_t dlg_trace_check(const dlg_gpu* g, uint32_t rayFlags, int& stack, char* err, int32_t errLen) {
   const bool bytes = !(rayFlags & 1u);   // any-hit on: a slot transmits
   const dlg_scene& s = g->scene;
   if (bytes && s.nTris > 0 && (g->nSlots == 0 || int64_t(s.slotMax) >= g->nSlots)) {
@@ -284,6 +320,16 @@ int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags,
                               " slots are set (dlg_set_slots)");
     return 1;
   }
+  const int need = dlw::wide_stack(s.height) + 2;
+  if (need > 160) { dlg_fail(err, errLen, "dlg_trace: the scene's tree is too deep for the traversal stack"); return 1; }
+  stack = need <= 24 ? 24 : need <= 32 ? 32 : need <= 48 ? 48 : need <= 64 ? 64 : need <= 96 ? 96 : 160;
+  return 0;
+}
+
+int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen) {
+  if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_trace: sensor range exceeds the scene"); return 1; }
+  if (dlg_trace_check(g, rayFlags, stack, err, errLen)) return 1;
+  const bool bytes = !(rayFlags & 1u);
   DLG_CHECK(g->bits.grow(size_t(g->nWords) * size_t(nc) * (bytes ? 32 : 4)));
   if (!g->traceCount.p) {   // the running count of traced rays, read after each launch
     DLG_CHECK(g->traceCount.grow(sizeof(unsigned long long)));
@@ -292,43 +338,21 @@ int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags,
   g->i0 = i0;
   g->nc = nc;
   g->bytes = bytes ? 1 : 0;
-  const int need = dlw::wide_stack(s.height) + 2;
-  if (need > 160) { dlg_fail(err, errLen, "dlg_trace: the scene's tree is too deep for the traversal stack"); return 1; }
-  stack = need <= 24 ? 24 : need <= 32 ? 32 : need <= 48 ? 48 : need <= 64 ? 64 : need <= 96 ? 96 : 160;
   return 0;
 }
 
 // The trace's kernel enqueued on the library's stream (no wait: a graph may capture it); counts non-null runs the
 // counting variant.
 void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts) {
-  const int64_t nc = g->nc;
-  if (nc == 0 || g->nWords == 0) return;
-  const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
-  const dim3 grid(unsigned((nc + kWave - 1) / kWave), unsigned(g->nWords));
-  const float3* P = g->P.as<float3>() + g->i0;
-  const float3* N = g->N.as<float3>() + g->i0;
-  const auto launch = [&](auto count) {
-    constexpr bool C_ = decltype(count)::value;
-    switch (stack) {
-      case 24: launch_bits<24, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
-      case 32: launch_bits<32, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
-      case 48: launch_bits<48, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
-      case 64: launch_bits<64, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
-      case 96: launch_bits<96, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
-      default: launch_bits<160, C_>(g, grid, P, N, uint32_t(nc), cull, counts); break;
-    }
-  };
-  if (g->bytes) {
-    switch (stack) {
-      case 24: launch_bytes<24>(g, grid, P, N, uint32_t(nc), cull); break;
-      case 32: launch_bytes<32>(g, grid, P, N, uint32_t(nc), cull); break;
-      case 48: launch_bytes<48>(g, grid, P, N, uint32_t(nc), cull); break;
-      case 64: launch_bytes<64>(g, grid, P, N, uint32_t(nc), cull); break;
-      case 96: launch_bytes<96>(g, grid, P, N, uint32_t(nc), cull); break;
-      default: launch_bytes<160>(g, grid, P, N, uint32_t(nc), cull); break;
-    }
-  } else if (counts) launch(std::true_type{});
-  else launch(std::false_type{});
+  const TraceBufs b{g->P.as<float3>() + g->i0, g->N.as<float3>() + g->i0, uint32_t(g->nc), g->bits.as<uint32_t>(),
+                    g->traceCount.as<unsigned long long>(), counts};
+  launch_trace(g, b, rayFlags, stack, g->bytes != 0);
+}
+
+void dlg_trace_enqueue_into(dlg_gpu* g, const float* P, const float* N, uint32_t nc, uint32_t rayFlags, int stack, uint32_t* vis,
+                            unsigned long long* traced) {
+  const TraceBufs b{reinterpret_cast<const float3*>(P), reinterpret_cast<const float3*>(N), nc, vis, traced, nullptr};
+  launch_trace(g, b, rayFlags, stack, !(rayFlags & 1u));
 }
 
 // the statistics of a finished trace launch of ms milliseconds

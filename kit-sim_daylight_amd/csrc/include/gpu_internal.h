@@ -114,6 +114,7 @@ struct dlg_gpu {
   double meshC[3] = {0, 0, 0};
 };
 
+
 // chunk.hip: frees the chunk's graph
 void dlg_graph_free(dlg_gpu* g);
 
@@ -131,10 +132,19 @@ int64_t dlg_sah_bytes(int64_t nt);
 // what a capture can hold (kernel launches on g->stream, no waits). trace.hip:
 int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen);
 void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts);
+// the checks of a trace with these flags against the attached scene and the slots, and its stack size (no buffers)
+int32_t dlg_trace_check(const dlg_gpu* g, uint32_t rayFlags, int& stack, char* err, int32_t errLen);
+// the trace of nc sensors P, N ([nc] float3 on the device) into caller buffers: vis (bits [nWords x nc] or byte quads
+// [8 nWords x nc]) and the traced-ray count (added to)
+void dlg_trace_enqueue_into(dlg_gpu* g, const float* P, const float* N, uint32_t nc, uint32_t rayFlags, int stack, uint32_t* vis,
+                            unsigned long long* traced);
 int32_t dlg_trace_account(dlg_gpu* g, float ms, char* err, int32_t errLen);
 // reduce.hip: W [nQ x nDir] uploaded, the output grown; the quantities' cosine mask
 int32_t dlg_reduce_prepare(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, uint32_t& mask, char* err, int32_t errLen);
 void dlg_reduce_enqueue(dlg_gpu* g, uint32_t nc, int32_t nQ, uint32_t mask);
+// the reduce of caller buffers: vis as dlg_trace_enqueue_into wrote it, N [nc] float3, W [nQ x nDir], out [nQ x nc]
+void dlg_reduce_enqueue_from(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float* N, const float* W, int32_t nQ, uint32_t mask,
+                             double* out);
 // series.hip: the series inputs uploaded (groundRow >= 0: G taken on the device from row groundRow of the reduce's
 // output instead of in->G), the output and operands grown
 int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int32_t method, int32_t groundRow, char* err, int32_t errLen);

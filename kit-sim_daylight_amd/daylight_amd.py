@@ -45,6 +45,11 @@ class SeriesIn(C.Structure):
                 ("sunW", C.c_void_p), ("reflect", C.c_int32)]
 
 
+class ChunkIn(C.Structure):
+    _fields_ = [("i0", C.c_int64), ("nc", C.c_int64), ("rayFlags", C.c_uint32), ("nQ", C.c_int32), ("W", C.c_void_p),
+                ("useCos", C.c_void_p), ("series", C.c_void_p), ("method", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("buildMs", "traceMs", "reduceMs", "seriesMs", "skyMs", "viewMs")] + \
                [(n, C.c_int64) for n in ("rays", "traced", "launches", "sceneBytes", "nodes", "height")] + \
@@ -139,7 +144,7 @@ SEAM = [
     ("dlg_set_sky_series", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
     ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
-    ("dlg_chunk", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("dlg_chunk", C.c_int32, [C.c_void_p, C.POINTER(ChunkIn), C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),
     ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
     ("dlg_count_fetches", None, [C.c_void_p, C.c_int32]),
@@ -293,6 +298,15 @@ class Gpu:
         W = np.ascontiguousarray(W, np.float32).reshape(len(useCos), -1)
         out = np.zeros(len(useCos) * nc, np.float64)
         _check(self.L.dlg_reduce(self.h, nc, len(useCos), W.reshape(-1), np.array(useCos, np.int32), out, self.err, 1024), self.err)
+        return out.reshape(len(useCos), nc)
+
+    def chunk(self, i0, nc, W, useCos, cull=False, translucent=False):
+        """dlg_chunk without a series: the sums [nQ x nc] of sensors [i0, i0 + nc) of set_scene"""
+        W = np.ascontiguousarray(W, np.float32).reshape(len(useCos), -1)
+        cos = np.array(useCos, np.int32)
+        out = np.zeros(len(useCos) * nc, np.float64)
+        ci = ChunkIn(i0, nc, self.L.dlg_ray_flags(int(cull), int(translucent)), len(useCos), W.ctypes.data, cos.ctypes.data, None, 0)
+        _check(self.L.dlg_chunk(self.h, C.byref(ci), out.ctypes.data, None, self.err, 1024), self.err)
         return out.reshape(len(useCos), nc)
 
     def sky(self, steps, lat, lon, tz, leap, north, grefl, mf, tcos, wantRad=True, wantVis=False, series=False, seriesVis=False, reflect=False):
