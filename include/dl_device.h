@@ -227,6 +227,17 @@ typedef struct {
   int32_t method;
 } dlg_chunk_in;
 int32_t dlg_chunk(dlg_gpu* g, const dlg_chunk_in* in, double* sums, float* E, char* err, int32_t errLen);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under stream capture, serving
+ * loops that forbid allocation): dlg_chunk without a series, allocating nothing. Sensors P, N ([nc] x 3 floats), weights
+ * W [nQ x nDir] and the output sums [nQ x nc] (as dlg_reduce's out) are the caller's device memory; directions, scene
+ * and slots are those already set on g (dlg_set_scene, dlg_build / dlg_attach, dlg_set_slots); useCos [nQ] is host
+ * memory. The visibility goes to a fixed device array of dlg_chunk_noalloc_limit() bytes per device, held until the
+ * process ends: 4 nWords bytes per sensor (byte mode, rayFlags without bit 0: 32 nWords), and a call over more
+ * sensors than fit fails with a message naming the limit. Kernels are enqueued on the library's stream; wait != 0
+ * waits for them. Every call on a device shares that device's array: calls that overlap on one device race. */
+int32_t dlg_chunk_noalloc(dlg_gpu* g, const float* P, const float* N, int64_t nc, uint32_t rayFlags, int32_t nQ, const float* W,
+                          const int32_t* useCos, double* sums, int32_t wait, char* err, int32_t errLen);
+int64_t dlg_chunk_noalloc_limit(void);
 
 /* gpu_stats (internal.h:211-216), plus per-kernel device times of the last calls (HIP events on the library's stream). */
 typedef struct {

@@ -58,6 +58,10 @@ int32_t dlg_init(dlg_gpu** out, char* err, int32_t errLen) {
     dlg_destroy(g);
     return 1;
   }
+  if (dlg_noalloc_setup(g, err, errLen)) {
+    dlg_destroy(g);
+    return 1;
+  }
   *out = g;
   return 0;
 }

@@ -112,8 +112,12 @@ struct dlg_gpu {
   int64_t meshNv = 0, meshNt = 0, meshCapV = 0, meshCapT = 0;
   int32_t meshSlotted = 0, meshOpen = 0;
   double meshC[3] = {0, 0, 0};
+  // the device address of noalloc.hip's fixed array on this context's device (dlg_noalloc_setup)
+  void* noalloc = nullptr;
 };
 
+// noalloc.hip: at context creation, with its device current: the address of the fixed array of dlg_chunk_noalloc
+int32_t dlg_noalloc_setup(dlg_gpu* g, char* err, int32_t errLen);
 
 // chunk.hip: frees the chunk's graph
 void dlg_graph_free(dlg_gpu* g);

@@ -145,6 +145,9 @@ SEAM = [
     ("dlg_sky_series", C.c_int32, [C.c_void_p, F32P, F32P, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_series", C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(SeriesIn), C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
     ("dlg_chunk", C.c_int32, [C.c_void_p, C.POINTER(ChunkIn), C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("dlg_chunk_noalloc", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
+    ("dlg_chunk_noalloc_limit", C.c_int64, []),
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),
     ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
     ("dlg_count_fetches", None, [C.c_void_p, C.c_int32]),
@@ -308,6 +311,15 @@ class Gpu:
         ci = ChunkIn(i0, nc, self.L.dlg_ray_flags(int(cull), int(translucent)), len(useCos), W.ctypes.data, cos.ctypes.data, None, 0)
         _check(self.L.dlg_chunk(self.h, C.byref(ci), out.ctypes.data, None, self.err, 1024), self.err)
         return out.reshape(len(useCos), nc)
+
+    def chunk_noalloc(self, P, N, nc, W, useCos, sums, cull=False, translucent=False, wait=True):
+        """dlg_chunk_noalloc: P, N, W, sums are device addresses (ints); the directions are set_scene's"""
+        cos = np.array(useCos, np.int32)
+        _check(self.L.dlg_chunk_noalloc(self.h, P, N, nc, self.L.dlg_ray_flags(int(cull), int(translucent)), len(useCos), W, cos.ctypes.data,
+                                        sums, int(wait), self.err, 1024), self.err)
+
+    def noalloc_limit(self):
+        return self.L.dlg_chunk_noalloc_limit()
 
     def sky(self, steps, lat, lon, tz, leap, north, grefl, mf, tcos, wantRad=True, wantVis=False, series=False, seriesVis=False, reflect=False):
         T = len(steps["month"])
