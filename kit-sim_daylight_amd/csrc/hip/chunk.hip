@@ -19,7 +19,7 @@ namespace {
 // every value a kernel of the chunk reads as an argument
 struct GraphKey {
   const void* p[24];
-  int64_t n[24];
+  int64_t n[32];
 };
 
 int64_t pad_bits(float f) {
@@ -33,13 +33,13 @@ GraphKey key_of(const dlg_gpu* g, uint32_t rayFlags, int stack, int32_t nQ, uint
   std::memset(&k, 0, sizeof k);
   const dlg_scene& s = g->scene;
   const void* ptrs[] = {s.wide.p, s.leaves.p, g->P.p, g->N.p, g->D.p, g->dirTab.p, g->bits.p, g->slots.p, g->traceCount.p, g->scratch.p,
-                        g->out.p, g->seriesIn.p, g->seriesE.p, g->seriesWork.p, g->skyS.p, g->skyG.p, g->skySdc.p, g->dc.p};
+                        g->out.p, g->seriesIn.p, g->seriesE.p, g->seriesWork.p, g->skyS.p, g->skyG.p, g->skySdc.p, g->dc.p, g->stackOvf.p, g->tileNext.p};
   std::memcpy(k.p, ptrs, sizeof ptrs);
   const dlg_series_plan& q = g->seriesPlan;
   const int64_t vals[] = {s.nTris, pad_bits(s.pad), s.nNodes, g->nDir, g->nWords, g->nc, g->i0, int64_t(rayFlags), stack, g->bytes,
                           g->nSlots, nQ, int64_t(mask), series ? 1 : 0, q.method, q.groundRow, q.reflect, q.T, int64_t(q.nc) << 32 | q.nSky,
                           int64_t(q.nBins) << 32 | q.sun0, int64_t(q.oRow ^ (q.oW << 16) ^ (q.oG << 32)), int64_t(q.oCol ^ (q.oSw << 32)), g->dcBins,
-                          q.shape};
+                          q.shape, g->traceGrid};
   std::memcpy(k.n, vals, sizeof vals);
   return k;
 }

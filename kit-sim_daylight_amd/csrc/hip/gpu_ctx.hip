@@ -71,7 +71,7 @@ void dlg_destroy(dlg_gpu* g) {
   dlg_graph_free(g);
   g->scene.release();
   for (DevBuf* b : {&g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->slots, &g->dc, &g->skySdc,
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->slots, &g->dc, &g->skySdc,
                     &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
@@ -91,7 +91,7 @@ int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmB
 }
 
 void dlg_trim(dlg_gpu* g, int32_t sensors) {
-  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
   g->meshOpen = 0;
   g->meshNv = g->meshNt = 0;
   if (sensors) {

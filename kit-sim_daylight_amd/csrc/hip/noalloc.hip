@@ -4,11 +4,13 @@
 // of this file: the call allocates nothing. It has no counterpart in the reference; it is for callers that cannot
 // allocate device memory (code under stream capture, serving loops that forbid allocation).
 //
-// The array holds the traced-ray count and the visibility (bits [nWords x nc], or byte mode's quads [8 nWords x nc]).
-// There is one per device, held until the process ends, shared by every call on that device and keeping the last call's
-// contents: the count is cleared by a kernel at each call and the visibility is written whole before it is read, so a
-// call depends on nothing an earlier one left. Calls that overlap on one device race on the array: they must be
-// ordered (one stream, or the caller's waits).
+// The array holds the traced-ray count and the trace's tile counters, the visibility (bits [nWords x nc], or byte mode's quads [8 nWords x nc]) and
+// the traversal stacks' entries beyond their LDS part (one area per block of the persistent grid; the grid is cut to the
+// areas that fit, which deep trees - many entries per area - pay for in waves, not in results). There is one array per
+// device, held until the process ends, shared by every call on that device and keeping the last call's contents: the
+// counts are cleared by kernels at each call, the visibility is written whole before it is read and a stack entry is read
+// only after its lane wrote it, so a call depends on nothing an earlier one left. Calls that overlap on one device race
+// on the array: they must be ordered (one stream, or the caller's waits).
 #include <mutex>
 #include <string>
 
@@ -16,14 +18,15 @@
 
 namespace {
 
-constexpr size_t kCountBytes = 256;              // the traced-ray count (8 bytes), padded to the visibility's alignment
+constexpr size_t kCountBytes = 1024;             // the traced-ray count (8 bytes) at 0, the tile counters at 512
 constexpr size_t kVisBytes = size_t(64) << 20;   // the largest visibility of a call
+constexpr size_t kOvfBytes = size_t(32) << 20;   // the stacks' global areas
 constexpr int kMaxDevices = 64;
 
 }  // namespace
 
 // the fixed device array; host code uses the address that hipGetSymbolAddress gives for it on each device, never this name
-static __device__ __align__(256) unsigned char scratch[kCountBytes + kVisBytes];
+static __device__ __align__(256) unsigned char scratch[kCountBytes + kVisBytes + kOvfBytes];
 
 namespace {
 
@@ -80,8 +83,13 @@ int32_t dlg_chunk_noalloc(dlg_gpu* g, const float* P, const float* N, int64_t nc
   for (int q = 0; q < nQ; ++q) mask |= useCos[q] ? 1u << q : 0u;
   unsigned long long* count = static_cast<unsigned long long*>(g->noalloc);
   uint32_t* vis = reinterpret_cast<uint32_t*>(static_cast<char*>(g->noalloc) + kCountBytes);
+  uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(g->noalloc) + kCountBytes + kVisBytes);
+  dlg_trace_shape sh = dlg_trace_grid(g, nc, stack, bytes, false);
+  if (sh.ovfPerBlock && sh.grid * sh.ovfPerBlock > kOvfBytes) sh.grid = uint32_t(kOvfBytes / sh.ovfPerBlock / 8 * 8);
   k_clear_count<<<1, 64, 0, g->stream>>>(count);
-  dlg_trace_enqueue_into(g, P, N, uint32_t(nc), rayFlags, stack, vis, count);
+  uint32_t* next = reinterpret_cast<uint32_t*>(static_cast<char*>(g->noalloc) + 512);
+  static_assert(512 + dlg_tile_counter_bytes <= kCountBytes, "the counters fit their region");
+  dlg_trace_enqueue_into(g, P, N, uint32_t(nc), rayFlags, stack, vis, count, ovf, sh.grid, next);
   dlg_reduce_enqueue_from(g, vis, bytes, uint32_t(nc), N, W, nQ, mask, sums);
   DLG_CHECK(hipGetLastError());
   if (wait) DLG_CHECK(hipStreamSynchronize(g->stream));

@@ -3,15 +3,20 @@
 // __raygen__bytes + __anyhit__tr (:115-165) and gpu_trace's launch loop (src/gpu.cu:499-531). No RT cores: a software traversal per ray of the 4-wide records built from the LBVH (lbvh.hip,
 // wide.h), each visit two levels of the binary tree.
 //
-// Launch: a wave64 per 64 consecutive sensors and one 32-direction word (grid y). All lanes of a wave trace the same
-// direction from neighbouring origins, so their record and triangle fetches mostly coincide (one 128-byte record per
-// visit, served by L1/L2 for the whole wave). Each lane keeps its traversal stack in LDS (stack[depth][lane], no bank
-// conflicts), sized by the tree's height.
+// Launch: persistent waves, as many as the CUs hold at once; each takes tiles of 64 consecutive sensors x one 32-direction
+// word in turn. All lanes of a wave trace the same direction from neighbouring origins, so their record and triangle
+// fetches mostly coincide (one 128-byte record per visit, served by L1/L2 for the whole wave). Each lane keeps the top
+// of its traversal stack in LDS (stack[depth][lane], no bank conflicts) and the deeper entries, which only deep trees
+// reach, in its wave's area of a global buffer: the LDS a wave needs stays small, so the waves per SIMD are what the
+// registers allow (a 3M-triangle city: 6 instead of 3; the traversal waits on dependent fetches, and more waves hide
+// them).
 // A ray is open when every hit on [1e-5, inf) is an own-plane hit (dl_math.h own_hit) and they lie at 3 distinct
 // distances at most (the reference's closest-hit walk past the sensor's own planes, trace_dev.cu:76-88); the first other
 // hit ends the traversal (any-hit, as TERMINATE_ON_FIRST_HIT).
+#include <mutex>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "dl_math.h"
 #include "gpu_internal.h"
@@ -47,11 +52,46 @@ struct OwnSet {
 // iterations its wave executed
 struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
 
+// The traversal stack of a lane, STACK entries: [0, kL) in LDS (lds[depth][lane]), [kL, STACK) in the wave's area of a
+// global buffer (ovf[depth - kL][lane]). A lane reads back only what it wrote itself, so no wave waits on another.
+constexpr int kLdsStack = 24;
+template <int STACK>
+struct LaneStack {
+  static constexpr int kL = STACK < kLdsStack ? STACK : kLdsStack;
+  static constexpr bool kDeep = STACK > kL;
+  uint32_t* lds;   // [kL][64]
+  uint32_t* ovf;   // [STACK - kL][64] of this wave (unused unless kDeep)
+  __device__ __forceinline__ void put(int k, int lane, uint32_t v) const {
+    if (!kDeep || k < kL) lds[k * kWave + lane] = v;
+    else ovf[(k - kL) * kWave + lane] = v;
+  }
+  // the three pushes of a visit, at sp, sp + h1, sp + h1 + h2 (all below sp + 3); LDS alone while every lane is low
+  __device__ __forceinline__ void push3(int sp, int h1, int h2, int lane, uint32_t a, uint32_t b, uint32_t c) const {
+    if (!kDeep || __ballot(sp + 2 >= kL) == 0) {
+      lds[sp * kWave + lane] = a;
+      lds[(sp + h1) * kWave + lane] = b;
+      lds[(sp + h1 + h2) * kWave + lane] = c;
+    } else {
+      put(sp, lane, a);
+      put(sp + h1, lane, b);
+      put(sp + h1 + h2, lane, c);
+    }
+  }
+  __device__ __forceinline__ uint32_t at(int k, int lane) const {
+    if (!kDeep || __ballot(k >= kL) == 0) return lds[k * kWave + lane];
+    return k < kL ? lds[k * kWave + lane] : ovf[(k - kL) * kWave + lane];
+  }
+};
+template <int STACK>
+constexpr int lds_stack() { return LaneStack<STACK>::kL; }
+template <int STACK>
+constexpr int ovf_stack() { return STACK - LaneStack<STACK>::kL; }
+
 // every hit through the 4-wide records (wide.h) until hit(leaf) returns false (then false: the ray is blocked); true when
 // the walk ends. The stack holds record indices, nearest child taken first.
 // OFF32: fewer than 2^25 records, walked through uniform bases and 32-bit offsets (wide.h visit32)
 template <int STACK, bool COUNT, bool OFF32, class Hit>
-__device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw::Slab& sl, const dlw::Bases& bs, uint32_t* __restrict__ stk,
+__device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw::Slab& sl, const dlw::Bases& bs, const LaneStack<STACK>& stk,
                                      int lane, Fetches& f, Hit&& hit) {
   uint32_t cur = 0;
   int sp = 0;
@@ -73,7 +113,7 @@ __device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw:
     if (v.nInner == 0) {
       if (sp == 0) return true;
       --sp;
-      cur = stk[sp * kWave + lane];
+      cur = stk.at(sp, lane);
       continue;
     }
     // slot 0 holds the nearest hit inner child; the other hit ones (finite entry distance) are pushed by unconditional
@@ -81,9 +121,7 @@ __device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw:
     // the stack holds two entries beyond the deepest top for them)
     const float inf = __int_as_float(0x7f800000);
     const int h1 = v.tInner[1] < inf, h2 = v.tInner[2] < inf, h3 = v.tInner[3] < inf;
-    stk[sp * kWave + lane] = v.inner[1];
-    stk[(sp + h1) * kWave + lane] = v.inner[2];
-    stk[(sp + h1 + h2) * kWave + lane] = v.inner[3];
+    stk.push3(sp, h1, h2, lane, v.inner[1], v.inner[2], v.inner[3]);
     sp += h1 + h2 + h3;
     cur = v.inner[0];
   }
@@ -92,7 +130,7 @@ __device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw:
 // any hit of a visibility ray: open when every hit is an own-plane hit at 3 distinct distances at most
 template <int STACK, bool COUNT, bool OFF32>
 __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restrict__ leaves, float pad, float3 o, float3 nrm, float3 d,
-                         float4 dirA, float4 dirB, bool cull, uint32_t* __restrict__ stk, int lane, Fetches& f) {
+                         float4 dirA, float4 dirB, bool cull, const LaneStack<STACK>& stk, int lane, Fetches& f) {
   Ray r;
   dir_unpack(r, d, dirA, dirB);   // the direction's constants, made once per direction (k_dir_table)
   ray_origin(r, o, pad);
@@ -100,6 +138,41 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
   const dlw::Bases bs = dlw::bases(wide, sl);
   OwnSet own;
   return walk<STACK, COUNT, OFF32>(wide, sl, bs, stk, lane, f, [&](uint32_t leaf) { return own.add(r, nrm, leaves, leaf, cull); });
+}
+
+// The tiles of a persistent wave: tile t is sensor block t % nSB of word t / nSB. Tiles are dealt to the 8 XCDs in runs
+// of kRun (XCD x owns the runs r with r % 8 == x), so the waves of an XCD take neighbouring sensors of one word, which
+// share records in that XCD's L2, while every XCD draws from the whole chunk. A wave takes its next tile from its XCD's
+// counter (workgroup b runs on XCD b % 8) and, once that XCD's tiles are gone, from the other XCDs' counters in turn:
+// tiles vary in cost, and taking them as waves come free keeps every wave busy to the end. Each counter only grows, so
+// every wave leaves once all 8 hand out tiles past the last. The counters (kTileCounters uint32, one per 64-byte line)
+// start at zero each launch.
+constexpr uint32_t kRun = 16;
+constexpr uint32_t kTileCounters = 8 * 16;
+static_assert(kTileCounters * 4 == dlg_tile_counter_bytes, "the counters' bytes");
+struct Tiles {
+  uint32_t nSB, nTiles;
+  uint32_t* next;   // [8 x 16]: XCD x's count at 16 x
+  __device__ __forceinline__ uint64_t tile(uint32_t j, uint32_t xcd) const { return (uint64_t(j / kRun) * 8 + xcd) * kRun + j % kRun; }
+};
+template <class F>
+__device__ __forceinline__ void for_tiles(const Tiles& tl, F&& body) {
+  const uint32_t home = blockIdx.x % 8;
+  for (uint32_t v = 0; v < 8; ++v) {
+    const uint32_t x = (home + v) % 8;
+    for (;;) {
+      uint32_t j = 0;
+      if (threadIdx.x == 0) j = atomicAdd(&tl.next[16 * x], 1u);
+      j = __builtin_amdgcn_readfirstlane(j);
+      const uint64_t t = tl.tile(j, x);
+      if (t >= tl.nTiles) break;
+      body(uint32_t(t) % tl.nSB, uint32_t(t) / tl.nSB);
+    }
+  }
+}
+
+__global__ void k_zero_counters(uint32_t* c) {
+  if (threadIdx.x < kTileCounters) c[threadIdx.x] = 0u;
 }
 
 // the rays a wave traced (past the hemisphere cull) added to *traced: a full-wave sum, one atomic per wave
@@ -115,24 +188,30 @@ template <int STACK, bool COUNT, bool OFF32>
 __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull, uint32_t* __restrict__ bits,
-                                               unsigned long long* __restrict__ counts, unsigned long long* __restrict__ traced) {
-  __shared__ uint32_t stk[STACK * kWave];
-  const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
-  const bool live = i < nc;   // a lane past the chunk keeps a zero normal: every direction culled, nothing stored
-  const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
-  const float3 o = live ? offset_ray(P[i], n) : n;
-  uint32_t word = 0, nTraced = 0;
+                                               unsigned long long* __restrict__ counts, unsigned long long* __restrict__ traced,
+                                               uint32_t* __restrict__ ovf, Tiles tl) {
+  __shared__ uint32_t lds[lds_stack<STACK>() * kWave];
+  const LaneStack<STACK> stk{lds, ovf + size_t(blockIdx.x) * ovf_stack<STACK>() * kWave};
+  const uint32_t lane = threadIdx.x;
+  uint32_t nTraced = 0;
   Fetches f;
-  for (uint32_t k = 0; k < 32; ++k) {
-    const uint32_t d = w * 32 + k;
-    if (d >= nDir) break;
-    const float3 dir = D[d];
-    if (n.x * dir.x + n.y * dir.y + n.z * dir.z <= 0.f) continue;   // hemisphere cull (trace_dev.cu:101)
-    ++nTraced;
-    if (nTris == 0 || ray_open<STACK, COUNT, OFF32>(wide, leaves, pad, o, n, dir, dirTab[2 * d], dirTab[2 * d + 1], cull != 0, stk, int(lane), f))
-      word |= 1u << k;
-  }
-  if (live) bits[size_t(w) * nc + i] = word;
+  for_tiles(tl, [&](uint32_t sb, uint32_t w) {
+    const uint32_t i = sb * kWave + lane;
+    const bool live = i < nc;   // a lane past the chunk keeps a zero normal: every direction culled, nothing stored
+    const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+    const float3 o = live ? offset_ray(P[i], n) : n;
+    uint32_t word = 0;
+    for (uint32_t k = 0; k < 32; ++k) {
+      const uint32_t d = w * 32 + k;
+      if (d >= nDir) break;
+      const float3 dir = D[d];
+      if (n.x * dir.x + n.y * dir.y + n.z * dir.z <= 0.f) continue;   // hemisphere cull (trace_dev.cu:101)
+      ++nTraced;
+      if (nTris == 0 || ray_open<STACK, COUNT, OFF32>(wide, leaves, pad, o, n, dir, dirTab[2 * d], dirTab[2 * d + 1], cull != 0, stk, int(lane), f))
+        word |= 1u << k;
+    }
+    if (live) bits[size_t(w) * nc + i] = word;
+  });
   add_traced(nTraced, lane, traced);
   if (COUNT) {
     atomicAdd(&counts[0], (unsigned long long)f.nodes);
@@ -214,45 +293,51 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
                                                 const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                 const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull,
                                                 const float4* __restrict__ slots, uint32_t slotLast, uint32_t* __restrict__ vis,
-                                                unsigned long long* __restrict__ traced) {
-  __shared__ uint32_t stk[STACK * kWave];
-  const uint32_t lane = threadIdx.x, i = blockIdx.x * kWave + lane, w = blockIdx.y;
-  const bool live = i < nc;   // as k_bits: a zero normal past the chunk, nothing stored
-  const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
-  const float3 o = live ? offset_ray(P[i], n) : n;
+                                                unsigned long long* __restrict__ traced, uint32_t* __restrict__ ovf, Tiles tl) {
+  __shared__ uint32_t lds[lds_stack<STACK>() * kWave];
+  const LaneStack<STACK> stk{lds, ovf + size_t(blockIdx.x) * ovf_stack<STACK>() * kWave};
+  const uint32_t lane = threadIdx.x;
   Fetches f;
-  uint32_t quad = 0, nTraced = 0;
-  for (uint32_t k = 0; k < 32; ++k) {
-    const uint32_t d = w * 32 + k;
-    uint32_t b = 0;
-    if (d < nDir) {
-      const float3 dir = D[d];
-      if (n.x * dir.x + n.y * dir.y + n.z * dir.z > 0.f) {   // hemisphere cull (trace_dev.cu:131)
-        ++nTraced;
-        if (nTris == 0) b = 255;
-        else {
-          Ray r;
-          dir_unpack(r, dir, dirTab[2 * d], dirTab[2 * d + 1]);
-          ray_origin(r, o, pad);
-          const dlw::Slab sl = dlw::slab(r);
-          const dlw::Bases bs = dlw::bases(wide, sl);
-          ByteRay br;
-          if (walk<STACK, false, OFF32>(wide, sl, bs, stk, int(lane), f,
-                                        [&](uint32_t leaf) { return br.add(r, n, leaves, slots, slotLast, leaf, cull != 0); }))
-            b = T_byte(br.value());
+  uint32_t nTraced = 0;
+  for_tiles(tl, [&](uint32_t sb, uint32_t w) {
+    const uint32_t i = sb * kWave + lane;
+    const bool live = i < nc;   // as k_bits: a zero normal past the chunk, nothing stored
+    const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+    const float3 o = live ? offset_ray(P[i], n) : n;
+    uint32_t quad = 0;
+    for (uint32_t k = 0; k < 32; ++k) {
+      const uint32_t d = w * 32 + k;
+      uint32_t b = 0;
+      if (d < nDir) {
+        const float3 dir = D[d];
+        if (n.x * dir.x + n.y * dir.y + n.z * dir.z > 0.f) {   // hemisphere cull (trace_dev.cu:131)
+          ++nTraced;
+          if (nTris == 0) b = 255;
+          else {
+            Ray r;
+            dir_unpack(r, dir, dirTab[2 * d], dirTab[2 * d + 1]);
+            ray_origin(r, o, pad);
+            const dlw::Slab sl = dlw::slab(r);
+            const dlw::Bases bs = dlw::bases(wide, sl);
+            ByteRay br;
+            if (walk<STACK, false, OFF32>(wide, sl, bs, stk, int(lane), f,
+                                          [&](uint32_t leaf) { return br.add(r, n, leaves, slots, slotLast, leaf, cull != 0); }))
+              b = T_byte(br.value());
+          }
         }
       }
+      quad |= b << (8 * (k & 3));
+      if ((k & 3) == 3) {
+        if (live) vis[size_t(w * 8 + k / 4) * nc + i] = quad;
+        quad = 0;
+      }
     }
-    quad |= b << (8 * (k & 3));
-    if ((k & 3) == 3) {
-      if (live) vis[size_t(w * 8 + k / 4) * nc + i] = quad;
-      quad = 0;
-    }
-  }
+  });
   add_traced(nTraced, lane, traced);
 }
 
-// the launch's buffers: sensors, the visibility out, the traced-ray count (added to) and the counting variant's counters
+// the launch's buffers: sensors, the visibility out, the traced-ray count (added to), the counting variant's counters,
+// the stacks' global part and the persistent grid
 struct TraceBufs {
   const float3* P;
   const float3* N;
@@ -260,51 +345,76 @@ struct TraceBufs {
   uint32_t* vis;
   unsigned long long* traced;
   unsigned long long* counts;
+  uint32_t* ovf;
+  uint32_t grid;
+  uint32_t* next;   // the tile counters
 };
 
+// f(std::integral_constant<int, STACK>) for the instantiated stack sizes
+template <class F>
+void with_stack(int stack, F&& f) {
+  switch (stack) {
+    case 24: f(std::integral_constant<int, 24>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 48: f(std::integral_constant<int, 48>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 96: f(std::integral_constant<int, 96>{}); break;
+    default: f(std::integral_constant<int, 160>{}); break;
+  }
+}
+
 template <int STACK>
-void launch_bytes(dlg_gpu* g, dim3 grid, const TraceBufs& b, uint32_t cull) {
+const void* trace_kernel(bool bytes, bool count, bool off32) {
+  if (bytes) return off32 ? reinterpret_cast<const void*>(&k_bytes<STACK, true>) : reinterpret_cast<const void*>(&k_bytes<STACK, false>);
+  if (count) return off32 ? reinterpret_cast<const void*>(&k_bits<STACK, true, true>) : reinterpret_cast<const void*>(&k_bits<STACK, true, false>);
+  return off32 ? reinterpret_cast<const void*>(&k_bits<STACK, false, true>) : reinterpret_cast<const void*>(&k_bits<STACK, false, false>);
+}
+
+// the one-wave blocks of kernel fn a CU holds at once (asked once per kernel)
+uint32_t blocks_per_cu(const void* fn) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> known;
+  std::lock_guard<std::mutex> lk(mu);
+  for (const auto& [f, n] : known)
+    if (f == fn) return uint32_t(n);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWave, 0) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    n = 4;
+  }
+  known.push_back({fn, n});
+  return uint32_t(n);
+}
+
+template <int STACK>
+void launch_bytes(dlg_gpu* g, const Tiles& tl, const TraceBufs& b, uint32_t cull) {
   const dlg_scene& s = g->scene;
   const auto k = s.nNodes < (int64_t(1) << 25) ? k_bytes<STACK, true> : k_bytes<STACK, false>;
-  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, b.P, b.N, g->D.as<float3>(), g->dirTab.as<float4>(),
-                                   uint32_t(g->nDir), b.nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), b.vis, b.traced);
+  k<<<b.grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, b.P, b.N, g->D.as<float3>(), g->dirTab.as<float4>(),
+                                     uint32_t(g->nDir), b.nc, cull, g->slots.as<float4>(), uint32_t(g->nSlots - 1), b.vis, b.traced, b.ovf, tl);
 }
 
 template <int STACK, bool COUNT>
-void launch_bits(dlg_gpu* g, dim3 grid, const TraceBufs& b, uint32_t cull) {
+void launch_bits(dlg_gpu* g, const Tiles& tl, const TraceBufs& b, uint32_t cull) {
   const dlg_scene& s = g->scene;
   const auto k = s.nNodes < (int64_t(1) << 25) ? k_bits<STACK, COUNT, true> : k_bits<STACK, COUNT, false>;
-  k<<<grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, b.P, b.N, g->D.as<float3>(), g->dirTab.as<float4>(),
-                                   uint32_t(g->nDir), b.nc, cull, b.vis, b.counts, b.traced);
+  k<<<b.grid, kWave, 0, g->stream>>>(s.wide.as<float4>(), s.leaves.as<float4>(), s.nTris, s.pad, b.P, b.N, g->D.as<float3>(), g->dirTab.as<float4>(),
+                                     uint32_t(g->nDir), b.nc, cull, b.vis, b.counts, b.traced, b.ovf, tl);
 }
 
 // the trace kernel of the stack size and the mode, over the buffers b
 void launch_trace(dlg_gpu* g, const TraceBufs& b, uint32_t rayFlags, int stack, bool bytes) {
   if (b.nc == 0 || g->nWords == 0) return;
   const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
-  const dim3 grid(unsigned((b.nc + kWave - 1) / kWave), unsigned(g->nWords));
-  const auto launch = [&](auto count) {
-    constexpr bool C_ = decltype(count)::value;
-    switch (stack) {
-      case 24: launch_bits<24, C_>(g, grid, b, cull); break;
-      case 32: launch_bits<32, C_>(g, grid, b, cull); break;
-      case 48: launch_bits<48, C_>(g, grid, b, cull); break;
-      case 64: launch_bits<64, C_>(g, grid, b, cull); break;
-      case 96: launch_bits<96, C_>(g, grid, b, cull); break;
-      default: launch_bits<160, C_>(g, grid, b, cull); break;
-    }
-  };
-  if (bytes) {
-    switch (stack) {
-      case 24: launch_bytes<24>(g, grid, b, cull); break;
-      case 32: launch_bytes<32>(g, grid, b, cull); break;
-      case 48: launch_bytes<48>(g, grid, b, cull); break;
-      case 64: launch_bytes<64>(g, grid, b, cull); break;
-      case 96: launch_bytes<96>(g, grid, b, cull); break;
-      default: launch_bytes<160>(g, grid, b, cull); break;
-    }
-  } else if (b.counts) launch(std::true_type{});
-  else launch(std::false_type{});
+  const uint32_t nSB = (b.nc + kWave - 1) / kWave;
+  const Tiles tl{nSB, nSB * uint32_t(g->nWords), b.next};
+  k_zero_counters<<<1, kTileCounters, 0, g->stream>>>(b.next);
+  with_stack(stack, [&](auto S) {
+    constexpr int ST = decltype(S)::value;
+    if (bytes) launch_bytes<ST>(g, tl, b, cull);
+    else if (b.counts) launch_bits<ST, true>(g, tl, b, cull);
+    else launch_bits<ST, false>(g, tl, b, cull);
+  });
 }
 
 }  // namespace
@@ -326,11 +436,29 @@ int32_t dlg_trace_check(const dlg_gpu* g, uint32_t rayFlags, int& stack, char* e
   return 0;
 }
 
+dlg_trace_shape dlg_trace_grid(const dlg_gpu* g, int64_t nc, int stack, bool bytes, bool count) {
+  dlg_trace_shape t{};
+  const uint64_t nTiles = uint64_t((nc + kWave - 1) / kWave) * uint64_t(g->nWords);
+  with_stack(stack, [&](auto S) {
+    constexpr int ST = decltype(S)::value;
+    uint64_t grid = uint64_t(blocks_per_cu(trace_kernel<ST>(bytes, count, g->scene.nNodes < (int64_t(1) << 25)))) * uint64_t(g->cus > 0 ? g->cus : 1);
+    grid = grid < 8 ? 8 : grid / 8 * 8;   // whole rounds of the 8 XCDs
+    const uint64_t need = (nTiles + 7) / 8 * 8;
+    t.grid = uint32_t(need < grid ? (need < 8 ? 8 : need) : grid);
+    t.ovfPerBlock = size_t(ovf_stack<ST>()) * kWave * 4;
+  });
+  return t;
+}
+
 int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen) {
   if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_trace: sensor range exceeds the scene"); return 1; }
   if (dlg_trace_check(g, rayFlags, stack, err, errLen)) return 1;
   const bool bytes = !(rayFlags & 1u);
   DLG_CHECK(g->bits.grow(size_t(g->nWords) * size_t(nc) * (bytes ? 32 : 4)));
+  const dlg_trace_shape sh = dlg_trace_grid(g, nc, stack, bytes, g->countFetches != 0);
+  g->traceGrid = sh.grid;
+  if (sh.ovfPerBlock) DLG_CHECK(g->stackOvf.grow(sh.ovfPerBlock * sh.grid));
+  DLG_CHECK(g->tileNext.grow(kTileCounters * 4));
   if (!g->traceCount.p) {   // the running count of traced rays, read after each launch
     DLG_CHECK(g->traceCount.grow(sizeof(unsigned long long)));
     DLG_CHECK(hipMemsetAsync(g->traceCount.p, 0, sizeof(unsigned long long), g->stream));
@@ -345,13 +473,13 @@ int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags,
 // counting variant.
 void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts) {
   const TraceBufs b{g->P.as<float3>() + g->i0, g->N.as<float3>() + g->i0, uint32_t(g->nc), g->bits.as<uint32_t>(),
-                    g->traceCount.as<unsigned long long>(), counts};
+                    g->traceCount.as<unsigned long long>(), counts, g->stackOvf.as<uint32_t>(), g->traceGrid, g->tileNext.as<uint32_t>()};
   launch_trace(g, b, rayFlags, stack, g->bytes != 0);
 }
 
 void dlg_trace_enqueue_into(dlg_gpu* g, const float* P, const float* N, uint32_t nc, uint32_t rayFlags, int stack, uint32_t* vis,
-                            unsigned long long* traced) {
-  const TraceBufs b{reinterpret_cast<const float3*>(P), reinterpret_cast<const float3*>(N), nc, vis, traced, nullptr};
+                            unsigned long long* traced, uint32_t* ovf, uint32_t grid, uint32_t* next) {
+  const TraceBufs b{reinterpret_cast<const float3*>(P), reinterpret_cast<const float3*>(N), nc, vis, traced, nullptr, ovf, grid, next};
   launch_trace(g, b, rayFlags, stack, !(rayFlags & 1u));
 }
 

@@ -104,6 +104,10 @@ struct dlg_gpu {
   DevBuf counts;
   // the rays traced past the hemisphere cull since the context was made (k_bits / k_bytes add to it)
   DevBuf traceCount;
+  // the traversal stacks' entries past the LDS part, one area per block of the persistent grid, and that grid
+  DevBuf stackOvf;
+  uint32_t traceGrid = 0;
+  DevBuf tileNext;   // the persistent trace's tile counters
   // the tree dlg_build makes (dlg_set_tree): DLG_TREE_SAH or DLG_TREE_LBVH
   int32_t tree = DLG_TREE_SAH;
   // mesh ingest (dlg_mesh_*): staged vertices [meshNv] float3, triangles [meshNt] uint3, their global slots, the
@@ -138,10 +142,15 @@ int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags,
 void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts);
 // the checks of a trace with these flags against the attached scene and the slots, and its stack size (no buffers)
 int32_t dlg_trace_check(const dlg_gpu* g, uint32_t rayFlags, int& stack, char* err, int32_t errLen);
+// the persistent grid of a trace (one-wave blocks, a multiple of 8) and the bytes of each block's global stack area
+struct dlg_trace_shape { uint32_t grid; size_t ovfPerBlock; };
+dlg_trace_shape dlg_trace_grid(const dlg_gpu* g, int64_t nc, int stack, bool bytes, bool count);
 // the trace of nc sensors P, N ([nc] float3 on the device) into caller buffers: vis (bits [nWords x nc] or byte quads
-// [8 nWords x nc]) and the traced-ray count (added to)
+// [8 nWords x nc]), the traced-ray count (added to), the stacks' global area (grid x ovfPerBlock bytes) and the tile
+// counters (dlg_tile_counter_bytes; cleared by the launch)
 void dlg_trace_enqueue_into(dlg_gpu* g, const float* P, const float* N, uint32_t nc, uint32_t rayFlags, int stack, uint32_t* vis,
-                            unsigned long long* traced);
+                            unsigned long long* traced, uint32_t* ovf, uint32_t grid, uint32_t* next);
+constexpr size_t dlg_tile_counter_bytes = 512;
 int32_t dlg_trace_account(dlg_gpu* g, float ms, char* err, int32_t errLen);
 // reduce.hip: W [nQ x nDir] uploaded, the output grown; the quantities' cosine mask
 int32_t dlg_reduce_prepare(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, uint32_t& mask, char* err, int32_t errLen);
