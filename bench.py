@@ -146,7 +146,7 @@ def timed_runs(native, torch, fn, n):
 # ---------------------------------------------------------------- the CPU baseline (the oracle, on a bounded sample)
 
 def cpu_baseline(ctx, mesh, dni, dhi, threads, model, target_s):
-    """The oracle (host BVH, FP64 sums and series, the same ray rules) on the first faces of the 1,000-face mesh, the
+    """The oracle (host BVH, FP64 sums and series, the same ray rules) on the first faces of the 150,000-face mesh, the
     full context, 2305 patches and the full year; two probes give its fixed cost (the year's sky, the host BVH of the
     3M triangles) and its cost per sensor, and the sample fills target_s."""
     def sample(nf):
@@ -163,7 +163,7 @@ def cpu_baseline(ctx, mesh, dni, dhi, threads, model, target_s):
     t, hours = sample(nf)
     return {"value": round(nf * hours / t, 1), "unit": "sensor-hours/s", "cores": threads, "kind": "port",
             "cpu": model, "nproc": threads,
-            "sample": "%d sensor faces of the 1,000-face mesh, the full 3M-triangle context, 2305 patches, full year (%.0f h); "
+            "sample": "%d sensor faces of the 150,000-face mesh, the full 3M-triangle context, 2305 patches, full year (%.0f h); "
                       "the oracle: host BVH + trace + FP64 sums + FP64 series, %d threads (all the host's), %.1f s including "
                       "the host BVH build and the year's sky (%.1f s at %d sensors)" % (nf, hours, threads, t, t0, n0)}
 
@@ -187,10 +187,10 @@ def views_leg(native, torch, threads, steps=3, n_check=64):
     assert r.stats["scene_reused"]
     wall /= steps
     rays = n * 10800
-    # accuracy: the first n_check sensors as a run of their own (same element indices: the jitter seeds agree)
+    # accuracy: the first n_check sensors of the grid as a run of their own (same element indices: the jitter seeds agree)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    sub = W.floor_grid(n=n_check)
+    sub = W.submesh((gp, gc, gi), 0, n_check)
     rs = native.run("views", VIEW_ARGS, native.SimulationMesh("grid", *sub), context=room)
     o = orc.views([orc.MeshArrays("grid", *sub, sensor=True), orc.MeshArrays("interior", P, C, I, sensor=False)], fov=(360.0, -70.0, 50.0),
                   cell=2.0, jitter=True, far=1000.0, reach=(6.0, 20.0, 50.0), distance={"isovist", "mean", "min"}, offset=1.2,
@@ -492,7 +492,7 @@ def main():
         legs["transmission"] = transmission_leg(native, torch, epw, tmp)
         print("bench: transmission %s" % json.dumps(legs["transmission"]), file=sys.stderr, flush=True)
     refs = None if a.no_reference_benches else reference_benches()
-    cpu = None if a.no_cpu else cpu_baseline(ctx, meshes[0][1], dni, dhi, threads, model, a.cpu_seconds)
+    cpu = None if a.no_cpu else cpu_baseline(ctx, meshes[1][1], dni, dhi, threads, model, a.cpu_seconds)
     lat_ms = np.array(lat) * 1e3
     line = {
         "metric": "sensor-hours/s at <=2e-6 rel vs FP64 (annual radiation, direct, diffuse, illuminance + hourly series)",
