@@ -119,6 +119,35 @@ DL_API int32_t dlx_ladybug_sky(int32_t mf, int64_t T, const int32_t* jd, const d
                                double lon, double fileTz, double grefl, double dur, double* dir, double* dif, double* rad, double* gnd,
                                float* dump);
 
+/* The serving loop. No counterpart in the reference (its C ABI runs one call at a time behind one mutex, c_api.cpp:31,
+ * 231, 250): a server with `lanes` device contexts (1..16; each a HIP stream of its own, so runs overlap on the GPU)
+ * takes requests of both tools, parsed as dl_run parses them (the same flags, errors and results), and runs them as
+ * lanes come free:
+ *  - admission: a request starts when the device memory its run may need (an estimate from its sensors and flags) fits
+ *    hbm_budget beside the runs already going (0: 60 % of the memory free at open); one that alone exceeds it runs
+ *    when the device is otherwise idle;
+ *  - resident scenes: the context scenes stay in HBM between requests (the engine's store), pinned while a run uses them;
+ *  - continuous batching: a lane that comes free takes the oldest queued request and every queued daylight request
+ *    that merges with it - the same flags and context handles and the same centre of the run's meshes, without
+ *    self-occlusion, file outputs or flags naming meshes - up to batch_sensors sensor elements (0: 4,000,000), runs
+ *    them as one run over all their sensor meshes and splits the rows; each request gets the values and manifest it
+ *    gets alone.
+ * dlx_server_submit returns the request's id (> 0), or -1 with dl_last_error() (the flags' usage errors are found
+ * here); the mesh handles must stay valid until the request is waited for. dlx_server_wait blocks until the request is
+ * done and gives its result (as dl_run; DL_RUNTIME with dl_last_error() when its run failed) and, when times is not
+ * NULL, times[3] = {queued ms, run ms, requests served by its run}; each id is waited for once. dlx_server_stats gives
+ * DLX_SERVER_STATS counters: submitted, completed, failed, runs, requests served by merged runs, peak queue length, peak
+ * admitted bytes, the budget, lanes, peak runs at once. dlx_server_close finishes the queued requests and frees the
+ * server. A process may run dl_run beside a server: it takes its turn on the primary context. */
+typedef struct dlx_server dlx_server;
+#define DLX_SERVER_STATS 10
+DL_API dlx_server* dlx_server_open(int32_t lanes, int64_t hbm_budget, int64_t batch_sensors);
+DL_API int64_t dlx_server_submit(dlx_server* server, int32_t tool, int32_t argc, const char* const* argv, dl_mesh* const* sensors, int32_t n_sensors,
+                                 dl_mesh* const* context, int32_t n_context);
+DL_API int32_t dlx_server_wait(dlx_server* server, int64_t id, dl_result** out, double* times);
+DL_API int32_t dlx_server_stats(dlx_server* server, double* out, int32_t n);
+DL_API void dlx_server_close(dlx_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,6 +132,8 @@ int32_t make_request(int32_t tool, int32_t argc, const char* const* argv, dl_mes
   return DL_OK;
 }
 
+int32_t abi_status(int32_t code, const std::string& msg) { return status(code, msg); }
+
 std::unique_ptr<dl_result> make_result(const Request& r, Result& res) {
   auto out = std::make_unique<dl_result>();
   if (r.tool == DL_DAYLIGHT) out->manifest = cli::manifest_json(res);

@@ -97,6 +97,7 @@ class DaylightRun {
   Lane* lane = nullptr;
   dlg_gpu* g = nullptr;
   Scene sc;
+  SceneLease lease;   // a run on a server's lane: its scene stays resident until the run ends
   size_t chunkBytes = 0, perSensor = 1;
   dlg_stats_t st0{};
 
@@ -176,7 +177,8 @@ class DaylightRun {
     }
     translucent = !lit.empty();
     slotted = translucent || differs;
-    if (!O.sceneKey.empty()) key = O.sceneKey + (translucent ? "\ntransmitting " + lit : slotted ? "\nslot records" : "") + open_key(open, O.selfOcclude);
+    if (!O.sceneKey.empty())
+      key = scene_key_of(O.sceneKey, O.selfOcclude) + (translucent ? "\ntransmitting " + lit : slotted ? "\nslot records" : "") + open_key(open, O.selfOcclude);
     nS = O.sides == Sides::Both ? 2 : 1;
     for (const SimulationMeshView& v : M)
       if (v.sensor) nElem += O.onVertices ? v.mesh->numPoints() : v.mesh->numFaces();
@@ -194,7 +196,7 @@ class DaylightRun {
     if (cancelled()) return false;
     const auto t0 = Clk::now();
     stage = "computing the sky";
-    lane = Engine::get().primary(err);
+    lane = Engine::get().run_lane(err);
     if (!lane) return false;
     g = lane->gpu;
     mark("gpu");
@@ -324,35 +326,32 @@ class DaylightRun {
     chunkBytes = O.chunkBytes;
     const auto blocksK = [this](size_t k) { return blocks(k); };
     dlg_gpu* gp = nullptr;
-    if (!Engine::get().acquire(M, blocksK, slotted, key, O.residentScenes, O.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err,
-                               [this](const char* w) { mark(w); }))
+    if (!Engine::get().acquire(lane, M, blocksK, slotted, key, O.residentScenes, O.cancel, nDir > 0 ? &gp : nullptr, sc, out, stage, err,
+                               [this](const char* w) { mark(w); }, [this](Scene& fresh) { if (translucent) thin_solids(fresh); }, &lease))
       return false;
     mark("scene");
     if (!gp) { g = nullptr; return true; }
+    g = gp;
     dlg_stats(g, &st0);
-    if (translucent && !out.sceneReused) thin_solids();
     // the visibility, the bounce rows, and the series' output and two operand sets per sensor of a chunk
     const size_t buffers = 1 + (reflect ? 1 : 0) + (series ? 3 : 0);
     return Engine::get().admit(*lane, nSens, perSensor, buffers * perSensor + 8 * (nQ() + 1), 0, O.residentScenes, chunkBytes, err);
   }
 
-  // a fresh slotted build's thin solids (reference mesh.cpp:275-304 per transmitting slot), kept with the scene
-  void thin_solids() {
+  // a fresh slotted build's thin solids (reference mesh.cpp:275-304 per transmitting slot), kept with the scene (the store
+  // calls this before any other run can reuse the build)
+  void thin_solids(Scene& cur) {
     stage = "finding thin solids";
     const auto t0 = Clk::now();
-    Scene& cur = Engine::get().installed();
-    cur.key.clear();   // no run reuses the scene before they are known
-    std::vector<uint32_t>& thin = sc.thin;
+    std::vector<uint32_t>& thin = cur.thin;
     thin.assign(slots.size(), 0);
     std::vector<size_t> base(M.size() + 1, 0);
     for (size_t k = 0; k < M.size(); ++k) base[k + 1] = base[k] + slot_count(*M[k].mesh);
     par_for(M.size(), [&](size_t k) {
       if (!blocks(k)) return;
       for (size_t s = 0; s < slot_count(*M[k].mesh); ++s)
-        if (slots[base[k] + s].tau > 0) thin[base[k] + s] = thin_solid(*M[k].mesh, uint32_t(s), sc.c);
+        if (slots[base[k] + s].tau > 0) thin[base[k] + s] = thin_solid(*M[k].mesh, uint32_t(s), cur.c);
     });
-    cur.thin = thin;
-    cur.key = key;
     out.stats.prepMs += ms_since(t0);
   }
 
@@ -367,7 +366,7 @@ class DaylightRun {
     stage = "making the sensors";
     const auto t0 = Clk::now();
     std::string warn;
-    if (!lane) { lane = Engine::get().primary(err); if (!lane) return false; }
+    if (!lane) { lane = Engine::get().run_lane(err); if (!lane) return false; }
     sens = lane_sensors(*lane, M, O, sc.c, nSens, scratch, out, warn, err, ladybug);
     if (!sens) return false;
     out.stats.prepMs += ms_since(t0);

@@ -5,6 +5,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cstdio>
 
@@ -25,6 +26,14 @@ Engine& Engine::get() {
   return e;
 }
 
+namespace {
+thread_local Lane* boundLane = nullptr;
+}
+
+void Engine::bind(Lane* l) { boundLane = l; }
+
+Lane* Engine::run_lane(std::string& err) { return boundLane ? boundLane : primary(err); }
+
 Lane* Engine::primary(std::string& err) {
   if (!primary_.gpu) {
     char e[512] = {};
@@ -34,15 +43,15 @@ Lane* Engine::primary(std::string& err) {
 }
 
 Lane* Engine::new_lane(std::string& err) {
-  Lane* p = primary(err);
-  if (!p) return nullptr;
+  {
+    std::lock_guard<std::mutex> pk(primaryMu_);
+    if (!primary(err)) return nullptr;
+  }
   auto* l = new Lane;
   char e[512] = {};
   if (dlg_init(&l->gpu, e, sizeof e)) { err = e; delete l; return nullptr; }
   std::lock_guard<std::mutex> lk(lanesMu_);
-  dlg_borrow_scene(l->gpu, p->gpu);
   lanes_.push_back(l);
-  ++borrowers_;
   return l;
 }
 
@@ -51,7 +60,6 @@ void Engine::release_lane(Lane* l) {
   {
     std::lock_guard<std::mutex> lk(lanesMu_);
     lanes_.erase(std::remove(lanes_.begin(), lanes_.end(), l), lanes_.end());
-    --borrowers_;
   }
   dlg_destroy(l->gpu);   // a borrowed scene is left alone
   delete l;
@@ -59,11 +67,45 @@ void Engine::release_lane(Lane* l) {
 
 bool Engine::room(size_t need) const { return free_bytes() >= need; }
 
-// The installed scene leaves the primary lane: parked under its key when more scenes may stay resident (or a lane
-// borrows it), else freed.
+bool Engine::pinned(uint64_t id) const {
+  for (const auto& [k, n] : pins_)
+    if (k == id) return n > 0;
+  return false;
+}
+
+void Engine::unpin(uint64_t id) {
+  std::lock_guard<std::mutex> sk(storeMu_);
+  for (auto it = pins_.begin(); it != pins_.end(); ++it) {
+    if (it->first != id) continue;
+    if (--it->second > 0) return;
+    pins_.erase(it);
+    break;
+  }
+  for (auto it = forgotten_.begin(); it != forgotten_.end(); ++it)
+    if (it->first == id) { dlg_free_scene(it->second); forgotten_.erase(it); return; }
+}
+
+void SceneLease::release() {
+  if (id_) Engine::get().unpin(id_);
+  id_ = 0;
+}
+
+bool Engine::evict_one() {
+  for (auto it = parked_.end(); it != parked_.begin();) {
+    --it;
+    if (pinned(it->id)) continue;
+    dlg_free_scene(it->parked);
+    parked_.erase(it);
+    return true;
+  }
+  return false;
+}
+
+// The installed scene leaves the primary lane: parked under its key when more scenes may stay resident or a lease pins
+// it, else freed.
 void Engine::park_or_free_installed(int residentScenes) {
   dlg_scene* s = dlg_detach(primary_.gpu);
-  const bool keep = !installed_.key.empty() && (residentScenes > 1 || borrowers_ > 0);
+  const bool keep = (!installed_.key.empty() && residentScenes > 1) || pinned(installed_.id);
   if (keep) {
     installed_.parked = s;
     parked_.push_front(installed_);
@@ -71,21 +113,62 @@ void Engine::park_or_free_installed(int residentScenes) {
   installed_ = Scene{};
 }
 
-bool Engine::acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
-                     int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage,
-                     std::string& err, const std::function<void(const char*)>& mark) {
+bool Engine::acquire(Lane* lane, const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted,
+                     const std::string& key, int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out,
+                     const char*& stage, std::string& err, const std::function<void(const char*)>& mark, const std::function<void(Scene&)>& fresh,
+                     SceneLease* lease) {
+  const bool other = lane && lane != &primary_;
+  if (!gpu || !other) {   // the primary lane's runs hold its lock already
+    std::lock_guard<std::mutex> sk(storeMu_);
+    if (!acquire_locked(meshes, blocks, slotted, key, residentScenes, cancel, gpu != nullptr, s, out, stage, err, mark, fresh)) return false;
+    if (gpu) *gpu = primary_.gpu;
+    return true;
+  }
+  std::lock_guard<std::mutex> pk(primaryMu_);
+  std::lock_guard<std::mutex> sk(storeMu_);
+  if (!primary(err)) return false;
+  if (!acquire_locked(meshes, blocks, slotted, key, residentScenes, cancel, true, s, out, stage, err, mark, fresh)) return false;
+  dlg_borrow_scene(lane->gpu, primary_.gpu);
+  *gpu = lane->gpu;
+  if (lease) {
+    lease->release();
+    bool found = false;
+    for (auto& [k, n] : pins_)
+      if (k == installed_.id) { ++n; found = true; }
+    if (!found) pins_.push_back({installed_.id, 1});
+    lease->id_ = installed_.id;
+  }
+  return true;
+}
+
+bool Engine::acquire_locked(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted,
+                            const std::string& key, int residentScenes, const std::atomic<bool>* cancel, bool install, Scene& s, Result& out,
+                            const char*& stage, std::string& err, const std::function<void(const char*)>& mark, const std::function<void(Scene&)>& fresh) {
   const auto note = [&](const char* what) { if (mark) mark(what); };
+  // A resident scene under the key holds the blocking meshes; it serves this run when the run's centre (its box: the
+  // scene's, grown by the meshes that do not block) is the scene's, bit for bit, so the recentred coordinates are the
+  // ones a fresh build would make. Sensors that do not block can then change from run to run without a rebuild.
   const bool keyed = !key.empty() && primary_.gpu;
-  const bool isInstalled = keyed && key == installed_.key;
-  auto parkedAt = std::find_if(parked_.begin(), parked_.end(), [&](const Scene& p) { return keyed && p.key == key; });
+  double other[6];
+  bool otherDone = false;
+  const auto serves = [&](const Scene& sc) {
+    if (!keyed || sc.key != key) return false;
+    if (!otherDone) { bbox_of(meshes, [&](size_t k) { return !blocks(k); }, other); otherDone = true; }
+    double box[6], c[3];
+    std::copy(sc.box, sc.box + 6, box);
+    box_union(box, other);
+    box_centre(box, c);
+    return c[0] == sc.c[0] && c[1] == sc.c[1] && c[2] == sc.c[2];
+  };
+  const bool isInstalled = serves(installed_);
+  auto parkedAt = isInstalled ? parked_.end() : std::find_if(parked_.begin(), parked_.end(), serves);
   out.sceneReused = isInstalled || parkedAt != parked_.end();
   s = Scene{};
   if (out.sceneReused) { s = isInstalled ? installed_ : *parkedAt; s.parked = nullptr; }
-  if (!gpu) return true;   // no rays: nothing to install
+  if (!install) return true;   // no rays: nothing to install
   if (cancel && cancel->load()) { err = "cancelled"; return false; }
   Lane* lane = primary(err);
   if (!lane) return false;
-  *gpu = lane->gpu;
   if (isInstalled) return true;
   park_or_free_installed(residentScenes);
   if (parkedAt != parked_.end()) {   // back from the parked ones
@@ -99,7 +182,14 @@ bool Engine::acquire(const std::vector<SimulationMeshView>& meshes, const std::f
   // a fresh build: the blocking meshes' points recentred and their faces fanned on the GPU (n-gons ear-clipped here)
   stage = "triangulating the scene";
   const auto tPrep = Clk::now();
-  bbox_centre(meshes, s.c);
+  {
+    double all[6];
+    bbox_of(meshes, blocks, s.box);
+    if (!otherDone) bbox_of(meshes, [&](size_t k) { return !blocks(k); }, other);
+    std::copy(s.box, s.box + 6, all);
+    box_union(all, other);
+    box_centre(all, s.c);
+  }
   const size_t n = meshes.size();
   std::vector<size_t> slotBase(n + 1, 0);
   size_t nPts = 0, nTri = 0;
@@ -118,9 +208,7 @@ bool Engine::acquire(const std::vector<SimulationMeshView>& meshes, const std::f
   const size_t staged = nPts * 12 + nTri * (slotted ? 16 : 12);
   const size_t build = size_t(dlg_build_bytes(int64_t(nPts), int64_t(nTri))) + (slotted ? nTri * 4 : 0);
   const size_t need = std::max(build, staged + size_t(meshPeak)) + kVramMargin;
-  while (!room(need) && !parked_.empty()) {   // the least recently used parked scenes go first
-    dlg_free_scene(parked_.back().parked);
-    parked_.pop_back();
+  while (!room(need) && evict_one()) {   // the least recently used parked scenes go first
   }
   if (!room(need)) {
     err = "out of memory: building the scene (" + std::to_string(nTri) + " triangles) needs " + gb(need) + " of GPU memory, " + gb(free_bytes()) +
@@ -155,6 +243,8 @@ bool Engine::acquire(const std::vector<SimulationMeshView>& meshes, const std::f
   note("build");
   s.nTris = tri0;
   s.key = key;
+  s.id = nextId_++;
+  if (fresh) fresh(s);
   installed_ = s;
   return true;
 }
@@ -166,10 +256,12 @@ bool Engine::admit(Lane& lane, size_t nSens, size_t perSensor, size_t chunkSenso
     return nSens * 24 + perChunk * chunkSensor + fixedBytes + kVramMargin;
   };
   if (!room(need())) dlg_trim(lane.gpu, 1);   // the lane's last buffers are kept for the next run unless the room is needed
+  std::lock_guard<std::mutex> sk(storeMu_);
   const size_t keep = size_t(std::max(1, residentScenes)) - 1;
-  while (!parked_.empty() && (parked_.size() > keep || !room(need()))) {
-    dlg_free_scene(parked_.back().parked);
-    parked_.pop_back();
+  size_t unpinned = 0;
+  for (const Scene& p : parked_) unpinned += pinned(p.id) ? 0 : 1;
+  while (unpinned > keep && evict_one()) --unpinned;
+  while (!room(need()) && evict_one()) {
   }
   while (!room(need()) && chunkBytes > kMinChunk) chunkBytes /= 2;
   if (room(need())) return true;
@@ -179,9 +271,14 @@ bool Engine::admit(Lane& lane, size_t nSens, size_t perSensor, size_t chunkSenso
 }
 
 void Engine::forget(const std::string& key) {
+  std::lock_guard<std::mutex> sk(storeMu_);
   if (installed_.key == key) installed_.key.clear();
-  for (auto it = parked_.begin(); it != parked_.end();)
-    if (it->key == key) { dlg_free_scene(it->parked); it = parked_.erase(it); } else ++it;
+  for (auto it = parked_.begin(); it != parked_.end();) {
+    if (it->key != key) { ++it; continue; }
+    if (pinned(it->id)) forgotten_.push_back({it->id, it->parked});
+    else dlg_free_scene(it->parked);
+    it = parked_.erase(it);
+  }
 }
 
 const Sensors* lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
@@ -207,35 +304,18 @@ const Sensors* lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& m
   return &lane.sensors;
 }
 
-// ------------------------------------------------------------------------------------------------ the core.h forms
-dlg_gpu* process_gpu(std::string& err) {
-  Lane* l = Engine::get().primary(err);
-  return l ? l->gpu : nullptr;
-}
-
-Scene& installed_scene() { return Engine::get().installed(); }
-
-bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
-                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err,
-                   const std::function<void(const char*)>& mark) {
-  return Engine::get().acquire(meshes, blocks, slotted, key, residentScenes, cancel, gpu, s, out, stage, err, mark);
-}
-
-bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
-                   std::string& err) {
-  std::string e;
-  Lane* l = Engine::get().primary(e);
-  if (!l || l->gpu != g) { err = "internal: admission of a context that is not the primary lane's"; return false; }
-  return Engine::get().admit(*l, nSens, perSensor, chunkSensor, fixedBytes, residentScenes, chunkBytes, err);
-}
-
-void scene_forget(const std::string& key) { Engine::get().forget(key); }
-
-const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
-                              Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug) {
-  Lane* l = Engine::get().primary(err);
-  if (!l) return nullptr;
-  return lane_sensors(*l, meshes, opt, c, nSens, scratch, out, warn, err, ladybug);
+std::string scene_key_of(const std::string& meshKey, bool selfOcclude) {
+  if (selfOcclude) return meshKey;
+  std::string k;
+  for (size_t i = 0; i < meshKey.size();) {   // drop every " s<digits>" token
+    if (meshKey.compare(i, 2, " s") == 0 && i + 2 < meshKey.size() && std::isdigit(static_cast<unsigned char>(meshKey[i + 2]))) {
+      i += 2;
+      while (i < meshKey.size() && std::isdigit(static_cast<unsigned char>(meshKey[i]))) ++i;
+      continue;
+    }
+    k += meshKey[i++];
+  }
+  return k;
 }
 
 std::string open_key(const std::vector<char>& open, bool selfOcclude) {

@@ -198,24 +198,33 @@ void par_for(size_t n, const std::function<void(size_t)>& body) {
   if (first) std::rethrow_exception(first);
 }
 
-void bbox_centre(const std::vector<SimulationMeshView>& meshes, double c[3]) {
+void bbox_of(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& which, double box[6]) {
   // blocks of points bounded on the host threads (min and max are exact in any order), then merged
   constexpr size_t kBlock = size_t(1) << 18;
   struct Span { const double* p; size_t n; };
   std::vector<Span> spans;
-  for (const SimulationMeshView& v : meshes)
-    for (size_t i = 0, n = v.mesh->numPoints(); i < n; i += kBlock) spans.push_back({v.mesh->points.data() + 3 * i, std::min(kBlock, n - i)});
-  std::vector<std::array<double, 6>> box(spans.size());
+  for (size_t k = 0; k < meshes.size(); ++k) {
+    if (which && !which(k)) continue;
+    const SimulationMesh& m = *meshes[k].mesh;
+    for (size_t i = 0, n = m.numPoints(); i < n; i += kBlock) spans.push_back({m.points.data() + 3 * i, std::min(kBlock, n - i)});
+  }
+  std::vector<std::array<double, 6>> part(spans.size());
   par_for(spans.size(), [&](size_t s) {
     std::array<double, 6> b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (const double *p = spans[s].p, *e = p + 3 * spans[s].n; p < e; p += 3)
       for (int a = 0; a < 3; ++a) { b[size_t(a)] = std::fmin(b[size_t(a)], p[a]); b[size_t(3 + a)] = std::fmax(b[size_t(3 + a)], p[a]); }
-    box[s] = b;
+    part[s] = b;
   });
-  std::array<double, 6> all = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (const auto& b : box)
-    for (size_t a = 0; a < 3; ++a) { all[a] = std::fmin(all[a], b[a]); all[3 + a] = std::fmax(all[3 + a], b[3 + a]); }
-  for (size_t a = 0; a < 3; ++a) c[a] = all[a] <= all[3 + a] ? (all[a] + all[3 + a]) / 2 : 0;
+  for (size_t a = 0; a < 3; ++a) { box[a] = INFINITY; box[3 + a] = -INFINITY; }
+  for (const auto& b : part) box_union(box, b.data());
+}
+
+void box_union(double box[6], const double other[6]) {
+  for (size_t a = 0; a < 3; ++a) { box[a] = std::fmin(box[a], other[a]); box[3 + a] = std::fmax(box[3 + a], other[3 + a]); }
+}
+
+void box_centre(const double box[6], double c[3]) {
+  for (size_t a = 0; a < 3; ++a) c[a] = box[a] <= box[3 + a] ? (box[a] + box[3 + a]) / 2 : 0;
 }
 
 size_t tri_cap(const SimulationMesh& m) {

@@ -116,6 +116,7 @@ class ViewsRun {
   Lane* lane = nullptr;
   dlg_gpu* g = nullptr;
   Scene sc;
+  SceneLease lease;   // a run on a server's lane: its scene stays resident until the run ends
   std::vector<uint8_t> label;
   size_t chunkBytes = 0, perSensor = 1;
   dlg_stats_t st0{};
@@ -145,7 +146,7 @@ class ViewsRun {
       anyPass = anyPass || n > 0;
       if (clsOf[k] != 2 && passAll[k]) return fail("--target " + M[k].name + ": rays pass through it (glass with --transmission on), none can end on it");
     }
-    if (!O.sceneKey.empty()) key = O.sceneKey + open_key(open, O.selfOcclude);
+    if (!O.sceneKey.empty()) key = scene_key_of(O.sceneKey, O.selfOcclude) + open_key(open, O.selfOcclude);
     nS = O.sides == Sides::Both ? 2 : 1;
     for (const SimulationMeshView& v : M)
       if (v.sensor) nElem += O.onVertices ? v.mesh->numPoints() : v.mesh->numFaces();
@@ -220,16 +221,16 @@ class ViewsRun {
 
   bool scene() {
     const auto blocksK = [this](size_t k) { return blocks(k); };
-    if (!Engine::get().acquire(M, blocksK, false, key, O.residentScenes, O.cancel, &g, sc, out, stage, err, {})) return false;
+    lane = Engine::get().run_lane(err);
+    if (!lane) return false;
+    if (!Engine::get().acquire(lane, M, blocksK, false, key, O.residentScenes, O.cancel, &g, sc, out, stage, err, {}, {}, &lease)) return false;
     stage = "labelling the targets and the glass";
     if ((!groups.empty() || anyPass) && !labels()) {
       Engine::get().forget(key);
-      if (!Engine::get().acquire(M, blocksK, false, key, O.residentScenes, O.cancel, &g, sc, out, stage, err, {})) return false;
+      if (!Engine::get().acquire(lane, M, blocksK, false, key, O.residentScenes, O.cancel, &g, sc, out, stage, err, {}, {}, &lease)) return false;
       if (!labels()) return fail("internal: a target or glass mesh is not in the scene as built");
     }
     dlg_stats(g, &st0);
-    lane = Engine::get().primary(err);
-    if (!lane) return false;
     chunkBytes = O.chunkBytes;
     perSensor = std::max(4 * K, NA * sizeof(double));
     return Engine::get().admit(*lane, nSens, perSensor, 5 * K + NA * sizeof(double), label.size(), O.residentScenes, chunkBytes, err);

@@ -62,7 +62,12 @@ inline std::vector<SimulationMeshView> views_of(const std::vector<SimulationMesh
   return v;
 }
 void par_for(size_t n, const std::function<void(size_t)>& f);
-void bbox_centre(const std::vector<SimulationMeshView>& meshes, double c[3]);
+// The bounding box (lo xyz, hi xyz) of the points of the meshes k with which(k) (all when which is empty); empty: lo
+// +inf, hi -inf. The scene and its sensors are recentred on the centre of the box of every mesh of the run (reference
+// run.cpp:68-156), midway per axis (0 on an empty axis).
+void bbox_of(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& which, double box[6]);
+void box_union(double box[6], const double other[6]);
+void box_centre(const double box[6], double c[3]);
 size_t tri_cap(const SimulationMesh& m);
 // m's triangles (points recentred by c, indices from base) into v / tri; with slots non-null, each triangle's global slot
 // (slotBase + its face's slot) into slots. Returns the triangles made (degenerate n-gons skipped and counted).
@@ -97,34 +102,22 @@ bool npy_write_header(std::FILE* f, size_t rows, size_t cols, const char* descr)
 struct SceneMesh { std::string name; size_t tri0 = 0, nTri = 0; const SimulationMesh* mesh = nullptr; };
 struct Scene {
   std::string key;
+  uint64_t id = 0;   // the store's number of the build (keys can repeat after a forget; ids do not)
   double c[3] = {0, 0, 0};
+  double box[6] = {0, 0, 0, 0, 0, 0};   // the bounding box of the meshes it was built from (the blocking ones)
   size_t nTris = 0;
   int skipped = 0;
   std::vector<SceneMesh> order;
   std::vector<uint32_t> thin;   // per global slot: thin_solid of a transmitting slot (a slotted build's)
   dlg_scene* parked = nullptr;
 };
-// The process's GPU (made on first use, never destroyed), or null with err.
-dlg_gpu* process_gpu(std::string& err);
-// The run's scene: the installed or a parked one under key, else built from the blocking meshes (slotted: each
-// triangle's global slot kept with the leaves, dlg_set_tri_slots). mark (may be empty) is called at the build's stage
-// boundaries ("centre", "ear-clip", "ingest", "build") for the run's -v timing line.
-bool scene_acquire(const std::vector<SimulationMeshView>& meshes, const std::function<bool(size_t)>& blocks, bool slotted, const std::string& key,
-                   int residentScenes, const std::atomic<bool>* cancel, dlg_gpu** gpu, Scene& s, Result& out, const char*& stage, std::string& err,
-                   const std::function<void(const char*)>& mark = {});
-// the process's installed scene (a run sets the thin solids of a fresh slotted build on it)
-Scene& installed_scene();
-bool run_admission(dlg_gpu* g, size_t nSens, size_t perSensor, size_t chunkSensor, size_t fixedBytes, int residentScenes, size_t& chunkBytes,
-                   std::string& err);
 std::string open_key(const std::vector<char>& open, bool selfOcclude);
-void scene_forget(const std::string& key);
+// The scene key of a run from its mesh key (RunOptions::sceneKey, " s<serial>" per sensor handle and " c<serial>" per
+// context handle): without self-occlusion the sensor meshes are not in the scene, so their handles leave the key and
+// runs over different sensors share the context's scene.
+std::string scene_key_of(const std::string& meshKey, bool selfOcclude);
 bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
                 std::string& warn, std::string& err, bool ladybug = false);
-// The run's sensors: those of the last run when it had the same meshes (opt.sceneKey: the mesh handles, which never
-// change) and the same sensor options, else made by sensors_of and kept for the next run. Without a scene key, made
-// into `scratch`. The pointer stays valid until the next run.
-const Sensors* sensors_cached(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
-                              Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug = false);
 double ms_since(const void* clockStart);
 // daylight::run with the --timeseries output kept on the device: per chunk of sensors, sink(i0, nc, T, E) with E the
 // device array float [nc x T] (null when the run has no series columns); an empty sink drops the series.

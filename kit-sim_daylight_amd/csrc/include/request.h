@@ -50,6 +50,9 @@ int32_t make_request(int32_t tool, int32_t argc, const char* const* argv, dl_mes
 // finish).
 std::unique_ptr<dl_result> make_result(const Request& r, Result& res);
 
+// records msg as the calling thread's dl_last_error() and returns code
+int32_t abi_status(int32_t code, const std::string& msg);
+
 // The "C" locale for the calling thread while it lives: the flags' and the weather file's numbers and the manifest
 // do not depend on the caller's locale.
 class CLocaleScope {

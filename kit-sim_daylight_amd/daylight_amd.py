@@ -71,6 +71,7 @@ class ViewIn(C.Structure):
 
 SERIES_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p)
 DLX_COUNTERS = 19
+DLX_SERVER_STATS = 10
 
 
 # every function include/*.h declares: (name, restype, argtypes)
@@ -98,6 +99,11 @@ C_ABI = [
     ("dlx_run_series_device", C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
     ("dlx_gpu_counters", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("dlx_server_open", C.c_void_p, [C.c_int32, C.c_int64, C.c_int64]),
+    ("dlx_server_submit", C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
+    ("dlx_server_wait", C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    ("dlx_server_stats", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),
+    ("dlx_server_close", None, [C.c_void_p]),
     ("dlx_count_fetches", None, [C.c_int32]),
     ("dlx_chunk_bytes", None, [C.c_int64]),
     ("dlx_read_epw", C.c_int32, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),

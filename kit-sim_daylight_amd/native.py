@@ -8,7 +8,9 @@ and stats keys) over this build's C ABI (include/daylight_c.h)::
         r = run('daylight', ['--epw', 'site.epw', '--quantity', 'radiation'], box)
     print(r.manifest['columns'], r.values.shape)
 
-run_series_device() is this build's addition: a daylight run whose --timeseries output stays on the GPU.
+run_series_device() is this build's addition: a daylight run whose --timeseries output stays on the GPU; so is Server,
+the serving loop (dlx_server_*): requests of both tools queued, admitted against an HBM budget, run on lanes of their
+own over resident scenes, compatible daylight requests merged into one run.
 """
 import ctypes as C
 import json
@@ -20,7 +22,7 @@ import numpy as np
 
 import daylight_amd as _d
 
-__all__ = ["Control", "DaylightError", "SimulationMesh", "Result", "run", "run_series_device", "usage", "gpu_counters",
+__all__ = ["Control", "DaylightError", "SimulationMesh", "Result", "Server", "run", "run_series_device", "usage", "gpu_counters",
            "count_fetches", "chunk_bytes", "read_epw"]
 
 # dl_result_stats order (reference bindings/python/daylight/native.py:24-25)
@@ -212,3 +214,43 @@ def read_epw(path):
     dni, dhi, loc = np.zeros(n.value, np.float32), np.zeros(n.value, np.float32), np.zeros(3)
     _check(L.dlx_read_epw(os.fsencode(path), dni.ctypes.data, dhi.ctypes.data, n.value, C.byref(n), loc.ctypes.data))
     return dni, dhi, tuple(loc.tolist())
+
+
+# dlx_server_stats order (include/daylight_c.h)
+_SERVER_STATS = ("submitted", "completed", "failed", "runs", "merged", "peak_queue", "peak_bytes", "budget", "lanes", "peak_running")
+
+
+class Server(_Handle):
+    """The serving loop (dlx_server_*): submit(tool, args, sensors, context) queues a request and returns its id at once
+    (DaylightError for flag errors); wait(id) blocks until it is done -> (Result as run(), times dict: queued_ms, run_ms,
+    batch). The meshes of a request must stay open until it is waited for."""
+
+    def __init__(self, lanes=2, hbm_budget=0, batch_sensors=0):
+        L = _d.lib()
+        self._own(L.dlx_server_open(int(lanes), int(hbm_budget), int(batch_sensors)), L.dlx_server_close, _d.DL_RUNTIME)
+        self._keep = {}
+
+    def submit(self, tool, args, sensors, context=()):
+        t = _tool(tool)
+        n, argv = _argv(args)
+        sens, ctx = _meshes(sensors), _meshes(context)
+        rid = _d.lib().dlx_server_submit(self._h, t, n, argv, _handles(sens), len(sens), _handles(ctx), len(ctx))
+        if rid < 0:
+            msg = (_d.lib().dl_last_error() or b"").decode(errors="replace")
+            raise DaylightError(_d.DL_USAGE, msg)
+        self._keep[rid] = (sens, ctx)   # the handles stay alive until the request is waited for
+        return rid
+
+    def wait(self, rid):
+        out = C.c_void_p()
+        times = np.zeros(3)
+        try:
+            _check(_d.lib().dlx_server_wait(self._h, int(rid), C.byref(out), times.ctypes.data))
+        finally:
+            self._keep.pop(rid, None)
+        return _take_result(out), {"queued_ms": float(times[0]), "run_ms": float(times[1]), "batch": int(times[2])}
+
+    def stats(self):
+        a = np.zeros(_d.DLX_SERVER_STATS)
+        _check(_d.lib().dlx_server_stats(self._h, a.ctypes.data, len(a)))
+        return dict(zip(_SERVER_STATS, a.tolist()))
