@@ -259,6 +259,90 @@ def transmission_leg(native, torch, epw, tmp):
                             "traced": round(d["traced"] / (d["trace_ms"] * 1e-3) / 1e9, 3)}}
 
 
+SERVING_VIEW_ARGS = ["--fov", "360", "--cell", "2", "--far", "1000", "--jitter", "on", "--distance", "isovist,mean,min"]
+
+
+def serving_leg(native, epw, n=100, lanes=3, seed=11, load_factor=0.8):
+    """The serving mix over the resident 5M-triangle city (417,316 boxes): n requests, 70/30 daylight/views, sensor
+    faces log-uniform in [1k, 500k] cut from the central buildings' faces, time span instant / 24 h / year and sky
+    density 1 / 2 / 4 drawn uniformly (workloads.serving_load). First one request at a time through dl_run (its service
+    times, and its latencies under the arrivals below, FIFO); then Poisson arrivals at load_factor / (its mean service
+    time) replayed against a server of `lanes` lanes (admission by HBM budget, compatible daylight requests merged);
+    then every request at once (capacity). Each server result is compared with the dl_run one."""
+    import threading
+    boxes, central = W.city_5m()
+    ctx = W.box_mesh(boxes)
+    pool = W.city_sensor_meshes(boxes, central, (("pool", 500_000),))[0][1]
+    Cx = native.SimulationMesh("city5m", *ctx)
+    load = W.serving_load(n, seed)
+    meshes = [native.SimulationMesh("request%d" % k, *W.submesh(pool, r["f0"], r["f0"] + r["faces"])) for k, r in enumerate(load)]
+
+    def args_of(r):
+        if r["tool"] == "views":
+            return SERVING_VIEW_ARGS
+        return ["--epw", epw, "--sky-density", str(r["mf"]), "--quantity", "radiation,illuminance", "--no-self-occlude"] + W.SPANS[r["span"]]
+
+    args = [args_of(r) for r in load]
+    native.run("daylight", args_of(dict(load[0], tool="daylight")), meshes[0], Cx)   # the scene resident before either path
+    service, want = [], []
+    for k, r in enumerate(load):
+        t0 = time.perf_counter()
+        want.append(native.run(r["tool"], args[k], meshes[k], Cx).values)
+        service.append(time.perf_counter() - t0)
+    rate = load_factor / float(np.mean(service))
+    arrive = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+    fin, serial_lat = 0.0, []
+    for a, sv in zip(arrive, service):
+        fin = max(a, fin) + sv
+        serial_lat.append(fin - a)
+    srv = native.Server(lanes=lanes)
+    try:
+        got = [None] * n
+
+        def waiter(k, rid):
+            got[k] = srv.wait(rid)
+
+        threads = []
+        t0 = time.perf_counter()
+        for k in range(n):
+            d = arrive[k] - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            th = threading.Thread(target=waiter, args=(k, srv.submit(load[k]["tool"], args[k], meshes[k], Cx)))
+            th.start()
+            threads.append(th)
+        for th in threads:
+            th.join()
+        replay = time.perf_counter() - t0
+        st = srv.stats()
+        server_lat = [(t["queued_ms"] + t["run_ms"]) / 1e3 for _, t in got]
+        identical = all(np.array_equal(got[k][0].values, want[k], equal_nan=True) for k in range(n))
+        t1 = time.perf_counter()
+        ids = [srv.submit(load[k]["tool"], args[k], meshes[k], Cx) for k in range(n)]
+        burst = [srv.wait(i) for i in ids]
+        burst_s = time.perf_counter() - t1
+        identical = identical and all(np.array_equal(burst[k][0].values, want[k], equal_nan=True) for k in range(n))
+        st_burst = srv.stats()
+    finally:
+        srv.close()
+    for m in meshes:
+        m.close()
+    pct = lambda x, q: round(1e3 * float(np.percentile(x, q)), 1)
+    return {"workload": "seeded 5M-triangle city (417,316 boxes, %d triangles) resident; %d requests, %d daylight / %d views; sensor "
+                        "faces log-uniform 1k-500k (mean %d); spans instant / 24 h / year and sky density 1 / 2 / 4 uniform; "
+                        "views 360 x 120 degrees in 2-degree cells" % (2 * len(ctx[1]), n, sum(r["tool"] == "daylight" for r in load),
+                                                                      sum(r["tool"] == "views" for r in load), int(np.mean([r["faces"] for r in load]))),
+            "arrival_rate_per_s": round(rate, 3),
+            "one_at_a_time": {"service_ms_mean": round(1e3 * float(np.mean(service)), 1), "capacity_requests_per_s": round(n / float(np.sum(service)), 3),
+                              "p50_ms": pct(serial_lat, 50), "p99_ms": pct(serial_lat, 99)},
+            "server": {"lanes": lanes, "p50_ms": pct(server_lat, 50), "p99_ms": pct(server_lat, 99), "replay_s": round(replay, 2),
+                       "capacity_requests_per_s": round(n / burst_s, 3), "runs": st_burst["runs"], "merged_requests": st_burst["merged"],
+                       "peak_running": st_burst["peak_running"], "budget_gb": round(st["budget"] / 1e9, 1)},
+            "identical_to_one_at_a_time": identical,
+            "note": "latency = submit to result (the flags and weather parse included, as in dl_run's time); one_at_a_time latencies "
+                    "are its measured service times queued FIFO under the same arrivals"}
+
+
 BOSTON = ["--clear-sky", "--lat", "42.35", "--lon", "-71.07", "--tz", "-5"]
 # (label, tool arguments, the reference's published figure on an RTX 2060: docs/performance.md)
 REFERENCE_BENCHES = [
@@ -341,7 +425,8 @@ def main():
     ap.add_argument("--no-count", action="store_true", help="skip the traversal's counted run")
     ap.add_argument("--no-legs", action="store_true", help="skip the urban-block, views and byte-mode legs")
     ap.add_argument("--no-reference-benches", action="store_true", help="skip the reference's --bench runs through the CLI")
-    ap.add_argument("--only", choices=("headline", "views", "block", "transmission"), help="run one leg alone (profiling)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--only", choices=("headline", "views", "block", "transmission", "serving"), help="run one leg alone (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
@@ -373,6 +458,9 @@ def main():
         return
     if a.only == "transmission":
         print(json.dumps({"transmission": transmission_leg(native, torch, epw, tmp)}), flush=True)
+        return
+    if a.only == "serving":
+        print(json.dumps({"serving": serving_leg(native, epw)}), flush=True)
         return
 
     boxes, central = W.city_3m()
@@ -491,6 +579,9 @@ def main():
         print("bench: views %s" % json.dumps({k: v for k, v in legs["views"].items() if k != "columns"}), file=sys.stderr, flush=True)
         legs["transmission"] = transmission_leg(native, torch, epw, tmp)
         print("bench: transmission %s" % json.dumps(legs["transmission"]), file=sys.stderr, flush=True)
+    if not a.no_legs and not a.no_serving:
+        legs["serving"] = serving_leg(native, epw)
+        print("bench: serving %s" % json.dumps(legs["serving"]), file=sys.stderr, flush=True)
     refs = None if a.no_reference_benches else reference_benches()
     cpu = None if a.no_cpu else cpu_baseline(ctx, meshes[1][1], dni, dhi, threads, model, a.cpu_seconds)
     lat_ms = np.array(lat) * 1e3

@@ -99,6 +99,7 @@ class Server {
   int64_t submit(int32_t tool, int32_t argc, const char* const* argv, dl_mesh* const* sensors, int32_t nSensors, dl_mesh* const* context,
                  int32_t nContext, std::string& err, int32_t& rc) {
     auto j = std::make_shared<Job>();
+    j->submitted = Clk::now();   // the latency counts the parse (the weather file read) as dl_run's time does
     const CLocaleScope cLocale;
     rc = make_request(tool, argc, argv, sensors, nSensors, context, nContext, nullptr, false, j->req, err);
     if (rc) return -1;
@@ -110,7 +111,6 @@ class Server {
       if (v.sensor) { j->elems += o.onVertices ? v.mesh->numPoints() : v.mesh->numFaces(); ++j->meshes; }
     j->bytes = estimate(j->req, j->elems);
     j->batchKey = batch_key(*j);
-    j->submitted = Clk::now();
     std::lock_guard<std::mutex> lk(mu_);
     j->id = nextId_++;
     jobs_[j->id] = j;

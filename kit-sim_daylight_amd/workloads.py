@@ -7,6 +7,7 @@
   3M-triangle city is the 500 x 500 grid (250,000 boxes, 3,000,000 triangles).
 - sensor_mesh(): roofs and facades of boxes in quads of at most `cell` metres, facing out.
 - city_sensor_meshes(): the 1k / 150k / 850k-face sensor meshes over the 3M-triangle city's central buildings.
+- city_5m(), serving_load(): the 5M-triangle city and the serving mix drawn over its central buildings' faces.
 - interior(), floor_grid(): the seeded 2M-triangle interior and its 0.5 m floor grid of sensors.
 - weather(), write_epw(): the cloud-modulated clear-sky year (Beta(2, 3) cover) at lat 52.52, lon 13.41, tz 1.
 
@@ -132,6 +133,32 @@ def city_sensor_meshes(boxes, central, sizes=CITY3M_MESHES):
     for name, n in sizes:
         out.append((name, submesh(allm, f, f + n)))
         f += n
+    return out
+
+
+# ---------------------------------------------------------------- the serving load over the 5M-triangle city
+
+CITY5M = dict(seed=5, n_side=646, pitch=50.0)
+
+
+def city_5m(n_sensor_buildings=400):
+    """The 5M-triangle city: 417,316 boxes (12 triangles each) on a 646 x 646 grid of 50 m plots, seed 5."""
+    return city(n_sensor_buildings=n_sensor_buildings, **CITY5M)
+
+
+SPANS = {"instant": ["--time", "06-21T12:00"], "day": ["--from", "06-21T00:00", "--to", "06-22T00:00", "--step", "60"], "year": []}
+
+
+def serving_load(n, seed=11, pool_faces=500_000, views_share=0.3):
+    """n requests of the serving mix: tool (daylight, or views with probability views_share), sensor faces log-uniform in
+    [1,000, 500,000] cut at a random offset from a pool of pool_faces faces, a time span (instant / 24 h at 1 h / year)
+    and a sky density (1 / 2 / 4) drawn uniformly -> [dict(tool, f0, faces, span, mf)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        faces = int(round(math.exp(rng.uniform(math.log(1_000), math.log(500_000)))))
+        out.append(dict(tool="views" if rng.uniform() < views_share else "daylight", faces=faces,
+                        f0=int(rng.integers(0, pool_faces - faces + 1)), span=str(rng.choice(list(SPANS))), mf=int(rng.choice([1, 2, 4]))))
     return out
 
 
