@@ -97,11 +97,34 @@ int32_t dlg_set_slots(dlg_gpu* g, const dlg_slot* slots, int64_t n, char* err, i
  * offsets). A scene built without them has every triangle in slot 0. */
 int32_t dlg_set_tri_slots(dlg_gpu* g, const uint32_t* slot, int64_t nt, char* err, int32_t errLen);
 
-/* gpu_set_scene (internal.h:146): resident sensors (points P[3 n], normals N[3 n], recentred) and directions D[3 nDir]. */
+/* gpu_set_scene (internal.h:146): resident sensors (points P[3 n], normals N[3 n], recentred) and directions D[3 nDir];
+ * nSens = -1 keeps the resident sensors (those dlg_sensors made). */
 int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens, const float* D, int64_t nDir, char* err, int32_t errLen);
 /* gpu_ray_flags (internal.h:218): the reference's OptiX flag values (TERMINATE_ON_FIRST_HIT 4 | DISABLE_ANYHIT 1 |
  * CULL_BACK_FACING_TRIANGLES 16), kept so callers and tests read the same numbers. */
 uint32_t dlg_ray_flags(int32_t cullBackfaces, int32_t translucent);
+
+/* Sensor preparation on the GPU, in place of the host's face_geom / make_sensors_into / sensors_of (reference
+ * mesh.cpp:45-71, 315-349, run.cpp:192-213): dlg_sensors_reserve sizes the resident sensors to nSens (dlg_set_scene's
+ * P / N, left as they are: each is then written by dlg_sensors); dlg_sensors writes one mesh's sensors from sensor
+ * j0 on - per face (onVertices 0) or per point, sides 0 front / 1 back / 2 both per element, offset along the normal,
+ * recentred on centre[3], ladybug's face rules when ladybug != 0 - with the host's FP64 arithmetic (bit-identical),
+ * and gives the elements made, the faces under 1e-12 of area and the points whose normal sum is 0 (set to +z). The
+ * mesh's indices must be in range (they are clamped, not checked). Directions come from dlg_set_scene with nSens = -1
+ * (keep the sensors). */
+typedef struct {
+  const double* xyz;
+  int64_t nPts;
+  const int32_t* counts;
+  int64_t nFaces;
+  const int32_t* idx;
+  int64_t nIdx;
+} dlg_sensor_mesh;
+int32_t dlg_sensors_reserve(dlg_gpu* g, int64_t nSens, char* err, int32_t errLen);
+int32_t dlg_sensors(dlg_gpu* g, const dlg_sensor_mesh* m, int64_t j0, int32_t onVertices, int32_t sides, double offset, const double* centre,
+                    int32_t ladybug, int64_t* made, int32_t* degenerate, int32_t* zeroNormals, char* err, int32_t errLen);
+/* The first n resident sensors (points and normals, 3 floats each), for checks. */
+int32_t dlg_download_sensors(dlg_gpu* g, float* P, float* N, int64_t n, char* err, int32_t errLen);
 /* gpu_trace (internal.h:151): visibility bits of sensors [i0, i0 + nc) against every direction, on the device. Ray
  * flags without DISABLE_ANYHIT (dlg_ray_flags(., 1): a slot transmits): byte mode, the transmittance of every direction
  * as a byte (x 255, 0 = culled or blocked) instead of a bit, through the slots of dlg_set_slots / dlg_set_tri_slots

@@ -71,7 +71,7 @@ void dlg_destroy(dlg_gpu* g) {
   dlg_graph_free(g);
   g->scene.release();
   for (DevBuf* b : {&g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->slots, &g->dc, &g->skySdc,
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->sensWork, &g->slots, &g->dc, &g->skySdc,
                     &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
     b->release();
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
@@ -91,7 +91,7 @@ int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmB
 }
 
 void dlg_trim(dlg_gpu* g, int32_t sensors) {
-  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->sensWork, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
   g->meshOpen = 0;
   g->meshNv = g->meshNt = 0;
   if (sensors) {
@@ -133,13 +133,15 @@ int64_t dlg_free_bytes(void) {
 }
 
 int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens, const float* D, int64_t nDir, char* err, int32_t errLen) {
-  g->nSens = nSens;
   g->nDir = nDir;
   g->nWords = (nDir + 31) / 32;
-  DLG_CHECK(g->P.grow(size_t(nSens) * 12));
-  DLG_CHECK(g->N.grow(size_t(nSens) * 12));
   DLG_CHECK(g->D.grow(size_t(nDir) * 12));
-  if (nSens) {
+  if (nSens >= 0) {   // -1: the sensors dlg_sensors made stay
+    g->nSens = nSens;
+    DLG_CHECK(g->P.grow(size_t(nSens) * 12));
+    DLG_CHECK(g->N.grow(size_t(nSens) * 12));
+  }
+  if (nSens > 0) {
     DLG_CHECK(hipMemcpyAsync(g->P.p, P, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
     DLG_CHECK(hipMemcpyAsync(g->N.p, N, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
   }

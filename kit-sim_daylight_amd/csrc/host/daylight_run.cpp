@@ -356,9 +356,6 @@ class DaylightRun {
   }
 
   // ------------------------------------------------------------------------------------------------ sensors
-  const Sensors* sens = nullptr;
-  Sensors scratch;
-
   bool sensors() {
     // a thin solid's exit face passes everything; with --cull-backfaces a closed solid is crossed through one front
     // face anyway (the other is culled): tau once without the rule
@@ -367,8 +364,7 @@ class DaylightRun {
     const auto t0 = Clk::now();
     std::string warn;
     if (!lane) { lane = Engine::get().run_lane(err); if (!lane) return false; }
-    sens = lane_sensors(*lane, M, O, sc.c, nSens, scratch, out, warn, err, ladybug);
-    if (!sens) return false;
+    if (!lane_sensors(*lane, M, O, sc.c, nSens, out, warn, err, ladybug)) return false;
     out.stats.prepMs += ms_since(t0);
     mark("sensors");
     if (O.verbose) {
@@ -384,15 +380,12 @@ class DaylightRun {
     }
     if (g) {
       char e[512] = {};
-      if (dlg_set_scene(g, reinterpret_cast<const float*>(sens->p.data()), reinterpret_cast<const float*>(sens->n.data()), int64_t(nSens),
-                        reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e))
-        return fail(e);
+      if (dlg_set_scene(g, nullptr, nullptr, -1, reinterpret_cast<const float*>(D.data()), int64_t(nDir), e, sizeof e)) return fail(e);
       // a scene without per-triangle slots reads slot 0: one record of the scene slots' common optics
       const std::vector<SlotOptics> one{firstSlot == SIZE_MAX ? SlotOptics{} : slots[firstSlot]};
       const std::vector<SlotOptics>& rows = slotted ? slots : one;
       if ((translucent || reflect) && dlg_set_slots(g, reinterpret_cast<const dlg_slot*>(rows.data()), int64_t(rows.size()), e, sizeof e)) return fail(e);
     }
-    scratch = Sensors{};
     mark("upload");
     return true;
   }
@@ -651,31 +644,6 @@ bool npy_write_header(std::FILE* f, size_t rows, size_t cols, const char* descr)
   h += '\n';
   const uint16_t len = uint16_t(h.size());
   return std::fwrite("\x93NUMPY\x01\x00", 1, 8, f) == 8 && std::fwrite(&len, 2, 1, f) == 1 && std::fwrite(h.data(), 1, h.size(), f) == h.size();
-}
-
-bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
-                std::string& warn, std::string& err, bool ladybug) {
-  const size_t per = opt.sides == Sides::Both ? 2 : 1, n = meshes.size();
-  std::vector<size_t> at(n + 1, 0), made(n, 0);
-  for (size_t k = 0; k < n; ++k) {
-    const SimulationMesh& m = *meshes[k].mesh;
-    at[k + 1] = at[k] + (meshes[k].sensor ? (opt.onVertices ? m.numPoints() : m.numFaces()) * per : 0);
-  }
-  if (at[n] != nSens) { err = "internal: " + std::to_string(at[n]) + " sensors for " + std::to_string(nSens); return false; }
-  s.p.resize(nSens);
-  s.n.resize(nSens);
-  std::vector<std::string> notes(n);
-  par_for(n, [&](size_t k) {
-    if (meshes[k].sensor) made[k] = make_sensors_into(*meshes[k].mesh, opt, c, s.p.data() + at[k], s.n.data() + at[k], notes[k], ladybug);
-  });
-  out.ranges.clear();
-  for (size_t k = 0; k < n; ++k) {
-    if (!meshes[k].sensor) continue;
-    if (notes[k].rfind("error:", 0) == 0) { err = notes[k]; return false; }   // ladybug mode's n-gon error
-    if (!notes[k].empty()) warn += (warn.empty() ? "" : "; ") + notes[k];
-    out.ranges.emplace_back(meshes[k].name, made[k]);
-  }
-  return true;
 }
 
 bool run_series_device(const std::vector<SimulationMeshView>& meshes, const Weather* w, const Options& opt, const SeriesSink& sink, Result& out,

@@ -205,6 +205,7 @@ bool Engine::acquire_locked(const std::vector<SimulationMeshView>& meshes, const
   note("centre");
   stage = "making room on the GPU";
   dlg_trim(lane->gpu, 1);
+  lane->sensorKey.clear();
   const size_t staged = nPts * 12 + nTri * (slotted ? 16 : 12);
   const size_t build = size_t(dlg_build_bytes(int64_t(nPts), int64_t(nTri))) + (slotted ? nTri * 4 : 0);
   const size_t need = std::max(build, staged + size_t(meshPeak)) + kVramMargin;
@@ -255,7 +256,10 @@ bool Engine::admit(Lane& lane, size_t nSens, size_t perSensor, size_t chunkSenso
     const size_t perChunk = std::min(nSens, std::max<size_t>(1, chunkBytes / perSensor));
     return nSens * 24 + perChunk * chunkSensor + fixedBytes + kVramMargin;
   };
-  if (!room(need())) dlg_trim(lane.gpu, 1);   // the lane's last buffers are kept for the next run unless the room is needed
+  if (!room(need())) {   // the lane's last buffers are kept for the next run unless the room is needed
+    dlg_trim(lane.gpu, 1);
+    lane.sensorKey.clear();
+  }
   std::lock_guard<std::mutex> sk(storeMu_);
   const size_t keep = size_t(std::max(1, residentScenes)) - 1;
   size_t unpinned = 0;
@@ -281,27 +285,58 @@ void Engine::forget(const std::string& key) {
   }
 }
 
-const Sensors* lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
-                            Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug) {
+bool lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Result& out,
+                  std::string& warn, std::string& err, bool ladybug) {
   std::string key;
-  if (!opt.sceneKey.empty()) {   // the sensors depend on the meshes (named by the scene key) and these options
-    char b[96];
-    std::snprintf(b, sizeof b, "\nsensors v%d s%d o%a n%zu m%d", opt.onVertices ? 1 : 0, int(opt.sides), opt.offset, nSens, ladybug ? 1 : 0);
+  if (!opt.sceneKey.empty()) {   // the sensors depend on the meshes (named by the mesh key), these options and the centre
+    char b[192];
+    std::snprintf(b, sizeof b, "\nsensors v%d s%d o%a n%zu m%d c%a,%a,%a", opt.onVertices ? 1 : 0, int(opt.sides), opt.offset, nSens, ladybug ? 1 : 0,
+                  c[0], c[1], c[2]);
     key = opt.sceneKey + b;
   }
   if (!key.empty() && key == lane.sensorKey) {
     out.ranges = lane.ranges;
     warn = lane.sensorWarn;
-    return &lane.sensors;
+    return true;
   }
   lane.sensorKey.clear();
-  if (!sensors_of(meshes, opt, c, nSens, scratch, out, warn, err, ladybug)) return nullptr;
-  if (key.empty()) return &scratch;
-  lane.sensors = std::move(scratch);
+  const size_t per = opt.sides == Sides::Both ? 2 : 1;
+  size_t total = 0;
+  for (const SimulationMeshView& v : meshes)
+    if (v.sensor) total += (opt.onVertices ? v.mesh->numPoints() : v.mesh->numFaces()) * per;
+  if (total != nSens) { err = "internal: " + std::to_string(total) + " sensors for " + std::to_string(nSens); return false; }
+  for (const SimulationMeshView& v : meshes) {   // ladybug mode's rule (RadiationStudy: tri / quad faces only)
+    if (!v.sensor || !ladybug) continue;
+    for (size_t f = 0; f < v.mesh->counts.size(); ++f)
+      if (v.mesh->counts[f] > 4) { err = "error: ladybug mode supports tri/quad faces only (mesh " + v.mesh->name + ", face " + std::to_string(f) + ")"; return false; }
+  }
+  char e[512] = {};
+  if (dlg_sensors_reserve(lane.gpu, int64_t(nSens), e, sizeof e)) { err = e; return false; }
+  out.ranges.clear();
+  warn.clear();
+  size_t j = 0;
+  for (const SimulationMeshView& v : meshes) {
+    if (!v.sensor) continue;
+    const SimulationMesh& m = *v.mesh;
+    if (!mesh_ok(m, m.name, err)) return false;   // the device clamps indices; a mesh that needs it is refused here
+    const dlg_sensor_mesh dm{m.points.data(), int64_t(m.numPoints()), m.counts.data(), int64_t(m.counts.size()), m.indices.data(),
+                             int64_t(m.indices.size())};
+    int64_t made = 0;
+    int32_t degenerate = 0, zero = 0;
+    if (dlg_sensors(lane.gpu, &dm, int64_t(j), opt.onVertices ? 1 : 0, opt.sides == Sides::Front ? 0 : opt.sides == Sides::Back ? 1 : 2, opt.offset, c,
+                    ladybug ? 1 : 0, &made, &degenerate, &zero, e, sizeof e)) { err = e; return false; }
+    j += size_t(made) * per;
+    out.ranges.emplace_back(v.name, size_t(made));
+    std::string note;
+    if (degenerate) note = "mesh " + m.name + ": " + std::to_string(degenerate) + " degenerate face(s)";
+    if (zero) note += std::string(degenerate ? "; " : "mesh " + m.name + ": ") + std::to_string(zero) + " zero-length vertex normal(s) set to (0,0,1)";
+    if (!note.empty()) warn += (warn.empty() ? "" : "; ") + note;
+  }
+  if (key.empty()) return true;
   lane.ranges = out.ranges;
   lane.sensorWarn = warn;
   lane.sensorKey = key;
-  return &lane.sensors;
+  return true;
 }
 
 std::string scene_key_of(const std::string& meshKey, bool selfOcclude) {

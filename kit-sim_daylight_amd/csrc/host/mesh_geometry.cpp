@@ -1,12 +1,8 @@
 // SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
-// Geometry of the meshes on the host threads: the bounding-box centre every coordinate is recentred by, the triangles a
-// scene draws (a quad as (0 1 2) (0 2 3); faces of 5 or more corners ear-clipped in their Newell plane), the sensors of
-// a mesh by the mode's rules, and the thin-solid test of a transmitting slot.
+// Geometry of the meshes on the host threads: the bounding boxes every coordinate is recentred by, the triangles a scene
+// draws (a quad as (0 1 2) (0 2 3); faces of 5 or more corners ear-clipped in their Newell plane), and the thin-solid
+// test of a transmitting slot. The sensors of a mesh are made on the GPU (sensors.hip).
 //
-// The sensor rules are those of reference src/mesh.cpp:45-69 and :315-349 (physical: the unit Newell normal, zero
-// under 1e-12 of area, and the signed-area-weighted centroid of the fan from corner 0; ladybug: RadiationStudy's corner
-// mean and quad normal (0 1 2) + (2 3 1); vertices: area-weighted face normals, (0, 0, 1) when they cancel). Their FP64
-// arithmetic is evaluated in the order the oracle restates it (oracle/orc_mesh.c), so that sensors agree bit for bit.
 // The reference triangulates n-gons with earcut.hpp (v2.2.4, fetched at its build time, absent here); this build's ear
 // clipping covers the same polygon, possibly with other diagonals.
 #include <algorithm>
@@ -61,62 +57,6 @@ struct Face {
     D3 c;
     for (int i = 0; i < n; ++i) c = c + at(i);
     return c * (1.0 / n);
-  }
-};
-
-struct FaceGeom {
-  D3 normal, centre;
-  double area = 0;
-};
-
-FaceGeom physical(const Face& f) {
-  FaceGeom g;
-  if (f.n < 3) return g;
-  const D3 N = f.newell(), u = normalised(N), p0 = f.at(0);
-  g.area = norm(N) / 2;
-  double wsum = 0;
-  for (int i = 1; i + 1 < f.n; ++i) {   // the fan's triangles, each by its signed area along u
-    const D3 a = f.at(i), b = f.at(i + 1);
-    const double w = dot(cross(a - p0, b - p0), u) / 2;
-    g.centre = g.centre + (p0 + a + b) * (w / 3);
-    wsum += w;
-  }
-  g.centre = wsum > 1e-12 ? g.centre * (1 / wsum) : f.corner_mean();
-  g.normal = g.area < 1e-12 ? D3{} : normalised(N);
-  return g;
-}
-
-FaceGeom ladybug(const Face& f) {
-  FaceGeom g;
-  if (f.n < 3) return g;
-  g.centre = f.corner_mean();
-  const D3 p0 = f.at(0);
-  D3 N;
-  if (f.n == 3) {
-    N = cross(f.at(1) - p0, f.at(2) - p0);
-    g.area = norm(N) / 2;
-  } else if (f.n == 4) {
-    const D3 p1 = f.at(1), p2 = f.at(2), p3 = f.at(3);
-    const D3 n1 = cross(p1 - p0, p2 - p0), n2 = cross(p3 - p2, p1 - p2);
-    N = n1 + n2;
-    g.area = (norm(n1) + norm(n2)) / 2;
-  }
-  g.normal = g.area < 1e-12 ? D3{} : normalised(N);
-  return g;
-}
-
-// The sensors of one element into p / n from slot j on: the front (point + offset n, normal n) and / or the back
-// (point - offset n, normal -n), recentred by C
-struct SensorOut {
-  float3* p;
-  float3* n;
-  Sides sides;
-  double offset;
-  D3 C;
-  size_t put(size_t j, D3 q, D3 nr) const {
-    if (sides != Sides::Back) { p[j] = to_f3(q + nr * offset - C); n[j] = to_f3(nr); ++j; }
-    if (sides != Sides::Front) { p[j] = to_f3(q - nr * offset - C); n[j] = to_f3(nr * -1); ++j; }
-    return j;
   }
 };
 
@@ -321,54 +261,6 @@ uint32_t thin_solid(const SimulationMesh& m, uint32_t s, const double c[3]) {
   std::sort(reversed.begin(), reversed.end());
   if (edges != reversed) return 0;
   return DLG_SLOT_THIN | (thickness < 1e-5 ? DLG_SLOT_TWIN : volume < 0 ? DLG_SLOT_INWARD : 0u);
-}
-
-size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* pOut, float3* nOut, std::string& warn, bool lb) {
-  if (lb)   // RadiationStudy's rule: tri / quad faces only (nothing made)
-    for (size_t f = 0; f < m.counts.size(); ++f)
-      if (m.counts[f] > 4) {
-        warn = "error: ladybug mode supports tri/quad faces only (mesh " + m.name + ", face " + std::to_string(f) + ")";
-        return 0;
-      }
-  const SensorOut out{pOut, nOut, opt.sides, opt.offset, D3{c[0], c[1], c[2]}};
-  const size_t nF = m.counts.size(), per = opt.sides == Sides::Both ? 2 : 1;
-  const auto geom = [&](const int* idx, int n) { const Face f{m, idx, n}; return lb ? ladybug(f) : physical(f); };
-  // the corner offset of every block of faces, so that blocks of faces run on the host threads (a face's sensors sit
-  // at its index x the sides per face)
-  constexpr size_t kBlock = 4096;
-  const size_t nB = opt.onVertices ? 1 : std::max<size_t>(1, (nF + kBlock - 1) / kBlock);
-  std::vector<size_t> start(nB + 1, 0);
-  for (size_t b = 0, f = 0, off = 0; b < nB; ++b) {
-    start[b] = off;
-    for (const size_t e = opt.onVertices ? nF : std::min(nF, f + kBlock); f < e; ++f) off += size_t(m.counts[f]);
-  }
-  std::atomic<int> degenerate{0};
-  std::vector<D3> vn(opt.onVertices ? m.numPoints() : 0);   // area-weighted normals per point
-  const auto block = [&](size_t b) {
-    const size_t f0 = opt.onVertices ? 0 : b * kBlock, f1 = opt.onVertices ? nF : std::min(nF, f0 + kBlock);
-    const int* idx = m.indices.data() + start[b];
-    size_t j = f0 * per;
-    int bad = 0;
-    for (size_t f = f0; f < f1; idx += m.counts[f], ++f) {
-      const FaceGeom g = geom(idx, m.counts[f]);
-      bad += g.area < 1e-12;
-      if (opt.onVertices)
-        for (int k = 0; k < m.counts[f]; ++k) vn[size_t(idx[k])] = vn[size_t(idx[k])] + g.normal * g.area;
-      else j = out.put(j, g.centre, g.normal);
-    }
-    degenerate += bad;
-  };
-  if (opt.onVertices || nF <= 2 * kBlock) for (size_t b = 0; b < nB; ++b) block(b);
-  else par_for(nB, block);
-  int zero = 0;
-  for (size_t v = 0, j = 0; v < vn.size(); ++v) {
-    D3 n = vn[v];
-    if (norm(n) == 0) { n = {0, 0, 1}; ++zero; }
-    j = out.put(j, D3{m.points[3 * v], m.points[3 * v + 1], m.points[3 * v + 2]}, normalised(n));
-  }
-  if (degenerate) warn += "mesh " + m.name + ": " + std::to_string(degenerate.load()) + " degenerate face(s)";
-  if (zero) warn += std::string(degenerate ? "; " : "mesh " + m.name + ": ") + std::to_string(zero) + " zero-length vertex normal(s) set to (0,0,1)";
-  return opt.onVertices ? m.numPoints() : nF;
 }
 
 }  // namespace daylight

@@ -239,10 +239,8 @@ class ViewsRun {
   bool sensors() {
     stage = "making the sensors";
     const auto t0 = Clk::now();
-    Sensors scratch;
     std::string warn;
-    const Sensors* s = lane_sensors(*lane, M, O, sc.c, nSens, scratch, out, warn, err);
-    if (!s) return false;
+    if (!lane_sensors(*lane, M, O, sc.c, nSens, out, warn, err)) return false;
     out.stats.prepMs += ms_since(t0);
     if (O.verbose) {
       char vg[48] = "off";
@@ -254,7 +252,7 @@ class ViewsRun {
       if (O.transmission) std::fprintf(stderr, "info: --transmission on: rays pass %zu glass triangles\n", size_t(std::count(label.begin(), label.end(), kPass)));
     }
     char e[512] = {};
-    if (dlg_set_scene(g, reinterpret_cast<const float*>(s->p.data()), reinterpret_cast<const float*>(s->n.data()), int64_t(nSens), nullptr, 0, e, sizeof e) ||
+    if (dlg_set_scene(g, nullptr, nullptr, -1, nullptr, 0, e, sizeof e) ||
         dlg_set_labels(g, label.data(), int64_t(label.size()), e, sizeof e))
       return fail(e);
     label = {};

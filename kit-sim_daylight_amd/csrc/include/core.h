@@ -54,7 +54,6 @@ void ineichen(double apparentZenithDeg, double turbidity, double altitudeM, doub
 
 // ---------------------------------------------------------------- meshes (reference mesh.cpp)
 struct Tris { std::vector<float3> v; std::vector<uint3> i; };
-struct Sensors { std::vector<float3> p, n; };
 inline std::vector<SimulationMeshView> views_of(const std::vector<SimulationMesh>& meshes) {
   std::vector<SimulationMeshView> v;
   v.reserve(meshes.size());
@@ -94,8 +93,6 @@ bool resolve_optics(const std::vector<SimulationMeshView>& meshes, const std::ve
 bool resolve_no_occlude(const std::vector<SimulationMeshView>& meshes, const std::vector<std::string>& names, std::vector<char>& open, std::string& err);
 bool resolve_targets(const std::vector<SimulationMeshView>& meshes, const std::vector<views::Target>& targets, bool selfOcclude,
                      const std::vector<char>& open, std::vector<std::string>& groups, std::vector<uint8_t>& cls, std::string& err);
-size_t make_sensors_into(const SimulationMesh& m, const RunOptions& opt, const double c[3], float3* p, float3* n, std::string& warn,
-                         bool ladybug = false);
 
 // ---------------------------------------------------------------- scene residency and runs (reference run.cpp:38-213)
 bool npy_write_header(std::FILE* f, size_t rows, size_t cols, const char* descr);
@@ -116,8 +113,6 @@ std::string open_key(const std::vector<char>& open, bool selfOcclude);
 // context handle): without self-occlusion the sensor meshes are not in the scene, so their handles leave the key and
 // runs over different sensors share the context's scene.
 std::string scene_key_of(const std::string& meshKey, bool selfOcclude);
-bool sensors_of(const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Sensors& s, Result& out,
-                std::string& warn, std::string& err, bool ladybug = false);
 double ms_since(const void* clockStart);
 // daylight::run with the --timeseries output kept on the device: per chunk of sensors, sink(i0, nc, T, E) with E the
 // device array float [nc x T] (null when the run has no series columns); an empty sink drops the series.

@@ -28,9 +28,9 @@ namespace daylight {
 
 struct Lane {
   dlg_gpu* gpu = nullptr;
-  // the sensors of the last run whose meshes and sensor options gave `key` (empty: none kept)
+  // the sensors resident on the lane's context (dlg_sensors), made by the last run whose meshes, sensor options and
+  // centre gave `sensorKey` (empty: none kept; cleared whenever the context's sensor arrays are freed)
   std::string sensorKey;
-  Sensors sensors;
   std::vector<std::pair<std::string, size_t>> ranges;
   std::string sensorWarn;
 };
@@ -103,9 +103,10 @@ class Engine {
   std::vector<Lane*> lanes_;
 };
 
-// The sensors of a run on a lane: those kept from its last run when the meshes (the scene key names the handles) and
-// the sensor options are the same, else made (and kept when the run has a key). Valid until the lane's next run.
-const Sensors* lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens,
-                            Sensors& scratch, Result& out, std::string& warn, std::string& err, bool ladybug = false);
+// The sensors of a run on a lane, made on the GPU into its context's resident sensor arrays (dlg_sensors) unless they
+// are there from its last run (the same meshes - the mesh key names the handles - sensor options and centre); the
+// run's ranges into out, the meshes' notes (degenerate faces, zero vertex normals) into warn.
+bool lane_sensors(Lane& lane, const std::vector<SimulationMeshView>& meshes, const RunOptions& opt, const double c[3], size_t nSens, Result& out,
+                  std::string& warn, std::string& err, bool ladybug = false);
 
 }  // namespace daylight

@@ -108,6 +108,7 @@ struct dlg_gpu {
   DevBuf stackOvf;
   uint32_t traceGrid = 0;
   DevBuf tileNext;   // the persistent trace's tile counters
+  DevBuf sensWork;   // dlg_sensors: a sensor mesh on the device and its work arrays
   // the tree dlg_build makes (dlg_set_tree): DLG_TREE_SAH or DLG_TREE_LBVH
   int32_t tree = DLG_TREE_SAH;
   // mesh ingest (dlg_mesh_*): staged vertices [meshNv] float3, triangles [meshNt] uint3, their global slots, the

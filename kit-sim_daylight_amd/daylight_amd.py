@@ -45,6 +45,10 @@ class SeriesIn(C.Structure):
                 ("sunW", C.c_void_p), ("reflect", C.c_int32)]
 
 
+class SensorMesh(C.Structure):
+    _fields_ = [("xyz", C.c_void_p), ("nPts", C.c_int64), ("counts", C.c_void_p), ("nFaces", C.c_int64), ("idx", C.c_void_p), ("nIdx", C.c_int64)]
+
+
 class ChunkIn(C.Structure):
     _fields_ = [("i0", C.c_int64), ("nc", C.c_int64), ("rayFlags", C.c_uint32), ("nQ", C.c_int32), ("W", C.c_void_p),
                 ("useCos", C.c_void_p), ("series", C.c_void_p), ("method", C.c_int32)]
@@ -154,6 +158,10 @@ SEAM = [
     ("dlg_chunk_noalloc", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
     ("dlg_chunk_noalloc_limit", C.c_int64, []),
+    ("dlg_sensors_reserve", C.c_int32, [C.c_void_p, C.c_int64, C.c_char_p, C.c_int32]),
+    ("dlg_sensors", C.c_int32, [C.c_void_p, C.POINTER(SensorMesh), C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_void_p, C.c_int32,
+                                C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
+    ("dlg_download_sensors", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_int32]),
     ("dlg_series_device", C.c_void_p, [C.c_void_p]),
     ("dlg_stats", C.c_int32, [C.c_void_p, C.POINTER(Stats)]),
     ("dlg_count_fetches", None, [C.c_void_p, C.c_int32]),
@@ -323,6 +331,29 @@ class Gpu:
         cos = np.array(useCos, np.int32)
         _check(self.L.dlg_chunk_noalloc(self.h, P, N, nc, self.L.dlg_ray_flags(int(cull), int(translucent)), len(useCos), W, cos.ctypes.data,
                                         sums, int(wait), self.err, 1024), self.err)
+
+    def sensors(self, meshes, onVertices=False, sides="front", offset=0.0, centre=(0.0, 0.0, 0.0), ladybug=False):
+        """dlg_sensors of each (points, counts, indices) mesh in turn -> (P [n, 3], N [n, 3]) float32, the elements made per mesh,
+        the degenerate faces and the zero vertex normals per mesh"""
+        per = 2 if sides == "both" else 1
+        ms = [(np.ascontiguousarray(p, np.float64).reshape(-1), np.ascontiguousarray(c, np.int32), np.ascontiguousarray(i, np.int32))
+              for p, c, i in meshes]
+        n = sum((len(p) // 3 if onVertices else len(c)) * per for p, c, i in ms)
+        _check(self.L.dlg_sensors_reserve(self.h, n, self.err, 1024), self.err)
+        cen = np.ascontiguousarray(centre, np.float64)
+        j, made, deg, zero = 0, [], [], []
+        for p, c, i in ms:
+            sm = SensorMesh(p.ctypes.data, len(p) // 3, c.ctypes.data, len(c), i.ctypes.data, len(i))
+            mk, dg, zr = C.c_int64(), C.c_int32(), C.c_int32()
+            _check(self.L.dlg_sensors(self.h, C.byref(sm), j, int(onVertices), {"front": 0, "back": 1, "both": 2}[sides], float(offset),
+                                      cen.ctypes.data, int(ladybug), C.byref(mk), C.byref(dg), C.byref(zr), self.err, 1024), self.err)
+            j += mk.value * per
+            made.append(mk.value)
+            deg.append(dg.value)
+            zero.append(zr.value)
+        P, N = np.zeros((max(n, 1), 3), np.float32), np.zeros((max(n, 1), 3), np.float32)
+        _check(self.L.dlg_download_sensors(self.h, P.ctypes.data, N.ctypes.data, n, self.err, 1024), self.err)
+        return P[:n], N[:n], made, deg, zero
 
     def noalloc_limit(self):
         return self.L.dlg_chunk_noalloc_limit()
