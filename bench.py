@@ -155,11 +155,12 @@ def cpu_baseline(ctx, mesh, dni, dhi, threads, model, target_s):
         r = oracle_run([("sensors", sub, True), ("city", ctx, False)], dni, dhi, threads, quantities=("radiation",), mf=4, series=True)
         return time.perf_counter() - t0, r["durationHours"]
 
-    n0, n1 = 16, 128
+    n0, n1 = 256, 2048   # probes large enough that the per-sensor part stands out of the fixed part's noise
     t0, _ = sample(n0)
     t1, _ = sample(n1)
-    per = max((t1 - t0) / (n1 - n0), 1e-6)
-    nf = int(min(len(mesh[1]), max(n1, (target_s - max(t0 - n0 * per, 0.0)) / per)))
+    per = max((t1 - t0) / (n1 - n0), t1 / n1 / 4, 1e-6)
+    fixed = max(t0 - n0 * per, 0.0)
+    nf = int(min(len(mesh[1]), max(n1, (target_s - fixed) / per)))
     t, hours = sample(nf)
     return {"value": round(nf * hours / t, 1), "unit": "sensor-hours/s", "cores": threads, "kind": "port",
             "cpu": model, "nproc": threads,
