@@ -132,11 +132,8 @@ __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
   float mn[NC];
   for (uint32_t c = 0; c < NC; ++c) { w[c] = 0; wl[c] = 0; mn[c] = __int_as_float(0x7f800000); }
   for (uint32_t r = 0; r < 8; ++r) rc[r] = 0;
-  for (uint32_t d = lane; d < a.K; d += kWave) {
-    uint32_t c = a.cls[size_t(i) * a.K + d];
-    const float t = a.dist[size_t(i) * a.K + d];
-    const double wd = a.wRow[d / a.nAz];
-    if (c == 254) { own += wd; continue; }
+  const auto take = [&](uint32_t c, float t, double wd) {
+    if (c == 254) { own += wd; return; }
     if (c & 0x80) { gz += wd; c &= 0x7f; }
     const float dd = fminf(fmaxf(t, 0.1f), a.farF);
     const double ld = dd == a.farF ? a.lnFar : dd == 0.1f ? -2.302585092994045684 : log((double)dd);
@@ -146,6 +143,36 @@ __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
       if (q == c) { w[q] += wd; wl[q] += wd * ld; mn[q] = fminf(mn[q], dd); }
 #pragma unroll
     for (uint32_t r = 0; r < 8; ++r) if (r < a.nReach && t >= a.reach[r]) rc[r] += wd;
+  };
+  // lane l takes directions l, l + 64, ... in ascending order (the oracle's order), four at a time so that their loads
+  // are in flight together; the row of each (its weight) is followed without a division per ray
+  const uint8_t* cls = a.cls + size_t(i) * a.K;
+  const float* dist = a.dist + size_t(i) * a.K;
+  uint32_t row = a.nAz ? lane / a.nAz : 0, col = a.nAz ? lane % a.nAz : 0;
+  const auto next = [&]() {
+    col += kWave;
+    while (col >= a.nAz) { col -= a.nAz; ++row; }
+  };
+  uint32_t d = lane;
+  for (; d + 3 * kWave < a.K; d += 4 * kWave) {
+    const uint32_t c0 = cls[d], c1 = cls[d + kWave], c2 = cls[d + 2 * kWave], c3 = cls[d + 3 * kWave];
+    const float t0 = dist[d], t1 = dist[d + kWave], t2 = dist[d + 2 * kWave], t3 = dist[d + 3 * kWave];
+    const double w0 = a.wRow[row];
+    next();
+    const double w1 = a.wRow[row];
+    next();
+    const double w2 = a.wRow[row];
+    next();
+    const double w3 = a.wRow[row];
+    next();
+    take(c0, t0, w0);
+    take(c1, t1, w1);
+    take(c2, t2, w2);
+    take(c3, t3, w3);
+  }
+  for (; d < a.K; d += kWave) {
+    take(cls[d], dist[d], a.wRow[row]);
+    next();
   }
   double* o = a.out + size_t(i) * a.NA;
 #pragma unroll
