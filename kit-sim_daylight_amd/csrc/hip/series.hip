@@ -149,19 +149,35 @@ __device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k,
   return ((size_t(g) * nKS + k / 16) * 64 + r + 32 * ((k / 8) & 1)) * 8 + (k & 7);
 }
 
-// A's columns: the sky directions' coefficients, then (reflected light) the chunk's bounce rows dc[b nc + i]
+// A's columns: the sky directions' coefficients, then (reflected light) the chunk's bounce rows dc[b nc + i]. A wave
+// writes one fragment per plane: lane l the 8 values of row 32 g + l % 32 at k = 16 ks + 8 (l / 32) + 0..7, as one
+// 16-byte store (a wave's store one contiguous kilobyte); its visibility word is one load for the 8 directions.
 template <int NS>
-__global__ void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N, const float3* __restrict__ D,
-                          uint32_t nSky, uint32_t Kp, uint32_t nRG, const float* __restrict__ wsky, const float* __restrict__ dc, uint32_t nBins,
-                          uint16_t* __restrict__ A) {
-  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-  if (d >= Kp) return;
-  const float c = i >= nc ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, N[i], D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
-  uint16_t p[NS];
-  split<NS>(c, p);
-  const size_t plane = size_t(nRG) * 32 * Kp, at = frag_index(i / 32, i % 32, d, Kp / 16);
+__global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N,
+                                                 const float3* __restrict__ D, uint32_t nSky, uint32_t nKS, uint32_t nRG, const float* __restrict__ wsky,
+                                                 const float* __restrict__ dc, uint32_t nBins, u32x4* __restrict__ A) {
+  const uint32_t lane = threadIdx.x & 63, ks = blockIdx.x * 4 + (threadIdx.x >> 6), g = blockIdx.y;
+  if (ks >= nKS) return;
+  const uint32_t i = 32 * g + lane % 32, d0 = 16 * ks + 8 * (lane / 32);
+  const bool live = i < nc;
+  const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
+  uint32_t pk[NS][4];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) A[s * plane + at] = p[s];
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) pk[s][h] = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t d = d0 + j;
+    const float c = !live ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, n, D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
+    uint16_t p[NS];
+    split<NS>(c, p);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) pk[s][j / 2] |= uint32_t(p[s]) << (16 * (j & 1));
+  }
+  const size_t plane = size_t(nRG) * nKS * 64, at = (size_t(g) * nKS + ks) * 64 + lane;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) A[s * plane + at] = u32x4{pk[s][0], pk[s][1], pk[s][2], pk[s][3]};
 }
 
 // B's rows: S of each sky direction's patch, then (reflected light) the bins' radiance Sdc [b T + t]
@@ -396,9 +412,9 @@ void enqueue_mfma(dlg_gpu* g) {
   uint16_t* B = A + sh.aBytes / 2;
   const float3* N = g->N.as<float3>() + g->i0;
   const int* dRow = reinterpret_cast<const int*>(b + q.oRow);
-  k_split_a<NS><<<dim3((sh.Kp + 255) / 256, sh.nRG * 32), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(),
-                                                                               q.nSky, sh.Kp, sh.nRG, reinterpret_cast<const float*>(b + q.oW),
-                                                                               g->dc.as<float>(), q.nBins, A);
+  k_split_a<NS><<<dim3((sh.nKS + 3) / 4, sh.nRG), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(), q.nSky,
+                                                                       sh.nKS, sh.nRG, reinterpret_cast<const float*>(b + q.oW), g->dc.as<float>(), q.nBins,
+                                                                       reinterpret_cast<u32x4*>(A));
   k_split_s<NS><<<dim3((sh.Kp + 255) / 256, sh.Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, q.nSky, sh.Kp, q.T, sh.Tp, g->skySdc.as<float>(),
                                                                         q.nBins, B);
   launch_shape<NS>(g, sh, reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), q.shape);
