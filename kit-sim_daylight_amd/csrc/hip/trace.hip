@@ -87,11 +87,38 @@ constexpr int lds_stack() { return LaneStack<STACK>::kL; }
 template <int STACK>
 constexpr int ovf_stack() { return STACK - LaneStack<STACK>::kL; }
 
+// A ray's record fetches through a buffer descriptor (fewer than 2^25 records: byte offsets below 2^32). Each of a
+// visit's six plane float4 is one buffer load at the record's byte offset n * 128 (the only per-lane address) plus
+// the plane's offset for the ray's direction signs in a scalar register (soffset: axis a's near planes at float4 a or
+// a + 3, wide.h Slab), the references at the instruction's immediate 96. The offsets are the same numbers the global
+// addresses of wide.h visit() name; one descriptor and six 32-bit offsets instead of seven 64-bit bases.
+struct PlaneRsrc {
+  __amdgpu_buffer_rsrc_t r;
+  uint32_t so[6];
+};
+__device__ __forceinline__ PlaneRsrc plane_rsrc(const float4* __restrict__ wide, const dlw::Slab& sl) {
+  PlaneRsrc b;
+  // raw buffer, 32-bit data format (descriptor word 3), no range limit below 4 GiB
+  b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(wide), short(0), int(0xffffffffu), 0x00020000);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b.so[a] = uint32_t(sl.nOff[a]) * 16u;
+    b.so[3 + a] = uint32_t(2 * a + 3 - sl.nOff[a]) * 16u;
+  }
+  return b;
+}
+__device__ __forceinline__ void visit_buf(const PlaneRsrc& b, uint32_t n, const dlw::Slab& sl, float tmin, float tmax, dlw::Visit& v) {
+  const uint32_t off = n * 128u;
+  const auto f4 = [&](int k) { return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, int(off), int(b.so[k]), 0)); };
+  const uint4 rf = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(b.r, int(off + 96u), 0, 0));
+  dlw::decide<false>(f4(0), f4(1), f4(2), f4(3), f4(4), f4(5), rf, sl, tmin, tmax, v);
+}
+
 // every hit through the 4-wide records (wide.h) until hit(leaf) returns false (then false: the ray is blocked); true when
 // the walk ends. The stack holds record indices, nearest child taken first.
-// OFF32: fewer than 2^25 records, walked through uniform bases and 32-bit offsets (wide.h visit32)
+// OFF32: fewer than 2^25 records, fetched through the buffer descriptor bs (visit_buf)
 template <int STACK, bool COUNT, bool OFF32, class Hit>
-__device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw::Slab& sl, const dlw::Bases& bs, const LaneStack<STACK>& stk,
+__device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw::Slab& sl, const PlaneRsrc& bs, const LaneStack<STACK>& stk,
                                      int lane, Fetches& f, Hit&& hit) {
   uint32_t cur = 0;
   int sp = 0;
@@ -101,7 +128,7 @@ __device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw:
       if (__builtin_amdgcn_readfirstlane(lane) == lane) ++f.waveIters;   // one per iteration the wave executes
     }
     dlw::Visit v;
-    if (OFF32) dlw::visit32<false>(bs, cur, sl, 1e-5f, 3.40282347e38f, v);
+    if (OFF32) visit_buf(bs, cur, sl, 1e-5f, 3.40282347e38f, v);
     else dlw::visit<false>(wide, cur, sl, 1e-5f, 3.40282347e38f, v);
     // the lane's hit leaves, one triangle test per iteration (lanes with fewer wait less than with a block per slot)
     for (uint32_t m = v.leafMask; m; m &= m - 1) {
@@ -135,7 +162,7 @@ __device__ bool ray_open(const float4* __restrict__ wide, const float4* __restri
   dir_unpack(r, d, dirA, dirB);   // the direction's constants, made once per direction (k_dir_table)
   ray_origin(r, o, pad);
   const dlw::Slab sl = dlw::slab(r);
-  const dlw::Bases bs = dlw::bases(wide, sl);
+  const PlaneRsrc bs = plane_rsrc(wide, sl);
   OwnSet own;
   return walk<STACK, COUNT, OFF32>(wide, sl, bs, stk, lane, f, [&](uint32_t leaf) { return own.add(r, nrm, leaves, leaf, cull); });
 }
@@ -318,7 +345,7 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
             dir_unpack(r, dir, dirTab[2 * d], dirTab[2 * d + 1]);
             ray_origin(r, o, pad);
             const dlw::Slab sl = dlw::slab(r);
-            const dlw::Bases bs = dlw::bases(wide, sl);
+            const PlaneRsrc bs = plane_rsrc(wide, sl);
             ByteRay br;
             if (walk<STACK, false, OFF32>(wide, sl, bs, stk, int(lane), f,
                                           [&](uint32_t leaf) { return br.add(r, n, leaves, slots, slotLast, leaf, cull != 0); }))

@@ -102,26 +102,6 @@ DL_HD void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, fl
                *reinterpret_cast<const uint4*>(q + 6), sl, tmin, tmax, v);   // axis a's far planes: 2a + 3 - nOff[a]
 }
 
-// The same where every lane of the wave traces one direction: the seven arrays' bases are uniform (scalar registers)
-// and a record is a 32-bit byte offset n * 128 from each (fewer than 2^25 records), so each load is one instruction
-// with no address arithmetic per lane.
-struct Bases { const char* p[7]; };   // near x, y, z, far x, y, z, references
-DL_HD Bases bases(const float4* wide, const Slab& sl) {
-  Bases b;
-  for (int a = 0; a < 3; ++a) {
-    b.p[a] = reinterpret_cast<const char*>(wide + sl.nOff[a]);
-    b.p[3 + a] = reinterpret_cast<const char*>(wide + (2 * a + 3 - sl.nOff[a]));
-  }
-  b.p[6] = reinterpret_cast<const char*>(wide + 6);
-  return b;
-}
-template <bool FULL>
-DL_HD void visit32(const Bases& b, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
-  const uint32_t off = n * 128u;
-  const auto f4 = [&](int k) { return *reinterpret_cast<const float4*>(b.p[k] + off); };
-  decide<FULL>(f4(0), f4(1), f4(2), f4(3), f4(4), f4(5), *reinterpret_cast<const uint4*>(b.p[6] + off), sl, tmin, tmax, v);
-}
-
 // Stack entries a traversal of a tree of the given LBVH height can hold at once: records sit at even depths (at most
 // ceil(height / 2) on a path) and a visit pushes at most 3.
 inline int wide_stack(int height) { return 3 * ((height + 1) / 2) + 1; }
