@@ -55,11 +55,14 @@ struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
 // The traversal stack of a lane, STACK entries: [0, kL) in LDS (lds[depth][lane]), [kL, STACK) in the wave's area of a
 // global buffer (ovf[depth - kL][lane]). A lane reads back only what it wrote itself, so no wave waits on another.
 constexpr int kLdsStack = 24;
+// LDS-typed pointers: the stack's LDS accesses stay ds_read / ds_write (through a generic pointer the compiler may
+// merge an LDS and a global access of the two paths into one flat access, which takes the vector-memory path)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 template <int STACK>
 struct LaneStack {
   static constexpr int kL = STACK < kLdsStack ? STACK : kLdsStack;
   static constexpr bool kDeep = STACK > kL;
-  uint32_t* lds;   // [kL][64]
+  lds_u32* lds;    // [kL][64]
   uint32_t* ovf;   // [STACK - kL][64] of this wave (unused unless kDeep)
   __device__ __forceinline__ void put(int k, int lane, uint32_t v) const {
     if (!kDeep || k < kL) lds[k * kWave + lane] = v;
@@ -79,7 +82,10 @@ struct LaneStack {
   }
   __device__ __forceinline__ uint32_t at(int k, int lane) const {
     if (!kDeep || __ballot(k >= kL) == 0) return lds[k * kWave + lane];
-    return k < kL ? lds[k * kWave + lane] : ovf[(k - kL) * kWave + lane];
+    uint32_t v;
+    if (k < kL) v = lds[k * kWave + lane];
+    else v = ovf[(k - kL) * kWave + lane];
+    return v;
   }
 };
 template <int STACK>
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide,
                                                unsigned long long* __restrict__ counts, unsigned long long* __restrict__ traced,
                                                uint32_t* __restrict__ ovf, Tiles tl) {
   __shared__ uint32_t lds[lds_stack<STACK>() * kWave];
-  const LaneStack<STACK> stk{lds, ovf + size_t(blockIdx.x) * ovf_stack<STACK>() * kWave};
+  const LaneStack<STACK> stk{(lds_u32*)lds, ovf + size_t(blockIdx.x) * ovf_stack<STACK>() * kWave};
   const uint32_t lane = threadIdx.x;
   uint32_t nTraced = 0;
   Fetches f;
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
                                                 const float4* __restrict__ slots, uint32_t slotLast, uint32_t* __restrict__ vis,
                                                 unsigned long long* __restrict__ traced, uint32_t* __restrict__ ovf, Tiles tl) {
   __shared__ uint32_t lds[lds_stack<STACK>() * kWave];
-  const LaneStack<STACK> stk{lds, ovf + size_t(blockIdx.x) * ovf_stack<STACK>() * kWave};
+  const LaneStack<STACK> stk{(lds_u32*)lds, ovf + size_t(blockIdx.x) * ovf_stack<STACK>() * kWave};
   const uint32_t lane = threadIdx.x;
   Fetches f;
   uint32_t nTraced = 0;
