@@ -48,7 +48,7 @@ DL_HD float3 mul3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * 
 // it reflected and escaped, with its bin and weight.
 template <int STACK>
 __device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3 o0, uint32_t gi, uint32_t s, float r0, float r1,
-                                           uint32_t* __restrict__ stk, int lane, uint32_t& bin, float& val) {
+                                           dlc::lds_u32* __restrict__ stk, int lane, uint32_t& bin, float& val) {
   const bool cull = a.cull != 0;
   const float u = float(s) / float(a.M) + r0, v = u01(brev32(s)) + r1;
   float3 o = o0, d = cosine_dir(n, u - floorf(u), v - floorf(v));
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kWave, 3) void k_bounce(BounceArgs a, uint32_t c0) 
     const uint32_t s = s0 + lane;
     uint32_t bin = 0;
     float val = 0.f;
-    const bool has = s < a.M && trace_path<STACK>(a, n, o0, gi, s, r0, r1, stk, int(lane), bin, val);
+    const bool has = s < a.M && trace_path<STACK>(a, n, o0, gi, s, r0, r1, (dlc::lds_u32*)stk, int(lane), bin, val);
     // in lane order: one lane at a time adds its value (a bin's sum is the sequential one)
     for (uint64_t m = __ballot(has); m; m &= m - 1) {
       const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
       uint32_t own = 0, panes = 0;
       for (;;) {
         Closest c{};
-        if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, tmin, tmax, a.cullBack != 0, stk, int(lane));
+        if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, tmin, tmax, a.cullBack != 0, (dlc::lds_u32*)stk, int(lane));
         if (!c.hit) { cl = dir.z >= 0.f ? 0 : 1; t = tg; break; }
         t = c.t;
         const float4 A = a.leaves[3 * c.leaf], B = a.leaves[3 * c.leaf + 1], E = a.leaves[3 * c.leaf + 2];

@@ -12,6 +12,9 @@ constexpr int kWave = 64;
 
 struct Closest { float t; uint32_t leaf, prim; bool front, hit; };
 
+// the lanes' stacks in LDS, typed so: their accesses are ds_read / ds_write, never flat
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 __device__ __forceinline__ void leaf_test(const Ray& r, const float4* __restrict__ leaves, uint32_t leaf, float tmin, bool cull, Closest& c) {
   const float4 a = leaves[3 * leaf], b = leaves[3 * leaf + 1], e = leaves[3 * leaf + 2];
   float t;
@@ -25,7 +28,7 @@ __device__ __forceinline__ void leaf_test(const Ray& r, const float4* __restrict
 // closest hit through the 4-wide records (wide.h): children nearest first, those starting beyond the best hit dropped
 template <int STACK>
 __device__ Closest closest_hit(const float4* __restrict__ wide, const float4* __restrict__ leaves, const Ray& r, float tmin, float tmax, bool cull,
-                               uint32_t* __restrict__ stk, int lane) {
+                               lds_u32* __restrict__ stk, int lane) {
   Closest c{tmax, 0u, 0u, false, false};
   const dlw::Slab sl = dlw::slab(r);
   uint32_t cur = 0;

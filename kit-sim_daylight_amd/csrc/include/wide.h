@@ -41,9 +41,13 @@ DL_HD Slab slab(const Ray& r) {
   s.nOff[2] = r.inv.z >= 0.f ? 2 : 5;
   return s;
 }
+// TMAX false: the exit is not clamped to tmax (any hit, whose tmax is FLT_MAX: the clamp only matters where the exit
+// is +inf, and a box kept there is one more visit, never a lost hit)
+template <bool TMAX>
 DL_HD BoxHit slab_hit(const Slab& s, float nx, float ny, float nz, float fx, float fy, float fz, float tmin, float tmax) {
   const float t0 = fmaxf(fmaxf(fmaf(nx, s.inv.x, -s.cN.x), fmaf(ny, s.inv.y, -s.cN.y)), fmaxf(fmaf(nz, s.inv.z, -s.cN.z), tmin));
-  const float t1 = fminf(fminf(fmaf(fx, s.inv.x, -s.cF.x), fmaf(fy, s.inv.y, -s.cF.y)), fminf(fmaf(fz, s.inv.z, -s.cF.z), tmax));
+  const float f = fminf(fminf(fmaf(fx, s.inv.x, -s.cF.x), fmaf(fy, s.inv.y, -s.cF.y)), fmaf(fz, s.inv.z, -s.cF.z));
+  const float t1 = TMAX ? fminf(f, tmax) : f;
   return {t0 <= t1, t0};
 }
 
@@ -64,17 +68,18 @@ DL_HD void cx(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
   ka = k0; kb = k1; ra = r0; rb = r1;
 }
 
-// FULL: all four inner slots sorted (closest hit: the pops come nearest first, which prunes); else only the nearest is
-// moved to slot 0 (any hit: the order of the rest does not matter, and three fewer compare-exchanges per visit).
+// FULL: all four inner slots sorted (closest hit: the pops come nearest first, which prunes) and the boxes cut at tmax;
+// else only the nearest is moved to slot 0 (any hit: the order of the rest does not matter, and three fewer
+// compare-exchanges per visit) and tmax, FLT_MAX there, is not applied.
 // nx..fz: the record's near and far plane float4s (slab), rf its references.
 template <bool FULL>
 DL_HD void decide(float4 nx, float4 ny, float4 nz, float4 fx, float4 fy, float4 fz, uint4 rf, const Slab& sl, float tmin, float tmax,
                   Visit& v) {
   // an empty slot (lo +inf, hi -inf) is never hit: its entry is +inf and its exit -inf whatever the direction
-  const BoxHit h0 = slab_hit(sl, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, tmin, tmax);
-  const BoxHit h1 = slab_hit(sl, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, tmin, tmax);
-  const BoxHit h2 = slab_hit(sl, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, tmin, tmax);
-  const BoxHit h3 = slab_hit(sl, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, tmin, tmax);
+  const BoxHit h0 = slab_hit<FULL>(sl, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, tmin, tmax);
+  const BoxHit h1 = slab_hit<FULL>(sl, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, tmin, tmax);
+  const BoxHit h2 = slab_hit<FULL>(sl, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, tmin, tmax);
+  const BoxHit h3 = slab_hit<FULL>(sl, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, tmin, tmax);
   const bool l0 = (rf.x & 0x80000000u) != 0, l1 = (rf.y & 0x80000000u) != 0, l2 = (rf.z & 0x80000000u) != 0, l3 = (rf.w & 0x80000000u) != 0;
   v.leafMask = (h0.hit && l0 ? 1u : 0u) | (h1.hit && l1 ? 2u : 0u) | (h2.hit && l2 ? 4u : 0u) | (h3.hit && l3 ? 8u : 0u);
   v.leaf[0] = rf.x & 0x7fffffffu; v.leaf[1] = rf.y & 0x7fffffffu; v.leaf[2] = rf.z & 0x7fffffffu; v.leaf[3] = rf.w & 0x7fffffffu;
