@@ -21,7 +21,7 @@ BF16x3, BF16x6) with their series error; the per-column error against the oracle
 traversal's roofline (HBM bytes from the committed rocprofv3 counters over the live launch time, and its limiter) and the
 contraction's MFMA roofline (executed and algorithmic flops over the live launch time); traced and slot Grays/s; the
 urban block (577 patches), the interior views and the byte-mode legs; the reference's own --bench commands through
-this build's CLI; and the oracle's CPU time on a bounded sample of the headline workload.
+this library's CLI; and the oracle's CPU time on a bounded sample of the headline workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -376,7 +376,7 @@ REFERENCE_BENCHES = [
 
 
 def reference_benches():
-    """Each of the reference's published --bench runs with this build's CLI (a child process on the same GPU): its
+    """Each of the reference's published --bench runs with this library's CLI (a child process on the same GPU): its
     printed rays, trace time and run total. The annual Boston EPW of the reference's table is not available here; the
     Boston clear sky stands in (the same sun steps; radiation + sun hours)."""
     import re
@@ -432,7 +432,7 @@ def main():
                     help="only copy MIB MiB device to device 5 times (the HBM counters' scale) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (the path shards by sensor rows, "
+        print("bench.py: measures one GPU; --gpus %d is not supported (the path shards by sensor rows, "
               "see the sharding notes)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 

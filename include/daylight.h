@@ -1,7 +1,7 @@
 // SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // daylight: annual daylight / solar simulation on meshes, on MI355X. The public C++ API: the declarations of reference
 // src/daylight.h (the same types, members, defaults and functions, so callers of the reference compile against this
-// header unchanged); the implementation behind them is this build's (HIP kernels for gfx950, see dl_device.h).
+// header unchanged); the implementation behind them is this library's (HIP kernels for gfx950, see dl_device.h).
 #pragma once
 #include <atomic>
 #include <cmath>

@@ -45,6 +45,10 @@ void dlg_set_tree(dlg_gpu* g, int32_t tree);
 int64_t dlg_build_bytes(int64_t nv, int64_t nt);
 /* gpu_trim (internal.h:138): frees the per-run buffers (with sensors: the sensors too). */
 void dlg_trim(dlg_gpu* g, int32_t sensors);
+/* Pinned host memory for n doubles of a chunk's sums (dlg_chunk's `sums`), owned by g, kept and grown across runs,
+   freed by dlg_trim and dlg_destroy; null when pinned memory cannot be had. No counterpart in the reference (its runs
+   download into their own vectors). */
+double* dlg_host_sums(dlg_gpu* g, int64_t n);
 /* Mesh ingest on the GPU: the scene's triangles made where they are built (the reference triangulates on the host,
  * run.cpp:120-147 over mesh.cpp:137-168 triangulate_into, one mesh per host thread). dlg_mesh_begin stages room for
  * nPts vertices and up to nTriCap triangles (sum of n - 2 over the faces), with per-triangle global slots when slotted;

@@ -1,15 +1,20 @@
 // SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
-// The device work of one daylight chunk as a HIP graph: the visibility trace, the reduce of every quantity, and with
-// a series its contraction (the ground view factor taken from the reduce's last row on the device), launched on the
-// library's stream as one graph. The reference's chunk loop launches and waits kernel by kernel and sizes TDR-safe
-// sub-launches from the last one's time (reference src/run.cpp:625-688, src/gpu.cu:505-531); here a chunk is one
-// launch of each kernel over all its sensors, and the host meets the device once per chunk.
+// The device work of one daylight chunk as HIP graphs: the visibility trace, the reduce of every quantity, and with a
+// series its contraction (the ground view factor taken from the reduce's last row on the device), one graph per stage,
+// launched back to back on the library's stream with an event between stages (their times), and the host waiting once.
+// The reference's chunk loop launches and waits kernel by kernel and sizes TDR-safe sub-launches from the last one's
+// time (reference src/run.cpp:625-688, src/gpu.cu:505-531); here a chunk is one launch of each kernel over all its
+// sensors, and the host meets the device once per chunk.
 //
-// The graph is captured on the first chunk of a shape and replayed while every kernel argument stays the same: the
-// buffers (which only grow), the sizes, the flags and the chosen kernels make its key. Uploads (weights, series inputs)
-// run before the launch on the same stream; nothing in the graph sets memory from the host (no memset nodes).
+// The graphs are captured on the first chunk of a shape and replayed while every kernel argument stays the same: the
+// buffers (which only grow), the sizes, the flags and the chosen kernels make their key. Uploads (weights, series
+// inputs) run before the launch on the same stream; nothing in a graph sets memory from the host (no memset nodes).
+// The stage events are recorded on the stream between graph launches, not inside a graph (this runtime refuses event
+// record nodes under capture).
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "gpu_internal.h"
@@ -48,21 +53,20 @@ GraphKey key_of(const dlg_gpu* g, uint32_t rayFlags, int stack, int32_t nQ, uint
 
 struct dlg_graph_cache {
   GraphKey key;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipEvent_t ev[4] = {};   // recorded by the graph around its stages (external event nodes): trace, reduce, series
-  hipEvent_t launch[2] = {};   // recorded around each launch, outside the graph
-  bool staged = true;          // whether the stage events give times (else the launch's time is the trace's)
+  hipGraph_t graph[3] = {};        // the stages: trace, reduce, series
+  hipGraphExec_t exec[3] = {};
+  hipEvent_t ev[4] = {};           // recorded on the stream around the stage launches
   void drop() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    for (int k = 0; k < 3; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      exec[k] = nullptr;
+      graph[k] = nullptr;
+    }
   }
   ~dlg_graph_cache() {
     drop();
     for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : launch) if (e) (void)hipEventDestroy(e);
   }
 };
 
@@ -91,30 +95,39 @@ bool graphs_off() {
   return e && (!std::strcmp(e, "off") || !std::strcmp(e, "0"));
 }
 
-// Captures the chunk's kernels on g's stream into c (false: the capture failed; the stream is out of capture again).
-bool capture(dlg_gpu* g, dlg_graph_cache& c, const dlg_chunk_in* in, int stack, uint32_t mask, bool series) {
-  if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
-  // an event recorded under capture is only an ordering point unless it is external: then it is a node that records
-  const auto mark = [&](hipEvent_t e) { return hipEventRecordWithFlags(e, g->stream, hipEventRecordExternal) == hipSuccess; };
-  bool ok = mark(c.ev[0]);
-  dlg_trace_enqueue(g, in->rayFlags, stack, nullptr);
-  ok = mark(c.ev[1]) && ok;
-  dlg_reduce_enqueue(g, uint32_t(in->nc), in->nQ, mask);
-  ok = mark(c.ev[2]) && ok;
-  if (series) dlg_series_enqueue(g);
-  ok = mark(c.ev[3]) && ok;
-  ok = hipGetLastError() == hipSuccess && ok;
+// Captures one stage's launches (enqueue) on g's stream into graph k of c (false: the capture failed, why in `why`;
+// the stream is out of capture again).
+template <class F>
+bool capture_stage(dlg_gpu* g, dlg_graph_cache& c, int k, const char* name, F&& enqueue, std::string& why) {
+  (void)hipGetLastError();   // an error left by an earlier call is not the capture's
+  hipError_t r = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+  if (r != hipSuccess) { why = std::string(name) + ": hipStreamBeginCapture: " + hipGetErrorString(r); return false; }
+  enqueue();
+  const hipError_t launched = hipGetLastError();
   hipGraph_t graph = nullptr;
-  ok = hipStreamEndCapture(g->stream, &graph) == hipSuccess && ok && graph;
-  if (ok) ok = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0) == hipSuccess;
-  if (!ok) {
+  r = hipStreamEndCapture(g->stream, &graph);
+  if (launched != hipSuccess || r != hipSuccess || !graph) {
+    why = std::string(name) + ": " + hipGetErrorString(launched != hipSuccess ? launched : r);
     if (graph) (void)hipGraphDestroy(graph);
-    c.exec = nullptr;
     (void)hipGetLastError();
     return false;
   }
-  c.graph = graph;
+  r = hipGraphInstantiate(&c.exec[k], graph, nullptr, nullptr, 0);
+  if (r != hipSuccess) {
+    why = std::string(name) + ": hipGraphInstantiate: " + hipGetErrorString(r);
+    (void)hipGraphDestroy(graph);
+    c.exec[k] = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  c.graph[k] = graph;
   return true;
+}
+
+bool capture(dlg_gpu* g, dlg_graph_cache& c, const dlg_chunk_in* in, int stack, uint32_t mask, bool series, std::string& why) {
+  return capture_stage(g, c, 0, "trace", [&] { dlg_trace_enqueue(g, in->rayFlags, stack, nullptr); }, why) &&
+         capture_stage(g, c, 1, "reduce", [&] { dlg_reduce_enqueue(g, uint32_t(in->nc), in->nQ, mask); }, why) &&
+         (!series || capture_stage(g, c, 2, "series", [&] { dlg_series_enqueue(g); }, why));
 }
 
 }  // namespace
@@ -135,33 +148,34 @@ extern "C" int32_t dlg_chunk(dlg_gpu* g, const dlg_chunk_in* in, double* sums, f
   if (!g->graph) {
     g->graph = new dlg_graph_cache;
     for (hipEvent_t& e : g->graph->ev) DLG_CHECK(hipEventCreate(&e));
-    for (hipEvent_t& e : g->graph->launch) DLG_CHECK(hipEventCreate(&e));
   }
   dlg_graph_cache& c = *g->graph;
   const GraphKey key = key_of(g, in->rayFlags, stack, in->nQ, mask, series);
-  if (!c.exec || std::memcmp(&key, &c.key, sizeof key) != 0) {
+  if (!c.exec[0] || std::memcmp(&key, &c.key, sizeof key) != 0) {
     c.drop();
-    if (!capture(g, c, in, stack, mask, series)) {   // this context runs its chunks eagerly from now on
+    std::string why;
+    if (!capture(g, c, in, stack, mask, series, why)) {   // this context runs its chunks eagerly from now on
+      c.drop();
       g->graphBroken = 1;
+      std::fprintf(stderr, "daylight: chunk graph capture failed (%s); this device context runs its chunks stage by stage\n", why.c_str());
       return chunk_eager(g, in, series, sums, E, err, errLen);
     }
     c.key = key;
     g->stats.graphCaptures += 1;
   }
-  DLG_CHECK(hipEventRecord(c.launch[0], g->stream));
-  DLG_CHECK(hipGraphLaunch(c.exec, g->stream));
-  DLG_CHECK(hipEventRecord(c.launch[1], g->stream));
+  DLG_CHECK(hipEventRecord(c.ev[0], g->stream));
+  DLG_CHECK(hipGraphLaunch(c.exec[0], g->stream));
+  DLG_CHECK(hipEventRecord(c.ev[1], g->stream));
+  DLG_CHECK(hipGraphLaunch(c.exec[1], g->stream));
+  DLG_CHECK(hipEventRecord(c.ev[2], g->stream));
+  if (series) DLG_CHECK(hipGraphLaunch(c.exec[2], g->stream));
+  DLG_CHECK(hipEventRecord(c.ev[3], g->stream));
   g->stats.graphLaunches += 1;
   DLG_CHECK(hipMemcpyAsync(sums, g->out.p, size_t(in->nQ) * size_t(nc) * 8, hipMemcpyDeviceToHost, g->stream));
   if (series && E) DLG_CHECK(hipMemcpyAsync(E, g->seriesE.p, size_t(nc) * size_t(in->series->T) * 4, hipMemcpyDeviceToHost, g->stream));
   DLG_CHECK(hipStreamSynchronize(g->stream));
   float ms[3] = {0, 0, 0};
-  for (int k = 0; k < 3 && c.staged; ++k) c.staged = hipEventElapsedTime(&ms[k], c.ev[k], c.ev[k + 1]) == hipSuccess;
-  if (!c.staged) {   // a runtime without event nodes: the whole launch counts as the trace's
-    (void)hipGetLastError();
-    ms[1] = ms[2] = 0;
-    DLG_CHECK(hipEventElapsedTime(&ms[0], c.launch[0], c.launch[1]));
-  }
+  for (int k = 0; k < 3; ++k) DLG_CHECK(hipEventElapsedTime(&ms[k], c.ev[k], c.ev[k + 1]));
   if (dlg_trace_account(g, ms[0], err, errLen)) return 1;
   g->stats.reduceMs += ms[1];
   g->stats.lastReduceMs = ms[1];

@@ -74,6 +74,7 @@ void dlg_destroy(dlg_gpu* g) {
                     &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->sensWork, &g->slots, &g->dc, &g->skySdc,
                     &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
     b->release();
+  if (g->hostSums) (void)hipHostFree(g->hostSums);
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
@@ -90,7 +91,28 @@ int32_t dlg_device_info(char* name, int32_t nameLen, int32_t* cus, int64_t* hbmB
   return 0;
 }
 
+// Pinned host memory for n doubles of a chunk's sums, kept and grown across runs (null when it cannot be had: the
+// caller uses its own memory). The downloads into it run at the link's rate, and a run touches no fresh pages.
+double* dlg_host_sums(dlg_gpu* g, int64_t n) {
+  const size_t bytes = size_t(n > 0 ? n : 1) * sizeof(double);
+  if (bytes <= g->hostSumsBytes) return g->hostSums;
+  if (g->hostSums) (void)hipHostFree(g->hostSums);
+  g->hostSums = nullptr;
+  g->hostSumsBytes = 0;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  g->hostSums = static_cast<double*>(p);
+  g->hostSumsBytes = bytes;
+  return g->hostSums;
+}
+
 void dlg_trim(dlg_gpu* g, int32_t sensors) {
+  if (g->hostSums) (void)hipHostFree(g->hostSums);
+  g->hostSums = nullptr;
+  g->hostSumsBytes = 0;
   for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->sensWork, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
   g->meshOpen = 0;
   g->meshNv = g->meshNt = 0;

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <thread>
 
 #include "dl_math.h"
 #include "engine.h"
@@ -391,8 +392,8 @@ class DaylightRun {
   }
 
   // ------------------------------------------------------------------------------------------------ chunks (run.cpp:607-690)
+  // (also run on a helper thread while the first chunk is on the device: it touches only out.columns / values / f32)
   void columns() {
-    stage = "allocating the result";
     out.columns.clear();
     for (size_t qi = 0; qi < nQ(); ++qi)
       for (int s = 0; s < nS; ++s) out.columns.push_back(std::string(kQuantityNames[size_t(O.quantities[qi])]) + side_suffix(O.sides, s));
@@ -415,6 +416,12 @@ class DaylightRun {
 
   File seriesFile{nullptr, &std::fclose}, bitsFile{nullptr, &std::fclose};
   std::vector<double> seriesHoys;
+  std::thread alloc;   // columns() while the first chunk runs
+  bool allocFailed = false;
+  bool ready() {
+    if (alloc.joinable()) alloc.join();
+    return !allocFailed || fail("out of host memory for the result");
+  }
 
   bool open_outputs() {
     const size_t nWords = (nDir + 31) / 32;
@@ -434,8 +441,23 @@ class DaylightRun {
 
   bool chunks() {
     if (!open_outputs()) return false;
-    columns();
-    if (!g) return true;
+    if (!g) {
+      columns();
+      return true;
+    }
+    // the result's columns are allocated (their fresh pages touched) on a helper thread while the device works; every
+    // store waits for it (ready)
+    alloc = std::thread([this] {
+      try {
+        columns();
+      } catch (const std::bad_alloc&) {
+        allocFailed = true;
+      }
+    });
+    struct Join {
+      std::thread& t;
+      ~Join() { if (t.joinable()) t.join(); }
+    } join{alloc};
     stage = "tracing";
     const size_t ncMax = std::max<size_t>(1, chunkBytes / perSensor), nBins = size_t(pa.P) + 1, nWords = (nDir + 31) / 32;
     const uint32_t flags = dlg_ray_flags(O.cullBackfaces ? 1 : 0, translucent ? 1 : 0);
@@ -472,6 +494,7 @@ class DaylightRun {
       const auto tRed = Clk::now();
       sums.assign(size_t(nR) * nc, 0.0);
       if (dlg_reduce(g, int64_t(nc), nR, Wq.data(), useCos.data(), sums.data(), e, sizeof e)) return fail(e);
+      if (!ready()) return false;
       for (size_t qi = 0; qi < nQ(); ++qi) {
         const double* s = sums.data() + qi * nc;
         if (!hasDc[qi]) { for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, s[i]); continue; }
@@ -497,6 +520,7 @@ class DaylightRun {
       if (compute && dlg_series(g, int64_t(nc), &in, method, E.data(), e, sizeof e)) return fail(e);
       if (std::fwrite(E.data(), sizeof(float), E.size(), seriesFile.get()) != E.size()) return fail("write failed: " + O.seriesPath);
     }
+    if (!ready()) return false;
     mark("trace, reduce, series");
     stats();
     return true;
@@ -512,15 +536,21 @@ class DaylightRun {
     const auto t0 = Clk::now();
     dlg_stats_t a{}, b{};
     dlg_stats(g, &a);
-    sums.assign(size_t(nR) * nc, 0.0);
+    // the sums land in the context's pinned buffer (else in the run's own vector)
+    double* s = dlg_host_sums(g, int64_t(nR) * int64_t(nc));
+    if (!s) {
+      sums.resize(size_t(nR) * nc);
+      s = sums.data();
+    }
     const bool toFile = series && !sink;
     E.assign(toFile ? nc * size_t(Tup) : 0, 0.f);
     char e[512] = {};
-    if (dlg_chunk(g, &ci, sums.data(), toFile && compute ? E.data() : nullptr, e, sizeof e)) return fail(e);
+    if (dlg_chunk(g, &ci, s, toFile && compute ? E.data() : nullptr, e, sizeof e)) return fail(e);
     dlg_stats(g, &b);
     const auto tStore = Clk::now();
+    if (!ready()) return false;
     for (size_t qi = 0; qi < nQ(); ++qi)
-      for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, sums[qi * nc + i]);
+      for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, s[qi * nc + i]);
     out.stats.reduceMs += (b.reduceMs - a.reduceMs) + ms_since(tStore);
     (void)t0;
     if (O.progress) O.progress->store(double(i0 + nc) / double(nSens));

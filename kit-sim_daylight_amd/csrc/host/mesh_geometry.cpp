@@ -3,7 +3,7 @@
 // draws (a quad as (0 1 2) (0 2 3); faces of 5 or more corners ear-clipped in their Newell plane), and the thin-solid
 // test of a transmitting slot. The sensors of a mesh are made on the GPU (sensors.hip).
 //
-// The reference triangulates n-gons with earcut.hpp (v2.2.4, fetched at its build time, absent here); this build's ear
+// The reference triangulates n-gons with earcut.hpp (v2.2.4, fetched at its build time, absent here); this library's ear
 // clipping covers the same polygon, possibly with other diagonals.
 #include <algorithm>
 #include <array>

@@ -3,7 +3,7 @@
 // reference src/main.cpp's: the flags and rules of cli.h, exit codes 0 / 1 (usage, with the usage text) / 2 (runtime),
 // values on stdout or in --out FILE.txt ("%.10g", one row per element) or FILE.npy (float64 [N x C]) with FILE.json the
 // manifest, and --bench NSENS NTRIS: the reference's synthetic benchmark scenes, generated by the same steps from the
-// same seeded draws so that its published throughputs compare with this build's on the same scene.
+// same seeded draws so that its published throughputs compare with this library's on the same scene.
 #include "glibc_guard.h"
 
 #include <algorithm>
@@ -55,7 +55,7 @@ bool load_meshes(const std::string& path, bool sensor, bool verbose, std::vector
   bool ok = false;
   if (e == ".obj") ok = read_obj(path, into, err);
   else if (e == ".dmesh") ok = read_dmesh(path, into, err);
-  else if (e == ".usd" || e == ".usda" || e == ".usdc" || e == ".usdz") err = path + ": USD is not read by this build; export .obj or .dmesh";
+  else if (e == ".usd" || e == ".usda" || e == ".usdc" || e == ".usdz") err = path + ": USD is not read by this library; export .obj or .dmesh";
   else err = path + ": unknown extension; .obj or .dmesh";
   if (!ok) return false;
   size_t faces = 0, points = 0;

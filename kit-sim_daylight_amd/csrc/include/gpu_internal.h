@@ -70,6 +70,8 @@ struct dlg_gpu {
   hipStream_t stream = nullptr;
   dlg_graph_cache* graph = nullptr;
   int32_t graphBroken = 0;   // a capture failed: chunks run their stages one by one
+  double* hostSums = nullptr;   // pinned host memory for a chunk's sums (dlg_host_sums)
+  size_t hostSumsBytes = 0;
   hipEvent_t ev[2] = {};
   dlg_scene scene;
   // resident sensors and directions; per direction its ray constants (dl_math.h dir_pack: 2 float4)

@@ -124,6 +124,7 @@ SEAM = [
     ("dlg_build", C.c_int32, [C.c_void_p, F32P, C.c_int64, U32P, C.c_int64, C.c_char_p, C.c_int32]),
     ("dlg_build_bytes", C.c_int64, [C.c_int64, C.c_int64]),
     ("dlg_trim", None, [C.c_void_p, C.c_int32]),
+    ("dlg_host_sums", C.c_void_p, [C.c_void_p, C.c_int64]),
     ("dlg_detach", C.c_void_p, [C.c_void_p]),
     ("dlg_attach", None, [C.c_void_p, C.c_void_p]),
     ("dlg_free_scene", None, [C.c_void_p]),

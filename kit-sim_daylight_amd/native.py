@@ -1,14 +1,14 @@
 # SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 """The daylight and views tools in-process through libdaylight_amd.so: the interface of the reference's Python binding
 (bindings/python/daylight/native.py: SimulationMesh, Control, Result, run, usage; same names, argument meaning, errors
-and stats keys) over this build's C ABI (include/daylight_c.h)::
+and stats keys) over the library's C ABI (include/daylight_c.h)::
 
     from native import SimulationMesh, run
     with SimulationMesh.load('tests/golden/box.obj') as box:
         r = run('daylight', ['--epw', 'site.epw', '--quantity', 'radiation'], box)
     print(r.manifest['columns'], r.values.shape)
 
-run_series_device() is this build's addition: a daylight run whose --timeseries output stays on the GPU; so is Server,
+run_series_device() is an addition of this library: a daylight run whose --timeseries output stays on the GPU; so is Server,
 the serving loop (dlx_server_*): requests of both tools queued, admitted against an HBM budget, run on lanes of their
 own over resident scenes, compatible daylight requests merged into one run.
 """

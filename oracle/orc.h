@@ -15,7 +15,7 @@
  *   trace   - tests/check_gpu.cpp slab-test box scene, km-sized own planes, the 1 cm curb, the L foot; end to end the
  *             printed outputs of docs/daylight.md:133-150 and :163-180.
  *   views   - tests/check_views.cpp closed forms (kSky, far, isovist polygon/sphere) and docs/views.md:333-350.
- * Triangle intersection is this build's own watertight test (Woop, Benthin, Wald 2013), not OptiX's hardware test
+ * Triangle intersection is this library's own watertight test (Woop, Benthin, Wald 2013), not OptiX's hardware test
  * (not reproducible off an NVIDIA RT core); it is pinned by the cases above, not by OptiX bit for bit.
  */
 #ifndef ORC_H

@@ -6,7 +6,7 @@
  *  - k_series (gpu.cu:119-149): in FP32 as the reference's kernel accumulates (fma per direction, in direction order),
  *    and the FP64 contraction every GPU form is measured against.
  *  - k_view / k_iso (gpu.cu:171-274): per sensor the class weights, log distances, minima, own surface, reach, glazing
- *    and the isovist. The reference sums a warp of 32 lanes through a shuffle tree; this build's wave64 kernels sum
+ *    and the isovist. The reference sums a warp of 32 lanes through a shuffle tree; the library's wave64 kernels sum
  *    64 lanes (lane l takes directions l, l + 64, ...) through the tree s = 32, 16, ..., 1, and so does this oracle.
  *    Distances' logs are double log here (the reference's float logf: ~1e-7 relative in a log mean).
  */

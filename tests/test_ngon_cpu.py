@@ -1,4 +1,4 @@
-"""Faces of 5 or more corners (reference mesh.cpp:137-168, earcut there; this build's own ear clipper here), through the
+"""Faces of 5 or more corners (reference mesh.cpp:137-168, earcut there; this library's own ear clipper here), through the
 C ABI's dlx_triangulate (the host triangulation the scene ingest places on the GPU), against the reference's own checks
 (tests/check_mesh.cpp:103-133): a concave L hexagon after a triangle draws 4 triangles of total area 3, every one wound
 as the polygon (+Z); a collinear pentagon is skipped and counted; and what any triangulation of a simple polygon must
