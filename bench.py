@@ -595,7 +595,8 @@ def main():
                                "1,000 / 150,000 / 850,000 faces in one run, 2305 patches, 8760 h, Beta(2,3) cloudy clear-sky "
                                "weather, radiation + direct + diffuse + illuminance + the radiometric device series",
                    "sensors": n_sens, "context_triangles": 2 * len(ctx[1]), "patches": P, "hours": hours, "series_columns": Tup,
-                   "contraction": "bf16x6", "trace_launches_per_step": launches},
+                   "contraction": "bf16x6", "trace_launches_per_step": launches,
+                   "graph_launches_per_step": d["graph_launches"] / a.steps},
         "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3),
                         "traced": round(d["traced"] / (d["trace_ms"] * 1e-3) / 1e9, 3),
                         "slots_per_step": d["rays"] / a.steps, "traced_per_step": d["traced"] / a.steps,
