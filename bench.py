@@ -8,7 +8,7 @@ sensor meshes of 1,000 / 150,000 / 850,000 faces (the roofs and facades of the c
 1,001,000 sensors) in ONE run, so one launch of each kernel covers all three. Weather: the library's clear sky
 (Ineichen, turbidity 3) at lat 52.52 lon 13.41 tz 1 with an hourly cloud cover c ~ Beta(2, 3), written to an EPW. Run:
 --sky-density 4 (2305 patches), 8760 hourly steps, radiation, direct, diffuse and illuminance, plus the radiometric
-hourly series (1,001,000 x 4,456 non-night steps) by the BF16x6 contraction on MFMA, kept on the GPU
+hourly series (1,001,000 x 4,456 non-night steps) by the default FP16x3 contraction on MFMA, kept on the GPU
 (dlx_run_series_device; file output excluded from the timing). The sensors cast no shade of their own
 (--no-self-occlude): they lie on the context boxes' faces.
 
@@ -17,7 +17,7 @@ step (FP64), the visibility traversal, the per-sensor reductions and the series 
 run. The timed region holds exactly --steps runs, bracketed by torch.cuda.synchronize().
 
 The line also carries: the contraction methods side by side on the same workload (FP32 = the reference's arithmetic,
-BF16x3, BF16x6) with their series error; the per-column error against the oracle's FP64 run of a sensor sample; the
+BF16x3, BF16x6, FP16x3) with their series error; the per-column error against the oracle's FP64 run of a sensor sample; the
 traversal's roofline (HBM bytes from the committed rocprofv3 counters over the live launch time, and its limiter) and the
 contraction's MFMA roofline (executed and algorithmic flops over the live launch time); traced and slot Grays/s; the
 urban block (577 patches), the interior views and the byte-mode legs; the reference's own --bench commands through
@@ -44,7 +44,9 @@ import workloads as W  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
-NODE_BYTES, TRI_BYTES = 112, 48   # a 4-wide record visit loads 7 x 16 bytes; a triangle 3 x 16
+NODE_BYTES, TRI_BYTES = 112, 48
+PRODUCTS = {"bf16x3": 3, "bf16x6": 6, "fp16x3": 3}   # MFMA products per k-step and tile of each split method
+DEFAULT_METHOD = "fp16x3"                              # the library's default contraction (DAYLIGHT_SERIES unset)   # a 4-wide record visit loads 7 x 16 bytes; a triangle 3 x 16
 PROFILES = os.path.join(ROOT, "profiles")
 
 
@@ -228,7 +230,7 @@ def urban_block_leg(native, torch, tmp, epw, dni, dhi, threads, steps=5):
     out = {"workload": "seeded urban block: 400 boxes (4,800 triangles), roofs + facades of the 40 central buildings in <=1 m quads "
                        "(%d sensors), 577 patches, 8760 h, the cloudy year, radiation + device series" % len(sens[1]), "methods": {}}
     hours = 8760.0
-    for name in ("fp32", "bf16x3", "bf16x6"):
+    for name in ("fp32", "bf16x3", "bf16x6", "fp16x3"):
         set_method(name)
         native.run_series_device(args, sensors=S, context=Cx)
         _, wall, d, _ = timed_runs(native, torch, lambda: native.run_series_device(args, sensors=S, context=Cx), steps)
@@ -238,8 +240,9 @@ def urban_block_leg(native, torch, tmp, epw, dni, dhi, threads, steps=5):
                                 "series_max_rel_err": rel_err(E, o["series"])}
     set_method(None)
     m = out["methods"]
-    out["bf16x6_over_fp32"] = {"step": round(m["fp32"]["ms_per_step"] / m["bf16x6"]["ms_per_step"], 2),
-                               "series": round(m["fp32"]["series_ms_per_step"] / m["bf16x6"]["series_ms_per_step"], 2)}
+    for name in ("bf16x6", "fp16x3"):
+        out[name + "_over_fp32"] = {"step": round(m["fp32"]["ms_per_step"] / m[name]["ms_per_step"], 2),
+                                    "series": round(m["fp32"]["series_ms_per_step"] / m[name]["series_ms_per_step"], 2)}
     return out
 
 
@@ -479,7 +482,7 @@ def main():
     def step():
         return native.run_series_device(args, sensors=S, context=Cx)
 
-    set_method(None)                  # the default contraction (BF16x6 on MFMA)
+    set_method(None)                  # the default contraction (FP16x3 on MFMA)
     r0 = step()
     assert r0.values.shape == (n_sens, len(quantities)), r0.values.shape
     hours = float(r0.manifest["duration_h"])
@@ -501,8 +504,9 @@ def main():
     P = 2305
     alg_flops = 2.0 * n_sens * P * Tup
     kp, tp, np_ = (P + 31) // 32 * 32, (Tup + 127) // 128 * 128, (n_sens + 127) // 128 * 128
-    exec_flops = 6 * 2.0 * np_ * kp * tp                 # BF16x6: six products over the padded tiles
-    contraction = {"kernel": "k_series_mfma<3> (BF16x6, v_mfma_f32_32x32x16_bf16)", "bound": "mfma",
+    exec_flops = PRODUCTS[DEFAULT_METHOD] * 2.0 * np_ * kp * tp   # FP16x3: three products over the padded tiles
+    contraction = {"kernel": "k_series_mfma<2, WM, fp16> (FP16x3, v_mfma_f32_32x32x16_f16)", "bound": "mfma",
+                   "executed_tflops": round(exec_flops / (series_ms * 1e-3) / 1e12, 1),
                    "achieved": round(exec_flops / (series_ms * 1e-3) / 1e12, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": round(exec_flops / (series_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
                    "algorithmic_tflops": round(alg_flops / (series_ms * 1e-3) / 1e12, 1),
@@ -546,12 +550,12 @@ def main():
         o = oracle_run([(meshes[0][0], sub, True), ("city", ctx, False)], dni, dhi, threads, quantities=tuple(quantities), mf=4, series=True)
         rs, Es = series_of(native, args, [Ssub], Cx, n_acc)
         acc = {"sensors": n_acc, "columns": {q: rel_err(rs.values[:, c], o["values"][:, c]) for c, q in enumerate(quantities)},
-               "series_bf16x6": rel_err(Es, o["series"]), "bar": 2e-6}
-        acc["within_bar"] = bool(max(list(acc["columns"].values()) + [acc["series_bf16x6"]]) <= 2e-6)
+               "series_" + DEFAULT_METHOD: rel_err(Es, o["series"]), "bar": 2e-6}
+        acc["within_bar"] = bool(max(list(acc["columns"].values()) + [acc["series_" + DEFAULT_METHOD]]) <= 2e-6)
         print("bench: accuracy %s" % json.dumps(acc), file=sys.stderr, flush=True)
     # the contraction methods side by side on the headline workload
     if not a.no_methods:
-        for name in ("fp32", "bf16x3", "bf16x6"):
+        for name in ("fp32", "bf16x3", "bf16x6", "fp16x3"):
             set_method(name)
             step()
             _, wm, dm, _ = timed_runs(native, torch, step, 2)
@@ -561,15 +565,15 @@ def main():
             if name == "fp32":
                 row["fp32_peak_frac"] = round(alg_flops / (sm * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
             else:
-                ns = 2 if name == "bf16x3" else 3
-                row["mfma_executed_frac"] = round(ns * (ns + 1) // 2 * 2.0 * np_ * kp * tp / (sm * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
+                row["mfma_executed_frac"] = round(PRODUCTS[name] * 2.0 * np_ * kp * tp / (sm * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
             if acc is not None:
                 _, Em = series_of(native, args, [Ssub], Cx, n_acc)
                 row["series_max_rel_err"] = rel_err(Em, o["series"])
             methods[name] = row
         set_method(None)
-        methods["bf16x6_over_fp32"] = {"step": round(methods["fp32"]["ms_per_step"] / methods["bf16x6"]["ms_per_step"], 2),
-                                       "series": round(methods["fp32"]["series_ms_per_step"] / methods["bf16x6"]["series_ms_per_step"], 2)}
+        for name in ("bf16x6", "fp16x3"):
+            methods[name + "_over_fp32"] = {"step": round(methods["fp32"]["ms_per_step"] / methods[name]["ms_per_step"], 2),
+                                            "series": round(methods["fp32"]["series_ms_per_step"] / methods[name]["series_ms_per_step"], 2)}
         print("bench: contraction methods %s" % json.dumps(methods), file=sys.stderr, flush=True)
 
     legs = {}
@@ -590,12 +594,12 @@ def main():
         "metric": "sensor-hours/s at <=2e-6 rel vs FP64 (annual radiation, direct, diffuse, illuminance + hourly series)",
         "value": round(value, 1), "unit": "sensor-hours/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 trace, f64 sky and sums, bf16x6 mfma series (fp32 accumulation in 256-direction blocks)", "data": "synthetic",
+        "dtype": "f32 trace, f64 sky and sums, fp16x3 mfma series (fp32 accumulation in 256-direction blocks)", "data": "synthetic",
         "config": {"workload": "batched precision-form daylight: seeded 3M-triangle city (250,000 boxes), three sensor meshes of "
                                "1,000 / 150,000 / 850,000 faces in one run, 2305 patches, 8760 h, Beta(2,3) cloudy clear-sky "
                                "weather, radiation + direct + diffuse + illuminance + the radiometric device series",
                    "sensors": n_sens, "context_triangles": 2 * len(ctx[1]), "patches": P, "hours": hours, "series_columns": Tup,
-                   "contraction": "bf16x6", "trace_launches_per_step": launches,
+                   "contraction": DEFAULT_METHOD, "trace_launches_per_step": launches,
                    "graph_launches_per_step": d["graph_launches"] / a.steps},
         "grays_per_s": {"slots": round(d["rays"] / (d["trace_ms"] * 1e-3) / 1e9, 3),
                         "traced": round(d["traced"] / (d["trace_ms"] * 1e-3) / 1e9, 3),

@@ -196,8 +196,10 @@ int32_t dlg_set_sky_series(dlg_gpu* g, const float* S, const float* gS, int32_t 
 
 /* gpu_series (internal.h:155-171): E[i T + t] for the chunk's sensors, S from the last dlg_sky (P x T, T = Tup).
  * method 0: FP32 on the vector units, each sum in direction order by fma (the reference kernel's arithmetic);
- * method 1: BF16x3 on MFMA (operands split in bf16 hi + lo, three products, FP32 accumulation in blocks of 32
- * directions combined in FP64); method 2: BF16x6 (hi + mid + lo, six products), FP32-level accuracy.
+ * method 1: BF16x3 on MFMA (operands split in bf16 hi + lo, three products, FP32 accumulation in blocks of 256
+ * directions added to a running FP32 total, the epilogue in FP64); method 2: BF16x6 (hi + mid + lo, six products),
+ * FP32-level accuracy; method 3: FP16x3 (operands scaled by powers of two into fp16's range, split in fp16 hi + lo,
+ * three products on the fp16 MFMA, accumulated as method 1): method 1's cost, FP32-level accuracy.
  * E: host array [nc x T], or null to keep it on the device (dlg_series_device gives the pointer). */
 typedef struct {
   uint32_t nSky;                /* sky directions 0 .. nSky - 1 (P K) */

@@ -44,7 +44,7 @@ GraphKey key_of(const dlg_gpu* g, uint32_t rayFlags, int stack, int32_t nQ, uint
   const int64_t vals[] = {s.nTris, pad_bits(s.pad), s.nNodes, g->nDir, g->nWords, g->nc, g->i0, int64_t(rayFlags), stack, g->bytes,
                           g->nSlots, nQ, int64_t(mask), series ? 1 : 0, q.method, q.groundRow, q.reflect, q.T, int64_t(q.nc) << 32 | q.nSky,
                           int64_t(q.nBins) << 32 | q.sun0, int64_t(q.oRow ^ (q.oW << 16) ^ (q.oG << 32)), int64_t(q.oCol ^ (q.oSw << 32)), g->dcBins,
-                          q.shape, g->traceGrid};
+                          q.shape, g->traceGrid, int64_t(q.oMx)};
   std::memcpy(k.n, vals, sizeof vals);
   return k;
 }

@@ -8,6 +8,9 @@
 //   method 1, BF16x3: A and S split into bf16 hi + lo; hi.hi + hi.lo + lo.hi on MFMA (v_mfma_f32_32x32x16 bf16),
 //             FP32 accumulation in blocks of 256 directions summed into a running total; epilogue in FP64.
 //   method 2, BF16x6: hi + mid + lo; the six products down to 2^-16 relative, as method 1 otherwise: FP32-level.
+//   method 3, FP16x3: A and S scaled by powers of two into fp16's range and split into fp16 hi + lo (11 significant bits
+//             each: 22 bits kept, against 16 for two bf16 pieces); hi.hi + hi.lo + lo.hi on v_mfma_f32_32x32x16_f16,
+//             accumulated as method 1: method 1's cost at about method 2's accuracy (2^-21 per product at worst).
 // Operands: fragment-major bf16 planes (Kp = nSky rounded up to 32, zero padded), described at k_split_a. Reflected light
 // (dlg_bounce rows dc and the sky's bin radiance Sdc) extends K: E += dc Sdc in the same products (methods 1 / 2) or a
 // second FP32 pass after the reference's (method 0).
@@ -22,6 +25,7 @@
 namespace {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes of operand (a native vector: kept in registers)
 
@@ -133,12 +137,66 @@ __global__ __launch_bounds__(256) void k_series_dc_f32(const float* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------------- methods 1 / 2: MFMA
-// split x into NS bf16 pieces, each the rounding of what the previous ones leave
-template <int NS>
-__device__ __forceinline__ void split(float x, uint16_t* out) {
-  float r = x;
+// split x into NS pieces, each the rounding of what the previous ones leave (r - piece is exact in float): bf16, or
+// (H) fp16 of x 2^e, e the operand's scale exponent (scale_exp)
+template <int NS, bool H>
+__device__ __forceinline__ void split(float x, int e, uint16_t* out) {
+  if (H) {
+    float r = __builtin_ldexpf(x, e);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) { out[s] = bf16_rn(r); r = r - bf16_f(out[s]); }
+    for (int s = 0; s < NS; ++s) {
+      const _Float16 h = (_Float16)r;
+      out[s] = __builtin_bit_cast(uint16_t, h);
+      r = r - (float)h;
+    }
+  } else {
+    float r = x;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) { out[s] = bf16_rn(r); r = r - bf16_f(out[s]); }
+  }
+}
+
+// The fp16 operands' scales (method 3). fp16 is normal from 2^-14 to 65504: each operand is multiplied by 2^e so that
+// its largest magnitude m lies below 2^14 (e = 14 - k for m < 2^k), which keeps every value within 2^28 of m at full
+// precision; what lies below that adds at most 2^-25 absolute per piece (2^-39 of m). Powers of two scale exactly, and
+// the epilogue takes 2^-(eA + eS) back off. mx[0]: the largest |A| bound (the sky weights, as A = cos T wsky with cos
+// and T at most 1, and the bounce rows), mx[1]: the largest |S| (sky rows and bins), both as float bits (for
+// non-negative floats the integer order is the float order), cleared by k_scale_zero and raised by k_scale_max.
+__device__ __forceinline__ int scale_exp(uint32_t bits) {
+  if (bits == 0u) return 0;
+  const int k = int(bits >> 23) - 126;   // m < 2^k
+  const int e = 14 - k;
+  return e < -120 ? -120 : e > 120 ? 120 : e;
+}
+__global__ void k_scale_zero(uint32_t* mx) {
+  if (threadIdx.x < 2) mx[threadIdx.x] = 0u;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    const uint32_t o = uint32_t(__shfl_xor(int(v), m));
+    v = o > v ? o : v;
+  }
+  return v;
+}
+// grid-stride over the A bound's inputs (wsky [nW], dc [nDc]) and S's (S [nS], Sdc [nSdc]); one atomic per wave and operand
+__global__ __launch_bounds__(256) void k_scale_max(const float* __restrict__ wsky, uint32_t nW, const float* __restrict__ dc, size_t nDc,
+                                                   const float* __restrict__ S, size_t nS, const float* __restrict__ Sdc, size_t nSdc,
+                                                   uint32_t* __restrict__ mx) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x, t0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t a = 0u, b = 0u;
+  const auto mag = [](float f) { return __float_as_uint(f) & 0x7fffffffu; };
+  const auto up = [](uint32_t& m, uint32_t v) { m = v > m ? v : m; };
+  for (size_t k = t0; k < nW; k += stride) up(a, mag(wsky[k]));
+  for (size_t k = t0; k < nDc; k += stride) up(a, mag(dc[k]));
+  for (size_t k = t0; k < nS; k += stride) up(b, mag(S[k]));
+  for (size_t k = t0; k < nSdc; k += stride) up(b, mag(Sdc[k]));
+  a = wave_max(a);
+  b = wave_max(b);
+  if ((threadIdx.x & 63) == 0) {
+    if (a) atomicMax(&mx[0], a);
+    if (b) atomicMax(&mx[1], b);
+  }
 }
 
 // Operand layouts: fragment-major, so that an MFMA operand is one 16-byte load per lane and a wave's load one
@@ -152,12 +210,13 @@ __device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k,
 // A's columns: the sky directions' coefficients, then (reflected light) the chunk's bounce rows dc[b nc + i]. A wave
 // writes one fragment per plane: lane l the 8 values of row 32 g + l % 32 at k = 16 ks + 8 (l / 32) + 0..7, as one
 // 16-byte store (a wave's store one contiguous kilobyte); its visibility word is one load for the 8 directions.
-template <int NS>
+template <int NS, bool H>
 __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N,
                                                  const float3* __restrict__ D, uint32_t nSky, uint32_t nKS, uint32_t nRG, const float* __restrict__ wsky,
-                                                 const float* __restrict__ dc, uint32_t nBins, u32x4* __restrict__ A) {
+                                                 const float* __restrict__ dc, uint32_t nBins, const uint32_t* __restrict__ mx, u32x4* __restrict__ A) {
   const uint32_t lane = threadIdx.x & 63, ks = blockIdx.x * 4 + (threadIdx.x >> 6), g = blockIdx.y;
   if (ks >= nKS) return;
+  const int e = H ? scale_exp(mx[0]) : 0;
   const uint32_t i = 32 * g + lane % 32, d0 = 16 * ks + 8 * (lane / 32);
   const bool live = i < nc;
   const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
@@ -171,7 +230,7 @@ __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bi
     const uint32_t d = d0 + j;
     const float c = !live ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, n, D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
     uint16_t p[NS];
-    split<NS>(c, p);
+    split<NS, H>(c, e, p);
 #pragma unroll
     for (int s = 0; s < NS; ++s) pk[s][j / 2] |= uint32_t(p[s]) << (16 * (j & 1));
   }
@@ -181,14 +240,14 @@ __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bi
 }
 
 // B's rows: S of each sky direction's patch, then (reflected light) the bins' radiance Sdc [b T + t]
-template <int NS>
+template <int NS, bool H>
 __global__ void k_split_s(const float* __restrict__ S, const int* __restrict__ row, uint32_t nSky, uint32_t Kp, int T, uint32_t Tp,
-                          const float* __restrict__ Sdc, uint32_t nBins, uint16_t* __restrict__ B) {
+                          const float* __restrict__ Sdc, uint32_t nBins, const uint32_t* __restrict__ mx, uint16_t* __restrict__ B) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
   if (d >= Kp || t >= Tp) return;
   const float s = int(t) >= T ? 0.f : d < nSky ? S[size_t(row[d]) * T + t] : d - nSky < nBins ? Sdc[size_t(d - nSky) * T + t] : 0.f;
   uint16_t p[NS];
-  split<NS>(s, p);
+  split<NS, H>(s, H ? scale_exp(mx[1]) : 0, p);
   const size_t plane = size_t(Tp) * Kp, at = frag_index(t / 32, t % 32, d, Kp / 16);
 #pragma unroll
   for (int k = 0; k < NS; ++k) B[k * plane + at] = p[k];
@@ -214,12 +273,20 @@ constexpr uint32_t kTileN = 128, kSliceK = 32, kMaxTileM = 256;
 // 3.9e-7, BF16x3 2.5e-6 -> 1.3e-6 relative to FP64. The totals take the registers of a third wave per SIMD (2 waves, 252
 // VGPRs, no spills; 2 % slower than the single chain at 3 waves; 3 waves with the totals spilled: 30 % slower).
 constexpr uint32_t kBlockSlices = 8;
-template <int NS, int WM>
+// one product of 16-byte operand fragments: bf16 pieces, or (H) fp16 ones
+template <bool H>
+__device__ __forceinline__ f32x16 mma(u32x4 a, u32x4 b, f32x16 c) {
+  if constexpr (H)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+template <int NS, int WM, bool H>
 __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) void k_series_mfma(
     const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG, uint32_t nCG, uint32_t nKS, int T,
     const uint32_t* __restrict__ bits, uint32_t bytes, const float3* __restrict__ N, const float3* __restrict__ D, const float* __restrict__ G,
-    const float* __restrict__ g, uint32_t sun0, const int* __restrict__ sunOfCol, const float* __restrict__ sunW, float* __restrict__ E,
-    uint32_t tilesT, uint32_t tiles, uint32_t group) {
+    const float* __restrict__ g, uint32_t sun0, const int* __restrict__ sunOfCol, const float* __restrict__ sunW, const uint32_t* __restrict__ mx,
+    float* __restrict__ E, uint32_t tilesT, uint32_t tiles, uint32_t group) {
   constexpr int kThreads = 128 * WM, kRows = 64 * WM, kGA = 2 * WM, kGB = 4;
   constexpr int kFragA = NS * kGA * 128, kFragB = NS * kGB * 128;   // uint4 per slice: planes x groups x 2 k-steps x 64 lanes
   constexpr int kPerA = kFragA / kThreads, kPerB = kFragB / kThreads;
@@ -279,14 +346,13 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
-      bf16x8 a[2][NS], b[2][NS];
+      u32x4 a[2][NS], b[2][NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-          const u32x4 va = sA[((s * kGA + 2 * wm + f) * 2 + kk) * 64 + lane], vb = sB[((s * kGB + 2 * wn + f) * 2 + kk) * 64 + lane];
-          a[f][s] = __builtin_bit_cast(bf16x8, va);
-          b[f][s] = __builtin_bit_cast(bf16x8, vb);
+          a[f][s] = sA[((s * kGA + 2 * wm + f) * 2 + kk) * 64 + lane];
+          b[f][s] = sB[((s * kGB + 2 * wn + f) * 2 + kk) * 64 + lane];
         }
       // products whose exponent sum is at most NS - 1 pieces below the leading one, smallest first
 #pragma unroll
@@ -296,7 +362,7 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
           for (int o = NS - 1; o >= 0; --o)
 #pragma unroll
-            for (int s = 0; s <= o; ++s) acc[fm][fn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[fm][s], b[fn][o - s], acc[fm][fn], 0, 0, 0);
+            for (int s = 0; s <= o; ++s) acc[fm][fn] = mma<H>(a[fm][s], b[fn][o - s], acc[fm][fn]);
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
@@ -309,7 +375,8 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
   }
   // Epilogue: the tile's rows (N, G), columns (g, the step's sun: D, weight) and the sun visibility of every (row,
   // column) staged in the operand buffers (free after the loop's last barrier), so that the 64 elements of a lane read
-  // LDS instead of gathering from HBM; E = acc + G g + the sun's term, in FP64.
+  // LDS instead of gathering from HBM; E = acc (H: times 2^-(eA + eS), exact) + G g + the sun's term, in FP64.
+  const double unscale = H ? __builtin_ldexp(1.0, -(scale_exp(mx[0]) + scale_exp(mx[1]))) : 1.0;
   float* sRow = reinterpret_cast<float*>(sB);       // [kRows][4]: N x y z, G
   float* sCol = sRow + kRows * 4;                    // [128][6]: g, D x y z, sun weight, sun index (bits; -1: none)
   uint8_t* sVis = reinterpret_cast<uint8_t*>(sA);   // [kRows][128 columns]: the sun's transmittance byte
@@ -347,7 +414,7 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
         const uint32_t r = 32 * (2 * wm + fm) + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4), i = i0 + r;
         if (i >= nc) continue;
         const float* rr = sRow + 4 * r;
-        double e = (double)tot[fm][fn][v] + (double)rr[3] * gt;
+        double e = (double)tot[fm][fn][v] * unscale + (double)rr[3] * gt;
         const uint32_t sv = sVis[r * 128 + c];
         if (sv) e += (double)(fmaxf(dlm::dot3(make_float3(rr[0], rr[1], rr[2]), dj), 0.f) * vis_T(sv)) * sw;
         E[size_t(i) * T + t] = (float)e;
@@ -385,25 +452,28 @@ struct TileShape { int wm; uint32_t group; };
 constexpr TileShape kShapes[] = {{2, 4}, {2, 8}, {2, 16}, {4, 2}, {4, 4}, {4, 8}};
 constexpr int kNumShapes = int(sizeof kShapes / sizeof kShapes[0]);
 
-template <int NS, int WM>
+// the pieces each operand is split into by a method (1: BF16x3, 2: BF16x6, 3: FP16x3)
+constexpr int pieces(int method) { return method == 2 ? 3 : 2; }
+
+template <int NS, int WM, bool H>
 void launch_mfma(dlg_gpu* g, const MfmaShape& sh, const u32x4* A, const u32x4* B, uint32_t group) {
   const dlg_series_plan& q = g->seriesPlan;
   const char* b = g->seriesIn.as<char>();
   const uint32_t tilesI = (q.nc + 64 * WM - 1) / (64 * WM), tiles = sh.tilesT * tilesI;
-  k_series_mfma<NS, WM><<<(tiles + 7) / 8 * 8, 128 * WM, 0, g->stream>>>(
+  k_series_mfma<NS, WM, H><<<(tiles + 7) / 8 * 8, 128 * WM, 0, g->stream>>>(
       A, B, q.nc, sh.nRG, sh.nCG, sh.nKS, q.T, g->bits.as<uint32_t>(), uint32_t(g->bytes), g->N.as<float3>() + g->i0, g->D.as<float3>(),
       reinterpret_cast<const float*>(b + q.oG), g->skyG.as<float>(), q.sun0, reinterpret_cast<const int*>(b + q.oCol),
-      reinterpret_cast<const float*>(b + q.oSw), g->seriesE.as<float>(), sh.tilesT, tiles, group);
+      reinterpret_cast<const float*>(b + q.oSw), reinterpret_cast<const uint32_t*>(b + q.oMx), g->seriesE.as<float>(), sh.tilesT, tiles, group);
 }
 
-template <int NS>
+template <int NS, bool H>
 void launch_shape(dlg_gpu* g, const MfmaShape& sh, const u32x4* A, const u32x4* B, int shape) {
   const TileShape t = kShapes[shape];
-  if (t.wm == 4) launch_mfma<NS, 4>(g, sh, A, B, t.group);
-  else launch_mfma<NS, 2>(g, sh, A, B, t.group);
+  if (t.wm == 4) launch_mfma<NS, 4, H>(g, sh, A, B, t.group);
+  else launch_mfma<NS, 2, H>(g, sh, A, B, t.group);
 }
 
-template <int NS>
+template <int NS, bool H>
 void enqueue_mfma(dlg_gpu* g) {
   const dlg_series_plan& q = g->seriesPlan;
   const char* b = g->seriesIn.as<char>();
@@ -412,28 +482,35 @@ void enqueue_mfma(dlg_gpu* g) {
   uint16_t* B = A + sh.aBytes / 2;
   const float3* N = g->N.as<float3>() + g->i0;
   const int* dRow = reinterpret_cast<const int*>(b + q.oRow);
-  k_split_a<NS><<<dim3((sh.nKS + 3) / 4, sh.nRG), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(), q.nSky,
-                                                                       sh.nKS, sh.nRG, reinterpret_cast<const float*>(b + q.oW), g->dc.as<float>(), q.nBins,
-                                                                       reinterpret_cast<u32x4*>(A));
-  k_split_s<NS><<<dim3((sh.Kp + 255) / 256, sh.Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, q.nSky, sh.Kp, q.T, sh.Tp, g->skySdc.as<float>(),
-                                                                        q.nBins, B);
-  launch_shape<NS>(g, sh, reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), q.shape);
+  uint32_t* mx = reinterpret_cast<uint32_t*>(g->seriesIn.as<char>() + q.oMx);
+  if (H) {   // the operands' largest magnitudes, for their scales (kernels only: the enqueue may be captured)
+    const size_t nDc = size_t(q.nBins) * q.nc, nS = size_t(g->skyP) * size_t(q.T), nSdc = size_t(q.nBins) * size_t(q.T);
+    k_scale_zero<<<1, 64, 0, g->stream>>>(mx);
+    k_scale_max<<<1024, 256, 0, g->stream>>>(reinterpret_cast<const float*>(b + q.oW), q.nSky, g->dc.as<float>(), nDc, g->skyS.as<float>(), nS,
+                                             g->skySdc.as<float>(), nSdc, mx);
+  }
+  k_split_a<NS, H><<<dim3((sh.nKS + 3) / 4, sh.nRG), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(),
+                                                                          q.nSky, sh.nKS, sh.nRG, reinterpret_cast<const float*>(b + q.oW),
+                                                                          g->dc.as<float>(), q.nBins, mx, reinterpret_cast<u32x4*>(A));
+  k_split_s<NS, H><<<dim3((sh.Kp + 255) / 256, sh.Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, q.nSky, sh.Kp, q.T, sh.Tp,
+                                                                           g->skySdc.as<float>(), q.nBins, mx, B);
+  launch_shape<NS, H>(g, sh, reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), q.shape);
 }
 
 // ------------------------------------------------------------------------------------------------ the tuner
-// The shape chosen per problem (products, sensors to the next power of two, padded directions, steps), for the process.
+// The shape chosen per problem (method, sensors to the next power of two, padded directions, steps), for the process.
 struct TuneKey {
-  int ns;
+  int method;
   uint32_t ncPow2, Kp, T;
-  bool operator==(const TuneKey& o) const { return ns == o.ns && ncPow2 == o.ncPow2 && Kp == o.Kp && T == o.T; }
+  bool operator==(const TuneKey& o) const { return method == o.method && ncPow2 == o.ncPow2 && Kp == o.Kp && T == o.T; }
 };
 std::mutex tuneMu;
 std::vector<std::pair<TuneKey, int>> tuned;
 
-TuneKey tune_key(const dlg_series_plan& q, int ns) {
+TuneKey tune_key(const dlg_series_plan& q) {
   uint32_t p = 1;
   while (p < q.nc) p <<= 1;
-  return {ns, p, MfmaShape(q.nc, q.nSky + q.nBins, uint32_t(q.T), ns).Kp, uint32_t(q.T)};
+  return {q.method, p, MfmaShape(q.nc, q.nSky + q.nBins, uint32_t(q.T), pieces(q.method)).Kp, uint32_t(q.T)};
 }
 
 // the forced shape of DAYLIGHT_SERIES_TILE ("WMxGROUP"), or -1
@@ -449,9 +526,9 @@ int forced_shape() {
 }
 
 // the tuned shape of the plan's problem, or -1 when it has not been timed
-int known_shape(const dlg_series_plan& q, int ns) {
+int known_shape(const dlg_series_plan& q) {
   if (const int f = forced_shape(); f >= 0) return f;
-  const TuneKey k = tune_key(q, ns);
+  const TuneKey k = tune_key(q);
   std::lock_guard<std::mutex> lk(tuneMu);
   for (const auto& [key, shape] : tuned)
     if (key == k) return shape;
@@ -459,7 +536,7 @@ int known_shape(const dlg_series_plan& q, int ns) {
 }
 
 // Times every shape on the chunk whose operands are split (each launch writes the same E) and records the fastest.
-template <int NS>
+template <int NS, bool H>
 int tune(dlg_gpu* g) {
   const dlg_series_plan& q = g->seriesPlan;
   const MfmaShape sh(q.nc, q.nSky + q.nBins, uint32_t(q.T), NS);
@@ -472,7 +549,7 @@ int tune(dlg_gpu* g) {
   if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 0; }
   for (int k = 0; k < kNumShapes; ++k) {
     (void)hipEventRecord(e0, g->stream);
-    launch_shape<NS>(g, sh, A, B, k);
+    launch_shape<NS, H>(g, sh, A, B, k);
     (void)hipEventRecord(e1, g->stream);
     float ms = 0;
     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
@@ -481,7 +558,7 @@ int tune(dlg_gpu* g) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lk(tuneMu);
-  tuned.push_back({tune_key(q, NS), best});
+  tuned.push_back({tune_key(q), best});
   return best;
 }
 
@@ -494,7 +571,7 @@ int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int3
     dlg_fail(err, errLen, "dlg_series: reflected light needs the chunk's dlg_bounce rows and a sky with reflect on");
     return 1;
   }
-  if (method < 0 || method > 2) { dlg_fail(err, errLen, "dlg_series: method 0 (FP32), 1 (BF16x3) or 2 (BF16x6)"); return 1; }
+  if (method < 0 || method > 3) { dlg_fail(err, errLen, "dlg_series: method 0 (FP32), 1 (BF16x3), 2 (BF16x6) or 3 (FP16x3)"); return 1; }
   const int T = in->T;
   dlg_series_plan& q = g->seriesPlan;
   q = dlg_series_plan{};
@@ -514,6 +591,7 @@ int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int3
   q.oG = take(4 * size_t(nc));
   q.oCol = take(4 * size_t(T));
   q.oSw = take(4 * size_t(in->nSun ? in->nSun : 1));
+  q.oMx = take(8);   // the fp16 operands' largest magnitudes (method 3)
   DLG_CHECK(g->seriesIn.grow(off));
   char* b = g->seriesIn.as<char>();
   std::vector<int> sunOfCol(static_cast<size_t>(T), -1);
@@ -527,9 +605,9 @@ int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int3
   if (in->nSun) DLG_CHECK(hipMemcpyAsync(b + q.oSw, in->sunW, 4 * size_t(in->nSun), hipMemcpyHostToDevice, g->stream));
   DLG_CHECK(g->seriesE.grow(size_t(nc) * size_t(T) * 4));
   if (method) {
-    const MfmaShape sh(uint32_t(nc), in->nSky + q.nBins, uint32_t(T), method == 1 ? 2 : 3);
+    const MfmaShape sh(uint32_t(nc), in->nSky + q.nBins, uint32_t(T), pieces(method));
     DLG_CHECK(g->seriesWork.grow(sh.aBytes + sh.bBytes));
-    q.shape = known_shape(q, method == 1 ? 2 : 3);
+    q.shape = known_shape(q);
   }
   g->seriesN = nc;
   return 0;
@@ -540,8 +618,9 @@ void dlg_series_enqueue(dlg_gpu* g) {
   const char* b = g->seriesIn.as<char>();
   float* G = reinterpret_cast<float*>(g->seriesIn.as<char>() + q.oG);
   if (q.groundRow >= 0) k_ground_rows<<<(q.nc + 255) / 256, 256, 0, g->stream>>>(g->out.as<double>() + size_t(q.groundRow) * q.nc, q.nc, G);
-  if (q.method == 1) { enqueue_mfma<2>(g); return; }
-  if (q.method == 2) { enqueue_mfma<3>(g); return; }
+  if (q.method == 1) { enqueue_mfma<2, false>(g); return; }
+  if (q.method == 2) { enqueue_mfma<3, false>(g); return; }
+  if (q.method == 3) { enqueue_mfma<2, true>(g); return; }
   const float3* N = g->N.as<float3>() + g->i0;
   const int T = q.T;
   k_series_f32<<<dim3((T + kTile - 1) / kTile, (q.nc + kTile - 1) / kTile), 256, 0, g->stream>>>(
@@ -562,7 +641,7 @@ extern "C" int32_t dlg_series(dlg_gpu* g, int64_t nc, const dlg_series_in* in, i
   if (method && q.shape < 0) {   // the first chunk of this problem: split its operands and time every tile shape on it
     q.shape = 0;
     dlg_series_enqueue(g);
-    q.shape = method == 1 ? tune<2>(g) : tune<3>(g);
+    q.shape = method == 1 ? tune<2, false>(g) : method == 2 ? tune<3, false>(g) : tune<2, true>(g);
   }
   dlg_tbegin(g);
   dlg_series_enqueue(g);

@@ -33,10 +33,12 @@ File open_file(const std::string& path, const char* mode) { return File(std::fop
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kNight = 1e-4;   // dni + dhi at or below: a night step (no series column)
 
-// The contraction's arithmetic for --timeseries, DAYLIGHT_SERIES = fp32 | bf16x3 | bf16x6 (default bf16x6, FP32-level)
+// The contraction's arithmetic for --timeseries, DAYLIGHT_SERIES = fp32 | bf16x3 | bf16x6 | fp16x3 (default fp16x3:
+// three MFMA products at FP32-level accuracy; an unknown name takes the default)
 int series_method() {
   const char* e = std::getenv("DAYLIGHT_SERIES");
-  return !e ? 2 : !std::strcmp(e, "fp32") ? 0 : !std::strcmp(e, "bf16x3") ? 1 : 2;
+  if (!e) return 3;
+  return !std::strcmp(e, "fp32") ? 0 : !std::strcmp(e, "bf16x3") ? 1 : !std::strcmp(e, "bf16x6") ? 2 : 3;
 }
 
 std::string side_suffix(Sides sides, int s) { return (sides == Sides::Back || (sides == Sides::Both && s == 1)) ? "_back" : "_front"; }

@@ -61,7 +61,7 @@ struct dlg_series_plan {
   int32_t method = 0, groundRow = -1, reflect = 0, T = 0;
   int32_t shape = -1;   // the MFMA methods' tile shape (series.hip kShapes), -1 until tuned
   uint32_t nc = 0, nSky = 0, nBins = 0, sun0 = 0;
-  size_t oRow = 0, oW = 0, oG = 0, oCol = 0, oSw = 0;
+  size_t oRow = 0, oW = 0, oG = 0, oCol = 0, oSw = 0, oMx = 0;
 };
 
 struct dlg_graph_cache;   // chunk.hip: the chunk's captured graph and its events

@@ -5,7 +5,7 @@
   run in many chunks (one capture for the full chunks, one for the tail) against the one-chunk run.
 - Meshes of uneven size in one run give the rows each gives alone.
 - Every tile shape of the MFMA contraction (64 WM sensors x 128 steps per block, tile-group order) gives the same bits,
-  at both split methods: each element's products are summed in the same order whatever the shape; the tuned run too.
+  at every split method: each element's products are summed in the same order whatever the shape; the tuned run too.
 """
 import os
 
@@ -86,7 +86,7 @@ def test_uneven_meshes_in_one_run(native, block):
 SHAPES = ["2x4", "2x8", "2x16", "4x2", "4x4", "4x8"]
 
 
-@pytest.mark.parametrize("method", ["bf16x3", "bf16x6"])
+@pytest.mark.parametrize("method", ["bf16x3", "bf16x6", "fp16x3"])
 def test_tile_shapes_bit_identical(native, block, monkeypatch, method):
     epw, meshes, ctx, tmp = block
     args = ["--epw", epw, "--sky-density", "1", "--quantity", "radiation", "--no-self-occlude"]

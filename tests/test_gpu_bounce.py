@@ -210,7 +210,7 @@ def test_reduce_series_dc(gpu, orc):
     mism = np.argwhere(E32 != E32o)
     assert len(mism) == 0, (len(mism), mism[:5])
     scale = np.maximum(np.abs(E64), 1e-3 * np.abs(E64).max())
-    for method, bar in ((1, 2e-5), (2, 2e-6)):
+    for method, bar in ((1, 2e-5), (2, 2e-6), (3, 2e-6)):
         Em = gpu.series(nc, Pp, row, wsky, Pp, Tup, G, 2 * Pp, sunStep, sunW, method, reflect=True)
         err = np.max(np.abs(Em - E64) / scale)
         print("series with reflected light, method %d: max rel error vs FP64 %.3e" % (method, err))

@@ -322,7 +322,7 @@ def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc, mf):
     E64 = np.zeros((nc, Tup))
     orc.lib().orc_series64(*args, E64.reshape(-1), 8)
     scale = np.maximum(np.abs(E64), 1e-3 * np.abs(E64).max())
-    for method, bar in ((1, 2e-5), (2, 2e-6)):
+    for method, bar in ((1, 2e-5), (2, 2e-6), (3, 2e-6)):
         Em = gpu.series(nc, Pp, row, wsky, Pp, Tup, G, 2 * Pp, sunStep, sunW, method)
         err = np.max(np.abs(Em - E64) / scale)
         print("series method %d: max rel error vs FP64 %.3e" % (method, err))

@@ -97,7 +97,7 @@ def test_options_vs_oracle(native, box, orc, box_meshes, name, flags, kw):
                 assert rel_err(r.values[:, c], o[:, c]) <= 2e-6, (q, s, rel_err(r.values[:, c], o[:, c]))
 
 
-SERIES_BARS = {"fp32": 2e-6, "bf16x3": 2e-5, "bf16x6": 2e-6}
+SERIES_BARS = {"fp32": 2e-6, "bf16x3": 2e-5, "bf16x6": 2e-6, "fp16x3": 2e-6}
 
 
 @pytest.mark.parametrize("method", sorted(SERIES_BARS))
