@@ -4,11 +4,14 @@
 //   k_view_rays  a wave per sensor and 8 x 8 tile of cells (grid y, 4 tiles a block): frame, jittered cell, virtual ground,
 //                closest hit through the 4-wide records of the LBVH (exact ties by the lower triangle index), the sensor's own planes passed
 //                (at most 3; the 4th an obstruction there), the hit triangle's label. Sensor-major cls / dist [nc x K].
-//   k_view_sums  a wave per sensor, lane l sums directions l, l + 64, ... (class weights, weight x ln d, minima, own
-//                surface, reach, glazing), then a fixed shuffle tree (s = 32 .. 1): the same bits for any chunking.
+//   k_view_sums  a wave per sensor, lane l sums directions l, l + 64, ... (class weights, weight x ln d by a
+//                table-driven FP64 log, minima, own surface, reach, glazing), then a fixed shuffle tree (s = 32 .. 1):
+//                the same bits for any chunking.
 //   k_view_iso   a wave per sensor: the isovist's area (one row) or volume (tetrahedra between adjacent rows and
-//                columns), each piece in float, summed in double in the same fixed order.
+//                columns), each piece in float, summed in double in the same fixed order; a lane makes its own column's
+//                cells and takes the next column's from its neighbour.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "dl_math.h"
@@ -122,27 +125,58 @@ struct SumArgs {
   double* out;
 };
 
+// ln x in double for a positive normal float x, within about 2 ulp of the correctly rounded value: x = 2^e m with
+// m in [1, 2) taken to the nearest c = 1 + j / 128 (j = 0 .. 128), ln x = e ln 2 + ln c + log1p(r), r = m / c - 1 with
+// |r| <= 2^-8 and log1p(r) by its series through r^7 (the first term left out is below 2^-66). ln 2 in Cody and
+// Waite's two parts, so that e ln2_hi is exact; tab[j] = (ln c, 1 / c), made once per block (ln_table).
+constexpr int kLnTab = 129;
+__device__ __forceinline__ double ln_float(float x, const double2* tab) {
+  const uint32_t u = __float_as_uint(x), mb = u & 0x7fffffu;
+  const int e = int(u >> 23) - 127;
+  const double2 t = tab[(mb + (1u << 15)) >> 16];
+  const double m = 1.0 + double(mb) * 0x1p-23;   // exact
+  const double r = fma(m, t.y, -1.0);
+  double q = fma(r, 1.0 / 7.0, -1.0 / 6.0);
+  q = fma(r, q, 0.2);
+  q = fma(r, q, -0.25);
+  q = fma(r, q, 1.0 / 3.0);
+  q = fma(r, q, -0.5);
+  const double p = fma(r * r, q, r);
+  const double ln2Hi = 6.93147180369123816490e-01, ln2Lo = 1.90821492927058770002e-10;   // ln2Hi: 32 bits, e ln2Hi exact
+  return (double(e) * ln2Hi + t.x) + fma(double(e), ln2Lo, p);
+}
+__device__ __forceinline__ void ln_table(double2* tab) {
+  for (int j = threadIdx.x; j < kLnTab; j += blockDim.x) {
+    const double c = 1.0 + double(j) / 128.0;
+    tab[j] = make_double2(log(c), 1.0 / c);
+  }
+  __syncthreads();
+}
+
 // NC >= the run's classes: the per-class sums live in registers, so the common runs (sky, ground, obstruction and a
-// target group or two) get a small NC and the occupancy its few registers allow
-template <uint32_t NC>
+// target group or two) get a small NC and the occupancy its few registers allow. NR >= the reach distances (0, 3 or 8):
+// a run without reach (or with EN 17037's three) adds no work for the slots it does not use.
+template <uint32_t NC, uint32_t NR>
 __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
+  __shared__ double2 lnTab[kLnTab];
+  ln_table(lnTab);
   const uint32_t lane = threadIdx.x & (kWave - 1), i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (i >= a.nc) return;   // whole waves
-  double w[NC], wl[NC], rc[8], own = 0, gz = 0;
+  double w[NC], wl[NC], rc[NR > 0 ? NR : 1], own = 0, gz = 0;
   float mn[NC];
   for (uint32_t c = 0; c < NC; ++c) { w[c] = 0; wl[c] = 0; mn[c] = __int_as_float(0x7f800000); }
-  for (uint32_t r = 0; r < 8; ++r) rc[r] = 0;
+  for (uint32_t r = 0; r < NR; ++r) rc[r] = 0;
   const auto take = [&](uint32_t c, float t, double wd) {
     if (c == 254) { own += wd; return; }
     if (c & 0x80) { gz += wd; c &= 0x7f; }
     const float dd = fminf(fmaxf(t, 0.1f), a.farF);
-    const double ld = dd == a.farF ? a.lnFar : dd == 0.1f ? -2.302585092994045684 : log((double)dd);
+    const double ld = dd == a.farF ? a.lnFar : dd == 0.1f ? -2.302585092994045684 : ln_float(dd, lnTab);
     // registers indexed by class: a small switch keeps them out of scratch
 #pragma unroll
     for (uint32_t q = 0; q < NC; ++q)
       if (q == c) { w[q] += wd; wl[q] += wd * ld; mn[q] = fminf(mn[q], dd); }
 #pragma unroll
-    for (uint32_t r = 0; r < 8; ++r) if (r < a.nReach && t >= a.reach[r]) rc[r] += wd;
+    for (uint32_t r = 0; r < NR; ++r) if (r < a.nReach && t >= a.reach[r]) rc[r] += wd;
   };
   // lane l takes directions l, l + 64, ... in ascending order (the oracle's order), four at a time so that their loads
   // are in flight together; the row of each (its weight) is followed without a division per ray
@@ -185,7 +219,7 @@ __global__ __launch_bounds__(256) void k_view_sums(SumArgs a) {
   const double ow = tree_sum(own);
   if (!lane) o[3 * a.nClass] = ow;
 #pragma unroll
-  for (uint32_t r = 0; r < 8; ++r) {
+  for (uint32_t r = 0; r < NR; ++r) {
     if (r >= a.nReach) break;
     const double s = tree_sum(rc[r]);
     if (!lane) o[3 * a.nClass + 1 + r] = s;
@@ -208,28 +242,58 @@ __device__ __forceinline__ float det3(float3 a, float3 b, float3 c) {
   return a.x * (u.y * v.z - u.z * v.y) + a.y * (u.z * v.x - u.x * v.z) + a.z * (u.x * v.y - u.y * v.x);
 }
 
+// A wave per sensor; lane l takes columns l, l + 64, ... and walks the rows: the pieces between its column and the
+// next one (c1) in each pair of adjacent rows. The next column's cell and length are lane l + 1's own (a shuffle); the
+// lanes whose next column is not lane l + 1's (lane 63, and the wrap to column 0) take the pass's one extra column cx,
+// whose cells and lengths the lanes make beforehand, one row each, 64 rows at a time. Every value is the one the
+// direct evaluation gives (the same function of the same cell), so the pieces and their order are unchanged.
 __global__ __launch_bounds__(256) void k_view_iso(IsoArgs a) {
   const uint32_t lane = threadIdx.x & (kWave - 1), i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (i >= a.nc) return;
+  if (i >= a.nc) return;   // whole waves
   const uint32_t gi = a.i0 + i, element = gi / a.nS, side = a.nS == 2 ? gi % 2 : a.back;
   const size_t K = size_t(a.nAz) * a.nEl;
   const auto len = [&](uint32_t k) { const size_t at = size_t(i) * K + k; return a.cls[at] == 254 ? 0.f : fminf(a.dist[at], a.farF); };
+  const auto cell = [&](uint32_t r, uint32_t col) {
+    return view_cell(r * a.nAz + col, col, a.rowSin[2 * r], a.rowSin[2 * r + 1], a.az0, a.dAz, a.jitter != 0, element, side);
+  };
+  const auto shfl3 = [](float3 v, int src) { return make_float3(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)); };
   double sum = 0;
   for (uint32_t base = 0; base < a.nAz; base += kWave) {
     const uint32_t col = base + lane < a.nAz ? base + lane : a.nAz - 1, c1 = col + 1 == a.nAz ? 0 : col + 1;
     const bool on = base + lane < a.nAz && (a.world || col + 1 < a.nAz);
-    float3 pa = view_cell(col, col, a.rowSin[0], a.rowSin[1], a.az0, a.dAz, a.jitter != 0, element, side);
-    float3 pb = view_cell(c1, c1, a.rowSin[0], a.rowSin[1], a.az0, a.dAz, a.jitter != 0, element, side);
-    float l0x = len(col), l0y = len(c1);
+    // the pass's extra column: lane 63's next (base + 64, or 0 past the last column), which is also the wrap's
+    const uint32_t last = base + kWave - 1 < a.nAz ? base + kWave - 1 : a.nAz - 1, cx = last + 1 == a.nAz ? 0 : last + 1;
+    const bool extra = lane == kWave - 1 || c1 == 0;
+    float3 exP = make_float3(0.f, 0.f, 0.f);
+    float exL = 0.f;
+    const auto extras = [&](uint32_t r0) {   // rows r0 + lane
+      const uint32_t r = r0 + lane < a.nEl ? r0 + lane : a.nEl - 1;
+      exP = cell(r, cx);
+      exL = len(r * a.nAz + cx);
+    };
+    const auto nextOf = [&](uint32_t r, float3 p, float l, float3& pn, float& ln) {   // row r's cell and length at c1
+      pn = shfl3(p, int(lane) + 1 < kWave ? int(lane) + 1 : int(lane));
+      ln = __shfl(l, int(lane) + 1 < kWave ? int(lane) + 1 : int(lane));
+      const int src = int(r % kWave);
+      const float3 xp = shfl3(exP, src);
+      const float xl = __shfl(exL, src);
+      if (extra) { pn = xp; ln = xl; }
+    };
+    extras(0);
+    float3 pa = cell(0, col), pb;
+    float l0x = len(col), l0y;
+    nextOf(0, pa, l0x, pb, l0y);
     if (a.dim == 2) {
       if (on) sum += (double)(0.5f * l0x * l0y * (pa.y * pb.x - pa.x * pb.y));
       continue;
     }
     for (uint32_t r = 1; r < a.nEl; ++r) {
-      const uint32_t d = r * a.nAz + col;
-      const float l1x = len(r * a.nAz + col), l1y = len(r * a.nAz + c1);
-      const float3 pc = view_cell(d, col, a.rowSin[2 * r], a.rowSin[2 * r + 1], a.az0, a.dAz, a.jitter != 0, element, side);
-      const float3 pe = view_cell(d - col + c1, c1, a.rowSin[2 * r], a.rowSin[2 * r + 1], a.az0, a.dAz, a.jitter != 0, element, side);
+      if (r % kWave == 0) extras(r);
+      const float3 pc = cell(r, col);
+      const float l1x = len(r * a.nAz + col);
+      float3 pe;
+      float l1y;
+      nextOf(r, pc, l1x, pe, l1y);
       if (on) sum += (double)(l0x * l0y * l1x * det3(pa, pb, pc) + l0y * l1y * l1x * det3(pb, pe, pc));
       pa = pc; pb = pe; l0x = l1x; l0y = l1y;
     }
@@ -317,9 +381,15 @@ extern "C" int32_t dlg_reduce_view(dlg_gpu* g, int64_t nc, const double* wRow, u
   s.out = g->out.as<double>();
   dlg_tbegin(g);
   const unsigned grid = unsigned((nc + 3) / 4);
-  if (nClass <= 4) k_view_sums<4><<<grid, 256, 0, g->stream>>>(s);
-  else if (nClass <= 8) k_view_sums<8><<<grid, 256, 0, g->stream>>>(s);
-  else k_view_sums<kClasses><<<grid, 256, 0, g->stream>>>(s);
+  const auto sums = [&](auto nc) {
+    constexpr uint32_t NC = decltype(nc)::value;
+    if (nReach == 0) k_view_sums<NC, 0><<<grid, 256, 0, g->stream>>>(s);
+    else if (nReach <= 3) k_view_sums<NC, 3><<<grid, 256, 0, g->stream>>>(s);
+    else k_view_sums<NC, 8><<<grid, 256, 0, g->stream>>>(s);
+  };
+  if (nClass <= 4) sums(std::integral_constant<uint32_t, 4>{});
+  else if (nClass <= 8) sums(std::integral_constant<uint32_t, 8>{});
+  else sums(std::integral_constant<uint32_t, kClasses>{});
   if (isoDim) {
     IsoArgs a;
     a.cls = s.cls; a.dist = s.dist; a.rowSin = reinterpret_cast<const float*>(b + oSin);
