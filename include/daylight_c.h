@@ -72,6 +72,10 @@ DL_API void dl_result_free(dl_result* result);
 /* ---- Extensions: no counterpart in the reference's daylight_c.h. For callers that consume the annual series on the
  * GPU (a serving loop, a benchmark) and for monitoring; the calls above are untouched by them. */
 
+/* The result's rows x cols values (row-major doubles) where the result holds them, valid until dl_result_free: for
+ * bindings that view them in place instead of copying them out with dl_result_values (NULL for a NULL result). */
+DL_API const double* dlx_result_data(const dl_result* result);
+
 /* A chunk of the series: rows [row0, row0 + rows) x cols float32 values, row-major, in device memory, valid until the
  * callback returns (NULL when the run has no series columns). Return DL_OK to go on; anything else stops the run. */
 typedef int32_t (*dlx_series_sink)(void* user, int64_t row0, int64_t rows, int64_t cols, const float* device);

@@ -38,7 +38,8 @@ GraphKey key_of(const dlg_gpu* g, uint32_t rayFlags, int stack, int32_t nQ, uint
   std::memset(&k, 0, sizeof k);
   const dlg_scene& s = g->scene;
   const void* ptrs[] = {s.wide.p, s.leaves.p, g->P.p, g->N.p, g->D.p, g->dirTab.p, g->bits.p, g->slots.p, g->traceCount.p, g->scratch.p,
-                        g->out.p, g->seriesIn.p, g->seriesE.p, g->seriesWork.p, g->skyS.p, g->skyG.p, g->skySdc.p, g->dc.p, g->stackOvf.p, g->tileNext.p};
+                        g->out.p, g->seriesIn.p, g->seriesE.p, g->seriesWork.p, g->skyS.p, g->skyG.p, g->skySdc.p, g->dc.p, g->stackOvf.p, g->tileNext.p,
+                        g->tileOrder.p};
   std::memcpy(k.p, ptrs, sizeof ptrs);
   const dlg_series_plan& q = g->seriesPlan;
   const int64_t vals[] = {s.nTris, pad_bits(s.pad), s.nNodes, g->nDir, g->nWords, g->nc, g->i0, int64_t(rayFlags), stack, g->bytes,

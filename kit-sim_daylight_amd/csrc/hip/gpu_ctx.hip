@@ -72,7 +72,7 @@ void dlg_destroy(dlg_gpu* g) {
   g->scene.release();
   for (DevBuf* b : {&g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
                     &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->sensWork, &g->slots, &g->dc, &g->skySdc,
-                    &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork})
+                    &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork, &g->tileOrder, &g->orderWork})
     b->release();
   if (g->hostSums) (void)hipHostFree(g->hostSums);
   for (hipEvent_t e : g->ev) if (e) (void)hipEventDestroy(e);
@@ -113,12 +113,15 @@ void dlg_trim(dlg_gpu* g, int32_t sensors) {
   if (g->hostSums) (void)hipHostFree(g->hostSums);
   g->hostSums = nullptr;
   g->hostSumsBytes = 0;
-  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->sensWork, &g->meshV, &g->meshI, &g->meshS, &g->meshWork}) b->release();
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->sensWork, &g->meshV, &g->meshI, &g->meshS, &g->meshWork,
+                    &g->tileOrder, &g->orderWork}) b->release();
+  g->orderGen = -1;
   g->meshOpen = 0;
   g->meshNv = g->meshNt = 0;
   if (sensors) {
     g->P.release(); g->N.release(); g->labels.release();
     g->nSens = 0;
+    g->sensGen += 1;
   }
 }
 
@@ -164,6 +167,7 @@ int32_t dlg_set_scene(dlg_gpu* g, const float* P, const float* N, int64_t nSens,
     DLG_CHECK(g->N.grow(size_t(nSens) * 12));
   }
   if (nSens > 0) {
+    g->sensGen += 1;
     DLG_CHECK(hipMemcpyAsync(g->P.p, P, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
     DLG_CHECK(hipMemcpyAsync(g->N.p, N, size_t(nSens) * 12, hipMemcpyHostToDevice, g->stream));
   }

@@ -202,12 +202,14 @@ extern "C" int32_t dlg_sensors_reserve(dlg_gpu* g, int64_t nSens, char* err, int
   DLG_CHECK(g->P.grow(size_t(nSens) * 12));
   DLG_CHECK(g->N.grow(size_t(nSens) * 12));
   g->nSens = nSens;
+  g->sensGen += 1;
   return 0;
 }
 
 extern "C" int32_t dlg_sensors(dlg_gpu* g, const dlg_sensor_mesh* m, int64_t j0, int32_t onVertices, int32_t sides, double offset, const double* centre,
                                int32_t ladybug, int64_t* made, int32_t* degenerate, int32_t* zeroNormals, char* err, int32_t errLen) {
   const int64_t nP = m->nPts, nF = m->nFaces, nC = m->nIdx;
+  g->sensGen += 1;   // P / N are written: the trace's sensor order is made again
   const int per = sides == 2 ? 2 : 1;
   const int64_t n = (onVertices ? nP : nF) * per;
   *made = onVertices ? nP : nF;

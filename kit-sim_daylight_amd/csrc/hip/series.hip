@@ -209,34 +209,38 @@ __device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k,
 
 // A's columns: the sky directions' coefficients, then (reflected light) the chunk's bounce rows dc[b nc + i]. A wave
 // writes one fragment per plane: lane l the 8 values of row 32 g + l % 32 at k = 16 ks + 8 (l / 32) + 0..7, as one
-// 16-byte store (a wave's store one contiguous kilobyte); its visibility word is one load for the 8 directions.
+// 16-byte store (a wave's store one contiguous kilobyte); its visibility word is one load for the 8 directions. A block's
+// four waves take kSplitKs k-steps of one row group in turn (few blocks, each with a run of stores).
+constexpr uint32_t kSplitKs = 32;
 template <int NS, bool H>
 __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N,
                                                  const float3* __restrict__ D, uint32_t nSky, uint32_t nKS, uint32_t nRG, const float* __restrict__ wsky,
                                                  const float* __restrict__ dc, uint32_t nBins, const uint32_t* __restrict__ mx, u32x4* __restrict__ A) {
-  const uint32_t lane = threadIdx.x & 63, ks = blockIdx.x * 4 + (threadIdx.x >> 6), g = blockIdx.y;
-  if (ks >= nKS) return;
+  const uint32_t lane = threadIdx.x & 63, g = blockIdx.y, ksEnd = min(nKS, (blockIdx.x + 1) * kSplitKs);
   const int e = H ? scale_exp(mx[0]) : 0;
-  const uint32_t i = 32 * g + lane % 32, d0 = 16 * ks + 8 * (lane / 32);
+  const uint32_t i = 32 * g + lane % 32;
   const bool live = i < nc;
   const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
-  uint32_t pk[NS][4];
+  for (uint32_t ks = blockIdx.x * kSplitKs + (threadIdx.x >> 6); ks < ksEnd; ks += 4) {
+    const uint32_t d0 = 16 * ks + 8 * (lane / 32);
+    uint32_t pk[NS][4];
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
-    for (int h = 0; h < 4; ++h) pk[s][h] = 0;
+      for (int h = 0; h < 4; ++h) pk[s][h] = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < 8; ++j) {
-    const uint32_t d = d0 + j;
-    const float c = !live ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, n, D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
-    uint16_t p[NS];
-    split<NS, H>(c, e, p);
+    for (uint32_t j = 0; j < 8; ++j) {
+      const uint32_t d = d0 + j;
+      const float c = !live ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, n, D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
+      uint16_t p[NS];
+      split<NS, H>(c, e, p);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) pk[s][j / 2] |= uint32_t(p[s]) << (16 * (j & 1));
+      for (int s = 0; s < NS; ++s) pk[s][j / 2] |= uint32_t(p[s]) << (16 * (j & 1));
+    }
+    const size_t plane = size_t(nRG) * nKS * 64, at = (size_t(g) * nKS + ks) * 64 + lane;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) A[s * plane + at] = u32x4{pk[s][0], pk[s][1], pk[s][2], pk[s][3]};
   }
-  const size_t plane = size_t(nRG) * nKS * 64, at = (size_t(g) * nKS + ks) * 64 + lane;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) A[s * plane + at] = u32x4{pk[s][0], pk[s][1], pk[s][2], pk[s][3]};
 }
 
 // B's rows: S of each sky direction's patch, then (reflected light) the bins' radiance Sdc [b T + t]
@@ -489,7 +493,7 @@ void enqueue_mfma(dlg_gpu* g) {
     k_scale_max<<<1024, 256, 0, g->stream>>>(reinterpret_cast<const float*>(b + q.oW), q.nSky, g->dc.as<float>(), nDc, g->skyS.as<float>(), nS,
                                              g->skySdc.as<float>(), nSdc, mx);
   }
-  k_split_a<NS, H><<<dim3((sh.nKS + 3) / 4, sh.nRG), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(),
+  k_split_a<NS, H><<<dim3((sh.nKS + kSplitKs - 1) / kSplitKs, sh.nRG), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(),
                                                                           q.nSky, sh.nKS, sh.nRG, reinterpret_cast<const float*>(b + q.oW),
                                                                           g->dc.as<float>(), q.nBins, mx, reinterpret_cast<u32x4*>(A));
   k_split_s<NS, H><<<dim3((sh.Kp + 255) / 256, sh.Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, q.nSky, sh.Kp, q.T, sh.Tp,

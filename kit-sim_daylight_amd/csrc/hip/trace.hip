@@ -18,6 +18,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "dl_math.h"
 #include "gpu_internal.h"
 #include "wide.h"
@@ -185,8 +187,10 @@ constexpr uint32_t kTileCounters = 8 * 16;
 static_assert(kTileCounters * 4 == dlg_tile_counter_bytes, "the counters' bytes");
 struct Tiles {
   uint32_t nSB, nTiles;
-  uint32_t* next;   // [8 x 16]: XCD x's count at 16 x
+  uint32_t* next;          // [8 x 16]: XCD x's count at 16 x
+  const uint32_t* order;   // the sensor at each position of the chunk (dlg_gpu::tileOrder), or null: position = sensor
   __device__ __forceinline__ uint64_t tile(uint32_t j, uint32_t xcd) const { return (uint64_t(j / kRun) * 8 + xcd) * kRun + j % kRun; }
+  __device__ __forceinline__ uint32_t sensor(uint32_t p) const { return order ? order[p] : p; }
 };
 template <class F>
 __device__ __forceinline__ void for_tiles(const Tiles& tl, F&& body) {
@@ -203,6 +207,32 @@ __device__ __forceinline__ void for_tiles(const Tiles& tl, F&& body) {
     }
   }
 }
+
+// The trace's sensor order (dlg_gpu::tileOrder): each sensor's key is its normal's components rounded to -1 / 0 / 1
+// (27 orientations, bits 57-61) above the Morton code of its position's 1 m cell (19 bits an axis around the recentred
+// origin, clamped), and the chunk-local indices are sorted by it; the sort is stable, so equal keys keep their order.
+// Which wave traces a sensor changes nothing in its bits: the order only makes a wave's 64 rays start close together.
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {   // bit b of v to bit 3 b
+  uint64_t x = v & 0x7ffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+__global__ void k_order_keys(const float3* __restrict__ P, const float3* __restrict__ N, uint32_t nc, uint64_t* __restrict__ key,
+                             uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const float3 p = P[i], n = N[i];
+  const auto side = [](float c) { return c > 0.5f ? 2u : c < -0.5f ? 0u : 1u; };
+  const auto cell = [](float c) { return uint32_t(fminf(fmaxf(floorf(c) + 262144.f, 0.f), 524287.f)); };
+  const uint64_t orient = side(n.x) * 9u + side(n.y) * 3u + side(n.z);
+  key[i] = (orient << 57) | spread3(cell(p.x)) | (spread3(cell(p.y)) << 1) | (spread3(cell(p.z)) << 2);
+  idx[i] = i;
+}
+constexpr int kOrderKeyBits = 62;
 
 __global__ void k_zero_counters(uint32_t* c) {
   if (threadIdx.x < kTileCounters) c[threadIdx.x] = 0u;
@@ -229,8 +259,9 @@ __global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide,
   uint32_t nTraced = 0;
   Fetches f;
   for_tiles(tl, [&](uint32_t sb, uint32_t w) {
-    const uint32_t i = sb * kWave + lane;
-    const bool live = i < nc;   // a lane past the chunk keeps a zero normal: every direction culled, nothing stored
+    const uint32_t p = sb * kWave + lane;   // a tile is 64 consecutive positions of the trace's sensor order
+    const bool live = p < nc;   // a lane past the chunk keeps a zero normal: every direction culled, nothing stored
+    const uint32_t i = live ? tl.sensor(p) : p;
     const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
     const float3 o = live ? offset_ray(P[i], n) : n;
     uint32_t word = 0;
@@ -333,8 +364,9 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
   Fetches f;
   uint32_t nTraced = 0;
   for_tiles(tl, [&](uint32_t sb, uint32_t w) {
-    const uint32_t i = sb * kWave + lane;
-    const bool live = i < nc;   // as k_bits: a zero normal past the chunk, nothing stored
+    const uint32_t p = sb * kWave + lane;
+    const bool live = p < nc;   // as k_bits: a zero normal past the chunk, nothing stored
+    const uint32_t i = live ? tl.sensor(p) : p;
     const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
     const float3 o = live ? offset_ray(P[i], n) : n;
     uint32_t quad = 0;
@@ -374,6 +406,7 @@ __global__ __launch_bounds__(kWave, 4) void k_bytes(const float4* __restrict__ w
 struct TraceBufs {
   const float3* P;
   const float3* N;
+  const uint32_t* order;   // null: the chunk's own order
   uint32_t nc;
   uint32_t* vis;
   unsigned long long* traced;
@@ -440,7 +473,7 @@ void launch_trace(dlg_gpu* g, const TraceBufs& b, uint32_t rayFlags, int stack, 
   if (b.nc == 0 || g->nWords == 0) return;
   const uint32_t cull = (rayFlags & 16u) ? 1u : 0u;
   const uint32_t nSB = (b.nc + kWave - 1) / kWave;
-  const Tiles tl{nSB, nSB * uint32_t(g->nWords), b.next};
+  const Tiles tl{nSB, nSB * uint32_t(g->nWords), b.next, b.order};
   k_zero_counters<<<1, kTileCounters, 0, g->stream>>>(b.next);
   with_stack(stack, [&](auto S) {
     constexpr int ST = decltype(S)::value;
@@ -485,6 +518,7 @@ dlg_trace_shape dlg_trace_grid(const dlg_gpu* g, int64_t nc, int stack, bool byt
 
 int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen) {
   if (i0 + nc > g->nSens) { dlg_fail(err, errLen, "dlg_trace: sensor range exceeds the scene"); return 1; }
+  if (nc > int64_t(0x7fffffff)) { dlg_fail(err, errLen, "dlg_trace: at most 2^31 - 1 sensors a chunk"); return 1; }
   if (dlg_trace_check(g, rayFlags, stack, err, errLen)) return 1;
   const bool bytes = !(rayFlags & 1u);
   DLG_CHECK(g->bits.grow(size_t(g->nWords) * size_t(nc) * (bytes ? 32 : 4)));
@@ -496,6 +530,26 @@ int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags,
     DLG_CHECK(g->traceCount.grow(sizeof(unsigned long long)));
     DLG_CHECK(hipMemsetAsync(g->traceCount.p, 0, sizeof(unsigned long long), g->stream));
   }
+  if (g->orderGen != g->sensGen || g->orderI0 != i0 || g->orderN != nc) {   // the chunk's sensor order
+    DLG_CHECK(g->tileOrder.grow(size_t(nc > 0 ? nc : 1) * 4));
+    if (nc > 0) {
+      size_t tmp = 0;
+      DLG_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, static_cast<const uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+                                                   static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), int(nc), 0, kOrderKeyBits,
+                                                   g->stream));
+      const size_t oKey2 = size_t(nc) * 8, oIdx = 2 * oKey2, oTmp = (oIdx + size_t(nc) * 4 + 255) & ~size_t(255);
+      DLG_CHECK(g->orderWork.grow(oTmp + tmp));
+      char* w = g->orderWork.as<char>();
+      uint64_t* key = reinterpret_cast<uint64_t*>(w);
+      uint32_t* idx = reinterpret_cast<uint32_t*>(w + oIdx);
+      k_order_keys<<<unsigned((nc + 255) / 256), 256, 0, g->stream>>>(g->P.as<float3>() + i0, g->N.as<float3>() + i0, uint32_t(nc), key, idx);
+      DLG_CHECK(hipcub::DeviceRadixSort::SortPairs(w + oTmp, tmp, key, reinterpret_cast<uint64_t*>(w + oKey2), idx, g->tileOrder.as<uint32_t>(),
+                                                   int(nc), 0, kOrderKeyBits, g->stream));
+    }
+    g->orderGen = g->sensGen;
+    g->orderI0 = i0;
+    g->orderN = nc;
+  }
   g->i0 = i0;
   g->nc = nc;
   g->bytes = bytes ? 1 : 0;
@@ -505,14 +559,14 @@ int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags,
 // The trace's kernel enqueued on the library's stream (no wait: a graph may capture it); counts non-null runs the
 // counting variant.
 void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts) {
-  const TraceBufs b{g->P.as<float3>() + g->i0, g->N.as<float3>() + g->i0, uint32_t(g->nc), g->bits.as<uint32_t>(),
+  const TraceBufs b{g->P.as<float3>() + g->i0, g->N.as<float3>() + g->i0, g->tileOrder.as<uint32_t>(), uint32_t(g->nc), g->bits.as<uint32_t>(),
                     g->traceCount.as<unsigned long long>(), counts, g->stackOvf.as<uint32_t>(), g->traceGrid, g->tileNext.as<uint32_t>()};
   launch_trace(g, b, rayFlags, stack, g->bytes != 0);
 }
 
 void dlg_trace_enqueue_into(dlg_gpu* g, const float* P, const float* N, uint32_t nc, uint32_t rayFlags, int stack, uint32_t* vis,
                             unsigned long long* traced, uint32_t* ovf, uint32_t grid, uint32_t* next) {
-  const TraceBufs b{reinterpret_cast<const float3*>(P), reinterpret_cast<const float3*>(N), nc, vis, traced, nullptr, ovf, grid, next};
+  const TraceBufs b{reinterpret_cast<const float3*>(P), reinterpret_cast<const float3*>(N), nullptr, nc, vis, traced, nullptr, ovf, grid, next};
   launch_trace(g, b, rayFlags, stack, !(rayFlags & 1u));
 }
 

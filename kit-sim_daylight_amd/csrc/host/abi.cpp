@@ -290,6 +290,8 @@ void dl_result_free(dl_result* result) { delete result; }
 
 // ------------------------------------------------------------------------------------------------ extensions (dlx_)
 
+const double* dlx_result_data(const dl_result* result) { return result ? result->values.data() : nullptr; }
+
 int32_t dlx_run_series_device(int32_t argc, const char* const* argv, dl_mesh* const* sensors, int32_t n_sensors, dl_mesh* const* context,
                               int32_t n_context, dl_control* control, dlx_series_sink sink, void* user, dl_result** out) {
   int32_t sinkRc = DL_OK;

@@ -416,6 +416,19 @@ class DaylightRun {
     else out.values[el * out.columns.size() + col] = v;
   }
 
+  // store() of every quantity of sensors i0 .. i0 + nc from the chunk's sums s[qi nc + i]: sensor by sensor, so that an
+  // element's columns are written together, in blocks over the host's threads for large chunks
+  void store_rows(size_t i0, size_t nc, const double* s) {
+    constexpr size_t kBlock = size_t(1) << 15;
+    const auto rows = [&](size_t b) {
+      for (size_t i = b * kBlock, end = std::min(nc, i + kBlock); i < end; ++i)
+        for (size_t qi = 0; qi < nQ(); ++qi) store(i0 + i, qi, s[qi * nc + i]);
+    };
+    const size_t blocks = (nc + kBlock - 1) / kBlock;
+    if (blocks > 1) par_for(blocks, rows);
+    else if (blocks) rows(0);
+  }
+
   File seriesFile{nullptr, &std::fclose}, bitsFile{nullptr, &std::fclose};
   std::vector<double> seriesHoys;
   std::thread alloc;   // columns() while the first chunk runs
@@ -551,8 +564,7 @@ class DaylightRun {
     dlg_stats(g, &b);
     const auto tStore = Clk::now();
     if (!ready()) return false;
-    for (size_t qi = 0; qi < nQ(); ++qi)
-      for (size_t i = 0; i < nc; ++i) store(i0 + i, qi, s[qi * nc + i]);
+    store_rows(i0, nc, s);
     out.stats.reduceMs += (b.reduceMs - a.reduceMs) + ms_since(tStore);
     (void)t0;
     if (O.progress) O.progress->store(double(i0 + nc) / double(nSens));

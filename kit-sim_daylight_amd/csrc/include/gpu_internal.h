@@ -110,6 +110,12 @@ struct dlg_gpu {
   DevBuf stackOvf;
   uint32_t traceGrid = 0;
   DevBuf tileNext;   // the persistent trace's tile counters
+  // The order the trace takes the sensors of a chunk in (dlg_trace_prepare): chunk-local indices sorted by the normal's
+  // rounded components and the Morton code of the position's 1 m cell, so that a wave's 64 sensors are a compact patch
+  // of one orientation. For the chunk [orderI0, orderI0 + orderN) of the sensors of generation orderGen; sensGen counts
+  // every write of P / N.
+  DevBuf tileOrder, orderWork;
+  int64_t orderI0 = -1, orderN = -1, orderGen = -1, sensGen = 0;
   DevBuf sensWork;   // dlg_sensors: a sensor mesh on the device and its work arrays
   // the tree dlg_build makes (dlg_set_tree): DLG_TREE_SAH or DLG_TREE_LBVH
   int32_t tree = DLG_TREE_SAH;

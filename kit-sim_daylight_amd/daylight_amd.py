@@ -100,6 +100,7 @@ C_ABI = [
     ("dl_result_values_f32", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("dl_result_stats", C.c_int32, [C.c_void_p, C.c_void_p]),
     ("dl_result_free", None, [C.c_void_p]),
+    ("dlx_result_data", C.c_void_p, [C.c_void_p]),
     ("dlx_run_series_device", C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
     ("dlx_gpu_counters", C.c_int32, [C.c_void_p, C.c_int32]),

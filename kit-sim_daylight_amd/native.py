@@ -142,17 +142,34 @@ def _argv(args):
     return len(argv), (C.c_char_p * max(1, len(argv)))(*argv)
 
 
+class _ResultOwner:
+    """Frees a dl_result once nothing views its values any more (the last reference to their buffer goes)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        if self.handle:
+            _d.lib().dl_result_free(self.handle)
+            self.handle = None
+
+
 def _take_result(out):
+    """Result of a dl_result: its values viewed in place (dlx_result_data), the result freed with their last view."""
     L = _d.lib()
-    try:
-        values = np.empty((L.dl_result_rows(out), L.dl_result_cols(out)))
-        st = np.empty(_d.DL_STATS)
-        _check(L.dl_result_values(out, values.ctypes.data, values.size))
-        _check(L.dl_result_stats(out, st.ctypes.data))
-        manifest = L.dl_result_manifest(out).decode()
-        return Result(values, json.loads(manifest), {k: f(x) for (k, f), x in zip(_STATS, st.tolist())})
-    finally:
-        L.dl_result_free(out)
+    owner = _ResultOwner(out)
+    rows, cols = L.dl_result_rows(out), L.dl_result_cols(out)
+    st = np.empty(_d.DL_STATS)
+    _check(L.dl_result_stats(out, st.ctypes.data))
+    manifest = L.dl_result_manifest(out).decode()
+    ptr = L.dlx_result_data(out) if rows * cols else None
+    if ptr:
+        buf = (C.c_double * (rows * cols)).from_address(ptr)
+        buf._owner = owner   # the values' buffer keeps the result alive
+        values = np.frombuffer(buf, dtype=np.float64).reshape(rows, cols)
+    else:
+        values = np.empty((rows, cols))
+    return Result(values, json.loads(manifest), {k: f(x) for (k, f), x in zip(_STATS, st.tolist())})
 
 
 def run(tool, args, sensors, context=(), control=None):
