@@ -6,9 +6,11 @@
 // byte mode), in FP64, so the result is the same bits for any chunking. The order makes each sum a dependent chain; the
 // chains of different sensors run side by side:
 //   a lane per sensor; every lane of a wave walks the directions together (the direction and its weights are the same
-//   for all: staged in LDS per chunk of 256 directions and read as broadcasts), reads its visibility word once per 32
-//   directions (one coalesced load per wave) and adds the product where its bit is set. Words that no lane of the wave
-//   sees through are skipped.
+//   for all: staged in LDS per chunk of 256 directions, the weights widened to double there, and read as broadcasts),
+//   reads its visibility word once per 32 directions and adds the product where its bit is set. Words that no lane of
+//   the wave sees through are skipped. The lanes take the sensors in the trace's order (dlg_gpu::tileOrder, when the
+//   chunk has one): a wave's 64 sensors are then a compact patch of one orientation, which see mostly the same
+//   directions, so the wave steps over fewer directions that only some of its lanes see. Each sum is the same.
 // HBM: 4 nWords bytes of bits (byte mode: 32 nWords) + 12 (N) + 8 nQ per sensor; directions and weights stay in L2.
 #include "dl_math.h"
 #include "gpu_internal.h"
@@ -20,16 +22,17 @@ constexpr int kMaxQ = 8, kBlock = 256;
 template <int NQ, bool BYTES>
 __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ vis, uint32_t nc, uint32_t nWords, const float3* __restrict__ N,
                                                    const float3* __restrict__ D, const float* __restrict__ W, uint32_t nDir,
-                                                   uint32_t cosMask, double* __restrict__ out) {
+                                                   uint32_t cosMask, const uint32_t* __restrict__ order, double* __restrict__ out) {
   // chunks of 8 words: their 256 directions and weights staged in LDS (every lane reads the same entry: a broadcast);
   // bits: the 8 visibility words of the lane loaded up front; bytes: per word its 8 quads, the fraction of a byte from a
   // table in LDS
   constexpr uint32_t kChunkW = 8, kChunkD = kChunkW * 32;
   __shared__ float sD[kChunkD * 3];
-  __shared__ float sW[NQ][kChunkD];
+  __shared__ double sW[NQ][kChunkD];
   __shared__ float sT[BYTES ? 256 : 1];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool live = i < nc;
+  const uint32_t pos = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = pos < nc;
+  const uint32_t i = live && order ? order[pos] : pos;
   const float3 n = live ? N[i] : make_float3(0.f, 0.f, 0.f);
   if (BYTES) sT[threadIdx.x] = dlm::byte_T(threadIdx.x);   // ordered before its reads by the chunk loop's barriers
   double acc[NQ];
@@ -38,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
   const auto term = [&](uint32_t e, float T) {
     const float c = fmaxf(dlm::dot3(n, make_float3(sD[3 * e], sD[3 * e + 1], sD[3 * e + 2])), 0.f);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] += (double)(((cosMask >> q) & 1u ? c : 1.f) * T) * (double)sW[q][e];
+    for (int q = 0; q < NQ; ++q) acc[q] += (double)(((cosMask >> q) & 1u ? c : 1.f) * T) * sW[q][e];
   };
   for (uint32_t w0 = 0; w0 < nWords; w0 += kChunkW) {
     const uint32_t d0 = w0 * 32, dn = nDir - d0 < kChunkD ? nDir - d0 : kChunkD;
@@ -52,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
       const float3 v = D[d0 + k];
       sD[3 * k] = v.x; sD[3 * k + 1] = v.y; sD[3 * k + 2] = v.z;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) sW[q][k] = W[size_t(q) * nDir + d0 + k];
+      for (int q = 0; q < NQ; ++q) sW[q][k] = (double)W[size_t(q) * nDir + d0 + k];
     }
     __syncthreads();
     if (BYTES) {
@@ -93,21 +96,24 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const uint32_t* __restrict__ 
 }
 
 template <int NQ>
-void launch(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float3* N, const float* W, uint32_t mask, double* out) {
+void launch(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float3* N, const float* W, uint32_t mask, const uint32_t* order,
+            double* out) {
   const auto k = bytes ? k_reduce<NQ, true> : k_reduce<NQ, false>;
-  k<<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(vis, nc, uint32_t(g->nWords), N, g->D.as<float3>(), W, uint32_t(g->nDir), mask, out);
+  k<<<(nc + kBlock - 1) / kBlock, kBlock, 0, g->stream>>>(vis, nc, uint32_t(g->nWords), N, g->D.as<float3>(), W, uint32_t(g->nDir), mask, order,
+                                                          out);
 }
 
-void launch_nq(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float3* N, const float* W, int32_t nQ, uint32_t mask, double* out) {
+void launch_nq(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float3* N, const float* W, int32_t nQ, uint32_t mask,
+               const uint32_t* order, double* out) {
   switch (nQ) {
-    case 1: launch<1>(g, vis, bytes, nc, N, W, mask, out); break;
-    case 2: launch<2>(g, vis, bytes, nc, N, W, mask, out); break;
-    case 3: launch<3>(g, vis, bytes, nc, N, W, mask, out); break;
-    case 4: launch<4>(g, vis, bytes, nc, N, W, mask, out); break;
-    case 5: launch<5>(g, vis, bytes, nc, N, W, mask, out); break;
-    case 6: launch<6>(g, vis, bytes, nc, N, W, mask, out); break;
-    case 7: launch<7>(g, vis, bytes, nc, N, W, mask, out); break;
-    default: launch<8>(g, vis, bytes, nc, N, W, mask, out); break;
+    case 1: launch<1>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    case 2: launch<2>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    case 3: launch<3>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    case 4: launch<4>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    case 5: launch<5>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    case 6: launch<6>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    case 7: launch<7>(g, vis, bytes, nc, N, W, mask, order, out); break;
+    default: launch<8>(g, vis, bytes, nc, N, W, mask, order, out); break;
   }
 }
 
@@ -125,12 +131,14 @@ int32_t dlg_reduce_prepare(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, c
 }
 
 void dlg_reduce_enqueue(dlg_gpu* g, uint32_t nc, int32_t nQ, uint32_t mask) {
-  launch_nq(g, g->bits.as<uint32_t>(), g->bytes != 0, nc, g->N.as<float3>() + g->i0, g->scratch.as<float>(), nQ, mask, g->out.as<double>());
+  const bool ordered = g->orderGen == g->sensGen && g->orderI0 == g->i0 && g->orderN == int64_t(nc);   // the trace's order of this chunk
+  launch_nq(g, g->bits.as<uint32_t>(), g->bytes != 0, nc, g->N.as<float3>() + g->i0, g->scratch.as<float>(), nQ, mask,
+            ordered ? g->tileOrder.as<uint32_t>() : nullptr, g->out.as<double>());
 }
 
 void dlg_reduce_enqueue_from(dlg_gpu* g, const uint32_t* vis, bool bytes, uint32_t nc, const float* N, const float* W, int32_t nQ, uint32_t mask,
                              double* out) {
-  launch_nq(g, vis, bytes, nc, reinterpret_cast<const float3*>(N), W, nQ, mask, out);
+  launch_nq(g, vis, bytes, nc, reinterpret_cast<const float3*>(N), W, nQ, mask, nullptr, out);
 }
 
 extern "C" int32_t dlg_reduce(dlg_gpu* g, int64_t nc, int32_t nQ, const float* W, const int32_t* useCos, double* out, char* err, int32_t errLen) {

@@ -37,10 +37,21 @@ struct StepOut {
   float binVis[4], binRad[4];
 };
 
+// the patch centres' unit vectors as AddDirect's search dots them with the sun (cos alt sin azi, cos alt cos azi,
+// sin alt, each from the float angles in FP64): made once per sky instead of once per step and patch
+__global__ void k_patch_dirs(int P, const float* __restrict__ palt, const float* __restrict__ pazi, double* __restrict__ pd) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double pa = cos((double)palt[p]);
+  pd[3 * p] = pa * sin((double)pazi[p]);
+  pd[3 * p + 1] = pa * cos((double)pazi[p]);
+  pd[3 * p + 2] = sin((double)palt[p]);
+}
+
 __global__ void k_steps(int T, const int* __restrict__ month, const int* __restrict__ day, const int* __restrict__ jd,
                         const double* __restrict__ hour, const double* __restrict__ dt, const float* __restrict__ dni,
                         const float* __restrict__ dhi, double lat, double lon, double tz, double north, int leap, float grefl, int wantVis,
-                        int reflect, int P, const float* __restrict__ palt, const float* __restrict__ pazi, const float* __restrict__ omega,
+                        int reflect, int P, const double* __restrict__ pd, const float* __restrict__ omega,
                         StepOut* __restrict__ so, double* __restrict__ alt, double* __restrict__ azi, int* __restrict__ up,
                         float* __restrict__ vec) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,8 +75,7 @@ __global__ void k_steps(int T, const int* __restrict__ month, const int* __restr
     int bestP[4] = {0, 0, 0, 0};
     const double ca = cos(o.altRad), sv0 = ca * sin(o.aziRad), sv1 = ca * cos(o.aziRad), sv2 = sin(o.altRad);
     for (int p = 0; p < P; ++p) {
-      const double pa = cos((double)palt[p]);
-      const double dp = (pa * sin((double)pazi[p])) * sv0 + (pa * cos((double)pazi[p])) * sv1 + sin((double)palt[p]) * sv2;
+      const double dp = pd[3 * p] * sv0 + pd[3 * p + 1] * sv1 + pd[3 * p + 2] * sv2;
       for (int i = 0; i < 4; ++i)
         if (dp > best[i]) {
           for (int j = 3; j > i; --j) { best[j] = best[j - 1]; bestP[j] = bestP[j - 1]; }
@@ -109,15 +119,30 @@ __global__ void k_lum(int T, int P, const StepOut* __restrict__ so, const float*
   L[size_t(t) * P + p] = o.q.night ? 0.f : perez_patch(o.q, palt[p], pazi[p], o.aziRad);
 }
 
-__global__ void k_norm(int T, int P, StepOut* __restrict__ so, const float* __restrict__ L, const double* __restrict__ tsinP,
-                       const float* __restrict__ omega) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// A step's normalisation, the sum over its patches in patch order of L tsin omega (sky.cpp's order): a block of 64 steps
+// stages tiles of 64 patches of its steps' rows in LDS (each row's segment one coalesced load), and each lane sums its
+// step's row from there.
+constexpr int kNormT = 64;
+__global__ __launch_bounds__(kNormT) void k_norm(int T, int P, StepOut* __restrict__ so, const float* __restrict__ L, const double* __restrict__ tsinP,
+                                                 const float* __restrict__ omega) {
+  __shared__ float tile[kNormT][kNormT + 1];
+  __shared__ double ws[kNormT];
+  __shared__ float om[kNormT];
+  const int t0 = blockIdx.x * kNormT, t = t0 + int(threadIdx.x);
+  double norm = 0.0;
+  for (int p0 = 0; p0 < P; p0 += kNormT) {
+    const int np = P - p0 < kNormT ? P - p0 : kNormT;
+    __syncthreads();   // the last tile's reads are done
+    for (int r = 0; r < kNormT; ++r)
+      if (t0 + r < T && int(threadIdx.x) < np) tile[r][threadIdx.x] = L[size_t(t0 + r) * P + p0 + threadIdx.x];
+    if (int(threadIdx.x) < np) { ws[threadIdx.x] = tsinP[p0 + threadIdx.x]; om[threadIdx.x] = omega[p0 + threadIdx.x]; }
+    __syncthreads();
+    if (t < T)
+      for (int k = 0; k < np; ++k) norm += tile[threadIdx.x][k] * ws[k] * om[k];
+  }
   if (t >= T) return;
   StepOut& o = so[t];
   if (o.q.night) return;
-  double norm = 0.0;
-  const float* Lt = L + size_t(t) * P;
-  for (int p = 0; p < P; ++p) norm += Lt[p] * tsinP[p] * omega[p];
   o.uniform = norm <= 1e-6;
   if (o.uniform) norm = kPi;
   o.sfRad = (o.q.difRad / norm) * (1. / 179.);
@@ -270,7 +295,7 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
                oSo = take(sizeof(StepOut) * T), oL = take(size_t(4) * T * P), oLr = take(in->wantRad ? size_t(8) * T * P : 8),
                oLv = take(in->wantVis ? size_t(8) * T * P : 8), oAlt = take(8 * T), oAzi = take(8 * T), oUp = take(4 * T),
                oVec = take(12 * T), oSkyR = take(8 * P), oSkyV = take(8 * P), oGnd = take(16), oSunR = take(8 * T), oSunV = take(8 * T),
-               oGr = take(8 * T), oGv = take(8 * T);
+               oGr = take(8 * T), oGv = take(8 * T), oPd = take(24 * size_t(P));
   DLG_CHECK(g->skyWork.grow(off));
   char* b = g->skyWork.as<char>();
   const auto up = [&](size_t at, const void* src, size_t bytes) { return hipMemcpyAsync(b + at, src, bytes, hipMemcpyHostToDevice, g->stream); };
@@ -288,13 +313,14 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
   }
 #define PTR(T_, o) reinterpret_cast<T_*>(b + (o))
   dlg_tbegin(g);
+  k_patch_dirs<<<blocks(P, 256), 256, 0, g->stream>>>(P, PTR(float, oPalt), PTR(float, oPazi), PTR(double, oPd));
   k_steps<<<blocks(T, 64), 64, 0, g->stream>>>(T, PTR(int, oMonth), PTR(int, oDay), PTR(int, oJd), PTR(double, oHour), PTR(double, oDt),
                                                PTR(float, oDni), PTR(float, oDhi), in->lat, in->lon, in->tz, in->north, in->leap,
-                                               (float)in->grefl, in->wantVis, in->reflect, P, PTR(float, oPalt), PTR(float, oPazi), PTR(float, oOm),
+                                               (float)in->grefl, in->wantVis, in->reflect, P, PTR(double, oPd), PTR(float, oOm),
                                                PTR(StepOut, oSo), PTR(double, oAlt), PTR(double, oAzi), PTR(int, oUp), PTR(float, oVec));
   if (T > 0) {
     k_lum<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oPalt), PTR(float, oPazi), PTR(float, oL));
-    k_norm<<<blocks(T, 64), 64, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oTs), PTR(float, oOm));
+    k_norm<<<blocks(T, kNormT), kNormT, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oTs), PTR(float, oOm));
     k_apply<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oDt), in->wantRad,
                                                                       in->wantVis, PTR(int, oCol), Tup, seriesMode, PTR(double, oLr),
                                                                       PTR(double, oLv), seriesMode ? g->skyS.as<float>() : nullptr);
