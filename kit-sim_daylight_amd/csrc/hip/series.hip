@@ -212,10 +212,13 @@ __device__ __forceinline__ size_t frag_index(uint32_t g, uint32_t r, uint32_t k,
 // 16-byte store (a wave's store one contiguous kilobyte); its visibility word is one load for the 8 directions. A block's
 // four waves take kSplitKs k-steps of one row group in turn (few blocks, each with a run of stores).
 constexpr uint32_t kSplitKs = 32;
+// nz[ks nRG + g]: 1 where fragment (g, ks) holds a coefficient other than 0 (a wave's ballot), else 0; the contraction
+// steps over the slices whose fragments are all 0 for a block's rows (their products are exact zeros).
 template <int NS, bool H>
 __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bits, uint32_t bytes, uint32_t nc, const float3* __restrict__ N,
                                                  const float3* __restrict__ D, uint32_t nSky, uint32_t nKS, uint32_t nRG, const float* __restrict__ wsky,
-                                                 const float* __restrict__ dc, uint32_t nBins, const uint32_t* __restrict__ mx, u32x4* __restrict__ A) {
+                                                 const float* __restrict__ dc, uint32_t nBins, const uint32_t* __restrict__ mx, u32x4* __restrict__ A,
+                                                 uint8_t* __restrict__ nz) {
   const uint32_t lane = threadIdx.x & 63, g = blockIdx.y, ksEnd = min(nKS, (blockIdx.x + 1) * kSplitKs);
   const int e = H ? scale_exp(mx[0]) : 0;
   const uint32_t i = 32 * g + lane % 32;
@@ -228,10 +231,12 @@ __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bi
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int h = 0; h < 4; ++h) pk[s][h] = 0;
+    bool any = false;
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) {
       const uint32_t d = d0 + j;
       const float c = !live ? 0.f : d < nSky ? coeff(bits, bytes, nc, i, d, n, D, wsky) : d - nSky < nBins ? dc[size_t(d - nSky) * nc + i] : 0.f;
+      any |= c != 0.f;
       uint16_t p[NS];
       split<NS, H>(c, e, p);
 #pragma unroll
@@ -240,6 +245,8 @@ __global__ __launch_bounds__(256) void k_split_a(const uint32_t* __restrict__ bi
     const size_t plane = size_t(nRG) * nKS * 64, at = (size_t(g) * nKS + ks) * 64 + lane;
 #pragma unroll
     for (int s = 0; s < NS; ++s) A[s * plane + at] = u32x4{pk[s][0], pk[s][1], pk[s][2], pk[s][3]};
+    const bool fragNz = __ballot(any) != 0;
+    if (lane == 0) nz[size_t(ks) * nRG + g] = fragNz ? 1 : 0;
   }
 }
 
@@ -277,6 +284,8 @@ constexpr uint32_t kTileN = 128, kSliceK = 32, kMaxTileM = 256;
 // 3.9e-7, BF16x3 2.5e-6 -> 1.3e-6 relative to FP64. The totals take the registers of a third wave per SIMD (2 waves, 252
 // VGPRs, no spills; 2 % slower than the single chain at 3 waves; 3 waves with the totals spilled: 30 % slower).
 constexpr uint32_t kBlockSlices = 8;
+// the slices a block's list holds (K up to 16,384 directions); a longer K multiplies every slice (nz null)
+constexpr uint32_t kMaxListed = 512;
 // one product of 16-byte operand fragments: bf16 pieces, or (H) fp16 ones
 template <bool H>
 __device__ __forceinline__ f32x16 mma(u32x4 a, u32x4 b, f32x16 c) {
@@ -290,7 +299,7 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
     const u32x4* __restrict__ A, const u32x4* __restrict__ B, uint32_t nc, uint32_t nRG, uint32_t nCG, uint32_t nKS, int T,
     const uint32_t* __restrict__ bits, uint32_t bytes, const float3* __restrict__ N, const float3* __restrict__ D, const float* __restrict__ G,
     const float* __restrict__ g, uint32_t sun0, const int* __restrict__ sunOfCol, const float* __restrict__ sunW, const uint32_t* __restrict__ mx,
-    float* __restrict__ E, uint32_t tilesT, uint32_t tiles, uint32_t group) {
+    const uint8_t* __restrict__ nz, float* __restrict__ E, uint32_t tilesT, uint32_t tiles, uint32_t group) {
   constexpr int kThreads = 128 * WM, kRows = 64 * WM, kGA = 2 * WM, kGB = 4;
   constexpr int kFragA = NS * kGA * 128, kFragB = NS * kGB * 128;   // uint4 per slice: planes x groups x 2 k-steps x 64 lanes
   constexpr int kPerA = kFragA / kThreads, kPerB = kFragB / kThreads;
@@ -320,19 +329,51 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
     const uint32_t j = threadIdx.x + kThreads * m;
     return (j / (kGB * 128)) * bPl + (cg0 + (j / 128) % kGB) * gStride + j % 128;
   };
+  // The slices this block multiplies: those where some fragment of its kGA row groups holds a coefficient other than 0
+  // (nz from k_split_a), in ascending order. The others add exact zeros: stepping over them leaves every sum's bits as
+  // they are, and the accumulation blocks keep their bounds (kBlockSlices slices by index).
+  const uint32_t nSlices = nKS / 2;
+  __shared__ uint16_t list[kMaxListed];
+  __shared__ uint32_t waveCount[kThreads / 64];
+  const bool listed = nz != nullptr;   // else every slice, q = idx (the host passes no flags past kMaxListed slices)
+  uint32_t nList = listed ? 0 : nSlices;
+  for (uint32_t base = 0; listed && base < nSlices; base += kThreads) {
+    const uint32_t q = base + threadIdx.x;
+    bool keep = q < nSlices;
+    if (keep) {
+      const uint32_t* f0 = reinterpret_cast<const uint32_t*>(nz + size_t(2 * q) * nRG + rg0);
+      const uint32_t* f1 = reinterpret_cast<const uint32_t*>(nz + size_t(2 * q + 1) * nRG + rg0);
+      uint32_t any = 0;
+#pragma unroll
+      for (int k = 0; k < kGA / 4; ++k) any |= f0[k] | f1[k];
+      keep = any != 0;
+    }
+    const uint64_t vote = __ballot(keep);
+    if (lane == 0) waveCount[w] = uint32_t(__popcll(vote));
+    __syncthreads();
+    uint32_t before = nList;
+    for (int v = 0; v < w; ++v) before += waveCount[v];
+    if (keep) list[before + uint32_t(__popcll(vote & ((1ull << lane) - 1ull)))] = uint16_t(q);
+    for (int v = 0; v < kThreads / 64; ++v) nList += waveCount[v];
+    __syncthreads();
+  }
+  const auto sliceAt = [&](uint32_t idx) { return listed ? uint32_t(list[idx]) : idx; };
   u32x4 pa[kPerA], pb[kPerB];
+  if (nList) {
+    const uint32_t step = 128 * sliceAt(0);
 #pragma unroll
-  for (int m = 0; m < kPerA; ++m) pa[m] = A[aAt(m)];
+    for (int m = 0; m < kPerA; ++m) pa[m] = A[aAt(m) + step];
 #pragma unroll
-  for (int m = 0; m < kPerB; ++m) pb[m] = B[bAt(m)];
+    for (int m = 0; m < kPerB; ++m) pb[m] = B[bAt(m) + step];
+  }
   // acc: the products of the current block of kBlockSlices slices; tot: the blocks summed
   f32x16 acc[2][2], tot[2][2];
 #pragma unroll
   for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
     for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = tot[fm][fn] = f32x16{};
-  const uint32_t nSlices = nKS / 2;
-  for (uint32_t q = 0; q < nSlices; ++q) {
+  for (uint32_t idx = 0; idx < nList; ++idx) {
+    const uint32_t q = sliceAt(idx), qNext = idx + 1 < nList ? sliceAt(idx + 1) : q;
 #pragma unroll
     for (int m = 0; m < kPerA; ++m) sA[threadIdx.x + kThreads * m] = pa[m];
 #pragma unroll
@@ -341,7 +382,7 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
     {   // the next slice in flight while this one's MFMAs run (the last iteration reloads its own: no branch); the
         // scheduling barriers keep the loads here, ahead of the MFMAs, and each k-step's fragments after the last's
         // products, so that the two do not share registers
-      const uint32_t step = 128 * (q + 1 < nSlices ? q + 1 : q);
+      const uint32_t step = 128 * qNext;
 #pragma unroll
       for (int m = 0; m < kPerA; ++m) pa[m] = A[aAt(m) + step];
 #pragma unroll
@@ -370,7 +411,7 @@ __global__ __launch_bounds__(128 * WM) __attribute__((amdgpu_waves_per_eu(2))) v
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
-    if ((q + 1) % kBlockSlices == 0 || q + 1 == nSlices) {   // a block of directions done: into the running sum
+    if (qNext / kBlockSlices != q / kBlockSlices || idx + 1 == nList) {   // a block of directions done: into the running sum
 #pragma unroll
       for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
@@ -436,7 +477,7 @@ __global__ void k_ground_rows(const double* __restrict__ sums, uint32_t nc, floa
 // the tallest tile (so that one split serves every tile shape), the column groups to a tile
 struct MfmaShape {
   uint32_t Kp, tilesT, nRG, nCG, Tp, nKS;
-  size_t aBytes, bBytes;
+  size_t aBytes, bBytes, nzBytes;   // the split planes of A and B, then k_split_a's fragment flags [nKS x nRG]
   MfmaShape(uint32_t nc, uint32_t K, uint32_t T, int NS) {
     Kp = (K + kSliceK - 1) / kSliceK * kSliceK;
     tilesT = (T + kTileN - 1) / kTileN;
@@ -446,7 +487,9 @@ struct MfmaShape {
     nKS = Kp / 16;
     aBytes = size_t(NS) * nRG * 32 * Kp * 2;
     bBytes = size_t(NS) * Tp * Kp * 2;
+    nzBytes = (size_t(nKS) * nRG + 255) & ~size_t(255);
   }
+  bool listed() const { return nKS / 2 <= kMaxListed; }
 };
 
 // The contraction's launch shapes: WM waves along the sensors (a tile of 64 WM sensors x 128 steps) and the tile-group
@@ -467,7 +510,8 @@ void launch_mfma(dlg_gpu* g, const MfmaShape& sh, const u32x4* A, const u32x4* B
   k_series_mfma<NS, WM, H><<<(tiles + 7) / 8 * 8, 128 * WM, 0, g->stream>>>(
       A, B, q.nc, sh.nRG, sh.nCG, sh.nKS, q.T, g->bits.as<uint32_t>(), uint32_t(g->bytes), g->N.as<float3>() + g->i0, g->D.as<float3>(),
       reinterpret_cast<const float*>(b + q.oG), g->skyG.as<float>(), q.sun0, reinterpret_cast<const int*>(b + q.oCol),
-      reinterpret_cast<const float*>(b + q.oSw), reinterpret_cast<const uint32_t*>(b + q.oMx), g->seriesE.as<float>(), sh.tilesT, tiles, group);
+      reinterpret_cast<const float*>(b + q.oSw), reinterpret_cast<const uint32_t*>(b + q.oMx),
+      sh.listed() ? g->seriesWork.as<uint8_t>() + sh.aBytes + sh.bBytes : nullptr, g->seriesE.as<float>(), sh.tilesT, tiles, group);
 }
 
 template <int NS, bool H>
@@ -495,7 +539,8 @@ void enqueue_mfma(dlg_gpu* g) {
   }
   k_split_a<NS, H><<<dim3((sh.nKS + kSplitKs - 1) / kSplitKs, sh.nRG), 256, 0, g->stream>>>(g->bits.as<uint32_t>(), uint32_t(g->bytes), q.nc, N, g->D.as<float3>(),
                                                                           q.nSky, sh.nKS, sh.nRG, reinterpret_cast<const float*>(b + q.oW),
-                                                                          g->dc.as<float>(), q.nBins, mx, reinterpret_cast<u32x4*>(A));
+                                                                          g->dc.as<float>(), q.nBins, mx, reinterpret_cast<u32x4*>(A),
+                                                                          g->seriesWork.as<uint8_t>() + sh.aBytes + sh.bBytes);
   k_split_s<NS, H><<<dim3((sh.Kp + 255) / 256, sh.Tp), 256, 0, g->stream>>>(g->skyS.as<float>(), dRow, q.nSky, sh.Kp, q.T, sh.Tp,
                                                                            g->skySdc.as<float>(), q.nBins, mx, B);
   launch_shape<NS, H>(g, sh, reinterpret_cast<const u32x4*>(A), reinterpret_cast<const u32x4*>(B), q.shape);
@@ -610,7 +655,7 @@ int32_t dlg_series_prepare(dlg_gpu* g, int64_t nc, const dlg_series_in* in, int3
   DLG_CHECK(g->seriesE.grow(size_t(nc) * size_t(T) * 4));
   if (method) {
     const MfmaShape sh(uint32_t(nc), in->nSky + q.nBins, uint32_t(T), pieces(method));
-    DLG_CHECK(g->seriesWork.grow(sh.aBytes + sh.bBytes));
+    DLG_CHECK(g->seriesWork.grow(sh.aBytes + sh.bBytes + sh.nzBytes));
     q.shape = known_shape(q);
   }
   g->seriesN = nc;
