@@ -506,6 +506,9 @@ def main():
     kp, tp, np_ = (P + 31) // 32 * 32, (Tup + 127) // 128 * 128, (n_sens + 127) // 128 * 128
     exec_flops = PRODUCTS[DEFAULT_METHOD] * 2.0 * np_ * kp * tp   # FP16x3: three products over the padded tiles
     contraction = {"kernel": "k_series_mfma<2, WM, fp16> (FP16x3, v_mfma_f32_32x32x16_f16)", "bound": "mfma",
+                   "note": "executed = the three products over the padded shape, counting the slices of all-zero coefficients "
+                           "that blocks step over as done (about 12 % of them on this workload; the counted MFMA ops are in "
+                           "profiles/k_series_mfma_city3m_counters.txt); the series stage time includes the operand splits",
                    "executed_tflops": round(exec_flops / (series_ms * 1e-3) / 1e12, 1),
                    "achieved": round(exec_flops / (series_ms * 1e-3) / 1e12, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": round(exec_flops / (series_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),

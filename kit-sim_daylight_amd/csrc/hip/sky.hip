@@ -74,15 +74,24 @@ __global__ void k_steps(int T, const int* __restrict__ month, const int* __restr
     double best[4] = {-1., -1., -1., -1.};
     int bestP[4] = {0, 0, 0, 0};
     const double ca = cos(o.altRad), sv0 = ca * sin(o.aziRad), sv1 = ca * cos(o.aziRad), sv2 = sin(o.altRad);
-    for (int p = 0; p < P; ++p) {
-      const double dp = pd[3 * p] * sv0 + pd[3 * p + 1] * sv1 + pd[3 * p + 2] * sv2;
-      for (int i = 0; i < 4; ++i)
-        if (dp > best[i]) {
-          for (int j = 3; j > i; --j) { best[j] = best[j - 1]; bestP[j] = bestP[j - 1]; }
-          best[i] = dp;
-          bestP[i] = p;
-          break;
-        }
+    constexpr int kRun = 8;   // the patch vectors' loads of a run issued ahead of its ordered insertions
+    for (int p0 = 0; p0 < P; p0 += kRun) {
+      double dps[kRun];
+#pragma unroll
+      for (int k = 0; k < kRun; ++k) {
+        const int p = p0 + k < P ? p0 + k : P - 1;
+        dps[k] = pd[3 * p] * sv0 + pd[3 * p + 1] * sv1 + pd[3 * p + 2] * sv2;
+      }
+      for (int k = 0; k < kRun && p0 + k < P; ++k) {
+        const double dp = dps[k];
+        for (int i = 0; i < 4; ++i)
+          if (dp > best[i]) {
+            for (int j = 3; j > i; --j) { best[j] = best[j - 1]; bestP[j] = bestP[j - 1]; }
+            best[i] = dp;
+            bestP[i] = p0 + k;
+            break;
+          }
+      }
     }
     double w[4], wsum = 0;
     for (int i = 3; i >= 0; --i) wsum += w[i] = 1. / (1.002 - best[i]);
@@ -119,30 +128,29 @@ __global__ void k_lum(int T, int P, const StepOut* __restrict__ so, const float*
   L[size_t(t) * P + p] = o.q.night ? 0.f : perez_patch(o.q, palt[p], pazi[p], o.aziRad);
 }
 
-// A step's normalisation, the sum over its patches in patch order of L tsin omega (sky.cpp's order): a block of 64 steps
-// stages tiles of 64 patches of its steps' rows in LDS (each row's segment one coalesced load), and each lane sums its
-// step's row from there.
-constexpr int kNormT = 64;
-__global__ __launch_bounds__(kNormT) void k_norm(int T, int P, StepOut* __restrict__ so, const float* __restrict__ L, const double* __restrict__ tsinP,
-                                                 const float* __restrict__ omega) {
-  __shared__ float tile[kNormT][kNormT + 1];
-  __shared__ double ws[kNormT];
-  __shared__ float om[kNormT];
-  const int t0 = blockIdx.x * kNormT, t = t0 + int(threadIdx.x);
-  double norm = 0.0;
-  for (int p0 = 0; p0 < P; p0 += kNormT) {
-    const int np = P - p0 < kNormT ? P - p0 : kNormT;
-    __syncthreads();   // the last tile's reads are done
-    for (int r = 0; r < kNormT; ++r)
-      if (t0 + r < T && int(threadIdx.x) < np) tile[r][threadIdx.x] = L[size_t(t0 + r) * P + p0 + threadIdx.x];
-    if (int(threadIdx.x) < np) { ws[threadIdx.x] = tsinP[p0 + threadIdx.x]; om[threadIdx.x] = omega[p0 + threadIdx.x]; }
-    __syncthreads();
-    if (t < T)
-      for (int k = 0; k < np; ++k) norm += tile[threadIdx.x][k] * ws[k] * om[k];
-  }
+// A step's normalisation, the sum over its patches in patch order of L tsin omega (sky.cpp's order): a lane per step,
+// its row read 16 patches at a time with the loads issued ahead of the 16 dependent adds.
+constexpr int kNormRun = 16;
+__global__ void k_norm(int T, int P, StepOut* __restrict__ so, const float* __restrict__ L, const double* __restrict__ tsinP,
+                       const float* __restrict__ omega) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   StepOut& o = so[t];
   if (o.q.night) return;
+  double norm = 0.0;
+  const float* Lt = L + size_t(t) * P;
+  for (int p0 = 0; p0 < P; p0 += kNormRun) {
+    float l[kNormRun], om[kNormRun];
+    double ts[kNormRun];
+#pragma unroll
+    for (int k = 0; k < kNormRun; ++k) {
+      const int p = p0 + k < P ? p0 + k : P - 1;
+      l[k] = Lt[p]; ts[k] = tsinP[p]; om[k] = omega[p];
+    }
+#pragma unroll
+    for (int k = 0; k < kNormRun; ++k)
+      if (p0 + k < P) norm += l[k] * ts[k] * om[k];
+  }
   o.uniform = norm <= 1e-6;
   if (o.uniform) norm = kPi;
   o.sfRad = (o.q.difRad / norm) * (1. / 179.);
@@ -320,7 +328,7 @@ extern "C" int32_t dlg_sky(dlg_gpu* g, const dlg_sky_in* in, dlg_sky_out* out, c
                                                PTR(StepOut, oSo), PTR(double, oAlt), PTR(double, oAzi), PTR(int, oUp), PTR(float, oVec));
   if (T > 0) {
     k_lum<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oPalt), PTR(float, oPazi), PTR(float, oL));
-    k_norm<<<blocks(T, kNormT), kNormT, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oTs), PTR(float, oOm));
+    k_norm<<<blocks(T, 64), 64, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oTs), PTR(float, oOm));
     k_apply<<<dim3(blocks(P, 256), unsigned(T)), 256, 0, g->stream>>>(T, P, PTR(StepOut, oSo), PTR(float, oL), PTR(double, oDt), in->wantRad,
                                                                       in->wantVis, PTR(int, oCol), Tup, seriesMode, PTR(double, oLr),
                                                                       PTR(double, oLv), seriesMode ? g->skyS.as<float>() : nullptr);
