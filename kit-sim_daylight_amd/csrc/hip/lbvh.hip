@@ -166,7 +166,8 @@ __global__ void k_refit(int64_t n, const uint32_t* __restrict__ L, const uint32_
   }
 }
 
-// record n of the 4-wide tree (wide.h): n's children, an inner child replaced by its two children
+// record n of the 4-wide tree (wide.h): n's children, an inner child replaced by its two children; a grandchild whose
+// children are two adjacent leaves becomes a pair slot (both triangles tested from here, no record visit of its own)
 __global__ void k_wide(int64_t nn, const float* __restrict__ nodes, float4* __restrict__ wide) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= nn) return;
@@ -188,7 +189,14 @@ __global__ void k_wide(int64_t nn, const float* __restrict__ nodes, float4* __re
       const float* cn = nodes + 16 * size_t(c);
       for (int s2 = 0; s2 < 2; ++s2) {
         for (int a = 0; a < 3; ++a) { lo[a][k] = cn[6 * s2 + a]; hi[a][k] = cn[6 * s2 + 3 + a]; }
-        rf[k++] = __float_as_uint(cn[12 + s2]);
+        uint32_t g = __float_as_uint(cn[12 + s2]);
+        if (!(g & 0x80000000u)) {   // an inner grandchild: a pair slot when its children are leaves l, l + 1 (l < 2^30 - 1)
+          const float* gn = nodes + 16 * size_t(g);
+          const uint32_t a0 = __float_as_uint(gn[12]), a1 = __float_as_uint(gn[13]);
+          if ((a0 & a1 & 0x80000000u) && (a0 & 0x7fffffffu) < dlw::kPair - 1 && (a1 & 0x7fffffffu) == (a0 & 0x7fffffffu) + 1)
+            g = 0x80000000u | dlw::kPair | (a0 & 0x7fffffffu);
+        }
+        rf[k++] = g;
       }
     }
   }

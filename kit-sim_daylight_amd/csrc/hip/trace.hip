@@ -138,10 +138,19 @@ __device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw:
     dlw::Visit v;
     if (OFF32) visit_buf(bs, cur, sl, 1e-5f, 3.40282347e38f, v);
     else dlw::visit<false>(wide, cur, sl, 1e-5f, 3.40282347e38f, v);
-    // the lane's hit leaves, one triangle test per iteration (lanes with fewer wait less than with a block per slot)
-    for (uint32_t m = v.leafMask; m; m &= m - 1) {
-      const int c = __ffs(m) - 1;
-      const uint32_t leaf = c == 0 ? v.leaf[0] : c == 1 ? v.leaf[1] : c == 2 ? v.leaf[2] : v.leaf[3];
+    // the lane's hit leaves, one triangle test per iteration (lanes with fewer wait less than with a block per slot); a
+    // pair's second triangle is the next iteration's (one test site: the rule's code is not duplicated)
+    uint32_t second = dlw::kEmpty;
+    for (uint32_t m = v.leafMask; m || second != dlw::kEmpty;) {
+      uint32_t leaf = second;
+      if (second != dlw::kEmpty) second = dlw::kEmpty;
+      else {
+        const int c = __ffs(m) - 1;
+        m &= m - 1;
+        const uint32_t ref = c == 0 ? v.leaf[0] : c == 1 ? v.leaf[1] : c == 2 ? v.leaf[2] : v.leaf[3];
+        leaf = dlw::leaf_first(ref);
+        if (ref & dlw::kPair) second = leaf + 1;
+      }
       if (COUNT) ++f.tris;
       if (!hit(leaf)) return false;
     }
@@ -248,7 +257,7 @@ __device__ __forceinline__ void add_traced(uint32_t n, uint32_t lane, unsigned l
 // COUNT: also adds the lane's node visits and triangle tests to counts[0] / counts[1] (the algorithmic fetches) and the
 // traversal iterations the wave executed to counts[2] (SIMD efficiency = node visits / (64 x wave iterations))
 template <int STACK, bool COUNT, bool OFF32>
-__global__ __launch_bounds__(kWave) void k_bits(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6))) void k_bits(const float4* __restrict__ wide, const float4* __restrict__ leaves, int64_t nTris, float pad,
                                                const float3* __restrict__ P, const float3* __restrict__ N, const float3* __restrict__ D,
                                                const float4* __restrict__ dirTab, uint32_t nDir, uint32_t nc, uint32_t cull, uint32_t* __restrict__ bits,
                                                unsigned long long* __restrict__ counts, unsigned long long* __restrict__ traced,

@@ -36,9 +36,18 @@ __device__ Closest closest_hit(const float4* __restrict__ wide, const float4* __
   for (;;) {
     dlw::Visit v;
     dlw::visit<true>(wide, cur, sl, tmin, c.t, v);
-    for (uint32_t m = v.leafMask; m; m &= m - 1) {
-      const int k = __ffs(m) - 1;
-      leaf_test(r, leaves, k == 0 ? v.leaf[0] : k == 1 ? v.leaf[1] : k == 2 ? v.leaf[2] : v.leaf[3], tmin, cull, c);
+    uint32_t second = dlw::kEmpty;   // a pair's second triangle, the next iteration's
+    for (uint32_t m = v.leafMask; m || second != dlw::kEmpty;) {
+      uint32_t leaf = second;
+      if (second != dlw::kEmpty) second = dlw::kEmpty;
+      else {
+        const int k = __ffs(m) - 1;
+        m &= m - 1;
+        const uint32_t ref = k == 0 ? v.leaf[0] : k == 1 ? v.leaf[1] : k == 2 ? v.leaf[2] : v.leaf[3];
+        leaf = dlw::leaf_first(ref);
+        if (ref & dlw::kPair) second = leaf + 1;
+      }
+      leaf_test(r, leaves, leaf, tmin, cull, c);
     }
     // slot 0 holds the nearest hit inner child; a child is taken where it starts within the best hit so far
     if (!(v.tInner[0] <= c.t)) {

@@ -5,7 +5,9 @@
 // children, so that one visit covers two levels of the binary tree (half the dependent fetches per ray; the traversal
 // is bound by their latency, not by bytes). 8 float4 = 128 bytes:
 //   q[0..5]  lo x, lo y, lo z, hi x, hi y, hi z of the 4 slots (SoA: one 16-byte load per coordinate)
-//   q[6]     the 4 references as bits: bit 31 set = leaf (index in the low 31 bits), else the record of an inner node;
+//   q[6]     the 4 references as bits: bit 31 set = leaf, else the record of an inner node; a leaf's index in the low 30
+//            bits, and bit 30 set = a pair: that leaf and the next (an inner node of two adjacent leaves, the two
+//            triangles of a quad in a mesh of quads, tested from this record instead of a record of its own);
 //            0xffffffff = empty slot (its box is empty: lo +inf, hi -inf, never hit)
 //   q[7]     unused
 // Visiting a record: the slot boxes are tested against the ray, hit leaves go to the caller's triangle test, hit inner
@@ -20,6 +22,8 @@ namespace dlw {
 using dlm::Ray;
 
 constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint32_t kPair = 0x40000000u;   // in a leaf reference (Visit::leaf): the leaf and the next one
+DL_HD uint32_t leaf_first(uint32_t ref) { return ref & 0x3fffffffu; }
 
 struct BoxHit { bool hit; float t; };
 
@@ -51,8 +55,8 @@ DL_HD BoxHit slab_hit(const Slab& s, float nx, float ny, float nz, float fx, flo
   return {t0 <= t1, t0};
 }
 
-// One record visit: leaf slots hit (bit c of leafMask, reference leaf[c]) and the inner slots with their entry distance
-// (+inf where not hit), slot 0 the nearest hit one.
+// One record visit: leaf slots hit (bit c of leafMask, reference leaf[c]: leaf_first and kPair) and the inner slots with
+// their entry distance (+inf where not hit), slot 0 the nearest hit one.
 struct Visit {
   uint32_t leafMask;
   uint32_t leaf[4];
