@@ -386,6 +386,12 @@ class Gpu:
                 out["Sdc"] = Sdc[: (P + 1) * out["Tup"]].reshape(P + 1, out["Tup"])
         return out
 
+    def set_sky_series(self, S, g):
+        """the series inputs S [P x Tup] and g [Tup] from the host (dlg_set_sky_series) in place of the last sky's"""
+        S = np.ascontiguousarray(S, np.float32)
+        g = np.ascontiguousarray(g, np.float32)
+        _check(self.L.dlg_set_sky_series(self.h, S.ctypes.data, g.ctypes.data, S.shape[0], S.shape[1], self.err, 1024), self.err)
+
     def series(self, nc, nSky, row, wsky, P, T, G, sun0, sunStep, sunW, method, download=True, reflect=False):
         row = np.ascontiguousarray(row, np.int32)
         wsky = np.ascontiguousarray(wsky, np.float32)

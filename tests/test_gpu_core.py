@@ -279,9 +279,9 @@ def test_sky_suns_weights_and_series_matrix(gpu, orc):
         assert abs(out["gndRad"][0] - gnd) <= 1e-12 * gnd
 
 
-@pytest.mark.parametrize("mf", [2, 4])
-def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc, mf):
-    print("case: series methods, sky density %d" % mf, flush=True)
+def series_case(gpu, orc, mf):
+    """1,500 sensors among a random city, a cloudy clear-sky year at sky density mf: the traced chunk on the device, its
+    inputs to the contraction and the oracle's arguments for the same contraction"""
     rng = np.random.default_rng(5)
     t = city(rng, 200, 120.0)
     nc = 1500
@@ -313,10 +313,22 @@ def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc, mf):
     Wg = np.zeros(nDir, np.float32)
     Wg[Pp:2 * Pp] = wsky
     G = gpu.reduce(nc, Wg, [1])[0].astype(np.float32)
-    E32 = gpu.series(nc, Pp, row, wsky, Pp, Tup, G, 2 * Pp, sunStep, sunW, 0)
     S, g = sky["S"], sky["g"]   # the same S and g on both sides: the contraction alone is compared
+    return dict(nc=nc, Pp=Pp, Tup=Tup, row=row, wsky=wsky, G=G, sunStep=sunStep, sunW=sunW, S=S, g=g,
+                oargs=lambda w, S_, g_, G_=G: (bits, nc, N.reshape(-1), D, Pp, row, w, np.ascontiguousarray(S_, np.float32).reshape(-1), Tup,
+                                               np.ascontiguousarray(g_, np.float32), np.ascontiguousarray(G_, np.float32), 2 * Pp, len(up), sunStep, sunVec.astype(np.float32).reshape(-1),
+                                               sunW))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mf", [2, 4])
+def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc, mf):
+    print("case: series methods, sky density %d" % mf, flush=True)
+    c = series_case(gpu, orc, mf)
+    nc, Pp, Tup, row, wsky, G, sunStep, sunW = (c[k] for k in ("nc", "Pp", "Tup", "row", "wsky", "G", "sunStep", "sunW"))
+    E32 = gpu.series(nc, Pp, row, wsky, Pp, Tup, G, 2 * Pp, sunStep, sunW, 0)
     E32o = np.zeros((nc, Tup), np.float32)
-    args = (bits, nc, N.reshape(-1), D, Pp, row, wsky, np.ascontiguousarray(S).reshape(-1), Tup, np.ascontiguousarray(g), G, 2 * Pp, len(up), sunStep, sunVec.astype(np.float32).reshape(-1), sunW)
+    args = c["oargs"](wsky, c["S"], c["g"])
     orc.lib().orc_series32(*args, E32o.reshape(-1))
     assert np.array_equal(E32, E32o)   # the reference kernel's fma order, bit for bit
     E64 = np.zeros((nc, Tup))
@@ -327,3 +339,28 @@ def test_series_fp32_bit_exact_and_mfma_vs_fp64(gpu, orc, mf):
         err = np.max(np.abs(Em - E64) / scale)
         print("series method %d: max rel error vs FP64 %.3e" % (method, err))
         assert err <= bar
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [40, -40])
+def test_series_split_scales_extreme_magnitudes(gpu, orc, k):
+    """The sky weights and G times 2^k, S and g times 2^-k (the same E, the operands far outside fp16's range unscaled):
+    every MFMA method against the oracle's FP64 series of the same inputs. FP16x3 holds its bar only by scaling each
+    operand by a power of two into fp16's range and back (unscaled, 2^40 x the weights overflows fp16, 2^-40 flushes)."""
+    print("case: series methods, weights x 2^%d, S x 2^%d" % (k, -k), flush=True)
+    c = series_case(gpu, orc, 2)
+    nc, Pp, Tup, row, sunStep, sunW = (c[k_] for k_ in ("nc", "Pp", "Tup", "row", "sunStep", "sunW"))
+    w = (c["wsky"] * np.float32(2.0 ** k)).astype(np.float32)
+    G = (c["G"] * np.float32(2.0 ** k)).astype(np.float32)
+    S = (c["S"] * np.float32(2.0 ** -k)).astype(np.float32)
+    g = (c["g"] * np.float32(2.0 ** -k)).astype(np.float32)
+    gpu.set_sky_series(S, g)
+    E64 = np.zeros((nc, Tup))
+    orc.lib().orc_series64(*c["oargs"](w, S, g, G), E64.reshape(-1), 8)
+    scale = np.maximum(np.abs(E64), 1e-3 * np.abs(E64).max())
+    for method, bar in ((1, 2e-5), (2, 2e-6), (3, 2e-6)):
+        Em = gpu.series(nc, Pp, row, w, Pp, Tup, G, 2 * Pp, sunStep, sunW, method)
+        assert np.all(np.isfinite(Em)), method
+        err = np.max(np.abs(Em - E64) / scale)
+        print("series method %d: max rel error vs FP64 %.3e" % (method, err))
+        assert err <= bar, method
