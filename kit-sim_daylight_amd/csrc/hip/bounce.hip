@@ -46,9 +46,9 @@ DL_HD float3 mul3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * 
 
 // One path: sample s of global sensor gi (normal n, origin o0), the rotation (r0, r1) of its sample pattern. True when
 // it reflected and escaped, with its bin and weight.
-template <int STACK>
+template <int STACK, bool OFF32>
 __device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3 o0, uint32_t gi, uint32_t s, float r0, float r1,
-                                           dlc::lds_u32* __restrict__ stk, int lane, uint32_t& bin, float& val) {
+                                           const dls::LaneStack<STACK, STACK>& stk, int lane, uint32_t& bin, float& val) {
   const bool cull = a.cull != 0;
   const float u = float(s) / float(a.M) + r0, v = u01(brev32(s)) + r1;
   float3 o = o0, d = cosine_dir(n, u - floorf(u), v - floorf(v));
@@ -59,7 +59,7 @@ __device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3
     Ray ray;
     ray_setup(ray, o, d, a.pad);
     Closest c{};
-    if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, 1e-5f, 3.40282347e38f, cull, stk, lane);
+    if (a.nTris > 0) c = closest_hit<OFF32>(a.wide, a.leaves, ray, 1e-5f, 3.40282347e38f, cull, stk, lane);
     if (!c.hit) {
       if (!reflected) return false;
       bin = sky_bin(d, a.mf, a.alpha, a.nBins, a.cosN, a.sinN);
@@ -115,10 +115,12 @@ __device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3
 // A wave per sensor of [c0, c0 + gridDim.x): lane l traces samples l, l + 64, ... (rounds of 64 from one origin, so the
 // top of the tree is shared); each round's escaping paths add to the sensor's row in LDS in lane order, i.e. in sample
 // order, which keeps every bin's float sum that of one thread walking the samples in turn; the row is written at the end.
-template <int STACK>
+// OFF32: fewer than 2^25 records (wide.h visit)
+template <int STACK, bool OFF32>
 // 3 waves per SIMD (168 VGPRs, a few spilled) trace 17 % faster than the 202 VGPRs of 2 waves
 __global__ __launch_bounds__(kWave, 3) void k_bounce(BounceArgs a, uint32_t c0) {
-  __shared__ uint32_t stk[STACK * kWave];
+  __shared__ uint32_t stkLds[STACK * kWave];
+  const dls::LaneStack<STACK, STACK> stk{(dls::lds_u32*)stkLds, nullptr};   // the whole stack in LDS
   extern __shared__ float sRow[];   // [nBins]
   const uint32_t lane = threadIdx.x, i = c0 + blockIdx.x;
   if (i >= a.nc) return;   // whole blocks
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(kWave, 3) void k_bounce(BounceArgs a, uint32_t c0) 
     const uint32_t s = s0 + lane;
     uint32_t bin = 0;
     float val = 0.f;
-    const bool has = s < a.M && trace_path<STACK>(a, n, o0, gi, s, r0, r1, (dlc::lds_u32*)stk, int(lane), bin, val);
+    const bool has = s < a.M && trace_path<STACK, OFF32>(a, n, o0, gi, s, r0, r1, stk, int(lane), bin, val);
     // in lane order: one lane at a time adds its value (a bin's sum is the sequential one)
     for (uint64_t m = __ballot(has); m; m &= m - 1) {
       const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);
@@ -147,7 +149,8 @@ __global__ __launch_bounds__(kWave, 3) void k_bounce(BounceArgs a, uint32_t c0) 
 
 template <int STACK>
 void launch_bounce(dlg_gpu* g, const BounceArgs& a, uint32_t c0, uint32_t n) {
-  k_bounce<STACK><<<n, kWave, a.nBins * 4, g->stream>>>(a, c0);
+  if (g->scene.nNodes < (int64_t(1) << 25)) k_bounce<STACK, true><<<n, kWave, a.nBins * 4, g->stream>>>(a, c0);
+  else k_bounce<STACK, false><<<n, kWave, a.nBins * 4, g->stream>>>(a, c0);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_dc(const float* __restrict__ dc, uint32_t nc, uint32_t nBins, const float* __restrict__ W,

@@ -71,7 +71,7 @@ void dlg_destroy(dlg_gpu* g) {
   dlg_graph_free(g);
   g->scene.release();
   for (DevBuf* b : {&g->P, &g->N, &g->D, &g->dirTab, &g->bits, &g->scratch, &g->out, &g->skyS,
-                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->sensWork, &g->slots, &g->dc, &g->skySdc,
+                    &g->skyG, &g->skyWork, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->viewOvf, &g->labels, &g->counts, &g->traceCount, &g->stackOvf, &g->tileNext, &g->sensWork, &g->slots, &g->dc, &g->skySdc,
                     &g->meshV, &g->meshI, &g->meshS, &g->meshWord, &g->meshWork, &g->tileOrder, &g->orderWork})
     b->release();
   if (g->hostSums) (void)hipHostFree(g->hostSums);
@@ -113,7 +113,7 @@ void dlg_trim(dlg_gpu* g, int32_t sensors) {
   if (g->hostSums) (void)hipHostFree(g->hostSums);
   g->hostSums = nullptr;
   g->hostSumsBytes = 0;
-  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->dc, &g->stackOvf, &g->sensWork, &g->meshV, &g->meshI, &g->meshS, &g->meshWork,
+  for (DevBuf* b : {&g->bits, &g->scratch, &g->out, &g->seriesE, &g->seriesWork, &g->seriesIn, &g->view, &g->viewOvf, &g->dc, &g->stackOvf, &g->sensWork, &g->meshV, &g->meshI, &g->meshS, &g->meshWork,
                     &g->tileOrder, &g->orderWork}) b->release();
   g->orderGen = -1;
   g->meshOpen = 0;

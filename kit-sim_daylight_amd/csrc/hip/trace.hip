@@ -22,6 +22,7 @@
 
 #include "dl_math.h"
 #include "gpu_internal.h"
+#include "lanestack.h"
 #include "wide.h"
 
 namespace {
@@ -54,46 +55,11 @@ struct OwnSet {
 // iterations its wave executed
 struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
 
-// The traversal stack of a lane, STACK entries: [0, kL) in LDS (lds[depth][lane]), [kL, STACK) in the wave's area of a
-// global buffer (ovf[depth - kL][lane]). A lane reads back only what it wrote itself, so no wave waits on another.
-constexpr int kLdsStack = 24;
-// LDS-typed pointers: the stack's LDS accesses stay ds_read / ds_write (through a generic pointer the compiler may
-// merge an LDS and a global access of the two paths into one flat access, which takes the vector-memory path)
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
-template <int STACK>
-struct LaneStack {
-  static constexpr int kL = STACK < kLdsStack ? STACK : kLdsStack;
-  static constexpr bool kDeep = STACK > kL;
-  lds_u32* lds;    // [kL][64]
-  uint32_t* ovf;   // [STACK - kL][64] of this wave (unused unless kDeep)
-  __device__ __forceinline__ void put(int k, int lane, uint32_t v) const {
-    if (!kDeep || k < kL) lds[k * kWave + lane] = v;
-    else ovf[(k - kL) * kWave + lane] = v;
-  }
-  // the three pushes of a visit, at sp, sp + h1, sp + h1 + h2 (all below sp + 3); LDS alone while every lane is low
-  __device__ __forceinline__ void push3(int sp, int h1, int h2, int lane, uint32_t a, uint32_t b, uint32_t c) const {
-    if (!kDeep || __ballot(sp + 2 >= kL) == 0) {
-      lds[sp * kWave + lane] = a;
-      lds[(sp + h1) * kWave + lane] = b;
-      lds[(sp + h1 + h2) * kWave + lane] = c;
-    } else {
-      put(sp, lane, a);
-      put(sp + h1, lane, b);
-      put(sp + h1 + h2, lane, c);
-    }
-  }
-  __device__ __forceinline__ uint32_t at(int k, int lane) const {
-    if (!kDeep || __ballot(k >= kL) == 0) return lds[k * kWave + lane];
-    uint32_t v;
-    if (k < kL) v = lds[k * kWave + lane];
-    else v = ovf[(k - kL) * kWave + lane];
-    return v;
-  }
-};
-template <int STACK>
-constexpr int lds_stack() { return LaneStack<STACK>::kL; }
-template <int STACK>
-constexpr int ovf_stack() { return STACK - LaneStack<STACK>::kL; }
+// the lanes' traversal stacks: the top entries in LDS, the deeper ones in the wave's area of a global buffer
+using dls::LaneStack;
+using dls::lds_stack;
+using dls::lds_u32;
+using dls::ovf_stack;
 
 // A ray's record fetches through a buffer descriptor (fewer than 2^25 records: byte offsets below 2^32). Each of a
 // visit's six plane float4 is one buffer load at the record's byte offset n * 128 (the only per-lane address) plus
@@ -445,22 +411,6 @@ const void* trace_kernel(bool bytes, bool count, bool off32) {
   return off32 ? reinterpret_cast<const void*>(&k_bits<STACK, false, true>) : reinterpret_cast<const void*>(&k_bits<STACK, false, false>);
 }
 
-// the one-wave blocks of kernel fn a CU holds at once (asked once per kernel)
-uint32_t blocks_per_cu(const void* fn) {
-  static std::mutex mu;
-  static std::vector<std::pair<const void*, int>> known;
-  std::lock_guard<std::mutex> lk(mu);
-  for (const auto& [f, n] : known)
-    if (f == fn) return uint32_t(n);
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWave, 0) != hipSuccess || n < 1) {
-    (void)hipGetLastError();
-    n = 4;
-  }
-  known.push_back({fn, n});
-  return uint32_t(n);
-}
-
 template <int STACK>
 void launch_bytes(dlg_gpu* g, const Tiles& tl, const TraceBufs& b, uint32_t cull) {
   const dlg_scene& s = g->scene;
@@ -494,6 +444,21 @@ void launch_trace(dlg_gpu* g, const TraceBufs& b, uint32_t rayFlags, int stack, 
 
 }  // namespace
 
+uint32_t dlg_blocks_per_cu(const void* fn) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> known;
+  std::lock_guard<std::mutex> lk(mu);
+  for (const auto& [f, n] : known)
+    if (f == fn) return uint32_t(n);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWave, 0) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    n = 4;
+  }
+  known.push_back({fn, n});
+  return uint32_t(n);
+}
+
 // The checks and buffers of a trace of sensors [i0, i0 + nc): 0, or 1 with err. stack: the LDS stack entries the
 // scene's tree needs (the smallest instantiation that covers it, with the two entries above the top that the
 // unconditional pushes write).
@@ -516,7 +481,7 @@ dlg_trace_shape dlg_trace_grid(const dlg_gpu* g, int64_t nc, int stack, bool byt
   const uint64_t nTiles = uint64_t((nc + kWave - 1) / kWave) * uint64_t(g->nWords);
   with_stack(stack, [&](auto S) {
     constexpr int ST = decltype(S)::value;
-    uint64_t grid = uint64_t(blocks_per_cu(trace_kernel<ST>(bytes, count, g->scene.nNodes < (int64_t(1) << 25)))) * uint64_t(g->cus > 0 ? g->cus : 1);
+    uint64_t grid = uint64_t(dlg_blocks_per_cu(trace_kernel<ST>(bytes, count, g->scene.nNodes < (int64_t(1) << 25)))) * uint64_t(g->cus > 0 ? g->cus : 1);
     grid = grid < 8 ? 8 : grid / 8 * 8;   // whole rounds of the 8 XCDs
     const uint64_t need = (nTiles + 7) / 8 * 8;
     t.grid = uint32_t(need < grid ? (need < 8 ? 8 : need) : grid);

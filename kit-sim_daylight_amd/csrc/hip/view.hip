@@ -1,9 +1,10 @@
 // SPDX-License-Identifier: Apache-2.0 AND LicenseRef-Commons-Clause-1.0
 // Views on CDNA4: replaces __raygen__view (reference src/trace_dev.cu:177-232), gpu_view (gpu.cu:572-610), k_view /
 // k_iso / gpu_reduce_view (gpu.cu:171-274, 671-697).
-//   k_view_rays  a wave per sensor and 8 x 8 tile of cells (grid y, 4 tiles a block): frame, jittered cell, virtual ground,
-//                closest hit through the 4-wide records of the LBVH (exact ties by the lower triangle index), the sensor's own planes passed
-//                (at most 3; the 4th an obstruction there), the hit triangle's label. Sensor-major cls / dist [nc x K].
+//   k_view_rays  persistent waves, each taking (sensor, group of 4 tiles of 8 x 8 cells) items in turn: frame, jittered
+//                cell, virtual ground, closest hit through the 4-wide records of the tree (exact ties by the lower
+//                triangle index), the sensor's own planes passed (at most 3; the 4th an obstruction there), the hit
+//                triangle's label. Sensor-major cls / dist [nc x K].
 //   k_view_sums  a wave per sensor, lane l sums directions l, l + 64, ... (class weights, weight x ln d by a
 //                table-driven FP64 log, minima, own surface, reach, glazing), then a fixed shuffle tree (s = 32 .. 1):
 //                the same bits for any chunking.
@@ -17,6 +18,7 @@
 #include "dl_math.h"
 #include "gpu_internal.h"
 #include "closest.h"
+#include "lanestack.h"
 #include "wide.h"
 
 namespace {
@@ -35,15 +37,19 @@ struct ViewArgs {
   uint32_t nAz, nEl, nc, i0, nS, back, world, cull, jitter, vground, cullBack;
   float az0, dAz, groundZ, cosN, sinN;
   uint8_t* cls; float* dist;
+  uint32_t nTG, nItems;   // tile groups per sensor, sensors x tile groups
+  uint32_t* ovf;          // the stacks' global part, [gridDim.x][STACK - LDS part][64]
 };
 
-// A block (one wave) per sensor and group of kTiles cell tiles: lane l takes cell (row, col) of an 8 x 8 tile (1 x 64 on
-// a flat field), so a wave's rays leave one origin in neighbouring directions and meet the same part of the tree.
+// Work items: a sensor and a group of kTiles cell tiles; lane l takes cell (row, col) of an 8 x 8 tile (1 x 64 on a flat
+// field), so a wave's rays leave one origin in neighbouring directions and meet the same part of the tree. The grid is
+// persistent (as many one-wave blocks as the CUs hold): block b takes items b, b + gridDim.x, ..., sensor-major, and
+// keeps the top of its lanes' stacks in LDS and the rest in its own area of a global buffer (lanestack.h), so that a
+// wave's LDS stays small and the waves per SIMD are what the registers allow.
 constexpr uint32_t kTiles = 4;
-template <int STACK>
-__global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
-  __shared__ uint32_t stk[STACK * kWave];
-  const uint32_t lane = threadIdx.x, i = blockIdx.x;
+// the rays of sensor i's tile group tg
+template <int STACK, bool OFF32>
+__device__ __forceinline__ void view_item(const ViewArgs& a, uint32_t i, uint32_t tg, uint32_t lane, const dls::LaneStack<STACK>& stk) {
   const uint32_t gi = a.i0 + i, element = gi / a.nS, side = a.nS == 2 ? gi % 2 : a.back;
   const float3 n = a.N[i];
   const float3 o = offset_ray(a.P[i], n);
@@ -68,7 +74,7 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
   }
   const uint32_t K = a.nAz * a.nEl, th = a.nEl > 1 ? 8u : 1u, tw = kWave / th, tilesAz = (a.nAz + tw - 1) / tw;
   for (uint32_t tk = 0; tk < kTiles; ++tk) {
-    const uint32_t tile = blockIdx.y * kTiles + tk, row = (tile / tilesAz) * th + lane / tw, col = (tile % tilesAz) * tw + lane % tw;
+    const uint32_t tile = tg * kTiles + tk, row = (tile / tilesAz) * th + lane / tw, col = (tile % tilesAz) * tw + lane % tw;
     if (row >= a.nEl || col >= a.nAz) continue;
     const uint32_t d = row * a.nAz + col;
     const float3 lc = view_cell(d, col, a.rowSin[2 * row], a.rowSin[2 * row + 1], a.az0, a.dAz, a.jitter != 0, element, side);
@@ -87,7 +93,7 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
       uint32_t own = 0, panes = 0;
       for (;;) {
         Closest c{};
-        if (a.nTris > 0) c = closest_hit<STACK>(a.wide, a.leaves, ray, tmin, tmax, a.cullBack != 0, (dlc::lds_u32*)stk, int(lane));
+        if (a.nTris > 0) c = closest_hit<OFF32>(a.wide, a.leaves, ray, tmin, tmax, a.cullBack != 0, stk, int(lane));
         if (!c.hit) { cl = dir.z >= 0.f ? 0 : 1; t = tg; break; }
         t = c.t;
         const float4 A = a.leaves[3 * c.leaf], B = a.leaves[3 * c.leaf + 1], E = a.leaves[3 * c.leaf + 2];
@@ -106,6 +112,17 @@ __global__ __launch_bounds__(kWave) void k_view_rays(ViewArgs a) {
     a.cls[at] = cl;
     a.dist[at] = t;
   }
+}
+
+// OFF32: fewer than 2^25 records (wide.h visit); 5 waves per SIMD under a register bound (96 registers, a few spilled)
+// trace 7 % faster than the 4 that 111 registers allow
+template <int STACK, bool OFF32>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(5))) void k_view_rays(ViewArgs a) {
+  __shared__ uint32_t lds[dls::lds_stack<STACK>() * kWave];
+  const dls::LaneStack<STACK> stk{(dls::lds_u32*)lds, a.ovf + size_t(blockIdx.x) * dls::ovf_stack<STACK>() * kWave};
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t item = blockIdx.x; item < a.nItems; item += gridDim.x)
+    view_item<STACK, OFF32>(a, item / a.nTG, item % a.nTG, lane, stk);
 }
 
 __device__ __forceinline__ double tree_sum(double v) {
@@ -339,18 +356,37 @@ extern "C" int32_t dlg_view(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlag
   a.cls = reinterpret_cast<uint8_t*>(b); a.dist = reinterpret_cast<float*>(b + oDist);
   const uint32_t th = v->nEl > 1 ? 8u : 1u, tw = uint32_t(kWave) / th;
   const size_t tiles = size_t((v->nEl + th - 1) / th) * ((v->nAz + tw - 1) / tw);
-  const dim3 grid(unsigned(nc), unsigned((tiles + kTiles - 1) / kTiles));
-  dlg_tbegin(g);
-  // the smallest LDS stack that covers the tree (a smaller stack lets more waves share a CU), with the entries above
-  // the top that the unconditional pushes write
+  const size_t nTG = (tiles + kTiles - 1) / kTiles, nItems = size_t(nc) * nTG;
+  if (nItems > size_t(0xffffffffu)) { dlg_fail(err, errLen, "dlg_view: too many sensors x cell tiles in one chunk"); return 1; }
+  a.nTG = uint32_t(nTG);
+  a.nItems = uint32_t(nItems);
+  // the smallest stack that covers the tree, with the entries above the top that the unconditional pushes write; the
+  // persistent grid (whole rounds of the 8 XCDs) and each block's global stack area
   const int need = dlw::wide_stack(s.height) + 3;
-  if (need <= 24) k_view_rays<24><<<grid, kWave, 0, g->stream>>>(a);
-  else if (need <= 32) k_view_rays<32><<<grid, kWave, 0, g->stream>>>(a);
-  else if (need <= 48) k_view_rays<48><<<grid, kWave, 0, g->stream>>>(a);
-  else if (need <= 64) k_view_rays<64><<<grid, kWave, 0, g->stream>>>(a);
-  else if (need <= 96) k_view_rays<96><<<grid, kWave, 0, g->stream>>>(a);
-  else if (need <= 160) k_view_rays<160><<<grid, kWave, 0, g->stream>>>(a);
+  const auto launch = [&](auto S) -> int32_t {
+    constexpr int ST = decltype(S)::value;
+    const bool off32 = s.nNodes < (int64_t(1) << 25);
+    const void* fn = off32 ? reinterpret_cast<const void*>(&k_view_rays<ST, true>) : reinterpret_cast<const void*>(&k_view_rays<ST, false>);
+    uint64_t grid = uint64_t(dlg_blocks_per_cu(fn)) * uint64_t(g->cus > 0 ? g->cus : 1);
+    grid = grid < 8 ? 8 : grid / 8 * 8;
+    if (grid > nItems) grid = nItems;
+    const size_t ovf = size_t(dls::ovf_stack<ST>()) * kWave * 4;
+    if (ovf) DLG_CHECK(g->viewOvf.grow(ovf * grid));
+    a.ovf = g->viewOvf.as<uint32_t>();
+    dlg_tbegin(g);
+    if (off32) k_view_rays<ST, true><<<unsigned(grid), kWave, 0, g->stream>>>(a);
+    else k_view_rays<ST, false><<<unsigned(grid), kWave, 0, g->stream>>>(a);
+    return 0;
+  };
+  int32_t rc = 0;
+  if (need <= 24) rc = launch(std::integral_constant<int, 24>{});
+  else if (need <= 32) rc = launch(std::integral_constant<int, 32>{});
+  else if (need <= 48) rc = launch(std::integral_constant<int, 48>{});
+  else if (need <= 64) rc = launch(std::integral_constant<int, 64>{});
+  else if (need <= 96) rc = launch(std::integral_constant<int, 96>{});
+  else if (need <= 160) rc = launch(std::integral_constant<int, 160>{});
   else { dlg_fail(err, errLen, "dlg_view: the scene's tree is too deep for the traversal stack"); return 1; }
+  if (rc) return rc;
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   g->stats.viewMs += ms;

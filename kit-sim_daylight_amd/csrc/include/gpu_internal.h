@@ -95,8 +95,9 @@ struct dlg_gpu {
   int64_t seriesN = 0;
   dlg_series_plan seriesPlan;
   std::vector<int> seriesCols;
-  // views: classes | distances | row sines of the last dlg_view
-  DevBuf view, labels;
+  // views: classes | distances | row sines of the last dlg_view; the view rays' stack entries past the LDS part, one
+  // area per block of their persistent grid
+  DevBuf view, labels, viewOvf;
   int64_t viewN = 0, viewK = 0, viewI0 = 0;
   dlg_view_in viewIn{};
   dlg_stats_t stats{};
@@ -149,6 +150,8 @@ int64_t dlg_sah_bytes(int64_t nt);
 // what a capture can hold (kernel launches on g->stream, no waits). trace.hip:
 int32_t dlg_trace_prepare(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, int& stack, char* err, int32_t errLen);
 void dlg_trace_enqueue(dlg_gpu* g, uint32_t rayFlags, int stack, unsigned long long* counts);
+// the one-wave blocks of kernel fn a CU holds at once (the occupancy API, asked once per kernel)
+uint32_t dlg_blocks_per_cu(const void* fn);
 // the checks of a trace with these flags against the attached scene and the slots, and its stack size (no buffers)
 int32_t dlg_trace_check(const dlg_gpu* g, uint32_t rayFlags, int& stack, char* err, int32_t errLen);
 // the persistent grid of a trace (one-wave blocks, a multiple of 8) and the bytes of each block's global stack area

@@ -103,9 +103,19 @@ DL_HD void decide(float4 nx, float4 ny, float4 nz, float4 fx, float4 fy, float4 
   v.tInner[0] = k0; v.tInner[1] = k1; v.tInner[2] = k2; v.tInner[3] = k3;
 }
 
-// record n, its planes read by the ray's own signs (any lane its own direction)
-template <bool FULL>
+// record n, its planes read by the ray's own signs (any lane its own direction). OFF32: fewer than 2^25 records, so a
+// plane's byte offset n * 128 + 16 k fits 32 bits: each load is the records' base plus one 32-bit offset (the same
+// addresses; no 64-bit address arithmetic per plane)
+template <bool FULL, bool OFF32 = false>
 DL_HD void visit(const float4* __restrict__ wide, uint32_t n, const Slab& sl, float tmin, float tmax, Visit& v) {
+  if (OFF32) {
+    const char* b = reinterpret_cast<const char*>(wide);
+    const uint32_t o = n * 128u;
+    const auto at = [&](int k) { return *reinterpret_cast<const float4*>(b + (o + uint32_t(k) * 16u)); };
+    decide<FULL>(at(sl.nOff[0]), at(sl.nOff[1]), at(sl.nOff[2]), at(3 - sl.nOff[0]), at(5 - sl.nOff[1]), at(7 - sl.nOff[2]),
+                 *reinterpret_cast<const uint4*>(b + (o + 96u)), sl, tmin, tmax, v);
+    return;
+  }
   const float4* q = wide + 8 * size_t(n);
   decide<FULL>(q[sl.nOff[0]], q[sl.nOff[1]], q[sl.nOff[2]], q[3 - sl.nOff[0]], q[5 - sl.nOff[1]], q[7 - sl.nOff[2]],
                *reinterpret_cast<const uint4*>(q + 6), sl, tmin, tmax, v);   // axis a's far planes: 2a + 3 - nOff[a]
