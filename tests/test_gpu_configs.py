@@ -9,6 +9,9 @@
 - Deep trees: the 2M-triangle interior (binned SAH, height 29: the 48-entry traversal stack) and the 3M-triangle city
   (LBVH, height 32: the 64-entry stack), visibility bits and transmittance bytes of a few hundred sensors looking every
   way, bit for bit.
+- Deep stacks: a comb of 2^20 triangles whose boxes a ray along it crosses without a hit, so that the walks' stacks go
+  past the entries kept in LDS into their global part: bits and bytes (SAH and LBVH trees) and the views' classes and
+  distances, bit for bit.
 - The interior seen all around (360 x 120 degrees in 2-degree cells, jitter, isovist, reach) from 200 floor sensors:
   every ray's class and distance bit for bit, the columns as the views tests compare them.
 """
@@ -167,31 +170,21 @@ def _looking_everywhere(rng, lo, hi, n):
     return P, N.astype(np.float32)
 
 
-@pytest.mark.parametrize("scene,tree,stack", [("interior", "sah", 48), ("city3m", "lbvh", 64)])
-def test_deep_tree_bits_and_bytes(orc, city3m, scene, tree, stack):
+def _bits_and_bytes_vs_oracle(orc, V, I, P, N, D, tree, check_stack=None):
+    """Visibility bits, then byte mode with a porous, a glass and an opaque slot spread over the triangles, of sensors P, N
+    along directions D over the tree `tree` of triangles V, I: every bit and byte against the oracle. check_stack(height,
+    need) sees the tree's height and the any-hit stack it needs."""
     import daylight_amd
-    rng = np.random.default_rng(7)
-    if scene == "interior":
-        V, I = _soup(W.interior())
-        P, N = _looking_everywhere(rng, (5.0, 5.0, 0.2), (150.0, 150.0, 2.8), 256)
-    else:
-        V, I = _soup(city3m[0])
-        P, N = _looking_everywhere(rng, (-400.0, -400.0, 0.5), (400.0, 400.0, 40.0), 256)
-    nt = len(I) // 3
-    pa = orc.patches(1)
-    extra = rng.normal(size=(40, 3))
-    extra[:, 2] = np.abs(extra[:, 2]) + 0.05
-    extra /= np.linalg.norm(extra, axis=1, keepdims=True)
-    D = np.concatenate([pa["centre"], -pa["centre"], extra]).astype(np.float32).reshape(-1)
-    nc, nDir = len(P), len(D) // 3
+    nt, nc, nDir = len(I) // 3, len(P), len(D) // 3
     g = daylight_amd.Gpu()
     try:
         g.set_tree(tree)
         g.build(V, I)
         height = g.scene()[2]
         need = 3 * ((height + 1) // 2) + 1 + 2
-        print("case: %s, %d triangles, %s tree of height %d: stack %d entries" % (scene, nt, tree, height, stack), flush=True)
-        assert need <= stack and (stack == 48 and need > 32 or stack == 64 and need > 48), (height, need)
+        print("case: %d triangles, %s tree of height %d: stack of %d entries" % (nt, tree, height, need), flush=True)
+        if check_stack:
+            check_stack(height, need)
         g.set_scene(P.reshape(-1), N.reshape(-1), D)
         g.trace(0, nc)
         got = g.bits(nc)
@@ -201,7 +194,6 @@ def test_deep_tree_bits_and_bytes(orc, city3m, scene, tree, stack):
         gm, wm = bits_matrix(got, nc, nDir), bits_matrix(want, nc, nDir)
         print("bits: %d open of %d rays" % (wm.sum(), wm.size), flush=True)
         assert np.array_equal(gm, wm), np.argwhere(gm != wm)[:8]
-        # byte mode on the same tree: a porous, a glass and an opaque slot spread over the triangles
         tn = float(np.float32(orc.lib().orc_tn_from_Tn(0.64)))
         slots = [(0.2, 0.0, 0.0, 0), (0.2, 0.5, 0.0, 0), (0.0, 0.64, tn, GLASS)]
         ts = (np.arange(nt, dtype=np.uint32) // 7) % 3
@@ -220,6 +212,79 @@ def test_deep_tree_bits_and_bytes(orc, city3m, scene, tree, stack):
         assert len(bad) == 0, (len(bad), [(int(d), int(i), int(by[d, i]), int(want_b[d, i])) for d, i in bad[:8]])
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("scene,tree,stack", [("interior", "sah", 48), ("city3m", "lbvh", 64)])
+def test_deep_tree_bits_and_bytes(orc, city3m, scene, tree, stack):
+    rng = np.random.default_rng(7)
+    if scene == "interior":
+        V, I = _soup(W.interior())
+        P, N = _looking_everywhere(rng, (5.0, 5.0, 0.2), (150.0, 150.0, 2.8), 256)
+    else:
+        V, I = _soup(city3m[0])
+        P, N = _looking_everywhere(rng, (-400.0, -400.0, 0.5), (400.0, 400.0, 40.0), 256)
+    pa = orc.patches(1)
+    extra = rng.normal(size=(40, 3))
+    extra[:, 2] = np.abs(extra[:, 2]) + 0.05
+    extra /= np.linalg.norm(extra, axis=1, keepdims=True)
+    D = np.concatenate([pa["centre"], -pa["centre"], extra]).astype(np.float32).reshape(-1)
+
+    def check(height, need):
+        assert need <= stack and (stack == 48 and need > 32 or stack == 64 and need > 48), (height, need)
+    _bits_and_bytes_vs_oracle(orc, V, I, P, N, D, tree, check)
+
+
+COMB_H = 1.0e7
+
+
+def _comb(lg=20):
+    """2^lg triangles along x, triangle k over [k, k + 0.9] x [0, H]^2 in the plane y = z (H = 10^7 m). A ray from x < 0
+    with a positive x component, started far from y = z, passes through every box and meets no triangle, so a
+    depth-first walk pushes the hit siblings at every record level: about 30 entries on these trees, past the 24 that
+    each lane keeps in LDS, so the walks' global stack part is written and read (a few hundred thousand entries in
+    these cases, where the other scenes of these tests reach it a few times at most)."""
+    nt = 1 << lg
+    k = np.arange(nt, dtype=np.float64)
+    z = np.zeros(nt)
+    V = np.stack([np.stack([k, z, z], 1), np.stack([k + 0.9, z, z], 1), np.stack([k, z + COMB_H, z + COMB_H], 1)], 1)
+    return V.astype(np.float32).reshape(-1), np.arange(3 * nt, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("tree", ["sah", "lbvh"])
+def test_deep_stacks_comb_bits_and_bytes(orc, tree):
+    rng = np.random.default_rng(11)
+    V, I = _comb()
+    P = np.stack([np.full(2, -5.0), rng.uniform(0.6, 0.9, 2) * COMB_H, rng.uniform(0.1, 0.4, 2) * COMB_H], 1).astype(np.float32)
+    N = np.tile(np.array([1.0, 0.0, 0.0], np.float32), (2, 1))
+    D = rng.normal(size=(24, 3))
+    D[:16] = D[:16] * [0.2, 0.2, 0.2] + [1.0, 0.0, 0.0]   # 16 of them within a few degrees of +x
+    D /= np.linalg.norm(D, axis=1, keepdims=True)
+    _bits_and_bytes_vs_oracle(orc, V, I, P, N, D.astype(np.float32).reshape(-1), tree)
+
+
+def test_deep_stacks_comb_views(native, orc, tmp_path):
+    V, I = _comb()
+    nt = len(I) // 3
+    pts = V.reshape(-1, 3).astype(np.float64)
+    cnt, idx = np.full(nt, 3, np.int32), np.arange(3 * nt, dtype=np.int32)
+    y0, z0 = 0.7 * COMB_H, 0.2 * COMB_H
+    gp = np.array([[-5.0, y0, z0], [-5.0, y0 + 1, z0], [-5.0, y0 + 1, z0 + 1], [-5.0, y0, z0 + 1]])   # one quad facing +x
+    gc, gi = np.array([4], np.int32), np.arange(4, dtype=np.int32)
+    print("case: views from one quad along a comb of %d triangles, 360 x 120 degrees in 15-degree cells" % nt, flush=True)
+    dump = str(tmp_path / "rays.npy")
+    r = native.run("views", ["--fov", "360", "--cell", "15", "--dump-rays", dump], native.SimulationMesh("grid", gp, gc, gi),
+                   context=native.SimulationMesh("comb", pts, cnt, idx))
+    o = orc.views([orc.MeshArrays("grid", gp, gc, gi, sensor=True), orc.MeshArrays("comb", pts, cnt, idx, sensor=False)],
+                  fov=(360.0, -70.0, 50.0), cell=15.0, raw=True, nthreads=min(16, os.cpu_count() or 1))
+    assert r.manifest["columns"] == o["columns"]
+    cls = np.load(dump)
+    assert cls.shape == o["cls"].shape
+    bad = np.argwhere(cls != o["cls"])
+    assert len(bad) == 0, (len(bad), bad[:8])
+    assert np.array_equal(np.load(dump + ".depth.npy").view(np.uint32), o["dist"].view(np.uint32))
+    from test_gpu_views import compare
+    compare(r.values, o["values"], o["columns"])
+    print("classes: sky %d, ground %d, obstruction %d" % tuple((cls == c).sum() for c in range(3)), flush=True)
 
 
 def test_interior_views_slice_vs_oracle(native, orc, tmp_path):
