@@ -9,9 +9,9 @@
 - Deep trees: the 2M-triangle interior (binned SAH, height 29: the 48-entry traversal stack) and the 3M-triangle city
   (LBVH, height 32: the 64-entry stack), visibility bits and transmittance bytes of a few hundred sensors looking every
   way, bit for bit.
-- Deep stacks: a comb of 2^20 triangles whose boxes a ray along it crosses without a hit, so that the walks' stacks go
-  past the entries kept in LDS into their global part: bits and bytes (SAH and LBVH trees) and the views' classes and
-  distances, bit for bit.
+- Deep stacks: a comb of 2^20 triangles whose boxes a ray along it crosses without a hit, and a wall across it found
+  only through the entries the walk pops back from past the ones kept in LDS: bits and bytes (SAH and LBVH trees) and
+  the views' classes and distances, bit for bit.
 - The interior seen all around (360 x 120 degrees in 2-degree cells, jitter, isovist, reach) from 200 floor sensors:
   every ray's class and distance bit for bit, the columns as the views tests compare them.
 """
@@ -238,26 +238,41 @@ COMB_H = 1.0e7
 
 
 def _comb(lg=20):
-    """2^lg triangles along x, triangle k over [k, k + 0.9] x [0, H]^2 in the plane y = z (H = 10^7 m). A ray from x < 0
-    with a positive x component, started far from y = z, passes through every box and meets no triangle, so a
-    depth-first walk pushes the hit siblings at every record level: about 30 entries on these trees, past the 24 that
-    each lane keeps in LDS, so the walks' global stack part is written and read (a few hundred thousand entries in
-    these cases, where the other scenes of these tests reach it a few times at most)."""
+    """2^lg triangles along x, triangle k over [k, k + 0.9] x [0, H]^2 in the plane y = z (H = 10^7 m), and an opaque
+    wall across them at x = 24.5 (two triangles, the last). A ray from x < 0 with a positive x component, started far
+    from y = z, is inside every comb box and meets no comb triangle, so a nearest-first walk pushes the hit siblings at
+    every record level on its way down to the first leaves (about 30 entries: past the 24 that each lane keeps in LDS,
+    into the walk's global part), and then finds the wall only in the siblings it pops back from the deepest entries:
+    a wrong global entry would lose the wall, and the ray would run on along the comb. (An instrumented build counted,
+    over the SAH tree, about 10,000 global writes, every one read back, for 48 sensors x 32 directions along the comb,
+    and about 750 writes and 75 reads for the views of one sensor in 15-degree cells, 6,700 and 690 in 5-degree ones; the measured scenes' walks reach the
+    global part a few times at most.)"""
     nt = 1 << lg
     k = np.arange(nt, dtype=np.float64)
     z = np.zeros(nt)
-    V = np.stack([np.stack([k, z, z], 1), np.stack([k + 0.9, z, z], 1), np.stack([k, z + COMB_H, z + COMB_H], 1)], 1)
-    return V.astype(np.float32).reshape(-1), np.arange(3 * nt, dtype=np.uint32)
+    V = np.stack([np.stack([k, z, z], 1), np.stack([k + 0.9, z, z], 1), np.stack([k, z + COMB_H, z + COMB_H], 1)], 1).reshape(-1, 3)
+    h, w = COMB_H, 24.5
+    wall = np.array([[w, 0, 0], [w, h, 0], [w, h, h], [w, 0, 0], [w, h, h], [w, 0, h]])
+    V = np.concatenate([V, wall]).astype(np.float32).reshape(-1)
+    return V, np.arange(len(V) // 3, dtype=np.uint32)
+
+
+def _comb_sensors(rng, n):
+    """n sensors facing +x: three quarters before the comb (their rays along it end at the wall), the rest beyond its
+    end (open)"""
+    P = np.stack([np.full(n, -5.0), rng.uniform(0.6, 0.9, n) * COMB_H, rng.uniform(0.1, 0.4, n) * COMB_H], 1).astype(np.float32)
+    P[3 * n // 4:, 0] = (1 << 20) + 10.0
+    return P, np.tile(np.array([1.0, 0.0, 0.0], np.float32), (n, 1))
 
 
 @pytest.mark.parametrize("tree", ["sah", "lbvh"])
 def test_deep_stacks_comb_bits_and_bytes(orc, tree):
     rng = np.random.default_rng(11)
     V, I = _comb()
-    P = np.stack([np.full(2, -5.0), rng.uniform(0.6, 0.9, 2) * COMB_H, rng.uniform(0.1, 0.4, 2) * COMB_H], 1).astype(np.float32)
-    N = np.tile(np.array([1.0, 0.0, 0.0], np.float32), (2, 1))
-    D = rng.normal(size=(24, 3))
-    D[:16] = D[:16] * [0.2, 0.2, 0.2] + [1.0, 0.0, 0.0]   # 16 of them within a few degrees of +x
+    # few rays along the comb: the oracle's byte mode walks all of it, past the wall
+    P, N = _comb_sensors(rng, 24)
+    D = rng.normal(size=(16, 3))
+    D[:8] = D[:8] * 0.2 + [1.0, 0.0, 0.0]   # 8 of them within a few degrees of +x
     D /= np.linalg.norm(D, axis=1, keepdims=True)
     _bits_and_bytes_vs_oracle(orc, V, I, P, N, D.astype(np.float32).reshape(-1), tree)
 
@@ -270,12 +285,12 @@ def test_deep_stacks_comb_views(native, orc, tmp_path):
     y0, z0 = 0.7 * COMB_H, 0.2 * COMB_H
     gp = np.array([[-5.0, y0, z0], [-5.0, y0 + 1, z0], [-5.0, y0 + 1, z0 + 1], [-5.0, y0, z0 + 1]])   # one quad facing +x
     gc, gi = np.array([4], np.int32), np.arange(4, dtype=np.int32)
-    print("case: views from one quad along a comb of %d triangles, 360 x 120 degrees in 15-degree cells" % nt, flush=True)
+    print("case: views from one quad along a comb of %d triangles, 360 x 120 degrees in 10-degree cells" % nt, flush=True)
     dump = str(tmp_path / "rays.npy")
-    r = native.run("views", ["--fov", "360", "--cell", "15", "--dump-rays", dump], native.SimulationMesh("grid", gp, gc, gi),
+    r = native.run("views", ["--fov", "360", "--cell", "10", "--dump-rays", dump], native.SimulationMesh("grid", gp, gc, gi),
                    context=native.SimulationMesh("comb", pts, cnt, idx))
     o = orc.views([orc.MeshArrays("grid", gp, gc, gi, sensor=True), orc.MeshArrays("comb", pts, cnt, idx, sensor=False)],
-                  fov=(360.0, -70.0, 50.0), cell=15.0, raw=True, nthreads=min(16, os.cpu_count() or 1))
+                  fov=(360.0, -70.0, 50.0), cell=10.0, raw=True, nthreads=min(16, os.cpu_count() or 1))
     assert r.manifest["columns"] == o["columns"]
     cls = np.load(dump)
     assert cls.shape == o["cls"].shape

@@ -2,7 +2,8 @@
 
 Bit for bit, through the C ABI: two calls in a row over the same directions, the second over fewer sensors (so that
 the fixed device array holds the first call's larger visibility when the second reads it), in bit mode and in byte
-mode (translucent slots). Sensors, weights and sums live in device memory that the test allocates through the HIP
+mode (translucent slots); and on a comb of 2^20 triangles whose walks go past the stacks' LDS part into the array's
+stack area (where deep trees get a smaller persistent grid). Sensors, weights and sums live in device memory that the test allocates through the HIP
 runtime the library links (one runtime in the process). A call over more sensors than the array holds fails with the
 limit in its message, before any kernel runs.
 """
@@ -112,5 +113,40 @@ def test_noalloc_equals_chunk(gpu):
         big = limit // per + 1
         with pytest.raises(Exception, match="device array"):
             gpu.chunk_noalloc(dP, dN, big, dW, useCos, dS)
+    finally:
+        hip.free()
+
+
+def test_noalloc_equals_chunk_deep_stacks(gpu):
+    from test_gpu_configs import _comb, _comb_sensors
+    rng = np.random.default_rng(9)
+    V, I = _comb()
+    nt = len(I) // 3
+    gpu.build(V, I)
+    gpu.set_tri_slots((np.arange(nt, dtype=np.uint32) // 7) % 3)
+    gpu.set_slots([(0.2, 0.0, 0.0, 0), (0.2, 0.5, 0.0, 0), (0.1, 0.3, 0.0, 0)])
+    n = 256
+    P, N = _comb_sensors(rng, n)
+    D = rng.normal(size=(48, 3))
+    D[:32] = D[:32] * 0.2 + [1.0, 0.0, 0.0]   # along the comb
+    D = (D / np.linalg.norm(D, axis=1, keepdims=True)).astype(np.float32)
+    gpu.set_scene(P.reshape(-1), N.reshape(-1), D.reshape(-1))
+    useCos = [1, 0]
+    W = rng.uniform(0, 2, (len(useCos), len(D))).astype(np.float32)
+    hip = Hip()
+    try:
+        dW = hip.put(W.reshape(-1))
+        for translucent in (False, True):
+            for i0, nc in ((0, n), (150, 100)):   # the second call smaller, and with open sensors (those past 192)
+                print("case: no-alloc chunk on the comb, translucent %s, sensors [%d, %d)" % (translucent, i0, i0 + nc), flush=True)
+                want = gpu.chunk(i0, nc, W, useCos, translucent=translucent)
+                dP = hip.put(P[i0:i0 + nc].reshape(-1))
+                dN = hip.put(N[i0:i0 + nc].reshape(-1))
+                blank = np.full(len(useCos) * nc, np.nan)
+                dS = hip.put(blank)
+                gpu.chunk_noalloc(dP, dN, nc, dW, useCos, dS, translucent=translucent)
+                got = hip.get(dS, blank).reshape(len(useCos), nc)
+                assert np.array_equal(got, want), (translucent, i0, nc)
+                assert np.count_nonzero(got) > 0
     finally:
         hip.free()
