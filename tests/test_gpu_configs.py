@@ -244,9 +244,9 @@ def _comb(lg=20):
     every record level on its way down to the first leaves (about 30 entries: past the 24 that each lane keeps in LDS,
     into the walk's global part), and then finds the wall only in the siblings it pops back from the deepest entries:
     a wrong global entry would lose the wall, and the ray would run on along the comb. (An instrumented build counted,
-    over the SAH tree, about 10,000 global writes, every one read back, for 48 sensors x 32 directions along the comb,
-    and about 750 writes and 75 reads for the views of one sensor in 15-degree cells, 6,700 and 690 in 5-degree ones; the measured scenes' walks reach the
-    global part a few times at most.)"""
+    over the SAH tree, about 9 global writes per ray along the comb, every one read back, and for the views of one
+    sensor about 750 writes and 75 reads in 15-degree cells, 6,700 and 690 in 5-degree ones; the measured scenes' walks
+    reach the global part a few times in all.)"""
     nt = 1 << lg
     k = np.arange(nt, dtype=np.float64)
     z = np.zeros(nt)
