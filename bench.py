@@ -526,8 +526,11 @@ def main():
         native.count_fetches(False)
         fetch_bytes = (k1["node_fetches"] - k0["node_fetches"]) * NODE_BYTES + (k1["tri_fetches"] - k0["tri_fetches"]) * TRI_BYTES
         waves = k1["wave_iters"] - k0["wave_iters"]
+        leaf_waves = k1["leaf_iters"] - k0["leaf_iters"]
         cache = {"bytes_per_launch": round(fetch_bytes / launches), "gbs": round(fetch_bytes / launches / (trace_launch_ms * 1e-3) / 1e9, 1),
                  "simd_efficiency": None if not waves else round((k1["node_fetches"] - k0["node_fetches"]) / (64.0 * waves), 3),
+                 "leaf_simd_efficiency": None if not leaf_waves else round((k1["tri_fetches"] - k0["tri_fetches"]) / (64.0 * leaf_waves), 3),
+                 "leaf_iters_per_wave_iter": None if not waves else round(leaf_waves / waves, 3),
                  "note": "the counted 4-wide record (112 B) and triangle (48 B) fetches: served by L1/L2, not HBM"}
     traffic = profile_value("k_bits_city3m_traffic.txt", "traffic_bytes_per_launch")
     hbm = None if traffic is None else traffic / (trace_launch_ms * 1e-3) / 1e9

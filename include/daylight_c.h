@@ -88,8 +88,9 @@ DL_API int32_t dlx_run_series_device(int32_t argc, const char* const* argv, dl_m
  * variant, dlx_count_fetches), [6] reduce ms, [7] series ms, [8] sky ms, [9] build ms, [10] views ms, [11] last trace
  * launch ms, [12] last series ms, [13] resident scene bytes, [14] tree height, [15] traversal iterations executed by
  * the waves (counting variant), [16] rays traced (the ray slots past the hemisphere cull), [17] chunk graphs launched,
- * [18] chunk graphs captured. Writes min(n, DLX_COUNTERS) values. */
-#define DLX_COUNTERS 19
+ * [18] chunk graphs captured, [19] triangle-test iterations executed by the waves (counting variant). Writes
+ * min(n, DLX_COUNTERS) values. */
+#define DLX_COUNTERS 20
 DL_API int32_t dlx_gpu_counters(double* out, int32_t n);
 /* on != 0: the traversal runs its counting variant (slower) and fills counters [3]-[5]. */
 DL_API void dlx_count_fetches(int32_t on);

@@ -281,6 +281,8 @@ typedef struct {
   int64_t paths, bounceLaunches;
   /* dlg_chunk: graphs captured, graph launches */
   int64_t graphCaptures, graphLaunches;
+  /* the counting variant's triangle-test iterations executed by the waves */
+  int64_t leafIters;
 } dlg_stats_t;
 int32_t dlg_stats(const dlg_gpu* g, dlg_stats_t* out);
 /* No counterpart in the reference (its OptiX traversal is opaque): on != 0 makes dlg_trace run the traversal's counting

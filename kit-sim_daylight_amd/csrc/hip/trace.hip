@@ -51,9 +51,10 @@ struct OwnSet {
   }
 };
 
-// fetch counts of one lane (the counting variant of the kernel): records visited, triangles tested, and the traversal
-// iterations its wave executed
-struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0; };
+// fetch counts of one lane (the counting variant of the kernel): records visited, triangles tested, the traversal
+// iterations its wave executed and the triangle-test iterations its wave executed (the leaf loop's SIMD efficiency is
+// triangle tests / (64 x leaf iterations))
+struct Fetches { uint32_t nodes = 0, tris = 0, waveIters = 0, leafIters = 0; };
 
 // the lanes' traversal stacks: the top entries in LDS, the deeper ones in the wave's area of a global buffer
 using dls::LaneStack;
@@ -117,7 +118,10 @@ __device__ __forceinline__ bool walk(const float4* __restrict__ wide, const dlw:
         leaf = dlw::leaf_first(ref);
         if (ref & dlw::kPair) second = leaf + 1;
       }
-      if (COUNT) ++f.tris;
+      if (COUNT) {
+        ++f.tris;
+        if (__builtin_amdgcn_readfirstlane(lane) == lane) ++f.leafIters;
+      }
       if (!hit(leaf)) return false;
     }
     if (v.nInner == 0) {
@@ -256,6 +260,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6))) void
     atomicAdd(&counts[0], (unsigned long long)f.nodes);
     atomicAdd(&counts[1], (unsigned long long)f.tris);
     if (f.waveIters) atomicAdd(&counts[2], (unsigned long long)f.waveIters);
+    if (f.leafIters) atomicAdd(&counts[3], (unsigned long long)f.leafIters);
   }
 }
 
@@ -562,20 +567,21 @@ extern "C" int32_t dlg_trace(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFla
   if (nc == 0 || g->nWords == 0) return 0;
   unsigned long long* counts = nullptr;
   if (g->countFetches && !g->bytes) {
-    DLG_CHECK(g->counts.grow(3 * sizeof(unsigned long long)));
+    DLG_CHECK(g->counts.grow(4 * sizeof(unsigned long long)));
     counts = g->counts.as<unsigned long long>();
-    DLG_CHECK(hipMemsetAsync(counts, 0, 3 * sizeof(unsigned long long), g->stream));
+    DLG_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(unsigned long long), g->stream));
   }
   dlg_tbegin(g);
   dlg_trace_enqueue(g, rayFlags, stack, counts);
   DLG_CHECK(hipGetLastError());
   const float ms = dlg_tend(g);
   if (counts) {
-    unsigned long long c[3] = {0, 0, 0};
+    unsigned long long c[4] = {0, 0, 0, 0};
     DLG_CHECK(hipMemcpy(c, counts, sizeof c, hipMemcpyDeviceToHost));
     g->stats.nodeFetches += int64_t(c[0]);
     g->stats.triFetches += int64_t(c[1]);
     g->stats.waveIters += int64_t(c[2]);
+    g->stats.leafIters += int64_t(c[3]);
     g->stats.countedRays += nc * g->nDir;
   }
   return dlg_trace_account(g, ms, err, errLen);

@@ -314,7 +314,8 @@ int32_t dlx_gpu_counters(double* out, int32_t n) {
   dlg_stats(l->gpu, &s);
   const double c[DLX_COUNTERS] = {s.traceMs, double(s.launches), double(s.rays), double(s.nodeFetches), double(s.triFetches), double(s.countedRays),
                                   s.reduceMs, s.seriesMs, s.skyMs, s.buildMs, s.viewMs, s.lastTraceMs, s.lastSeriesMs, double(s.sceneBytes),
-                                  double(s.height), double(s.waveIters), double(s.traced), double(s.graphLaunches), double(s.graphCaptures)};
+                                  double(s.height), double(s.waveIters), double(s.traced), double(s.graphLaunches), double(s.graphCaptures),
+                                  double(s.leafIters)};
   std::copy(c, c + std::min<int32_t>(n, DLX_COUNTERS), out);
   return DL_OK;
 }

@@ -60,7 +60,7 @@ class Stats(C.Structure):
                [(n, C.c_double) for n in ("lastTraceMs", "lastReduceMs", "lastSeriesMs", "lastSkyMs", "lastViewMs", "lastViewReduceMs")] + \
                [(n, C.c_int64) for n in ("nodeFetches", "triFetches", "countedRays", "waveIters")] + \
                [("bounceMs", C.c_double), ("paths", C.c_int64), ("bounceLaunches", C.c_int64)] + \
-               [("graphCaptures", C.c_int64), ("graphLaunches", C.c_int64)]
+               [("graphCaptures", C.c_int64), ("graphLaunches", C.c_int64), ("leafIters", C.c_int64)]
 
 
 class BounceIn(C.Structure):
@@ -74,7 +74,7 @@ class ViewIn(C.Structure):
 
 
 SERIES_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p)
-DLX_COUNTERS = 19
+DLX_COUNTERS = 20
 DLX_SERVER_STATS = 10
 
 

@@ -31,7 +31,7 @@ _STATS = (("scene_reused", bool), ("prep_ms", float), ("sky_ms", float), ("build
 # dlx_gpu_counters order (include/daylight_c.h)
 _COUNTERS = ("trace_ms", "trace_launches", "rays", "node_fetches", "tri_fetches", "counted_rays", "reduce_ms", "series_ms",
              "sky_ms", "build_ms", "views_ms", "last_trace_ms", "last_series_ms", "scene_bytes", "tree_height", "wave_iters",
-             "traced", "graph_launches", "graph_captures")
+             "traced", "graph_launches", "graph_captures", "leaf_iters")
 
 
 class DaylightError(Exception):
