@@ -107,3 +107,34 @@ def test_views_chunking_invariance(native, box, tmp_path):
         assert np.array_equal(np.load(tmp_path / "a.npy"), np.load(tmp_path / "b.npy"))
         assert np.array_equal(np.load(str(tmp_path / "a.npy") + ".depth.npy").view(np.uint32),
                               np.load(str(tmp_path / "b.npy") + ".depth.npy").view(np.uint32))
+
+
+def test_views_sensor_blocks(native, box, tmp_path):
+    """The rays' work items come in blocks of 512 sensors (tile-group major within a block, the last block partial):
+    a 1,300-sensor grid around the box in one chunk (blocks 512 + 512 + 276) against chunks of about 100 and 700
+    sensors (one partial block; a full and a partial one), values and every ray."""
+    print("case: views sensor blocks (1,300-sensor grid, 360 degrees in 4-degree cells)", flush=True)
+    nx, ny = 26, 50
+    xs, ys = np.meshgrid(np.linspace(-3, 5, nx + 1), np.linspace(-3, 5, ny + 1), indexing="ij")
+    pts = np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, 0.25)], 1)
+    i = np.arange(nx)[:, None] * (ny + 1) + np.arange(ny)[None, :]
+    quads = np.stack([i, i + ny + 1, i + ny + 2, i + 1], -1).reshape(-1, 4).astype(np.int32)
+    grid = native.SimulationMesh("grid", pts, np.full(len(quads), 4, np.int32), quads.ravel())
+    try:
+        flags = ["--fov", "360", "--cell", "4", "--distance", "isovist,mean,min"]
+        a = native.run("views", flags + ["--dump-rays", str(tmp_path / "a.npy")], grid, context=box)
+        rays = np.load(tmp_path / "a.npy")
+        assert a.values.shape[0] == nx * ny and rays.shape[0] == nx * ny
+        per_sensor = rays[0].size * 5   # a class byte and a float distance per ray
+        for n in (100, 700):
+            try:
+                native.chunk_bytes(per_sensor * n)
+                b = native.run("views", flags + ["--dump-rays", str(tmp_path / "b.npy")], grid, context=box)
+            finally:
+                native.chunk_bytes(0)
+            assert np.array_equal(a.values, b.values, equal_nan=True), n
+            assert np.array_equal(rays, np.load(tmp_path / "b.npy")), n
+            assert np.array_equal(np.load(str(tmp_path / "a.npy") + ".depth.npy").view(np.uint32),
+                                  np.load(str(tmp_path / "b.npy") + ".depth.npy").view(np.uint32)), n
+    finally:
+        grid.close()
