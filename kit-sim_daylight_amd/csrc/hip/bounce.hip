@@ -38,6 +38,7 @@ struct BounceArgs {
   float* dc;
   uint32_t nc, i0, M, bounces, mf, nBins, cull;
   float alpha, cosN, sinN;
+  const uint32_t* order;   // the sensor at each position of the chunk (the trace's orientation + Morton order), or null
 };
 
 DL_HD float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -112,8 +113,10 @@ __device__ __forceinline__ bool trace_path(const BounceArgs& a, float3 n, float3
   return false;
 }
 
-// A wave per sensor of [c0, c0 + gridDim.x): lane l traces samples l, l + 64, ... (rounds of 64 from one origin, so the
-// top of the tree is shared); each round's escaping paths add to the sensor's row in LDS in lane order, i.e. in sample
+// A wave per position of [c0, c0 + gridDim.x) of the chunk's sensor order (the trace's orientation + Morton order when
+// the chunk has one: the waves in flight trace a compact patch of sensors; seeds and rows belong to the sensor, so the
+// order changes no result): lane l traces samples l, l + 64, ... (rounds of 64 from one origin, so the top of the tree
+// is shared); each round's escaping paths add to the sensor's row in LDS in lane order, i.e. in sample
 // order, which keeps every bin's float sum that of one thread walking the samples in turn; the row is written at the end.
 // OFF32: fewer than 2^25 records (wide.h visit)
 template <int STACK, bool OFF32>
@@ -122,8 +125,9 @@ __global__ __launch_bounds__(kWave, 3) void k_bounce(BounceArgs a, uint32_t c0) 
   __shared__ uint32_t stkLds[STACK * kWave];
   const dls::LaneStack<STACK, STACK> stk{(dls::lds_u32*)stkLds, nullptr};   // the whole stack in LDS
   extern __shared__ float sRow[];   // [nBins]
-  const uint32_t lane = threadIdx.x, i = c0 + blockIdx.x;
-  if (i >= a.nc) return;   // whole blocks
+  const uint32_t lane = threadIdx.x, p = c0 + blockIdx.x;
+  if (p >= a.nc) return;   // whole blocks
+  const uint32_t i = a.order ? a.order[p] : p;
   for (uint32_t b = lane; b < a.nBins; b += kWave) sRow[b] = 0.f;
   __syncthreads();
   const uint32_t gi = a.i0 + i;
@@ -198,6 +202,7 @@ int32_t dlg_bounce(dlg_gpu* g, int64_t i0, int64_t nc, uint32_t rayFlags, const 
   a.nBins = nBins; a.cull = (rayFlags & 16u) ? 1u : 0u;
   a.alpha = float((3.14159265358979323846 / 2.) / (7 * b->mf + .5));   // the patch rows' height (the sky's Patches::alpha)
   a.cosN = float(std::cos(rad)); a.sinN = float(std::sin(rad));
+  a.order = g->orderGen == g->sensGen && g->orderI0 == i0 && g->orderN == nc ? g->tileOrder.as<uint32_t>() : nullptr;
   // sensors in launches of about 2^24 paths (a launch's runtime bounded; each sensor's row is one block's)
   const uint32_t per = std::max<uint32_t>(1, (1u << 24) / std::max<uint32_t>(1, b->M));
   for (uint32_t c0 = 0; c0 < uint32_t(nc); c0 += per) {
